@@ -1,0 +1,201 @@
+"""A solve case: configuration, multiblock topology and geometric inputs of
+every multigrid level, in the reference's flat array layout.
+
+This is the host-side bookkeeping the reference does in ``input``/``alloc``
+/``setoffs`` and ``grid`` (src/input.f90, src/grid.f90) before the
+pseudo-time loop starts.  It produces exactly the arrays that the Fortran
+driver would upload through the C ABI.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import math
+import numpy as np
+
+from . import grids
+from .config import default_config, hb_operator
+from .layout import Block, Cut, Level, Patch, build_levels, check_coarsenable
+
+
+@dataclass
+class Case:
+    config: Dict
+    blocks: List[Block]
+    cuts: List[Cut]
+    xyz: List[Tuple[np.ndarray, np.ndarray, np.ndarray]]   # fine-level nodes per block, [k, j, i]
+    levels: List[Level] = field(default_factory=list)
+    geom: List[Dict[str, np.ndarray]] = field(default_factory=list)
+    name: str = "case"
+
+    def __post_init__(self):
+        nlevel = self.config["nlevel"]
+        check_coarsenable(self.blocks, nlevel)
+        self.levels = build_levels(self.blocks, self.cuts, nlevel)
+        self.geom = [self._level_geometry(L) for L in self.levels]
+
+    # -- sizes ---------------------------------------------------------------
+    @property
+    def npde(self) -> int:
+        return self.config["npde"]
+
+    @property
+    def nsnap(self) -> int:
+        return 2 * self.config["nharms"] + 1
+
+    @property
+    def nsnap_met(self) -> int:
+        return 2 * self.config["nharms"] * self.config["hbmove"] + 1
+
+    def size(self, nl: int, kind: str) -> int:
+        """Doubles in a whole-level array of a given kind."""
+        L = self.levels[nl - 1]
+        lmet = self.config["lmet"]
+        return {
+            "p3": L.total(3) * self.npde * self.nsnap,
+            "p3s": L.total(3) * self.nsnap,
+            "met": L.total(2) * lmet * self.nsnap_met,
+            "x2": L.total(2) * self.nsnap_met,
+            "p1": L.total(1) * self.nsnap,
+            "vol": L.total(1),
+            "flux": L.total(0) * self.npde * self.nsnap,
+        }[kind]
+
+    def ncell(self, nl: int = 1) -> int:
+        return self.levels[nl - 1].ncell()
+
+    # -- geometry ------------------------------------------------------------
+    def _level_geometry(self, L: Level) -> Dict[str, np.ndarray]:
+        lmet = self.config["lmet"]
+        nsm = self.nsnap_met
+        si = np.zeros(self.size(L.nl, "met"))
+        sj = np.zeros_like(si)
+        sk = np.zeros_like(si)
+        vol = np.zeros(self.size(L.nl, "vol"))
+        o2 = L.offsets(2)
+        o1 = L.offsets(1)
+        for b in range(L.nblocks):
+            x, y, z = (grids.coarsen(a, L.nl) for a in self.xyz[b])
+            assert x.shape == (L.kmax[b], L.jmax[b], L.imax[b])
+            mets = grids.face_metrics(x, y, z, lmet)
+            n2 = int(L.points(2)[b])
+            for arr, m in zip((si, sj, sk), mets):
+                blk = arr[o2[b] * lmet * nsm: (o2[b] + n2) * lmet * nsm]
+                flat = m.reshape(-1)                       # [k][j][i][lmet]
+                for n in range(nsm):
+                    blk[n * n2 * lmet:(n + 1) * n2 * lmet] = flat
+            v = grids.cell_volumes(x, y, z)
+            vol[o1[b]: o1[b] + L.points(1)[b]] = v.reshape(-1)
+        return {"si": si, "sj": sj, "sk": sk, "vol": vol}
+
+    # -- states --------------------------------------------------------------
+    def freestream(self, nl: int = 1) -> np.ndarray:
+        """init2fs (src/input.f90:4741-4800), conservative, halos included."""
+        c = self.config
+        L = self.levels[nl - 1]
+        q = np.empty(self.size(nl, "p3"))
+        vals = [c["rhoinf"],
+                c["rhoinf"] * c["machfs"] * math.cos(c["alpha"]) * math.cos(c["beta"]),
+                c["rhoinf"] * c["machfs"] * math.sin(c["alpha"]) * math.cos(c["beta"]),
+                c["rhoinf"] * c["machfs"] * math.sin(c["beta"]),
+                1 / c["rhoinf"] / c["gamma"] / (c["gamma"] - 1) + 0.5 * c["machfs"] ** 2]
+        o3 = L.offsets(3)
+        for b in range(L.nblocks):
+            n3 = int(L.points(3)[b])
+            base = o3[b] * self.npde * self.nsnap
+            for n in range(self.nsnap):
+                for v in range(self.npde):
+                    s = base + (n * self.npde + v) * n3
+                    q[s:s + n3] = vals[v] if v < 5 else 0.0
+        return q
+
+    def perturbed(self, amp: float = 0.01, seed: int = 1, nl: int = 1) -> np.ndarray:
+        """Freestream times (1 + uniform(-amp, amp)) per value (C4 recipe)."""
+        q = self.freestream(nl)
+        rng = np.random.default_rng(seed)
+        return q * (1.0 + rng.uniform(-amp, amp, q.shape))
+
+    def hb_matrix(self) -> Optional[np.ndarray]:
+        if self.config["nharms"] == 0:
+            return None
+        return hb_operator(self.config["nharms"], self.config["omega"])
+
+
+# ---------------------------------------------------------------------------
+# configurations
+
+def naca_case(ni=193, nj=49, nk=5, nlevel=1, name="naca0012", **cfg) -> Case:
+    """NACA 0012 O-grid, single block: j=1 inviscid wall (14), j=jmax
+    farfield (30), k faces symmetry (80), seam i=1 <-> i=imax as a self-cut."""
+    cfg.setdefault("machfs", 0.5)
+    cfg.setdefault("alpha_deg", 1.25)
+    c = default_config(nlevel=nlevel, **cfg)
+    x, y, z = grids.naca0012_ogrid(ni, nj, nk)
+    b = Block(ni, nj, nk, [
+        Patch((1, 1, 1), (ni, 1, nk), 14),
+        Patch((1, nj, 1), (ni, nj, nk), 30),
+        Patch((1, 1, 1), (ni, nj, 1), 80),
+        Patch((1, 1, nk), (ni, nj, nk), 80),
+    ])
+    cut = Cut(1, (1, 1, 1), (1, nj, nk), 1, (ni, 1, 1), (ni, nj, nk), (1, 2, 3))
+    return Case(c, [b], [cut], [(x, y, z)], name=name)
+
+
+def channel_block(ni, nj, nk) -> Block:
+    return Block(ni, nj, nk, [
+        Patch((1, 1, 1), (1, nj, nk), 30),
+        Patch((ni, 1, 1), (ni, nj, nk), 30),
+        Patch((1, 1, 1), (ni, 1, nk), 14),
+        Patch((1, nj, 1), (ni, nj, nk), 14),
+        Patch((1, 1, 1), (ni, nj, 1), 80),
+        Patch((1, 1, nk), (ni, nj, nk), 80),
+    ])
+
+
+def uneven_dims(nblocks=96, seed=0, sides=(17, 33, 65, 129)):
+    """Block node counts of the uneven multiblock configuration."""
+    rng = np.random.default_rng(seed)
+    return [tuple(int(v) for v in rng.choice(sides, 3)) for _ in range(nblocks)]
+
+
+def channel_case(dims: Sequence[Tuple[int, int, int]], nlevel=3, name="bump_channels", **cfg) -> Case:
+    """Independent bump channels (one block each): inflow/outflow farfield
+    (30) on i faces, inviscid walls (14) on j faces, symmetry (80) on k."""
+    cfg.setdefault("machfs", 0.7)
+    c = default_config(nlevel=nlevel, **cfg)
+    blocks, xyz = [], []
+    for (ni, nj, nk) in dims:
+        blocks.append(channel_block(ni, nj, nk))
+        xyz.append(grids.bump_channel(ni, nj, nk))
+    return Case(c, blocks, [], xyz, name=name)
+
+
+def two_block_case(ni=17, nj=9, nk=9, nlevel=1, distort=0.0, name="two_blocks", **cfg) -> Case:
+    """Two blocks joined by a 1:1 cut across i (block 1 i=imax <-> block 2
+    i=1), farfield elsewhere except a wall and symmetry planes; used by the
+    parity tests to exercise cuts between different blocks."""
+    cfg.setdefault("machfs", 0.6)
+    cfg.setdefault("alpha_deg", 3.0)
+    c = default_config(nlevel=nlevel, **cfg)
+    x, y, z = grids.bump_channel(2 * ni - 1, nj, nk, length=3.0, height=1.0, width=0.6)
+    if distort:
+        x, y, z = grids.distort(x, y, z, distort, seed=3)
+    a = tuple(arr[:, :, :ni].copy() for arr in (x, y, z))
+    b = tuple(arr[:, :, ni - 1:].copy() for arr in (x, y, z))
+    b1 = Block(ni, nj, nk, [
+        Patch((1, 1, 1), (1, nj, nk), 30),
+        Patch((1, 1, 1), (ni, 1, nk), 14),
+        Patch((1, nj, 1), (ni, nj, nk), 30),
+        Patch((1, 1, 1), (ni, nj, 1), 80),
+        Patch((1, 1, nk), (ni, nj, nk), 80),
+    ])
+    b2 = Block(ni, nj, nk, [
+        Patch((ni, 1, 1), (ni, nj, nk), 4),
+        Patch((1, 1, 1), (ni, 1, nk), 14),
+        Patch((1, nj, 1), (ni, nj, nk), 30),
+        Patch((1, 1, 1), (ni, nj, 1), 80),
+        Patch((1, 1, nk), (ni, nj, nk), 80),
+    ])
+    cut = Cut(1, (ni, 1, 1), (ni, nj, nk), 2, (1, 1, 1), (1, nj, nk), (1, 2, 3))
+    return Case(c, [b1, b2], [cut], [a, b], name=name)

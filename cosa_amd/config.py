@@ -1,0 +1,128 @@
+"""Solver configuration: the ctypes mirror of ``cosahip_config``
+(include/cosahip.h) and the reference's defaults.
+
+Field meanings follow common_variables (src/common_variables.f90:19-65);
+defaults follow ``input`` (src/input.f90:2400-2520): gamma 1.4, rho_inf =
+a_inf = 1, p_inf = 1/gamma, qmin, RK coefficients ar4 = (0.25, 0.333333, 0.5,
+1.0) as the decimal literals of src/input.f90:2403.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Any, Dict
+
+_INT_FIELDS = [
+    "npde", "nharms", "hbmove", "lmet", "nlevel", "nl_crs", "nstage", "irsop", "cutcirs", "ilimf", "ilimt",
+    "iupdt", "iprint", "ncycle", "kom", "kom_bsl", "kom_sst", "viscous", "moving", "harbal", "foturb",
+    "efx_setup_f", "efx_all_f", "eig_limit_f", "eig_lim_all_f", "efx_setup_t", "efx_all_t", "eig_limit_t",
+    "eig_lim_all_t",
+]
+_DBL_FIELDS = [
+    "gamma", "machfs", "alpha", "beta", "rhoinf", "pinf", "ainf", "cfl", "cflt", "rat", "ratt", "psirs", "rkap",
+    "epslimf", "epslimt", "cntrpy_f", "cntrpy_t", "entfxctf_f", "entfxctf_t", "eig_cutoff_f", "eig_cutoff_t",
+    "omega", "toler",
+]
+
+
+class CosahipConfig(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_int32) for n in _INT_FIELDS]
+                + [("entfx_f_w", ctypes.c_int32 * 3), ("entfx_t_w", ctypes.c_int32 * 3),
+                   ("bln_prol", ctypes.c_int32), ("ho_rest", ctypes.c_int32), ("pad0", ctypes.c_int32)]
+                + [(n, ctypes.c_double) for n in _DBL_FIELDS]
+                + [("qmin", ctypes.c_double * 7), ("alphas", ctypes.c_double * 5),
+                   ("tkefar", ctypes.c_double), ("omefar", ctypes.c_double)])
+
+
+def default_config(**kw: Any) -> Dict[str, Any]:
+    gamma = kw.get("gamma", 1.4)
+    c: Dict[str, Any] = dict(
+        npde=5, nharms=0, hbmove=0, lmet=4, nlevel=1, nl_crs=4, nstage=4, irsop=1, cutcirs=0, ilimf=4, ilimt=4,
+        iupdt=1, iprint=1, ncycle=100, kom=0, kom_bsl=0, kom_sst=0, viscous=0, moving=0, harbal=0, foturb=0,
+        efx_setup_f=0, efx_all_f=0, eig_limit_f=0, eig_lim_all_f=0, efx_setup_t=0, efx_all_t=0, eig_limit_t=0,
+        eig_lim_all_t=0, entfx_f_w=(0, 0, 0), entfx_t_w=(0, 0, 0), bln_prol=1, ho_rest=0,
+        gamma=gamma, machfs=0.5, alpha=0.0, beta=0.0, rhoinf=1.0, pinf=1.0 / gamma, ainf=1.0,
+        cfl=2.0, cflt=2.0, rat=1.0, ratt=1.0, psirs=0.0, rkap=0.333333, epslimf=1e-6, epslimt=1e-6,
+        cntrpy_f=0.0, cntrpy_t=0.0, entfxctf_f=0.0, entfxctf_t=0.0, eig_cutoff_f=0.0, eig_cutoff_t=0.0,
+        omega=0.0, toler=0.0,
+        qmin=(1e-8, 1.0, 1.0, 1.0, 1e-8, 1e-12, 1e-8), alphas=(0.25, 0.333333, 0.5, 1.0, 0.0),
+        tkefar=0.0, omefar=0.0,
+    )
+    for k, v in kw.items():
+        if k not in c and k not in ("alpha_deg", "beta_deg", "cfl_irs"):
+            raise KeyError(f"unknown config field {k}")
+        c[k] = v
+    if "alpha_deg" in kw:
+        c["alpha"] = kw["alpha_deg"] * math.pi / 180.0
+    if "beta_deg" in kw:
+        c["beta"] = kw["beta_deg"] * math.pi / 180.0
+    if "cfl_irs" in kw:
+        # input.f90:4206-4216: with IRS, cfl = vcfli(3) and rat = vcfli(3)/vcfl(3)
+        c["rat"] = kw["cfl_irs"] / c["cfl"]
+        c["cfl"] = kw["cfl_irs"]
+    c.pop("alpha_deg", None)
+    c.pop("beta_deg", None)
+    c.pop("cfl_irs", None)
+    return c
+
+
+def to_struct(c: Dict[str, Any]) -> CosahipConfig:
+    s = CosahipConfig()
+    for n in _INT_FIELDS:
+        setattr(s, n, int(c[n]))
+    for n in _DBL_FIELDS:
+        setattr(s, n, float(c[n]))
+    for i in range(3):
+        s.entfx_f_w[i] = int(c["entfx_f_w"][i])
+        s.entfx_t_w[i] = int(c["entfx_t_w"][i])
+    s.bln_prol = int(c["bln_prol"])
+    s.ho_rest = int(c["ho_rest"])
+    for i in range(7):
+        s.qmin[i] = float(c["qmin"][i])
+    for i in range(5):
+        s.alphas[i] = float(c["alphas"][i])
+    s.tkefar = float(c["tkefar"])
+    s.omefar = float(c["omefar"])
+    return s
+
+
+def hb_operator(nharms: int, omega: float):
+    """dhb of mathb (src/input.f90:5206-5282) for N harmonics, as a
+    (2N+1)x(2N+1) numpy array (row i, column j); zeit(n) = n*2*pi/omega/(2N+1)
+    (src/input.f90:3185-3191)."""
+    import numpy as np
+    n = 2 * nharms + 1
+    dt = 2 * math.pi / omega / n
+    zeit = [i * dt for i in range(n)]
+    ahb = np.zeros((n, n))
+    for i in range(1, nharms + 1):
+        ahb[2 * (i - 1) + 1, 2 * (i - 1) + 2] = i
+        ahb[2 * (i - 1) + 2, 2 * (i - 1) + 1] = -i
+    eihb = np.zeros((n, n))
+    eihb[:, 0] = 1.0
+    for j in range(1, nharms + 1):
+        for i in range(n):
+            eihb[i, 2 * (j - 1) + 1] = math.cos(j * omega * zeit[i])
+            eihb[i, 2 * (j - 1) + 2] = math.sin(j * omega * zeit[i])
+    fact = 2.0 / n
+    ehb = np.zeros((n, n))
+    ehb[0, :] = fact / 2
+    for i in range(1, nharms + 1):
+        for j in range(n):
+            ehb[2 * (i - 1) + 1, j] = fact * math.cos(i * omega * zeit[j])
+            ehb[2 * (i - 1) + 2, j] = fact * math.sin(i * omega * zeit[j])
+    d1 = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            s = 0.0
+            for k in range(n):
+                s = s + ahb[i, k] * ehb[k, j]
+            d1[i, j] = s
+    d = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            s = 0.0
+            for k in range(n):
+                s = s + eihb[i, k] * d1[k, j]
+            d[i, j] = s
+    return d

@@ -1,0 +1,203 @@
+/*
+ * cosahip.h -- C ABI of libcosahip, the gfx950 (MI355X) implementation of
+ * COSA's pseudo-time residual/update loop.
+ *
+ * The caller is COSA's Fortran driver (program cosa -> mg -> smooth), bound
+ * through iso_c_binding, or any host that mirrors it (the Python package
+ * cosa_amd does).  Every entry point below replaces one reference routine
+ * (cited as file:line into LANCASTER-CFD/COSA src/) and keeps its meaning:
+ * same level numbering (nl = 1 is the finest grid), same 1-based block and
+ * BC/cut topology records, same flat array layout as the reference's
+ * Cray-pointer allocations (src/input.f90:4878-5131, offsets of setoffs
+ * src/input.f90:5502-5582), so host<->device transfers are single copies.
+ *
+ * Conventions
+ *   - All functions return COSAHIP_OK (0) or a positive error code; the text
+ *     of the last error is available from cosahip_last_error().  The
+ *     reference writes a message and stops in the same situations.
+ *   - The library owns device memory.  State stays resident in HBM between
+ *     calls; host arrays are touched only by upload/download.
+ *   - One host thread per solve.  Different solves may share a device.
+ *   - Floating point is IEEE binary64 (Fortran real(8) with the reference's
+ *     -fdefault-real-8 promotion); integers are 32-bit (cosa_int).
+ */
+#ifndef COSAHIP_H
+#define COSAHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COSAHIP_ABI_VERSION 1
+
+/* ---- error codes ------------------------------------------------------ */
+enum {
+  COSAHIP_OK = 0,
+  COSAHIP_ERR_ARG = 1,         /* bad argument / unsupported option        */
+  COSAHIP_ERR_STATE = 2,       /* call out of order (level not defined...) */
+  COSAHIP_ERR_DEVICE = 3,      /* HIP runtime failure                      */
+  COSAHIP_ERR_NOMEM = 4,       /* device allocation failed                 */
+  COSAHIP_ERR_POSITIVITY = 5,  /* prtest: p <= qmin(5) or rho <= qmin(1)   */
+  COSAHIP_ERR_UNSUPPORTED = 6  /* option of the reference not built here   */
+};
+
+/* ---- solver configuration ---------------------------------------------
+ * Mirror of the common_variables members the hot path reads
+ * (src/common_variables.f90:19-65, src/cosa_variables.f90:31-36).
+ * Logicals are int32 (0/1).  Angles in radians (the reference converts the
+ * degrees of input.dat on input).  Field order is fixed: the Fortran binding
+ * declares the same members in a bind(C) derived type.                    */
+typedef struct cosahip_config {
+  int32_t npde;        /* 5 Euler/laminar, 7 k-omega                       */
+  int32_t nharms;      /* harmonic-balance harmonics N (0: steady)         */
+  int32_t hbmove;      /* 1: metrics carry the snapshot index              */
+  int32_t lmet;        /* metric rows per face: 4 static, 7 moving         */
+  int32_t nlevel;      /* multigrid levels (reference max 3)               */
+  int32_t nl_crs;      /* first-order MUSCL on levels nl >= nl_crs         */
+  int32_t nstage;      /* RK stages (4)                                    */
+  int32_t irsop;       /* 0 no_irs, 1 cirs_c, 2 cirs_v1, 3 cirs_v2         */
+  int32_t cutcirs;     /* 0 only (ccirs not built)                         */
+  int32_t ilimf;       /* flow limiter 0..4                                */
+  int32_t ilimt;       /* turbulence limiter 0..4                          */
+  int32_t iupdt;       /* local time step refresh period                   */
+  int32_t iprint;      /* residual print period                            */
+  int32_t ncycle;      /* number of MG cycles (lmax)                       */
+  int32_t kom, kom_bsl, kom_sst, viscous;
+  int32_t moving, harbal, foturb;
+  int32_t efx_setup_f, efx_all_f, eig_limit_f, eig_lim_all_f;
+  int32_t efx_setup_t, efx_all_t, eig_limit_t, eig_lim_all_t;
+  int32_t entfx_f_w[3], entfx_t_w[3];
+  int32_t bln_prol;    /* 1: bilinear prolongation, 0: linear          */
+  int32_t ho_rest;     /* 1: 64-weight residual restriction             */
+  int32_t pad0;
+  double gamma, machfs, alpha, beta;       /* alpha, beta in radians       */
+  double rhoinf, pinf, ainf;
+  double cfl, cflt, rat, ratt, psirs, rkap;
+  double epslimf, epslimt;
+  double cntrpy_f, cntrpy_t, entfxctf_f, entfxctf_t;
+  double eig_cutoff_f, eig_cutoff_t;
+  double omega;        /* HB angular frequency                             */
+  double toler;        /* convergence: max(resrms(1:5)) < toler            */
+  double qmin[7];
+  double alphas[5];    /* RK stage coefficients (ar4 literals)             */
+  double tkefar, omefar;
+} cosahip_config;
+
+/* ---- device fields (per level; reference layout and extents) ----------
+ *   P3 family: (-1:imax+1,-1:jmax+1,-1:kmax+1, ncomp, 0:2N)   off_p3
+ *   P2 family: (lmet, 0:imax+1, 0:jmax+1, 0:kmax+1, 0:2N*hbmove) off_p2
+ *   P1 family: (0:imax, 0:jmax, 0:kmax [, 0:2N])              off_p1     */
+enum {
+  COSAHIP_Q = 0,        /* conservative state (P3, npde)                   */
+  COSAHIP_QOLD = 1,     /* state at iteration start (P3, npde)             */
+  COSAHIP_RES = 2,      /* residual (P3, npde)                             */
+  COSAHIP_RHS = 3,      /* FAS forcing (P3, npde)                          */
+  COSAHIP_SI = 4,       /* xi-face metrics (P2, lmet)                      */
+  COSAHIP_SJ = 5,
+  COSAHIP_SK = 6,
+  COSAHIP_VOL = 7,      /* cell volumes (P1, no snapshot index)            */
+  COSAHIP_DTVOL = 8,    /* dt/vol (P1, 0:2N)                               */
+  COSAHIP_BETAXI = 9,   /* CIRS coefficients (P1, 0:2N)                    */
+  COSAHIP_BETAETA = 10,
+  COSAHIP_BETAZETA = 11,
+  COSAHIP_XDOT = 12,    /* cell-centre grid velocity (P2 scalar, 0:2N*hbmove) */
+  COSAHIP_YDOT = 13,
+  COSAHIP_ZDOT = 14,
+  COSAHIP_NFIELDS = 15
+};
+
+typedef struct cosahip_solve cosahip_solve;
+
+/* ---- lifetime ---------------------------------------------------------- */
+
+/* Replaces alloc (src/input.f90:4878-5131) for the hot-path arrays: copies
+ * the configuration, selects device `device` (-1: current).                */
+int cosahip_create(const cosahip_config* cfg, int device, cosahip_solve** out);
+int cosahip_destroy(cosahip_solve* s);
+const char* cosahip_last_error(const cosahip_solve* s);
+/* On COSAHIP_ERR_POSITIVITY: info = {level, block(1-based), i, j, k, iter,
+ * kind (1: pressure, 2: density), snapshot}.  Mirrors the message of
+ * bprtest (src/cosa.f90:960-1120).                                         */
+int cosahip_error_info(const cosahip_solve* s, int32_t info[8]);
+
+/* HB time-derivative operator dhb(0:2N,0:2N) as built by mathb
+ * (src/input.f90:5206-5282); Fortran column-major, nsnap = 2N+1.          */
+int cosahip_set_hb_operator(cosahip_solve* s, int nsnap, const double* dhb);
+
+/* Level topology: block node counts i_imax/j_jmax/k_kmax of level nl
+ * (src/input.f90:4393-4434), BC records bctopo(10,nbcs(b)) of setdim_bbc
+ * (src/input.f90:5155-5203) concatenated over blocks, cut records
+ * cutopo(21,ncuts) of setdimcut (src/input.f90:5285-5344).  Blocks are
+ * numbered 1..nblocks as in the reference.                                 */
+int cosahip_define_level(cosahip_solve* s, int nl, int nblocks,
+                         const int32_t* imax, const int32_t* jmax,
+                         const int32_t* kmax, const int32_t* nbcs,
+                         const int32_t* bctopo, int ncuts,
+                         const int32_t* cutopo);
+
+/* Number of doubles of `field` on level nl (whole level, all blocks).     */
+int64_t cosahip_field_size(const cosahip_solve* s, int nl, int field);
+int cosahip_upload(cosahip_solve* s, int nl, int field, const double* host,
+                   int64_t count);
+int cosahip_download(cosahip_solve* s, int nl, int field, double* host,
+                     int64_t count);
+int cosahip_synchronize(cosahip_solve* s);
+
+/* ---- reference routines, whole level, in place on device state ------- */
+
+/* cons2prim(nl,0,q,work): interior only (src/update.f90:2112-2188).
+ * `field` is COSAHIP_Q or COSAHIP_QOLD.                                     */
+int cosahip_cons2prim(cosahip_solve* s, int nl, int field);
+/* prim2cons: interior + face halo strips (src/update.f90:1851-2084).      */
+int cosahip_prim2cons(cosahip_solve* s, int nl, int field);
+/* bc (src/bc.f90:2-121): BC patches of every block, in input order.       */
+int cosahip_bc(cosahip_solve* s, int nl, int field);
+/* cutman_q (src/bc.f90:1801-1891): 1:1 cut halo copies.                   */
+int cosahip_cutman_q(cosahip_solve* s, int nl, int field);
+/* q_edges + q_corns (src/flux.f90:3889-4263, 3132-3471) on field.        */
+int cosahip_edges_corners(cosahip_solve* s, int nl, int field);
+/* prtest (src/cosa.f90:933-1120) on Q: COSAHIP_ERR_POSITIVITY on failure. */
+int cosahip_prtest(cosahip_solve* s, int nl, int iter);
+/* lambda + deltat (src/cosa.f90:1214-1591, 4183-4237, 4701-5203).         */
+int cosahip_lambda_deltat(cosahip_solve* s, int nl);
+/* zero(res) on 0..imax (src/util.f90:238-369), then for idir=1..3:
+ * muscl + roflux + resid (src/flux.f90:1183-1812, 2-387, 3472-3888;
+ * src/cosa.f90:1123-1211).  Q must hold primitive variables with halos.   */
+int cosahip_residual(cosahip_solve* s, int nl);
+/* src_hb (src/src.f90:530-613): res += omega*vol*D*q, q conservative.     */
+int cosahip_src_hb(cosahip_solve* s, int nl);
+/* resg_rms / resg_rms_hb (src/cosa.f90:4458-4697) -> resrms(1:npde+1).    */
+int cosahip_resg_rms(cosahip_solve* s, int nl, double* resrms);
+/* add_array(..., off_l=-1, fact) on the interior (src/util.f90:137-236):
+ * dst += fact*src, fields RES/RHS.                                        */
+int cosahip_add_array(cosahip_solve* s, int nl, int dst, int src, double fact);
+/* copy_array interior (src/util.f90:2-135): dst = src.                   */
+int cosahip_copy_array(cosahip_solve* s, int nl, int dst, int src);
+/* cirs (src/cosa.f90:5206-5623), cutcirs = 0.                             */
+int cosahip_cirs(cosahip_solve* s, int nl);
+/* update (src/update.f90:2-100, 568-762, 1301-1408), explicit RK.         */
+int cosahip_update(cosahip_solve* s, int nl, double ar);
+
+/* ---- drivers --------------------------------------------------------- */
+
+/* smooth (src/cosa.f90:1594-2032): niter pseudo-time iterations on level
+ * nl (one more residual-only pass when to_coarse).  mgit is the current MG
+ * cycle (controls the RMS print cycles).  On return *done is 1 when
+ * max(resrms(1:5)) < toler; resrms holds the last RMS evaluated on nl=1.  */
+int cosahip_smooth(cosahip_solve* s, int nl, int from_fine, int to_coarse,
+                   int niter, int mgit, int32_t* done, double* resrms);
+/* restrict (src/cosa.f90:2672-2754) from nl to nl+1.                     */
+int cosahip_restrict(cosahip_solve* s, int nl);
+/* prolong (src/cosa.f90:3167-3249) correction from nl+1 onto nl.         */
+int cosahip_prolong(cosahip_solve* s, int nl);
+
+/* Fraction of the last cosahip_smooth's device time spent per kernel family
+ * is not tracked here; use rocprofv3.  Library build string:               */
+const char* cosahip_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COSAHIP_H */

@@ -1,0 +1,1535 @@
+/*
+ * cosa_oracle.c -- CPU restatement of COSA's pseudo-time residual/update
+ * path.  TEST INFRASTRUCTURE ONLY (see cosa_oracle.h): it is the checker of
+ * libcosahip and is never linked into the product.
+ *
+ * Citations are file:line into LANCASTER-CFD/COSA src/.  Index conventions
+ * follow the Fortran explicit-shape dummies exactly:
+ *   P3 arrays  a(-1:imax+1,-1:jmax+1,-1:kmax+1,ncomp,0:2N)
+ *   P2 metrics s(lmet,0:imax+1,0:jmax+1,0:kmax+1,0:2N*hbmove)
+ *   P1 arrays  a(0:imax,0:jmax,0:kmax[,0:2N])
+ *   face flux  f(imax,jmax,kmax,npde,0:2N)
+ * Expressions keep the reference's evaluation order (Fortran and C agree on
+ * precedence and left-to-right association for + - * /).
+ */
+#include "cosa_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static cosahip_config P;
+static int NPDE = 5, NH = 0, DIM5 = 1, DIM5H = 1, LMET = 4;
+static double DHB[131 * 131];
+static double RESRMS[9];
+
+typedef long long ll;
+
+typedef struct {
+  int imax, jmax, kmax;
+  ll n3, n2, n1, n0;
+} bg;
+
+static bg mkbg(int imax, int jmax, int kmax) {
+  bg g;
+  g.imax = imax;
+  g.jmax = jmax;
+  g.kmax = kmax;
+  g.n3 = (ll)(imax + 3) * (jmax + 3) * (kmax + 3);
+  g.n2 = (ll)(imax + 2) * (jmax + 2) * (kmax + 2);
+  g.n1 = (ll)(imax + 1) * (jmax + 1) * (kmax + 1);
+  g.n0 = (ll)imax * jmax * kmax;
+  return g;
+}
+
+#define I3(g, i, j, k) \
+  ((ll)((i) + 1) + (ll)((g).imax + 3) * ((ll)((j) + 1) + (ll)((g).jmax + 3) * (ll)((k) + 1)))
+#define Q(a, g, i, j, k, v, n) (a)[I3(g, i, j, k) + (g).n3 * ((ll)((v) - 1) + (ll)NPDE * (n))]
+#define S1(a, g, i, j, k, n) (a)[I3(g, i, j, k) + (g).n3 * (ll)(n)]
+#define M(a, g, m, i, j, k, nh)                                              \
+  (a)[((m) - 1) + (ll)LMET * ((ll)(i) + (ll)((g).imax + 2) *                 \
+                                            ((ll)(j) + (ll)((g).jmax + 2) *  \
+                                                           ((ll)(k) + (ll)((g).kmax + 2) * (nh))))]
+#define X2(a, g, i, j, k, nh) \
+  (a)[(ll)(i) + (ll)((g).imax + 2) * ((ll)(j) + (ll)((g).jmax + 2) * ((ll)(k) + (ll)((g).kmax + 2) * (nh)))]
+#define P1(a, g, i, j, k, n) \
+  (a)[(ll)(i) + (ll)((g).imax + 1) * ((ll)(j) + (ll)((g).jmax + 1) * ((ll)(k) + (ll)((g).kmax + 1) * (n)))]
+#define V1(a, g, i, j, k) (a)[(ll)(i) + (ll)((g).imax + 1) * ((ll)(j) + (ll)((g).jmax + 1) * (ll)(k))]
+#define F0(a, g, i, j, k, v, n)                                                          \
+  (a)[(ll)((i) - 1) + (ll)(g).imax * ((ll)((j) - 1) + (ll)(g).jmax *                      \
+                                                      ((ll)((k) - 1) + (ll)(g).kmax *     \
+                                                                          ((ll)((v) - 1) + (ll)NPDE * (n))))]
+
+static int cyc[3][3] = {{1, 2, 3}, {2, 3, 1}, {3, 1, 2}}; /* cyc(idir,m) */
+#define CYC(d, m) cyc[(d) - 1][(m) - 1]
+#define KRD(a, b) ((a) == (b) ? 1 : 0)
+
+static int is_kom(void) { return P.kom || P.kom_bsl || P.kom_sst; }
+
+void or_set_config(const cosahip_config* cfg) {
+  P = *cfg;
+  NPDE = P.npde;
+  NH = P.nharms;
+  DIM5 = 2 * NH + 1;
+  DIM5H = 2 * NH * P.hbmove + 1;
+  LMET = P.lmet;
+}
+
+void or_set_dhb(int nsnap, const double* dhb) {
+  memcpy(DHB, dhb, sizeof(double) * (size_t)nsnap * nsnap);
+}
+#define D(i, j) DHB[(i) + (ll)DIM5 * (j)] /* dhb(i,j), column-major, 0-based */
+
+double or_get_resrms(int i) { return RESRMS[i]; }
+void or_set_resrms(int i, double v) { RESRMS[i] = v; }
+
+/* ---------------------------------------------------------------------- */
+/* cons2prim_b  (src/update.f90:2136-2188)                                 */
+void or_cons2prim_b(double* q, int imax, int jmax, int kmax, int xtnd) {
+  bg g = mkbg(imax, jmax, kmax);
+  int istr = 1 - xtnd, jstr = 1 - xtnd, kstr = 1 - xtnd;
+  int iend = imax - 1 + xtnd, jend = jmax - 1 + xtnd, kend = kmax - 1 + xtnd;
+  for (int n = 0; n <= 2 * NH; n++) {
+    for (int ipde = 2; ipde <= NPDE; ipde++) {
+      if (ipde == 5) continue;
+      for (int k = kstr; k <= kend; k++)
+        for (int j = jstr; j <= jend; j++)
+          for (int i = istr; i <= iend; i++)
+            Q(q, g, i, j, k, ipde, n) = Q(q, g, i, j, k, ipde, n) / Q(q, g, i, j, k, 1, n);
+    }
+    for (int k = kstr; k <= kend; k++)
+      for (int j = jstr; j <= jend; j++)
+        for (int i = istr; i <= iend; i++) {
+          double rho = Q(q, g, i, j, k, 1, n);
+          double u = Q(q, g, i, j, k, 2, n);
+          double v = Q(q, g, i, j, k, 3, n);
+          double w = Q(q, g, i, j, k, 4, n);
+          double rhoe = Q(q, g, i, j, k, 5, n);
+          double tke = is_kom() ? Q(q, g, i, j, k, 6, n) : 0.0;
+          Q(q, g, i, j, k, 5, n) = (P.gamma - 1) * (rhoe - rho * (u * u + v * v + w * w) / 2 - rho * tke);
+        }
+  }
+}
+
+/* prim2cons_b  (src/update.f90:1875-2084): interior + i-halo rows,
+ * j-halo and k-halo strips (no edges/corners).                           */
+static void p2c_range(double* q, bg g, int n, int i0, int i1, int j0, int j1, int k0, int k1) {
+  /* tke saved before the momentum scaling (work(...,1) = q(...,6)) */
+  for (int ipde = 2; ipde <= NPDE; ipde++) {
+    if (ipde == 5) continue;
+    for (int k = k0; k <= k1; k++)
+      for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) Q(q, g, i, j, k, ipde, n) = Q(q, g, i, j, k, ipde, n) * Q(q, g, i, j, k, 1, n);
+  }
+}
+static void p2c_energy(double* q, const double* tke_save, bg g, int n, int i0, int i1, int j0, int j1, int k0,
+                       int k1) {
+  for (int k = k0; k <= k1; k++)
+    for (int j = j0; j <= j1; j++)
+      for (int i = i0; i <= i1; i++) {
+        double rho = Q(q, g, i, j, k, 1, n);
+        double rhou = Q(q, g, i, j, k, 2, n);
+        double rhov = Q(q, g, i, j, k, 3, n);
+        double rhow = Q(q, g, i, j, k, 4, n);
+        double p = Q(q, g, i, j, k, 5, n);
+        double tke = tke_save ? tke_save[I3(g, i, j, k)] : 0.0;
+        Q(q, g, i, j, k, 5, n) = p / (P.gamma - 1) + (rhou * rhou + rhov * rhov + rhow * rhow) / 2 / rho + rho * tke;
+      }
+}
+void or_prim2cons_b(double* q, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  double* tk = NULL;
+  if (is_kom()) tk = (double*)malloc(sizeof(double) * (size_t)g.n3);
+  for (int n = 0; n <= 2 * NH; n++) {
+    int R[5][6] = {{-1, imax + 1, 1, jmax - 1, 1, kmax - 1},
+                   {1, imax - 1, -1, 0, 1, kmax - 1},
+                   {1, imax - 1, jmax, jmax + 1, 1, kmax - 1},
+                   {1, imax - 1, 1, jmax - 1, -1, 0},
+                   {1, imax - 1, 1, jmax - 1, kmax, kmax + 1}};
+    if (tk)
+      for (int r = 0; r < 5; r++)
+        for (int k = R[r][4]; k <= R[r][5]; k++)
+          for (int j = R[r][2]; j <= R[r][3]; j++)
+            for (int i = R[r][0]; i <= R[r][1]; i++) tk[I3(g, i, j, k)] = Q(q, g, i, j, k, 6, n);
+    /* the reference scales each variable over all five ranges in turn; the
+       ranges are disjoint, so range-by-range is the same arithmetic */
+    for (int r = 0; r < 5; r++) p2c_range(q, g, n, R[r][0], R[r][1], R[r][2], R[r][3], R[r][4], R[r][5]);
+    for (int r = 0; r < 5; r++) p2c_energy(q, tk, g, n, R[r][0], R[r][1], R[r][2], R[r][3], R[r][4], R[r][5]);
+  }
+  free(tk);
+}
+
+/* bprtest (src/cosa.f90:960-1120): interior + face halo strips; returns 0
+ * or 1 (pressure) / 2 (density) with where = {i,j,k,n}.                   */
+int or_bprtest(const double* q, int imax, int jmax, int kmax, int32_t* where) {
+  bg g = mkbg(imax, jmax, kmax);
+  int R[5][6] = {{-1, imax + 1, 1, jmax - 1, 1, kmax - 1},
+                 {1, imax - 1, -1, 0, 1, kmax - 1},
+                 {1, imax - 1, jmax, jmax + 1, 1, kmax - 1},
+                 {1, imax - 1, 1, jmax - 1, -1, 0},
+                 {1, imax - 1, 1, jmax - 1, kmax, kmax + 1}};
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int r = 0; r < 5; r++)
+      for (int k = R[r][4]; k <= R[r][5]; k++)
+        for (int j = R[r][2]; j <= R[r][3]; j++)
+          for (int i = R[r][0]; i <= R[r][1]; i++) {
+            double rho = Q(q, g, i, j, k, 1, n), p = Q(q, g, i, j, k, 5, n);
+            int bad = 0;
+            if (p <= P.qmin[4]) bad = 1;
+            else if (rho <= P.qmin[0]) bad = 2;
+            if (bad) {
+              if (where) { where[0] = i; where[1] = j; where[2] = k; where[3] = n; }
+              return bad;
+            }
+          }
+  return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* boundary conditions                                                     */
+
+typedef struct {
+  int idir, inrout, istrt[4], iend[4], ic1, ic2, ic3;
+} bcgeo;
+
+static bcgeo bc_decode(const int32_t* t) {
+  bcgeo b;
+  b.idir = t[1];
+  b.inrout = t[2];
+  b.istrt[1] = t[3]; b.iend[1] = t[4];
+  b.istrt[2] = t[5]; b.iend[2] = t[6];
+  b.istrt[3] = t[7]; b.iend[3] = t[8];
+  b.ic1 = CYC(b.idir, 1);
+  b.ic2 = CYC(b.idir, 2);
+  b.ic3 = CYC(b.idir, 3);
+  return b;
+}
+/* ijk index of (face coordinate a along idir, i2, i3 transverse) */
+static void bc_ijk(const bcgeo* b, int a, int i2, int i3, int* i, int* j, int* k) {
+  *i = a * KRD(b->ic1, 1) + i2 * KRD(b->ic2, 1) + i3 * KRD(b->ic3, 1);
+  *j = a * KRD(b->ic1, 2) + i2 * KRD(b->ic2, 2) + i3 * KRD(b->ic3, 2);
+  *k = a * KRD(b->ic1, 3) + i2 * KRD(b->ic2, 3) + i3 * KRD(b->ic3, 3);
+}
+
+/* bc_far0 (src/bc.f90:124-483): Riemann-invariant farfield (aircraft/vawt
+ * freestream), high-speed branch.                                         */
+static void or_bc_far0(double* q, const double* si, const double* sj, const double* sk, const int32_t* t, bg g) {
+  double qfst[5];
+  qfst[0] = P.rhoinf;
+  qfst[1] = P.machfs * cos(P.alpha) * cos(P.beta);
+  qfst[2] = P.machfs * sin(P.alpha) * cos(P.beta);
+  qfst[3] = P.machfs * sin(P.beta);
+  qfst[4] = P.pinf;
+  double gamma = P.gamma;
+  double sinf = P.pinf / pow(P.rhoinf, gamma);
+  int ijkmax[4] = {0, g.imax, g.jmax, g.kmax};
+  bcgeo b = bc_decode(t);
+  int ibcpt, ibcpt2, ibcn, ibcm;
+  double sgnm;
+  if (b.inrout == 1) {
+    ibcpt = 0; ibcpt2 = -1; ibcn = 1; ibcm = 1; sgnm = -1.0;
+  } else {
+    ibcpt = ijkmax[b.idir]; ibcpt2 = ijkmax[b.idir] + 1; ibcn = ijkmax[b.idir] - 1; ibcm = ijkmax[b.idir]; sgnm = 1.0;
+  }
+  const double* s = b.idir == 1 ? si : (b.idir == 2 ? sj : sk);
+  int kom = is_kom();
+  for (int n = 0; n <= 2 * NH; n++) {
+    int nh = n * P.hbmove;
+    for (int i3 = b.istrt[b.ic3]; i3 <= b.iend[b.ic3]; i3++)
+      for (int i2 = b.istrt[b.ic2]; i2 <= b.iend[b.ic2]; i2++) {
+        int ib, jb, kb, ib2, jb2, kb2, in, jn, kn, im, jm, km;
+        bc_ijk(&b, ibcpt, i2, i3, &ib, &jb, &kb);
+        bc_ijk(&b, ibcpt2, i2, i3, &ib2, &jb2, &kb2);
+        bc_ijk(&b, ibcn, i2, i3, &in, &jn, &kn);
+        bc_ijk(&b, ibcm, i2, i3, &im, &jm, &km);
+        double sx = M(s, g, 1, im, jm, km, nh) * sgnm;
+        double sy = M(s, g, 2, im, jm, km, nh) * sgnm;
+        double sz = M(s, g, 3, im, jm, km, nh) * sgnm;
+        double qext = Q(q, g, in, jn, kn, 2, n) * sx + Q(q, g, in, jn, kn, 3, n) * sy + Q(q, g, in, jn, kn, 4, n) * sz;
+        double qinf = qfst[1] * sx + qfst[2] * sy + qfst[3] * sz;
+        double rhoext = Q(q, g, in, jn, kn, 1, n);
+        double pext = Q(q, g, in, jn, kn, 5, n);
+        double aext = sqrt(gamma * pext / rhoext);
+        double rext = qext + 2 * aext / (gamma - 1);
+        double rinf = qinf - 2 * P.ainf / (gamma - 1);
+        double qdotn = (rext + rinf) / 2;
+        double a = (gamma - 1) * (rext - rinf) / 4;
+        double mach = qdotn / a;
+        double sext = pext / pow(rhoext, gamma);
+        double hv[5];
+        int set = 0;
+        if (qdotn <= 0.0) {
+          if (fabs(mach) >= 1.0) {
+            for (int v = 0; v < 5; v++) hv[v] = qfst[v];
+            set = 1;
+          } else if (fabs(mach) < 1.0) {
+            double rhobc = pow(a * a / (gamma * sinf), 1.0 / (gamma - 1));
+            double p = rhobc * a * a / gamma;
+            hv[0] = rhobc;
+            hv[1] = qfst[1] + (qdotn - qinf) * sx;
+            hv[2] = qfst[2] + (qdotn - qinf) * sy;
+            hv[3] = qfst[3] + (qdotn - qinf) * sz;
+            hv[4] = p;
+            set = 1;
+          }
+        } else if (qdotn > 0.0) {
+          if (fabs(mach) >= 1.0) {
+            for (int v = 0; v < 5; v++) hv[v] = Q(q, g, in, jn, kn, v + 1, n);
+            set = 1;
+          } else if (fabs(mach) < 1.0) {
+            double rhobc = pow(a * a / (gamma * sext), 1.0 / (gamma - 1));
+            double p = rhobc * a * a / gamma;
+            hv[0] = rhobc;
+            hv[1] = Q(q, g, in, jn, kn, 2, n) + (qdotn - qext) * sx;
+            hv[2] = Q(q, g, in, jn, kn, 3, n) + (qdotn - qext) * sy;
+            hv[3] = Q(q, g, in, jn, kn, 4, n) + (qdotn - qext) * sz;
+            hv[4] = p;
+            set = 1;
+          }
+        }
+        if (set)
+          for (int v = 0; v < 5; v++) {
+            Q(q, g, ib, jb, kb, v + 1, n) = hv[v];
+            Q(q, g, ib2, jb2, kb2, v + 1, n) = hv[v];
+          }
+        if (kom) {
+          /* src/bc.f90:404-479: k, omega fixed on inflow, extrapolated on outflow */
+          double t6, t7;
+          if (qdotn <= 0.0) {
+            t6 = P.tkefar; t7 = P.omefar;
+          } else {
+            t6 = Q(q, g, in, jn, kn, 6, n); t7 = Q(q, g, in, jn, kn, 7, n);
+          }
+          if (qdotn <= 0.0 || qdotn > 0.0) {
+            Q(q, g, ib, jb, kb, 6, n) = t6; Q(q, g, ib, jb, kb, 7, n) = t7;
+            Q(q, g, ib2, jb2, kb2, 6, n) = t6; Q(q, g, ib2, jb2, kb2, 7, n) = t7;
+          }
+        }
+      }
+  }
+}
+
+/* bc_walinv_14 (src/bc.f90:1457-1675): second-order extrapolation minus
+ * the wall-normal (relative) velocity.                                    */
+static void or_bc_walinv14(double* q, const double* si, const double* sj, const double* sk, const int32_t* t, bg g) {
+  int ijkmax[4] = {0, g.imax, g.jmax, g.kmax};
+  bcgeo b = bc_decode(t);
+  int ibcpt, ibcpt2, ibcn, ibcn2, ibcm;
+  if (b.inrout == 1) {
+    ibcpt = 0; ibcpt2 = -1; ibcn = 1; ibcn2 = 2; ibcm = 1;
+  } else {
+    ibcpt = ijkmax[b.idir]; ibcpt2 = ijkmax[b.idir] + 1; ibcn = ijkmax[b.idir] - 1; ibcn2 = ijkmax[b.idir] - 2;
+    ibcm = ijkmax[b.idir];
+  }
+  const double* s = b.idir == 1 ? si : (b.idir == 2 ? sj : sk);
+  for (int n = 0; n <= 2 * NH; n++) {
+    int nh = n * P.hbmove;
+    for (int i3 = b.istrt[b.ic3]; i3 <= b.iend[b.ic3]; i3++)
+      for (int i2 = b.istrt[b.ic2]; i2 <= b.iend[b.ic2]; i2++) {
+        int ib, jb, kb, ib2, jb2, kb2, in, jn, kn, in2, jn2, kn2, im, jm, km;
+        bc_ijk(&b, ibcpt, i2, i3, &ib, &jb, &kb);
+        bc_ijk(&b, ibcpt2, i2, i3, &ib2, &jb2, &kb2);
+        bc_ijk(&b, ibcn, i2, i3, &in, &jn, &kn);
+        bc_ijk(&b, ibcn2, i2, i3, &in2, &jn2, &kn2);
+        bc_ijk(&b, ibcm, i2, i3, &im, &jm, &km);
+        double nx = M(s, g, 1, im, jm, km, nh), ny = M(s, g, 2, im, jm, km, nh), nz = M(s, g, 3, im, jm, km, nh);
+        double xwdot = 0, ywdot = 0, zwdot = 0;
+        if (P.moving) {
+          xwdot = M(s, g, 5, im, jm, km, nh);
+          ywdot = M(s, g, 6, im, jm, km, nh);
+          zwdot = M(s, g, 7, im, jm, km, nh);
+        }
+        double rho1 = Q(q, g, in, jn, kn, 1, n), u1 = Q(q, g, in, jn, kn, 2, n), v1 = Q(q, g, in, jn, kn, 3, n);
+        double w1 = Q(q, g, in, jn, kn, 4, n), p1 = Q(q, g, in, jn, kn, 5, n);
+        double rho2 = Q(q, g, in2, jn2, kn2, 1, n), u2 = Q(q, g, in2, jn2, kn2, 2, n), v2 = Q(q, g, in2, jn2, kn2, 3, n);
+        double w2 = Q(q, g, in2, jn2, kn2, 4, n), p2 = Q(q, g, in2, jn2, kn2, 5, n);
+        double rhow = (3 * rho1 - rho2) / 2;
+        double uw = (3 * u1 - u2) / 2;
+        double vw = (3 * v1 - v2) / 2;
+        double ww = (3 * w1 - w2) / 2;
+        double pw = (3 * p1 - p2) / 2;
+        double un1 = u1 * nx + v1 * ny + w1 * nz;
+        double un2 = u2 * nx + v2 * ny + w2 * nz;
+        double unw = (3 * un1 - un2) / 2;
+        if (P.moving) unw = unw - (xwdot * nx + ywdot * ny + zwdot * nz);
+        double hv[5] = {rhow, uw - unw * nx, vw - unw * ny, ww - unw * nz, pw};
+        for (int v = 0; v < 5; v++) {
+          Q(q, g, ib, jb, kb, v + 1, n) = hv[v];
+          Q(q, g, ib2, jb2, kb2, v + 1, n) = hv[v];
+        }
+      }
+  }
+  if (is_kom()) {
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int i3 = b.istrt[b.ic3]; i3 <= b.iend[b.ic3]; i3++)
+        for (int i2 = b.istrt[b.ic2]; i2 <= b.iend[b.ic2]; i2++) {
+          int ib, jb, kb, ib2, jb2, kb2, in, jn, kn;
+          bc_ijk(&b, ibcpt, i2, i3, &ib, &jb, &kb);
+          bc_ijk(&b, ibcpt2, i2, i3, &ib2, &jb2, &kb2);
+          bc_ijk(&b, ibcn, i2, i3, &in, &jn, &kn);
+          for (int v = 6; v <= 7; v++) {
+            Q(q, g, ib, jb, kb, v, n) = Q(q, g, in, jn, kn, v, n);
+            Q(q, g, ib2, jb2, kb2, v, n) = Q(q, g, ib, jb, kb, v, n);
+          }
+        }
+  }
+}
+
+/* bc_extrapol (src/bc.f90:2494-2592): zeroth-order extrapolation. */
+static void or_bc_extrapol(double* q, const int32_t* t, bg g) {
+  int ijkmax[4] = {0, g.imax, g.jmax, g.kmax};
+  bcgeo b = bc_decode(t);
+  int ibcpt, ibcpt2, ibcn;
+  if (b.inrout == 1) {
+    ibcpt = 0; ibcpt2 = -1; ibcn = 1;
+  } else {
+    ibcpt = ijkmax[b.idir]; ibcpt2 = ijkmax[b.idir] + 1; ibcn = ijkmax[b.idir] - 1;
+  }
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int ipde = 1; ipde <= NPDE; ipde++)
+      for (int i3 = b.istrt[b.ic3]; i3 <= b.iend[b.ic3]; i3++)
+        for (int i2 = b.istrt[b.ic2]; i2 <= b.iend[b.ic2]; i2++) {
+          int ib, jb, kb, ib2, jb2, kb2, in, jn, kn;
+          bc_ijk(&b, ibcpt, i2, i3, &ib, &jb, &kb);
+          bc_ijk(&b, ibcpt2, i2, i3, &ib2, &jb2, &kb2);
+          bc_ijk(&b, ibcn, i2, i3, &in, &jn, &kn);
+          Q(q, g, ib, jb, kb, ipde, n) = Q(q, g, in, jn, kn, ipde, n);
+          Q(q, g, ib2, jb2, kb2, ipde, n) = Q(q, g, in, jn, kn, ipde, n);
+        }
+}
+
+/* bc_sym (src/bc.f90:2595-2710): mirror layers 1,2 with the velocity
+ * component idir negated.                                                 */
+static void or_bc_sym(double* q, const int32_t* t, bg g) {
+  int ijkmax[4] = {0, g.imax, g.jmax, g.kmax};
+  bcgeo b = bc_decode(t);
+  int ibcpt, ibcpt2, ibcn, ibcn2;
+  if (b.inrout == 1) {
+    ibcpt = 0; ibcpt2 = -1; ibcn = 1; ibcn2 = 2;
+  } else {
+    ibcpt = ijkmax[b.idir]; ibcpt2 = ijkmax[b.idir] + 1; ibcn = ijkmax[b.idir] - 1; ibcn2 = ijkmax[b.idir] - 2;
+  }
+  int isym[8];
+  for (int v = 1; v <= NPDE; v++) isym[v] = 1;
+  isym[b.idir + 1] = -1;
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int ipde = 1; ipde <= NPDE; ipde++)
+      for (int i3 = b.istrt[b.ic3]; i3 <= b.iend[b.ic3]; i3++)
+        for (int i2 = b.istrt[b.ic2]; i2 <= b.iend[b.ic2]; i2++) {
+          int ib, jb, kb, ib2, jb2, kb2, in, jn, kn, in2, jn2, kn2;
+          bc_ijk(&b, ibcpt, i2, i3, &ib, &jb, &kb);
+          bc_ijk(&b, ibcpt2, i2, i3, &ib2, &jb2, &kb2);
+          bc_ijk(&b, ibcn, i2, i3, &in, &jn, &kn);
+          bc_ijk(&b, ibcn2, i2, i3, &in2, &jn2, &kn2);
+          Q(q, g, ib, jb, kb, ipde, n) = isym[ipde] * Q(q, g, in, jn, kn, ipde, n);
+          Q(q, g, ib2, jb2, kb2, ipde, n) = isym[ipde] * Q(q, g, in2, jn2, kn2, ipde, n);
+        }
+}
+
+/* bbc (src/bc.f90:35-121): patches in input order. */
+void or_bbc(double* q, const double* si, const double* sj, const double* sk, const int32_t* bctopo, int nbcs,
+            int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  for (int ibc = 0; ibc < nbcs; ibc++) {
+    const int32_t* t = bctopo + 10 * ibc;
+    if (t[0] == 80) or_bc_sym(q, t, g);
+    else if (t[0] == 4) or_bc_extrapol(q, t, g);
+    else if (t[0] == 30) or_bc_far0(q, si, sj, sk, t, g);
+    else if (t[0] == 14) or_bc_walinv14(q, si, sj, sk, t, g);
+    /* other types are not built here; the library rejects them at define time */
+  }
+}
+
+/* cut_q, same-rank branch (src/bc.f90:1894-2141): two interior layers of
+ * block 2 into the two halo layers of block 1.                            */
+void or_cut_q(int imax1, int jmax1, int kmax1, double* q1, int imax2, int jmax2, int kmax2, const double* q2,
+              const int32_t* cutopo) {
+  bg g1 = mkbg(imax1, jmax1, kmax1), g2 = mkbg(imax2, jmax2, kmax2);
+  int ijkmax1[4] = {0, imax1, jmax1, kmax1}, ijkmax2[4] = {0, imax2, jmax2, kmax2};
+  int idir1 = cutopo[1], inout1 = cutopo[2];
+  int istr1[4], iend1[4], istr2[4], iend2[4], iord[4], isgn1[4], isgn2[4], len[4];
+  istr1[1] = cutopo[3]; iend1[1] = cutopo[4];
+  istr1[2] = cutopo[5]; iend1[2] = cutopo[6];
+  istr1[3] = cutopo[7]; iend1[3] = cutopo[8];
+  int idir2 = cutopo[10], inout2 = cutopo[11];
+  istr2[1] = cutopo[12]; iend2[1] = cutopo[13];
+  istr2[2] = cutopo[14]; iend2[2] = cutopo[15];
+  istr2[3] = cutopo[16]; iend2[3] = cutopo[17];
+  iord[1] = cutopo[18]; iord[2] = cutopo[19]; iord[3] = cutopo[20];
+  int ibcpt, ibcinc, inr, inrinc;
+  if (inout1 == 1) { ibcpt = 0; ibcinc = -1; } else { ibcpt = ijkmax1[idir1]; ibcinc = 1; }
+  if (inout2 == 1) { inr = 1; inrinc = 1; } else { inr = ijkmax2[idir2] - 1; inrinc = -1; }
+  for (int l = 1; l <= 3; l++) {
+    len[l] = abs(iend1[l] - istr1[l]);
+    isgn1[l] = iend1[l] > istr1[l] ? 1 : -1;
+    isgn2[l] = iend2[l] > istr2[l] ? 1 : -1;
+  }
+  int ic1 = CYC(idir1, 1), ic2 = CYC(idir1, 2), ic3 = CYC(idir1, 3);
+  int jc1 = iord[ic1], jc2 = iord[ic2], jc3 = iord[ic3];
+  for (int i3 = 0; i3 <= len[ic3]; i3++)
+    for (int i2 = 0; i2 <= len[ic2]; i2++) {
+      int a2 = istr1[ic2] + isgn1[ic2] * i2, a3 = istr1[ic3] + isgn1[ic3] * i3;
+      int ii1 = ibcpt * KRD(ic1, 1) + a2 * KRD(ic2, 1) + a3 * KRD(ic3, 1);
+      int jj1 = ibcpt * KRD(ic1, 2) + a2 * KRD(ic2, 2) + a3 * KRD(ic3, 2);
+      int kk1 = ibcpt * KRD(ic1, 3) + a2 * KRD(ic2, 3) + a3 * KRD(ic3, 3);
+      int ii2 = ii1 + ibcinc * KRD(ic1, 1), jj2 = jj1 + ibcinc * KRD(ic1, 2), kk2 = kk1 + ibcinc * KRD(ic1, 3);
+      int b2 = istr2[jc2] + isgn2[jc2] * i2, b3 = istr2[jc3] + isgn2[jc3] * i3;
+      int in1 = inr * KRD(jc1, 1) + b2 * KRD(jc2, 1) + b3 * KRD(jc3, 1);
+      int jn1 = inr * KRD(jc1, 2) + b2 * KRD(jc2, 2) + b3 * KRD(jc3, 2);
+      int kn1 = inr * KRD(jc1, 3) + b2 * KRD(jc2, 3) + b3 * KRD(jc3, 3);
+      int in2 = in1 + inrinc * KRD(jc1, 1), jn2 = jn1 + inrinc * KRD(jc1, 2), kn2 = kn1 + inrinc * KRD(jc1, 3);
+      for (int n = 0; n <= 2 * NH; n++)
+        for (int ipde = 1; ipde <= NPDE; ipde++) {
+          Q(q1, g1, ii1, jj1, kk1, ipde, n) = Q(q2, g2, in1, jn1, kn1, ipde, n);
+          Q(q1, g1, ii2, jj2, kk2, ipde, n) = Q(q2, g2, in2, jn2, kn2, ipde, n);
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* spectral radii and local time step                                      */
+
+/* eig_eul (src/cosa.f90:4183-4237) */
+static void eig_eul(const double* qc, const double* sim, const double* sip, const double* sjm, const double* sjp,
+                    const double* skm, const double* skp, double xdot, double ydot, double zdot, double* lami,
+                    double* lamj, double* lamk) {
+  double rho = qc[0], u = qc[1], v = qc[2], w = qc[3], p = qc[4];
+  double a = sqrt(P.gamma * p / rho);
+  double xix = 0.5 * (sim[0] * sim[3] + sip[0] * sip[3]);
+  double xiy = 0.5 * (sim[1] * sim[3] + sip[1] * sip[3]);
+  double xiz = 0.5 * (sim[2] * sim[3] + sip[2] * sip[3]);
+  double etax = 0.5 * (sjm[0] * sjm[3] + sjp[0] * sjp[3]);
+  double etay = 0.5 * (sjm[1] * sjm[3] + sjp[1] * sjp[3]);
+  double etaz = 0.5 * (sjm[2] * sjm[3] + sjp[2] * sjp[3]);
+  double zetax = 0.5 * (skm[0] * skm[3] + skp[0] * skp[3]);
+  double zetay = 0.5 * (skm[1] * skm[3] + skp[1] * skp[3]);
+  double zetaz = 0.5 * (skm[2] * skm[3] + skp[2] * skp[3]);
+  double axi = (sim[3] + sip[3]) / 2;
+  double aeta = (sjm[3] + sjp[3]) / 2;
+  double azeta = (skm[3] + skp[3]) / 2;
+  if (P.moving) {
+    u = u - xdot;
+    v = v - ydot;
+    w = w - zdot;
+  }
+  *lami = fabs(u * xix + v * xiy + w * xiz) + a * axi;
+  *lamj = fabs(u * etax + v * etay + w * etaz) + a * aeta;
+  *lamk = fabs(u * zetax + v * zetay + w * zetaz) + a * azeta;
+}
+
+/* lambdab, inviscid branch (src/cosa.f90:1248-1346); lambdas is a P3
+ * array with npde components of which 1..3 are written.                  */
+void or_lambdab(const double* q, const double* si, const double* sj, const double* sk, const double* xdot,
+                const double* ydot, const double* zdot, double* lambdas, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  for (int n = 0; n <= 2 * NH; n++) {
+    int nh = n * P.hbmove;
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          double qc[5], sim[4], sip[4], sjm[4], sjp[4], skm[4], skp[4];
+          for (int v = 0; v < 5; v++) qc[v] = Q(q, g, i, j, k, v + 1, n);
+          for (int m = 0; m < 4; m++) {
+            sim[m] = M(si, g, m + 1, i, j, k, nh);
+            sip[m] = M(si, g, m + 1, i + 1, j, k, nh);
+            sjm[m] = M(sj, g, m + 1, i, j, k, nh);
+            sjp[m] = M(sj, g, m + 1, i, j + 1, k, nh);
+            skm[m] = M(sk, g, m + 1, i, j, k, nh);
+            skp[m] = M(sk, g, m + 1, i, j, k + 1, nh);
+          }
+          double xd = 0, yd = 0, zd = 0;
+          if (P.moving) {
+            xd = X2(xdot, g, i, j, k, nh);
+            yd = X2(ydot, g, i, j, k, nh);
+            zd = X2(zdot, g, i, j, k, nh);
+          }
+          double li, lj, lk;
+          eig_eul(qc, sim, sip, sjm, sjp, skm, skp, xd, yd, zd, &li, &lj, &lk);
+          Q(lambdas, g, i, j, k, 1, n) = li;
+          Q(lambdas, g, i, j, k, 2, n) = lj;
+          Q(lambdas, g, i, j, k, 3, n) = lk;
+        }
+  }
+}
+
+/* bdeltat (src/cosa.f90:4733-5203), inviscid, no k-omega ramping:
+ * dtvol = cfl / sum(lambda); CIRS coefficients for irsop = 2 and 3.       */
+void or_bdeltat(double* dtvol, const double* lambdas, double* betaxi, double* betaeta, double* betazeta, int imax,
+                int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  const double zeta = 0.4;
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++)
+          P1(dtvol, g, i, j, k, n) = P.cfl / (Q(lambdas, g, i, j, k, 1, n) + Q(lambdas, g, i, j, k, 2, n) +
+                                              Q(lambdas, g, i, j, k, 3, n));
+  if (P.irsop == 2) {
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int k = 1; k <= kmax - 1; k++)
+        for (int j = 1; j <= jmax - 1; j++)
+          for (int i = 1; i <= imax - 1; i++) {
+            double lami = Q(lambdas, g, i, j, k, 1, n), lamj = Q(lambdas, g, i, j, k, 2, n),
+                   lamk = Q(lambdas, g, i, j, k, 3, n);
+            double phixi = 1 + pow(lamj / lami, zeta) + pow(lamk / lami, zeta);
+            double phieta = 1 + pow(lami / lamj, zeta) + pow(lamk / lamj, zeta);
+            double phizeta = 1 + pow(lami / lamk, zeta) + pow(lamj / lamk, zeta);
+            double t, bx, by, bz;
+            t = P.rat * lami * phixi / (lami + lamj + lamk);
+            bx = (t * t - 1) / 4;
+            P1(betaxi, g, i, j, k, n) = fmax(0.0, bx);
+            t = P.rat * lamj * phieta / (lami + lamj + lamk);
+            by = (t * t - 1) / 4;
+            P1(betaeta, g, i, j, k, n) = fmax(0.0, by);
+            t = P.rat * lamk * phizeta / (lami + lamj + lamk);
+            bz = (t * t - 1) / 4;
+            P1(betazeta, g, i, j, k, n) = fmax(0.0, bz);
+          }
+  } else if (P.irsop == 3) {
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int k = 1; k <= kmax - 1; k++)
+        for (int j = 1; j <= jmax - 1; j++)
+          for (int i = 1; i <= imax - 1; i++) {
+            double lami = Q(lambdas, g, i, j, k, 1, n), lamj = Q(lambdas, g, i, j, k, 2, n),
+                   lamk = Q(lambdas, g, i, j, k, 3, n);
+            double rji = lamj / lami, rki = lamk / lami, rij = lami / lamj, rkj = lamk / lamj, rik = lami / lamk,
+                   rjk = lamj / lamk;
+            double t, bx, by, bz;
+            t = P.rat / (1 + P.psirs * (rji + rki));
+            bx = 0.25 * (t * t - 1);
+            P1(betaxi, g, i, j, k, n) = fmax(0.0, bx);
+            t = P.rat / (1 + P.psirs * (rij + rkj));
+            by = 0.25 * (t * t - 1);
+            P1(betaeta, g, i, j, k, n) = fmax(0.0, by);
+            t = P.rat / (1 + P.psirs * (rik + rjk));
+            bz = 0.25 * (t * t - 1);
+            P1(betazeta, g, i, j, k, n) = fmax(0.0, bz);
+          }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* MUSCL reconstruction, limiters, Roe flux, flux difference               */
+
+/* direction-generic indexing: position a along idir, (b,c) transverse in
+ * the reference's loop order (idir 1: j,k; idir 2: i,k; idir 3: i,j).     */
+static ll dI3(bg g, int idir, int a, int t1, int t2) {
+  if (idir == 1) return I3(g, a, t1, t2);
+  if (idir == 2) return I3(g, t1, a, t2);
+  return I3(g, t1, t2, a);
+}
+#define QD(arr, v, n, idx) (arr)[(idx) + g.n3 * ((ll)((v) - 1) + (ll)NPDE * (n))]
+
+static int strong_bc(int t) {
+  return t == 13 || t == 14 || t / 100 == 15 || t == 22 || t == 23 || t == 92 || t == 93 || t == 5;
+}
+
+/* limit_i/j/k (src/flux.f90:3472-3888) along idir for variables vs..ve.  */
+static void or_limit(int idir, const double* dq, double* dqp, double* dqm, double epslim, double rkap, int vs, int ve,
+                     int ilim, bg g) {
+  int L = idir == 1 ? g.imax : (idir == 2 ? g.jmax : g.kmax);
+  int T1 = idir == 1 ? g.jmax : g.imax;
+  int T2 = idir == 3 ? g.jmax : g.kmax;
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int v = vs; v <= ve; v++)
+      for (int t2 = 1; t2 <= T2 - 1; t2++)
+        for (int t1 = 1; t1 <= T1 - 1; t1++) {
+          if (ilim == 0) {
+            double rkp = 1 + rkap, rkm = 1 - rkap;
+            for (int a = 1; a <= L; a++) {
+              double dm1 = QD(dq, v, n, dI3(g, idir, a - 1, t1, t2)), d0 = QD(dq, v, n, dI3(g, idir, a, t1, t2)),
+                     dp1 = QD(dq, v, n, dI3(g, idir, a + 1, t1, t2));
+              QD(dqp, v, n, dI3(g, idir, a, t1, t2)) = -(rkp * d0 + rkm * dp1) / 4;
+              QD(dqm, v, n, dI3(g, idir, a, t1, t2)) = (rkm * dm1 + rkp * d0) / 4;
+            }
+          } else if (ilim == 1) {
+            for (int a = 1; a <= L; a++) {
+              double dm1 = QD(dq, v, n, dI3(g, idir, a - 1, t1, t2)), d0 = QD(dq, v, n, dI3(g, idir, a, t1, t2)),
+                     dp1 = QD(dq, v, n, dI3(g, idir, a + 1, t1, t2));
+              double s = (2 * dp1 * d0 + epslim) / (dp1 * dp1 + d0 * d0 + epslim);
+              double rkp = 1 + rkap * s, rkm = 1 - rkap * s;
+              QD(dqp, v, n, dI3(g, idir, a, t1, t2)) = -s * (rkp * d0 + rkm * dp1) / 4;
+              s = (2 * d0 * dm1 + epslim) / (d0 * d0 + dm1 * dm1 + epslim);
+              rkp = 1 + rkap * s;
+              rkm = 1 - rkap * s;
+              QD(dqm, v, n, dI3(g, idir, a, t1, t2)) = s * (rkm * dm1 + rkp * d0) / 4;
+            }
+          } else if (ilim == 2) {
+            double rkp = 1 + rkap, rkm = 1 - rkap, b = (3.0 - rkap) / (1.0 - rkap);
+            for (int a = 0; a <= L; a++) {
+              double d0 = QD(dq, v, n, dI3(g, idir, a, t1, t2)), d1 = QD(dq, v, n, dI3(g, idir, a + 1, t1, t2));
+              double dqmx = (copysign(1.0, d0) + copysign(1.0, d1)) / 2 * fmin(fabs(d0), fabs(d1) * b);
+              double dqpx = (copysign(1.0, d1) + copysign(1.0, d0)) / 2 * fmin(fabs(d1), fabs(d0) * b);
+              QD(dqp, v, n, dI3(g, idir, a, t1, t2)) = -(rkp * dqmx + rkm * dqpx) / 4;
+              QD(dqm, v, n, dI3(g, idir, a + 1, t1, t2)) = (rkm * dqmx + rkp * dqpx) / 4;
+            }
+          } else if (ilim == 3) {
+            double epsq = epslim * epslim;
+            for (int a = 0; a <= L; a++) {
+              double d1 = QD(dq, v, n, dI3(g, idir, a + 1, t1, t2)), d2 = QD(dq, v, n, dI3(g, idir, a, t1, t2));
+              double dqmx = (d1 * (d2 * d2 + 2 * epsq) + d2 * (2 * d1 * d1 + epsq)) /
+                            (2 * d1 * d1 - d1 * d2 + 2 * d2 * d2 + 3 * epsq);
+              double dqpx = (d2 * (d1 * d1 + 2 * epsq) + d1 * (2 * d2 * d2 + epsq)) /
+                            (2 * d2 * d2 - d2 * d1 + 2 * d1 * d1 + 3 * epsq);
+              QD(dqp, v, n, dI3(g, idir, a, t1, t2)) = -dqpx / 2;
+              QD(dqm, v, n, dI3(g, idir, a + 1, t1, t2)) = dqmx / 2;
+            }
+          } else if (ilim == 4) {
+            double epsq = epslim * epslim;
+            for (int a = 0; a <= L; a++) {
+              double d1 = QD(dq, v, n, dI3(g, idir, a + 1, t1, t2)), d2 = QD(dq, v, n, dI3(g, idir, a, t1, t2));
+              double dqmx = (d1 * (d2 * d2 + epsq) + d2 * (d1 * d1 + epsq)) / (d1 * d1 + d2 * d2 + 2 * epsq);
+              double dqpx = dqmx;
+              QD(dqp, v, n, dI3(g, idir, a, t1, t2)) = -dqpx / 2;
+              QD(dqm, v, n, dI3(g, idir, a + 1, t1, t2)) = dqmx / 2;
+            }
+          }
+        }
+}
+
+/* muscl_bi/bj/bk (src/flux.f90:1221-1812) */
+void or_muscl_b(int idir, int nl, const int32_t* bctopo, int nbcs, const double* q, double* qp, double* qm, double* dq,
+                double* dqp, double* dqm, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  int L = idir == 1 ? imax : (idir == 2 ? jmax : kmax);
+  int T1 = idir == 1 ? jmax : imax;
+  int T2 = idir == 3 ? jmax : kmax;
+  int fo_beg = 0, fo_end = -1, so_beg = 1, so_end = NPDE, dofirst = 0, doscnd = 1;
+  if (nl >= P.nl_crs) {
+    fo_beg = 1; fo_end = NPDE; so_beg = 0; so_end = -1; dofirst = 1; doscnd = 0;
+  } else if (P.foturb) {
+    fo_beg = 6; fo_end = NPDE; so_beg = 1; so_end = 5; dofirst = 1; doscnd = 1;
+  }
+  /* transverse index ranges of a patch: idir 1 -> (6,7),(8,9); idir 2 ->
+     (4,5),(8,9); idir 3 -> (4,5),(6,7) */
+  int r1 = idir == 1 ? 5 : 3, r2 = idir == 3 ? 5 : 7;
+  if (dofirst)
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int v = fo_beg; v <= fo_end; v++)
+        for (int t2 = 1; t2 <= T2 - 1; t2++)
+          for (int t1 = 1; t1 <= T1 - 1; t1++)
+            for (int a = 1; a <= L; a++) {
+              QD(qp, v, n, dI3(g, idir, a, t1, t2)) = QD(q, v, n, dI3(g, idir, a, t1, t2));
+              QD(qm, v, n, dI3(g, idir, a, t1, t2)) = QD(q, v, n, dI3(g, idir, a - 1, t1, t2));
+            }
+  if (doscnd) {
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int v = so_beg; v <= so_end; v++)
+        for (int t2 = 1; t2 <= T2 - 1; t2++)
+          for (int t1 = 1; t1 <= T1 - 1; t1++)
+            for (int a = 0; a <= L + 1; a++)
+              QD(dq, v, n, dI3(g, idir, a, t1, t2)) =
+                  QD(q, v, n, dI3(g, idir, a, t1, t2)) - QD(q, v, n, dI3(g, idir, a - 1, t1, t2));
+    for (int ibc = 0; ibc < nbcs; ibc++) {
+      const int32_t* t = bctopo + 10 * ibc;
+      if (t[1] != idir || !strong_bc(t[0])) continue;
+      int abc = t[2] == 1 ? 1 : L;
+      for (int n = 0; n <= 2 * NH; n++)
+        for (int v = so_beg; v <= so_end; v++)
+          for (int t2 = t[r2]; t2 <= t[r2 + 1]; t2++)
+            for (int t1 = t[r1]; t1 <= t[r1 + 1]; t1++)
+              QD(dq, v, n, dI3(g, idir, abc, t1, t2)) =
+                  (QD(q, v, n, dI3(g, idir, abc, t1, t2)) - QD(q, v, n, dI3(g, idir, abc - 1, t1, t2))) * 2;
+    }
+    or_limit(idir, dq, dqp, dqm, P.epslimf, P.rkap, 1, 5, P.ilimf, g);
+    or_limit(idir, dq, dqp, dqm, P.epslimt, P.rkap, 6, NPDE, P.ilimt, g);
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int v = so_beg; v <= so_end; v++)
+        for (int t2 = 1; t2 <= T2 - 1; t2++)
+          for (int t1 = 1; t1 <= T1 - 1; t1++)
+            for (int a = 1; a <= L; a++) {
+              QD(qp, v, n, dI3(g, idir, a, t1, t2)) =
+                  QD(q, v, n, dI3(g, idir, a, t1, t2)) + QD(dqp, v, n, dI3(g, idir, a, t1, t2));
+              QD(qm, v, n, dI3(g, idir, a, t1, t2)) =
+                  QD(q, v, n, dI3(g, idir, a - 1, t1, t2)) + QD(dqm, v, n, dI3(g, idir, a, t1, t2));
+            }
+  }
+  for (int ibc = 0; ibc < nbcs; ibc++) {
+    const int32_t* t = bctopo + 10 * ibc;
+    if (t[1] != idir || !strong_bc(t[0])) continue;
+    int aqpm = t[2] == 1 ? 1 : L;
+    int abc = t[2] == 1 ? 0 : L;
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int v = 1; v <= NPDE; v++)
+        for (int t2 = t[r2]; t2 <= t[r2 + 1]; t2++)
+          for (int t1 = t[r1]; t1 <= t[r1 + 1]; t1++) {
+            double val = QD(q, v, n, dI3(g, idir, abc, t1, t2));
+            QD(qp, v, n, dI3(g, idir, aqpm, t1, t2)) = val;
+            QD(qm, v, n, dI3(g, idir, aqpm, t1, t2)) = val;
+          }
+  }
+}
+
+/* roflux_b (src/flux.f90:45-387), mean flow part (k-omega transport of the
+ * turbulent variables is not built in this restatement).                  */
+void or_roflux_b(int idir, const double* qr, const double* ql, double* flux, const double* sijk, int imax, int jmax,
+                 int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  int ic = idir == 1 ? imax : imax - 1, jc = idir == 2 ? jmax : jmax - 1, kc = idir == 3 ? kmax : kmax - 1;
+  double gamma = P.gamma;
+  for (int n = 0; n <= 2 * NH; n++) {
+    int nh = n * P.hbmove;
+    for (int k = 1; k <= kc; k++)
+      for (int j = 1; j <= jc; j++)
+        for (int i = 1; i <= ic; i++) {
+          double ubn = 0.0;
+          double kx = M(sijk, g, 1, i, j, k, nh), ky = M(sijk, g, 2, i, j, k, nh), kz = M(sijk, g, 3, i, j, k, nh),
+                 ds = M(sijk, g, 4, i, j, k, nh);
+          if (P.moving)
+            ubn = M(sijk, g, 5, i, j, k, nh) * kx + M(sijk, g, 6, i, j, k, nh) * ky + M(sijk, g, 7, i, j, k, nh) * kz;
+          double fr[5], fl[5], df[5];
+          double rhor = Q(qr, g, i, j, k, 1, n), ur = Q(qr, g, i, j, k, 2, n), vr = Q(qr, g, i, j, k, 3, n),
+                 wr = Q(qr, g, i, j, k, 4, n), pr = Q(qr, g, i, j, k, 5, n);
+          double unr = ur * kx + vr * ky + wr * kz - ubn;
+          double q2r = ur * ur + vr * vr + wr * wr;
+          double tker = 0.0;
+          double h0r = gamma / (gamma - 1) * pr / rhor + q2r / 2 + tker;
+          double rh0r = rhor * h0r;
+          fr[0] = rhor * unr;
+          fr[1] = (rhor * ur * unr + pr * kx);
+          fr[2] = (rhor * vr * unr + pr * ky);
+          fr[3] = (rhor * wr * unr + pr * kz);
+          fr[4] = (rh0r * unr + pr * ubn);
+          double rhol = Q(ql, g, i, j, k, 1, n), ul = Q(ql, g, i, j, k, 2, n), vl = Q(ql, g, i, j, k, 3, n),
+                 wl = Q(ql, g, i, j, k, 4, n), pl = Q(ql, g, i, j, k, 5, n);
+          double unl = ul * kx + vl * ky + wl * kz - ubn;
+          double q2l = ul * ul + vl * vl + wl * wl;
+          double tkel = 0.0;
+          double h0l = gamma / (gamma - 1) * pl / rhol + q2l / 2 + tkel;
+          double rh0l = rhol * h0l;
+          fl[0] = rhol * unl;
+          fl[1] = (rhol * ul * unl + pl * kx);
+          fl[2] = (rhol * vl * unl + pl * ky);
+          fl[3] = (rhol * wl * unl + pl * kz);
+          fl[4] = (rh0l * unl + pl * ubn);
+          double drho = rhor - rhol, dp = pr - pl, du = ur - ul, dv = vr - vl, dw = wr - wl, dun = unr - unl,
+                 dtke = tker - tkel;
+          double rhols = sqrt(rhol), rhors = sqrt(rhor);
+          double rho = rhols * rhors;
+          double denom = rhols + rhors;
+          double u = (rhols * ul + rhors * ur) / denom;
+          double v = (rhols * vl + rhors * vr) / denom;
+          double w = (rhols * wl + rhors * wr) / denom;
+          double un = u * kx + v * ky + w * kz;
+          double h0 = (rhols * h0l + rhors * h0r) / denom;
+          double tke = (rhols * tkel + rhors * tker) / denom;
+          double q2 = u * u + v * v + w * w;
+          double a2 = (gamma - 1) * (h0 - q2 / 2 - tke);
+          double a = sqrt(a2);
+          double rm123 = fabs(un - ubn), rm4 = fabs(un + a - ubn), rm5 = fabs(un - a - ubn);
+          if (P.efx_setup_f) {
+            double rdelta = P.cntrpy_f * (P.entfx_f_w[0] * P.entfxctf_f + P.entfx_f_w[1] * (sqrt(u * u + v * v + w * w) + a) +
+                                          P.entfx_f_w[2] * 0.2 * (fabs(un) + a + fabs(ubn)));
+            rdelta = fmax(rdelta, 1.e-12);
+            if (rm123 <= rdelta && P.efx_all_f) rm123 = 0.5 * (rm123 * rm123 + rdelta * rdelta) / rdelta;
+            if (rm4 <= rdelta) rm4 = 0.5 * (rm4 * rm4 + rdelta * rdelta) / rdelta;
+            if (rm5 <= rdelta) rm5 = 0.5 * (rm5 * rm5 + rdelta * rdelta) / rdelta;
+          }
+          if (P.eig_limit_f) {
+            if (rm123 <= P.eig_cutoff_f) rm123 = P.eig_cutoff_f;
+            if (P.eig_lim_all_f && rm4 <= P.eig_cutoff_f) rm4 = P.eig_cutoff_f;
+            if (P.eig_lim_all_f && rm5 <= P.eig_cutoff_f) rm5 = P.eig_cutoff_f;
+          }
+          double al1 = rm123 * (drho - dp / a2);
+          double al2 = rm123 * rho;
+          double al3 = rm4 * (dp / a2 + rho * dun / a) / 2;
+          double al4 = rm5 * (dp / a2 - rho * dun / a) / 2;
+          df[0] = al1 + al3 + al4;
+          df[1] = al1 * u + al2 * (du - dun * kx) + al3 * (u + a * kx) + al4 * (u - a * kx);
+          df[2] = al1 * v + al2 * (dv - dun * ky) + al3 * (v + a * ky) + al4 * (v - a * ky);
+          df[3] = al1 * w + al2 * (dw - dun * kz) + al3 * (w + a * kz) + al4 * (w - a * kz);
+          df[4] = al1 * (q2 / 2 + tke) + al2 * (u * du + v * dv + w * dw - un * dun + dtke) + al3 * (h0 + a * un) +
+                  al4 * (h0 - a * un);
+          for (int v5 = 0; v5 < 5; v5++) F0(flux, g, i, j, k, v5 + 1, n) = (fr[v5] + fl[v5] - df[v5]) / 2 * ds;
+        }
+  }
+}
+
+/* bresid (src/cosa.f90:1149-1211) */
+void or_bresid(int idir, const double* flux, double* res, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  int di = idir == 1, dj = idir == 2, dk = idir == 3;
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int ipde = 1; ipde <= NPDE; ipde++)
+      for (int k = 1; k <= kmax - 1; k++)
+        for (int j = 1; j <= jmax - 1; j++)
+          for (int i = 1; i <= imax - 1; i++)
+            Q(res, g, i, j, k, ipde, n) =
+                Q(res, g, i, j, k, ipde, n) + F0(flux, g, i + di, j + dj, k + dk, ipde, n) - F0(flux, g, i, j, k, ipde, n);
+}
+
+/* ---------------------------------------------------------------------- */
+/* harmonic balance source  src_bhb (src/src.f90:556-613)                  */
+void or_src_bhb(const double* q, const double* vol, double* res, double* qhb, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  for (int ni = 0; ni <= 2 * NH; ni++)
+    for (int ipde = 1; ipde <= NPDE; ipde++)
+      for (int k = 1; k <= kmax - 1; k++)
+        for (int j = 1; j <= jmax - 1; j++)
+          for (int i = 1; i <= imax - 1; i++) Q(qhb, g, i, j, k, ipde, ni) = 0.0;
+  for (int ni = 0; ni <= 2 * NH; ni++)
+    for (int nj = 0; nj <= 2 * NH; nj++)
+      for (int ipde = 1; ipde <= NPDE; ipde++)
+        for (int k = 1; k <= kmax - 1; k++)
+          for (int j = 1; j <= jmax - 1; j++)
+            for (int i = 1; i <= imax - 1; i++)
+              Q(qhb, g, i, j, k, ipde, ni) =
+                  Q(qhb, g, i, j, k, ipde, ni) + P.omega * V1(vol, g, i, j, k) * D(ni, nj) * Q(q, g, i, j, k, ipde, nj);
+  for (int ni = 0; ni <= 2 * NH; ni++)
+    for (int ipde = 1; ipde <= NPDE; ipde++)
+      for (int k = 1; k <= kmax - 1; k++)
+        for (int j = 1; j <= jmax - 1; j++)
+          for (int i = 1; i <= imax - 1; i++)
+            Q(res, g, i, j, k, ipde, ni) = Q(res, g, i, j, k, ipde, ni) + Q(qhb, g, i, j, k, ipde, ni);
+}
+
+/* ---------------------------------------------------------------------- */
+/* residual norms  resg_brms (src/cosa.f90:4503-4582), explicit branch     */
+void or_resg_brms(const double* res, const double* rhs, double* rb, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  for (int p = 0; p <= NPDE; p++) rb[p] = 0;
+  for (int ipde = 1; ipde <= NPDE; ipde++)
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          double d = Q(res, g, i, j, k, ipde, 0) - Q(rhs, g, i, j, k, ipde, 0);
+          rb[ipde - 1] = rb[ipde - 1] + d * d;
+        }
+  for (int ipde = 1; ipde <= NPDE; ipde++) rb[NPDE] = rb[NPDE] + rb[ipde - 1];
+}
+
+/* resg_brms_hb (src/cosa.f90:4629-4697), explicit branch */
+void or_resg_brms_hb(const double* work1, double* rb, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  for (int p = 0; p <= NPDE; p++) rb[p] = 0;
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int ipde = 1; ipde <= NPDE; ipde++)
+      for (int k = 1; k <= kmax - 1; k++)
+        for (int j = 1; j <= jmax - 1; j++)
+          for (int i = 1; i <= imax - 1; i++) {
+            double d = Q(work1, g, i, j, k, ipde, n);
+            rb[ipde - 1] = rb[ipde - 1] + d * d;
+          }
+  for (int ipde = 1; ipde <= NPDE; ipde++) rb[NPDE] = rb[NPDE] + rb[ipde - 1];
+}
+
+/* ---------------------------------------------------------------------- */
+/* implicit residual smoothing  cirs_b + tridi (src/cosa.f90:5238-5623,
+ * 2418-2443), Euler variables 1..5 (k-omega variables are not built).     */
+
+/* tridi (src/cosa.f90:2418-2443).  For dim = 1 the reference reads r(0)
+ * and t(0) out of bounds; here they are taken as 0 (x = x/b).             */
+void or_tridi(const double* a, const double* b, const double* c, double* x, int dim) {
+  double* r = (double*)malloc(sizeof(double) * (dim + 1));
+  double* t = (double*)malloc(sizeof(double) * (dim + 1));
+  r[0] = 0; t[0] = 0; /* 1-based below */
+  r[1] = c[0] / b[0];
+  t[1] = x[0] / b[0];
+  for (int j = 2; j <= dim - 1; j++) {
+    r[j] = c[j - 1] / (b[j - 1] - r[j - 1] * a[j - 1]);
+    t[j] = (x[j - 1] - t[j - 1] * a[j - 1]) / (b[j - 1] - r[j - 1] * a[j - 1]);
+  }
+  if (dim >= 2)
+    t[dim] = (x[dim - 1] - t[dim - 1] * a[dim - 1]) / (b[dim - 1] - r[dim - 1] * a[dim - 1]);
+  x[dim - 1] = t[dim];
+  for (int j = dim - 1; j >= 1; j--) x[j - 1] = t[j] - x[j] * r[j];
+  free(r);
+  free(t);
+}
+
+void or_cirs_b(double* res, const double* dtvol, const double* betaxi, const double* betaeta, const double* betazeta,
+               int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  int im = imax - 1, jm = jmax - 1, km = kmax - 1;
+  int nmax = imax > jmax ? imax : jmax;
+  if (kmax > nmax) nmax = kmax;
+  double *wa = (double*)malloc(sizeof(double) * nmax), *wb = (double*)malloc(sizeof(double) * nmax),
+         *wc = (double*)malloc(sizeof(double) * nmax), *wx = (double*)malloc(sizeof(double) * nmax);
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int ipde = 1; ipde <= 5; ipde++)
+      for (int k = 1; k <= km; k++)
+        for (int j = 1; j <= jm; j++)
+          for (int i = 1; i <= im; i++) Q(res, g, i, j, k, ipde, n) = Q(res, g, i, j, k, ipde, n) * P1(dtvol, g, i, j, k, n);
+  const double* beta[4] = {NULL, betaxi, betaeta, betazeta};
+  for (int idir = 1; idir <= 3; idir++) {
+    int L = idir == 1 ? im : (idir == 2 ? jm : km);
+    int T1 = idir == 1 ? jm : im;
+    int T2 = idir == 3 ? jm : km;
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int ipde = 1; ipde <= 5; ipde++)
+        for (int t2 = 1; t2 <= T2; t2++)
+          for (int t1 = 1; t1 <= T1; t1++) {
+            for (int a = 1; a <= L; a++) {
+              int i = idir == 1 ? a : t1, j = idir == 2 ? a : (idir == 1 ? t1 : t2), k = idir == 3 ? a : t2;
+              double eps = P.irsop == 1 ? 0.25 * (P.rat * P.rat - 1.0) : P1(beta[idir], g, i, j, k, n);
+              wa[a - 1] = -eps;
+              wb[a - 1] = 1 + 2 * eps;
+              wc[a - 1] = -eps;
+              wx[a - 1] = Q(res, g, i, j, k, ipde, n);
+            }
+            wa[0] = 0;
+            if (L >= 1) {
+              /* the reference writes work(1,1)=0 first and then the last row,
+                 so for L = 1 the last-row values win */
+              int i = idir == 1 ? L : t1, j = idir == 2 ? L : (idir == 1 ? t1 : t2), k = idir == 3 ? L : t2;
+              double eps = P.irsop == 1 ? 0.25 * (P.rat * P.rat - 1.0) : P1(beta[idir], g, i, j, k, n);
+              wa[L - 1] = -eps;
+              wc[L - 1] = 0;
+            }
+            or_tridi(wa, wb, wc, wx, L);
+            for (int a = 1; a <= L; a++) {
+              int i = idir == 1 ? a : t1, j = idir == 2 ? a : (idir == 1 ? t1 : t2), k = idir == 3 ? a : t2;
+              Q(res, g, i, j, k, ipde, n) = wx[a - 1];
+            }
+          }
+  }
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int ipde = 1; ipde <= 5; ipde++)
+      for (int k = 1; k <= km; k++)
+        for (int j = 1; j <= jm; j++)
+          for (int i = 1; i <= im; i++) Q(res, g, i, j, k, ipde, n) = Q(res, g, i, j, k, ipde, n) / P1(dtvol, g, i, j, k, n);
+  free(wa); free(wb); free(wc); free(wx);
+}
+
+/* ---------------------------------------------------------------------- */
+/* explicit RK update: update_b / update_bhb, .not.rkimp, Euler variables
+ * (src/update.f90:568-624 and 1359-1408): q = qold - ar*res*dtvol.       */
+void or_update_b(double* res, double* q, const double* qold, const double* dtvol, double ar, int imax, int jmax,
+                 int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  (void)res;
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int ipde = 1; ipde <= 5; ipde++)
+      for (int k = 1; k <= kmax - 1; k++)
+        for (int j = 1; j <= jmax - 1; j++)
+          for (int i = 1; i <= imax - 1; i++)
+            Q(q, g, i, j, k, ipde, n) =
+                Q(qold, g, i, j, k, ipde, n) - ar * Q(res, g, i, j, k, ipde, n) * P1(dtvol, g, i, j, k, n);
+}
+
+/* ---------------------------------------------------------------------- */
+/* multigrid transfer                                                      */
+
+/* rest1_b (src/cosa.f90:2788-2832) */
+void or_rest1_b(int imax1, int jmax1, int kmax1, int imax2, int jmax2, int kmax2, const double* q1, double* q2,
+                const double* vol1, const double* vol2) {
+  bg g1 = mkbg(imax1, jmax1, kmax1), g2 = mkbg(imax2, jmax2, kmax2);
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int k1 = 1; k1 <= kmax1 - 2; k1 += 2)
+      for (int j1 = 1; j1 <= jmax1 - 2; j1 += 2)
+        for (int i1 = 1; i1 <= imax1 - 2; i1 += 2) {
+          int i2 = 1 + i1 / 2, j2 = 1 + j1 / 2, k2 = 1 + k1 / 2;
+          for (int ipde = 1; ipde <= NPDE; ipde++) {
+#define R1(di, dj, dk) (V1(vol1, g1, i1 + di, j1 + dj, k1 + dk) * Q(q1, g1, i1 + di, j1 + dj, k1 + dk, ipde, n))
+            Q(q2, g2, i2, j2, k2, ipde, n) = (R1(0, 0, 0) + R1(0, 1, 0) + R1(1, 0, 0) + R1(1, 1, 0) + R1(0, 0, 1) +
+                                              R1(0, 1, 1) + R1(1, 0, 1) + R1(1, 1, 1)) /
+                                             V1(vol2, g2, i2, j2, k2);
+#undef R1
+          }
+        }
+}
+
+/* rest2_lo_b / rest2_ho_b (src/cosa.f90:2871-3011) */
+void or_rest2_b(int ho, int imax1, int jmax1, int kmax1, int imax2, int jmax2, int kmax2, double* rhs2,
+                const double* res1) {
+  bg g1 = mkbg(imax1, jmax1, kmax1), g2 = mkbg(imax2, jmax2, kmax2);
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int ipde = 1; ipde <= NPDE; ipde++)
+      for (int k1 = 1; k1 <= kmax1 - 2; k1 += 2)
+        for (int j1 = 1; j1 <= jmax1 - 2; j1 += 2)
+          for (int i1 = 1; i1 <= imax1 - 2; i1 += 2) {
+            int i2 = 1 + i1 / 2, j2 = 1 + j1 / 2, k2 = 1 + k1 / 2;
+#define RR(di, dj, dk) Q(res1, g1, i1 + (di), j1 + (dj), k1 + (dk), ipde, n)
+            if (!ho) {
+              Q(rhs2, g2, i2, j2, k2, ipde, n) =
+                  -(RR(0, 0, 0) + RR(0, 1, 0) + RR(1, 0, 0) + RR(1, 1, 0) + RR(0, 0, 1) + RR(0, 1, 1) + RR(1, 0, 1) +
+                    RR(1, 1, 1));
+            } else {
+              double s27 = RR(0, 0, 0) + RR(0, 1, 0) + RR(1, 0, 0) + RR(1, 1, 0) + RR(0, 0, 1) + RR(0, 1, 1) +
+                           RR(1, 0, 1) + RR(1, 1, 1);
+              double s9 = RR(-1, 0, 0) + RR(-1, 1, 0) + RR(2, 0, 0) + RR(2, 1, 0) + RR(-1, 0, 1) + RR(-1, 1, 1) +
+                          RR(2, 0, 1) + RR(2, 1, 1) + RR(0, -1, 0) + RR(1, -1, 0) + RR(0, 2, 0) + RR(1, 2, 0) +
+                          RR(0, -1, 1) + RR(1, -1, 1) + RR(0, 2, 1) + RR(1, 2, 1) + RR(0, 0, -1) + RR(0, 1, -1) +
+                          RR(0, 0, 2) + RR(0, 1, 2) + RR(1, 0, -1) + RR(1, 1, -1) + RR(1, 0, 2) + RR(1, 1, 2);
+              double s3 = RR(0, -1, -1) + RR(1, -1, -1) + RR(0, 2, -1) + RR(1, 2, -1) + RR(-1, 0, -1) + RR(-1, 1, -1) +
+                          RR(2, 0, -1) + RR(2, 1, -1) + RR(0, -1, 2) + RR(1, -1, 2) + RR(0, 2, 2) + RR(1, 2, 2) +
+                          RR(-1, 0, 2) + RR(-1, 1, 2) + RR(2, 0, 2) + RR(2, 1, 2) + RR(-1, -1, 0) + RR(-1, -1, 1) +
+                          RR(-1, 2, 0) + RR(-1, 2, 1) + RR(2, -1, 0) + RR(2, -1, 1) + RR(2, 2, 0) + RR(2, 2, 1);
+              double s1 = RR(-1, -1, -1) + RR(2, -1, -1) + RR(-1, 2, -1) + RR(2, 2, -1) + RR(-1, -1, 2) +
+                          RR(2, -1, 2) + RR(-1, 2, 2) + RR(2, 2, 2);
+              Q(rhs2, g2, i2, j2, k2, ipde, n) = -(27 * s27 + 9 * s9 + 3 * s3 + s1) / 64;
+            }
+#undef RR
+          }
+}
+
+/* prol_lin_b / prol_bln_b (src/cosa.f90:3315-3887), Euler variables. */
+void or_prol_b(int bln, int imax1, int jmax1, int kmax1, int imax2, int jmax2, int kmax2, double* q1, const double* q2,
+               const double* vol1) {
+  bg g1 = mkbg(imax1, jmax1, kmax1), g2 = mkbg(imax2, jmax2, kmax2);
+  int nv = bln ? 5 : NPDE;
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int ipde = 1; ipde <= nv; ipde++)
+      for (int k1 = 0; k1 <= kmax1 - 1; k1 += 2)
+        for (int j1 = 0; j1 <= jmax1 - 1; j1 += 2)
+          for (int i1 = 0; i1 <= imax1 - 1; i1 += 2) {
+            int i2 = i1 / 2, j2 = j1 / 2, k2 = k1 / 2;
+#define C2(di, dj, dk) Q(q2, g2, i2 + (di), j2 + (dj), k2 + (dk), ipde, n)
+#define QF(di, dj, dk) Q(q1, g1, i1 + (di), j1 + (dj), k1 + (dk), ipde, n)
+#define VV(di, dj, dk) V1(vol1, g1, i1 + (di), j1 + (dj), k1 + (dk))
+            if (bln) {
+              double cnec_l = C2(1, 1, 0), cnwc_l = C2(0, 1, 0), csec_l = C2(1, 0, 0), cswc_l = C2(0, 0, 0);
+              double cnec_u = C2(1, 1, 1), cnwc_u = C2(0, 1, 1), csec_u = C2(1, 0, 1), cswc_u = C2(0, 0, 1);
+              double cswf_l = (27 * cswc_l + 9 * csec_l + 9 * cnwc_l + 3 * cnec_l + 9 * cswc_u + 3 * csec_u +
+                               3 * cnwc_u + cnec_u) / 64;
+              double csef_l = (27 * csec_l + 9 * cswc_l + 9 * cnec_l + 3 * cnwc_l + 9 * csec_u + 3 * cswc_u +
+                               3 * cnec_u + cnwc_u) / 64;
+              double cnef_l = (27 * cnec_l + 9 * cnwc_l + 9 * csec_l + 3 * cswc_l + 9 * cnec_u + 3 * cnwc_u +
+                               3 * csec_u + cswc_u) / 64;
+              double cnwf_l = (27 * cnwc_l + 9 * cnec_l + 9 * cswc_l + 3 * csec_l + 9 * cnwc_u + 3 * cnec_u +
+                               3 * cswc_u + csec_u) / 64;
+              double cswf_u = (27 * cswc_u + 9 * csec_u + 9 * cnwc_u + 3 * cnec_u + 9 * cswc_l + 3 * csec_l +
+                               3 * cnwc_l + cnec_l) / 64;
+              double csef_u = (27 * csec_u + 9 * cswc_u + 9 * cnec_u + 3 * cnwc_u + 9 * csec_l + 3 * cswc_l +
+                               3 * cnec_l + cnwc_l) / 64;
+              double cnef_u = (27 * cnec_u + 9 * cnwc_u + 9 * csec_u + 3 * cswc_u + 9 * cnec_l + 3 * cnwc_l +
+                               3 * csec_l + cswc_l) / 64;
+              double cnwf_u = (27 * cnwc_u + 9 * cnec_u + 9 * cswc_u + 3 * csec_u + 9 * cnwc_l + 3 * cnec_l +
+                               3 * cswc_l + csec_l) / 64;
+              QF(0, 0, 0) = QF(0, 0, 0) + cswf_l;
+              QF(1, 0, 0) = QF(1, 0, 0) + csef_l;
+              QF(1, 1, 0) = QF(1, 1, 0) + cnef_l;
+              QF(0, 1, 0) = QF(0, 1, 0) + cnwf_l;
+              QF(0, 0, 1) = QF(0, 0, 1) + cswf_u;
+              QF(1, 0, 1) = QF(1, 0, 1) + csef_u;
+              QF(1, 1, 1) = QF(1, 1, 1) + cnef_u;
+              QF(0, 1, 1) = QF(0, 1, 1) + cnwf_u;
+            } else {
+              double f[2][4]; /* [layer][sw,se,ne,nw] */
+              for (int l = 0; l < 2; l++) {
+                double cnec = C2(1, 1, l), cnwc = C2(0, 1, l), csec = C2(1, 0, l), cswc = C2(0, 0, l);
+                double dcnc = cnec - cnwc, dcsc = csec - cswc, dcec = cnec - csec, dcwc = cnwc - cswc;
+                f[l][0] = cswc + dcsc / 2 * VV(0, 0, l) / (VV(0, 0, l) + VV(1, 0, l)) +
+                          dcwc / 2 * VV(0, 0, l) / (VV(0, 0, l) + VV(0, 1, l));
+                f[l][1] = csec - dcsc / 2 * VV(1, 0, l) / (VV(0, 0, l) + VV(1, 0, l)) +
+                          dcec / 2 * VV(1, 0, l) / (VV(1, 0, l) + VV(1, 1, l));
+                f[l][3] = cnwc + dcnc / 2 * VV(0, 1, l) / (VV(0, 1, l) + VV(1, 1, l)) -
+                          dcwc / 2 * VV(0, 1, l) / (VV(0, 0, l) + VV(0, 1, l));
+                f[l][2] = cnec - dcnc / 2 * VV(1, 1, l) / (VV(1, 1, l) + VV(0, 1, l)) -
+                          dcec / 2 * VV(1, 1, l) / (VV(1, 1, l) + VV(1, 0, l));
+              }
+              int di[4] = {0, 1, 1, 0}, dj[4] = {0, 0, 1, 1};
+              for (int c = 0; c < 4; c++) {
+                double fact = VV(di[c], dj[c], 0) / 2 / (VV(di[c], dj[c], 0) + VV(di[c], dj[c], 1));
+                double lo = f[0][c], up = f[1][c];
+                QF(di[c], dj[c], 0) = QF(di[c], dj[c], 0) + lo + fact / 2 * (up - lo);
+                QF(di[c], dj[c], 1) = QF(di[c], dj[c], 1) + up - (1 - fact) / 2 * (up - lo);
+              }
+            }
+#undef C2
+#undef QF
+#undef VV
+          }
+}
+
+/* q_bedges (src/flux.f90:3914-4263, calmut = .false.): first-layer edge
+ * cells by plane extrapolation from three neighbours.                     */
+static const int EDG[12][4][2] = {
+    /* ic:  p0, p1, p2, p3 as (first, second) index pairs; symbols resolved
+       at run time: 100 = imax, 99 = imax-1, 200 = jmax, 199 = jmax-1,
+       300 = kmax, 299 = kmax-1 */
+    {{0, 0}, {0, 1}, {1, 0}, {1, 1}},
+    {{100, 0}, {100, 1}, {99, 0}, {99, 1}},
+    {{100, 200}, {100, 199}, {99, 200}, {99, 199}},
+    {{0, 200}, {0, 199}, {1, 200}, {1, 199}},
+    {{0, 0}, {0, 1}, {1, 0}, {1, 1}},
+    {{100, 0}, {100, 1}, {99, 0}, {99, 1}},
+    {{100, 300}, {100, 299}, {99, 300}, {99, 299}},
+    {{0, 300}, {0, 299}, {1, 300}, {1, 299}},
+    {{0, 0}, {0, 1}, {1, 0}, {1, 1}},
+    {{200, 0}, {200, 1}, {199, 0}, {199, 1}},
+    {{200, 300}, {200, 299}, {199, 300}, {199, 299}},
+    {{0, 300}, {0, 299}, {1, 300}, {1, 299}}};
+static int sym_res(int s, int imax, int jmax, int kmax) {
+  switch (s) {
+    case 100: return imax;
+    case 99: return imax - 1;
+    case 200: return jmax;
+    case 199: return jmax - 1;
+    case 300: return kmax;
+    case 299: return kmax - 1;
+    default: return s;
+  }
+}
+void or_q_bedges(double* q, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  int c[12][4][2];
+  for (int e = 0; e < 12; e++)
+    for (int p = 0; p < 4; p++)
+      for (int d = 0; d < 2; d++) c[e][p][d] = sym_res(EDG[e][p][d], imax, jmax, kmax);
+  for (int grp = 0; grp < 3; grp++) {
+    int len = grp == 0 ? kmax - 1 : (grp == 1 ? jmax - 1 : imax - 1);
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int ipde = 1; ipde <= NPDE; ipde++)
+        for (int s = 1; s <= len; s++)
+          for (int e = grp * 4; e < grp * 4 + 4; e++) {
+            int x[4], y[4];
+            for (int p = 0; p < 4; p++) { x[p] = c[e][p][0]; y[p] = c[e][p][1]; }
+#define QE(p)                                                        \
+  (grp == 0 ? Q(q, g, x[p], y[p], s, ipde, n)                        \
+            : (grp == 1 ? Q(q, g, x[p], s, y[p], ipde, n) : Q(q, g, s, x[p], y[p], ipde, n)))
+            double b = ((QE(3) - QE(1)) * (x[2] - x[1]) - (QE(2) - QE(1)) * (x[3] - x[1])) /
+                       ((y[3] - y[1]) * (x[2] - x[1]) - (y[2] - y[1]) * (x[3] - x[1]));
+            double a = (QE(2) - QE(1) - b * (y[2] - y[1])) / (x[2] - x[1]);
+            double cc = QE(1) - a * x[1] - b * y[1];
+            double val = a * x[0] + b * y[0] + cc;
+            if (grp == 0) Q(q, g, x[0], y[0], s, ipde, n) = val;
+            else if (grp == 1) Q(q, g, x[0], s, y[0], ipde, n) = val;
+            else Q(q, g, s, x[0], y[0], ipde, n) = val;
+#undef QE
+          }
+  }
+}
+
+/* q_bcorns (src/flux.f90:3156-3471, calmut = .false.): first-layer corner
+ * cells by linear extrapolation from four neighbours.  The integer
+ * coefficient expressions are evaluated with Fortran integer semantics (all
+ * operands are unit offsets, so every integer division is exact).          */
+void or_q_bcorns(double* q, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  int I = imax, J = jmax, K = kmax;
+  int pc[8][5][3] = {
+      {{0, 0, 0}, {0, 1, 1}, {1, 1, 0}, {1, 0, 1}, {1, 1, 1}},
+      {{I, 0, 0}, {I, 1, 1}, {I - 1, 1, 0}, {I - 1, 0, 1}, {I - 1, 1, 1}},
+      {{I, J, 0}, {I, J - 1, 1}, {I - 1, J - 1, 0}, {I - 1, J, 1}, {I - 1, J - 1, 1}},
+      {{0, J, 0}, {0, J - 1, 1}, {1, J - 1, 0}, {1, J, 1}, {1, J - 1, 1}},
+      {{0, 0, K}, {0, 1, K - 1}, {1, 1, K}, {1, 0, K - 1}, {1, 1, K - 1}},
+      {{I, 0, K}, {I, 1, K - 1}, {I - 1, 1, K}, {I - 1, 0, K - 1}, {I - 1, 1, K - 1}},
+      {{I, J, K}, {I, J - 1, K - 1}, {I - 1, J - 1, K}, {I - 1, J, K - 1}, {I - 1, J - 1, K - 1}},
+      {{0, J, K}, {0, J - 1, K - 1}, {1, J - 1, K}, {1, J, K - 1}, {1, J - 1, K - 1}}};
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int ipde = 1; ipde <= NPDE; ipde++)
+      for (int ic = 0; ic < 8; ic++) {
+        int(*p)[3] = pc[ic];
+#define X(m, d) p[m][(d) - 1]
+#define QC(m) Q(q, g, p[m][0], p[m][1], p[m][2], ipde, n)
+        int c1 = ((X(3, 3) - X(1, 3)) * (X(4, 2) - X(1, 2)) - (X(4, 3) - X(1, 3)) * (X(3, 2) - X(1, 2))) /
+                 (X(2, 3) - X(1, 3));
+        int c2 = ((X(2, 3) - X(1, 3)) * (X(1, 2) - X(4, 2)) + (X(4, 3) - X(1, 3)) * (X(2, 2) - X(1, 2))) /
+                 (X(2, 3) - X(1, 3));
+        int c3 = ((X(2, 3) - X(1, 3)) * (X(3, 2) - X(1, 2)) + (X(3, 3) - X(1, 3)) * (X(1, 2) - X(2, 2))) /
+                 (X(2, 3) - X(1, 3));
+        int dn = ((X(4, 1) - X(1, 1)) * (X(3, 2) - X(1, 2)) + (X(1, 1) - X(3, 1)) * (X(4, 2) - X(1, 2))) +
+                 ((X(4, 1) - X(1, 1)) * (X(1, 2) - X(2, 2)) + (X(2, 1) - X(1, 1)) * (X(4, 2) - X(1, 2))) *
+                     ((X(3, 3) - X(1, 3))) / ((X(2, 3) - X(1, 3))) +
+                 ((X(1, 1) - X(2, 1)) * (X(3, 2) - X(1, 2)) + (X(1, 1) - X(3, 1)) * (X(1, 2) - X(2, 2))) *
+                     ((X(4, 3) - X(1, 3))) / ((X(2, 3) - X(1, 3)));
+        double a = (c1 * (QC(2) - QC(1)) + c2 * (QC(3) - QC(1)) + c3 * (QC(4) - QC(1))) / dn;
+        int e1 = (X(1, 1) - X(3, 1)) + (X(2, 1) - X(1, 1)) * (X(3, 3) - X(1, 3)) / (X(2, 3) - X(1, 3));
+        int e2 = (X(3, 3) - X(1, 3)) / (X(2, 3) - X(1, 3));
+        int e3 = (X(3, 2) - X(1, 2)) + (X(1, 2) - X(2, 2)) * (X(3, 3) - X(1, 3)) / (X(2, 3) - X(1, 3));
+        double b = (e1 * a + e2 * (QC(1) - QC(2)) + (QC(3) - QC(1))) / e3;
+        double c = ((X(1, 1) - X(2, 1)) * a + (X(1, 2) - X(2, 2)) * b + (QC(2) - QC(1))) / (X(2, 3) - X(1, 3));
+        double d = QC(1) - a * X(1, 1) - b * X(1, 2) - c * X(1, 3);
+        QC(0) = a * X(0, 1) + b * X(0, 2) + c * X(0, 3) + d;
+#undef X
+#undef QC
+      }
+}
+
+/* ---------------------------------------------------------------------- */
+/* whole-level wrappers (the reference's do iblock loops)                   */
+
+static ll off_of(const or_level* L, int b, int fam) {
+  ll o = 0;
+  for (int bb = 0; bb < b; bb++) {
+    ll im = L->imax[bb], jm = L->jmax[bb], km = L->kmax[bb];
+    if (fam == 3) o += (im + 3) * (jm + 3) * (km + 3);
+    else if (fam == 2) o += (im + 2) * (jm + 2) * (km + 2);
+    else if (fam == 1) o += (im + 1) * (jm + 1) * (km + 1);
+    else o += im * jm * km;
+  }
+  return o;
+}
+static ll off_bct(const or_level* L, int b) {
+  ll o = 0;
+  for (int bb = 0; bb < b; bb++) o += L->nbcs[bb];
+  return o * 10;
+}
+#define QB(arr, L, b) ((arr) + off_of(L, b, 3) * NPDE * DIM5)
+#define MB(arr, L, b) ((arr) + off_of(L, b, 2) * LMET * DIM5H)
+#define XB(arr, L, b) ((arr) + off_of(L, b, 2) * DIM5H)
+#define PB(arr, L, b) ((arr) + off_of(L, b, 1) * DIM5)
+#define VB(arr, L, b) ((arr) + off_of(L, b, 1))
+#define FB(arr, L, b) ((arr) + off_of(L, b, 0) * NPDE * DIM5)
+
+void or_cons2prim(or_case* c, int nl, double* q) {
+  or_level* L = &c->lev[nl - 1];
+  for (int b = 0; b < L->nblocks; b++) or_cons2prim_b(QB(q, L, b), L->imax[b], L->jmax[b], L->kmax[b], 0);
+}
+void or_prim2cons(or_case* c, int nl, double* q) {
+  or_level* L = &c->lev[nl - 1];
+  for (int b = 0; b < L->nblocks; b++) or_prim2cons_b(QB(q, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+}
+void or_bc(or_case* c, int nl, double* q) {
+  or_level* L = &c->lev[nl - 1];
+  for (int b = 0; b < L->nblocks; b++)
+    or_bbc(QB(q, L, b), MB(L->si, L, b), MB(L->sj, L, b), MB(L->sk, L, b), L->bctopo + off_bct(L, b), L->nbcs[b],
+           L->imax[b], L->jmax[b], L->kmax[b]);
+}
+void or_cutman_q(or_case* c, int nl, double* q) {
+  or_level* L = &c->lev[nl - 1];
+  for (int icut = 0; icut < L->ncuts; icut++) {
+    const int32_t* t = L->cutopo + 21 * icut;
+    int b1 = t[0] - 1, b2 = t[9] - 1;
+    or_cut_q(L->imax[b1], L->jmax[b1], L->kmax[b1], QB(q, L, b1), L->imax[b2], L->jmax[b2], L->kmax[b2],
+             QB(q, L, b2), t);
+  }
+}
+void or_edges_corners(or_case* c, int nl, double* q) {
+  or_level* L = &c->lev[nl - 1];
+  for (int b = 0; b < L->nblocks; b++) or_q_bedges(QB(q, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+  for (int b = 0; b < L->nblocks; b++) or_q_bcorns(QB(q, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+}
+int or_prtest(or_case* c, int nl, int32_t* where) {
+  or_level* L = &c->lev[nl - 1];
+  for (int b = 0; b < L->nblocks; b++) {
+    int r = or_bprtest(QB(L->q, L, b), L->imax[b], L->jmax[b], L->kmax[b], where ? where + 2 : NULL);
+    if (r) {
+      if (where) { where[0] = nl; where[1] = b + 1; where[6] = r; }
+      return r;
+    }
+  }
+  return 0;
+}
+void or_lambda_deltat(or_case* c, int nl) {
+  or_level* L = &c->lev[nl - 1];
+  for (int b = 0; b < L->nblocks; b++)
+    or_lambdab(QB(L->q, L, b), MB(L->si, L, b), MB(L->sj, L, b), MB(L->sk, L, b),
+               L->xdot ? XB(L->xdot, L, b) : NULL, L->ydot ? XB(L->ydot, L, b) : NULL,
+               L->zdot ? XB(L->zdot, L, b) : NULL, QB(L->qp, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+  for (int b = 0; b < L->nblocks; b++)
+    or_bdeltat(PB(L->dtvol, L, b), QB(L->qp, L, b), PB(L->betaxi, L, b), PB(L->betaeta, L, b),
+               PB(L->betazeta, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+}
+/* zero(...,off_l=1): res(0:imax,0:jmax,0:kmax,:,:) = 0 (src/util.f90:238-369) */
+void or_zero_res(or_case* c, int nl) {
+  or_level* L = &c->lev[nl - 1];
+  for (int b = 0; b < L->nblocks; b++) {
+    bg g = mkbg(L->imax[b], L->jmax[b], L->kmax[b]);
+    double* r = QB(L->res, L, b);
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int v = 1; v <= NPDE; v++)
+        for (int k = 0; k <= g.kmax; k++)
+          for (int j = 0; j <= g.jmax; j++)
+            for (int i = 0; i <= g.imax; i++) Q(r, g, i, j, k, v, n) = 0.0;
+  }
+}
+void or_residual(or_case* c, int nl) {
+  or_level* L = &c->lev[nl - 1];
+  or_zero_res(c, nl);
+  for (int idir = 1; idir <= 3; idir++) {
+    for (int b = 0; b < L->nblocks; b++)
+      or_muscl_b(idir, nl, L->bctopo + off_bct(L, b), L->nbcs[b], QB(L->q, L, b), QB(L->qp, L, b), QB(L->qm, L, b),
+                 QB(L->dq, L, b), QB(L->dqp, L, b), QB(L->dqm, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+    const double* s = idir == 1 ? L->si : (idir == 2 ? L->sj : L->sk);
+    for (int b = 0; b < L->nblocks; b++)
+      or_roflux_b(idir, QB(L->qp, L, b), QB(L->qm, L, b), FB(L->flux, L, b), MB(s, L, b), L->imax[b], L->jmax[b],
+                  L->kmax[b]);
+    for (int b = 0; b < L->nblocks; b++)
+      or_bresid(idir, FB(L->flux, L, b), QB(L->res, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+  }
+}
+void or_src_hb(or_case* c, int nl) {
+  or_level* L = &c->lev[nl - 1];
+  for (int b = 0; b < L->nblocks; b++)
+    or_src_bhb(QB(L->q, L, b), VB(L->vol, L, b), QB(L->res, L, b), QB(L->dq, L, b), L->imax[b], L->jmax[b],
+               L->kmax[b]);
+}
+/* resg_rms / resg_rms_hb (src/cosa.f90:4458-4500, 4585-4626) */
+void or_resg_rms(or_case* c, int nl, double* resrms) {
+  or_level* L = &c->lev[nl - 1];
+  double rb[8];
+  ll ncell = 0;
+  for (int p = 0; p <= NPDE; p++) resrms[p] = 0;
+  for (int b = 0; b < L->nblocks; b++) {
+    if (P.harbal) {
+      /* work1 = copy of res on the interior (src/cosa.f90:1930-1931) */
+      or_resg_brms_hb(QB(L->work1, L, b), rb, L->imax[b], L->jmax[b], L->kmax[b]);
+    } else {
+      or_resg_brms(QB(L->res, L, b), QB(L->rhs, L, b), rb, L->imax[b], L->jmax[b], L->kmax[b]);
+    }
+    for (int p = 0; p <= NPDE; p++) resrms[p] = resrms[p] + rb[p];
+    ncell = ncell + (ll)(L->imax[b] - 1) * (L->jmax[b] - 1) * (L->kmax[b] - 1);
+  }
+  if (P.harbal) {
+    for (int p = 0; p < NPDE; p++) resrms[p] = sqrt(resrms[p] / ncell / DIM5);
+    resrms[NPDE] = sqrt(resrms[NPDE] / ncell / DIM5 / NPDE);
+  } else {
+    for (int p = 0; p < NPDE; p++) resrms[p] = sqrt(resrms[p] / ncell);
+    resrms[NPDE] = sqrt(resrms[NPDE] / ncell / NPDE);
+  }
+}
+/* add_array / copy_array with off_l = -1 (interior) */
+void or_add_array(or_case* c, int nl, double* dst, const double* src, double fact) {
+  or_level* L = &c->lev[nl - 1];
+  for (int b = 0; b < L->nblocks; b++) {
+    bg g = mkbg(L->imax[b], L->jmax[b], L->kmax[b]);
+    double* d = QB(dst, L, b);
+    const double* s = QB(src, L, b);
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int v = 1; v <= NPDE; v++)
+        for (int k = 1; k <= g.kmax - 1; k++)
+          for (int j = 1; j <= g.jmax - 1; j++)
+            for (int i = 1; i <= g.imax - 1; i++) Q(d, g, i, j, k, v, n) = Q(d, g, i, j, k, v, n) + fact * Q(s, g, i, j, k, v, n);
+  }
+}
+void or_copy_array(or_case* c, int nl, double* dst, const double* src) {
+  or_level* L = &c->lev[nl - 1];
+  for (int b = 0; b < L->nblocks; b++) {
+    bg g = mkbg(L->imax[b], L->jmax[b], L->kmax[b]);
+    double* d = QB(dst, L, b);
+    const double* s = QB(src, L, b);
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int v = 1; v <= NPDE; v++)
+        for (int k = 1; k <= g.kmax - 1; k++)
+          for (int j = 1; j <= g.jmax - 1; j++)
+            for (int i = 1; i <= g.imax - 1; i++) Q(d, g, i, j, k, v, n) = Q(s, g, i, j, k, v, n);
+  }
+}
+void or_cirs(or_case* c, int nl) {
+  or_level* L = &c->lev[nl - 1];
+  for (int b = 0; b < L->nblocks; b++)
+    or_cirs_b(QB(L->res, L, b), PB(L->dtvol, L, b), PB(L->betaxi, L, b), PB(L->betaeta, L, b), PB(L->betazeta, L, b),
+              L->imax[b], L->jmax[b], L->kmax[b]);
+}
+void or_update(or_case* c, int nl, double ar) {
+  or_level* L = &c->lev[nl - 1];
+  for (int b = 0; b < L->nblocks; b++)
+    or_update_b(QB(L->res, L, b), QB(L->q, L, b), QB(L->qold, L, b), PB(L->dtvol, L, b), ar, L->imax[b], L->jmax[b],
+                L->kmax[b]);
+}
+
+/* ---------------------------------------------------------------------- */
+/* smooth (src/cosa.f90:1594-2032), Euler / HB explicit path               */
+int or_smooth(or_case* c, int nl, int from_fine, int to_coarse, int niter, int mgit, int32_t* done, int32_t* where) {
+  or_level* L = &c->lev[nl - 1];
+  *done = 0;
+  int niterp = to_coarse ? niter + 1 : niter;
+  for (int iter = 1; iter <= niterp; iter++) {
+    if (iter <= niter) or_copy_array(c, nl, L->qold, L->q);
+    for (int irk = 1; irk <= P.nstage; irk++) {
+      or_cons2prim(c, nl, L->q);
+      or_bc(c, nl, L->q);
+      or_cutman_q(c, nl, L->q);
+      int bad = or_prtest(c, nl, where);
+      if (bad) {
+        if (where) where[5] = iter;
+        return COSAHIP_ERR_POSITIVITY;
+      }
+      if (iter % P.iupdt == 0 && irk == 1) or_lambda_deltat(c, nl);
+      or_residual(c, nl);
+      or_prim2cons(c, nl, L->q);
+      if (P.harbal) or_src_hb(c, nl);
+      if (nl == 1 && irk == 1 && iter != niter + 1 &&
+          (mgit == 1 || mgit % P.iprint == 0 || mgit == P.ncycle)) {
+        if (P.harbal) or_copy_array(c, nl, L->work1, L->res);
+        double rr[8];
+        or_resg_rms(c, nl, rr);
+        for (int p = 0; p <= NPDE; p++) RESRMS[p + 1] = rr[p];
+      }
+      if (from_fine && iter == 1 && irk == 1) or_add_array(c, nl, L->rhs, L->res, 1.0);
+      if (nl > 1) or_add_array(c, nl, L->res, L->rhs, -1.0);
+      if (iter == niter + 1) return 0;
+      if (P.irsop > 0 && P.irsop <= 3) or_cirs(c, nl);
+      or_update(c, nl, P.alphas[irk - 1]);
+    }
+    double errmax = RESRMS[1];
+    for (int p = 2; p <= 5; p++) errmax = fmax(errmax, RESRMS[p]);
+    if (errmax < P.toler) {
+      *done = 1;
+      return 0;
+    }
+  }
+  return 0;
+}
+
+/* restrict (src/cosa.f90:2672-2754), Euler: rest1 + rest2 */
+void or_restrict(or_case* c, int nl) {
+  or_level *L1 = &c->lev[nl - 1], *L2 = &c->lev[nl];
+  for (int b = 0; b < L1->nblocks; b++)
+    or_rest1_b(L1->imax[b], L1->jmax[b], L1->kmax[b], L2->imax[b], L2->jmax[b], L2->kmax[b], QB(L1->q, L1, b),
+               QB(L2->q, L2, b), VB(L1->vol, L1, b), VB(L2->vol, L2, b));
+  for (int b = 0; b < L1->nblocks; b++)
+    or_rest2_b(P.ho_rest, L1->imax[b], L1->jmax[b], L1->kmax[b], L2->imax[b], L2->jmax[b], L2->kmax[b],
+               QB(L2->rhs, L2, b), QB(L1->res, L1, b));
+}
+
+/* prolong (src/cosa.f90:3167-3249) */
+void or_prolong(or_case* c, int nl) {
+  or_level *L1 = &c->lev[nl - 1], *L2 = &c->lev[nl];
+  for (int b = 0; b < L1->nblocks; b++)
+    or_rest1_b(L1->imax[b], L1->jmax[b], L1->kmax[b], L2->imax[b], L2->jmax[b], L2->kmax[b], QB(L1->q, L1, b),
+               QB(L2->qold, L2, b), VB(L1->vol, L1, b), VB(L2->vol, L2, b));
+  double* arrs[2] = {L2->qold, L2->q};
+  for (int a = 0; a < 2; a++) {
+    or_cons2prim(c, nl + 1, arrs[a]);
+    or_bc(c, nl + 1, arrs[a]);
+    or_cutman_q(c, nl + 1, arrs[a]);
+    or_prim2cons(c, nl + 1, arrs[a]);
+    or_edges_corners(c, nl + 1, arrs[a]);
+  }
+  /* add_array(..., q2, qold2, off_l = 1, -1): 0..imax incl. first halo */
+  for (int b = 0; b < L2->nblocks; b++) {
+    bg g = mkbg(L2->imax[b], L2->jmax[b], L2->kmax[b]);
+    double* d = QB(L2->q, L2, b);
+    const double* s = QB(L2->qold, L2, b);
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int v = 1; v <= NPDE; v++)
+        for (int k = 0; k <= g.kmax; k++)
+          for (int j = 0; j <= g.jmax; j++)
+            for (int i = 0; i <= g.imax; i++) Q(d, g, i, j, k, v, n) = Q(d, g, i, j, k, v, n) + -1.0 * Q(s, g, i, j, k, v, n);
+  }
+  for (int b = 0; b < L1->nblocks; b++)
+    or_prol_b(P.bln_prol, L1->imax[b], L1->jmax[b], L1->kmax[b], L2->imax[b], L2->jmax[b], L2->kmax[b],
+              QB(L1->q, L1, b), QB(L2->q, L2, b), VB(L1->vol, L1, b));
+}
+
+/* mg (src/cosa.f90:806-930) */
+int or_mg(or_case* c, int nstart, int npre, int npost, int ncrs, double* hist, int32_t* where) {
+  int nlevel = P.nlevel, rc;
+  int32_t done = 0;
+  int nl = 1;
+  for (int mgit = 1; mgit <= P.ncycle; mgit++) {
+    int niter = nlevel == 1 ? 1 : (mgit == 1 ? nstart : npost);
+    nl = 1;
+    rc = or_smooth(c, nl, 0, nlevel > 1, niter, mgit, &done, where);
+    if (rc) return -rc;
+    for (int p = 0; p < 8; p++) hist[(ll)(mgit - 1) * 8 + p] = RESRMS[p + 1];
+    if (done) return mgit;
+    if (nlevel > 1) {
+      or_restrict(c, nl);
+      nl = 2;
+      niter = nlevel == 2 ? ncrs : npre;
+      rc = or_smooth(c, nl, 1, nlevel > 2, niter, mgit, &done, where);
+      if (rc) return -rc;
+      if (nlevel == 3) {
+        or_restrict(c, nl);
+        nl = 3;
+        rc = or_smooth(c, nl, 1, 0, ncrs, mgit, &done, where);
+        if (rc) return -rc;
+        nl = 2;
+        or_prolong(c, nl);
+        rc = or_smooth(c, nl, 0, 0, npost, mgit, &done, where);
+        if (rc) return -rc;
+      }
+      nl = 1;
+      or_prolong(c, nl);
+    }
+  }
+  if (nlevel > 1) {
+    rc = or_smooth(c, nl, 0, 0, npost, P.ncycle, &done, where);
+    if (rc) return -rc;
+  }
+  return P.ncycle;
+}

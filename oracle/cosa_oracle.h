@@ -1,0 +1,130 @@
+/*
+ * cosa_oracle.h -- CPU restatement of COSA's pseudo-time residual/update
+ * path (LANCASTER-CFD/COSA, src/), used ONLY as the checker of libcosahip.
+ *
+ * TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load this library.  The product (cosa_amd,
+ * libcosahip) never links, imports or calls it.
+ *
+ * Every function restates one reference routine (file:line in its comment)
+ * on the reference's own flat array layout, one block at a time (*_b) or a
+ * whole level.  Arithmetic is written in the reference's operation order so
+ * that a build with -O2 -ffp-contract=off reproduces the reference's
+ * binary64 results (see oracle/Makefile).
+ */
+#ifndef COSA_ORACLE_H
+#define COSA_ORACLE_H
+#include <stdint.h>
+#include "cosahip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One multigrid level: block sizes, offsets (setoffs) and the arrays the
+ * hot path touches.  Arrays are owned by the caller.                     */
+typedef struct or_level {
+  int32_t nblocks;
+  int32_t ncuts;
+  const int32_t* imax;        /* [nblocks] node counts                      */
+  const int32_t* jmax;
+  const int32_t* kmax;
+  const int32_t* nbcs;        /* [nblocks]                                  */
+  const int32_t* bctopo;      /* 10 per patch, concatenated over blocks     */
+  const int32_t* cutopo;      /* 21 per cut                                 */
+  double *q, *qold, *qp, *qm, *dq, *dqp, *dqm, *flux, *res, *rhs, *work1;
+  double *si, *sj, *sk, *vol, *dtvol, *betaxi, *betaeta, *betazeta;
+  double *xdot, *ydot, *zdot, *pr;
+} or_level;
+
+typedef struct or_case {
+  int32_t nlevel;
+  int32_t pad;
+  or_level lev[4];
+} or_case;
+
+/* parameters (common_variables mirror) */
+void or_set_config(const cosahip_config* cfg);
+void or_set_dhb(int nsnap, const double* dhb);
+double or_get_resrms(int i);            /* resrms(i), i = 1..8            */
+void or_set_resrms(int i, double v);
+
+/* per-block restatements (explicit-shape dummies of the reference) */
+void or_cons2prim_b(double* q, int imax, int jmax, int kmax, int xtnd);
+void or_prim2cons_b(double* q, int imax, int jmax, int kmax);
+int or_bprtest(const double* q, int imax, int jmax, int kmax, int32_t* where);
+void or_bbc(double* q, const double* si, const double* sj, const double* sk,
+            const int32_t* bctopo, int nbcs, int imax, int jmax, int kmax);
+void or_cut_q(int imax1, int jmax1, int kmax1, double* q1, int imax2,
+              int jmax2, int kmax2, const double* q2, const int32_t* cutopo);
+void or_lambdab(const double* q, const double* si, const double* sj,
+                const double* sk, const double* xdot, const double* ydot,
+                const double* zdot, double* lambdas, int imax, int jmax,
+                int kmax);
+void or_bdeltat(double* dtvol, const double* lambdas, double* betaxi,
+                double* betaeta, double* betazeta, int imax, int jmax,
+                int kmax);
+void or_muscl_b(int idir, int nl, const int32_t* bctopo, int nbcs,
+                const double* q, double* qp, double* qm, double* dq,
+                double* dqp, double* dqm, int imax, int jmax, int kmax);
+void or_roflux_b(int idir, const double* qr, const double* ql, double* flux,
+                 const double* sijk, int imax, int jmax, int kmax);
+void or_bresid(int idir, const double* flux, double* res, int imax, int jmax,
+               int kmax);
+void or_src_bhb(const double* q, const double* vol, double* res, double* qhb,
+                int imax, int jmax, int kmax);
+void or_resg_brms(const double* res, const double* rhs, double* resrms_b,
+                  int imax, int jmax, int kmax);
+void or_resg_brms_hb(const double* work1, double* resrms_b, int imax,
+                     int jmax, int kmax);
+void or_cirs_b(double* res, const double* dtvol, const double* betaxi,
+               const double* betaeta, const double* betazeta, int imax,
+               int jmax, int kmax);
+void or_tridi(const double* a, const double* b, const double* c, double* x,
+              int dim);
+void or_update_b(double* res, double* q, const double* qold,
+                 const double* dtvol, double ar, int imax, int jmax,
+                 int kmax);
+void or_rest1_b(int imax1, int jmax1, int kmax1, int imax2, int jmax2,
+                int kmax2, const double* q1, double* q2, const double* vol1,
+                const double* vol2);
+void or_rest2_b(int ho, int imax1, int jmax1, int kmax1, int imax2,
+                int jmax2, int kmax2, double* rhs2, const double* res1);
+void or_prol_b(int bln, int imax1, int jmax1, int kmax1, int imax2,
+               int jmax2, int kmax2, double* q1, const double* q2,
+               const double* vol1);
+void or_q_bedges(double* q, int imax, int jmax, int kmax);
+void or_q_bcorns(double* q, int imax, int jmax, int kmax);
+
+/* whole-level routines (loop over blocks, as the reference wrappers) */
+void or_cons2prim(or_case* c, int nl, double* q);
+void or_prim2cons(or_case* c, int nl, double* q);
+void or_bc(or_case* c, int nl, double* q);
+void or_cutman_q(or_case* c, int nl, double* q);
+void or_edges_corners(or_case* c, int nl, double* q);
+int or_prtest(or_case* c, int nl, int32_t* where);
+void or_lambda_deltat(or_case* c, int nl);
+void or_zero_res(or_case* c, int nl);
+void or_residual(or_case* c, int nl);
+void or_src_hb(or_case* c, int nl);
+void or_resg_rms(or_case* c, int nl, double* resrms);
+void or_add_array(or_case* c, int nl, double* dst, const double* src,
+                  double fact);
+void or_copy_array(or_case* c, int nl, double* dst, const double* src);
+void or_cirs(or_case* c, int nl);
+void or_update(or_case* c, int nl, double ar);
+
+/* drivers */
+int or_smooth(or_case* c, int nl, int from_fine, int to_coarse, int niter,
+              int mgit, int32_t* done, int32_t* where);
+void or_restrict(or_case* c, int nl);
+void or_prolong(or_case* c, int nl);
+/* mg (src/cosa.f90:806-930) for ncycle cycles; hist receives resrms(1:8)
+ * per cycle (rows of 8); returns cycles run, or -error.                  */
+int or_mg(or_case* c, int nstart, int npre, int npost, int ncrs,
+          double* hist, int32_t* where);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,246 @@
+!-----------------------------------------------------------------------
+! ref_shim.f90 -- bind(C) entry points that drive COSA's OWN compiled
+! routines (src/flux.f90, src/update.f90, src/src.f90, src/util.f90 and the
+! self-contained subroutines of src/bc.f90 and src/cosa.f90), so tests can
+! feed the reference and the C restatement (cosa_oracle.c) the same inputs.
+!
+! TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile into oracle/_ref/
+! (git-ignored) from the sources where they lie under /root/reference; no
+! reference source is copied into this repository.
+!-----------------------------------------------------------------------
+module ref_shim
+  use iso_c_binding
+  use common_variables
+  use cosa_variables
+  use cosa_precision
+  implicit none
+
+  ! layout of cosahip_config (include/cosahip.h)
+  type, bind(C) :: cfg_t
+    integer(c_int32_t) :: npde, nharms, hbmove, lmet, nlevel, nl_crs, nstage
+    integer(c_int32_t) :: irsop, cutcirs, ilimf, ilimt, iupdt, iprint, ncycle
+    integer(c_int32_t) :: kom, kom_bsl, kom_sst, viscous
+    integer(c_int32_t) :: moving, harbal, foturb
+    integer(c_int32_t) :: efx_setup_f, efx_all_f, eig_limit_f, eig_lim_all_f
+    integer(c_int32_t) :: efx_setup_t, efx_all_t, eig_limit_t, eig_lim_all_t
+    integer(c_int32_t) :: entfx_f_w(3), entfx_t_w(3)
+    integer(c_int32_t) :: bln_prol, ho_rest, pad0
+    real(c_double) :: gamma, machfs, alpha, beta
+    real(c_double) :: rhoinf, pinf, ainf
+    real(c_double) :: cfl, cflt, rat, ratt, psirs, rkap
+    real(c_double) :: epslimf, epslimt
+    real(c_double) :: cntrpy_f, cntrpy_t, entfxctf_f, entfxctf_t
+    real(c_double) :: eig_cutoff_f, eig_cutoff_t
+    real(c_double) :: omega, toler
+    real(c_double) :: qmin(7), alphas(5)
+    real(c_double) :: tkefar, omefar
+  end type cfg_t
+
+contains
+
+  integer(c_int) function ref_cfg_size() bind(C, name='ref_cfg_size')
+    type(cfg_t) :: c
+    ref_cfg_size = int(c_sizeof(c), c_int)
+  end function
+
+  subroutine ref_set_config(c) bind(C, name='ref_set_config')
+    type(cfg_t), intent(in) :: c
+    cyc = reshape((/1,2,3,2,3,1,3,1,2/),(/3,3/))
+    krd = reshape((/1,0,0,0,1,0,0,0,1/),(/3,3/))
+    npde = c%npde; nharms = c%nharms; hbmove = c%hbmove; lmet = c%lmet
+    dim5 = 2*nharms+1; dim5h = 2*nharms*hbmove+1
+    nlevel = c%nlevel; nl_crs = c%nl_crs; nstage = c%nstage
+    irsop = c%irsop; cutcirs = c%cutcirs; ilimf = c%ilimf; ilimt = c%ilimt
+    iupdt = c%iupdt; iprint = c%iprint; ncycle = c%ncycle
+    kom = c%kom /= 0; kom_bsl = c%kom_bsl /= 0; kom_sst = c%kom_sst /= 0
+    viscous = c%viscous /= 0; moving = c%moving /= 0; harbal = c%harbal /= 0
+    foturb = c%foturb /= 0
+    efx_setup_f = c%efx_setup_f /= 0; efx_all_f = c%efx_all_f /= 0
+    eig_limit_f = c%eig_limit_f /= 0; eig_lim_all_f = c%eig_lim_all_f /= 0
+    efx_setup_t = c%efx_setup_t /= 0; efx_all_t = c%efx_all_t /= 0
+    eig_limit_t = c%eig_limit_t /= 0; eig_lim_all_t = c%eig_lim_all_t /= 0
+    entfx_f_w = c%entfx_f_w; entfx_t_w = c%entfx_t_w
+    bln_prol = c%bln_prol /= 0; ho_rest = c%ho_rest /= 0
+    gamma = c%gamma; machfs = c%machfs; alpha = c%alpha; beta = c%beta
+    rhoinf = c%rhoinf; pinf = c%pinf; ainf = c%ainf
+    cfl = c%cfl; cflt = c%cflt; rat = c%rat; ratt = c%ratt; psirs = c%psirs
+    rkap = c%rkap; epslimf = c%epslimf; epslimt = c%epslimt
+    cntrpy_f = c%cntrpy_f; cntrpy_t = c%cntrpy_t
+    entfxctf_f = c%entfxctf_f; entfxctf_t = c%entfxctf_t
+    eig_cutoff_f = c%eig_cutoff_f; eig_cutoff_t = c%eig_cutoff_t
+    omega = c%omega; toler = c%toler; qmin = c%qmin
+    alphas(1:5) = c%alphas; tkefar = c%tkefar; omefar = c%omefar
+    ! options of the configurations exercised here
+    aircraft = .true.; vawt = .false.; hawt = .false.; rotating = .false.
+    tpitch = .false.; lomach = .false.; rkimp = .false.; rgkuns = .false.
+    dualt = .false.; relframe = .false.; unsteady = harbal
+    ramping1 = .false.; ramping2 = .false.; mgit = 1
+    lim_corr = .false.
+  end subroutine
+
+  subroutine ref_set_dhb(nsnap, d) bind(C, name='ref_set_dhb')
+    integer(c_int), value :: nsnap
+    real(c_double), intent(in) :: d(nsnap, nsnap)
+    dhb(0:nsnap-1, 0:nsnap-1) = d
+  end subroutine
+
+  subroutine ref_cons2prim_b(q, work, imax, jmax, kmax, xtnd) bind(C, name='ref_cons2prim_b')
+    real(c_double) :: q(*), work(*)
+    integer(c_int), value :: imax, jmax, kmax, xtnd
+    call cons2prim_b(q, work, xtnd, npde, imax, jmax, kmax, nharms)
+  end subroutine
+
+  subroutine ref_prim2cons_b(q, work, imax, jmax, kmax) bind(C, name='ref_prim2cons_b')
+    real(c_double) :: q(*), work(*)
+    integer(c_int), value :: imax, jmax, kmax
+    call prim2cons_b(q, work, npde, imax, jmax, kmax, nharms)
+  end subroutine
+
+  subroutine ref_bbc(q, si, sj, sk, dist, nl, bctopo, nbcs, imax, jmax, kmax) bind(C, name='ref_bbc')
+    real(c_double) :: q(*), si(*), sj(*), sk(*), dist(*)
+    integer(c_int) :: bctopo(*)
+    integer(c_int), value :: nl, nbcs, imax, jmax, kmax
+    integer(c_int) :: ijkmax
+    ijkmax = max(imax, jmax, kmax)
+    call bbc(q, si, sj, sk, dist, nl, bctopo, nbcs, imax, jmax, kmax, ijkmax, npde, nharms, lmet)
+  end subroutine
+
+  subroutine ref_lambdab(q, si, sj, sk, xdot, ydot, zdot, vol, mut, work, lambdas, cut1, cut2, &
+                         imax, jmax, kmax) bind(C, name='ref_lambdab')
+    real(c_double) :: q(*), si(*), sj(*), sk(*), xdot(*), ydot(*), zdot(*), vol(*), mut(*)
+    real(c_double) :: work(*), lambdas(*), cut1(*), cut2(*)
+    integer(c_int), value :: imax, jmax, kmax
+    call lambdab(1, 1, q, si, sj, sk, xdot, ydot, zdot, vol, mut, work, lambdas, cut1, cut2, &
+                 imax, jmax, kmax, npde, nharms, lmet)
+  end subroutine
+
+  subroutine ref_bdeltat(dtvol, dtvolt, lambdas, lambdasv, bx, be, bz, tbx, tbe, tbz, &
+                         imax, jmax, kmax) bind(C, name='ref_bdeltat')
+    real(c_double) :: dtvol(*), dtvolt(*), lambdas(*), lambdasv(*), bx(*), be(*), bz(*)
+    real(c_double) :: tbx(*), tbe(*), tbz(*)
+    integer(c_int), value :: imax, jmax, kmax
+    call bdeltat(1, 1, dtvol, dtvolt, lambdas, lambdasv, bx, be, bz, tbx, tbe, tbz, &
+                 imax, jmax, kmax, npde, nharms, lmet)
+  end subroutine
+
+  subroutine ref_muscl_b(idir, nl, bctopo, nbcs, q, qp, qm, dq, dqp, dqm, imax, jmax, kmax) &
+      bind(C, name='ref_muscl_b')
+    integer(c_int), value :: idir, nl, nbcs, imax, jmax, kmax
+    integer(c_int) :: bctopo(*)
+    real(c_double) :: q(*), qp(*), qm(*), dq(*), dqp(*), dqm(*)
+    if (idir == 1) then
+      call muscl_bi(nl, bctopo, q, qp, qm, dq, dqp, dqm, nbcs, imax, jmax, kmax, npde, nharms)
+    else if (idir == 2) then
+      call muscl_bj(nl, bctopo, q, qp, qm, dq, dqp, dqm, nbcs, imax, jmax, kmax, npde, nharms)
+    else
+      call muscl_bk(nl, bctopo, q, qp, qm, dq, dqp, dqm, nbcs, imax, jmax, kmax, npde, nharms)
+    end if
+  end subroutine
+
+  subroutine ref_roflux_b(idir, qr, ql, alphar, work, flux, sijk, imax, jmax, kmax) &
+      bind(C, name='ref_roflux_b')
+    integer(c_int), value :: idir, imax, jmax, kmax
+    real(c_double) :: qr(*), ql(*), alphar(*), work(*), flux(*), sijk(*)
+    integer(c_int) :: ic, jc, kc
+    ic = imax - 1; jc = jmax - 1; kc = kmax - 1
+    if (idir == 1) ic = imax
+    if (idir == 2) jc = jmax
+    if (idir == 3) kc = kmax
+    call roflux_b(qr, ql, alphar, work, flux, sijk, imax, jmax, kmax, npde, nharms, lmet, ic, jc, kc)
+  end subroutine
+
+  subroutine ref_bresid(idir, flux, res, imax, jmax, kmax) bind(C, name='ref_bresid')
+    integer(c_int), value :: idir, imax, jmax, kmax
+    real(c_double) :: flux(*), res(*)
+    call bresid(flux, res, idir, imax, jmax, kmax, npde, nharms)
+  end subroutine
+
+  subroutine ref_src_bhb(q, vol, res, qhb, imax, jmax, kmax) bind(C, name='ref_src_bhb')
+    real(c_double) :: q(*), vol(*), res(*), qhb(*)
+    integer(c_int), value :: imax, jmax, kmax
+    call src_bhb(q, vol, res, qhb, imax, jmax, kmax, npde, nharms)
+  end subroutine
+
+  subroutine ref_resg_brms(q, res, rhs, vol, rb, imax, jmax, kmax) bind(C, name='ref_resg_brms')
+    real(c_double) :: q(*), res(*), rhs(*), vol(*), rb(*)
+    integer(c_int), value :: imax, jmax, kmax
+    call resg_brms(q, res, rhs, vol, rb, imax, jmax, kmax, npde)
+  end subroutine
+
+  subroutine ref_resg_brms_hb(q, work1, vol, rb, imax, jmax, kmax) bind(C, name='ref_resg_brms_hb')
+    real(c_double) :: q(*), work1(*), vol(*), rb(*)
+    integer(c_int), value :: imax, jmax, kmax
+    call resg_brms_hb(q, work1, vol, rb, imax, jmax, kmax, npde, nharms)
+  end subroutine
+
+  subroutine ref_cirs_b(res, dtvol, bx, be, bz, dtvolt, tbx, tbe, tbz, work, imax, jmax, kmax) &
+      bind(C, name='ref_cirs_b')
+    real(c_double) :: res(*), dtvol(*), bx(*), be(*), bz(*), dtvolt(*), tbx(*), tbe(*), tbz(*), work(*)
+    integer(c_int), value :: imax, jmax, kmax
+    call cirs_b(res, dtvol, bx, be, bz, dtvolt, tbx, tbe, tbz, work, imax, jmax, kmax, npde, nharms, &
+                max(imax, jmax, kmax))
+  end subroutine
+
+  subroutine ref_tridi(a, b, c, x, dimx, n) bind(C, name='ref_tridi')
+    real(c_double) :: a(*), b(*), c(*), x(*)
+    integer(c_int), value :: dimx, n
+    call tridi(a, b, c, x, dimx, n)
+  end subroutine
+
+  subroutine ref_update(res, q, qold, vol, dtvol, dtvolt, work1, qhb0, delpls, prd, ttrm, ar, &
+                        imax, jmax, kmax) bind(C, name='ref_update')
+    real(c_double) :: res(*), q(*), qold(*), vol(*), dtvol(*), dtvolt(*), work1(*), qhb0(*)
+    real(c_double) :: delpls(*), prd(*), ttrm(*)
+    real(c_double), value :: ar
+    integer(c_int), value :: imax, jmax, kmax
+    if (harbal) then
+      call update_bhb(res, q, qold, vol, dtvol, dtvolt, work1, qhb0, delpls, prd, ttrm, ar, &
+                      imax, jmax, kmax, npde, nharms)
+    else
+      call update_b(res, q, qold, vol, dtvol, dtvolt, delpls, prd, ttrm, ar, imax, jmax, kmax, npde, nharms)
+    end if
+  end subroutine
+
+  subroutine ref_rest1_b(i1, j1, k1, i2, j2, k2, q1, q2, vol1, vol2) bind(C, name='ref_rest1_b')
+    integer(c_int), value :: i1, j1, k1, i2, j2, k2
+    real(c_double) :: q1(*), q2(*), vol1(*), vol2(*)
+    call rest1_b(i1, j1, k1, i2, j2, k2, npde, nharms, q1, q2, vol1, vol2)
+  end subroutine
+
+  subroutine ref_rest2_b(ho, i1, j1, k1, i2, j2, k2, rhs1, rhs2, res1) bind(C, name='ref_rest2_b')
+    integer(c_int), value :: ho, i1, j1, k1, i2, j2, k2
+    real(c_double) :: rhs1(*), rhs2(*), res1(*)
+    if (ho /= 0) then
+      call rest2_ho_b(i1, j1, k1, i2, j2, k2, npde, nharms, rhs1, rhs2, res1)
+    else
+      call rest2_lo_b(i1, j1, k1, i2, j2, k2, npde, nharms, rhs1, rhs2, res1)
+    end if
+  end subroutine
+
+  subroutine ref_prol_b(bln, i1, j1, k1, i2, j2, k2, q1, q2, vol1) bind(C, name='ref_prol_b')
+    integer(c_int), value :: bln, i1, j1, k1, i2, j2, k2
+    real(c_double) :: q1(*), q2(*), vol1(*)
+    logical :: lc(3)
+    lc = .false.
+    if (bln /= 0) then
+      call prol_bln_b(i1, j1, k1, i2, j2, k2, npde, nharms, q1, q2, vol1, lc)
+    else
+      call prol_lin_b(i1, j1, k1, i2, j2, k2, npde, nharms, q1, q2, vol1)
+    end if
+  end subroutine
+
+  subroutine ref_q_bedges(q, mut, imax, jmax, kmax) bind(C, name='ref_q_bedges')
+    real(c_double) :: q(*), mut(*)
+    integer(c_int), value :: imax, jmax, kmax
+    calmut = .false.
+    call q_bedges(q, mut, imax, jmax, kmax, npde, nharms)
+  end subroutine
+
+  subroutine ref_q_bcorns(q, mut, imax, jmax, kmax) bind(C, name='ref_q_bcorns')
+    real(c_double) :: q(*), mut(*)
+    integer(c_int), value :: imax, jmax, kmax
+    calmut = .false.
+    call q_bcorns(q, mut, imax, jmax, kmax, npde, nharms)
+  end subroutine
+
+end module ref_shim

@@ -1,0 +1,68 @@
+"""Random single-block inputs for kernel-level parity tests."""
+from __future__ import annotations
+
+import numpy as np
+
+from cosa_amd import grids
+from cosa_amd.layout import Patch, bcdata_of, bctopo_level
+
+
+def n3(imax, jmax, kmax):
+    return (imax + 3) * (jmax + 3) * (kmax + 3)
+
+
+def rand_prim(rng, imax, jmax, kmax, npde=5, nsnap=1):
+    """Primitive state everywhere (interior and both halo layers)."""
+    n = n3(imax, jmax, kmax)
+    q = np.empty((nsnap, npde, n))
+    q[:, 0] = rng.uniform(0.8, 1.2, (nsnap, n))
+    q[:, 1] = 0.5 + rng.uniform(-0.2, 0.2, (nsnap, n))
+    q[:, 2] = rng.uniform(-0.2, 0.2, (nsnap, n))
+    q[:, 3] = rng.uniform(-0.1, 0.1, (nsnap, n))
+    q[:, 4] = rng.uniform(0.6, 0.8, (nsnap, n))
+    for v in range(5, npde):
+        q[:, v] = rng.uniform(1e-3, 1e-2, (nsnap, n))
+    return q.reshape(-1).copy()
+
+
+def prim_to_cons(qp, imax, jmax, kmax, gamma=1.4, npde=5, nsnap=1):
+    q = qp.reshape(nsnap, npde, -1).copy()
+    rho = q[:, 0]
+    u, v, w, p = q[:, 1], q[:, 2], q[:, 3], q[:, 4]
+    out = q.copy()
+    out[:, 1] = rho * u
+    out[:, 2] = rho * v
+    out[:, 3] = rho * w
+    out[:, 4] = p / (gamma - 1) + 0.5 * rho * (u * u + v * v + w * w)
+    return out.reshape(-1).copy()
+
+
+def box_block(imax, jmax, kmax, amp=0.02, seed=0, lmet=4, nsnap_met=1):
+    """Distorted box grid: metrics [P2, lmet] and volumes [P1]."""
+    x, y, z = np.meshgrid(np.linspace(0, 1.3, imax), np.linspace(0, 0.9, jmax), np.linspace(0, 0.7, kmax),
+                          indexing="ij")
+    x, y, z = (np.ascontiguousarray(a.transpose(2, 1, 0)) for a in (x, y, z))
+    x, y, z = grids.distort(x, y, z, amp * min(1.3 / imax, 0.9 / jmax, 0.7 / kmax) * 4, seed=seed)
+    si, sj, sk = grids.face_metrics(x, y, z, lmet)
+    vol = grids.cell_volumes(x, y, z).reshape(-1).copy()
+    mets = []
+    for s in (si, sj, sk):
+        f = s.reshape(-1)
+        mets.append(np.concatenate([f] * nsnap_met).copy())
+    return mets[0], mets[1], mets[2], vol
+
+
+def all_faces_patches(imax, jmax, kmax, types=(30, 14, 30, 14, 80, 80)):
+    """One patch per block face: i=1, i=imax, j=1, j=jmax, k=1, k=kmax."""
+    return [
+        Patch((1, 1, 1), (1, jmax, kmax), types[0]),
+        Patch((imax, 1, 1), (imax, jmax, kmax), types[1]),
+        Patch((1, 1, 1), (imax, 1, kmax), types[2]),
+        Patch((1, jmax, 1), (imax, jmax, kmax), types[3]),
+        Patch((1, 1, 1), (imax, jmax, 1), types[4]),
+        Patch((1, 1, kmax), (imax, jmax, kmax), types[5]),
+    ]
+
+
+def bctopo_of(patches, nl=1):
+    return np.array([v for p in patches for v in bctopo_level(bcdata_of(p), nl)], np.int32)

@@ -1,0 +1,150 @@
+"""ctypes bindings of the checkers (test infrastructure):
+
+* ``Oracle``  -- oracle/liboracle.so, the C restatement of the reference path
+* ``Ref``     -- oracle/_ref/libcosaref.so, the reference's own compiled
+                 routines driven through oracle/ref_shim.f90 (built only
+                 where /root/reference exists)
+* ``OrCase``  -- the or_case struct over numpy arrays of a cosa_amd Case
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from cosa_amd.config import CosahipConfig, to_struct
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libcosaref.so")
+
+DP = ctypes.POINTER(ctypes.c_double)
+IP = ctypes.POINTER(ctypes.c_int32)
+
+
+def dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(DP)
+
+
+def ip(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(IP)
+
+
+def _build_oracle():
+    src = os.path.join(ROOT, "oracle", "cosa_oracle.c")
+    if (not os.path.exists(ORACLE_SO)) or os.path.getmtime(ORACLE_SO) < os.path.getmtime(src):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "liboracle.so"])
+
+
+class OrLevel(ctypes.Structure):
+    _fields_ = [("nblocks", ctypes.c_int32), ("ncuts", ctypes.c_int32),
+                ("imax", IP), ("jmax", IP), ("kmax", IP), ("nbcs", IP), ("bctopo", IP), ("cutopo", IP)] + [
+        (n, DP) for n in ("q", "qold", "qp", "qm", "dq", "dqp", "dqm", "flux", "res", "rhs", "work1", "si", "sj",
+                          "sk", "vol", "dtvol", "betaxi", "betaeta", "betazeta", "xdot", "ydot", "zdot", "pr")]
+
+
+class OrCaseS(ctypes.Structure):
+    _fields_ = [("nlevel", ctypes.c_int32), ("pad", ctypes.c_int32), ("lev", OrLevel * 4)]
+
+
+class Oracle:
+    def __init__(self):
+        _build_oracle()
+        self.lib = ctypes.CDLL(ORACLE_SO)
+        L = self.lib
+        L.or_get_resrms.restype = ctypes.c_double
+        L.or_set_resrms.argtypes = [ctypes.c_int, ctypes.c_double]
+        for n in ("or_smooth", "or_mg", "or_prtest", "or_bprtest"):
+            getattr(L, n).restype = ctypes.c_int
+
+    def set_config(self, cfg: dict, dhb=None):
+        self._cfg = to_struct(cfg)
+        self.lib.or_set_config(ctypes.byref(self._cfg))
+        if dhb is not None:
+            d = np.ascontiguousarray(np.asarray(dhb, np.float64).flatten(order="F"))
+            self.lib.or_set_dhb(ctypes.c_int(dhb.shape[0]), dp(d))
+
+    def __getattr__(self, name):
+        return getattr(self.lib, name)
+
+
+_REF = None
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+class Ref:
+    def __init__(self):
+        self.lib = ctypes.CDLL(REF_SO, mode=os.RTLD_LAZY)
+        assert self.lib.ref_cfg_size() == ctypes.sizeof(CosahipConfig)
+
+    def set_config(self, cfg: dict, dhb=None):
+        self._cfg = to_struct(cfg)
+        self.lib.ref_set_config(ctypes.byref(self._cfg))
+        if dhb is not None:
+            d = np.ascontiguousarray(np.asarray(dhb, np.float64).flatten(order="F"))
+            self.lib.ref_set_dhb(ctypes.c_int(dhb.shape[0]), dp(d))
+
+    def __getattr__(self, name):
+        return getattr(self.lib, name)
+
+
+class OrCase:
+    """Host arrays of every level of a Case, wired into an or_case struct."""
+
+    FIELDS = ("q", "qold", "qp", "qm", "dq", "dqp", "dqm", "flux", "res", "rhs", "work1", "si", "sj", "sk", "vol",
+              "dtvol", "betaxi", "betaeta", "betazeta", "xdot", "ydot", "zdot", "pr")
+
+    def __init__(self, case, q1=None):
+        self.case = case
+        self.s = OrCaseS()
+        self.s.nlevel = case.config["nlevel"]
+        self.arrays = []
+        self.keep = []
+        for li, L in enumerate(case.levels):
+            nl = li + 1
+            g = case.geom[li]
+            a = {}
+            for n in ("q", "qold", "qp", "qm", "dq", "dqp", "dqm", "res", "rhs", "work1"):
+                a[n] = np.zeros(case.size(nl, "p3"))
+            a["flux"] = np.zeros(case.size(nl, "flux"))
+            a["pr"] = np.zeros(case.size(nl, "p3s"))
+            for n in ("si", "sj", "sk"):
+                a[n] = g[n].copy()
+            a["vol"] = g["vol"].copy()
+            for n in ("dtvol", "betaxi", "betaeta", "betazeta"):
+                a[n] = np.zeros(case.size(nl, "p1"))
+            for n in ("xdot", "ydot", "zdot"):
+                a[n] = np.zeros(case.size(nl, "x2"))
+            a["q"][:] = case.freestream(nl)
+            if nl == 1 and q1 is not None:
+                a["q"][:] = q1
+            lev = self.s.lev[li]
+            lev.nblocks = L.nblocks
+            lev.ncuts = L.ncuts
+            ints = {n: np.ascontiguousarray(getattr(L, n), np.int32) for n in
+                    ("imax", "jmax", "kmax", "nbcs", "bctopo", "cutopo")}
+            if ints["cutopo"].size == 0:
+                ints["cutopo"] = np.zeros(21, np.int32)
+            self.keep.append(ints)
+            for n, v in ints.items():
+                setattr(lev, n, ip(v))
+            for n in self.FIELDS:
+                setattr(lev, n, dp(a[n]))
+            self.arrays.append(a)
+
+    def ptr(self):
+        return ctypes.byref(self.s)
+
+    def __getitem__(self, nl):
+        return self.arrays[nl - 1]
