@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the COSA pseudo-time residual/update path on one MI355X.
+
+Workload (default): the uneven multiblock configuration -- 96 independent
+bump-channel blocks whose node counts per direction are drawn from
+{17, 33, 65, 129} with numpy.random.default_rng(0) (24,285,184 cells), Euler
+at M = 0.7, freestream plus a 1% uniform perturbation (default_rng(1)),
+4-stage RK with central implicit residual smoothing (cirs_c), van Albada
+MUSCL, 3-level multigrid V-cycles, FP64.
+
+One step = one multigrid cycle of mg() (src/cosa.f90:822-912) over all
+blocks.  The metric is cell-updates per second: one RK stage applied to one
+interior cell, every level counted.  Inputs are resident in HBM before the
+timed region; the timed region is bracketed by device synchronisation.
+
+Prints ONE JSON line with the driver's contract plus ``roofline`` (dominant
+kernel family, achieved algorithmic GB/s from HIP events recorded on the
+library's stream over the timed region) and ``cpu_baseline`` (the CPU
+restatement timed on this host's cores on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per cell-update of each kernel family (FP64 Euler, static
+# grid; each required array read once and written once, halos not counted)
+BYTES_PER_CELL = {
+    "residual": 176.0,   # q 5x8 R + face metrics 3x4x8 R + res 5x8 W
+    "cirs": 248.0,       # res 5x8 R+W per sweep (3 sweeps) + dtvol 8 R
+    "update": 128.0,     # qold 40 R + res 40 R + dtvol 8 R + q 40 W
+    "halo": None,        # surface work
+    "convert": 80.0,     # q 40 R + 40 W
+    "lambda": 144.0,     # q 40 R + metrics 96 R + dtvol 8 W
+    "other": None,
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_case(workload: str):
+    from cosa_amd import case as cases
+    if workload == "uneven96":
+        dims = cases.uneven_dims(96, seed=0)
+        c = cases.channel_case(dims, nlevel=3, name="uneven96", machfs=0.7, irsop=1, cfl=2.0, cfl_irs=4.0,
+                               ilimf=4, epslimf=1e-6, nl_crs=2, ncycle=100000)
+        return c, dict(workload="uneven96", blocks=96, dims_seed=0, cells=c.ncell(), levels=3, physics="euler",
+                       mach=0.7, rk_stages=4, irs="cirs_c", limiter="van_albada", precision="fp64")
+    if workload == "naca":
+        c = cases.naca_case(193, 49, 9, nlevel=3, name="naca0012", irsop=1, cfl=2.0, cfl_irs=4.0, nl_crs=2,
+                            ncycle=100000)
+        return c, dict(workload="naca0012_193x49x9", cells=c.ncell(), levels=3, physics="euler", mach=0.5,
+                       alpha_deg=1.25, rk_stages=4, irs="cirs_c", precision="fp64")
+    raise SystemExit(f"unknown workload {workload}")
+
+
+def cpu_baseline(case_full, budget_s: float):
+    """Time the CPU restatement (oracle/liboracle.so, 1 thread) on the first
+    blocks of the same workload, for whole multigrid cycles."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import Oracle, OrCase, dp
+    from cosa_amd.case import Case
+    c0 = case_full
+    nb, cells = 0, 0
+    target = 1_500_000
+    for b in c0.blocks:
+        nb += 1
+        cells += (b.imax - 1) * (b.jmax - 1) * (b.kmax - 1)
+        if cells >= target:
+            break
+    sample = Case(dict(c0.config), c0.blocks[:nb], [], c0.xyz[:nb], name="sample")
+    o = Oracle()
+    o.set_config(sample.config)
+    oc = OrCase(sample, q1=sample.perturbed(amp=0.01, seed=1))
+    from cosa_amd.solver import Solve  # noqa: F401  (cell-update count helper below)
+    ns, st = 1, sample.config["nstage"]
+    per_cycle = st * (sample.ncell(1) + 2 * sample.ncell(2) + sample.ncell(3))
+    hist = np.zeros(8 * 64)
+    where = (ctypes.c_int32 * 8)()
+    done = (ctypes.c_int32 * 1)()
+    t0 = time.perf_counter()
+    ncyc = 0
+    cp = oc.ptr()
+    while True:
+        mgit = ncyc + 1
+        rc = o.or_smooth(cp, 1, 0, 1, 1, mgit, done, where)
+        o.or_restrict(cp, 1)
+        rc |= o.or_smooth(cp, 2, 1, 1, 1, mgit, done, where)
+        o.or_restrict(cp, 2)
+        rc |= o.or_smooth(cp, 3, 1, 0, 1, mgit, done, where)
+        o.or_prolong(cp, 2)
+        rc |= o.or_smooth(cp, 2, 0, 0, 1, mgit, done, where)
+        o.or_prolong(cp, 1)
+        ncyc += 1
+        if rc:
+            raise RuntimeError("oracle sample failed")
+        if time.perf_counter() - t0 > budget_s or ncyc >= 8:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(per_cycle * ncyc / dt / 1e6, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{nb} of 96 blocks ({sample.ncell()} cells), {ncyc} multigrid cycles, {dt:.1f} s, "
+                      f"oracle/liboracle.so (C restatement, gcc -O2, 1 thread)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="uneven96", choices=["uneven96", "naca"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs the path on one GPU only (one GPU's share of the blocks); "
+              "--gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    from cosa_amd.solver import Solve
+    t0 = time.time()
+    case, cfg_desc = make_case(a.workload)
+    log(f"case {case.name}: {case.ncell()} cells, built in {time.time() - t0:.1f} s")
+    q0 = case.perturbed(amp=0.01, seed=1)
+    s = Solve(case, device=0, q=q0)
+    del q0
+    log("device state resident; warmup")
+    mgit = 0
+    for _ in range(a.warmup):
+        mgit += 1
+        s.cycle(mgit)
+    s.synchronize()
+    s.timing_reset()
+    s.timing(True)
+    s.synchronize()
+    log(f"timed region: {a.steps} cycles")
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        mgit += 1
+        s.cycle(mgit)
+    s.synchronize()
+    t2 = time.perf_counter()
+    fam = s.timing_read()
+    s.timing(False)
+    elapsed = t2 - t1
+    per_cycle = s.cell_updates_per_cycle()
+    value = per_cycle * a.steps / elapsed / 1e6
+    res_hist = [float(h[0]) for h in s.history]
+    log(f"elapsed {elapsed:.3f} s, {value:.1f} Mcell-updates/s, resrms(1) {res_hist[0]:.3e} -> {res_hist[-1]:.3e}")
+
+    # roofline of the dominant family (most device time in the timed region)
+    fams = {}
+    for name, d in fam.items():
+        bpc = BYTES_PER_CELL.get(name)
+        if d["ms"] > 0 and bpc:
+            gbs = bpc * d["items"] / (d["ms"] / 1e3) / 1e9
+            fams[name] = {"ms_total": round(d["ms"], 3), "launches": d["launches"], "bytes_per_cell": bpc,
+                          "achieved_GBs": round(gbs, 1)}
+        elif d["ms"] > 0:
+            fams[name] = {"ms_total": round(d["ms"], 3), "launches": d["launches"]}
+    dom = max((n for n in fams if BYTES_PER_CELL.get(n)), key=lambda n: fams[n]["ms_total"])
+    d = fam[dom]
+    achieved = BYTES_PER_CELL[dom] * d["items"] / (d["ms"] / 1e3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_cell_update": BYTES_PER_CELL[dom], "avg_launch_ms": round(d["ms"] / max(d["launches"], 1), 4)}
+    s.close()
+
+    cpu = None
+    if not a.no_cpu:
+        log("cpu baseline (oracle, 1 thread)")
+        cpu = cpu_baseline(case, a.cpu_seconds)
+
+    out = {"metric": "Mcell-updates/s", "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(elapsed / a.steps * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": cfg_desc, "roofline": roof, "cpu_baseline": cpu, "families": fams,
+           "resrms1_first_last": [res_hist[0], res_hist[-1]]}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

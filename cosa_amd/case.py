@@ -171,28 +171,32 @@ def channel_case(dims: Sequence[Tuple[int, int, int]], nlevel=3, name="bump_chan
     return Case(c, blocks, [], xyz, name=name)
 
 
-def two_block_case(ni=17, nj=9, nk=9, nlevel=1, distort=0.0, name="two_blocks", **cfg) -> Case:
+def two_block_case(ni=17, nj=9, nk=9, nlevel=1, distort=0.0, name="two_blocks", wall=14, **cfg) -> Case:
     """Two blocks joined by a 1:1 cut across i (block 1 i=imax <-> block 2
     i=1), farfield elsewhere except a wall and symmetry planes; used by the
-    parity tests to exercise cuts between different blocks."""
+    parity tests to exercise cuts between different blocks.  wall = 30
+    gives a wall-free box (freestream is then an exact steady state)."""
     cfg.setdefault("machfs", 0.6)
     cfg.setdefault("alpha_deg", 3.0)
     c = default_config(nlevel=nlevel, **cfg)
     x, y, z = grids.bump_channel(2 * ni - 1, nj, nk, length=3.0, height=1.0, width=0.6)
     if distort:
-        x, y, z = grids.distort(x, y, z, distort, seed=3)
+        # distort: amplitude as a fraction of the smallest node spacing
+        h = min(np.min(np.abs(np.diff(a, axis=ax))[np.abs(np.diff(a, axis=ax)) > 0])
+                for a, ax in ((x, 2), (y, 1), (z, 0)))
+        x, y, z = grids.distort(x, y, z, distort * h, seed=3)
     a = tuple(arr[:, :, :ni].copy() for arr in (x, y, z))
     b = tuple(arr[:, :, ni - 1:].copy() for arr in (x, y, z))
     b1 = Block(ni, nj, nk, [
         Patch((1, 1, 1), (1, nj, nk), 30),
-        Patch((1, 1, 1), (ni, 1, nk), 14),
+        Patch((1, 1, 1), (ni, 1, nk), wall),
         Patch((1, nj, 1), (ni, nj, nk), 30),
         Patch((1, 1, 1), (ni, nj, 1), 80),
         Patch((1, 1, nk), (ni, nj, nk), 80),
     ])
     b2 = Block(ni, nj, nk, [
         Patch((ni, 1, 1), (ni, nj, nk), 4),
-        Patch((1, 1, 1), (ni, 1, nk), 14),
+        Patch((1, 1, 1), (ni, 1, nk), wall),
         Patch((1, nj, 1), (ni, nj, nk), 30),
         Patch((1, 1, 1), (ni, nj, 1), 80),
         Patch((1, 1, nk), (ni, nj, nk), 80),

@@ -1,0 +1,905 @@
+// api.hip -- the C ABI of libcosahip (include/cosahip.h): solve contexts,
+// level definition, host<->device transfers and the drivers smooth /
+// restrict / prolong, which issue the kernels in the order of COSA's own
+// smooth() (src/cosa.f90:1594-2032), restrict() (2672-2754) and prolong()
+// (3167-3249).
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "kernels.h"
+
+using namespace cosahip;
+
+#define CK(call)                                                                  \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) return fail(s, COSAHIP_ERR_DEVICE, #call, hipGetErrorString(e_)); \
+  } while (0)
+
+static int fail(Solve* s, int code, const char* what, const char* detail = "") {
+  if (s) {
+    s->err = std::string(what) + (detail && *detail ? std::string(": ") + detail : std::string());
+  }
+  return code;
+}
+
+// ---------------------------------------------------------------------------
+// small device helpers
+
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+// ---------------------------------------------------------------------------
+// instrumentation: HIP events recorded on the solve's stream around launches
+
+static hipEvent_t pool_event(Solve& s) {
+  if (!s.evpool.empty()) {
+    hipEvent_t e = s.evpool.back();
+    s.evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+namespace cosahip {
+TimeScope::TimeScope(Solve& s_, int fam_, ll items_) : s(s_), fam(fam_), items(items_) {
+  if (!s.timing) return;
+  a = pool_event(s);
+  if (a) (void)hipEventRecord(a, s.stream);
+}
+TimeScope::~TimeScope() {
+  if (!s.timing || !a) return;
+  hipEvent_t b = pool_event(s);
+  if (!b) return;
+  (void)hipEventRecord(b, s.stream);
+  s.pending.push_back(TimedLaunch{a, b, fam, items});
+}
+}  // namespace cosahip
+
+static void drain_timing(Solve& s) {
+  if (s.pending.empty()) return;
+  (void)hipStreamSynchronize(s.stream);
+  for (auto& t : s.pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      s.fam_ms[t.fam] += ms;
+      s.fam_launch[t.fam] += 1;
+      s.fam_items[t.fam] += t.items;
+    }
+    s.evpool.push_back(t.a);
+    s.evpool.push_back(t.b);
+  }
+  s.pending.clear();
+}
+
+static void free_level(Level& L) {
+  for (int f = 0; f < COSAHIP_NFIELDS; f++)
+    if (L.f[f]) (void)hipFree(L.f[f]);
+  void* ptrs[] = {L.dblk, L.dq, L.work1, L.cirs_r, L.rms_part, L.sbc, L.sbc_doff, L.dpatch, L.dcut};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (int k = 0; k < TK_NKIND; k++)
+    if (L.tile[k]) (void)hipFree(L.tile[k]);
+  L = Level();
+}
+
+static Par make_par(const cosahip_config& c) {
+  Par p;
+  memset(&p, 0, sizeof(p));
+  p.npde = c.npde;
+  p.nsnap = 2 * c.nharms + 1;
+  p.nsnap_met = 2 * c.nharms * c.hbmove + 1;
+  p.lmet = c.lmet;
+  p.hbmove = c.hbmove;
+  p.moving = c.moving;
+  p.ilimf = c.ilimf;
+  p.irsop = c.irsop;
+  p.nl_crs = c.nl_crs;
+  p.gamma = c.gamma;
+  p.rkap = c.rkap;
+  p.epslimf = c.epslimf;
+  p.cfl = c.cfl;
+  p.rat = c.rat;
+  p.psirs = c.psirs;
+  p.efx_setup_f = c.efx_setup_f;
+  p.efx_all_f = c.efx_all_f;
+  p.eig_limit_f = c.eig_limit_f;
+  p.eig_lim_all_f = c.eig_lim_all_f;
+  for (int i = 0; i < 3; i++) p.entfx_f_w[i] = c.entfx_f_w[i];
+  p.cntrpy_f = c.cntrpy_f;
+  p.entfxctf_f = c.entfxctf_f;
+  p.eig_cutoff_f = c.eig_cutoff_f;
+  p.machfs = c.machfs;
+  p.alpha = c.alpha;
+  p.beta = c.beta;
+  p.rhoinf = c.rhoinf;
+  p.pinf = c.pinf;
+  p.ainf = c.ainf;
+  // freestream of bc_far0, aircraft/vawt branch (src/bc.f90:145-152)
+  p.qfst[0] = c.rhoinf;
+  p.qfst[1] = c.machfs * cos(c.alpha) * cos(c.beta);
+  p.qfst[2] = c.machfs * sin(c.alpha) * cos(c.beta);
+  p.qfst[3] = c.machfs * sin(c.beta);
+  p.qfst[4] = c.pinf;
+  p.omega = c.omega;
+  p.qmin1 = c.qmin[0];
+  p.qmin5 = c.qmin[4];
+  return p;
+}
+
+static int check_config(Solve* s, const cosahip_config& c) {
+  if (c.npde != 5) return fail(s, COSAHIP_ERR_UNSUPPORTED, "npde != 5: k-omega transport is not built in this version");
+  if (c.kom || c.kom_bsl || c.kom_sst || c.viscous)
+    return fail(s, COSAHIP_ERR_UNSUPPORTED, "viscous / k-omega terms are not built in this version");
+  if (c.nharms < 0 || c.nharms > 65) return fail(s, COSAHIP_ERR_ARG, "nharms out of range 0..65");
+  if (c.nharms > 0 && !c.harbal) return fail(s, COSAHIP_ERR_ARG, "nharms > 0 requires harbal");
+  if (c.lmet != 4 && c.lmet != 7) return fail(s, COSAHIP_ERR_ARG, "lmet must be 4 or 7");
+  if (c.moving && c.lmet != 7) return fail(s, COSAHIP_ERR_ARG, "moving grids need lmet = 7");
+  if (c.nlevel < 1 || c.nlevel > 4) return fail(s, COSAHIP_ERR_ARG, "nlevel out of range 1..4");
+  if (c.irsop < 0 || c.irsop > 3) return fail(s, COSAHIP_ERR_UNSUPPORTED, "irsop 5/6 (viscous CIRS) not built");
+  if (c.irsop > 0 && c.cutcirs != 0) return fail(s, COSAHIP_ERR_UNSUPPORTED, "cutcirs = 1 (ccirs) not built");
+  if (c.ilimf < 0 || c.ilimf > 4) return fail(s, COSAHIP_ERR_ARG, "ilimf out of range 0..4");
+  if (c.nstage < 1 || c.nstage > 5) return fail(s, COSAHIP_ERR_ARG, "nstage out of range 1..5");
+  if (c.iupdt < 1 || c.iprint < 1) return fail(s, COSAHIP_ERR_ARG, "iupdt and iprint must be >= 1");
+  return COSAHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* cosahip_build_info(void) {
+  return "libcosahip " "gfx950 FP64, ABI " "1";
+}
+
+int cosahip_create(const cosahip_config* cfg, int device, cosahip_solve** out) {
+  if (!cfg || !out) return COSAHIP_ERR_ARG;
+  *out = nullptr;
+  Solve* s = new (std::nothrow) Solve();
+  if (!s) return COSAHIP_ERR_NOMEM;
+  s->cfg = *cfg;
+  int rc = check_config(s, *cfg);
+  if (rc) {
+    fprintf(stderr, "cosahip_create: %s\n", s->err.c_str());
+    delete s;
+    return rc;
+  }
+  s->par = make_par(*cfg);
+  if (device >= 0) {
+    if (hipSetDevice(device) != hipSuccess) {
+      delete s;
+      return COSAHIP_ERR_DEVICE;
+    }
+  }
+  if (hipGetDevice(&s->device) != hipSuccess || hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&s->dflag, 64) != hipSuccess || hipHostMalloc((void**)&s->hpinned, 64 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&s->ddhb, sizeof(double) * 131 * 131) != hipSuccess) {
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return COSAHIP_ERR_DEVICE;
+  }
+  s->lev.resize(cfg->nlevel);
+  *out = reinterpret_cast<cosahip_solve*>(s);
+  return COSAHIP_OK;
+}
+
+int cosahip_destroy(cosahip_solve* h) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_OK;
+  (void)hipSetDevice(s->device);
+  (void)hipStreamSynchronize(s->stream);
+  drain_timing(*s);
+  for (hipEvent_t e : s->evpool) (void)hipEventDestroy(e);
+  for (auto& L : s->lev) free_level(L);
+  if (s->dflag) (void)hipFree(s->dflag);
+  if (s->ddhb) (void)hipFree(s->ddhb);
+  if (s->hpinned) (void)hipHostFree(s->hpinned);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return COSAHIP_OK;
+}
+
+const char* cosahip_last_error(const cosahip_solve* h) {
+  const Solve* s = reinterpret_cast<const Solve*>(h);
+  return s ? s->err.c_str() : "null solve";
+}
+
+int cosahip_error_info(const cosahip_solve* h, int32_t info[8]) {
+  const Solve* s = reinterpret_cast<const Solve*>(h);
+  if (!s || !info) return COSAHIP_ERR_ARG;
+  memcpy(info, s->errinfo, sizeof(s->errinfo));
+  return COSAHIP_OK;
+}
+
+int cosahip_set_hb_operator(cosahip_solve* h, int nsnap, const double* dhb) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s || !dhb) return COSAHIP_ERR_ARG;
+  if (nsnap != s->par.nsnap) return fail(s, COSAHIP_ERR_ARG, "dhb size must be 2*nharms+1");
+  memcpy(s->dhb, dhb, sizeof(double) * nsnap * nsnap);
+  CK(hipSetDevice(s->device));
+  CK(hipMemcpy(s->ddhb, s->dhb, sizeof(double) * nsnap * nsnap, hipMemcpyHostToDevice));
+  return COSAHIP_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// level definition
+
+template <class T>
+static hipError_t upload_vec(T** d, const std::vector<T>& h) {
+  size_t n = std::max<size_t>(h.size(), 1);
+  hipError_t e = hipMalloc(d, n * sizeof(T));
+  if (e != hipSuccess) return e;
+  if (!h.empty()) e = hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+  return e;
+}
+
+static void add_tiles(std::vector<Tile>& v, int block, ll items, int per) {
+  for (ll b0 = 0; b0 < items; b0 += per) v.push_back(Tile{block, (int)b0});
+}
+
+static int build_tables(Solve* s, Level& L, const Level* Lc) {
+  const int ns = s->par.nsnap;
+  for (int k = 0; k < TK_NKIND; k++) L.htile[k].clear();
+  for (int b = 0; b < L.nblocks; b++) {
+    const Blk& B = L.hblk[b];
+    const ll im = B.imax - 1, jm = B.jmax - 1, km = B.kmax - 1;
+    add_tiles(L.htile[TK_CELLS], b, ns * im * jm * km, TS);
+    const ll nseg = (im + ROWSEG - 1) / ROWSEG;
+    add_tiles(L.htile[TK_ROWS_I], b, ns * jm * km * nseg, 4);
+    add_tiles(L.htile[TK_LINES_J], b, ns * im * km, TS);
+    add_tiles(L.htile[TK_LINES_K], b, ns * im * jm, TS);
+    add_tiles(L.htile[TK_LINES_I], b, ns * jm * km, TS);
+    add_tiles(L.htile[TK_CELLS_P1], b, ns * (ll)(B.imax + 1) * (B.jmax + 1) * (B.kmax + 1), TS);
+    add_tiles(L.htile[TK_HALO], b, ns * ((ll)(B.imax + 3) * jm * km + 4 * im * km + 4 * im * jm), TS);
+    if (Lc) {
+      const Blk& C = Lc->hblk[b];
+      add_tiles(L.htile[TK_COARSE], b, ns * (ll)(C.imax - 1) * (C.jmax - 1) * (C.kmax - 1), TS);
+      add_tiles(L.htile[TK_PROL], b, ns * (ll)((B.imax + 1) / 2) * ((B.jmax + 1) / 2) * ((B.kmax + 1) / 2), TS);
+    }
+  }
+  for (int k = 0; k < TK_NKIND; k++) {
+    if (L.tile[k]) (void)hipFree(L.tile[k]);
+    L.tile[k] = nullptr;
+    L.ntile[k] = (int)L.htile[k].size();
+    CK(upload_vec(&L.tile[k], L.htile[k]));
+  }
+  return COSAHIP_OK;
+}
+
+// Host replica of k_cut's index arithmetic: every halo target and interior
+// source of the cut must lie inside its block's P3 extents.
+static bool cut_in_bounds(const Blk& b1, const Blk& b2, const int32_t* t, int n2, int count) {
+  const int idir1 = t[1], inout1 = t[2], idir2 = t[10], inout2 = t[11];
+  int istr1[4], iend1[4], istr2[4], iend2[4], iord[4];
+  istr1[1] = t[3]; iend1[1] = t[4]; istr1[2] = t[5]; iend1[2] = t[6]; istr1[3] = t[7]; iend1[3] = t[8];
+  istr2[1] = t[12]; iend2[1] = t[13]; istr2[2] = t[14]; iend2[2] = t[15]; istr2[3] = t[16]; iend2[3] = t[17];
+  iord[1] = t[18]; iord[2] = t[19]; iord[3] = t[20];
+  for (int m = 1; m <= 3; m++)
+    if (iord[m] < 1 || iord[m] > 3) return false;
+  const int ijk1[4] = {0, b1.imax, b1.jmax, b1.kmax}, ijk2[4] = {0, b2.imax, b2.jmax, b2.kmax};
+  const int ibcpt = inout1 == 1 ? 0 : ijk1[idir1], ibcinc = inout1 == 1 ? -1 : 1;
+  const int inr = inout2 == 1 ? 1 : ijk2[idir2] - 1, inrinc = inout2 == 1 ? 1 : -1;
+  auto cyc = [](int d, int m) { return (d - 1 + m - 1) % 3 + 1; };
+  const int ic1 = cyc(idir1, 1), ic2 = cyc(idir1, 2), ic3 = cyc(idir1, 3);
+  const int jc1 = iord[ic1], jc2 = iord[ic2], jc3 = iord[ic3];
+  const int sg1c2 = iend1[ic2] > istr1[ic2] ? 1 : -1, sg1c3 = iend1[ic3] > istr1[ic3] ? 1 : -1;
+  const int sg2c2 = iend2[jc2] > istr2[jc2] ? 1 : -1, sg2c3 = iend2[jc3] > istr2[jc3] ? 1 : -1;
+  auto inside = [](const Blk& b, int i, int j, int k) {
+    return i >= -1 && i <= b.imax + 1 && j >= -1 && j <= b.jmax + 1 && k >= -1 && k <= b.kmax + 1;
+  };
+  for (int item = 0; item < count; item++) {
+    const int i2_ = item % n2, i3_ = item / n2;
+    const int a2 = istr1[ic2] + sg1c2 * i2_, a3 = istr1[ic3] + sg1c3 * i3_;
+    int x1[4] = {0, 0, 0, 0};
+    x1[ic1] = ibcpt;
+    x1[ic2] = a2;
+    x1[ic3] = a3;
+    int x2[4] = {0, x1[1], x1[2], x1[3]};
+    x2[ic1] += ibcinc;
+    const int c2 = istr2[jc2] + sg2c2 * i2_, c3 = istr2[jc3] + sg2c3 * i3_;
+    int y1[4] = {0, 0, 0, 0};
+    y1[jc1] = inr;
+    y1[jc2] = c2;
+    y1[jc3] = c3;
+    int y2[4] = {0, y1[1], y1[2], y1[3]};
+    y2[jc1] += inrinc;
+    if (!inside(b1, x1[1], x1[2], x1[3]) || !inside(b1, x2[1], x2[2], x2[3]) || !inside(b2, y1[1], y1[2], y1[3]) ||
+        !inside(b2, y2[1], y2[2], y2[3]))
+      return false;
+  }
+  return true;
+}
+
+static bool strong_bc(int t) {
+  return t == 13 || t == 14 || t / 100 == 15 || t == 22 || t == 23 || t == 92 || t == 93 || t == 5;
+}
+
+extern "C" int cosahip_define_level(cosahip_solve* h, int nl, int nblocks, const int32_t* imax, const int32_t* jmax,
+                                    const int32_t* kmax, const int32_t* nbcs, const int32_t* bctopo, int ncuts,
+                                    const int32_t* cutopo) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  if (nl < 1 || nl > (int)s->lev.size()) return fail(s, COSAHIP_ERR_ARG, "level out of range");
+  if (nblocks < 1 || !imax || !jmax || !kmax || !nbcs) return fail(s, COSAHIP_ERR_ARG, "bad block arrays");
+  CK(hipSetDevice(s->device));
+  Level& L = s->lev[nl - 1];
+  free_level(L);
+  L.nl = nl;
+  L.nblocks = nblocks;
+  L.ncuts = ncuts;
+  L.imax.assign(imax, imax + nblocks);
+  L.jmax.assign(jmax, jmax + nblocks);
+  L.kmax.assign(kmax, kmax + nblocks);
+  L.nbcs.assign(nbcs, nbcs + nblocks);
+  ll npatch = 0;
+  for (int b = 0; b < nblocks; b++) {
+    if (imax[b] < 2 || jmax[b] < 2 || kmax[b] < 2) return fail(s, COSAHIP_ERR_ARG, "block with fewer than 2 nodes");
+    npatch += nbcs[b];
+  }
+  if (npatch && !bctopo) return fail(s, COSAHIP_ERR_ARG, "bctopo missing");
+  L.bctopo.assign(bctopo, bctopo + 10 * npatch);
+  if (ncuts && !cutopo) return fail(s, COSAHIP_ERR_ARG, "cutopo missing");
+  L.cutopo.assign(cutopo, cutopo + 21 * (ll)ncuts);
+  // blocks and offsets (setoffs)
+  ll o3 = 0, o2 = 0, o1 = 0, o0 = 0;
+  for (int b = 0; b < nblocks; b++) {
+    Blk B;
+    B.imax = imax[b];
+    B.jmax = jmax[b];
+    B.kmax = kmax[b];
+    B.nbc = nbcs[b];
+    B.o3 = o3;
+    B.o2 = o2;
+    B.o1 = o1;
+    B.o0 = o0;
+    o3 += n3of(B);
+    o2 += n2of(B);
+    o1 += n1of(B);
+    o0 += (ll)B.imax * B.jmax * B.kmax;
+    L.ncell += (ll)(B.imax - 1) * (B.jmax - 1) * (B.kmax - 1);
+    if (B.imax - 1 < 2 || B.jmax - 1 < 2 || B.kmax - 1 < 2) L.thin = true;
+    L.hblk.push_back(B);
+  }
+  L.n3tot = o3;
+  L.n2tot = o2;
+  L.n1tot = o1;
+  L.n0tot = o0;
+  // BC patches and strong-BC face flags
+  ll pi = 0;
+  L.sbc_off.resize(nblocks);
+  ll soff = 0;
+  for (int b = 0; b < nblocks; b++) {
+    const Blk& B = L.hblk[b];
+    L.sbc_off[b] = soff;
+    soff += 2 * (ll)(B.jmax + 1) * (B.kmax + 1) + 2 * (ll)(B.imax + 1) * (B.kmax + 1) + 2 * (ll)(B.imax + 1) * (B.jmax + 1);
+  }
+  std::vector<unsigned char> sbc(std::max<ll>(soff, 1), 0);
+  for (int b = 0; b < nblocks; b++) {
+    const Blk& B = L.hblk[b];
+    for (int ib = 0; ib < nbcs[b]; ib++, pi++) {
+      const int32_t* t = bctopo + 10 * pi;
+      PatchRec r;
+      memset(&r, 0, sizeof(r));
+      r.block = b;
+      r.type = t[0];
+      for (int m = 0; m < 10; m++) r.t[m] = t[m];
+      const int idir = t[1];
+      if (idir < 1 || idir > 3) return fail(s, COSAHIP_ERR_ARG, "bctopo: idir out of range");
+      const int ic2 = (idir - 1 + 1) % 3 + 1, ic3 = (idir - 1 + 2) % 3 + 1;
+      const int n2 = t[2 * ic2 + 2] - t[2 * ic2 + 1] + 1, n3c = t[2 * ic3 + 2] - t[2 * ic3 + 1] + 1;
+      if (n2 < 0 || n3c < 0) return fail(s, COSAHIP_ERR_ARG, "bctopo: empty or reversed range");
+      const int ext[4] = {0, B.imax - 1, B.jmax - 1, B.kmax - 1};
+      if (t[2 * ic2 + 1] < 1 || t[2 * ic2 + 2] > ext[ic2] || t[2 * ic3 + 1] < 1 || t[2 * ic3 + 2] > ext[ic3])
+        return fail(s, COSAHIP_ERR_ARG, "bctopo: range outside the block");
+      r.count = n2 * n3c;
+      r.n2 = n2 > 0 ? n2 : 1;
+      const int ty = t[0];
+      if (ty != 30 && ty != 14 && ty != 80 && ty != 4) {
+        char msg[96];
+        snprintf(msg, sizeof msg, "BC type %d is not built in this version (30, 14, 80, 4 are)", ty);
+        return fail(s, COSAHIP_ERR_UNSUPPORTED, msg);
+      }
+      L.hpatch.push_back(r);
+      L.maxpatch = std::max(L.maxpatch, r.count);
+      if (strong_bc(ty)) {
+        const int face = 2 * (idir - 1) + (t[2] == 1 ? 0 : 1);
+        ll off = L.sbc_off[b];
+        ll w;
+        if (idir == 1) {
+          off += (face - 0) * (ll)(B.jmax + 1) * (B.kmax + 1);
+          w = B.jmax + 1;
+          for (int kk = t[7]; kk <= t[8]; kk++)
+            for (int jj = t[5]; jj <= t[6]; jj++) sbc[off + jj + w * kk] = 1;
+        } else if (idir == 2) {
+          off += 2 * (ll)(B.jmax + 1) * (B.kmax + 1) + (face - 2) * (ll)(B.imax + 1) * (B.kmax + 1);
+          w = B.imax + 1;
+          for (int kk = t[7]; kk <= t[8]; kk++)
+            for (int ii = t[3]; ii <= t[4]; ii++) sbc[off + ii + w * kk] = 1;
+        } else {
+          off += 2 * (ll)(B.jmax + 1) * (B.kmax + 1) + 2 * (ll)(B.imax + 1) * (B.kmax + 1) +
+                 (face - 4) * (ll)(B.imax + 1) * (B.jmax + 1);
+          w = B.imax + 1;
+          for (int jj = t[5]; jj <= t[6]; jj++)
+            for (int ii = t[3]; ii <= t[4]; ii++) sbc[off + ii + w * jj] = 1;
+        }
+      }
+    }
+  }
+  // cut records
+  for (int ic = 0; ic < ncuts; ic++) {
+    const int32_t* t = cutopo + 21 * (ll)ic;
+    CutRec r;
+    memset(&r, 0, sizeof(r));
+    r.b1 = t[0] - 1;
+    r.b2 = t[9] - 1;
+    if (r.b1 < 0 || r.b1 >= nblocks || r.b2 < 0 || r.b2 >= nblocks) return fail(s, COSAHIP_ERR_ARG, "cutopo: block");
+    for (int m = 0; m < 21; m++) r.t[m] = t[m];
+    const int idir1 = t[1];
+    if (idir1 < 1 || idir1 > 3 || t[10] < 1 || t[10] > 3) return fail(s, COSAHIP_ERR_ARG, "cutopo: idir");
+    const int ic2 = (idir1 - 1 + 1) % 3 + 1, ic3 = (idir1 - 1 + 2) % 3 + 1;
+    const int len2 = abs(t[2 * ic2 + 2] - t[2 * ic2 + 1]), len3 = abs(t[2 * ic3 + 2] - t[2 * ic3 + 1]);
+    r.n2 = len2 + 1;
+    r.count = (len2 + 1) * (len3 + 1);
+    if (!cut_in_bounds(L.hblk[r.b1], L.hblk[r.b2], t, r.n2, r.count))
+      return fail(s, COSAHIP_ERR_ARG, "cutopo: cut maps outside a block");
+    L.hcut.push_back(r);
+    L.maxcut = std::max(L.maxcut, r.count);
+  }
+  CK(upload_vec(&L.dblk, L.hblk));
+  CK(upload_vec(&L.dpatch, L.hpatch));
+  CK(upload_vec(&L.dcut, L.hcut));
+  CK(upload_vec(&L.sbc, sbc));
+  CK(upload_vec(&L.sbc_doff, L.sbc_off));
+  // fields
+  const Par& p = s->par;
+  const ll sizes[COSAHIP_NFIELDS] = {
+      L.n3tot * p.npde * p.nsnap, L.n3tot * p.npde * p.nsnap, L.n3tot * p.npde * p.nsnap, L.n3tot * p.npde * p.nsnap,
+      L.n2tot * p.lmet * p.nsnap_met, L.n2tot * p.lmet * p.nsnap_met, L.n2tot * p.lmet * p.nsnap_met, L.n1tot,
+      L.n1tot * p.nsnap, L.n1tot * p.nsnap, L.n1tot * p.nsnap, L.n1tot * p.nsnap,
+      p.moving ? L.n2tot * p.nsnap_met : 1, p.moving ? L.n2tot * p.nsnap_met : 1, p.moving ? L.n2tot * p.nsnap_met : 1};
+  for (int f = 0; f < COSAHIP_NFIELDS; f++) {
+    if (hipMalloc(&L.f[f], sizes[f] * sizeof(double)) != hipSuccess)
+      return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
+    CK(hipMemset(L.f[f], 0, sizes[f] * sizeof(double)));
+  }
+  const ll nq = L.n3tot * p.npde * p.nsnap;
+  if (hipMalloc(&L.dq, nq * sizeof(double)) != hipSuccess || hipMalloc(&L.work1, nq * sizeof(double)) != hipSuccess ||
+      hipMalloc(&L.cirs_r, L.n3tot * p.nsnap * sizeof(double)) != hipSuccess)
+    return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
+  CK(hipMemset(L.dq, 0, nq * sizeof(double)));
+  CK(hipMemset(L.work1, 0, nq * sizeof(double)));
+  int rc = build_tables(s, L, nl < (int)s->lev.size() && s->lev[nl].nblocks ? &s->lev[nl] : nullptr);
+  if (rc) return rc;
+  if (nl >= 2 && s->lev[nl - 2].nblocks) {
+    rc = build_tables(s, s->lev[nl - 2], &L);
+    if (rc) return rc;
+  }
+  CK(hipMalloc(&L.rms_part, ((ll)L.ntile[TK_CELLS] + 1) * 8 * sizeof(double)));
+  return COSAHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// transfers
+
+static ll field_size(const Solve* s, const Level& L, int field) {
+  const Par& p = s->par;
+  switch (field) {
+    case COSAHIP_Q: case COSAHIP_QOLD: case COSAHIP_RES: case COSAHIP_RHS: return L.n3tot * p.npde * p.nsnap;
+    case COSAHIP_SI: case COSAHIP_SJ: case COSAHIP_SK: return L.n2tot * p.lmet * p.nsnap_met;
+    case COSAHIP_VOL: return L.n1tot;
+    case COSAHIP_DTVOL: case COSAHIP_BETAXI: case COSAHIP_BETAETA: case COSAHIP_BETAZETA: return L.n1tot * p.nsnap;
+    case COSAHIP_XDOT: case COSAHIP_YDOT: case COSAHIP_ZDOT: return p.moving ? L.n2tot * p.nsnap_met : 0;
+  }
+  return -1;
+}
+
+static Level* get_level(Solve* s, int nl) {
+  if (!s || nl < 1 || nl > (int)s->lev.size() || !s->lev[nl - 1].nblocks) return nullptr;
+  return &s->lev[nl - 1];
+}
+
+extern "C" int64_t cosahip_field_size(const cosahip_solve* h, int nl, int field) {
+  const Solve* s = reinterpret_cast<const Solve*>(h);
+  if (!s || nl < 1 || nl > (int)s->lev.size() || !s->lev[nl - 1].nblocks) return -1;
+  if (field < 0 || field >= COSAHIP_NFIELDS) return -1;
+  return field_size(s, s->lev[nl - 1], field);
+}
+
+extern "C" int cosahip_upload(cosahip_solve* h, int nl, int field, const double* host, int64_t count) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  Level* L = get_level(s, nl);
+  if (!L) return fail(s, COSAHIP_ERR_STATE, "upload: level not defined");
+  if (field < 0 || field >= COSAHIP_NFIELDS || !host) return fail(s, COSAHIP_ERR_ARG, "upload: bad field");
+  if (count != field_size(s, *L, field)) return fail(s, COSAHIP_ERR_ARG, "upload: count does not match the field size");
+  if (!count) return COSAHIP_OK;
+  CK(hipSetDevice(s->device));
+  CK(hipStreamSynchronize(s->stream));
+  CK(hipMemcpy(L->f[field], host, count * sizeof(double), hipMemcpyHostToDevice));
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_download(cosahip_solve* h, int nl, int field, double* host, int64_t count) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  Level* L = get_level(s, nl);
+  if (!L) return fail(s, COSAHIP_ERR_STATE, "download: level not defined");
+  if (field < 0 || field >= COSAHIP_NFIELDS || !host) return fail(s, COSAHIP_ERR_ARG, "download: bad field");
+  if (count != field_size(s, *L, field)) return fail(s, COSAHIP_ERR_ARG, "download: count does not match the field size");
+  if (!count) return COSAHIP_OK;
+  CK(hipSetDevice(s->device));
+  CK(hipStreamSynchronize(s->stream));
+  CK(hipMemcpy(host, L->f[field], count * sizeof(double), hipMemcpyDeviceToHost));
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_synchronize(cosahip_solve* h) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  CK(hipStreamSynchronize(s->stream));
+  return COSAHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// whole-level reference routines
+
+static double* qfield(Level& L, int field) {
+  if (field == COSAHIP_Q || field == COSAHIP_QOLD) return L.f[field];
+  return nullptr;
+}
+
+#define LEVEL_OR_FAIL(s, nl, Lp)                                                   \
+  Solve* s = reinterpret_cast<Solve*>(h);                                          \
+  Level* Lp = get_level(s, nl);                                                    \
+  if (!Lp) return fail(s, COSAHIP_ERR_STATE, "level not defined");                 \
+  CK(hipSetDevice(s->device));
+
+extern "C" int cosahip_cons2prim(cosahip_solve* h, int nl, int field) {
+  LEVEL_OR_FAIL(s, nl, L);
+  double* q = qfield(*L, field);
+  if (!q) return fail(s, COSAHIP_ERR_ARG, "cons2prim: field must be Q or QOLD");
+  CK(launch_cons2prim(*s, *L, q));
+  return COSAHIP_OK;
+}
+extern "C" int cosahip_prim2cons(cosahip_solve* h, int nl, int field) {
+  LEVEL_OR_FAIL(s, nl, L);
+  double* q = qfield(*L, field);
+  if (!q) return fail(s, COSAHIP_ERR_ARG, "prim2cons: field must be Q or QOLD");
+  CK(launch_prim2cons(*s, *L, q));
+  return COSAHIP_OK;
+}
+extern "C" int cosahip_bc(cosahip_solve* h, int nl, int field) {
+  LEVEL_OR_FAIL(s, nl, L);
+  double* q = qfield(*L, field);
+  if (!q) return fail(s, COSAHIP_ERR_ARG, "bc: field must be Q or QOLD");
+  CK(launch_bc(*s, *L, q));
+  return COSAHIP_OK;
+}
+extern "C" int cosahip_cutman_q(cosahip_solve* h, int nl, int field) {
+  LEVEL_OR_FAIL(s, nl, L);
+  double* q = qfield(*L, field);
+  if (!q && field == COSAHIP_RES) q = L->f[COSAHIP_RES];
+  if (!q) return fail(s, COSAHIP_ERR_ARG, "cutman_q: field must be Q, QOLD or RES");
+  CK(launch_cut(*s, *L, q));
+  return COSAHIP_OK;
+}
+extern "C" int cosahip_edges_corners(cosahip_solve* h, int nl, int field) {
+  LEVEL_OR_FAIL(s, nl, L);
+  double* q = qfield(*L, field);
+  if (!q) return fail(s, COSAHIP_ERR_ARG, "edges_corners: field must be Q or QOLD");
+  CK(launch_edges_corners(*s, *L, q));
+  return COSAHIP_OK;
+}
+
+static int decode_prtest(Solve* s, int nl, unsigned long long key) {
+  const int kind = (int)(key & 3);
+  const int loc = (int)((key >> 2) & 0x7FFFFFF);
+  const unsigned long long pos = (key >> 29) & ((1ull << 26) - 1);
+  const int reg = (int)(pos % 5);
+  const int bn = (int)(pos / 5);
+  const int nsnap = s->par.nsnap;
+  const int b = bn / nsnap, n = bn % nsnap;
+  Level& L = s->lev[nl - 1];
+  const Blk& B = L.hblk[b];
+  const int im = B.imax - 1, jm = B.jmax - 1;
+  int i = 0, j = 0, k = 0, x = loc;
+  if (reg == 0) {
+    i = x % (B.imax + 3) - 1;
+    x /= (B.imax + 3);
+    j = x % jm + 1;
+    k = x / jm + 1;
+  } else if (reg <= 2) {
+    i = x % im + 1;
+    x /= im;
+    int jj = x % 2;
+    k = x / 2 + 1;
+    j = reg == 1 ? jj - 1 : B.jmax + jj;
+  } else {
+    i = x % im + 1;
+    x /= im;
+    j = x % jm + 1;
+    int kk = x / jm;
+    k = reg == 3 ? kk - 1 : B.kmax + kk;
+  }
+  int32_t info[8] = {nl, b + 1, i, j, k, (int)(key >> 55), kind, n};
+  memcpy(s->errinfo, info, sizeof(info));
+  char msg[160];
+  snprintf(msg, sizeof msg, "%s <= qmin at level %d block %d (i,j,k) = (%d,%d,%d) snapshot %d",
+           kind == 1 ? "pressure" : "density", nl, b + 1, i, j, k, n);
+  return fail(s, COSAHIP_ERR_POSITIVITY, msg);
+}
+
+static const unsigned long long NOFLAG = ~0ull;
+
+static int read_flag(Solve* s, unsigned long long* key) {
+  CK(hipMemcpyAsync(s->hpinned, s->dflag, sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream));
+  CK(hipStreamSynchronize(s->stream));
+  memcpy(key, s->hpinned, sizeof(*key));
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_prtest(cosahip_solve* h, int nl, int iter) {
+  LEVEL_OR_FAIL(s, nl, L);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, s->stream, s->dflag, NOFLAG);
+  CK(launch_prtest(*s, *L, L->f[COSAHIP_Q], iter));
+  unsigned long long key;
+  int rc = read_flag(s, &key);
+  if (rc) return rc;
+  if (key != NOFLAG) return decode_prtest(s, nl, key);
+  return COSAHIP_OK;
+}
+extern "C" int cosahip_lambda_deltat(cosahip_solve* h, int nl) {
+  LEVEL_OR_FAIL(s, nl, L);
+  CK(launch_lambda_deltat(*s, *L));
+  return COSAHIP_OK;
+}
+extern "C" int cosahip_residual(cosahip_solve* h, int nl) {
+  LEVEL_OR_FAIL(s, nl, L);
+  CK(launch_zero_res(*s, *L, L->f[COSAHIP_RES]));
+  CK(launch_residual(*s, *L, L->f[COSAHIP_Q], L->f[COSAHIP_RES]));
+  return COSAHIP_OK;
+}
+extern "C" int cosahip_src_hb(cosahip_solve* h, int nl) {
+  LEVEL_OR_FAIL(s, nl, L);
+  if (!s->cfg.harbal) return COSAHIP_OK;
+  CK(launch_src_hb(*s, *L));
+  return COSAHIP_OK;
+}
+
+static int rms(Solve* s, Level& L, double* out) {
+  const int np = s->par.npde;
+  if (s->cfg.harbal) {
+    CK(launch_copy_interior(*s, L, L.work1, L.f[COSAHIP_RES]));
+    CK(launch_rms(*s, L, L.work1, nullptr, 0));
+  } else {
+    CK(launch_rms(*s, L, L.f[COSAHIP_RES], L.f[COSAHIP_RHS], 1));
+  }
+  CK(hipMemcpyAsync(s->hpinned, L.rms_part + (ll)L.ntile[TK_CELLS] * 8, 5 * sizeof(double), hipMemcpyDeviceToHost,
+                    s->stream));
+  CK(hipStreamSynchronize(s->stream));
+  double sum[8] = {0};
+  for (int v = 0; v < np; v++) sum[v] = s->hpinned[v];
+  double tot = 0;
+  for (int v = 0; v < np; v++) tot = tot + sum[v];
+  const double nc = (double)L.ncell;
+  if (s->cfg.harbal) {
+    const double d5 = s->par.nsnap;
+    for (int v = 0; v < np; v++) out[v] = sqrt(sum[v] / nc / d5);
+    out[np] = sqrt(tot / nc / d5 / np);
+  } else {
+    for (int v = 0; v < np; v++) out[v] = sqrt(sum[v] / nc);
+    out[np] = sqrt(tot / nc / np);
+  }
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_resg_rms(cosahip_solve* h, int nl, double* resrms) {
+  LEVEL_OR_FAIL(s, nl, L);
+  if (!resrms) return fail(s, COSAHIP_ERR_ARG, "resrms null");
+  return rms(s, *L, resrms);
+}
+extern "C" int cosahip_add_array(cosahip_solve* h, int nl, int dst, int src, double fact) {
+  LEVEL_OR_FAIL(s, nl, L);
+  if (dst < 0 || dst > COSAHIP_RHS || src < 0 || src > COSAHIP_RHS) return fail(s, COSAHIP_ERR_ARG, "add_array: field");
+  CK(launch_add_interior(*s, *L, L->f[dst], L->f[src], fact));
+  return COSAHIP_OK;
+}
+extern "C" int cosahip_copy_array(cosahip_solve* h, int nl, int dst, int src) {
+  LEVEL_OR_FAIL(s, nl, L);
+  if (dst < 0 || dst > COSAHIP_RHS || src < 0 || src > COSAHIP_RHS) return fail(s, COSAHIP_ERR_ARG, "copy_array: field");
+  CK(launch_copy_interior(*s, *L, L->f[dst], L->f[src]));
+  return COSAHIP_OK;
+}
+extern "C" int cosahip_cirs(cosahip_solve* h, int nl) {
+  LEVEL_OR_FAIL(s, nl, L);
+  CK(launch_cirs(*s, *L));
+  return COSAHIP_OK;
+}
+extern "C" int cosahip_update(cosahip_solve* h, int nl, double ar) {
+  LEVEL_OR_FAIL(s, nl, L);
+  CK(launch_update(*s, *L, ar));
+  return COSAHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// smooth (src/cosa.f90:1594-2032)
+
+extern "C" int cosahip_smooth(cosahip_solve* h, int nl, int from_fine, int to_coarse, int niter, int mgit,
+                              int32_t* done, double* resrms) {
+  LEVEL_OR_FAIL(s, nl, Lp);
+  Level& L = *Lp;
+  const cosahip_config& c = s->cfg;
+  if (done) *done = 0;
+  double* q = L.f[COSAHIP_Q];
+  const int niterp = to_coarse ? niter + 1 : niter;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, s->stream, s->dflag, NOFLAG);
+  int seq = 0;
+  bool returned_early = false;
+  for (int iter = 1; iter <= niterp && !returned_early; iter++) {
+    const ll cs = L.ncell * s->par.nsnap;
+    if (iter <= niter) {
+      TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
+      CK(launch_copy_interior(*s, L, L.f[COSAHIP_QOLD], q));
+    }
+    for (int irk = 1; irk <= c.nstage; irk++, seq++) {
+      {
+        TimeScope t(*s, COSAHIP_FAM_CONVERT, cs);
+        CK(launch_cons2prim(*s, L, q));
+      }
+      {
+        TimeScope t(*s, COSAHIP_FAM_HALO, cs);
+        CK(launch_bc(*s, L, q));
+        CK(launch_cut(*s, L, q));
+      }
+      {
+        TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
+        CK(launch_prtest(*s, L, q, seq));
+      }
+      if (iter % c.iupdt == 0 && irk == 1) {
+        TimeScope t(*s, COSAHIP_FAM_LAMBDA, cs);
+        CK(launch_lambda_deltat(*s, L));
+      }
+      {
+        TimeScope t(*s, COSAHIP_FAM_RESIDUAL, cs);
+        CK(launch_residual(*s, L, q, L.f[COSAHIP_RES]));
+      }
+      {
+        TimeScope t(*s, COSAHIP_FAM_CONVERT, cs);
+        CK(launch_prim2cons(*s, L, q));
+      }
+      if (c.harbal) {
+        TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
+        CK(launch_src_hb(*s, L));
+      }
+      if (nl == 1 && irk == 1 && iter != niter + 1 && (mgit == 1 || mgit % c.iprint == 0 || mgit == c.ncycle)) {
+        double rr[8];
+        int rc = rms(s, L, rr);
+        if (rc) return rc;
+        for (int v = 0; v <= s->par.npde; v++) s->resrms[v + 1] = rr[v];
+      }
+      if (from_fine && iter == 1 && irk == 1) {
+        TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
+        CK(launch_add_interior(*s, L, L.f[COSAHIP_RHS], L.f[COSAHIP_RES], 1.0));
+      }
+      if (nl > 1) {
+        TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
+        CK(launch_add_interior(*s, L, L.f[COSAHIP_RES], L.f[COSAHIP_RHS], -1.0));
+      }
+      if (iter == niter + 1) {
+        returned_early = true;
+        break;
+      }
+      {
+        TimeScope t(*s, COSAHIP_FAM_CIRS, cs);
+        CK(launch_cirs(*s, L));
+      }
+      {
+        TimeScope t(*s, COSAHIP_FAM_UPDATE, cs);
+        CK(launch_update(*s, L, c.alphas[irk - 1]));
+      }
+    }
+    if (returned_early) break;
+    double errmax = s->resrms[1];
+    for (int v = 2; v <= 5; v++) errmax = std::max(errmax, s->resrms[v]);
+    if (errmax < c.toler) {
+      if (done) *done = 1;
+      break;
+    }
+  }
+  unsigned long long key;
+  int rc = read_flag(s, &key);
+  if (rc) return rc;
+  if (resrms)
+    for (int v = 0; v < 8; v++) resrms[v] = s->resrms[v + 1];
+  if (key != NOFLAG) return decode_prtest(s, nl, key);
+  return COSAHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// restrict / prolong
+
+static int transfer_ready(Solve* s, int nl) {
+  if (nl < 1 || nl + 1 > (int)s->lev.size() || !s->lev[nl - 1].nblocks || !s->lev[nl].nblocks)
+    return fail(s, COSAHIP_ERR_STATE, "restrict/prolong: both levels must be defined");
+  Level& L1 = s->lev[nl - 1];
+  Level& L2 = s->lev[nl];
+  if (L1.nblocks != L2.nblocks) return fail(s, COSAHIP_ERR_ARG, "levels have different block counts");
+  for (int b = 0; b < L1.nblocks; b++)
+    if ((L1.hblk[b].imax - 1) != 2 * (L2.hblk[b].imax - 1) || (L1.hblk[b].jmax - 1) != 2 * (L2.hblk[b].jmax - 1) ||
+        (L1.hblk[b].kmax - 1) != 2 * (L2.hblk[b].kmax - 1))
+      return fail(s, COSAHIP_ERR_ARG, "coarse level is not the 2:1 coarsening of the fine level");
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_restrict(cosahip_solve* h, int nl) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  int rc = transfer_ready(s, nl);
+  if (rc) return rc;
+  CK(hipSetDevice(s->device));
+  Level& L1 = s->lev[nl - 1];
+  Level& L2 = s->lev[nl];
+  TimeScope t(*s, COSAHIP_FAM_OTHER, L2.ncell * s->par.nsnap);
+  CK(launch_rest1(*s, L1, L2, L1.f[COSAHIP_Q], L2.f[COSAHIP_Q]));
+  CK(launch_rest2(*s, L1, L2, L2.f[COSAHIP_RHS], L1.f[COSAHIP_RES]));
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_prolong(cosahip_solve* h, int nl) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  int rc = transfer_ready(s, nl);
+  if (rc) return rc;
+  CK(hipSetDevice(s->device));
+  Level& L1 = s->lev[nl - 1];
+  Level& L2 = s->lev[nl];
+  TimeScope t(*s, COSAHIP_FAM_OTHER, L1.ncell * s->par.nsnap);
+  CK(launch_rest1(*s, L1, L2, L1.f[COSAHIP_Q], L2.f[COSAHIP_QOLD]));
+  double* arrs[2] = {L2.f[COSAHIP_QOLD], L2.f[COSAHIP_Q]};
+  for (double* a : arrs) {
+    CK(launch_cons2prim(*s, L2, a));
+    CK(launch_bc(*s, L2, a));
+    CK(launch_cut(*s, L2, a));
+    CK(launch_prim2cons(*s, L2, a));
+    CK(launch_edges_corners(*s, L2, a));
+  }
+  CK(launch_sub_box(*s, L2, L2.f[COSAHIP_Q], L2.f[COSAHIP_QOLD]));
+  CK(launch_prol(*s, L1, L2, L1.f[COSAHIP_Q], L2.f[COSAHIP_Q]));
+  return COSAHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// instrumentation API
+
+extern "C" int cosahip_timing_enable(cosahip_solve* h, int on) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  if (!on) drain_timing(*s);
+  s->timing = on != 0;
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_timing_reset(cosahip_solve* h) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  drain_timing(*s);
+  for (int f = 0; f < COSAHIP_NFAM; f++) {
+    s->fam_ms[f] = 0;
+    s->fam_launch[f] = 0;
+    s->fam_items[f] = 0;
+  }
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_timing_read(cosahip_solve* h, int family, double* total_ms, int64_t* launches,
+                                   int64_t* items) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s || family < 0 || family >= COSAHIP_NFAM) return COSAHIP_ERR_ARG;
+  drain_timing(*s);
+  if (total_ms) *total_ms = s->fam_ms[family];
+  if (launches) *launches = s->fam_launch[family];
+  if (items) *items = s->fam_items[family];
+  return COSAHIP_OK;
+}

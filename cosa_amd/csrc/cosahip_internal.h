@@ -1,0 +1,161 @@
+// cosahip_internal.h -- device layout helpers and solve state of libcosahip.
+//
+// Device arrays keep the reference's flat layout (src/input.f90:4878-5131):
+// whole-level allocations with blocks concatenated at the setoffs offsets,
+// i fastest, so one block's P3 array is
+//   a(-1:imax+1,-1:jmax+1,-1:kmax+1, ncomp, 0:2N)
+// and uploads/downloads are single copies.  Kernels address blocks through
+// a small per-level table of Blk records and iterate over per-level "tile"
+// tables (block, first work item) so one launch covers every block of a
+// level whatever their sizes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "cosahip.h"
+
+namespace cosahip {
+
+typedef long long ll;
+
+// one block of one level, as seen by kernels
+struct Blk {
+  int imax, jmax, kmax, nbc;   // node counts; number of BC patches
+  ll o3, o2, o1, o0;           // off_p3, off_p2, off_p1, off_0 (points)
+};
+
+// run-time constants every kernel needs (passed by value)
+struct Par {
+  int npde, nsnap, nsnap_met, lmet, hbmove, moving;
+  int ilimf, irsop, nl_crs;
+  double gamma, rkap, epslimf;
+  double cfl, rat, psirs;
+  int efx_setup_f, efx_all_f, eig_limit_f, eig_lim_all_f;
+  int entfx_f_w[3];
+  double cntrpy_f, entfxctf_f, eig_cutoff_f;
+  double machfs, alpha, beta, rhoinf, pinf, ainf;
+  double qfst[5];
+  double omega;
+  double qmin1, qmin5;
+};
+
+__host__ __device__ inline ll n3of(const Blk& b) { return (ll)(b.imax + 3) * (b.jmax + 3) * (b.kmax + 3); }
+__host__ __device__ inline ll n2of(const Blk& b) { return (ll)(b.imax + 2) * (b.jmax + 2) * (b.kmax + 2); }
+__host__ __device__ inline ll n1of(const Blk& b) { return (ll)(b.imax + 1) * (b.jmax + 1) * (b.kmax + 1); }
+// P3 point index of (i,j,k), i,j,k from -1
+__host__ __device__ inline ll i3(const Blk& b, int i, int j, int k) {
+  return (ll)(i + 1) + (ll)(b.imax + 3) * ((ll)(j + 1) + (ll)(b.jmax + 3) * (ll)(k + 1));
+}
+// P2 point index (0-based from 0)
+__host__ __device__ inline ll i2(const Blk& b, int i, int j, int k) {
+  return (ll)i + (ll)(b.imax + 2) * ((ll)j + (ll)(b.jmax + 2) * (ll)k);
+}
+// P1 point index
+__host__ __device__ inline ll i1(const Blk& b, int i, int j, int k) {
+  return (ll)i + (ll)(b.imax + 1) * ((ll)j + (ll)(b.jmax + 1) * (ll)k);
+}
+
+// work item tiles: (block, first item) -- one GPU workgroup per tile
+struct Tile {
+  int block, base;
+};
+
+enum TileKind {
+  TK_CELLS = 0,   // interior cells (imax-1)(jmax-1)(kmax-1), i fastest
+  TK_ROWS_I,      // i-direction flux rows: (jmax-1)(kmax-1) rows x segments
+  TK_LINES_J,     // (imax-1)(kmax-1) lines along j
+  TK_LINES_K,     // (imax-1)(jmax-1) lines along k
+  TK_LINES_I,     // (jmax-1)(kmax-1) lines along i
+  TK_CELLS_P1,    // 0..imax x 0..jmax x 0..kmax
+  TK_COARSE,      // cells of the next coarser level, per block (restriction)
+  TK_PROL,        // 2x2x2 prolongation groups 0..imax-1 step 2, per block
+  TK_HALO,        // prtest region: interior + face halo strips
+  TK_NKIND
+};
+
+// a BC patch or one directional cut record, flattened for kernels
+struct PatchRec {
+  int block;       // 0-based
+  int type;        // BC type (cuts: -1)
+  int t[10];       // bctopo(1:10)
+  int count;       // face cells of the patch (transverse range product)
+  int n2;          // length of the first transverse range
+};
+struct CutRec {
+  int b1, b2;      // 0-based
+  int t[21];       // cutopo(1:21)
+  int count, n2;   // (len(ic2)+1)*(len(ic3)+1), len(ic2)+1
+};
+
+struct Level {
+  int nl = 0, nblocks = 0, ncuts = 0;
+  std::vector<Blk> hblk;
+  std::vector<int32_t> imax, jmax, kmax, nbcs, bctopo, cutopo;
+  Blk* dblk = nullptr;
+  ll n3tot = 0, n2tot = 0, n1tot = 0, n0tot = 0, ncell = 0;
+  double* f[COSAHIP_NFIELDS] = {};
+  double* dq = nullptr;       // HB term (qhb) and scratch
+  double* work1 = nullptr;    // copy of res for the HB norm
+  double* cirs_r = nullptr;   // Thomas multipliers r(j), one per cell
+  double* rms_part = nullptr; // per-tile partial sums of squares
+  unsigned char* sbc = nullptr;  // strong-BC face flags (6 faces per block)
+  std::vector<ll> sbc_off;    // per block: offset of its 6 face flag planes
+  ll* sbc_doff = nullptr;
+  std::vector<Tile> htile[TK_NKIND];
+  Tile* tile[TK_NKIND] = {};
+  int ntile[TK_NKIND] = {};
+  std::vector<PatchRec> hpatch;
+  PatchRec* dpatch = nullptr;
+  int maxpatch = 0;
+  std::vector<CutRec> hcut;
+  CutRec* dcut = nullptr;
+  int maxcut = 0;
+  bool thin = false;          // a block has < 2 cells in some direction
+};
+
+struct TimedLaunch {
+  hipEvent_t a, b;
+  int fam;
+  ll items;
+};
+
+struct Solve {
+  cosahip_config cfg;
+  Par par;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<Level> lev;     // index nl-1
+  double dhb[131 * 131];      // dhb(i,j) column-major
+  double* ddhb = nullptr;
+  unsigned long long* dflag = nullptr;   // prtest first offender key
+  double* hpinned = nullptr;             // small pinned buffer for results
+  double resrms[9] = {};                 // resrms(1:8), index 0 unused
+  std::string err;
+  int32_t errinfo[8] = {};
+  // instrumentation
+  bool timing = false;
+  std::vector<TimedLaunch> pending;
+  std::vector<hipEvent_t> evpool;
+  double fam_ms[COSAHIP_NFAM] = {};
+  ll fam_launch[COSAHIP_NFAM] = {}, fam_items[COSAHIP_NFAM] = {};
+};
+
+// bracket a launch sequence with timing events when enabled
+struct TimeScope {
+  Solve& s;
+  int fam;
+  ll items;
+  hipEvent_t a = nullptr;
+  TimeScope(Solve& s_, int fam_, ll items_);
+  ~TimeScope();
+};
+
+// tile sizes
+constexpr int TS = 256;        // items per tile, 1-D kernels
+constexpr int ROWSEG = 63;     // cells per i-row segment (64 faces)
+
+}  // namespace cosahip

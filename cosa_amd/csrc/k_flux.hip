@@ -1,0 +1,523 @@
+// k_flux.hip -- spectral radii / local time step, the MUSCL + Roe residual
+// and the implicit residual smoothing of COSA's pseudo-time stage, for
+// gfx950.  Arithmetic follows the reference routines operation by
+// operation (the library is built with -ffp-contract=off), so FP64 results
+// match the CPU path to rounding of the few library calls (pow).
+//
+//   lambda + deltat : src/cosa.f90:1248-1346, 4183-4237, 4733-5203
+//   muscl + limit   : src/flux.f90:1221-1812, 3472-3888
+//   roflux          : src/flux.f90:45-260
+//   resid           : src/cosa.f90:1149-1211
+//   cirs_b + tridi  : src/cosa.f90:5238-5623, 2418-2443
+#include "kernels.h"
+
+namespace cosahip {
+
+// ------------------------------------------------------------------------
+// Roe flux-difference splitting for one face (mean-flow variables).
+__device__ __forceinline__ void roe_face(const Par& p, const double* qr, const double* ql, double kx, double ky,
+                                         double kz, double ds, double ubn, double* F) {
+  const double gamma = p.gamma;
+  double rhor = qr[0], ur = qr[1], vr = qr[2], wr = qr[3], pr = qr[4];
+  double unr = ur * kx + vr * ky + wr * kz - ubn;
+  double q2r = ur * ur + vr * vr + wr * wr;
+  double tker = 0.0;
+  double h0r = gamma / (gamma - 1) * pr / rhor + q2r / 2 + tker;
+  double rh0r = rhor * h0r;
+  double fr0 = rhor * unr;
+  double fr1 = (rhor * ur * unr + pr * kx);
+  double fr2 = (rhor * vr * unr + pr * ky);
+  double fr3 = (rhor * wr * unr + pr * kz);
+  double fr4 = (rh0r * unr + pr * ubn);
+  double rhol = ql[0], ul = ql[1], vl = ql[2], wl = ql[3], pl = ql[4];
+  double unl = ul * kx + vl * ky + wl * kz - ubn;
+  double q2l = ul * ul + vl * vl + wl * wl;
+  double tkel = 0.0;
+  double h0l = gamma / (gamma - 1) * pl / rhol + q2l / 2 + tkel;
+  double rh0l = rhol * h0l;
+  double fl0 = rhol * unl;
+  double fl1 = (rhol * ul * unl + pl * kx);
+  double fl2 = (rhol * vl * unl + pl * ky);
+  double fl3 = (rhol * wl * unl + pl * kz);
+  double fl4 = (rh0l * unl + pl * ubn);
+  double drho = rhor - rhol, dp = pr - pl, du = ur - ul, dv = vr - vl, dw = wr - wl, dun = unr - unl;
+  double dtke = tker - tkel;
+  double rhols = sqrt(rhol), rhors = sqrt(rhor);
+  double rho = rhols * rhors;
+  double denom = rhols + rhors;
+  double u = (rhols * ul + rhors * ur) / denom;
+  double v = (rhols * vl + rhors * vr) / denom;
+  double w = (rhols * wl + rhors * wr) / denom;
+  double un = u * kx + v * ky + w * kz;
+  double h0 = (rhols * h0l + rhors * h0r) / denom;
+  double tke = (rhols * tkel + rhors * tker) / denom;
+  double q2 = u * u + v * v + w * w;
+  double a2 = (gamma - 1) * (h0 - q2 / 2 - tke);
+  double a = sqrt(a2);
+  double rm123 = fabs(un - ubn), rm4 = fabs(un + a - ubn), rm5 = fabs(un - a - ubn);
+  if (p.efx_setup_f) {
+    double rdelta = p.cntrpy_f * (p.entfx_f_w[0] * p.entfxctf_f + p.entfx_f_w[1] * (sqrt(u * u + v * v + w * w) + a) +
+                                  p.entfx_f_w[2] * 0.2 * (fabs(un) + a + fabs(ubn)));
+    rdelta = fmax(rdelta, 1.e-12);
+    if (rm123 <= rdelta && p.efx_all_f) rm123 = 0.5 * (rm123 * rm123 + rdelta * rdelta) / rdelta;
+    if (rm4 <= rdelta) rm4 = 0.5 * (rm4 * rm4 + rdelta * rdelta) / rdelta;
+    if (rm5 <= rdelta) rm5 = 0.5 * (rm5 * rm5 + rdelta * rdelta) / rdelta;
+  }
+  if (p.eig_limit_f) {
+    if (rm123 <= p.eig_cutoff_f) rm123 = p.eig_cutoff_f;
+    if (p.eig_lim_all_f && rm4 <= p.eig_cutoff_f) rm4 = p.eig_cutoff_f;
+    if (p.eig_lim_all_f && rm5 <= p.eig_cutoff_f) rm5 = p.eig_cutoff_f;
+  }
+  double al1 = rm123 * (drho - dp / a2);
+  double al2 = rm123 * rho;
+  double al3 = rm4 * (dp / a2 + rho * dun / a) / 2;
+  double al4 = rm5 * (dp / a2 - rho * dun / a) / 2;
+  double df0 = al1 + al3 + al4;
+  double df1 = al1 * u + al2 * (du - dun * kx) + al3 * (u + a * kx) + al4 * (u - a * kx);
+  double df2 = al1 * v + al2 * (dv - dun * ky) + al3 * (v + a * ky) + al4 * (v - a * ky);
+  double df3 = al1 * w + al2 * (dw - dun * kz) + al3 * (w + a * kz) + al4 * (w - a * kz);
+  double df4 = al1 * (q2 / 2 + tke) + al2 * (u * du + v * dv + w * dw - un * dun + dtke) + al3 * (h0 + a * un) +
+               al4 * (h0 - a * un);
+  F[0] = (fr0 + fl0 - df0) / 2 * ds;
+  F[1] = (fr1 + fl1 - df1) / 2 * ds;
+  F[2] = (fr2 + fl2 - df2) / 2 * ds;
+  F[3] = (fr3 + fl3 - df3) / 2 * ds;
+  F[4] = (fr4 + fl4 - df4) / 2 * ds;
+}
+
+// ------------------------------------------------------------------------
+// Limited slopes of one variable at face a: dqp(a), dqm(a) from
+// dm1 = dq(a-1), d0 = dq(a), dp1 = dq(a+1)   (limit_i/j/k).
+__device__ __forceinline__ void limit_face(int ilim, double eps, double rkap, double dm1, double d0, double dp1,
+                                           double& dqp, double& dqm) {
+  if (ilim == 0) {
+    double rkp = 1 + rkap, rkm = 1 - rkap;
+    dqp = -(rkp * d0 + rkm * dp1) / 4;
+    dqm = (rkm * dm1 + rkp * d0) / 4;
+  } else if (ilim == 1) {
+    double s = (2 * dp1 * d0 + eps) / (dp1 * dp1 + d0 * d0 + eps);
+    double rkp = 1 + rkap * s, rkm = 1 - rkap * s;
+    dqp = -s * (rkp * d0 + rkm * dp1) / 4;
+    s = (2 * d0 * dm1 + eps) / (d0 * d0 + dm1 * dm1 + eps);
+    rkp = 1 + rkap * s;
+    rkm = 1 - rkap * s;
+    dqm = s * (rkm * dm1 + rkp * d0) / 4;
+  } else if (ilim == 2) {
+    double rkp = 1 + rkap, rkm = 1 - rkap, b = (3.0 - rkap) / (1.0 - rkap);
+    // dqp(a): loop index a with dq(a), dq(a+1)
+    double dqmx = (copysign(1.0, d0) + copysign(1.0, dp1)) / 2 * fmin(fabs(d0), fabs(dp1) * b);
+    double dqpx = (copysign(1.0, dp1) + copysign(1.0, d0)) / 2 * fmin(fabs(dp1), fabs(d0) * b);
+    dqp = -(rkp * dqmx + rkm * dqpx) / 4;
+    // dqm(a): loop index a-1 with dq(a-1), dq(a)
+    dqmx = (copysign(1.0, dm1) + copysign(1.0, d0)) / 2 * fmin(fabs(dm1), fabs(d0) * b);
+    dqpx = (copysign(1.0, d0) + copysign(1.0, dm1)) / 2 * fmin(fabs(d0), fabs(dm1) * b);
+    dqm = (rkm * dqmx + rkp * dqpx) / 4;
+  } else if (ilim == 3) {
+    double epsq = eps * eps;
+    double d1 = dp1, d2 = d0;
+    double dqpx = (d2 * (d1 * d1 + 2 * epsq) + d1 * (2 * d2 * d2 + epsq)) / (2 * d2 * d2 - d2 * d1 + 2 * d1 * d1 + 3 * epsq);
+    dqp = -dqpx / 2;
+    d1 = d0;
+    d2 = dm1;
+    double dqmx = (d1 * (d2 * d2 + 2 * epsq) + d2 * (2 * d1 * d1 + epsq)) / (2 * d1 * d1 - d1 * d2 + 2 * d2 * d2 + 3 * epsq);
+    dqm = dqmx / 2;
+  } else {
+    double epsq = eps * eps;
+    double d1 = dp1, d2 = d0;
+    double dqmx = (d1 * (d2 * d2 + epsq) + d2 * (d1 * d1 + epsq)) / (d1 * d1 + d2 * d2 + 2 * epsq);
+    dqp = -dqmx / 2;
+    d1 = d0;
+    d2 = dm1;
+    dqmx = (d1 * (d2 * d2 + epsq) + d2 * (d1 * d1 + epsq)) / (d1 * d1 + d2 * d2 + 2 * epsq);
+    dqm = dqmx / 2;
+  }
+}
+
+// Face states qp (right), qm (left) of face a from q(a-2..a+1) of one line.
+// fin/fout: the line's inner (a = 1) / outer (a = L) face is a strong BC.
+__device__ __forceinline__ void muscl_face(const Par& p, int nl, int a, int L, bool fin, bool fout, const double* qm2,
+                                           const double* qm1, const double* q0, const double* qp1, double* QR,
+                                           double* QL) {
+  if (nl >= p.nl_crs) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      QR[v] = q0[v];
+      QL[v] = qm1[v];
+    }
+  } else {
+    bool sm1 = (a - 1 == 1 && fin) || (a - 1 == L && fout);
+    bool s0 = (a == 1 && fin) || (a == L && fout);
+    bool sp1 = (a + 1 == 1 && fin) || (a + 1 == L && fout);
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      double dm1 = qm1[v] - qm2[v];
+      double d0 = q0[v] - qm1[v];
+      double dp1 = qp1[v] - q0[v];
+      if (sm1) dm1 = (qm1[v] - qm2[v]) * 2;
+      if (s0) d0 = (q0[v] - qm1[v]) * 2;
+      if (sp1) dp1 = (qp1[v] - q0[v]) * 2;
+      double dqp, dqm;
+      limit_face(p.ilimf, p.epslimf, p.rkap, dm1, d0, dp1, dqp, dqm);
+      QR[v] = q0[v] + dqp;
+      QL[v] = qm1[v] + dqm;
+    }
+  }
+  if (a == 1 && fin) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) QR[v] = QL[v] = qm1[v];
+  }
+  if (a == L && fout) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) QR[v] = QL[v] = q0[v];
+  }
+}
+
+__device__ __forceinline__ void face_metric(const Par& p, const double* s, double& kx, double& ky, double& kz,
+                                            double& ds, double& ubn) {
+  kx = s[0];
+  ky = s[1];
+  kz = s[2];
+  ds = s[3];
+  ubn = 0.0;
+  if (p.moving) ubn = s[4] * kx + s[5] * ky + s[6] * kz;
+}
+
+// ------------------------------------------------------------------------
+// i direction: one wavefront per row segment of ROWSEG cells; lane l
+// computes the flux of face i0+l and the cell flux difference takes the
+// neighbouring lane's face flux through a cross-lane shuffle.
+__global__ void __launch_bounds__(256) k_flux_i(Par p, int nl, const Blk* __restrict__ blks,
+                                                const Tile* __restrict__ tiles, const double* __restrict__ q,
+                                                const double* __restrict__ si, double* __restrict__ res,
+                                                const unsigned char* __restrict__ sbc, const ll* __restrict__ sbcoff) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int lane = threadIdx.x & 63;
+  const int item = t.base + (threadIdx.x >> 6);
+  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
+  const int nseg = (im + ROWSEG - 1) / ROWSEG;
+  const int nrows = jm * km;
+  if (item >= nseg * nrows * p.nsnap) return;   // whole wavefront exits
+  const int n = item / (nseg * nrows);
+  const int rem = item - n * nseg * nrows;
+  const int row = rem / nseg, seg = rem - (rem / nseg) * nseg;
+  const int j = row % jm + 1, k = row / jm + 1;
+  const int a = seg * ROWSEG + 1 + lane;   // face index
+  const int L = b.imax;
+  const ll n3 = n3of(b);
+  const double* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  double F[5] = {0, 0, 0, 0, 0};
+  if (a <= L) {
+    const unsigned char* fl = sbc + sbcoff[t.block];
+    const ll pl = (ll)(b.jmax + 1) * (b.kmax + 1);
+    bool fin = fl[j + (ll)(b.jmax + 1) * k] != 0;
+    bool fout = fl[pl + j + (ll)(b.jmax + 1) * k] != 0;
+    double w[4][5];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int v = 0; v < 5; v++) w[m][v] = qb[i3(b, a - 2 + m, j, k) + n3 * v];
+    double QR[5], QL[5];
+    muscl_face(p, nl, a, L, fin, fout, w[0], w[1], w[2], w[3], QR, QL);
+    const int nh = n * p.hbmove;
+    const double* s = si + b.o2 * p.lmet * p.nsnap_met + ((ll)nh * n2of(b) + i2(b, a, j, k)) * p.lmet;
+    double kx, ky, kz, ds, ubn;
+    face_metric(p, s, kx, ky, kz, ds, ubn);
+    roe_face(p, QR, QL, kx, ky, kz, ds, ubn, F);
+  }
+  double Fn[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) Fn[v] = __shfl_down(F[v], 1, 64);
+  const int c = a;   // cell c = face a, difference F(c+1) - F(c)
+  if (lane < ROWSEG && c <= im) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) rb[i3(b, c, j, k) + n3 * v] = (0.0 + Fn[v]) - F[v];
+  }
+}
+
+// j or k direction: one thread per line, marching along the line with a
+// four-cell register window; res(c) += F(c+1) - F(c) in the reference order.
+template <int DIR>
+__global__ void __launch_bounds__(256) k_flux_march(Par p, int nl, const Blk* __restrict__ blks,
+                                                    const Tile* __restrict__ tiles, const double* __restrict__ q,
+                                                    const double* __restrict__ sm, double* __restrict__ res,
+                                                    const unsigned char* __restrict__ sbc,
+                                                    const ll* __restrict__ sbcoff) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int item = t.base + threadIdx.x;
+  const int im = b.imax - 1;
+  const int t2n = DIR == 2 ? b.kmax - 1 : b.jmax - 1;
+  const int nlines = im * t2n;
+  if (item >= nlines * p.nsnap) return;
+  const int n = item / nlines;
+  const int rem = item - n * nlines;
+  const int i = rem % im + 1, t2 = rem / im + 1;
+  const int L = DIR == 2 ? b.jmax : b.kmax;
+  const ll n3 = n3of(b);
+  const double* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  const ll step = DIR == 2 ? (ll)(b.imax + 3) : (ll)(b.imax + 3) * (b.jmax + 3);
+  // point index of a = 0 on this line
+  const ll base = DIR == 2 ? i3(b, i, 0, t2) : i3(b, i, t2, 0);
+  const unsigned char* fl = sbc + sbcoff[t.block];
+  bool fin, fout;
+  if (DIR == 2) {
+    const ll off = 2 * (ll)(b.jmax + 1) * (b.kmax + 1);
+    const ll pl = (ll)(b.imax + 1) * (b.kmax + 1);
+    fin = fl[off + i + (ll)(b.imax + 1) * t2] != 0;
+    fout = fl[off + pl + i + (ll)(b.imax + 1) * t2] != 0;
+  } else {
+    const ll off = 2 * (ll)(b.jmax + 1) * (b.kmax + 1) + 2 * (ll)(b.imax + 1) * (b.kmax + 1);
+    const ll pl = (ll)(b.imax + 1) * (b.jmax + 1);
+    fin = fl[off + i + (ll)(b.imax + 1) * t2] != 0;
+    fout = fl[off + pl + i + (ll)(b.imax + 1) * t2] != 0;
+  }
+  const int nh = n * p.hbmove;
+  const double* sb = sm + b.o2 * p.lmet * p.nsnap_met + (ll)nh * n2of(b) * p.lmet;
+  double w0[5], w1[5], w2[5], w3[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) {
+    w0[v] = qb[base - step + n3 * v];
+    w1[v] = qb[base + n3 * v];
+    w2[v] = qb[base + step + n3 * v];
+    w3[v] = qb[base + 2 * step + n3 * v];
+  }
+  double Fp[5];
+  for (int a = 1; a <= L; a++) {
+    double QR[5], QL[5], F[5];
+    muscl_face(p, nl, a, L, fin, fout, w0, w1, w2, w3, QR, QL);
+    const ll sidx = DIR == 2 ? i2(b, i, a, t2) : i2(b, i, t2, a);
+    double kx, ky, kz, ds, ubn;
+    face_metric(p, sb + sidx * p.lmet, kx, ky, kz, ds, ubn);
+    roe_face(p, QR, QL, kx, ky, kz, ds, ubn, F);
+    if (a >= 2) {
+      const ll c = base + (ll)(a - 1) * step;
+#pragma unroll
+      for (int v = 0; v < 5; v++) rb[c + n3 * v] = (rb[c + n3 * v] + F[v]) - Fp[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      Fp[v] = F[v];
+      w0[v] = w1[v];
+      w1[v] = w2[v];
+      w2[v] = w3[v];
+    }
+    if (a + 2 <= L + 1) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) w3[v] = qb[base + (ll)(a + 2) * step + n3 * v];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// lambda (inviscid eig_eul) + deltat, one thread per (cell, snapshot)
+__global__ void __launch_bounds__(256) k_lambda_deltat(Par p, const Blk* __restrict__ blks,
+                                                       const Tile* __restrict__ tiles, const double* __restrict__ q,
+                                                       const double* __restrict__ si, const double* __restrict__ sj,
+                                                       const double* __restrict__ sk, const double* __restrict__ xdot,
+                                                       const double* __restrict__ ydot,
+                                                       const double* __restrict__ zdot, double* __restrict__ dtvol,
+                                                       double* __restrict__ bx, double* __restrict__ be,
+                                                       double* __restrict__ bz) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int item = t.base + threadIdx.x;
+  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
+  const int nc = im * jm * km;
+  if (item >= nc * p.nsnap) return;
+  const int n = item / nc;
+  int rem = item - n * nc;
+  const int i = rem % im + 1;
+  rem /= im;
+  const int j = rem % jm + 1, k = rem / jm + 1;
+  const ll n3 = n3of(b);
+  const double* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  const ll c = i3(b, i, j, k);
+  double rho = qb[c], u = qb[c + n3], v = qb[c + 2 * n3], w = qb[c + 3 * n3], pp = qb[c + 4 * n3];
+  const int nh = n * p.hbmove;
+  const ll mo = b.o2 * p.lmet * p.nsnap_met + (ll)nh * n2of(b) * p.lmet;
+  const double* sim = si + mo + i2(b, i, j, k) * p.lmet;
+  const double* sip = si + mo + i2(b, i + 1, j, k) * p.lmet;
+  const double* sjm = sj + mo + i2(b, i, j, k) * p.lmet;
+  const double* sjp = sj + mo + i2(b, i, j + 1, k) * p.lmet;
+  const double* skm = sk + mo + i2(b, i, j, k) * p.lmet;
+  const double* skp = sk + mo + i2(b, i, j, k + 1) * p.lmet;
+  double a = sqrt(p.gamma * pp / rho);
+  double xix = 0.5 * (sim[0] * sim[3] + sip[0] * sip[3]);
+  double xiy = 0.5 * (sim[1] * sim[3] + sip[1] * sip[3]);
+  double xiz = 0.5 * (sim[2] * sim[3] + sip[2] * sip[3]);
+  double etax = 0.5 * (sjm[0] * sjm[3] + sjp[0] * sjp[3]);
+  double etay = 0.5 * (sjm[1] * sjm[3] + sjp[1] * sjp[3]);
+  double etaz = 0.5 * (sjm[2] * sjm[3] + sjp[2] * sjp[3]);
+  double zetax = 0.5 * (skm[0] * skm[3] + skp[0] * skp[3]);
+  double zetay = 0.5 * (skm[1] * skm[3] + skp[1] * skp[3]);
+  double zetaz = 0.5 * (skm[2] * skm[3] + skp[2] * skp[3]);
+  double axi = (sim[3] + sip[3]) / 2;
+  double aeta = (sjm[3] + sjp[3]) / 2;
+  double azeta = (skm[3] + skp[3]) / 2;
+  if (p.moving) {
+    const ll xo = b.o2 * p.nsnap_met + (ll)nh * n2of(b) + i2(b, i, j, k);
+    u = u - xdot[xo];
+    v = v - ydot[xo];
+    w = w - zdot[xo];
+  }
+  double lami = fabs(u * xix + v * xiy + w * xiz) + a * axi;
+  double lamj = fabs(u * etax + v * etay + w * etaz) + a * aeta;
+  double lamk = fabs(u * zetax + v * zetay + w * zetaz) + a * azeta;
+  const ll po = b.o1 * p.nsnap + (ll)n * n1of(b) + i1(b, i, j, k);
+  dtvol[po] = p.cfl / (lami + lamj + lamk);
+  if (p.irsop == 2) {
+    const double zeta = 0.4;
+    double phixi = 1 + pow(lamj / lami, zeta) + pow(lamk / lami, zeta);
+    double phieta = 1 + pow(lami / lamj, zeta) + pow(lamk / lamj, zeta);
+    double phizeta = 1 + pow(lami / lamk, zeta) + pow(lamj / lamk, zeta);
+    double tt;
+    tt = p.rat * lami * phixi / (lami + lamj + lamk);
+    bx[po] = fmax(0.0, (tt * tt - 1) / 4);
+    tt = p.rat * lamj * phieta / (lami + lamj + lamk);
+    be[po] = fmax(0.0, (tt * tt - 1) / 4);
+    tt = p.rat * lamk * phizeta / (lami + lamj + lamk);
+    bz[po] = fmax(0.0, (tt * tt - 1) / 4);
+  } else if (p.irsop == 3) {
+    double rji = lamj / lami, rki = lamk / lami, rij = lami / lamj, rkj = lamk / lamj, rik = lami / lamk,
+           rjk = lamj / lamk;
+    double tt;
+    tt = p.rat / (1 + p.psirs * (rji + rki));
+    bx[po] = fmax(0.0, 0.25 * (tt * tt - 1));
+    tt = p.rat / (1 + p.psirs * (rij + rkj));
+    be[po] = fmax(0.0, 0.25 * (tt * tt - 1));
+    tt = p.rat / (1 + p.psirs * (rik + rjk));
+    bz[po] = fmax(0.0, 0.25 * (tt * tt - 1));
+  }
+}
+
+// ------------------------------------------------------------------------
+// CIRS: one thread per (line, snapshot) solving the tridiagonal system of
+// the five flow variables with the reference's Thomas recurrence; r(j) is
+// kept in a per-cell scratch, t(j) in place in res.  The first sweep (i)
+// scales res by dtvol, the last (k) divides it back (cirs_b order).
+template <int DIR>
+__global__ void __launch_bounds__(256) k_cirs(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
+                                              double* __restrict__ res, const double* __restrict__ dtvol,
+                                              const double* __restrict__ beta, double* __restrict__ rsc, int first,
+                                              int last) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int item = t.base + threadIdx.x;
+  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
+  const int T1 = DIR == 1 ? jm : im;
+  const int T2 = DIR == 3 ? jm : km;
+  const int nlines = T1 * T2;
+  if (item >= nlines * p.nsnap) return;
+  const int n = item / nlines;
+  const int rem = item - n * nlines;
+  const int t1 = rem % T1 + 1, t2 = rem / T1 + 1;
+  const int L = DIR == 1 ? im : (DIR == 2 ? jm : km);
+  const ll n3 = n3of(b);
+  double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  double* rs = rsc + b.o3 * p.nsnap + n3 * (ll)n;
+  const double* dtb = dtvol + b.o1 * p.nsnap + n1of(b) * (ll)n;
+  const double* bb = beta + b.o1 * p.nsnap + n1of(b) * (ll)n;
+  const double epsc = 0.25 * (p.rat * p.rat - 1.0);
+  double tp[5] = {0, 0, 0, 0, 0};
+  double rp = 0.0;
+  for (int a = 1; a <= L; a++) {
+    const int i = DIR == 1 ? a : t1, j = DIR == 2 ? a : (DIR == 1 ? t1 : t2), k = DIR == 3 ? a : t2;
+    const ll c3 = i3(b, i, j, k), c1 = i1(b, i, j, k);
+    const double eps = p.irsop == 1 ? epsc : bb[c1];
+    const double ca = a == 1 ? 0.0 : -eps;
+    const double cb = 1 + 2 * eps;
+    const double cc = a == L ? 0.0 : -eps;
+    const double dtv = first ? dtb[c1] : 1.0;
+    double x[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) x[v] = first ? rb[c3 + n3 * v] * dtv : rb[c3 + n3 * v];
+    double r;
+    if (a == 1) {
+      r = cc / cb;
+#pragma unroll
+      for (int v = 0; v < 5; v++) tp[v] = x[v] / cb;
+    } else {
+      r = cc / (cb - rp * ca);
+#pragma unroll
+      for (int v = 0; v < 5; v++) tp[v] = (x[v] - tp[v] * ca) / (cb - rp * ca);
+    }
+    rp = r;
+    rs[c3] = r;
+#pragma unroll
+    for (int v = 0; v < 5; v++) rb[c3 + n3 * v] = tp[v];
+  }
+  double xn[5];
+  for (int a = L; a >= 1; a--) {
+    const int i = DIR == 1 ? a : t1, j = DIR == 2 ? a : (DIR == 1 ? t1 : t2), k = DIR == 3 ? a : t2;
+    const ll c3 = i3(b, i, j, k);
+    double xv[5];
+    if (a == L) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) xv[v] = rb[c3 + n3 * v];
+    } else {
+      const double r = rs[c3];
+#pragma unroll
+      for (int v = 0; v < 5; v++) xv[v] = rb[c3 + n3 * v] - xn[v] * r;
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) xn[v] = xv[v];
+    if (last) {
+      const double dtv = dtb[i1(b, i, j, k)];
+#pragma unroll
+      for (int v = 0; v < 5; v++) rb[c3 + n3 * v] = xv[v] / dtv;
+    } else {
+#pragma unroll
+      for (int v = 0; v < 5; v++) rb[c3 + n3 * v] = xv[v];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// launchers
+
+static inline dim3 grid1(int n) { return dim3(n > 0 ? n : 1); }
+
+hipError_t launch_lambda_deltat(Solve& s, Level& L) {
+  if (!L.ntile[TK_CELLS]) return hipSuccess;
+  hipLaunchKernelGGL(k_lambda_deltat, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                     L.tile[TK_CELLS], L.f[COSAHIP_Q], L.f[COSAHIP_SI], L.f[COSAHIP_SJ], L.f[COSAHIP_SK],
+                     L.f[COSAHIP_XDOT], L.f[COSAHIP_YDOT], L.f[COSAHIP_ZDOT], L.f[COSAHIP_DTVOL],
+                     L.f[COSAHIP_BETAXI], L.f[COSAHIP_BETAETA], L.f[COSAHIP_BETAZETA]);
+  return hipGetLastError();
+}
+
+hipError_t launch_residual(Solve& s, Level& L, const double* q, double* res) {
+  hipError_t e = launch_zero_res_shell(s, L, res);
+  if (e != hipSuccess) return e;
+  if (L.ntile[TK_ROWS_I])
+    hipLaunchKernelGGL(k_flux_i, grid1(L.ntile[TK_ROWS_I]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
+                       L.tile[TK_ROWS_I], q, L.f[COSAHIP_SI], res, L.sbc, L.sbc_doff);
+  if (L.ntile[TK_LINES_J])
+    hipLaunchKernelGGL(k_flux_march<2>, grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.nl, L.dblk,
+                       L.tile[TK_LINES_J], q, L.f[COSAHIP_SJ], res, L.sbc, L.sbc_doff);
+  if (L.ntile[TK_LINES_K])
+    hipLaunchKernelGGL(k_flux_march<3>, grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.nl, L.dblk,
+                       L.tile[TK_LINES_K], q, L.f[COSAHIP_SK], res, L.sbc, L.sbc_doff);
+  return hipGetLastError();
+}
+
+hipError_t launch_cirs(Solve& s, Level& L) {
+  if (s.par.irsop < 1 || s.par.irsop > 3) return hipSuccess;
+  double* res = L.f[COSAHIP_RES];
+  const double* dt = L.f[COSAHIP_DTVOL];
+  if (L.ntile[TK_LINES_I])
+    hipLaunchKernelGGL(k_cirs<1>, grid1(L.ntile[TK_LINES_I]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                       L.tile[TK_LINES_I], res, dt, L.f[COSAHIP_BETAXI], L.cirs_r, 1, 0);
+  if (L.ntile[TK_LINES_J])
+    hipLaunchKernelGGL(k_cirs<2>, grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                       L.tile[TK_LINES_J], res, dt, L.f[COSAHIP_BETAETA], L.cirs_r, 0, 0);
+  if (L.ntile[TK_LINES_K])
+    hipLaunchKernelGGL(k_cirs<3>, grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                       L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, 0, 1);
+  return hipGetLastError();
+}
+
+}  // namespace cosahip

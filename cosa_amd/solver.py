@@ -1,0 +1,191 @@
+"""Host driver of a libcosahip solve, mirroring COSA's own driver.
+
+``Solve`` owns one device context: it defines every multigrid level of a
+:class:`cosa_amd.case.Case`, uploads the geometric inputs and the state, and
+runs the reference's multigrid cycle ``mg`` (src/cosa.f90:806-930) over the
+C ABI: ``smooth`` on the fine level, ``restrict``, coarse smoothing,
+``prolong``.  Method names follow the reference routines.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import _lib
+from .case import Case
+from .config import to_struct
+
+
+class Solve:
+    def __init__(self, case: Case, device: int = 0, q: Optional[np.ndarray] = None):
+        self.case = case
+        self.lib = _lib.load()
+        self._cfg = to_struct(case.config)
+        h = ctypes.c_void_p()
+        rc = self.lib.cosahip_create(ctypes.byref(self._cfg), device, ctypes.byref(h))
+        if rc:
+            raise _lib.CosahipError(rc, "cosahip_create failed (see stderr)")
+        self.h = h
+        self.history: List[np.ndarray] = []
+        try:
+            self._define()
+            if q is not None:
+                self.upload("q", q, 1)
+        except Exception:
+            self.close()
+            raise
+
+    # -- plumbing ------------------------------------------------------------
+    def _check(self, rc: int, what: str):
+        if rc:
+            msg = self.lib.cosahip_last_error(self.h).decode()
+            info = (ctypes.c_int32 * 8)()
+            self.lib.cosahip_error_info(self.h, info)
+            raise _lib.CosahipError(rc, f"{what}: {msg}", list(info))
+
+    def _define(self):
+        c = self.case
+        for L in c.levels:
+            keep = [np.ascontiguousarray(getattr(L, n), np.int32) for n in
+                    ("imax", "jmax", "kmax", "nbcs", "bctopo", "cutopo")]
+            cut = keep[5] if keep[5].size else np.zeros(21, np.int32)
+            self._check(self.lib.cosahip_define_level(self.h, L.nl, L.nblocks, _lib.iptr(keep[0]),
+                                                       _lib.iptr(keep[1]), _lib.iptr(keep[2]), _lib.iptr(keep[3]),
+                                                       _lib.iptr(keep[4]), L.ncuts, _lib.iptr(cut)),
+                        f"define_level {L.nl}")
+            g = c.geom[L.nl - 1]
+            for f in ("si", "sj", "sk", "vol"):
+                self.upload(f, g[f], L.nl)
+            self.upload("q", c.freestream(L.nl), L.nl)
+        d = c.hb_matrix()
+        if d is not None:
+            dd = np.ascontiguousarray(d.flatten(order="F"))
+            self._check(self.lib.cosahip_set_hb_operator(self.h, d.shape[0], _lib.dptr(dd)), "set_hb_operator")
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.cosahip_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def size(self, field: str, nl: int = 1) -> int:
+        return int(self.lib.cosahip_field_size(self.h, nl, _lib.FIELD_IDS[field]))
+
+    def upload(self, field: str, arr: np.ndarray, nl: int = 1):
+        a = np.ascontiguousarray(arr, np.float64)
+        self._check(self.lib.cosahip_upload(self.h, nl, _lib.FIELD_IDS[field], _lib.dptr(a), a.size),
+                    f"upload {field}")
+
+    def download(self, field: str, nl: int = 1) -> np.ndarray:
+        n = self.size(field, nl)
+        a = np.empty(n)
+        self._check(self.lib.cosahip_download(self.h, nl, _lib.FIELD_IDS[field], _lib.dptr(a), n),
+                    f"download {field}")
+        return a
+
+    def synchronize(self):
+        self._check(self.lib.cosahip_synchronize(self.h), "synchronize")
+
+    # -- reference routines ----------------------------------------------------
+    def call(self, name: str, *args):
+        self._check(getattr(self.lib, "cosahip_" + name)(self.h, *args), name)
+
+    def resg_rms(self, nl: int = 1) -> np.ndarray:
+        r = np.zeros(8)
+        self._check(self.lib.cosahip_resg_rms(self.h, nl, _lib.dptr(r)), "resg_rms")
+        return r
+
+    def smooth(self, nl: int, from_fine: bool, to_coarse: bool, niter: int, mgit: int):
+        done = (ctypes.c_int32 * 1)()
+        r = np.zeros(8)
+        self._check(self.lib.cosahip_smooth(self.h, nl, int(from_fine), int(to_coarse), niter, mgit, done,
+                                            _lib.dptr(r)), f"smooth level {nl}")
+        return bool(done[0]), r
+
+    def restrict(self, nl: int):
+        self._check(self.lib.cosahip_restrict(self.h, nl), "restrict")
+
+    def prolong(self, nl: int):
+        self._check(self.lib.cosahip_prolong(self.h, nl), "prolong")
+
+    # -- multigrid driver ------------------------------------------------------
+    def cycle(self, mgit: int, nstart: int = 1, npre: int = 1, npost: int = 1, ncrs: int = 1) -> bool:
+        """One pass of mg's cycle loop (src/cosa.f90:822-912): fine smooth,
+        then the coarse levels of the V and the prolongations.  Appends the
+        fine-level resrms(1:8) to ``history``; returns the convergence flag."""
+        nlevel = self.case.config["nlevel"]
+        niter = 1 if nlevel == 1 else (nstart if mgit == 1 else npost)
+        done, r = self.smooth(1, False, nlevel > 1, niter, mgit)
+        self.history.append(r)
+        if done or nlevel == 1:
+            return done
+        self.restrict(1)
+        niter = ncrs if nlevel == 2 else npre
+        self.smooth(2, True, nlevel > 2, niter, mgit)
+        if nlevel >= 3:
+            self.restrict(2)
+            self.smooth(3, True, False, ncrs, mgit)
+            self.prolong(2)
+            self.smooth(2, False, False, npost, mgit)
+        self.prolong(1)
+        return False
+
+    def mg(self, ncycle: Optional[int] = None, nstart: int = 1, npre: int = 1, npost: int = 1, ncrs: int = 1,
+           on_cycle=None) -> int:
+        """The V-cycle schedule of mg (src/cosa.f90:806-930).  Records
+        resrms(1:8) of every cycle in ``history``; returns cycles run."""
+        nlevel = self.case.config["nlevel"]
+        if nlevel > 3:
+            raise ValueError("the reference's V-cycle has at most 3 levels")
+        ncycle = ncycle or self.case.config["ncycle"]
+        for mgit in range(1, ncycle + 1):
+            done = self.cycle(mgit, nstart, npre, npost, ncrs)
+            if on_cycle:
+                on_cycle(mgit, self.history[-1])
+            if done:
+                return mgit
+        if nlevel > 1:
+            self.smooth(1, False, False, npost, ncycle)
+        return ncycle
+
+    # -- instrumentation -------------------------------------------------------
+    def timing(self, on: bool):
+        self._check(self.lib.cosahip_timing_enable(self.h, int(on)), "timing_enable")
+
+    def timing_reset(self):
+        self._check(self.lib.cosahip_timing_reset(self.h), "timing_reset")
+
+    def timing_read(self) -> Dict[str, Dict[str, float]]:
+        out = {}
+        for f, name in enumerate(_lib.FAMILIES):
+            ms = ctypes.c_double()
+            n = ctypes.c_int64()
+            it = ctypes.c_int64()
+            self._check(self.lib.cosahip_timing_read(self.h, f, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(it)),
+                        "timing_read")
+            out[name] = {"ms": ms.value, "launches": n.value, "items": it.value}
+        return out
+
+    def cell_updates_per_cycle(self, nstart: int = 1, npre: int = 1, npost: int = 1, ncrs: int = 1,
+                               first: bool = False) -> int:
+        """Interior (cell, snapshot) RK-stage updates of one mg cycle, every
+        level counted (the residual-only pass before restriction excluded)."""
+        c = self.case
+        ns = c.nsnap
+        st = c.config["nstage"]
+        nlevel = c.config["nlevel"]
+        n1 = c.ncell(1) * ns * st
+        tot = n1 * (nstart if first else (npost if nlevel > 1 else 1))
+        if nlevel >= 2:
+            n2 = c.ncell(2) * ns * st
+            tot += n2 * (ncrs if nlevel == 2 else npre)
+        if nlevel >= 3:
+            tot += c.ncell(3) * ns * st * ncrs + c.ncell(2) * ns * st * npost
+        return tot

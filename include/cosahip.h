@@ -193,8 +193,27 @@ int cosahip_restrict(cosahip_solve* s, int nl);
 /* prolong (src/cosa.f90:3167-3249) correction from nl+1 onto nl.         */
 int cosahip_prolong(cosahip_solve* s, int nl);
 
-/* Fraction of the last cosahip_smooth's device time spent per kernel family
- * is not tracked here; use rocprofv3.  Library build string:               */
+/* ---- instrumentation (no counterpart in the reference, whose only timers
+ * are whole-run wall clocks, src/cosa.f90:789-797) ----------------------
+ * When enabled, every kernel launch of a family is bracketed by HIP events
+ * recorded on the solve's own stream; cosahip_timing_read synchronises and
+ * returns the family's summed device time, launch count and work items
+ * (interior cell-snapshots it processed).  Disabled by default.          */
+enum {
+  COSAHIP_FAM_RESIDUAL = 0,  /* MUSCL + Roe + flux difference, 3 directions */
+  COSAHIP_FAM_CIRS = 1,      /* implicit residual smoothing, 3 sweeps       */
+  COSAHIP_FAM_UPDATE = 2,    /* RK update                                   */
+  COSAHIP_FAM_HALO = 3,      /* BC patches and cut copies                   */
+  COSAHIP_FAM_CONVERT = 4,   /* cons2prim / prim2cons                       */
+  COSAHIP_FAM_LAMBDA = 5,    /* spectral radii + local time step            */
+  COSAHIP_FAM_OTHER = 6,     /* copies, FAS, norms, HB, MG transfer, checks */
+  COSAHIP_NFAM = 7
+};
+int cosahip_timing_enable(cosahip_solve* s, int on);
+int cosahip_timing_read(cosahip_solve* s, int family, double* total_ms, int64_t* launches, int64_t* items);
+int cosahip_timing_reset(cosahip_solve* s);
+
+/* Library build string. */
 const char* cosahip_build_info(void);
 
 #ifdef __cplusplus

@@ -1,0 +1,68 @@
+"""Shared helpers of the GPU parity tests: build a case on both the oracle
+and libcosahip from the same inputs and compare with the FP64 bar
+|gpu - cpu| <= rtol * (|cpu| + floor * max|cpu|) per value."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from cosa_amd import case as cases
+from oracle_bind import Oracle, OrCase, dp
+
+RTOL = 1e-12     # per-value relative bar (FP64 GPU mode)
+FLOOR = 1e-14    # absolute floor, relative to the field's max magnitude
+
+
+def parity(gpu: np.ndarray, cpu: np.ndarray, rtol: float = RTOL, floor: float = FLOOR, what: str = "") -> float:
+    """Assert the FP64 parity bar; return the worst normalised error."""
+    assert gpu.shape == cpu.shape, what
+    scale = float(np.max(np.abs(cpu))) if cpu.size else 0.0
+    err = np.abs(gpu - cpu)
+    tol = rtol * (np.abs(cpu) + floor * scale)
+    bad = err > tol
+    if bad.any():
+        i = int(np.argmax(err - tol))
+        raise AssertionError(f"{what}: {int(bad.sum())} values outside the bar; worst at {i}: "
+                             f"gpu={gpu[i]!r} cpu={cpu[i]!r} |d|={err[i]:.3e} tol={tol[i]:.3e}")
+    with np.errstate(invalid="ignore", divide="ignore"):
+        rel = np.where(tol > 0, err / np.maximum(tol, 1e-300), 0.0)
+    return float(rel.max()) if rel.size else 0.0
+
+
+def interior_mask(case, nl=1, ncomp=None):
+    """Boolean mask of interior cells over a whole-level P3 array."""
+    L = case.levels[nl - 1]
+    ncomp = ncomp or case.npde
+    parts = []
+    for b in range(L.nblocks):
+        im, jm, km = int(L.imax[b]), int(L.jmax[b]), int(L.kmax[b])
+        m = np.zeros((km + 3, jm + 3, im + 3), bool)
+        m[2:km + 1, 2:jm + 1, 2:im + 1] = True
+        parts.append(np.tile(m.reshape(-1), ncomp * case.nsnap))
+    return np.concatenate(parts)
+
+
+def make_pair(case, q0, device=0):
+    """An oracle case and a GPU solve, both holding state q0 on level 1."""
+    from cosa_amd.solver import Solve
+    o = Oracle()
+    o.set_config(case.config, case.hb_matrix())
+    oc = OrCase(case, q1=q0.copy())
+    g = Solve(case, device=device, q=q0)
+    return o, oc, g
+
+
+def small_cases():
+    """Parity cases: distorted two-block with a cut and four BC types;
+    NACA O-grid with a self-cut; HB on the two-block case."""
+    out = []
+    c = cases.two_block_case(ni=13, nj=9, nk=9, nlevel=1, distort=0.15, name="two_block", irsop=2,
+                             cfl_irs=4.0, cfl=2.0, ilimf=4, epslimf=0.01)
+    out.append(c)
+    c = cases.naca_case(33, 13, 5, nlevel=1, name="naca_small", irsop=1, cfl_irs=3.0, cfl=2.0)
+    out.append(c)
+    c = cases.two_block_case(ni=9, nj=9, nk=5, nlevel=1, distort=0.1, name="two_block_hb2", irsop=3, psirs=0.125,
+                             cfl_irs=3.0, cfl=2.0, nharms=2, harbal=1, omega=0.5)
+    out.append(c)
+    return out
