@@ -1,6 +1,13 @@
 """Shared helpers of the GPU parity tests: build a case on both the oracle
 and libcosahip from the same inputs and compare with the FP64 bar
-|gpu - cpu| <= rtol * (|cpu| + floor * max|cpu|) per value."""
+
+    |gpu - cpu| <= 1e-12 |cpu| + 1e-14 max|cpu|      (per value)
+
+i.e. 1e-12 relative, with an absolute floor at 1e-14 of the field's largest
+magnitude: a residual value is a cancellation of fluxes of that size, and the
+device pow (farfield BC, CIRS coefficients) rounds differently from glibc's
+in the last bits.  Every comparison also reports how many values are
+bit-identical."""
 from __future__ import annotations
 
 import ctypes
@@ -11,15 +18,21 @@ from cosa_amd import case as cases
 from oracle_bind import Oracle, OrCase, dp
 
 RTOL = 1e-12     # per-value relative bar (FP64 GPU mode)
-FLOOR = 1e-14    # absolute floor, relative to the field's max magnitude
+FLOOR = 1e-14    # absolute floor as a fraction of the field's max magnitude
 
 
 def parity(gpu: np.ndarray, cpu: np.ndarray, rtol: float = RTOL, floor: float = FLOOR, what: str = "") -> float:
     """Assert the FP64 parity bar; return the worst normalised error."""
     assert gpu.shape == cpu.shape, what
+    ndiff = int(np.count_nonzero(gpu != cpu))
+    if ndiff:
+        ulp = np.abs(gpu.view(np.int64) - cpu.view(np.int64))
+        print(f"  [{what}] {ndiff}/{gpu.size} values differ, max {int(ulp.max())} ulp", flush=True)
+    else:
+        print(f"  [{what}] bit-identical ({gpu.size} values)", flush=True)
     scale = float(np.max(np.abs(cpu))) if cpu.size else 0.0
     err = np.abs(gpu - cpu)
-    tol = rtol * (np.abs(cpu) + floor * scale)
+    tol = rtol * np.abs(cpu) + floor * scale
     bad = err > tol
     if bad.any():
         i = int(np.argmax(err - tol))
