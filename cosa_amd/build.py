@@ -63,5 +63,21 @@ def _wait(procs):
     procs.clear()
 
 
+def build_fortran(verbose: bool = False) -> str:
+    """Compile the iso_c_binding module and the Fortran demo driver against
+    libcosahip.so (amdflang); returns the demo executable path."""
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    fdir = os.path.join(HERE, "fortran")
+    bdir = os.path.join(HERE, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    exe = os.path.join(fdir, "binding_demo")
+    cmd = [fc, "-O2", "-module-dir", bdir, "-I", bdir, os.path.join(fdir, "cosahip_binding.f90"),
+           os.path.join(fdir, "binding_demo.f90"), "-L", HERE, "-lcosahip", "-Wl,-rpath,$ORIGIN/..", "-o", exe]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.check_call(cmd)
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

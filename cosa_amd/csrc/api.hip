@@ -5,6 +5,7 @@
 // (3167-3249).
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -167,6 +168,8 @@ int cosahip_create(const cosahip_config* cfg, int device, cosahip_solve** out) {
     return rc;
   }
   s->par = make_par(*cfg);
+  const char* rk = getenv("COSAHIP_RESIDUAL");
+  s->split_residual = rk && strcmp(rk, "split") == 0;
   if (device >= 0) {
     if (hipSetDevice(device) != hipSuccess) {
       delete s;
@@ -252,9 +255,10 @@ static int build_tables(Solve* s, Level& L, const Level* Lc) {
     add_tiles(L.htile[TK_ROWS_I], b, ns * jm * km * nseg, 4);
     add_tiles(L.htile[TK_LINES_J], b, ns * im * km, TS);
     add_tiles(L.htile[TK_LINES_K], b, ns * im * jm, TS);
-    add_tiles(L.htile[TK_LINES_I], b, ns * jm * km, TS);
+    add_tiles(L.htile[TK_LINES_I], b, ns * jm * km, 64);
     add_tiles(L.htile[TK_CELLS_P1], b, ns * (ll)(B.imax + 1) * (B.jmax + 1) * (B.kmax + 1), TS);
     add_tiles(L.htile[TK_HALO], b, ns * ((ll)(B.imax + 3) * jm * km + 4 * im * km + 4 * im * jm), TS);
+    add_tiles(L.htile[TK_FUSED], b, ns * ((im + 30) / 31) * ((jm + 6) / 7) * ((km + 31) / 32), 1);
     if (Lc) {
       const Blk& C = Lc->hblk[b];
       add_tiles(L.htile[TK_COARSE], b, ns * (ll)(C.imax - 1) * (C.jmax - 1) * (C.kmax - 1), TS);

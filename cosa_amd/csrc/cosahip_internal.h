@@ -69,11 +69,12 @@ enum TileKind {
   TK_ROWS_I,      // i-direction flux rows: (jmax-1)(kmax-1) rows x segments
   TK_LINES_J,     // (imax-1)(kmax-1) lines along j
   TK_LINES_K,     // (imax-1)(jmax-1) lines along k
-  TK_LINES_I,     // (jmax-1)(kmax-1) lines along i
+  TK_LINES_I,     // (jmax-1)(kmax-1) lines along i, 64 per tile
   TK_CELLS_P1,    // 0..imax x 0..jmax x 0..kmax
   TK_COARSE,      // cells of the next coarser level, per block (restriction)
   TK_PROL,        // 2x2x2 prolongation groups 0..imax-1 step 2, per block
   TK_HALO,        // prtest region: interior + face halo strips
+  TK_FUSED,       // fused residual: (31 x 7)-cell (i,j) tiles x 32-plane k segments
   TK_NKIND
 };
 
@@ -136,6 +137,9 @@ struct Solve {
   double resrms[9] = {};                 // resrms(1:8), index 0 unused
   std::string err;
   int32_t errinfo[8] = {};
+  // residual kernel choice: fused single pass (default) or one pass per
+  // direction (COSAHIP_RESIDUAL=split in the environment at create time)
+  bool split_residual = false;
   // instrumentation
   bool timing = false;
   std::vector<TimedLaunch> pending;

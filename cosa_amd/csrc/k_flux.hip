@@ -312,6 +312,175 @@ __global__ void __launch_bounds__(256) k_flux_march(Par p, int nl, const Blk* __
 }
 
 // ------------------------------------------------------------------------
+// Fused residual, all three directions in one pass (2.5-D blocking).
+// A workgroup of 32 x 8 threads owns FTI x FTJ = 31 x 7 cells of an (i,j)
+// tile and marches k through a segment of up to KSEG planes.  Per plane:
+//   - the plane (primitive q, i0-2..i0+FTI+1 x j0-2..j0+FTJ+1, no corners)
+//     is staged in LDS: cell threads deposit their own value from their
+//     k-window, the halo ring is loaded from global memory;
+//   - lane (tx,ty) computes i-face i0+tx of row j0+ty and j-face j0+ty of
+//     column i0+tx from LDS, and cell threads compute k-face k+1 from their
+//     register window q(k-1..k+2) (face k is kept from the previous plane);
+//   - res(c) = ((((0 + Fi(c+1)) - Fi(c)) + Fj(c+1)) - Fj(c)) + Fk(c+1)) - Fk(c),
+//     the reference's accumulation order (zero, then resid for idir = 1,2,3).
+// Every face flux is computed from the same operands as in the split
+// kernels, so the results are identical.
+constexpr int FTI = 31, FTJ = 7, KSEG = 32;
+constexpr int PW = FTI + 4, PH = FTJ + 4;
+
+__global__ void __launch_bounds__(256) k_residual_fused(Par p, int nl, const Blk* __restrict__ blks,
+                                                        const Tile* __restrict__ tiles, const double* __restrict__ q,
+                                                        const double* __restrict__ si, const double* __restrict__ sj,
+                                                        const double* __restrict__ sk, double* __restrict__ res,
+                                                        const unsigned char* __restrict__ sbc,
+                                                        const ll* __restrict__ sbcoff) {
+  __shared__ double pl[5][PH][PW];
+  __shared__ double fi[5][FTJ][FTI + 1];
+  __shared__ double fj[5][FTJ + 1][FTI];
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
+  const int ntx = (im + FTI - 1) / FTI, nty = (jm + FTJ - 1) / FTJ, nks = (km + KSEG - 1) / KSEG;
+  int item = t.base;
+  const int n = item / (ntx * nty * nks);
+  item -= n * (ntx * nty * nks);
+  const int ks_i = item / (ntx * nty);
+  item -= ks_i * (ntx * nty);
+  const int ty_i = item / ntx, tx_i = item - (item / ntx) * ntx;
+  const int i0 = 1 + tx_i * FTI, j0 = 1 + ty_i * FTJ;
+  const int ks = 1 + ks_i * KSEG, ke = min(km, ks + KSEG - 1);
+  const ll n3 = n3of(b);
+  const double* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  const int nh = n * p.hbmove;
+  const ll mo = b.o2 * p.lmet * p.nsnap_met + (ll)nh * n2of(b) * p.lmet;
+  const unsigned char* fl = sbc + sbcoff[t.block];
+  const ll plI = (ll)(b.jmax + 1) * (b.kmax + 1);                  // i-face flag planes
+  const ll offJ = 2 * plI, plJ = (ll)(b.imax + 1) * (b.kmax + 1);  // j-face flag planes
+  const ll offK = offJ + 2 * plJ, plK = (ll)(b.imax + 1) * (b.jmax + 1);
+  const int ci = i0 + tx, cj = j0 + ty;
+  const bool cell = tx < FTI && ty < FTJ && ci <= im && cj <= jm;
+  const ll kstep = (ll)(b.imax + 3) * (b.jmax + 3);
+  // k window of this thread's column: w0..w3 = q(k-1..k+2) at plane k
+  double w0[5], w1[5], w2[5], w3[5], Fkp[5];
+  bool kfin = false, kfout = false;
+  if (cell) {
+    kfin = fl[offK + ci + (ll)(b.imax + 1) * cj] != 0;
+    kfout = fl[offK + plK + ci + (ll)(b.imax + 1) * cj] != 0;
+    const ll c0 = i3(b, ci, cj, ks);
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      const double* qv = qb + n3 * v;
+      const double a_m2 = qv[c0 - 2 * kstep], a_m1 = qv[c0 - kstep], a_0 = qv[c0], a_p1 = qv[c0 + kstep];
+      w0[v] = a_m2;   // temporarily q(ks-2..ks+1) for face ks
+      w1[v] = a_m1;
+      w2[v] = a_0;
+      w3[v] = a_p1;
+    }
+    double QR[5], QL[5], kx, ky, kz, ds, ubn;
+    muscl_face(p, nl, ks, b.kmax, kfin, kfout, w0, w1, w2, w3, QR, QL);
+    face_metric(p, sk + mo + i2(b, ci, cj, ks) * p.lmet, kx, ky, kz, ds, ubn);
+    roe_face(p, QR, QL, kx, ky, kz, ds, ubn, Fkp);
+    // shift to the window of plane ks: q(ks-1..ks+2)
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      w0[v] = w1[v];
+      w1[v] = w2[v];
+      w2[v] = w3[v];
+      w3[v] = qb[n3 * v + c0 + 2 * kstep];
+    }
+  }
+  // per-row / per-column strong-BC flags (constant over k only for the face
+  // planes they index by k, so looked up per plane below)
+  for (int k = ks; k <= ke; k++) {
+    // (a) stage plane k: halo ring from memory, cells from registers
+    for (int pidx = threadIdx.x; pidx < PW * PH; pidx += 256) {
+      const int ii = pidx % PW, jj = pidx / PW;
+      const bool in_i = ii >= 2 && ii <= FTI + 1, in_j = jj >= 2 && jj <= FTJ + 1;
+      if (!in_i && !in_j) continue;   // corner: not used
+      const int i = i0 - 2 + ii, j = j0 - 2 + jj;
+      if (in_i && in_j && i <= im && j <= jm) continue;   // deposited by its cell thread
+      if (i > b.imax + 1 || j > b.jmax + 1) continue;     // beyond the block
+      const ll x = i3(b, i, j, k);
+#pragma unroll
+      for (int v = 0; v < 5; v++) pl[v][jj][ii] = qb[n3 * v + x];
+    }
+    if (cell) {
+#pragma unroll
+      for (int v = 0; v < 5; v++) pl[v][ty + 2][tx + 2] = w1[v];
+    }
+    __syncthreads();
+    // (b) face fluxes
+    if (ty < FTJ && cj <= jm && ci <= b.imax) {   // i-face a = ci of row cj
+      const int a = ci;
+      const bool fin = fl[cj + (ll)(b.jmax + 1) * k] != 0;
+      const bool fout = fl[plI + cj + (ll)(b.jmax + 1) * k] != 0;
+      double m2[5], m1[5], z0[5], p1[5], QR[5], QL[5], F[5], kx, ky, kz, ds, ubn;
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        m2[v] = pl[v][ty + 2][tx];
+        m1[v] = pl[v][ty + 2][tx + 1];
+        z0[v] = pl[v][ty + 2][tx + 2];
+        p1[v] = pl[v][ty + 2][tx + 3];
+      }
+      muscl_face(p, nl, a, b.imax, fin, fout, m2, m1, z0, p1, QR, QL);
+      face_metric(p, si + mo + i2(b, a, cj, k) * p.lmet, kx, ky, kz, ds, ubn);
+      roe_face(p, QR, QL, kx, ky, kz, ds, ubn, F);
+#pragma unroll
+      for (int v = 0; v < 5; v++) fi[v][ty][tx] = F[v];
+    }
+    if (tx < FTI && ci <= im && cj <= b.jmax) {   // j-face a = cj of column ci
+      const int a = cj;
+      const bool fin = fl[offJ + ci + (ll)(b.imax + 1) * k] != 0;
+      const bool fout = fl[offJ + plJ + ci + (ll)(b.imax + 1) * k] != 0;
+      double m2[5], m1[5], z0[5], p1[5], QR[5], QL[5], F[5], kx, ky, kz, ds, ubn;
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        m2[v] = pl[v][ty][tx + 2];
+        m1[v] = pl[v][ty + 1][tx + 2];
+        z0[v] = pl[v][ty + 2][tx + 2];
+        p1[v] = pl[v][ty + 3][tx + 2];
+      }
+      muscl_face(p, nl, a, b.jmax, fin, fout, m2, m1, z0, p1, QR, QL);
+      face_metric(p, sj + mo + i2(b, ci, a, k) * p.lmet, kx, ky, kz, ds, ubn);
+      roe_face(p, QR, QL, kx, ky, kz, ds, ubn, F);
+#pragma unroll
+      for (int v = 0; v < 5; v++) fj[v][ty][tx] = F[v];
+    }
+    double Fkn[5];
+    if (cell) {   // k-face k+1 from the register window
+      double QR[5], QL[5], kx, ky, kz, ds, ubn;
+      muscl_face(p, nl, k + 1, b.kmax, kfin, kfout, w0, w1, w2, w3, QR, QL);
+      face_metric(p, sk + mo + i2(b, ci, cj, k + 1) * p.lmet, kx, ky, kz, ds, ubn);
+      roe_face(p, QR, QL, kx, ky, kz, ds, ubn, Fkn);
+    }
+    __syncthreads();
+    // (c) accumulate in the reference order and store
+    if (cell) {
+      const ll x = i3(b, ci, cj, k);
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        double r = (0.0 + fi[v][ty][tx + 1]) - fi[v][ty][tx];
+        r = (r + fj[v][ty + 1][tx]) - fj[v][ty][tx];
+        r = (r + Fkn[v]) - Fkp[v];
+        rb[n3 * v + x] = r;
+        Fkp[v] = Fkn[v];
+      }
+      if (k < ke) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+          w0[v] = w1[v];
+          w1[v] = w2[v];
+          w2[v] = w3[v];
+          w3[v] = qb[n3 * v + x + 3 * kstep];
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
 // lambda (inviscid eig_eul) + deltat, one thread per (cell, snapshot)
 __global__ void __launch_bounds__(256) k_lambda_deltat(Par p, const Blk* __restrict__ blks,
                                                        const Tile* __restrict__ tiles, const double* __restrict__ q,
@@ -475,6 +644,154 @@ __global__ void __launch_bounds__(256) k_cirs(Par p, const Blk* __restrict__ blk
   }
 }
 
+// CIRS along i, where each line is contiguous in memory: one wavefront per
+// 64 lines.  Chunks of CI cells of every line (five variables, dtvol and the
+// coefficients) are staged through LDS with coalesced loads and stores; the
+// per-line Thomas recurrence then runs from LDS with the same arithmetic as
+// k_cirs (forward pass r, t; backward pass x).  This is always the first
+// sweep: res is scaled by dtvol on load.
+constexpr int CI = 8;     // cells per chunk
+constexpr int LPAD = 65;  // padded line stride in LDS (conflict-free lane access)
+
+__global__ void __launch_bounds__(64) k_cirs_i(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
+                                               double* __restrict__ res, const double* __restrict__ dtvol,
+                                               const double* __restrict__ beta, double* __restrict__ rsc) {
+  __shared__ double xs[5][CI][LPAD];
+  __shared__ double rsh[CI][LPAD];
+  __shared__ double dsh[CI][LPAD];
+  __shared__ double bsh[CI][LPAD];
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int lane = threadIdx.x;
+  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
+  const int nlines = jm * km;
+  const int total = nlines * p.nsnap;
+  const int nl_here = min(64, total - t.base);
+  const int L = im;
+  const ll n3 = n3of(b), n1 = n1of(b);
+  const double epsc = 0.25 * (p.rat * p.rat - 1.0);
+  const bool cb_const = p.irsop == 1;
+  // load slots: position ii = lane % CI of the chunk for lines lane/CI + 8 r
+  const int ii_ld = lane % CI;
+  ll off3[8], offr[8], off1[8];
+  bool ok[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    const int ln = lane / CI + 8 * r;
+    ok[r] = ln < nl_here;
+    const int gl = t.base + (ok[r] ? ln : 0);
+    const int n = gl / nlines, rem = gl - (gl / nlines) * nlines;
+    const int j = rem % jm + 1, k = rem / jm + 1;
+    off3[r] = b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n + i3(b, 0, j, k);
+    offr[r] = b.o3 * p.nsnap + n3 * (ll)n + i3(b, 0, j, k);
+    off1[r] = b.o1 * p.nsnap + n1 * (ll)n + i1(b, 0, j, k);
+  }
+  const bool active = lane < nl_here;
+  double tp[5] = {0, 0, 0, 0, 0};
+  double rp = 0.0;
+  const int nchunk = (L + CI - 1) / CI;
+  // forward elimination
+  for (int c = 0; c < nchunk; c++) {
+    const int i0 = 1 + c * CI;
+    const int cnt = min(CI, L - i0 + 1);
+    if (ii_ld < cnt) {
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        if (!ok[r]) continue;
+        const int ln = lane / CI + 8 * r;
+        const int i = i0 + ii_ld;
+#pragma unroll
+        for (int v = 0; v < 5; v++) xs[v][ii_ld][ln] = res[off3[r] + n3 * v + i];
+        dsh[ii_ld][ln] = dtvol[off1[r] + i];
+        if (!cb_const) bsh[ii_ld][ln] = beta[off1[r] + i];
+      }
+    }
+    __syncthreads();
+    if (active) {
+      for (int ii = 0; ii < cnt; ii++) {
+        const int a = i0 + ii;
+        const double eps = cb_const ? epsc : bsh[ii][lane];
+        const double ca = a == 1 ? 0.0 : -eps;
+        const double cb = 1 + 2 * eps;
+        const double cc = a == L ? 0.0 : -eps;
+        const double dtv = dsh[ii][lane];
+        double r;
+        if (a == 1) {
+          r = cc / cb;
+#pragma unroll
+          for (int v = 0; v < 5; v++) tp[v] = (xs[v][ii][lane] * dtv) / cb;
+        } else {
+          r = cc / (cb - rp * ca);
+#pragma unroll
+          for (int v = 0; v < 5; v++) tp[v] = (xs[v][ii][lane] * dtv - tp[v] * ca) / (cb - rp * ca);
+        }
+        rp = r;
+        rsh[ii][lane] = r;
+#pragma unroll
+        for (int v = 0; v < 5; v++) xs[v][ii][lane] = tp[v];
+      }
+    }
+    __syncthreads();
+    if (ii_ld < cnt) {
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        if (!ok[r]) continue;
+        const int ln = lane / CI + 8 * r;
+        const int i = i0 + ii_ld;
+#pragma unroll
+        for (int v = 0; v < 5; v++) res[off3[r] + n3 * v + i] = xs[v][ii_ld][ln];
+        rsc[offr[r] + i] = rsh[ii_ld][ln];
+      }
+    }
+    __syncthreads();
+  }
+  // back substitution
+  double xn[5] = {0, 0, 0, 0, 0};
+  for (int c = nchunk - 1; c >= 0; c--) {
+    const int i0 = 1 + c * CI;
+    const int cnt = min(CI, L - i0 + 1);
+    if (ii_ld < cnt) {
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        if (!ok[r]) continue;
+        const int ln = lane / CI + 8 * r;
+        const int i = i0 + ii_ld;
+#pragma unroll
+        for (int v = 0; v < 5; v++) xs[v][ii_ld][ln] = res[off3[r] + n3 * v + i];
+        rsh[ii_ld][ln] = rsc[offr[r] + i];
+      }
+    }
+    __syncthreads();
+    if (active) {
+      for (int ii = cnt - 1; ii >= 0; ii--) {
+        const int a = i0 + ii;
+        if (a == L) {
+#pragma unroll
+          for (int v = 0; v < 5; v++) xn[v] = xs[v][ii][lane];
+        } else {
+          const double r = rsh[ii][lane];
+#pragma unroll
+          for (int v = 0; v < 5; v++) xn[v] = xs[v][ii][lane] - xn[v] * r;
+        }
+#pragma unroll
+        for (int v = 0; v < 5; v++) xs[v][ii][lane] = xn[v];
+      }
+    }
+    __syncthreads();
+    if (ii_ld < cnt) {
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        if (!ok[r]) continue;
+        const int ln = lane / CI + 8 * r;
+        const int i = i0 + ii_ld;
+#pragma unroll
+        for (int v = 0; v < 5; v++) res[off3[r] + n3 * v + i] = xs[v][ii_ld][ln];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------------
 // launchers
 
@@ -492,6 +809,13 @@ hipError_t launch_lambda_deltat(Solve& s, Level& L) {
 hipError_t launch_residual(Solve& s, Level& L, const double* q, double* res) {
   hipError_t e = launch_zero_res_shell(s, L, res);
   if (e != hipSuccess) return e;
+  if (!s.split_residual) {
+    if (L.ntile[TK_FUSED])
+      hipLaunchKernelGGL(k_residual_fused, grid1(L.ntile[TK_FUSED]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
+                         L.tile[TK_FUSED], q, L.f[COSAHIP_SI], L.f[COSAHIP_SJ], L.f[COSAHIP_SK], res, L.sbc,
+                         L.sbc_doff);
+    return hipGetLastError();
+  }
   if (L.ntile[TK_ROWS_I])
     hipLaunchKernelGGL(k_flux_i, grid1(L.ntile[TK_ROWS_I]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
                        L.tile[TK_ROWS_I], q, L.f[COSAHIP_SI], res, L.sbc, L.sbc_doff);
@@ -509,8 +833,8 @@ hipError_t launch_cirs(Solve& s, Level& L) {
   double* res = L.f[COSAHIP_RES];
   const double* dt = L.f[COSAHIP_DTVOL];
   if (L.ntile[TK_LINES_I])
-    hipLaunchKernelGGL(k_cirs<1>, grid1(L.ntile[TK_LINES_I]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                       L.tile[TK_LINES_I], res, dt, L.f[COSAHIP_BETAXI], L.cirs_r, 1, 0);
+    hipLaunchKernelGGL(k_cirs_i, grid1(L.ntile[TK_LINES_I]), dim3(64), 0, s.stream, s.par, L.dblk,
+                       L.tile[TK_LINES_I], res, dt, L.f[COSAHIP_BETAXI], L.cirs_r);
   if (L.ntile[TK_LINES_J])
     hipLaunchKernelGGL(k_cirs<2>, grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
                        L.tile[TK_LINES_J], res, dt, L.f[COSAHIP_BETAETA], L.cirs_r, 0, 0);

@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Write tests/golden/ref_kernels.npz: inputs and outputs of COSA's own
+compiled routines (oracle/_ref/libcosaref.so, built from /root/reference by
+oracle/build_ref.py) on small seeded blocks.  test_oracle_golden.py replays
+the inputs through the C restatement and requires identical outputs, so the
+oracle stays pinned where the reference is not available.
+
+Usage: python3 tests/golden/make_golden.py
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from blockgen import all_faces_patches, bctopo_of, box_block, n3, prim_to_cons, rand_prim  # noqa: E402
+from cosa_amd.config import default_config, hb_operator  # noqa: E402
+from oracle_bind import Ref, dp, ip  # noqa: E402
+
+DIMS = (5, 5, 4)
+
+
+def cfg(**kw):
+    nh = kw.get("nharms", 0)
+    if nh:
+        kw.setdefault("omega", 0.7)
+        kw["harbal"] = 1
+    return default_config(**kw)
+
+
+def cases():
+    """(name, config, inputs) of every fixture; `run` below knows each kind."""
+    rng = np.random.default_rng(2024)
+    d = DIMS
+    si, sj, sk, vol = box_block(*d, seed=12)
+    out = []
+    q = prim_to_cons(rand_prim(rng, *d, nsnap=3), *d, nsnap=3)
+    out.append(("cons_prim", cfg(nharms=1), dict(q=q)))
+    for types, mach, al in (((30, 14, 30, 14, 80, 80), 0.5, 1.25), ((4, 30, 14, 80, 80, 4), 1.6, 10.0)):
+        qq = rand_prim(rng, *d)
+        if mach > 1:
+            qq.reshape(5, -1)[1] += 1.2
+        out.append((f"bbc_{types[0]}_{types[1]}", cfg(machfs=mach, alpha_deg=al),
+                    dict(q=qq, si=si, sj=sj, sk=sk, bt=bctopo_of(all_faces_patches(*d, types)))))
+    out.append(("lambda_deltat_irs2", cfg(irsop=2, cfl=3.0, rat=1.7), dict(q=rand_prim(rng, *d), si=si, sj=sj, sk=sk)))
+    for idir in (1, 2, 3):
+        for ilim in (2, 4):
+            out.append((f"muscl_d{idir}_l{ilim}", cfg(ilimf=ilim, epslimf=0.05),
+                        dict(q=rand_prim(rng, *d), bt=bctopo_of(all_faces_patches(*d, (14, 30, 14, 14, 80, 14))),
+                             idir=np.array([idir]))))
+    for idir in (1, 2, 3):
+        out.append((f"roflux_d{idir}", cfg(efx_setup_f=1, efx_all_f=1, cntrpy_f=0.3, entfx_f_w=(0, 1, 0)),
+                    dict(qr=rand_prim(rng, *d), ql=rand_prim(rng, *d), s=(si, sj, sk)[idir - 1],
+                         res=rng.uniform(-1, 1, n3(*d) * 5), idir=np.array([idir]))))
+    qh = prim_to_cons(rand_prim(rng, *d, nsnap=5), *d, nsnap=5)
+    out.append(("src_hb_n2", cfg(nharms=2, omega=0.314),
+                dict(q=qh, vol=rng.uniform(0.5, 1.5, (d[0] + 1) * (d[1] + 1) * (d[2] + 1)),
+                     res=rng.uniform(-1, 1, qh.size))))
+    n1 = (d[0] + 1) * (d[1] + 1) * (d[2] + 1)
+    for irs in (1, 2):
+        out.append((f"cirs_irs{irs}", cfg(irsop=irs, rat=1.8),
+                    dict(res=rng.uniform(-1, 1, n3(*d) * 5), dtv=rng.uniform(0.5, 2, n1),
+                         bx=rng.uniform(0, 1.5, n1), be=rng.uniform(0, 1.5, n1), bz=rng.uniform(0, 1.5, n1))))
+    f, c = (5, 5, 5), (3, 3, 3)
+    out.append(("restrict_ho", cfg(), dict(q1=rng.uniform(0.5, 1.5, n3(*f) * 5),
+                                           v1=rng.uniform(0.5, 1.5, (f[0] + 1) * (f[1] + 1) * (f[2] + 1)),
+                                           v2=rng.uniform(3.5, 9.5, (c[0] + 1) * (c[1] + 1) * (c[2] + 1)),
+                                           res1=rng.uniform(-1, 1, n3(*f) * 5))))
+    for bln in (0, 1):
+        out.append((f"prolong_bln{bln}", cfg(bln_prol=bln),
+                    dict(q1=rng.uniform(0.5, 1.5, n3(*f) * 5), q2=rng.uniform(-0.1, 0.1, n3(*c) * 5),
+                         v1=rng.uniform(0.5, 1.5, (f[0] + 1) * (f[1] + 1) * (f[2] + 1)))))
+    out.append(("edges_corners", cfg(), dict(q=rng.uniform(0.5, 1.5, n3(*d) * 5))))
+    return out
+
+
+def run(lib, kind, name, c, a, oracle):
+    """Apply the routine of fixture `name` with library `lib` (the reference
+    shim when oracle is False, the C restatement when True)."""
+    d = DIMS
+    f, cc = (5, 5, 5), (3, 3, 3)
+    a = {k: np.ascontiguousarray(v) for k, v in a.items()}
+    if name == "cons_prim":
+        q = a["q"].copy()
+        w = np.zeros_like(q)
+        if oracle:
+            lib.or_cons2prim_b(dp(q), *d, 0)
+            p = q.copy()
+            lib.or_prim2cons_b(dp(q), *d)
+        else:
+            lib.ref_cons2prim_b(dp(q), dp(w), *d, 0)
+            p = q.copy()
+            lib.ref_prim2cons_b(dp(q), dp(w), *d)
+        return dict(prim=p, cons=q)
+    if name.startswith("bbc"):
+        q = a["q"].copy()
+        if oracle:
+            lib.or_bbc(dp(q), dp(a["si"]), dp(a["sj"]), dp(a["sk"]), ip(a["bt"]), 6, *d)
+        else:
+            lib.ref_bbc(dp(q), dp(a["si"]), dp(a["sj"]), dp(a["sk"]), dp(np.zeros(a["q"].size)), 1, ip(a["bt"]), 6, *d)
+        return dict(q=q)
+    if name.startswith("lambda"):
+        N3 = n3(*d)
+        n1 = (d[0] + 1) * (d[1] + 1) * (d[2] + 1)
+        n2 = (d[0] + 2) * (d[1] + 2) * (d[2] + 2)
+        lam = np.zeros(N3 * 5)
+        xd = np.zeros(n2)
+        outs = [np.zeros(n1) for _ in range(4)]
+        if oracle:
+            lib.or_lambdab(dp(a["q"]), dp(a["si"]), dp(a["sj"]), dp(a["sk"]), dp(xd), dp(xd), dp(xd), dp(lam), *d)
+            lib.or_bdeltat(dp(outs[0]), dp(lam), dp(outs[1]), dp(outs[2]), dp(outs[3]), *d)
+        else:
+            z = np.zeros(N3 * 5)
+            lib.ref_lambdab(dp(a["q"]), dp(a["si"]), dp(a["sj"]), dp(a["sk"]), dp(xd), dp(xd), dp(xd),
+                            dp(np.ones(n1)), dp(np.zeros(N3)), dp(z), dp(lam), dp(np.zeros(n1)), dp(np.zeros(n1)), *d)
+            t = [np.zeros(n1) for _ in range(4)]
+            lib.ref_bdeltat(dp(outs[0]), dp(t[0]), dp(lam), dp(np.zeros_like(lam)), dp(outs[1]), dp(outs[2]),
+                            dp(outs[3]), dp(t[1]), dp(t[2]), dp(t[3]), *d)
+        return dict(lam=lam, dtvol=outs[0], bx=outs[1], be=outs[2], bz=outs[3])
+    if name.startswith("muscl"):
+        arrs = [np.zeros_like(a["q"]) for _ in range(5)]
+        fn = lib.or_muscl_b if oracle else lib.ref_muscl_b
+        fn(int(a["idir"][0]), 1, ip(a["bt"]), 6, dp(a["q"]), *[dp(x) for x in arrs], *d)
+        return dict(qp=arrs[0], qm=arrs[1], dq=arrs[2], dqp=arrs[3], dqm=arrs[4])
+    if name.startswith("roflux"):
+        idir = int(a["idir"][0])
+        flux = np.zeros(d[0] * d[1] * d[2] * 5)
+        if oracle:
+            lib.or_roflux_b(idir, dp(a["qr"]), dp(a["ql"]), dp(flux), dp(a["s"]), *d)
+        else:
+            sc = [np.zeros_like(a["qr"]) for _ in range(2)]
+            lib.ref_roflux_b(idir, dp(a["qr"]), dp(a["ql"]), dp(sc[0]), dp(sc[1]), dp(flux), dp(a["s"]), *d)
+        res = a["res"].copy()
+        (lib.or_bresid if oracle else lib.ref_bresid)(idir, dp(flux), dp(res), *d)
+        return dict(flux=flux, res=res)
+    if name.startswith("src_hb"):
+        res = a["res"].copy()
+        qhb = np.zeros_like(res)
+        (lib.or_src_bhb if oracle else lib.ref_src_bhb)(dp(a["q"]), dp(a["vol"]), dp(res), dp(qhb), *d)
+        return dict(res=res, qhb=qhb)
+    if name.startswith("cirs"):
+        res = a["res"].copy()
+        if oracle:
+            lib.or_cirs_b(dp(res), dp(a["dtv"]), dp(a["bx"]), dp(a["be"]), dp(a["bz"]), *d)
+        else:
+            w = np.zeros(max(d) * 5 * 4)
+            lib.ref_cirs_b(dp(res), dp(a["dtv"]), dp(a["bx"]), dp(a["be"]), dp(a["bz"]), dp(a["dtv"]), dp(a["bx"]),
+                           dp(a["be"]), dp(a["bz"]), dp(w), *d)
+        return dict(res=res)
+    if name.startswith("restrict"):
+        q2 = np.zeros(n3(*cc) * 5)
+        rhs2 = np.zeros(n3(*cc) * 5)
+        if oracle:
+            lib.or_rest1_b(*f, *cc, dp(a["q1"]), dp(q2), dp(a["v1"]), dp(a["v2"]))
+            lib.or_rest2_b(1, *f, *cc, dp(rhs2), dp(a["res1"]))
+        else:
+            lib.ref_rest1_b(*f, *cc, dp(a["q1"]), dp(q2), dp(a["v1"]), dp(a["v2"]))
+            lib.ref_rest2_b(1, *f, *cc, dp(np.zeros_like(a["res1"])), dp(rhs2), dp(a["res1"]))
+        return dict(q2=q2, rhs2=rhs2)
+    if name.startswith("prolong"):
+        bln = int(name[-1])
+        q1 = a["q1"].copy()
+        (lib.or_prol_b if oracle else lib.ref_prol_b)(bln, *f, *cc, dp(q1), dp(a["q2"]), dp(a["v1"]))
+        return dict(q1=q1)
+    if name == "edges_corners":
+        q = a["q"].copy()
+        if oracle:
+            lib.or_q_bedges(dp(q), *d)
+            lib.or_q_bcorns(dp(q), *d)
+        else:
+            mut = np.zeros(n3(*d))
+            lib.ref_q_bedges(dp(q), dp(mut), *d)
+            lib.ref_q_bcorns(dp(q), dp(mut), *d)
+        return dict(q=q)
+    raise KeyError(name)
+
+
+def set_cfg(lib, c):
+    d = hb_operator(c["nharms"], c["omega"]) if c["nharms"] else None
+    lib.set_config(c, d)
+
+
+def main():
+    ref = Ref()
+    store = {}
+    names = []
+    for name, c, a in cases():
+        set_cfg(ref, c)
+        outs = run(ref, None, name, c, a, oracle=False)
+        names.append(name)
+        for k, v in a.items():
+            store[f"{name}/in/{k}"] = v
+        for k, v in outs.items():
+            store[f"{name}/out/{k}"] = v
+        store[f"{name}/cfg"] = np.frombuffer(repr(sorted(c.items())).encode(), np.uint8)
+    store["names"] = np.array(names)
+    path = os.path.join(HERE, "ref_kernels.npz")
+    np.savez_compressed(path, **store)
+    print(path, os.path.getsize(path), "bytes,", len(names), "fixtures")
+
+
+if __name__ == "__main__":
+    main()
