@@ -44,6 +44,11 @@ BYTES_PER_CELL = {
     "lambda": 144.0,     # q 40 R + metrics 96 R + dtvol 8 W
     "other": None,
 }
+# HBM bytes per cell-update measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled by a plain-copy calibration: FETCH x2.00, WRITE x1.00) on the
+# uneven96 workload; profiles/residual_uneven96_fp64_hbm_traffic.md.  Re-measure
+# when the kernel changes.
+MEASURED_TRAFFIC_PER_CELL = {("uneven96", "residual"): 322.7}
 
 
 def log(msg):
@@ -114,6 +119,21 @@ def cpu_baseline(case_full, budget_s: float):
                       f"oracle/liboracle.so (C restatement, gcc -O2, 1 thread)"}
 
 
+def copy_calibration():
+    """A plain copy of known size: 512 MiB read + 512 MiB written per launch.
+    The counters FETCH_SIZE / WRITE_SIZE read on it give the factor between
+    what they report and the bytes that really crossed HBM."""
+    import torch
+    n = 64 << 20
+    x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8,
+                      "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,11 +142,18 @@ def main():
     ap.add_argument("--workload", default="uneven96", choices=["uneven96", "naca"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 512 MiB device-to-device copy three times (scale for the HBM "
+                         "byte counters FETCH_SIZE / WRITE_SIZE) and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the path on one GPU only (one GPU's share of the blocks); "
               "--gpus must be 1", file=sys.stderr)
         sys.exit(2)
+
+    if a.copy_calibration:
+        copy_calibration()
+        return
 
     from cosa_amd.solver import Solve
     t0 = time.time()
@@ -172,8 +199,10 @@ def main():
     dom = max((n for n in fams if BYTES_PER_CELL.get(n)), key=lambda n: fams[n]["ms_total"])
     d = fam[dom]
     achieved = BYTES_PER_CELL[dom] * d["items"] / (d["ms"] / 1e3) / 1e9
+    tpc = MEASURED_TRAFFIC_PER_CELL.get((a.workload, dom))
+    traffic = round(tpc * d["items"] / max(d["launches"], 1)) if tpc else None
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "bytes_per_cell_update": BYTES_PER_CELL[dom], "avg_launch_ms": round(d["ms"] / max(d["launches"], 1), 4)}
     s.close()
 

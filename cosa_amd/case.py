@@ -28,6 +28,10 @@ class Case:
     levels: List[Level] = field(default_factory=list)
     geom: List[Dict[str, np.ndarray]] = field(default_factory=list)
     name: str = "case"
+    # rigid pitching of the whole grid for harmonic-balance moving-grid runs:
+    # {"dtheta0": amplitude [rad], "xh": hinge x, "yh": hinge y}; snapshot n
+    # sits at t_n = 2 pi n / ((2N+1) omega) with theta = dtheta0 sin(omega t)
+    motion: Optional[Dict] = None
 
     def __post_init__(self):
         nlevel = self.config["nlevel"]
@@ -66,28 +70,51 @@ class Case:
         return self.levels[nl - 1].ncell()
 
     # -- geometry ------------------------------------------------------------
+    def snapshot_pose(self, n: int) -> Tuple[float, float]:
+        """Pitch angle and rate of harmonic-balance snapshot n."""
+        m = self.motion
+        om = self.config["omega"]
+        t = 2.0 * math.pi * n / ((2 * self.config["nharms"] + 1) * om)
+        return m["dtheta0"] * math.sin(om * t), m["dtheta0"] * om * math.cos(om * t)
+
     def _level_geometry(self, L: Level) -> Dict[str, np.ndarray]:
         lmet = self.config["lmet"]
         nsm = self.nsnap_met
+        moving = bool(self.config["moving"])
+        if moving and (self.motion is None or lmet != 7):
+            raise ValueError("a moving case needs lmet = 7 and a motion")
         si = np.zeros(self.size(L.nl, "met"))
         sj = np.zeros_like(si)
         sk = np.zeros_like(si)
         vol = np.zeros(self.size(L.nl, "vol"))
+        vel = [np.zeros(self.size(L.nl, "x2")) for _ in range(3)] if moving else None
         o2 = L.offsets(2)
         o1 = L.offsets(1)
         for b in range(L.nblocks):
             x, y, z = (grids.coarsen(a, L.nl) for a in self.xyz[b])
             assert x.shape == (L.kmax[b], L.jmax[b], L.imax[b])
-            mets = grids.face_metrics(x, y, z, lmet)
             n2 = int(L.points(2)[b])
-            for arr, m in zip((si, sj, sk), mets):
-                blk = arr[o2[b] * lmet * nsm: (o2[b] + n2) * lmet * nsm]
-                flat = m.reshape(-1)                       # [k][j][i][lmet]
-                for n in range(nsm):
-                    blk[n * n2 * lmet:(n + 1) * n2 * lmet] = flat
+            for n in range(nsm):
+                if moving:
+                    th, thd = self.snapshot_pose(n)
+                    xn, yn, zn, *v = grids.pitch(x, y, z, th, thd, self.motion.get("xh", 0.25),
+                                                  self.motion.get("yh", 0.0))
+                    mets = grids.face_metrics(xn, yn, zn, lmet, vel=v)
+                    for dst, src in zip(vel, v):
+                        p2 = np.zeros((x.shape[0] + 2, x.shape[1] + 2, x.shape[2] + 2))
+                        p2[1:-1, 1:-1, 1:-1] = src        # node (i,j,k) at P2 index (i,j,k)
+                        dst[o2[b] * nsm + n * n2: o2[b] * nsm + (n + 1) * n2] = p2.reshape(-1)
+                elif n == 0:
+                    mets = grids.face_metrics(x, y, z, lmet)
+                for arr, m in zip((si, sj, sk), mets):
+                    blk = arr[o2[b] * lmet * nsm: (o2[b] + n2) * lmet * nsm]
+                    blk[n * n2 * lmet:(n + 1) * n2 * lmet] = m.reshape(-1)   # [k][j][i][lmet]
             v = grids.cell_volumes(x, y, z)
             vol[o1[b]: o1[b] + L.points(1)[b]] = v.reshape(-1)
-        return {"si": si, "sj": sj, "sk": sk, "vol": vol}
+        out = {"si": si, "sj": sj, "sk": sk, "vol": vol}
+        if moving:
+            out.update(xdot=vel[0], ydot=vel[1], zdot=vel[2])
+        return out
 
     # -- states --------------------------------------------------------------
     def freestream(self, nl: int = 1) -> np.ndarray:
@@ -125,11 +152,15 @@ class Case:
 # ---------------------------------------------------------------------------
 # configurations
 
-def naca_case(ni=193, nj=49, nk=5, nlevel=1, name="naca0012", **cfg) -> Case:
+def naca_case(ni=193, nj=49, nk=5, nlevel=1, name="naca0012", pitch=None, **cfg) -> Case:
     """NACA 0012 O-grid, single block: j=1 inviscid wall (14), j=jmax
-    farfield (30), k faces symmetry (80), seam i=1 <-> i=imax as a self-cut."""
+    farfield (30), k faces symmetry (80), seam i=1 <-> i=imax as a self-cut.
+    pitch = {"dtheta0", "xh", "yh"} makes it a harmonic-balance pitching
+    aerofoil on a moving grid (moving, hbmove = 1, lmet = 7)."""
     cfg.setdefault("machfs", 0.5)
     cfg.setdefault("alpha_deg", 1.25)
+    if pitch is not None:
+        cfg.update(moving=1, hbmove=1, lmet=7)
     c = default_config(nlevel=nlevel, **cfg)
     x, y, z = grids.naca0012_ogrid(ni, nj, nk)
     b = Block(ni, nj, nk, [
@@ -139,7 +170,7 @@ def naca_case(ni=193, nj=49, nk=5, nlevel=1, name="naca0012", **cfg) -> Case:
         Patch((1, 1, nk), (ni, nj, nk), 80),
     ])
     cut = Cut(1, (1, 1, 1), (1, nj, nk), 1, (ni, 1, 1), (ni, nj, nk), (1, 2, 3))
-    return Case(c, [b], [cut], [(x, y, z)], name=name)
+    return Case(c, [b], [cut], [(x, y, z)], name=name, motion=pitch)
 
 
 def channel_block(ni, nj, nk) -> Block:

@@ -25,13 +25,22 @@ _DBL_FIELDS = [
 ]
 
 
+# viscous / k-omega block appended after omefar
+_TURB_INT_FIELDS = ["turcmp", "lim_prodtke", "lim_prodomega", "wallwilc", "ifixq", "prd_all_lvls"]
+_TURB_DBL_FIELDS = ["reyno", "pranl", "prant", "stemp", "cdff", "epst", "cmu", "sigk", "sigw", "bkw", "bstrkw",
+                    "gkw", "prdlim", "roughk"]
+
+
 class CosahipConfig(ctypes.Structure):
     _fields_ = ([(n, ctypes.c_int32) for n in _INT_FIELDS]
                 + [("entfx_f_w", ctypes.c_int32 * 3), ("entfx_t_w", ctypes.c_int32 * 3),
                    ("bln_prol", ctypes.c_int32), ("ho_rest", ctypes.c_int32), ("pad0", ctypes.c_int32)]
                 + [(n, ctypes.c_double) for n in _DBL_FIELDS]
                 + [("qmin", ctypes.c_double * 7), ("alphas", ctypes.c_double * 5),
-                   ("tkefar", ctypes.c_double), ("omefar", ctypes.c_double)])
+                   ("tkefar", ctypes.c_double), ("omefar", ctypes.c_double)]
+                + [(n, ctypes.c_int32) for n in _TURB_INT_FIELDS]
+                + [("lim_corr", ctypes.c_int32 * 3), ("pad1", ctypes.c_int32)]
+                + [(n, ctypes.c_double) for n in _TURB_DBL_FIELDS])
 
 
 def default_config(**kw: Any) -> Dict[str, Any]:
@@ -47,6 +56,11 @@ def default_config(**kw: Any) -> Dict[str, Any]:
         omega=0.0, toler=0.0,
         qmin=(1e-8, 1.0, 1.0, 1.0, 1e-8, 1e-12, 1e-8), alphas=(0.25, 0.333333, 0.5, 1.0, 0.0),
         tkefar=0.0, omefar=0.0,
+        # viscous / Wilcox k-omega (src/input.f90:2501-2525; kappa = 0.41)
+        turcmp=0, lim_prodtke=0, lim_prodomega=0, wallwilc=1, ifixq=0, prd_all_lvls=0, lim_corr=(0, 0, 0),
+        reyno=1.0e6, pranl=0.72, prant=0.9, stemp=110.4 / 288.15, cdff=1.0, epst=1e-12, cmu=1.0, sigk=0.5,
+        sigw=0.5, bkw=3.0 / 40.0, bstrkw=0.09, gkw=(3.0 / 40.0) / 0.09 - 0.5 * 0.41 ** 2 / math.sqrt(0.09),
+        prdlim=20.0, roughk=0.0,
     )
     for k, v in kw.items():
         if k not in c and k not in ("alpha_deg", "beta_deg", "cfl_irs"):
@@ -83,7 +97,26 @@ def to_struct(c: Dict[str, Any]) -> CosahipConfig:
         s.alphas[i] = float(c["alphas"][i])
     s.tkefar = float(c["tkefar"])
     s.omefar = float(c["omefar"])
+    for n in _TURB_INT_FIELDS:
+        setattr(s, n, int(c[n]))
+    for i in range(3):
+        s.lim_corr[i] = int(c["lim_corr"][i])
+    for n in _TURB_DBL_FIELDS:
+        setattr(s, n, float(c[n]))
     return s
+
+
+def kw_turbulence(machfs: float, reyno: float, tu: float = 1e-4, mutfar: float = 1e-2, **kw) -> Dict[str, Any]:
+    """Wilcox k-omega settings of a viscous run (the 'komega' branch of
+    input, src/input.f90:2695-2720): farfield k from the turbulence
+    intensity squared tu, omega from mut/mu at the farfield mutfar."""
+    tkefar = tu * machfs ** 2
+    rhoinf = kw.get("rhoinf", 1.0)
+    cmu = kw.get("cmu", 1.0)
+    out = dict(npde=7, kom=1, viscous=1, machfs=machfs, reyno=reyno, tkefar=tkefar,
+               omefar=cmu * rhoinf * tkefar / mutfar * reyno / machfs)
+    out.update(kw)
+    return out
 
 
 def hb_operator(nharms: int, omega: float):

@@ -37,6 +37,11 @@ program binding_demo
   cfg%qmin = (/1.d-8, 1.d0, 1.d0, 1.d0, 1.d-8, 1.d-12, 1.d-8/)
   cfg%alphas = (/0.25d0, 0.333333d0, 0.5d0, 1.d0, 0.d0/)
   cfg%tkefar = 0.d0; cfg%omefar = 0.d0
+  cfg%turcmp = 0; cfg%lim_prodtke = 0; cfg%lim_prodomega = 0; cfg%wallwilc = 1; cfg%ifixq = 0
+  cfg%prd_all_lvls = 0; cfg%lim_corr = 0; cfg%pad1 = 0
+  cfg%reyno = 1.d6; cfg%pranl = 0.72d0; cfg%prant = 0.9d0; cfg%stemp = 0.383d0; cfg%cdff = 1.d0
+  cfg%epst = 1.d-12; cfg%cmu = 1.d0; cfg%sigk = 0.5d0; cfg%sigw = 0.5d0; cfg%bkw = 0.075d0
+  cfg%bstrkw = 0.09d0; cfg%gkw = 0.5527d0; cfg%prdlim = 20.d0; cfg%roughk = 0.d0
 
   rc = cosahip_create(cfg, 0, s)
   if (rc /= COSAHIP_OK) stop 'cosahip_create failed'

@@ -18,6 +18,11 @@ module cosahip_binding
   integer(c_int), parameter :: COSAHIP_Q = 0, COSAHIP_QOLD = 1, COSAHIP_RES = 2, COSAHIP_RHS = 3
   integer(c_int), parameter :: COSAHIP_SI = 4, COSAHIP_SJ = 5, COSAHIP_SK = 6, COSAHIP_VOL = 7
   integer(c_int), parameter :: COSAHIP_DTVOL = 8, COSAHIP_XDOT = 12, COSAHIP_YDOT = 13, COSAHIP_ZDOT = 14
+  integer(c_int), parameter :: COSAHIP_MUT = 15, COSAHIP_DIST = 16, COSAHIP_XIDERI = 17, COSAHIP_ETADERI = 18
+  integer(c_int), parameter :: COSAHIP_ZETADERI = 19, COSAHIP_XIDERJ = 20, COSAHIP_ETADERJ = 21
+  integer(c_int), parameter :: COSAHIP_ZETADERJ = 22, COSAHIP_XIDERK = 23, COSAHIP_ETADERK = 24
+  integer(c_int), parameter :: COSAHIP_ZETADERK = 25, COSAHIP_PRD = 26, COSAHIP_DIVVEL = 27
+  integer(c_int), parameter :: COSAHIP_DELPLS = 28, COSAHIP_DTVOLT = 29
 
   ! cosahip_config, member for member
   type, bind(C) :: cosahip_config
@@ -38,6 +43,10 @@ module cosahip_binding
     real(c_double) :: omega, toler
     real(c_double) :: qmin(7), alphas(5)
     real(c_double) :: tkefar, omefar
+    integer(c_int32_t) :: turcmp, lim_prodtke, lim_prodomega, wallwilc, ifixq, prd_all_lvls
+    integer(c_int32_t) :: lim_corr(3), pad1
+    real(c_double) :: reyno, pranl, prant, stemp, cdff, epst, cmu, sigk, sigw, bkw, bstrkw, gkw
+    real(c_double) :: prdlim, roughk
   end type cosahip_config
 
   interface

@@ -115,11 +115,15 @@ def coarsen(x: np.ndarray, nl: int) -> np.ndarray:
 # ---------------------------------------------------------------------------
 # metrics and volumes
 
-def face_metrics(x: np.ndarray, y: np.ndarray, z: np.ndarray, lmet: int = 4):
+def face_metrics(x: np.ndarray, y: np.ndarray, z: np.ndarray, lmet: int = 4, vel=None):
     """si, sj, sk of one block (imetrix_b, src/grid.f90:848-1253), shaped
-    [k, j, i, lmet] over 0:kmax+1 etc. (the P2 family with lmet fastest)."""
+    [k, j, i, lmet] over 0:kmax+1 etc. (the P2 family with lmet fastest).
+    With lmet = 7 and vel = (xdot, ydot, zdot) at the nodes, components 5-7
+    are the face-midpoint velocity, the mean of the face's four node
+    velocities (src/grid.f90:1031-1064)."""
     nk, nj, ni = x.shape
     out = []
+    V = np.stack(vel, axis=-1) if vel is not None else None
     for d in range(3):
         s = np.zeros((nk + 2, nj + 2, ni + 2, lmet))
         P = np.stack([x, y, z], axis=-1)  # [k, j, i, 3]
@@ -150,8 +154,30 @@ def face_metrics(x: np.ndarray, y: np.ndarray, z: np.ndarray, lmet: int = 4):
         s[sl + (1,)] = c1 / area
         s[sl + (2,)] = c2 / area
         s[sl + (3,)] = area
+        if V is not None and lmet >= 7:
+            if d == 0:
+                fv = 0.25 * (V[:-1, :-1, :] + V[:-1, 1:, :] + V[1:, :-1, :] + V[1:, 1:, :])
+            elif d == 1:
+                fv = 0.25 * (V[:-1, :, :-1] + V[:-1, :, 1:] + V[1:, :, :-1] + V[1:, :, 1:])
+            else:
+                fv = 0.25 * (V[:, :-1, :-1] + V[:, :-1, 1:] + V[:, 1:, :-1] + V[:, 1:, 1:])
+            s[sl + (slice(4, 7),)] = fv
         out.append(s)
     return out
+
+
+def pitch(x0: np.ndarray, y0: np.ndarray, z0: np.ndarray, theta: float, theta_dot: float,
+          xh: float = 0.25, yh: float = 0.0):
+    """Rigid pitching about the hinge (xh, yh) in the x-y plane: node
+    positions rotated by theta and node velocities for a pitch rate
+    theta_dot (the rigid-body motion of pitch, src/grid.f90:1872-1960)."""
+    c, s = np.cos(theta), np.sin(theta)
+    dx, dy = x0 - xh, y0 - yh
+    x = xh + dx * c - dy * s
+    y = yh + dx * s + dy * c
+    xd = -theta_dot * (dx * s + dy * c)
+    yd = theta_dot * (dx * c - dy * s)
+    return x, y, z0.copy(), xd, yd, np.zeros_like(z0)
 
 
 def cell_volumes(x: np.ndarray, y: np.ndarray, z: np.ndarray) -> np.ndarray:

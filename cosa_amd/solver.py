@@ -56,8 +56,9 @@ class Solve:
                                                        _lib.iptr(keep[4]), L.ncuts, _lib.iptr(cut)),
                         f"define_level {L.nl}")
             g = c.geom[L.nl - 1]
-            for f in ("si", "sj", "sk", "vol"):
-                self.upload(f, g[f], L.nl)
+            for f in ("si", "sj", "sk", "vol", "xdot", "ydot", "zdot"):
+                if f in g:
+                    self.upload(f, g[f], L.nl)
             self.upload("q", c.freestream(L.nl), L.nl)
         d = c.hb_matrix()
         if d is not None:

@@ -83,6 +83,23 @@ typedef struct cosahip_config {
   double qmin[7];
   double alphas[5];    /* RK stage coefficients (ar4 literals)             */
   double tkefar, omefar;
+  /* ---- viscous flow and Wilcox k-omega (src/common_variables.f90:23-65,
+   *      defaults and parsing src/input.f90:2501-2525, 2695-2770, 3700) ---- */
+  int32_t turcmp;        /* 1: compressibility term in div(u) of k-omega    */
+  int32_t lim_prodtke;   /* 1: limit k production (prdlim)                  */
+  int32_t lim_prodomega; /* 1: limit omega production too                   */
+  int32_t wallwilc;      /* type-15 wall omega: 1 Wilcox rough wall, 0 Menter */
+  int32_t ifixq;         /* 1: floor k, omega at qmin(6:7) after updates    */
+  int32_t prd_all_lvls;  /* 1: production on every level (0 only here)     */
+  int32_t lim_corr[3];   /* k-omega prolongation limiters (all 0 here)      */
+  int32_t pad1;
+  double reyno;          /* Reynolds number                                  */
+  double pranl, prant;   /* laminar / turbulent Prandtl numbers              */
+  double stemp;          /* Sutherland temperature / reference temperature   */
+  double cdff;           /* weight of the viscous spectral radii             */
+  double epst, cmu, sigk, sigw, bkw, bstrkw, gkw;   /* k-omega constants     */
+  double prdlim;         /* production limiter factor                       */
+  double roughk;         /* wall roughness (Wilcox wall, 0: smooth)          */
 } cosahip_config;
 
 /* ---- device fields (per level; reference layout and extents) ----------
@@ -102,10 +119,29 @@ enum {
   COSAHIP_BETAXI = 9,   /* CIRS coefficients (P1, 0:2N)                    */
   COSAHIP_BETAETA = 10,
   COSAHIP_BETAZETA = 11,
-  COSAHIP_XDOT = 12,    /* cell-centre grid velocity (P2 scalar, 0:2N*hbmove) */
+  COSAHIP_XDOT = 12,    /* node grid velocity (P2 scalar, 0:2N*hbmove); lambdab
+                           reads node (i,j,k) for cell (i,j,k)              */
   COSAHIP_YDOT = 13,
   COSAHIP_ZDOT = 14,
-  COSAHIP_NFIELDS = 15
+  COSAHIP_MUT = 15,     /* turbulent viscosity (P3 scalar, 0:2N)           */
+  COSAHIP_DIST = 16,    /* cell-centre wall distance (P1, no snapshot index) */
+  COSAHIP_XIDERI = 17,  /* metric derivatives (3, imax, jmax, kmax, 0:2N*hbmove),
+                           off_0 offsets: xi/eta/zeta derivatives on i, j, k
+                           faces in the order xideri, etaderi, zetaderi,
+                           xiderj, etaderj, zetaderj, xiderk, etaderk, zetaderk */
+  COSAHIP_ETADERI = 18,
+  COSAHIP_ZETADERI = 19,
+  COSAHIP_XIDERJ = 20,
+  COSAHIP_ETADERJ = 21,
+  COSAHIP_ZETADERJ = 22,
+  COSAHIP_XIDERK = 23,
+  COSAHIP_ETADERK = 24,
+  COSAHIP_ZETADERK = 25,
+  COSAHIP_PRD = 26,     /* k-omega production P_d (P3 scalar, 0:2N)        */
+  COSAHIP_DIVVEL = 27,  /* velocity divergence (P3 scalar, 0:2N)           */
+  COSAHIP_DELPLS = 28,  /* 2/3 max(div u, 0) (P3 scalar, 0:2N)             */
+  COSAHIP_DTVOLT = 29,  /* k-omega dt/vol (P1, 0:2N)                       */
+  COSAHIP_NFIELDS = 30
 };
 
 typedef struct cosahip_solve cosahip_solve;

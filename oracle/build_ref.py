@@ -35,7 +35,7 @@ BY_NAME = {
                "bc_extrapol", "bc_sym"],
     "cosa.f90": ["bresid", "lambdab", "eig_eul", "eig_ns", "eig_eul_p", "eig_ns_p", "bdeltat", "cirs_b",
                  "tridi", "resg_brms", "resg_brms_hb", "rest1_b", "rest2_ho_b", "rest2_lo_b", "prol_lin_b",
-                 "prol_bln_b"],
+                 "prol_bln_b", "rest4_b"],
 }
 FFLAGS = ["-cpp", "-fdefault-real-8", "-fdefault-double-8", "-O2", "-ffp-contract=off", "-fPIC"]
 

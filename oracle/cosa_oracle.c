@@ -427,8 +427,16 @@ static void or_bc_sym(double* q, const int32_t* t, bg g) {
 }
 
 /* bbc (src/bc.f90:35-121): patches in input order. */
+static void or_bc_walvis15(double* q, const double* si, const double* sj, const double* sk, const double* dist,
+                           const int32_t* t, bg g);
 void or_bbc(double* q, const double* si, const double* sj, const double* sk, const int32_t* bctopo, int nbcs,
             int imax, int jmax, int kmax) {
+  or_bbc_dist(q, si, sj, sk, NULL, bctopo, nbcs, imax, jmax, kmax);
+}
+/* bbc with the cell-centre wall distance dist(0:imax,0:jmax,0:kmax) that
+ * viscous walls (types 15xx) need.                                        */
+void or_bbc_dist(double* q, const double* si, const double* sj, const double* sk, const double* dist,
+                 const int32_t* bctopo, int nbcs, int imax, int jmax, int kmax) {
   bg g = mkbg(imax, jmax, kmax);
   for (int ibc = 0; ibc < nbcs; ibc++) {
     const int32_t* t = bctopo + 10 * ibc;
@@ -436,6 +444,7 @@ void or_bbc(double* q, const double* si, const double* sj, const double* sk, con
     else if (t[0] == 4) or_bc_extrapol(q, t, g);
     else if (t[0] == 30) or_bc_far0(q, si, sj, sk, t, g);
     else if (t[0] == 14) or_bc_walinv14(q, si, sj, sk, t, g);
+    else if (t[0] / 100 == 15 && dist) or_bc_walvis15(q, si, sj, sk, dist, t, g);
     /* other types are not built here; the library rejects them at define time */
   }
 }
@@ -549,6 +558,148 @@ void or_lambdab(const double* q, const double* si, const double* sj, const doubl
           Q(lambdas, g, i, j, k, 2, n) = lj;
           Q(lambdas, g, i, j, k, 3, n) = lk;
         }
+  }
+}
+
+/* lambdab, viscous branch (src/cosa.f90:1350-1430) with eig_ns
+ * (src/cosa.f90:4240-4297): inviscid radii into lambdas(1:3), viscous
+ * ones into lamv(1:3) (the reference's work = dq).  mut enters through
+ * work(:,:,:,4) for k-omega and is zero otherwise.  As in the reference,
+ * the temperature of the +1 neighbour divides by the density of the -1
+ * neighbour.                                                              */
+void or_lambdab_v(const double* q, const double* si, const double* sj, const double* sk, const double* xdot,
+                  const double* ydot, const double* zdot, const double* vol, const double* mut, double* lambdas,
+                  double* lamv, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  double gamma = P.gamma;
+  int kom = is_kom();
+  for (int n = 0; n <= 2 * NH; n++) {
+    int nh = n * P.hbmove;
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          int e[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+          double rho = Q(q, g, i, j, k, 1, n), p = Q(q, g, i, j, k, 5, n);
+          double t = gamma * p / rho;
+          double mu = (P.stemp + 1.0) / (t + P.stemp) * pow(t, 1.5);
+          double mutc = kom ? S1(mut, g, i, j, k, n) : 0.0;
+          double mud[3], mutd[3];
+          for (int d = 0; d < 3; d++) {
+            int a = e[d][0], b = e[d][1], c = e[d][2];
+            double rhom = Q(q, g, i - a, j - b, k - c, 1, n), pm = Q(q, g, i - a, j - b, k - c, 5, n);
+            double tm = gamma * pm / rhom;
+            double mum = (P.stemp + 1.0) / (tm + P.stemp) * pow(tm, 1.5);
+            double mutm = kom ? S1(mut, g, i - a, j - b, k - c, n) : 0.0;
+            double pp = Q(q, g, i + a, j + b, k + c, 5, n);
+            double tp = gamma * pp / rhom;
+            double mup = (P.stemp + 1.0) / (tp + P.stemp) * pow(tp, 1.5);
+            double mutp = kom ? S1(mut, g, i + a, j + b, k + c, n) : 0.0;
+            mud[d] = (mum + 2 * mu + mup) / 4;
+            mutd[d] = (mutm + 2 * mutc + mutp) / 4;
+          }
+          double sim[4], sip[4], sjm[4], sjp[4], skm[4], skp[4];
+          for (int m = 0; m < 4; m++) {
+            sim[m] = M(si, g, m + 1, i, j, k, nh);
+            sip[m] = M(si, g, m + 1, i + 1, j, k, nh);
+            sjm[m] = M(sj, g, m + 1, i, j, k, nh);
+            sjp[m] = M(sj, g, m + 1, i, j + 1, k, nh);
+            skm[m] = M(sk, g, m + 1, i, j, k, nh);
+            skp[m] = M(sk, g, m + 1, i, j, k + 1, nh);
+          }
+          double u = Q(q, g, i, j, k, 2, n), v = Q(q, g, i, j, k, 3, n), w = Q(q, g, i, j, k, 4, n);
+          double a = sqrt(t);
+          double xix = 0.5 * (sim[0] * sim[3] + sip[0] * sip[3]);
+          double xiy = 0.5 * (sim[1] * sim[3] + sip[1] * sip[3]);
+          double xiz = 0.5 * (sim[2] * sim[3] + sip[2] * sip[3]);
+          double etax = 0.5 * (sjm[0] * sjm[3] + sjp[0] * sjp[3]);
+          double etay = 0.5 * (sjm[1] * sjm[3] + sjp[1] * sjp[3]);
+          double etaz = 0.5 * (sjm[2] * sjm[3] + sjp[2] * sjp[3]);
+          double zetax = 0.5 * (skm[0] * skm[3] + skp[0] * skp[3]);
+          double zetay = 0.5 * (skm[1] * skm[3] + skp[1] * skp[3]);
+          double zetaz = 0.5 * (skm[2] * skm[3] + skp[2] * skp[3]);
+          double axi = (sim[3] + sip[3]) / 2, aeta = (sjm[3] + sjp[3]) / 2, azeta = (skm[3] + skp[3]) / 2;
+          if (P.moving) {
+            u = u - X2(xdot, g, i, j, k, nh);
+            v = v - X2(ydot, g, i, j, k, nh);
+            w = w - X2(zdot, g, i, j, k, nh);
+          }
+          double volc = V1(vol, g, i, j, k);
+          Q(lambdas, g, i, j, k, 1, n) = fabs(u * xix + v * xiy + w * xiz) + a * axi;
+          Q(lambdas, g, i, j, k, 2, n) = fabs(u * etax + v * etay + w * etaz) + a * aeta;
+          Q(lambdas, g, i, j, k, 3, n) = fabs(u * zetax + v * zetay + w * zetaz) + a * azeta;
+          double ar_[3] = {axi, aeta, azeta};
+          for (int d = 0; d < 3; d++)
+            Q(lamv, g, i, j, k, d + 1, n) = P.cdff * ar_[d] * ar_[d] * gamma * P.machfs / P.reyno *
+                                            (mud[d] / P.pranl + mutd[d] / P.prant) / (rho * volc);
+        }
+  }
+}
+
+/* bdeltat (src/cosa.f90:4733-5203) without CFL ramping: lamv (viscous
+ * radii, NULL when inviscid) join the sum of dtvol; k-omega adds
+ * dtvolt = cflt/cfl * dtvol and the turbulence CIRS coefficients tbeta*
+ * (same forms with ratt).                                                 */
+void or_bdeltat_v(double* dtvol, double* dtvolt, const double* lambdas, const double* lamv, double* betaxi,
+                  double* betaeta, double* betazeta, double* tbetaxi, double* tbetaeta, double* tbetazeta, int imax,
+                  int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  const double zeta = 0.4;
+  int kom = is_kom();
+  for (int n = 0; n <= 2 * NH; n++) {
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          double sum = Q(lambdas, g, i, j, k, 1, n) + Q(lambdas, g, i, j, k, 2, n) + Q(lambdas, g, i, j, k, 3, n);
+          if (lamv)
+            sum = sum + Q(lamv, g, i, j, k, 1, n) + Q(lamv, g, i, j, k, 2, n) + Q(lamv, g, i, j, k, 3, n);
+          P1(dtvol, g, i, j, k, n) = P.cfl / sum;
+        }
+    if (kom)
+      for (int k = 1; k <= kmax - 1; k++)
+        for (int j = 1; j <= jmax - 1; j++)
+          for (int i = 1; i <= imax - 1; i++) P1(dtvolt, g, i, j, k, n) = P.cflt / P.cfl * P1(dtvol, g, i, j, k, n);
+  }
+  for (int pass = 0; pass < (kom ? 2 : 1); pass++) {
+    double rr = pass == 0 ? P.rat : P.ratt;
+    double* bx_ = pass == 0 ? betaxi : tbetaxi;
+    double* by_ = pass == 0 ? betaeta : tbetaeta;
+    double* bz_ = pass == 0 ? betazeta : tbetazeta;
+    if (P.irsop == 2) {
+      for (int n = 0; n <= 2 * NH; n++)
+        for (int k = 1; k <= kmax - 1; k++)
+          for (int j = 1; j <= jmax - 1; j++)
+            for (int i = 1; i <= imax - 1; i++) {
+              double lami = Q(lambdas, g, i, j, k, 1, n), lamj = Q(lambdas, g, i, j, k, 2, n),
+                     lamk = Q(lambdas, g, i, j, k, 3, n);
+              double phixi = 1 + pow(lamj / lami, zeta) + pow(lamk / lami, zeta);
+              double phieta = 1 + pow(lami / lamj, zeta) + pow(lamk / lamj, zeta);
+              double phizeta = 1 + pow(lami / lamk, zeta) + pow(lamj / lamk, zeta);
+              double t;
+              t = rr * lami * phixi / (lami + lamj + lamk);
+              P1(bx_, g, i, j, k, n) = fmax(0.0, (t * t - 1) / 4);
+              t = rr * lamj * phieta / (lami + lamj + lamk);
+              P1(by_, g, i, j, k, n) = fmax(0.0, (t * t - 1) / 4);
+              t = rr * lamk * phizeta / (lami + lamj + lamk);
+              P1(bz_, g, i, j, k, n) = fmax(0.0, (t * t - 1) / 4);
+            }
+    } else if (P.irsop == 3) {
+      for (int n = 0; n <= 2 * NH; n++)
+        for (int k = 1; k <= kmax - 1; k++)
+          for (int j = 1; j <= jmax - 1; j++)
+            for (int i = 1; i <= imax - 1; i++) {
+              double lami = Q(lambdas, g, i, j, k, 1, n), lamj = Q(lambdas, g, i, j, k, 2, n),
+                     lamk = Q(lambdas, g, i, j, k, 3, n);
+              double rji = lamj / lami, rki = lamk / lami, rij = lami / lamj, rkj = lamk / lamj, rik = lami / lamk,
+                     rjk = lamj / lamk;
+              double t;
+              t = rr / (1 + P.psirs * (rji + rki));
+              P1(bx_, g, i, j, k, n) = fmax(0.0, 0.25 * (t * t - 1));
+              t = rr / (1 + P.psirs * (rij + rkj));
+              P1(by_, g, i, j, k, n) = fmax(0.0, 0.25 * (t * t - 1));
+              t = rr / (1 + P.psirs * (rik + rjk));
+              P1(bz_, g, i, j, k, n) = fmax(0.0, 0.25 * (t * t - 1));
+            }
+    }
   }
 }
 
@@ -760,13 +911,79 @@ void or_muscl_b(int idir, int nl, const int32_t* bctopo, int nbcs, const double*
   }
 }
 
-/* roflux_b (src/flux.f90:45-387), mean flow part (k-omega transport of the
- * turbulent variables is not built in this restatement).                  */
+/* turbulence part of roflux_b (src/flux.f90:250-383) for snapshot n */
+static void or_roflux_turb(const double* qr, const double* ql, double* flux, const double* sijk, bg g, int n, int ic,
+                           int jc, int kc, double h0l, double h0r) {
+  double gamma = P.gamma;
+  int nh = n * P.hbmove;
+  for (int k = 1; k <= kc; k++)
+    for (int j = 1; j <= jc; j++)
+      for (int i = 1; i <= ic; i++) {
+        double ubn = 0.0;
+        double kx = M(sijk, g, 1, i, j, k, nh), ky = M(sijk, g, 2, i, j, k, nh), kz = M(sijk, g, 3, i, j, k, nh),
+               ds = M(sijk, g, 4, i, j, k, nh);
+        if (P.moving)
+          ubn = M(sijk, g, 5, i, j, k, nh) * kx + M(sijk, g, 6, i, j, k, nh) * ky + M(sijk, g, 7, i, j, k, nh) * kz;
+        double rhor = Q(qr, g, i, j, k, 1, n), ur = Q(qr, g, i, j, k, 2, n), vr = Q(qr, g, i, j, k, 3, n),
+               wr = Q(qr, g, i, j, k, 4, n), pr = Q(qr, g, i, j, k, 5, n), omer = Q(qr, g, i, j, k, 7, n);
+        double unr = ur * kx + vr * ky + wr * kz - ubn;
+        double tker = Q(qr, g, i, j, k, 6, n);
+        double fr1 = rhor * unr * tker, fr2 = rhor * unr * omer;
+        double rhol = Q(ql, g, i, j, k, 1, n), ul = Q(ql, g, i, j, k, 2, n), vl = Q(ql, g, i, j, k, 3, n),
+               wl = Q(ql, g, i, j, k, 4, n), pl = Q(ql, g, i, j, k, 5, n), omel = Q(ql, g, i, j, k, 7, n);
+        double unl = ul * kx + vl * ky + wl * kz - ubn;
+        double tkel = Q(ql, g, i, j, k, 6, n);
+        double fl1 = rhol * unl * tkel, fl2 = rhol * unl * omel;
+        double drho = rhor - rhol, dp = pr - pl, dun = unr - unl, dtke = tker - tkel, dome = omer - omel;
+        double rhols = sqrt(rhol), rhors = sqrt(rhor);
+        double rho = rhols * rhors;
+        double denom = rhols + rhors;
+        double u = (rhols * ul + rhors * ur) / denom;
+        double v = (rhols * vl + rhors * vr) / denom;
+        double w = (rhols * wl + rhors * wr) / denom;
+        double un = u * kx + v * ky + w * kz;
+        double h0 = (rhols * h0l + rhors * h0r) / denom;
+        double tke = (rhols * tkel + rhors * tker) / denom;
+        double ome = (rhols * omel + rhors * omer) / denom;
+        double q2 = u * u + v * v + w * w;
+        double a2 = (gamma - 1) * (h0 - q2 / 2 - tke);
+        double a = sqrt(a2);
+        double rm123 = fabs(un - ubn), rm4 = fabs(un + a - ubn), rm5 = fabs(un - a - ubn);
+        if (P.efx_setup_t) {
+          double rdelta = P.cntrpy_t * (P.entfx_t_w[0] * P.entfxctf_t + P.entfx_t_w[1] * (sqrt(u * u + v * v + w * w) + a) +
+                                        P.entfx_t_w[2] * 0.2 * (fabs(un) + a + fabs(ubn)));
+          rdelta = fmax(rdelta, 1.e-12);
+          if (rm123 <= rdelta && P.efx_all_t) rm123 = 0.5 * (rm123 * rm123 + rdelta * rdelta) / rdelta;
+          if (rm4 <= rdelta) rm4 = 0.5 * (rm4 * rm4 + rdelta * rdelta) / rdelta;
+          if (rm5 <= rdelta) rm5 = 0.5 * (rm5 * rm5 + rdelta * rdelta) / rdelta;
+        }
+        if (P.eig_limit_t) {
+          if (rm123 <= P.eig_cutoff_t) rm123 = P.eig_cutoff_t;
+          if (P.eig_lim_all_t && rm4 <= P.eig_cutoff_t) rm4 = P.eig_cutoff_t;
+          if (P.eig_lim_all_t && rm5 <= P.eig_cutoff_t) rm5 = P.eig_cutoff_t;
+        }
+        double al1 = rm123 * (drho - dp / a2);
+        double al2 = rm123 * rho;
+        double al3 = rm4 * (dp / a2 + rho * dun / a) / 2;
+        double al4 = rm5 * (dp / a2 - rho * dun / a) / 2;
+        double df1 = (al1 + al3 + al4) * tke + al2 * dtke;
+        double df2 = (al1 + al3 + al4) * ome + al2 * dome;
+        F0(flux, g, i, j, k, 6, n) = (fr1 + fl1 - df1) / 2 * ds;
+        F0(flux, g, i, j, k, 7, n) = (fr2 + fl2 - df2) / 2 * ds;
+      }
+}
+
+/* roflux_b (src/flux.f90:45-387).  With k-omega the turbulent variables
+ * are upwinded with the mean-flow waves (src/flux.f90:250-383); as in the
+ * reference, that second loop reuses h0l/h0r of the last face of the
+ * mean-flow loop of the same snapshot for its Roe-averaged h0.            */
 void or_roflux_b(int idir, const double* qr, const double* ql, double* flux, const double* sijk, int imax, int jmax,
                  int kmax) {
   bg g = mkbg(imax, jmax, kmax);
   int ic = idir == 1 ? imax : imax - 1, jc = idir == 2 ? jmax : jmax - 1, kc = idir == 3 ? kmax : kmax - 1;
   double gamma = P.gamma;
+  int kom = is_kom();
+  double h0l_last = 0.0, h0r_last = 0.0;
   for (int n = 0; n <= 2 * NH; n++) {
     int nh = n * P.hbmove;
     for (int k = 1; k <= kc; k++)
@@ -782,7 +999,7 @@ void or_roflux_b(int idir, const double* qr, const double* ql, double* flux, con
                  wr = Q(qr, g, i, j, k, 4, n), pr = Q(qr, g, i, j, k, 5, n);
           double unr = ur * kx + vr * ky + wr * kz - ubn;
           double q2r = ur * ur + vr * vr + wr * wr;
-          double tker = 0.0;
+          double tker = kom ? Q(qr, g, i, j, k, 6, n) : 0.0;
           double h0r = gamma / (gamma - 1) * pr / rhor + q2r / 2 + tker;
           double rh0r = rhor * h0r;
           fr[0] = rhor * unr;
@@ -794,7 +1011,7 @@ void or_roflux_b(int idir, const double* qr, const double* ql, double* flux, con
                  wl = Q(ql, g, i, j, k, 4, n), pl = Q(ql, g, i, j, k, 5, n);
           double unl = ul * kx + vl * ky + wl * kz - ubn;
           double q2l = ul * ul + vl * vl + wl * wl;
-          double tkel = 0.0;
+          double tkel = kom ? Q(ql, g, i, j, k, 6, n) : 0.0;
           double h0l = gamma / (gamma - 1) * pl / rhol + q2l / 2 + tkel;
           double rh0l = rhol * h0l;
           fl[0] = rhol * unl;
@@ -841,7 +1058,10 @@ void or_roflux_b(int idir, const double* qr, const double* ql, double* flux, con
           df[4] = al1 * (q2 / 2 + tke) + al2 * (u * du + v * dv + w * dw - un * dun + dtke) + al3 * (h0 + a * un) +
                   al4 * (h0 - a * un);
           for (int v5 = 0; v5 < 5; v5++) F0(flux, g, i, j, k, v5 + 1, n) = (fr[v5] + fl[v5] - df[v5]) / 2 * ds;
+          h0l_last = h0l;
+          h0r_last = h0r;
         }
+    if (kom) or_roflux_turb(qr, ql, flux, sijk, g, n, ic, jc, kc, h0l_last, h0r_last);
   }
 }
 
@@ -939,7 +1159,17 @@ void or_tridi(const double* a, const double* b, const double* c, double* x, int 
 
 void or_cirs_b(double* res, const double* dtvol, const double* betaxi, const double* betaeta, const double* betazeta,
                int imax, int jmax, int kmax) {
+  or_cirs_kb(res, dtvol, betaxi, betaeta, betazeta, NULL, NULL, NULL, NULL, imax, jmax, kmax);
+}
+
+/* cirs_b with the k-omega equations (src/cosa.f90:5238-5623): res(6:7) is
+ * scaled by dtvolt; irsop 1 smooths every equation with the same epsilon,
+ * irsop 2/3 smooth 6:7 with tbeta*.                                        */
+void or_cirs_kb(double* res, const double* dtvol, const double* betaxi, const double* betaeta,
+                const double* betazeta, const double* dtvolt, const double* tbetaxi, const double* tbetaeta,
+                const double* tbetazeta, int imax, int jmax, int kmax) {
   bg g = mkbg(imax, jmax, kmax);
+  int kom = is_kom() && dtvolt;
   int im = imax - 1, jm = jmax - 1, km = kmax - 1;
   int nmax = imax > jmax ? imax : jmax;
   if (kmax > nmax) nmax = kmax;
@@ -950,13 +1180,25 @@ void or_cirs_b(double* res, const double* dtvol, const double* betaxi, const dou
       for (int k = 1; k <= km; k++)
         for (int j = 1; j <= jm; j++)
           for (int i = 1; i <= im; i++) Q(res, g, i, j, k, ipde, n) = Q(res, g, i, j, k, ipde, n) * P1(dtvol, g, i, j, k, n);
-  const double* beta[4] = {NULL, betaxi, betaeta, betazeta};
+  if (kom)
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int ipde = 6; ipde <= NPDE; ipde++)
+        for (int k = 1; k <= km; k++)
+          for (int j = 1; j <= jm; j++)
+            for (int i = 1; i <= im; i++)
+              Q(res, g, i, j, k, ipde, n) = Q(res, g, i, j, k, ipde, n) * P1(dtvolt, g, i, j, k, n);
+  const double* betas[2][4] = {{NULL, betaxi, betaeta, betazeta}, {NULL, tbetaxi, tbetaeta, tbetazeta}};
+  int nsets = P.irsop == 1 ? 1 : (kom ? 2 : 1);
+  for (int set = 0; set < nsets; set++)
   for (int idir = 1; idir <= 3; idir++) {
+    const double* const* beta = betas[set];
+    int v0 = set == 0 ? 1 : 6;
+    int v1 = set == 1 ? NPDE : (P.irsop == 1 && kom ? NPDE : 5);
     int L = idir == 1 ? im : (idir == 2 ? jm : km);
     int T1 = idir == 1 ? jm : im;
     int T2 = idir == 3 ? jm : km;
     for (int n = 0; n <= 2 * NH; n++)
-      for (int ipde = 1; ipde <= 5; ipde++)
+      for (int ipde = v0; ipde <= v1; ipde++)
         for (int t2 = 1; t2 <= T2; t2++)
           for (int t1 = 1; t1 <= T1; t1++) {
             for (int a = 1; a <= L; a++) {
@@ -988,6 +1230,13 @@ void or_cirs_b(double* res, const double* dtvol, const double* betaxi, const dou
       for (int k = 1; k <= km; k++)
         for (int j = 1; j <= jm; j++)
           for (int i = 1; i <= im; i++) Q(res, g, i, j, k, ipde, n) = Q(res, g, i, j, k, ipde, n) / P1(dtvol, g, i, j, k, n);
+  if (kom)
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int ipde = 6; ipde <= NPDE; ipde++)
+        for (int k = 1; k <= km; k++)
+          for (int j = 1; j <= jm; j++)
+            for (int i = 1; i <= im; i++)
+              Q(res, g, i, j, k, ipde, n) = Q(res, g, i, j, k, ipde, n) / P1(dtvolt, g, i, j, k, n);
   free(wa); free(wb); free(wc); free(wx);
 }
 
@@ -1005,6 +1254,634 @@ void or_update_b(double* res, double* q, const double* qold, const double* dtvol
           for (int i = 1; i <= imax - 1; i++)
             Q(q, g, i, j, k, ipde, n) =
                 Q(qold, g, i, j, k, ipde, n) - ar * Q(res, g, i, j, k, ipde, n) * P1(dtvol, g, i, j, k, n);
+}
+
+/* ---------------------------------------------------------------------- */
+/* viscous fluxes and Wilcox k-omega: per-block restatements              */
+
+/* metric derivative arrays xider(3,imax,jmax,kmax,0:2N*hbmove) */
+#define MD(a, g, c, i, j, k, nh)                                                        \
+  (a)[((c) - 1) + 3 * ((ll)((i) - 1) + (ll)(g).imax * ((ll)((j) - 1) + (ll)(g).jmax *    \
+                                                        ((ll)((k) - 1) + (ll)(g).kmax * (nh))))]
+
+/* the strong (wall-like) BC types of muscl_b and bq_face/bq_der
+ * (src/flux.f90:1320, 1898, 2471)                                         */
+static int wall_like(int t) {
+  return t == 13 || t == 14 || t / 100 == 15 || t == 22 || t == 23 || t == 92 || t == 93 || t == 5;
+}
+
+/* Sutherland viscosity (src/flux.f90:936-937) */
+static double suth(double t) { return (P.stemp + 1) / (t + P.stemp) * pow(t, 1.5); }
+
+/* face ranges of direction idir: idir index 1..ijkmax, others 1..n-1 */
+static void face_ranges(int idir, bg g, int* ic, int* jc, int* kc) {
+  *ic = idir == 1 ? g.imax : g.imax - 1;
+  *jc = idir == 2 ? g.jmax : g.jmax - 1;
+  *kc = idir == 3 ? g.kmax : g.kmax - 1;
+}
+
+/* bc_walvis_15 (src/bc.f90:2144-2492): no-slip wall (wall velocity from
+ * the face metrics on moving grids), pressure extrapolated linearly and
+ * density from isentropic scaling; k = 0 and omega from Wilcox's rough
+ * wall (wallwilc) or Menter's formula on the wall halo layers.           */
+static void or_bc_walvis15(double* q, const double* si, const double* sj, const double* sk, const double* dist,
+                           const int32_t* t, bg g) {
+  int ijkmax[4] = {0, g.imax, g.jmax, g.kmax};
+  bcgeo b = bc_decode(t);
+  int ibcpt, ibcpt2, ibcn, ibcn2, ibcm;
+  const double small = 1.e-12, smasq = 1.e-24;
+  double uw = 0.0, vw = 0.0, ww = 0.0;
+  const double mor = P.machfs / P.reyno;
+  if (b.inrout == 1) {
+    ibcpt = 0; ibcpt2 = -1; ibcn = 1; ibcn2 = 2; ibcm = 1;
+  } else {
+    ibcpt = ijkmax[b.idir]; ibcpt2 = ijkmax[b.idir] + 1; ibcn = ijkmax[b.idir] - 1; ibcn2 = ijkmax[b.idir] - 2;
+    ibcm = ijkmax[b.idir];
+  }
+  const double* s = b.idir == 1 ? si : (b.idir == 2 ? sj : sk);
+  for (int n = 0; n <= 2 * NH; n++) {
+    int nh = n * P.hbmove;
+    for (int i3 = b.istrt[b.ic3]; i3 <= b.iend[b.ic3]; i3++)
+      for (int i2 = b.istrt[b.ic2]; i2 <= b.iend[b.ic2]; i2++) {
+        int ib, jb, kb, ib2, jb2, kb2, in, jn, kn, in2, jn2, kn2, im, jm, km;
+        bc_ijk(&b, ibcpt, i2, i3, &ib, &jb, &kb);
+        bc_ijk(&b, ibcpt2, i2, i3, &ib2, &jb2, &kb2);
+        bc_ijk(&b, ibcn, i2, i3, &in, &jn, &kn);
+        bc_ijk(&b, ibcn2, i2, i3, &in2, &jn2, &kn2);
+        bc_ijk(&b, ibcm, i2, i3, &im, &jm, &km);
+        if (P.moving) {
+          uw = M(s, g, 5, im, jm, km, nh);
+          vw = M(s, g, 6, im, jm, km, nh);
+          ww = M(s, g, 7, im, jm, km, nh);
+        }
+        double p1 = Q(q, g, in, jn, kn, 5, n), p2 = Q(q, g, in2, jn2, kn2, 5, n);
+        double pw = 0.5 * (3 * p1 - p2);
+        double rhow = pow(pw / p1, 1 / P.gamma) * Q(q, g, in, jn, kn, 1, n);
+        double hv[5] = {rhow, uw, vw, ww, pw};
+        for (int v = 0; v < 5; v++) {
+          Q(q, g, ib, jb, kb, v + 1, n) = hv[v];
+          Q(q, g, ib2, jb2, kb2, v + 1, n) = hv[v];
+        }
+      }
+  }
+  if (!is_kom()) return;
+  for (int n = 0; n <= 2 * NH; n++) {
+    int nh = n * P.hbmove;
+    for (int i3 = b.istrt[b.ic3]; i3 <= b.iend[b.ic3]; i3++)
+      for (int i2 = b.istrt[b.ic2]; i2 <= b.iend[b.ic2]; i2++) {
+        int ib, jb, kb, ib2, jb2, kb2, in, jn, kn, im, jm, km;
+        bc_ijk(&b, ibcpt, i2, i3, &ib, &jb, &kb);
+        bc_ijk(&b, ibcpt2, i2, i3, &ib2, &jb2, &kb2);
+        bc_ijk(&b, ibcn, i2, i3, &in, &jn, &kn);
+        bc_ijk(&b, ibcm, i2, i3, &im, &jm, &km);
+        double rhow = Q(q, g, ib, jb, kb, 1, n), pw = Q(q, g, ib, jb, kb, 5, n);
+        double tw = pw * P.gamma / rhow;
+        double muw = suth(tw);
+        double omew;
+        if (P.wallwilc) {
+          double nx = M(s, g, 1, im, jm, km, nh), ny = M(s, g, 2, im, jm, km, nh), nz = M(s, g, 3, im, jm, km, nh);
+          if (P.moving) {
+            uw = M(s, g, 5, im, jm, km, nh);
+            vw = M(s, g, 6, im, jm, km, nh);
+            ww = M(s, g, 7, im, jm, km, nh);
+          }
+          double uc = Q(q, g, in, jn, kn, 2, n) - uw, vc = Q(q, g, in, jn, kn, 3, n) - vw,
+                 wc = Q(q, g, in, jn, kn, 4, n) - ww;
+          double uorth = nx * uc + ny * vc + nz * wc;
+          double a1 = uc - uorth * nx, a2 = vc - uorth * ny, a3 = wc - uorth * nz;
+          double upar = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+          double dupardn = upar / (V1(dist, g, in, jn, kn) + small);
+          double tauw = muw * fabs(dupardn);
+          double rkrpls;
+          if (P.roughk > 0.0) rkrpls = fmax(1.0, sqrt(rhow * tauw / mor) * P.roughk / muw);
+          else rkrpls = 1.0;
+          double sr = rkrpls >= 25.0 ? 100.0 / rkrpls : 2500.0 / (rkrpls * rkrpls);
+          omew = fabs(dupardn) * sr;
+        } else {
+          double d = V1(dist, g, in, jn, kn);
+          omew = 60.0 * muw / (P.bkw * rhow * (d * d) + smasq) * P.machfs / P.reyno;
+        }
+        Q(q, g, ib, jb, kb, 6, n) = 0.0;
+        Q(q, g, ib, jb, kb, 7, n) = omew;
+        Q(q, g, ib2, jb2, kb2, 6, n) = 0.0;
+        Q(q, g, ib2, jb2, kb2, 7, n) = omew;
+      }
+  }
+}
+
+/* bq_face_{i,j,k} (src/flux.f90:2432-2801): face averages of q into qf
+ * (and of mut into mutf when calmut); wall-like patches of direction idir
+ * take the halo value at the wall face.                                   */
+void or_q_face_b(int idir, const int32_t* bctopo, int nbcs, const double* q, double* qf, const double* mut,
+                 double* mutf, int calmut, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  int ic, jc, kc;
+  face_ranges(idir, g, &ic, &jc, &kc);
+  int di = idir == 1, dj = idir == 2, dk = idir == 3;
+  int ijkmax[4] = {0, imax, jmax, kmax};
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int ipde = 1; ipde <= NPDE; ipde++)
+      for (int k = 1; k <= kc; k++)
+        for (int j = 1; j <= jc; j++)
+          for (int i = 1; i <= ic; i++)
+            Q(qf, g, i, j, k, ipde, n) = (Q(q, g, i - di, j - dj, k - dk, ipde, n) + Q(q, g, i, j, k, ipde, n)) / 2;
+  for (int pass = 0; pass < (calmut ? 2 : 1); pass++) {
+    if (pass == 1)
+      for (int n = 0; n <= 2 * NH; n++)
+        for (int k = 1; k <= kc; k++)
+          for (int j = 1; j <= jc; j++)
+            for (int i = 1; i <= ic; i++)
+              F0(mutf, g, i, j, k, 1, n) = (S1(mut, g, i - di, j - dj, k - dk, n) + S1(mut, g, i, j, k, n)) / 2;
+    for (int ibc = 0; ibc < nbcs; ibc++) {
+      const int32_t* t = bctopo + 10 * ibc;
+      if (t[1] != idir || !wall_like(t[0])) continue;
+      bcgeo b = bc_decode(t);
+      int bcpt = b.inrout == 1 ? 0 : ijkmax[idir];
+      int face = b.inrout == 1 ? 1 : ijkmax[idir];
+      for (int n = 0; n <= 2 * NH; n++)
+        for (int i3 = b.istrt[b.ic3]; i3 <= b.iend[b.ic3]; i3++)
+          for (int i2 = b.istrt[b.ic2]; i2 <= b.iend[b.ic2]; i2++) {
+            int ia, ja, ka, iff, jf, kf;
+            bc_ijk(&b, bcpt, i2, i3, &ia, &ja, &ka);
+            bc_ijk(&b, face, i2, i3, &iff, &jf, &kf);
+            if (pass == 0)
+              for (int ipde = 1; ipde <= NPDE; ipde++) Q(qf, g, iff, jf, kf, ipde, n) = Q(q, g, ia, ja, ka, ipde, n);
+            else
+              F0(mutf, g, iff, jf, kf, 1, n) = S1(mut, g, ia, ja, ka, n);
+          }
+    }
+  }
+}
+
+/* bq_der_{i,j,k} (src/flux.f90:1848-2390): xi, eta, zeta differences of
+ * q at the faces of direction idir.  Across the face: q(p) - q(m); along
+ * a transverse direction t: (q(m)+q(p)+q(p+t)+q(m+t))/4 minus
+ * (q(m-t)+q(p-t)+q(p)+q(m))/4.  At wall-like patches the normal difference
+ * is the one-sided (-8 q_w + 9 q_1 - q_2)/3 and the transverse ones use the
+ * halo layer with one-sided ends.                                         */
+void or_q_der_b(int idir, const int32_t* bctopo, int nbcs, const double* q, double* dqxi, double* dqeta,
+                double* dqzeta, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  int ic, jc, kc;
+  face_ranges(idir, g, &ic, &jc, &kc);
+  int ijkmax[4] = {0, imax, jmax, kmax};
+  double* outs[4] = {NULL, dqxi, dqeta, dqzeta};
+  int e[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int ipde = 1; ipde <= NPDE; ipde++)
+      for (int k = 1; k <= kc; k++)
+        for (int j = 1; j <= jc; j++)
+          for (int i = 1; i <= ic; i++) {
+            int im = i - e[idir][0], jm = j - e[idir][1], km = k - e[idir][2];
+            for (int d = 1; d <= 3; d++) {
+              double val;
+              if (d == idir) {
+                val = Q(q, g, i, j, k, ipde, n) - Q(q, g, im, jm, km, ipde, n);
+              } else {
+                int ti = e[d][0], tj = e[d][1], tk = e[d][2];
+                double q2 = (Q(q, g, im, jm, km, ipde, n) + Q(q, g, i, j, k, ipde, n) +
+                             Q(q, g, i + ti, j + tj, k + tk, ipde, n) + Q(q, g, im + ti, jm + tj, km + tk, ipde, n)) /
+                            4;
+                double q1 = (Q(q, g, im - ti, jm - tj, km - tk, ipde, n) + Q(q, g, i - ti, j - tj, k - tk, ipde, n) +
+                             Q(q, g, i, j, k, ipde, n) + Q(q, g, im, jm, km, ipde, n)) /
+                            4;
+                val = q2 - q1;
+              }
+              Q(outs[d], g, i, j, k, ipde, n) = val;
+            }
+          }
+  for (int ibc = 0; ibc < nbcs; ibc++) {
+    const int32_t* t = bctopo + 10 * ibc;
+    if (t[1] != idir || !wall_like(t[0])) continue;
+    bcgeo b = bc_decode(t);
+    int aux, n1, n2, face;
+    double sgnm;
+    if (b.inrout == 1) {
+      aux = 0; n1 = 1; n2 = 2; face = 1; sgnm = 1.0;
+    } else {
+      aux = ijkmax[idir]; n1 = ijkmax[idir] - 1; n2 = ijkmax[idir] - 2; face = ijkmax[idir]; sgnm = -1.0;
+    }
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int ipde = 1; ipde <= NPDE; ipde++)
+        for (int i3 = b.istrt[b.ic3]; i3 <= b.iend[b.ic3]; i3++)
+          for (int i2 = b.istrt[b.ic2]; i2 <= b.iend[b.ic2]; i2++) {
+            int ia, ja, ka, i1_, j1_, k1_, i2_, j2_, k2_, iff, jf, kf;
+            bc_ijk(&b, aux, i2, i3, &ia, &ja, &ka);
+            bc_ijk(&b, n1, i2, i3, &i1_, &j1_, &k1_);
+            bc_ijk(&b, n2, i2, i3, &i2_, &j2_, &k2_);
+            bc_ijk(&b, face, i2, i3, &iff, &jf, &kf);
+            Q(outs[idir], g, iff, jf, kf, ipde, n) =
+                sgnm / 3 * (-Q(q, g, ia, ja, ka, ipde, n) * 8 + Q(q, g, i1_, j1_, k1_, ipde, n) * 9 -
+                            Q(q, g, i2_, j2_, k2_, ipde, n));
+            for (int d = 1; d <= 3; d++) {
+              if (d == idir) continue;
+              int ti = e[d][0], tj = e[d][1], tk = e[d][2];
+              int c = d == 1 ? ia : (d == 2 ? ja : ka);
+              double qc = Q(q, g, ia, ja, ka, ipde, n);
+              double qp1 = Q(q, g, ia + ti, ja + tj, ka + tk, ipde, n);
+              double qm1 = Q(q, g, ia - ti, ja - tj, ka - tk, ipde, n);
+              double q2, q1;
+              if (c == b.istrt[d]) {
+                q2 = (qc + qp1) / 2;
+                q1 = (qc * 3 - qp1) / 2;
+              } else if (c == b.iend[d]) {
+                q2 = (qc * 3 - qm1) / 2;
+                q1 = (qc + qm1) / 2;
+              } else {
+                q2 = (qc + qp1) / 2;
+                q1 = (qm1 + qc) / 2;
+              }
+              Q(outs[d], g, iff, jf, kf, ipde, n) = q2 - q1;
+            }
+          }
+  }
+}
+
+/* b_rtst (src/flux.f90:2846-2947): Reynolds stresses at the faces of
+ * direction idir into tau1(:,:,:,1:6) (txx, txy, txz, tyy, tyz, tzz).     */
+void or_rtst_b(int idir, const double* qf, const double* dqxi, const double* dqeta, const double* dqzeta,
+               const double* xider, const double* etader, const double* zetader, double* tau1, const double* mutf,
+               int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  int ic, jc, kc;
+  face_ranges(idir, g, &ic, &jc, &kc);
+  const double fact = 2.0 / 3.0;
+  for (int n = 0; n <= 2 * NH; n++) {
+    int nh = n * P.hbmove;
+    for (int k = 1; k <= kc; k++)
+      for (int j = 1; j <= jc; j++)
+        for (int i = 1; i <= ic; i++) {
+          double rho = Q(qf, g, i, j, k, 1, n), tke = Q(qf, g, i, j, k, 6, n);
+          double mut = F0(mutf, g, i, j, k, 1, n);
+          double xix = MD(xider, g, 1, i, j, k, nh), xiy = MD(xider, g, 2, i, j, k, nh), xiz = MD(xider, g, 3, i, j, k, nh);
+          double etax = MD(etader, g, 1, i, j, k, nh), etay = MD(etader, g, 2, i, j, k, nh),
+                 etaz = MD(etader, g, 3, i, j, k, nh);
+          double zetax = MD(zetader, g, 1, i, j, k, nh), zetay = MD(zetader, g, 2, i, j, k, nh),
+                 zetaz = MD(zetader, g, 3, i, j, k, nh);
+          double uxi = Q(dqxi, g, i, j, k, 2, n), ueta = Q(dqeta, g, i, j, k, 2, n), uzeta = Q(dqzeta, g, i, j, k, 2, n);
+          double vxi = Q(dqxi, g, i, j, k, 3, n), veta = Q(dqeta, g, i, j, k, 3, n), vzeta = Q(dqzeta, g, i, j, k, 3, n);
+          double wxi = Q(dqxi, g, i, j, k, 4, n), weta = Q(dqeta, g, i, j, k, 4, n), wzeta = Q(dqzeta, g, i, j, k, 4, n);
+          double dudx = uxi * xix + ueta * etax + uzeta * zetax;
+          double dudy = uxi * xiy + ueta * etay + uzeta * zetay;
+          double dudz = uxi * xiz + ueta * etaz + uzeta * zetaz;
+          double dvdx = vxi * xix + veta * etax + vzeta * zetax;
+          double dvdy = vxi * xiy + veta * etay + vzeta * zetay;
+          double dvdz = vxi * xiz + veta * etaz + vzeta * zetaz;
+          double dwdx = wxi * xix + weta * etax + wzeta * zetax;
+          double dwdy = wxi * xiy + weta * etay + wzeta * zetay;
+          double dwdz = wxi * xiz + weta * etaz + wzeta * zetaz;
+          double divv = (dudx + dvdy + dwdz) * (double)P.turcmp;
+          double txx = 2 * mut * (dudx - divv / 3);
+          double txy = mut * (dudy + dvdx);
+          double txz = mut * (dudz + dwdx);
+          double tyy = 2 * mut * (dvdy - divv / 3);
+          double tyz = mut * (dvdz + dwdy);
+          double tzz = 2 * mut * (dwdz - divv / 3);
+          double mr = P.machfs / P.reyno;
+          Q(tau1, g, i, j, k, 1, n) = mr * txx - fact * rho * tke;
+          Q(tau1, g, i, j, k, 2, n) = mr * txy;
+          Q(tau1, g, i, j, k, 3, n) = mr * txz;
+          Q(tau1, g, i, j, k, 4, n) = mr * tyy - fact * rho * tke;
+          Q(tau1, g, i, j, k, 5, n) = mr * tyz;
+          Q(tau1, g, i, j, k, 6, n) = mr * tzz - fact * rho * tke;
+        }
+  }
+}
+
+/* vflux_b (src/flux.f90:878-1182), Wilcox k-omega and laminar branches:
+ * viscous and turbulent diffusion fluxes at the faces of direction idir.
+ * On entry flux(:,:,:,1) holds mut at the faces (q_face with calmut).
+ * Kept as in the reference: the omega diffusion uses dk/dz in place of
+ * domega/dz, and the energy flux's z term repeats the y-row stresses.    */
+void or_vflux_b(int idir, const double* qf, const double* dqxi, const double* dqeta, const double* dqzeta,
+                const double* tau1, double* flux, const double* xider, const double* etader, const double* zetader,
+                const double* sijk, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  int ic, jc, kc;
+  face_ranges(idir, g, &ic, &jc, &kc);
+  double gamma = P.gamma;
+  if (is_kom()) {
+    for (int n = 0; n <= 2 * NH; n++) {
+      int nh = n * P.hbmove;
+      for (int k = 1; k <= kc; k++)
+        for (int j = 1; j <= jc; j++)
+          for (int i = 1; i <= ic; i++) {
+            double kx = M(sijk, g, 1, i, j, k, nh), ky = M(sijk, g, 2, i, j, k, nh), kz = M(sijk, g, 3, i, j, k, nh),
+                   ds = M(sijk, g, 4, i, j, k, nh);
+            double rho = Q(qf, g, i, j, k, 1, n), p = Q(qf, g, i, j, k, 5, n);
+            double t = gamma * p / rho;
+            double mu = suth(t);
+            double mut = F0(flux, g, i, j, k, 1, n);
+            double xix = MD(xider, g, 1, i, j, k, nh), xiy = MD(xider, g, 2, i, j, k, nh),
+                   xiz = MD(xider, g, 3, i, j, k, nh);
+            double etax = MD(etader, g, 1, i, j, k, nh), etay = MD(etader, g, 2, i, j, k, nh),
+                   etaz = MD(etader, g, 3, i, j, k, nh);
+            double zetax = MD(zetader, g, 1, i, j, k, nh), zetay = MD(zetader, g, 2, i, j, k, nh),
+                   zetaz = MD(zetader, g, 3, i, j, k, nh);
+            double tkexi = Q(dqxi, g, i, j, k, 6, n), tkeeta = Q(dqeta, g, i, j, k, 6, n),
+                   tkezeta = Q(dqzeta, g, i, j, k, 6, n);
+            double omexi = Q(dqxi, g, i, j, k, 7, n), omeeta = Q(dqeta, g, i, j, k, 7, n),
+                   omezeta = Q(dqzeta, g, i, j, k, 7, n);
+            double dtkedx = tkexi * xix + tkeeta * etax + tkezeta * zetax;
+            double dtkedy = tkexi * xiy + tkeeta * etay + tkezeta * zetay;
+            double dtkedz = tkexi * xiz + tkeeta * etaz + tkezeta * zetaz;
+            double domedx = omexi * xix + omeeta * etax + omezeta * zetax;
+            double domedy = omexi * xiy + omeeta * etay + omezeta * zetay;
+            double txx = Q(tau1, g, i, j, k, 1, n), txy = Q(tau1, g, i, j, k, 2, n), txz = Q(tau1, g, i, j, k, 3, n);
+            double tyy = Q(tau1, g, i, j, k, 4, n), tyz = Q(tau1, g, i, j, k, 5, n), tzz = Q(tau1, g, i, j, k, 6, n);
+            F0(flux, g, i, j, k, 2, n) = -((txx * kx + txy * ky + txz * kz) * ds);
+            F0(flux, g, i, j, k, 3, n) = -((txy * kx + tyy * ky + tyz * kz) * ds);
+            F0(flux, g, i, j, k, 4, n) = -((txz * kx + tyz * ky + tzz * kz) * ds);
+            double ctke = (mu + P.sigk * mut) * P.machfs / P.reyno;
+            double come = (mu + P.sigw * mut) * P.machfs / P.reyno;
+            F0(flux, g, i, j, k, 5, n) = -(ctke * (dtkedx * kx + dtkedy * ky + dtkedz * kz) * ds);
+            F0(flux, g, i, j, k, 6, n) = -(ctke * (dtkedx * kx + dtkedy * ky + dtkedz * kz) * ds);
+            F0(flux, g, i, j, k, 7, n) = -(come * (domedx * kx + domedy * ky + dtkedz * kz) * ds);
+          }
+    }
+  } else {
+    for (int n = 0; n <= 2 * NH; n++)
+      for (int k = 1; k <= kc; k++)
+        for (int j = 1; j <= jc; j++)
+          for (int i = 1; i <= ic; i++)
+            for (int v = 1; v <= 5; v++) F0(flux, g, i, j, k, v, n) = 0;
+  }
+  for (int n = 0; n <= 2 * NH; n++) {
+    int nh = n * P.hbmove;
+    for (int k = 1; k <= kc; k++)
+      for (int j = 1; j <= jc; j++)
+        for (int i = 1; i <= ic; i++) {
+          double kx = M(sijk, g, 1, i, j, k, nh), ky = M(sijk, g, 2, i, j, k, nh), kz = M(sijk, g, 3, i, j, k, nh),
+                 ds = M(sijk, g, 4, i, j, k, nh);
+          double rho = Q(qf, g, i, j, k, 1, n), u = Q(qf, g, i, j, k, 2, n), v = Q(qf, g, i, j, k, 3, n),
+                 w = Q(qf, g, i, j, k, 4, n), p = Q(qf, g, i, j, k, 5, n);
+          double t = gamma * p / rho;
+          double mu = suth(t);
+          double mut = F0(flux, g, i, j, k, 1, n);
+          double xix = MD(xider, g, 1, i, j, k, nh), xiy = MD(xider, g, 2, i, j, k, nh), xiz = MD(xider, g, 3, i, j, k, nh);
+          double etax = MD(etader, g, 1, i, j, k, nh), etay = MD(etader, g, 2, i, j, k, nh),
+                 etaz = MD(etader, g, 3, i, j, k, nh);
+          double zetax = MD(zetader, g, 1, i, j, k, nh), zetay = MD(zetader, g, 2, i, j, k, nh),
+                 zetaz = MD(zetader, g, 3, i, j, k, nh);
+          double uxi = Q(dqxi, g, i, j, k, 2, n), ueta = Q(dqeta, g, i, j, k, 2, n), uzeta = Q(dqzeta, g, i, j, k, 2, n);
+          double vxi = Q(dqxi, g, i, j, k, 3, n), veta = Q(dqeta, g, i, j, k, 3, n), vzeta = Q(dqzeta, g, i, j, k, 3, n);
+          double wxi = Q(dqxi, g, i, j, k, 4, n), weta = Q(dqeta, g, i, j, k, 4, n), wzeta = Q(dqzeta, g, i, j, k, 4, n);
+          double txi = t * (Q(dqxi, g, i, j, k, 5, n) / p - Q(dqxi, g, i, j, k, 1, n) / rho);
+          double teta = t * (Q(dqeta, g, i, j, k, 5, n) / p - Q(dqeta, g, i, j, k, 1, n) / rho);
+          double tzeta = t * (Q(dqzeta, g, i, j, k, 5, n) / p - Q(dqzeta, g, i, j, k, 1, n) / rho);
+          double dudx = uxi * xix + ueta * etax + uzeta * zetax;
+          double dudy = uxi * xiy + ueta * etay + uzeta * zetay;
+          double dudz = uxi * xiz + ueta * etaz + uzeta * zetaz;
+          double dvdx = vxi * xix + veta * etax + vzeta * zetax;
+          double dvdy = vxi * xiy + veta * etay + vzeta * zetay;
+          double dvdz = vxi * xiz + veta * etaz + vzeta * zetaz;
+          double dwdx = wxi * xix + weta * etax + wzeta * zetax;
+          double dwdy = wxi * xiy + weta * etay + wzeta * zetay;
+          double dwdz = wxi * xiz + weta * etaz + wzeta * zetaz;
+          double dtdx = txi * xix + teta * etax + tzeta * zetax;
+          double dtdy = txi * xiy + teta * etay + tzeta * zetay;
+          double dtdz = txi * xiz + teta * etaz + tzeta * zetaz;
+          double divv = dudx + dvdy + dwdz;
+          double mr = P.machfs / P.reyno;
+          double txx = mr * 2 * mu * (dudx - divv / 3);
+          double txy = mr * mu * (dudy + dvdx);
+          double txz = mr * mu * (dudz + dwdx);
+          double tyy = mr * 2 * mu * (dvdy - divv / 3);
+          double tyz = mr * mu * (dvdz + dwdy);
+          double tzz = mr * 2 * mu * (dwdz - divv / 3);
+          double cq = mr * (mu / P.pranl + mut / P.prant) / (gamma - 1);
+          double qx = -(cq * dtdx), qy = -(cq * dtdy), qz = -(cq * dtdz);
+          F0(flux, g, i, j, k, 1, n) = 0;
+          F0(flux, g, i, j, k, 2, n) = F0(flux, g, i, j, k, 2, n) - (txx * kx + txy * ky + txz * kz) * ds;
+          F0(flux, g, i, j, k, 3, n) = F0(flux, g, i, j, k, 3, n) - (txy * kx + tyy * ky + tyz * kz) * ds;
+          F0(flux, g, i, j, k, 4, n) = F0(flux, g, i, j, k, 4, n) - (txz * kx + tyz * ky + tzz * kz) * ds;
+          F0(flux, g, i, j, k, 5, n) = F0(flux, g, i, j, k, 5, n) -
+                                       ((u * txx + v * txy + w * txz) * kx + (u * txy + v * tyy + w * tyz) * ky +
+                                        (u * txy + v * tyy + w * tyz) * kz) * ds +
+                                       (qx * kx + qy * ky + qz * kz) * ds;
+        }
+  }
+}
+
+/* bq_der_c{i,j,k} (src/flux.f90:2979-3131): cell-centre differences of the
+ * face averages along idir, velocity components (kom) only.              */
+void or_q_der_c_b(int idir, const double* qf, double* dqc, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  int di = idir == 1, dj = idir == 2, dk = idir == 3;
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int ipde = 2; ipde <= 4; ipde++)
+      for (int k = 1; k <= kmax - 1; k++)
+        for (int j = 1; j <= jmax - 1; j++)
+          for (int i = 1; i <= imax - 1; i++)
+            Q(dqc, g, i, j, k, ipde, n) = Q(qf, g, i + di, j + dj, k + dk, ipde, n) - Q(qf, g, i, j, k, ipde, n);
+}
+
+/* turprod_b (src/src.f90:40-175): P_d (second, low-round-off form),
+ * div(u) and delpls = 2/3 max(div u, 0) at cell centres.                   */
+void or_turprod_b(const double* q, const double* dqxi, const double* dqeta, const double* dqzeta, double* prd,
+                  double* divvel, double* delpls, const double* const* xd, int imax, int jmax, int kmax) {
+  /* xd: xideri, etaderi, zetaderi, xiderj, etaderj, zetaderj, xiderk, etaderk, zetaderk */
+  bg g = mkbg(imax, jmax, kmax);
+  const double fact = 2.0 / 3.0;
+  for (int n = 0; n <= 2 * NH; n++) {
+    int nh = n * P.hbmove;
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          double xix = (MD(xd[0], g, 1, i + 1, j, k, nh) + MD(xd[0], g, 1, i, j, k, nh)) / 2;
+          double xiy = (MD(xd[3], g, 2, i, j + 1, k, nh) + MD(xd[3], g, 2, i, j, k, nh)) / 2;
+          double xiz = (MD(xd[6], g, 3, i, j, k + 1, nh) + MD(xd[6], g, 3, i, j, k, nh)) / 2;
+          double etax = (MD(xd[1], g, 1, i + 1, j, k, nh) + MD(xd[1], g, 1, i, j, k, nh)) / 2;
+          double etay = (MD(xd[4], g, 2, i, j + 1, k, nh) + MD(xd[4], g, 2, i, j, k, nh)) / 2;
+          double etaz = (MD(xd[7], g, 3, i, j, k + 1, nh) + MD(xd[7], g, 3, i, j, k, nh)) / 2;
+          double zetax = (MD(xd[2], g, 1, i + 1, j, k, nh) + MD(xd[2], g, 1, i, j, k, nh)) / 2;
+          double zetay = (MD(xd[5], g, 2, i, j + 1, k, nh) + MD(xd[5], g, 2, i, j, k, nh)) / 2;
+          double zetaz = (MD(xd[8], g, 3, i, j, k + 1, nh) + MD(xd[8], g, 3, i, j, k, nh)) / 2;
+          double uxi = Q(dqxi, g, i, j, k, 2, n), ueta = Q(dqeta, g, i, j, k, 2, n), uzeta = Q(dqzeta, g, i, j, k, 2, n);
+          double vxi = Q(dqxi, g, i, j, k, 3, n), veta = Q(dqeta, g, i, j, k, 3, n), vzeta = Q(dqzeta, g, i, j, k, 3, n);
+          double wxi = Q(dqxi, g, i, j, k, 4, n), weta = Q(dqeta, g, i, j, k, 4, n), wzeta = Q(dqzeta, g, i, j, k, 4, n);
+          double dudx = uxi * xix + ueta * etax + uzeta * zetax;
+          double dudy = uxi * xiy + ueta * etay + uzeta * zetay;
+          double dudz = uxi * xiz + ueta * etaz + uzeta * zetaz;
+          double dvdx = vxi * xix + veta * etax + vzeta * zetax;
+          double dvdy = vxi * xiy + veta * etay + vzeta * zetay;
+          double dvdz = vxi * xiz + veta * etaz + vzeta * zetaz;
+          double dwdx = wxi * xix + weta * etax + wzeta * zetax;
+          double dwdy = wxi * xiy + weta * etay + wzeta * zetay;
+          double dwdz = wxi * xiz + weta * etaz + wzeta * zetaz;
+          double divv = (dudx + dvdy + dwdz) * (double)P.turcmp;
+          double txx = dudx, txy = (dudy + dvdx) / 2, txz = (dudz + dwdx) / 2, tyy = dvdy, tyz = (dvdz + dwdy) / 2,
+                 tzz = dwdz;
+          double prod = 2 * (txx * txx + tyy * tyy + tzz * tzz + 2 * (txy * txy + txz * txz + tyz * tyz) -
+                             divv * divv / 3);
+          (void)q;
+          S1(prd, g, i, j, k, n) = prod;
+          S1(divvel, g, i, j, k, n) = divv;
+          S1(delpls, g, i, j, k, n) = fact * fmax(divv, 0.0);
+        }
+  }
+}
+
+/* srckw_b (src/src.f90:209-272): Wilcox k-omega sources into res(6:7);
+ * q primitive.                                                           */
+void or_srckw_b(const double* q, const double* mutg, const double* prd, const double* divvel, double* res,
+                const double* vol, int nl, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  const double fact = 2.0 / 3.0, mor = P.machfs / P.reyno;
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          double rho = Q(q, g, i, j, k, 1, n), tke = Q(q, g, i, j, k, 6, n), ome = Q(q, g, i, j, k, 7, n);
+          double mut = S1(mutg, g, i, j, k, n);
+          double divv = S1(divvel, g, i, j, k, n);
+          double prod = S1(prd, g, i, j, k, n), prodw = S1(prd, g, i, j, k, n);
+          if (P.lim_prodtke && (nl == 1 || P.prd_all_lvls)) {
+            double tstdis = P.prdlim * P.bstrkw * rho * tke * ome;
+            if (mor * mut * prod > tstdis) {
+              prod = tstdis / mor / mut;
+              if (P.lim_prodomega) prodw = prod;
+            }
+          }
+          double v = V1(vol, g, i, j, k);
+          Q(res, g, i, j, k, 6, n) =
+              Q(res, g, i, j, k, 6, n) - v * (mor * mut * prod - fact * divv * rho * tke - P.bstrkw * rho * tke * ome);
+          Q(res, g, i, j, k, 7, n) = Q(res, g, i, j, k, 7, n) - v * (P.gkw * P.cmu * rho * prodw -
+                                                                     P.gkw * fact * divv * rho * ome -
+                                                                     P.bkw * rho * (ome * ome));
+        }
+}
+
+/* lim_prod_mg_b (src/src.f90:708-757); q conservative as in the caller. */
+void or_lim_prod_mg_b(const double* q, double* prd, const double* mutg, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  const double mor = P.machfs / P.reyno;
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          double rho = Q(q, g, i, j, k, 1, n), tke = Q(q, g, i, j, k, 6, n), ome = Q(q, g, i, j, k, 7, n);
+          double mut = S1(mutg, g, i, j, k, n), prod = S1(prd, g, i, j, k, n);
+          double tstdis = P.prdlim * P.bstrkw * rho * tke * ome;
+          if (mor * mut * prod > tstdis) S1(prd, g, i, j, k, n) = tstdis / mor / mut;
+        }
+}
+
+/* fixq_b (src/update.f90:2216-2269): floor k and omega at qmin(6:7). */
+void or_fixq_b(double* q, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          double rho = Q(q, g, i, j, k, 1, n);
+          double tke = Q(q, g, i, j, k, 6, n) / rho;
+          double ome = Q(q, g, i, j, k, 7, n) / rho;
+          if (tke < P.qmin[5]) {
+            tke = P.qmin[5];
+            Q(q, g, i, j, k, 6, n) = rho * tke;
+          }
+          if (ome < P.qmin[6]) {
+            ome = P.qmin[6];
+            Q(q, g, i, j, k, 7, n) = rho * ome;
+          }
+        }
+}
+
+/* tur_vis_b (src/update.f90:2297-2337): mut over 0..imax, 0..jmax, 0..kmax. */
+void or_tur_vis_b(const double* q, double* mut, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int k = 0; k <= kmax; k++)
+      for (int j = 0; j <= jmax; j++)
+        for (int i = 0; i <= imax; i++) {
+          double rho = Q(q, g, i, j, k, 1, n), tke = Q(q, g, i, j, k, 6, n), ome = Q(q, g, i, j, k, 7, n);
+          S1(mut, g, i, j, k, n) = P.cmu * rho * tke / (ome + P.epst) * P.reyno / P.machfs;
+        }
+}
+
+/* update_b / update_bhb explicit branch with the Wilcox k-omega point-
+ * implicit rho*omega, rho*k update (src/update.f90:596-634, 1327-1366);
+ * ttrm(:,:,:,2:3) of kom are the constants gkw, bkw (update_ttrm_b,
+ * src/update.f90:3069-3081).                                              */
+void or_update_kb(double* res, double* q, const double* qold, const double* vol, const double* dtvol,
+                  const double* dtvolt, const double* delpls, const double* prd, double ar, int imax, int jmax,
+                  int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  for (int n = 0; n <= 2 * NH; n++) {
+    if (is_kom())
+      for (int k = 1; k <= kmax - 1; k++)
+        for (int j = 1; j <= jmax - 1; j++)
+          for (int i = 1; i <= imax - 1; i++) {
+            double rho = Q(q, g, i, j, k, 1, n);
+            double ome = Q(q, g, i, j, k, 7, n) / Q(q, g, i, j, k, 1, n);
+            double tke = Q(q, g, i, j, k, 6, n) / Q(q, g, i, j, k, 1, n);
+            double dtt = P1(dtvolt, g, i, j, k, n);
+            double dtau = dtt * V1(vol, g, i, j, k);
+            double gav = P.gkw, bav = P.bkw;
+            double dl = S1(delpls, g, i, j, k, n);
+            double rhomega = (Q(qold, g, i, j, k, 7, n) - ar * Q(res, g, i, j, k, 7, n) * dtt +
+                              ar * dtau * rho * ome * (gav * dl + 2 * bav * ome)) /
+                             (1 + ar * dtau * (gav * dl + 2 * bav * ome));
+            double rhomegamin = gav * P.cmu * rho * sqrt(S1(prd, g, i, j, k, n));
+            if (rhomega < rhomegamin) rhomega = rhomegamin;
+            Q(q, g, i, j, k, 7, n) = rhomega;
+            double drhomega = Q(q, g, i, j, k, 7, n) - rho * ome;
+            Q(q, g, i, j, k, 6, n) = (Q(qold, g, i, j, k, 6, n) - ar * Q(res, g, i, j, k, 6, n) * dtt +
+                                      ar * dtau * rho * tke * (dl + P.bstrkw * ome) -
+                                      ar * dtau * P.bstrkw * tke * drhomega) /
+                                     (1 + ar * dtau * (dl + P.bstrkw * ome));
+          }
+    for (int ipde = 1; ipde <= 5; ipde++)
+      for (int k = 1; k <= kmax - 1; k++)
+        for (int j = 1; j <= jmax - 1; j++)
+          for (int i = 1; i <= imax - 1; i++)
+            Q(q, g, i, j, k, ipde, n) =
+                Q(qold, g, i, j, k, ipde, n) - ar * Q(res, g, i, j, k, ipde, n) * P1(dtvol, g, i, j, k, n);
+  }
+}
+
+/* lim_res_ome_b (src/update.f90:2585-2835), explicit branch without
+ * low-speed preconditioning; q conservative.                              */
+void or_lim_res_ome_b(double* res, const double* q, const double* vol, const double* dtvolt, const double* delpls,
+                      const double* prd, double ar, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  for (int k = 1; k <= kmax - 1; k++)
+    for (int j = 1; j <= jmax - 1; j++)
+      for (int i = 1; i <= imax - 1; i++) {
+        double gav = P.gkw, bav = P.bkw;
+        double dtt = P1(dtvolt, g, i, j, k, 0);
+        double dtau = dtt * V1(vol, g, i, j, k);
+        double rho = Q(q, g, i, j, k, 1, 0);
+        double ome = Q(q, g, i, j, k, 7, 0) / Q(q, g, i, j, k, 1, 0);
+        double den1 = 1 + ar * dtau * (gav * S1(delpls, g, i, j, k, 0) + 2 * bav * ome);
+        double rhomega = Q(q, g, i, j, k, 7, 0) - ar * Q(res, g, i, j, k, 7, 0) * dtt / den1;
+        double rhomegamin = gav * P.cmu * rho * sqrt(S1(prd, g, i, j, k, 0));
+        if (rhomega < rhomegamin) Q(res, g, i, j, k, 7, 0) = den1 * (Q(q, g, i, j, k, 7, 0) - rhomegamin) / dtt / ar;
+      }
+}
+
+/* rest4_b (src/cosa.f90:3122-3164): volume-weighted restriction of a
+ * scalar P3 field (production, divergence, delpls).                       */
+void or_rest4_b(int imax1, int jmax1, int kmax1, int imax2, int jmax2, int kmax2, const double* prd1, double* prd2,
+                const double* vol1, const double* vol2) {
+  bg g1 = mkbg(imax1, jmax1, kmax1), g2 = mkbg(imax2, jmax2, kmax2);
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int k1 = 1; k1 <= kmax1 - 2; k1 += 2)
+      for (int j1 = 1; j1 <= jmax1 - 2; j1 += 2)
+        for (int i1 = 1; i1 <= imax1 - 2; i1 += 2) {
+          int i2 = 1 + i1 / 2, j2 = 1 + j1 / 2, k2 = 1 + k1 / 2;
+          S1(prd2, g2, i2, j2, k2, n) =
+              (V1(vol1, g1, i1, j1, k1) * S1(prd1, g1, i1, j1, k1, n) +
+               V1(vol1, g1, i1, j1 + 1, k1) * S1(prd1, g1, i1, j1 + 1, k1, n) +
+               V1(vol1, g1, i1 + 1, j1, k1) * S1(prd1, g1, i1 + 1, j1, k1, n) +
+               V1(vol1, g1, i1 + 1, j1 + 1, k1) * S1(prd1, g1, i1 + 1, j1 + 1, k1, n) +
+               V1(vol1, g1, i1, j1, k1 + 1) * S1(prd1, g1, i1, j1, k1 + 1, n) +
+               V1(vol1, g1, i1, j1 + 1, k1 + 1) * S1(prd1, g1, i1, j1 + 1, k1 + 1, n) +
+               V1(vol1, g1, i1 + 1, j1, k1 + 1) * S1(prd1, g1, i1 + 1, j1, k1 + 1, n) +
+               V1(vol1, g1, i1 + 1, j1 + 1, k1 + 1) * S1(prd1, g1, i1 + 1, j1 + 1, k1 + 1, n)) /
+              V1(vol2, g2, i2, j2, k2);
+        }
 }
 
 /* ---------------------------------------------------------------------- */

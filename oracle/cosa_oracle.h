@@ -35,6 +35,9 @@ typedef struct or_level {
   double *q, *qold, *qp, *qm, *dq, *dqp, *dqm, *flux, *res, *rhs, *work1;
   double *si, *sj, *sk, *vol, *dtvol, *betaxi, *betaeta, *betazeta;
   double *xdot, *ydot, *zdot, *pr;
+  /* viscous / k-omega (NULL otherwise) */
+  double *mut, *dist, *prd, *divvel, *delpls, *dtvolt, *tbetaxi, *tbetaeta, *tbetazeta;
+  double* xd[9];   /* xideri, etaderi, zetaderi, xiderj, ..., zetaderk */
 } or_level;
 
 typedef struct or_case {
@@ -94,6 +97,43 @@ void or_prol_b(int bln, int imax1, int jmax1, int kmax1, int imax2,
                int jmax2, int kmax2, double* q1, const double* q2,
                const double* vol1);
 void or_q_bedges(double* q, int imax, int jmax, int kmax);
+/* viscous / Wilcox k-omega */
+void or_bbc_dist(double* q, const double* si, const double* sj, const double* sk, const double* dist,
+                 const int32_t* bctopo, int nbcs, int imax, int jmax, int kmax);
+void or_lambdab_v(const double* q, const double* si, const double* sj, const double* sk, const double* xdot,
+                  const double* ydot, const double* zdot, const double* vol, const double* mut, double* lambdas,
+                  double* lamv, int imax, int jmax, int kmax);
+void or_bdeltat_v(double* dtvol, double* dtvolt, const double* lambdas, const double* lamv, double* betaxi,
+                  double* betaeta, double* betazeta, double* tbetaxi, double* tbetaeta, double* tbetazeta, int imax,
+                  int jmax, int kmax);
+void or_cirs_kb(double* res, const double* dtvol, const double* betaxi, const double* betaeta,
+                const double* betazeta, const double* dtvolt, const double* tbetaxi, const double* tbetaeta,
+                const double* tbetazeta, int imax, int jmax, int kmax);
+void or_q_face_b(int idir, const int32_t* bctopo, int nbcs, const double* q, double* qf, const double* mut,
+                 double* mutf, int calmut, int imax, int jmax, int kmax);
+void or_q_der_b(int idir, const int32_t* bctopo, int nbcs, const double* q, double* dqxi, double* dqeta,
+                double* dqzeta, int imax, int jmax, int kmax);
+void or_rtst_b(int idir, const double* qf, const double* dqxi, const double* dqeta, const double* dqzeta,
+               const double* xider, const double* etader, const double* zetader, double* tau1, const double* mutf,
+               int imax, int jmax, int kmax);
+void or_vflux_b(int idir, const double* qf, const double* dqxi, const double* dqeta, const double* dqzeta,
+                const double* tau1, double* flux, const double* xider, const double* etader, const double* zetader,
+                const double* sijk, int imax, int jmax, int kmax);
+void or_q_der_c_b(int idir, const double* qf, double* dqc, int imax, int jmax, int kmax);
+void or_turprod_b(const double* q, const double* dqxi, const double* dqeta, const double* dqzeta, double* prd,
+                  double* divvel, double* delpls, const double* const* xd, int imax, int jmax, int kmax);
+void or_srckw_b(const double* q, const double* mutg, const double* prd, const double* divvel, double* res,
+                const double* vol, int nl, int imax, int jmax, int kmax);
+void or_lim_prod_mg_b(const double* q, double* prd, const double* mutg, int imax, int jmax, int kmax);
+void or_fixq_b(double* q, int imax, int jmax, int kmax);
+void or_tur_vis_b(const double* q, double* mut, int imax, int jmax, int kmax);
+void or_update_kb(double* res, double* q, const double* qold, const double* vol, const double* dtvol,
+                  const double* dtvolt, const double* delpls, const double* prd, double ar, int imax, int jmax,
+                  int kmax);
+void or_lim_res_ome_b(double* res, const double* q, const double* vol, const double* dtvolt, const double* delpls,
+                      const double* prd, double ar, int imax, int jmax, int kmax);
+void or_rest4_b(int imax1, int jmax1, int kmax1, int imax2, int jmax2, int kmax2, const double* prd1, double* prd2,
+                const double* vol1, const double* vol2);
 void or_q_bcorns(double* q, int imax, int jmax, int kmax);
 
 /* whole-level routines (loop over blocks, as the reference wrappers) */

@@ -34,6 +34,10 @@ module ref_shim
     real(c_double) :: omega, toler
     real(c_double) :: qmin(7), alphas(5)
     real(c_double) :: tkefar, omefar
+    integer(c_int32_t) :: turcmp, lim_prodtke, lim_prodomega, wallwilc, ifixq, prd_all_lvls
+    integer(c_int32_t) :: lim_corr(3), pad1
+    real(c_double) :: reyno, pranl, prant, stemp, cdff, epst, cmu, sigk, sigw, bkw, bstrkw, gkw
+    real(c_double) :: prdlim, roughk
   end type cfg_t
 
 contains
@@ -75,7 +79,13 @@ contains
     tpitch = .false.; lomach = .false.; rkimp = .false.; rgkuns = .false.
     dualt = .false.; relframe = .false.; unsteady = harbal
     ramping1 = .false.; ramping2 = .false.; mgit = 1
-    lim_corr = .false.
+    lim_corr = c%lim_corr /= 0
+    turcmp = c%turcmp; lim_prodtke = c%lim_prodtke /= 0; lim_prodomega = c%lim_prodomega /= 0
+    wallwilc = c%wallwilc /= 0; wallment = .not. wallwilc; ifixq = c%ifixq
+    prd_all_lvls = c%prd_all_lvls /= 0
+    reyno = c%reyno; pranl = c%pranl; prant = c%prant; stemp = c%stemp; cdff = c%cdff
+    epst = c%epst; cmu = c%cmu; sigk = c%sigk; sigw = c%sigw; bkw = c%bkw; bstrkw = c%bstrkw
+    gkw = c%gkw; prdlim = c%prdlim; roughk = c%roughk
   end subroutine
 
   subroutine ref_set_dhb(nsnap, d) bind(C, name='ref_set_dhb')
@@ -241,6 +251,136 @@ contains
     integer(c_int), value :: imax, jmax, kmax
     calmut = .false.
     call q_bcorns(q, mut, imax, jmax, kmax, npde, nharms)
+  end subroutine
+
+  ! ---- viscous fluxes and Wilcox k-omega (src/flux.f90, src/src.f90, src/update.f90)
+
+  subroutine ref_q_face_b(idir, bctopo, nbcs, q, qf, mut, mutf, cm, imax, jmax, kmax) &
+      bind(C, name='ref_q_face_b')
+    integer(c_int), value :: idir, nbcs, cm, imax, jmax, kmax
+    integer(c_int) :: bctopo(*)
+    real(c_double) :: q(*), qf(*), mut(*), mutf(*)
+    calmut = cm /= 0
+    if (idir == 1) then
+      call bq_face_i(bctopo, nbcs, q, qf, mut, mutf, imax, jmax, kmax, npde, nharms)
+    else if (idir == 2) then
+      call bq_face_j(bctopo, nbcs, q, qf, mut, mutf, imax, jmax, kmax, npde, nharms)
+    else
+      call bq_face_k(bctopo, nbcs, q, qf, mut, mutf, imax, jmax, kmax, npde, nharms)
+    end if
+    calmut = .false.
+  end subroutine
+
+  subroutine ref_q_der_b(idir, bctopo, nbcs, q, dqxi, dqeta, dqzeta, imax, jmax, kmax) &
+      bind(C, name='ref_q_der_b')
+    integer(c_int), value :: idir, nbcs, imax, jmax, kmax
+    integer(c_int) :: bctopo(*)
+    real(c_double) :: q(*), dqxi(*), dqeta(*), dqzeta(*)
+    if (idir == 1) then
+      call bq_der_i(bctopo, nbcs, q, dqxi, dqeta, dqzeta, imax, jmax, kmax, npde, nharms)
+    else if (idir == 2) then
+      call bq_der_j(bctopo, nbcs, q, dqxi, dqeta, dqzeta, imax, jmax, kmax, npde, nharms)
+    else
+      call bq_der_k(bctopo, nbcs, q, dqxi, dqeta, dqzeta, imax, jmax, kmax, npde, nharms)
+    end if
+  end subroutine
+
+  subroutine ref_rtst_b(idir, qf, dqxi, dqeta, dqzeta, xider, etader, zetader, tau1, mutf, &
+                        imax, jmax, kmax) bind(C, name='ref_rtst_b')
+    integer(c_int), value :: idir, imax, jmax, kmax
+    real(c_double) :: qf(*), dqxi(*), dqeta(*), dqzeta(*), xider(*), etader(*), zetader(*)
+    real(c_double) :: tau1(*), mutf(*)
+    integer(c_int) :: ic, jc, kc
+    ic = imax - 1; jc = jmax - 1; kc = kmax - 1
+    if (idir == 1) ic = imax
+    if (idir == 2) jc = jmax
+    if (idir == 3) kc = kmax
+    call b_rtst(qf, dqxi, dqeta, dqzeta, xider, etader, zetader, tau1, mutf, imax, jmax, kmax, &
+                npde, nharms, ic, jc, kc)
+  end subroutine
+
+  subroutine ref_vflux_b(idir, qf, dqxi, dqeta, dqzeta, tau1, flux, fdist, xider, etader, zetader, &
+                         sijk, imax, jmax, kmax) bind(C, name='ref_vflux_b')
+    integer(c_int), value :: idir, imax, jmax, kmax
+    real(c_double) :: qf(*), dqxi(*), dqeta(*), dqzeta(*), tau1(*), flux(*), fdist(*)
+    real(c_double) :: xider(*), etader(*), zetader(*), sijk(*)
+    integer(c_int) :: ic, jc, kc
+    ic = imax - 1; jc = jmax - 1; kc = kmax - 1
+    if (idir == 1) ic = imax
+    if (idir == 2) jc = jmax
+    if (idir == 3) kc = kmax
+    call vflux_b(qf, dqxi, dqeta, dqzeta, tau1, flux, fdist, xider, etader, zetader, sijk, &
+                 imax, jmax, kmax, npde, nharms, lmet, ic, jc, kc)
+  end subroutine
+
+  subroutine ref_q_der_c_b(idir, qf, dqc, imax, jmax, kmax) bind(C, name='ref_q_der_c_b')
+    integer(c_int), value :: idir, imax, jmax, kmax
+    real(c_double) :: qf(*), dqc(*)
+    if (idir == 1) then
+      call bq_der_ci(qf, dqc, imax, jmax, kmax, npde, nharms)
+    else if (idir == 2) then
+      call bq_der_cj(qf, dqc, imax, jmax, kmax, npde, nharms)
+    else
+      call bq_der_ck(qf, dqc, imax, jmax, kmax, npde, nharms)
+    end if
+  end subroutine
+
+  subroutine ref_turprod_b(q, dqxi, dqeta, dqzeta, prd, divvel, delpls, xdi, edi, zdi, xdj, edj, zdj, &
+                           xdk, edk, zdk, imax, jmax, kmax) bind(C, name='ref_turprod_b')
+    real(c_double) :: q(*), dqxi(*), dqeta(*), dqzeta(*), prd(*), divvel(*), delpls(*)
+    real(c_double) :: xdi(*), edi(*), zdi(*), xdj(*), edj(*), zdj(*), xdk(*), edk(*), zdk(*)
+    integer(c_int), value :: imax, jmax, kmax
+    call turprod_b(q, dqxi, dqeta, dqzeta, prd, divvel, delpls, xdi, edi, zdi, xdj, edj, zdj, &
+                   xdk, edk, zdk, imax, jmax, kmax, npde, nharms)
+  end subroutine
+
+  subroutine ref_srckw_b(q, mut, prd, divvel, res, vol, nl, imax, jmax, kmax) bind(C, name='ref_srckw_b')
+    real(c_double) :: q(*), mut(*), prd(*), divvel(*), res(*), vol(*)
+    integer(c_int), value :: nl, imax, jmax, kmax
+    call srckw_b(q, mut, prd, divvel, res, vol, nl, imax, jmax, kmax, npde, nharms)
+  end subroutine
+
+  subroutine ref_lim_prod_mg_b(q, prd, mut, imax, jmax, kmax) bind(C, name='ref_lim_prod_mg_b')
+    real(c_double) :: q(*), prd(*), mut(*)
+    integer(c_int), value :: imax, jmax, kmax
+    call lim_prod_mg_b(q, prd, mut, imax, jmax, kmax, npde, nharms)
+  end subroutine
+
+  subroutine ref_fixq_b(q, imax, jmax, kmax) bind(C, name='ref_fixq_b')
+    real(c_double) :: q(*)
+    integer(c_int), value :: imax, jmax, kmax
+    call fixq_b(q, imax, jmax, kmax, npde, nharms, 1, 1)
+  end subroutine
+
+  subroutine ref_tur_vis_b(q, mut, imax, jmax, kmax) bind(C, name='ref_tur_vis_b')
+    real(c_double) :: q(*), mut(*)
+    integer(c_int), value :: imax, jmax, kmax
+    call tur_vis_b(q, mut, npde, imax, jmax, kmax, nharms)
+  end subroutine
+
+  subroutine ref_update_ttrm_b(q, dqxi, dqeta, dqzeta, xdi, edi, zdi, xdj, edj, zdj, xdk, edk, zdk, &
+                               ttrm, dist, imax, jmax, kmax) bind(C, name='ref_update_ttrm_b')
+    real(c_double) :: q(*), dqxi(*), dqeta(*), dqzeta(*), ttrm(*), dist(*)
+    real(c_double) :: xdi(*), edi(*), zdi(*), xdj(*), edj(*), zdj(*), xdk(*), edk(*), zdk(*)
+    integer(c_int), value :: imax, jmax, kmax
+    call update_ttrm_b(q, dqxi, dqeta, dqzeta, xdi, edi, zdi, xdj, edj, zdj, xdk, edk, zdk, ttrm, dist, &
+                       npde, imax, jmax, kmax, nharms)
+  end subroutine
+
+  subroutine ref_lim_res_ome_b(res, q, vol, dtvolt, delpls, prd, ttrm, ar, imax, jmax, kmax) &
+      bind(C, name='ref_lim_res_ome_b')
+    real(c_double) :: res(*), q(*), vol(*), dtvolt(*), delpls(*), prd(*), ttrm(*)
+    real(c_double), value :: ar
+    integer(c_int), value :: imax, jmax, kmax
+    real(c_double) :: dummy(1)
+    call lim_res_ome_b(res, q, vol, dtvolt, dummy, dummy, dummy, delpls, prd, ttrm, dummy, ar, &
+                       imax, jmax, kmax, npde)
+  end subroutine
+
+  subroutine ref_rest4_b(i1, j1, k1, i2, j2, k2, prd1, prd2, vol1, vol2) bind(C, name='ref_rest4_b')
+    integer(c_int), value :: i1, j1, k1, i2, j2, k2
+    real(c_double) :: prd1(*), prd2(*), vol1(*), vol2(*)
+    call rest4_b(i1, j1, k1, i2, j2, k2, nharms, prd1, prd2, vol1, vol2)
   end subroutine
 
 end module ref_shim

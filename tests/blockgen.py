@@ -52,6 +52,32 @@ def box_block(imax, jmax, kmax, amp=0.02, seed=0, lmet=4, nsnap_met=1):
     return mets[0], mets[1], mets[2], vol
 
 
+def box_block_moving(imax, jmax, kmax, nsnap, seed=0, dtheta0=0.2, omega=0.7):
+    """The distorted box of box_block pitching about (0.4, 0.3): metrics with
+    lmet = 7 (face velocities in components 5-7), one set per snapshot
+    (hbmove = 1), node velocities xdot/ydot/zdot [P2] per snapshot and the
+    volumes [P1] of the rest pose."""
+    import math
+    x, y, z = np.meshgrid(np.linspace(0, 1.3, imax), np.linspace(0, 0.9, jmax), np.linspace(0, 0.7, kmax),
+                          indexing="ij")
+    x, y, z = (np.ascontiguousarray(a.transpose(2, 1, 0)) for a in (x, y, z))
+    x, y, z = grids.distort(x, y, z, 0.02 * min(1.3 / imax, 0.9 / jmax, 0.7 / kmax) * 4, seed=seed)
+    mets = [[], [], []]
+    vel = [[], [], []]
+    for n in range(nsnap):
+        t = 2 * math.pi * n / (nsnap * omega)
+        th, thd = dtheta0 * math.sin(omega * t), dtheta0 * omega * math.cos(omega * t)
+        xn, yn, zn, *v = grids.pitch(x, y, z, th, thd, 0.4, 0.3)
+        for m, sm in zip(mets, grids.face_metrics(xn, yn, zn, 7, vel=v)):
+            m.append(sm.reshape(-1))
+        for dst, src in zip(vel, v):
+            p2 = np.zeros((kmax + 2, jmax + 2, imax + 2))
+            p2[1:-1, 1:-1, 1:-1] = src
+            dst.append(p2.reshape(-1))
+    vol = grids.cell_volumes(x, y, z).reshape(-1).copy()
+    return ([np.concatenate(m).copy() for m in mets], vol, [np.concatenate(v).copy() for v in vel])
+
+
 def all_faces_patches(imax, jmax, kmax, types=(30, 14, 30, 14, 80, 80)):
     """One patch per block face: i=1, i=imax, j=1, j=jmax, k=1, k=kmax."""
     return [

@@ -48,7 +48,9 @@ class OrLevel(ctypes.Structure):
     _fields_ = [("nblocks", ctypes.c_int32), ("ncuts", ctypes.c_int32),
                 ("imax", IP), ("jmax", IP), ("kmax", IP), ("nbcs", IP), ("bctopo", IP), ("cutopo", IP)] + [
         (n, DP) for n in ("q", "qold", "qp", "qm", "dq", "dqp", "dqm", "flux", "res", "rhs", "work1", "si", "sj",
-                          "sk", "vol", "dtvol", "betaxi", "betaeta", "betazeta", "xdot", "ydot", "zdot", "pr")]
+                          "sk", "vol", "dtvol", "betaxi", "betaeta", "betazeta", "xdot", "ydot", "zdot", "pr")] + [
+        (n, DP) for n in ("mut", "dist", "prd", "divvel", "delpls", "dtvolt", "tbetaxi", "tbetaeta", "tbetazeta")] + [
+        ("xd", DP * 9)]
 
 
 class OrCaseS(ctypes.Structure):
@@ -125,7 +127,7 @@ class OrCase:
             for n in ("dtvol", "betaxi", "betaeta", "betazeta"):
                 a[n] = np.zeros(case.size(nl, "p1"))
             for n in ("xdot", "ydot", "zdot"):
-                a[n] = np.zeros(case.size(nl, "x2"))
+                a[n] = g[n].copy() if n in g else np.zeros(case.size(nl, "x2"))
             a["q"][:] = case.freestream(nl)
             if nl == 1 and q1 is not None:
                 a["q"][:] = q1
@@ -141,6 +143,18 @@ class OrCase:
                 setattr(lev, n, ip(v))
             for n in self.FIELDS:
                 setattr(lev, n, dp(a[n]))
+            c = case.config
+            if c["viscous"] or c["kom"]:
+                for n in ("mut", "prd", "divvel", "delpls"):
+                    a[n] = g[n].copy() if n in g else np.zeros(case.size(nl, "p3s"))
+                for n in ("dtvolt", "tbetaxi", "tbetaeta", "tbetazeta"):
+                    a[n] = np.zeros(case.size(nl, "p1"))
+                a["dist"] = g["dist"].copy()
+                a["xd"] = [x.copy() for x in g["xd"]]
+                for n in ("mut", "dist", "prd", "divvel", "delpls", "dtvolt", "tbetaxi", "tbetaeta", "tbetazeta"):
+                    setattr(lev, n, dp(a[n]))
+                for m in range(9):
+                    lev.xd[m] = dp(a["xd"][m])
             self.arrays.append(a)
 
     def ptr(self):

@@ -68,7 +68,8 @@ def make_pair(case, q0, device=0):
 
 def small_cases():
     """Parity cases: distorted two-block with a cut and four BC types;
-    NACA O-grid with a self-cut; HB on the two-block case."""
+    NACA O-grid with a self-cut; HB on the two-block case; HB on the NACA
+    O-grid pitching about the quarter chord (moving grid, lmet = 7)."""
     out = []
     c = cases.two_block_case(ni=13, nj=9, nk=9, nlevel=1, distort=0.15, name="two_block", irsop=2,
                              cfl_irs=4.0, cfl=2.0, ilimf=4, epslimf=0.01)
@@ -77,5 +78,8 @@ def small_cases():
     out.append(c)
     c = cases.two_block_case(ni=9, nj=9, nk=5, nlevel=1, distort=0.1, name="two_block_hb2", irsop=3, psirs=0.125,
                              cfl_irs=3.0, cfl=2.0, nharms=2, harbal=1, omega=0.5)
+    out.append(c)
+    c = cases.naca_case(33, 13, 5, nlevel=1, name="naca_pitch_hb1", irsop=2, cfl_irs=3.0, cfl=2.0, nharms=1,
+                        harbal=1, omega=0.6, pitch=dict(dtheta0=0.05, xh=0.25, yh=0.0))
     out.append(c)
     return out

@@ -39,7 +39,7 @@ def test_library_exports_every_symbol():
 
 def test_config_struct_layout():
     # 38 int32 members then 37 doubles (include/cosahip.h)
-    assert ctypes.sizeof(CosahipConfig) == 38 * 4 + 37 * 8
+    assert ctypes.sizeof(CosahipConfig) == 38 * 4 + 37 * 8 + 10 * 4 + 14 * 8
     assert CosahipConfig.gamma.offset == 152
 
 

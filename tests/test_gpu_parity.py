@@ -110,11 +110,14 @@ def _mg_case(name):
     if name == "blocks3_lin_ho":
         return cases.two_block_case(ni=17, nj=9, nk=9, nlevel=3, distort=0.1, name="blocks3", irsop=2,
                                     cfl_irs=3.0, cfl=2.0, ncycle=5, bln_prol=0, ho_rest=1)
+    if name == "naca3_pitch_hb2":
+        return cases.naca_case(65, 17, 9, nlevel=3, name="naca3_pitch", irsop=2, cfl_irs=3.0, cfl=2.0, nl_crs=2,
+                               ncycle=5, nharms=2, harbal=1, omega=0.6, pitch=dict(dtheta0=0.05, xh=0.25, yh=0.0))
     return cases.two_block_case(ni=17, nj=9, nk=9, nlevel=2, distort=0.1, name="blocks2", irsop=3, psirs=0.125,
                                 cfl_irs=3.0, cfl=2.0, ncycle=5, nharms=1, harbal=1, omega=0.8)
 
 
-@pytest.mark.parametrize("name", ["naca3", "blocks3_lin_ho", "blocks2_hb1"])
+@pytest.mark.parametrize("name", ["naca3", "blocks3_lin_ho", "blocks2_hb1", "naca3_pitch_hb2"])
 def test_multigrid_cycles(name):
     """Whole V-cycles (mg, src/cosa.f90:806-930): resrms history per cycle
     and the final state."""

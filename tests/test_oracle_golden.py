@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "golden"))
 
 from make_golden import run, set_cfg  # noqa: E402
+from cosa_amd.config import default_config  # noqa: E402
 from oracle_bind import Oracle  # noqa: E402
 
 PATH = os.path.join(HERE, "golden", "ref_kernels.npz")
@@ -27,7 +28,10 @@ def oracle():
 
 @pytest.mark.parametrize("name", NAMES)
 def test_oracle_matches_reference_fixture(oracle, name):
-    cfg = dict(ast.literal_eval(bytes(DATA[f"{name}/cfg"]).decode()))
+    # fixtures store the configuration they were made with; fields added to
+    # the configuration since then take their defaults
+    cfg = default_config()
+    cfg.update(dict(ast.literal_eval(bytes(DATA[f"{name}/cfg"]).decode())))
     inputs = {k.split("/")[-1]: DATA[k] for k in DATA.files if k.startswith(f"{name}/in/")}
     want = {k.split("/")[-1]: DATA[k] for k in DATA.files if k.startswith(f"{name}/out/")}
     set_cfg(oracle, cfg)

@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from blockgen import all_faces_patches, bctopo_of, box_block, n3, prim_to_cons, rand_prim
+from blockgen import all_faces_patches, bctopo_of, box_block, box_block_moving, n3, prim_to_cons, rand_prim
 from cosa_amd.config import default_config, hb_operator
 from oracle_bind import Oracle, Ref, dp, ip, ref_available
 
@@ -90,6 +90,69 @@ def test_bbc_supersonic(libs):
     o.or_bbc(dp(a), dp(si), dp(sj), dp(sk), ip(bt), 6, *dims)
     r.ref_bbc(dp(b), dp(si), dp(sj), dp(sk), dp(np.zeros(vol.size)), 1, ip(bt), 6, *dims)
     same(a, b, "bbc supersonic")
+
+
+MOVING = dict(moving=1, hbmove=1, lmet=7)
+
+
+@pytest.mark.parametrize("nh", [1, 2])
+def test_bbc_moving(libs, nh):
+    """Inviscid wall on a pitching grid: the normal velocity is taken
+    relative to the face velocity (metric components 5-7 of each snapshot)."""
+    o, r = libs
+    dims = (8, 7, 5)
+    setcfg(libs, nharms=nh, machfs=0.5, alpha_deg=2.0, **MOVING)
+    ns = 2 * nh + 1
+    (si, sj, sk), vol, _ = box_block_moving(*dims, ns, seed=8)
+    q = rand_prim(np.random.default_rng(31), *dims, nsnap=ns)
+    bt = bctopo_of(all_faces_patches(*dims, (14, 30, 14, 14, 80, 14)))
+    a, b = q.copy(), q.copy()
+    o.or_bbc(dp(a), dp(si), dp(sj), dp(sk), ip(bt), 6, *dims)
+    r.ref_bbc(dp(b), dp(si), dp(sj), dp(sk), dp(np.zeros(vol.size)), 1, ip(bt), 6, *dims)
+    same(a, b, "bbc moving")
+    assert not np.array_equal(a, q)
+
+
+@pytest.mark.parametrize("nh", [1, 2])
+def test_lambda_moving(libs, nh):
+    """Spectral radii with the node velocity of each snapshot subtracted."""
+    o, r = libs
+    dims = (8, 7, 5)
+    setcfg(libs, nharms=nh, irsop=2, cfl=3.0, rat=1.7, **MOVING)
+    ns = 2 * nh + 1
+    (si, sj, sk), vol, (xd, yd, zd) = box_block_moving(*dims, ns, seed=9)
+    q = rand_prim(np.random.default_rng(32), *dims, nsnap=ns)
+    N3 = n3(*dims)
+    n1 = (dims[0] + 1) * (dims[1] + 1) * (dims[2] + 1)
+    lam_o, lam_r = np.zeros(N3 * 5 * ns), np.zeros(N3 * 5 * ns)
+    o.or_lambdab(dp(q), dp(si), dp(sj), dp(sk), dp(xd), dp(yd), dp(zd), dp(lam_o), *dims)
+    cut = np.zeros(n1 * ns)
+    r.ref_lambdab(dp(q), dp(si), dp(sj), dp(sk), dp(xd), dp(yd), dp(zd), dp(vol), dp(np.zeros(N3 * ns)),
+                  dp(np.zeros(N3 * 5 * ns)), dp(lam_r), dp(cut), dp(cut), *dims)
+    same(lam_o, lam_r, "lambda moving")
+    lam_s = np.zeros_like(lam_o)
+    o.or_lambdab(dp(q), dp(si), dp(sj), dp(sk), dp(np.zeros_like(xd)), dp(np.zeros_like(yd)), dp(np.zeros_like(zd)),
+                 dp(lam_s), *dims)
+    assert not np.array_equal(lam_s, lam_o)
+
+
+@pytest.mark.parametrize("idir", [1, 2, 3])
+def test_roflux_moving(libs, idir):
+    """Roe flux with the face velocity ubn (lmet = 7), three snapshots."""
+    o, r = libs
+    dims = (8, 6, 5)
+    setcfg(libs, nharms=1, efx_setup_f=1, efx_all_f=1, cntrpy_f=0.3, entfx_f_w=(0, 0, 1), **MOVING)
+    (si, sj, sk), vol, _ = box_block_moving(*dims, 3, seed=10)
+    s = (si, sj, sk)[idir - 1]
+    rng = np.random.default_rng(40 + idir)
+    qr = rand_prim(rng, *dims, nsnap=3)
+    ql = rand_prim(rng, *dims, nsnap=3)
+    nf = dims[0] * dims[1] * dims[2] * 5 * 3
+    fo, fr = np.zeros(nf), np.zeros(nf)
+    o.or_roflux_b(idir, dp(qr), dp(ql), dp(fo), dp(s), *dims)
+    scratch = [np.zeros_like(qr) for _ in range(2)]
+    r.ref_roflux_b(idir, dp(qr), dp(ql), dp(scratch[0]), dp(scratch[1]), dp(fr), dp(s), *dims)
+    same(fo, fr, "roflux moving")
 
 
 @pytest.mark.parametrize("irsop", [1, 2, 3])
