@@ -16,9 +16,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcosahip.so")
 
 # field ids (enum in include/cosahip.h)
-Q, QOLD, RES, RHS, SI, SJ, SK, VOL, DTVOL, BETAXI, BETAETA, BETAZETA, XDOT, YDOT, ZDOT = range(15)
+(Q, QOLD, RES, RHS, SI, SJ, SK, VOL, DTVOL, BETAXI, BETAETA, BETAZETA, XDOT, YDOT, ZDOT, MUT, DIST, XIDERI, ETADERI,
+ ZETADERI, XIDERJ, ETADERJ, ZETADERJ, XIDERK, ETADERK, ZETADERK, PRD, DIVVEL, DELPLS, DTVOLT) = range(30)
 FIELD_IDS = dict(q=Q, qold=QOLD, res=RES, rhs=RHS, si=SI, sj=SJ, sk=SK, vol=VOL, dtvol=DTVOL, betaxi=BETAXI,
-                 betaeta=BETAETA, betazeta=BETAZETA, xdot=XDOT, ydot=YDOT, zdot=ZDOT)
+                 betaeta=BETAETA, betazeta=BETAZETA, xdot=XDOT, ydot=YDOT, zdot=ZDOT, mut=MUT, dist=DIST,
+                 xideri=XIDERI, etaderi=ETADERI, zetaderi=ZETADERI, xiderj=XIDERJ, etaderj=ETADERJ,
+                 zetaderj=ZETADERJ, xiderk=XIDERK, etaderk=ETADERK, zetaderk=ZETADERK, prd=PRD, divvel=DIVVEL,
+                 delpls=DELPLS, dtvolt=DTVOLT)
+# the nine metric-derivative fields in the order of the reference's arrays
+XD_NAMES = ["xideri", "etaderi", "zetaderi", "xiderj", "etaderj", "zetaderj", "xiderk", "etaderk", "zetaderk"]
 
 FAMILIES = ["residual", "cirs", "update", "halo", "convert", "lambda", "other"]
 
@@ -73,6 +79,9 @@ def load() -> ctypes.CDLL:
         "cosahip_copy_array": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "cosahip_cirs": ([_VP, ctypes.c_int], ctypes.c_int),
         "cosahip_update": ([_VP, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+        "cosahip_tur_vis": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "cosahip_fixq": ([_VP, ctypes.c_int], ctypes.c_int),
+        "cosahip_turb_limits": ([_VP, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "cosahip_smooth": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _IP, _DP],
                            ctypes.c_int),
         "cosahip_restrict": ([_VP, ctypes.c_int], ctypes.c_int),
@@ -98,7 +107,8 @@ def exported_symbols():
             "cosahip_download", "cosahip_synchronize", "cosahip_cons2prim", "cosahip_prim2cons", "cosahip_bc",
             "cosahip_cutman_q", "cosahip_edges_corners", "cosahip_prtest", "cosahip_lambda_deltat",
             "cosahip_residual", "cosahip_src_hb", "cosahip_resg_rms", "cosahip_add_array", "cosahip_copy_array",
-            "cosahip_cirs", "cosahip_update", "cosahip_smooth", "cosahip_restrict", "cosahip_prolong",
+            "cosahip_cirs", "cosahip_update", "cosahip_tur_vis", "cosahip_fixq", "cosahip_turb_limits",
+            "cosahip_smooth", "cosahip_restrict", "cosahip_prolong",
             "cosahip_build_info", "cosahip_timing_enable", "cosahip_timing_read", "cosahip_timing_reset"]
 
 

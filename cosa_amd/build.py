@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcosahip.so")
-SOURCES = ["api.hip", "k_core.hip", "k_flux.hip"]
+SOURCES = ["api.hip", "k_core.hip", "k_flux.hip", "k_turb.hip"]
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-result"]
@@ -25,7 +25,7 @@ def hipcc() -> str:
     return shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
 
-def build(verbose: bool = False, jobs: int = 3) -> str:
+def build(verbose: bool = False, jobs: int = 4) -> str:
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
     objs = []
     procs = []

@@ -64,6 +64,7 @@ class Case:
             "p1": L.total(1) * self.nsnap,
             "vol": L.total(1),
             "flux": L.total(0) * self.npde * self.nsnap,
+            "d0": L.total(0) * 3 * self.nsnap_met,
         }[kind]
 
     def ncell(self, nl: int = 1) -> int:
@@ -114,7 +115,76 @@ class Case:
         out = {"si": si, "sj": sj, "sk": sk, "vol": vol}
         if moving:
             out.update(xdot=vel[0], ydot=vel[1], zdot=vel[2])
+        if self.config["viscous"]:
+            out.update(self._viscous_geometry(L))
         return out
+
+    def _viscous_geometry(self, L: Level) -> Dict[str, np.ndarray]:
+        """Metric derivatives (nine (3, imax, jmax, kmax) arrays, off_0
+        offsets) and the cell-centre distance to the nearest viscous wall
+        (types 15xx) of every block of level L; the nearest wall face centre
+        stands for the wall (dist2wall, src/grid.f90:2506-2808, is the
+        caller's set-up step)."""
+        from scipy.spatial import cKDTree
+        nsm = self.nsnap_met
+        xd = [np.zeros(self.size(L.nl, "d0")) for _ in range(9)]
+        dist = np.zeros(self.size(L.nl, "vol"))
+        o0, o1 = L.offsets(0), L.offsets(1)
+        walls = []
+        coords = [tuple(grids.coarsen(a, L.nl) for a in self.xyz[b]) for b in range(L.nblocks)]
+        for b, blk in enumerate(self.blocks):
+            x, y, z = coords[b]
+            P = np.stack([x, y, z], axis=-1)
+            f = 2 ** (L.nl - 1)
+            for pt in blk.patches:
+                if pt.bctype // 100 != 15:
+                    continue
+                lo = [(v - 1) // f for v in pt.istrt]
+                hi = [(v - 1) // f for v in pt.iend]
+                sub = P[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
+                # face centres of the wall patch: average over the two in-plane directions
+                axes = [a for a in range(3) if sub.shape[a] > 1]
+                c = sub
+                for a in axes:
+                    sl1 = [slice(None)] * 4
+                    sl2 = [slice(None)] * 4
+                    sl1[a] = slice(0, -1)
+                    sl2[a] = slice(1, None)
+                    c = 0.5 * (c[tuple(sl1)] + c[tuple(sl2)])
+                walls.append(c.reshape(-1, 3))
+        tree = cKDTree(np.concatenate(walls)) if walls else None
+        for b in range(L.nblocks):
+            x, y, z = coords[b]
+            md = grids.metric_derivatives(x, y, z)
+            n0 = int(L.points(0)[b])
+            for m in range(9):
+                flat = md[m].reshape(-1)
+                for n in range(nsm):
+                    xd[m][(o0[b] * nsm + n * n0) * 3:(o0[b] * nsm + (n + 1) * n0) * 3] = flat
+            cc = grids.cell_centres(x, y, z).reshape(-1, 3)
+            d = tree.query(cc)[0] if tree is not None else np.full(len(cc), 1.0e6)
+            dist[o1[b]:o1[b] + L.points(1)[b]] = d
+        return {"xd": xd, "dist": dist}
+
+    def turbulent_viscosity(self, q: np.ndarray, nl: int = 1) -> np.ndarray:
+        """tur_vis_b (src/update.f90:2297-2337) of a conservative state:
+        mut = cmu rho (rho k) / (rho omega + epst) Re / M over 0:imax,
+        0:jmax, 0:kmax of every block, zero elsewhere -- the driver's
+        initial mut after init2fs (src/cosa.f90:229-232)."""
+        c = self.config
+        L = self.levels[nl - 1]
+        mut = np.zeros(self.size(nl, "p3s"))
+        o3 = L.offsets(3)
+        for b in range(L.nblocks):
+            im, jm, km = int(L.imax[b]), int(L.jmax[b]), int(L.kmax[b])
+            n3 = (im + 3) * (jm + 3) * (km + 3)
+            for n in range(self.nsnap):
+                base = o3[b] * self.npde * self.nsnap + n * self.npde * n3
+                qq = q[base:base + self.npde * n3].reshape(self.npde, km + 3, jm + 3, im + 3)
+                m = mut[o3[b] * self.nsnap + n * n3: o3[b] * self.nsnap + (n + 1) * n3].reshape(km + 3, jm + 3, im + 3)
+                sl = (slice(1, km + 2), slice(1, jm + 2), slice(1, im + 2))
+                m[sl] = c["cmu"] * qq[0][sl] * qq[5][sl] / (qq[6][sl] + c["epst"]) * c["reyno"] / c["machfs"]
+        return mut
 
     # -- states --------------------------------------------------------------
     def freestream(self, nl: int = 1) -> np.ndarray:
@@ -127,6 +197,10 @@ class Case:
                 c["rhoinf"] * c["machfs"] * math.sin(c["alpha"]) * math.cos(c["beta"]),
                 c["rhoinf"] * c["machfs"] * math.sin(c["beta"]),
                 1 / c["rhoinf"] / c["gamma"] / (c["gamma"] - 1) + 0.5 * c["machfs"] ** 2]
+        if self.npde == 7:
+            # init2fs_b (src/input.f90:4741-4842): rho k, rho omega on the
+            # interior only (halos stay 0); the energy is the Euler value
+            vals += [c["rhoinf"] * c["tkefar"], c["rhoinf"] * c["omefar"]]
         o3 = L.offsets(3)
         for b in range(L.nblocks):
             n3 = int(L.points(3)[b])
@@ -134,7 +208,13 @@ class Case:
             for n in range(self.nsnap):
                 for v in range(self.npde):
                     s = base + (n * self.npde + v) * n3
-                    q[s:s + n3] = vals[v] if v < 5 else 0.0
+                    if v < 5:
+                        q[s:s + n3] = vals[v]
+                    else:
+                        im, jm, km = int(L.imax[b]), int(L.jmax[b]), int(L.kmax[b])
+                        blk = np.zeros((km + 3, jm + 3, im + 3))
+                        blk[2:km + 1, 2:jm + 1, 2:im + 1] = vals[v]
+                        q[s:s + n3] = blk.reshape(-1)
         return q
 
     def perturbed(self, amp: float = 0.01, seed: int = 1, nl: int = 1) -> np.ndarray:
@@ -152,7 +232,8 @@ class Case:
 # ---------------------------------------------------------------------------
 # configurations
 
-def naca_case(ni=193, nj=49, nk=5, nlevel=1, name="naca0012", pitch=None, **cfg) -> Case:
+def naca_case(ni=193, nj=49, nk=5, nlevel=1, name="naca0012", pitch=None, wall=14, r_far=20.0, ds_wall=1e-3,
+              **cfg) -> Case:
     """NACA 0012 O-grid, single block: j=1 inviscid wall (14), j=jmax
     farfield (30), k faces symmetry (80), seam i=1 <-> i=imax as a self-cut.
     pitch = {"dtheta0", "xh", "yh"} makes it a harmonic-balance pitching
@@ -162,9 +243,9 @@ def naca_case(ni=193, nj=49, nk=5, nlevel=1, name="naca0012", pitch=None, **cfg)
     if pitch is not None:
         cfg.update(moving=1, hbmove=1, lmet=7)
     c = default_config(nlevel=nlevel, **cfg)
-    x, y, z = grids.naca0012_ogrid(ni, nj, nk)
+    x, y, z = grids.naca0012_ogrid(ni, nj, nk, r_far=r_far, ds_wall=ds_wall)
     b = Block(ni, nj, nk, [
-        Patch((1, 1, 1), (ni, 1, nk), 14),
+        Patch((1, 1, 1), (ni, 1, nk), wall),
         Patch((1, nj, 1), (ni, nj, nk), 30),
         Patch((1, 1, 1), (ni, nj, 1), 80),
         Patch((1, 1, nk), (ni, nj, nk), 80),

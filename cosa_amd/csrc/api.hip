@@ -79,7 +79,8 @@ static void drain_timing(Solve& s) {
 static void free_level(Level& L) {
   for (int f = 0; f < COSAHIP_NFIELDS; f++)
     if (L.f[f]) (void)hipFree(L.f[f]);
-  void* ptrs[] = {L.dblk, L.dq, L.work1, L.cirs_r, L.rms_part, L.sbc, L.sbc_doff, L.dpatch, L.dcut};
+  void* ptrs[] = {L.dblk, L.dq, L.work1, L.cirs_r, L.rms_part, L.sbc, L.sbc_doff, L.dpatch, L.dcut,
+                  L.vflux, L.tbeta[0], L.tbeta[1], L.tbeta[2], L.h0tab, L.dxd};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < TK_NKIND; k++)
@@ -128,13 +129,61 @@ static Par make_par(const cosahip_config& c) {
   p.omega = c.omega;
   p.qmin1 = c.qmin[0];
   p.qmin5 = c.qmin[4];
+  p.kom = c.kom;
+  p.viscous = c.viscous;
+  p.foturb = c.foturb;
+  p.ilimt = c.ilimt;
+  p.turcmp = c.turcmp;
+  p.lim_prodtke = c.lim_prodtke;
+  p.lim_prodomega = c.lim_prodomega;
+  p.wallwilc = c.wallwilc;
+  p.prd_all_lvls = c.prd_all_lvls;
+  p.nlevel = c.nlevel;
+  p.efx_setup_t = c.efx_setup_t;
+  p.efx_all_t = c.efx_all_t;
+  p.eig_limit_t = c.eig_limit_t;
+  p.eig_lim_all_t = c.eig_lim_all_t;
+  for (int i = 0; i < 3; i++) p.entfx_t_w[i] = c.entfx_t_w[i];
+  p.epslimt = c.epslimt;
+  p.cflt = c.cflt;
+  p.ratt = c.ratt;
+  p.cntrpy_t = c.cntrpy_t;
+  p.entfxctf_t = c.entfxctf_t;
+  p.eig_cutoff_t = c.eig_cutoff_t;
+  p.tkefar = c.tkefar;
+  p.omefar = c.omefar;
+  p.reyno = c.reyno;
+  p.pranl = c.pranl;
+  p.prant = c.prant;
+  p.stemp = c.stemp;
+  p.cdff = c.cdff;
+  p.epst = c.epst;
+  p.cmu = c.cmu;
+  p.sigk = c.sigk;
+  p.sigw = c.sigw;
+  p.bkw = c.bkw;
+  p.bstrkw = c.bstrkw;
+  p.gkw = c.gkw;
+  p.prdlim = c.prdlim;
+  p.roughk = c.roughk;
+  p.qmin6 = c.qmin[5];
+  p.qmin7 = c.qmin[6];
   return p;
 }
 
 static int check_config(Solve* s, const cosahip_config& c) {
-  if (c.npde != 5) return fail(s, COSAHIP_ERR_UNSUPPORTED, "npde != 5: k-omega transport is not built in this version");
-  if (c.kom || c.kom_bsl || c.kom_sst || c.viscous)
-    return fail(s, COSAHIP_ERR_UNSUPPORTED, "viscous / k-omega terms are not built in this version");
+  if (c.kom_bsl || c.kom_sst)
+    return fail(s, COSAHIP_ERR_UNSUPPORTED, "k-omega BSL/SST are not built (Wilcox k-omega and laminar are)");
+  if (c.kom && !c.viscous) return fail(s, COSAHIP_ERR_ARG, "k-omega requires viscous");
+  if (c.npde != (c.kom ? 7 : 5)) return fail(s, COSAHIP_ERR_ARG, "npde must be 5 (Euler, laminar) or 7 (k-omega)");
+  if (c.kom && c.nharms > 0)
+    return fail(s, COSAHIP_ERR_UNSUPPORTED, "k-omega with harmonic balance (lim_res_ome_bhb) is not built");
+  if (c.kom && (c.prd_all_lvls || c.lim_corr[0] || c.lim_corr[1] || c.lim_corr[2]))
+    return fail(s, COSAHIP_ERR_UNSUPPORTED, "k-omega prd_all_lvls / lim_corr options are not built");
+  if (c.kom && c.ho_rest) return fail(s, COSAHIP_ERR_UNSUPPORTED, "k-omega with ho_rest (bc_res) is not built");
+  if (c.ilimt < 0 || c.ilimt > 4) return fail(s, COSAHIP_ERR_ARG, "ilimt out of range 0..4");
+  if (c.viscous && (c.reyno <= 0 || c.pranl <= 0 || c.prant <= 0))
+    return fail(s, COSAHIP_ERR_ARG, "viscous runs need reyno, pranl, prant > 0");
   if (c.nharms < 0 || c.nharms > 65) return fail(s, COSAHIP_ERR_ARG, "nharms out of range 0..65");
   if (c.nharms > 0 && !c.harbal) return fail(s, COSAHIP_ERR_ARG, "nharms > 0 requires harbal");
   if (c.lmet != 4 && c.lmet != 7) return fail(s, COSAHIP_ERR_ARG, "lmet must be 4 or 7");
@@ -256,6 +305,12 @@ static int build_tables(Solve* s, Level& L, const Level* Lc) {
     add_tiles(L.htile[TK_LINES_J], b, ns * im * km, TS);
     add_tiles(L.htile[TK_LINES_K], b, ns * im * jm, TS);
     add_tiles(L.htile[TK_LINES_I], b, ns * jm * km, 64);
+    add_tiles(L.htile[TK_LINES_I_T], b, ns * jm * km, TS);
+    if (s->par.viscous) {
+      add_tiles(L.htile[TK_FACES_I], b, ns * (im + 1) * jm * km, TS);
+      add_tiles(L.htile[TK_FACES_J], b, ns * im * (jm + 1) * km, TS);
+      add_tiles(L.htile[TK_FACES_K], b, ns * im * jm * (km + 1), TS);
+    }
     add_tiles(L.htile[TK_CELLS_P1], b, ns * (ll)(B.imax + 1) * (B.jmax + 1) * (B.kmax + 1), TS);
     add_tiles(L.htile[TK_HALO], b, ns * ((ll)(B.imax + 3) * jm * km + 4 * im * km + 4 * im * jm), TS);
     add_tiles(L.htile[TK_FUSED], b, ns * ((im + 30) / 31) * ((jm + 6) / 7) * ((km + 31) / 32), 1);
@@ -318,6 +373,7 @@ static bool cut_in_bounds(const Blk& b1, const Blk& b2, const int32_t* t, int n2
   return true;
 }
 
+static ll field_size(const Solve* s, const Level& L, int field);
 static bool strong_bc(int t) {
   return t == 13 || t == 14 || t / 100 == 15 || t == 22 || t == 23 || t == 92 || t == 93 || t == 5;
 }
@@ -402,9 +458,10 @@ extern "C" int cosahip_define_level(cosahip_solve* h, int nl, int nblocks, const
       r.count = n2 * n3c;
       r.n2 = n2 > 0 ? n2 : 1;
       const int ty = t[0];
-      if (ty != 30 && ty != 14 && ty != 80 && ty != 4) {
-        char msg[96];
-        snprintf(msg, sizeof msg, "BC type %d is not built in this version (30, 14, 80, 4 are)", ty);
+      if (ty != 30 && ty != 14 && ty != 80 && ty != 4 && !(ty / 100 == 15 && s->par.viscous)) {
+        char msg[112];
+        snprintf(msg, sizeof msg, "BC type %d is not built in this version (30, 14, 80, 4 and, viscous, 15xx are)",
+                 ty);
         return fail(s, COSAHIP_ERR_UNSUPPORTED, msg);
       }
       L.hpatch.push_back(r);
@@ -413,23 +470,28 @@ extern "C" int cosahip_define_level(cosahip_solve* h, int nl, int nblocks, const
         const int face = 2 * (idir - 1) + (t[2] == 1 ? 0 : 1);
         ll off = L.sbc_off[b];
         ll w;
+        // flag byte: bit 0 strong BC; bits 1/2 first / last cell of the patch
+        // along the lower-numbered transverse direction, bits 3/4 along the
+        // other one (one-sided differences of bq_der at patch ends)
+        int r1, r2;   // bctopo slots of the two transverse ranges
         if (idir == 1) {
           off += (face - 0) * (ll)(B.jmax + 1) * (B.kmax + 1);
           w = B.jmax + 1;
-          for (int kk = t[7]; kk <= t[8]; kk++)
-            for (int jj = t[5]; jj <= t[6]; jj++) sbc[off + jj + w * kk] = 1;
+          r1 = 5; r2 = 7;
         } else if (idir == 2) {
           off += 2 * (ll)(B.jmax + 1) * (B.kmax + 1) + (face - 2) * (ll)(B.imax + 1) * (B.kmax + 1);
           w = B.imax + 1;
-          for (int kk = t[7]; kk <= t[8]; kk++)
-            for (int ii = t[3]; ii <= t[4]; ii++) sbc[off + ii + w * kk] = 1;
+          r1 = 3; r2 = 7;
         } else {
           off += 2 * (ll)(B.jmax + 1) * (B.kmax + 1) + 2 * (ll)(B.imax + 1) * (B.kmax + 1) +
                  (face - 4) * (ll)(B.imax + 1) * (B.jmax + 1);
           w = B.imax + 1;
-          for (int jj = t[5]; jj <= t[6]; jj++)
-            for (int ii = t[3]; ii <= t[4]; ii++) sbc[off + ii + w * jj] = 1;
+          r1 = 3; r2 = 5;
         }
+        for (int c2 = t[r2]; c2 <= t[r2 + 1]; c2++)
+          for (int c1 = t[r1]; c1 <= t[r1 + 1]; c1++)
+            sbc[off + c1 + w * c2] = (unsigned char)(1 | (c1 == t[r1]) << 1 | (c1 == t[r1 + 1]) << 2 |
+                                                     (c2 == t[r2]) << 3 | (c2 == t[r2 + 1]) << 4);
       }
     }
   }
@@ -460,15 +522,30 @@ extern "C" int cosahip_define_level(cosahip_solve* h, int nl, int nblocks, const
   CK(upload_vec(&L.sbc_doff, L.sbc_off));
   // fields
   const Par& p = s->par;
-  const ll sizes[COSAHIP_NFIELDS] = {
-      L.n3tot * p.npde * p.nsnap, L.n3tot * p.npde * p.nsnap, L.n3tot * p.npde * p.nsnap, L.n3tot * p.npde * p.nsnap,
-      L.n2tot * p.lmet * p.nsnap_met, L.n2tot * p.lmet * p.nsnap_met, L.n2tot * p.lmet * p.nsnap_met, L.n1tot,
-      L.n1tot * p.nsnap, L.n1tot * p.nsnap, L.n1tot * p.nsnap, L.n1tot * p.nsnap,
-      p.moving ? L.n2tot * p.nsnap_met : 1, p.moving ? L.n2tot * p.nsnap_met : 1, p.moving ? L.n2tot * p.nsnap_met : 1};
   for (int f = 0; f < COSAHIP_NFIELDS; f++) {
-    if (hipMalloc(&L.f[f], sizes[f] * sizeof(double)) != hipSuccess)
+    const ll sz = std::max<ll>(field_size(s, L, f), 1);
+    if (hipMalloc(&L.f[f], sz * sizeof(double)) != hipSuccess)
       return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
-    CK(hipMemset(L.f[f], 0, sizes[f] * sizeof(double)));
+    CK(hipMemset(L.f[f], 0, sz * sizeof(double)));
+  }
+  if (p.viscous) {
+    const ll nv = L.n0tot * p.npde * p.nsnap;
+    if (hipMalloc(&L.vflux, nv * sizeof(double)) != hipSuccess)
+      return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
+    CK(hipMemset(L.vflux, 0, nv * sizeof(double)));
+  }
+  if (p.kom) {
+    for (int d = 0; d < 3; d++) {
+      if (hipMalloc(&L.tbeta[d], L.n1tot * p.nsnap * sizeof(double)) != hipSuccess)
+        return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
+      CK(hipMemset(L.tbeta[d], 0, L.n1tot * p.nsnap * sizeof(double)));
+    }
+    if (hipMalloc(&L.h0tab, (ll)nblocks * p.nsnap * 6 * sizeof(double)) != hipSuccess)
+      return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
+  }
+  if (p.viscous) {
+    std::vector<double*> xdp(L.f + COSAHIP_XIDERI, L.f + COSAHIP_XIDERI + 9);
+    CK(upload_vec(&L.dxd, xdp));
   }
   const ll nq = L.n3tot * p.npde * p.nsnap;
   if (hipMalloc(&L.dq, nq * sizeof(double)) != hipSuccess || hipMalloc(&L.work1, nq * sizeof(double)) != hipSuccess ||
@@ -497,7 +574,12 @@ static ll field_size(const Solve* s, const Level& L, int field) {
     case COSAHIP_VOL: return L.n1tot;
     case COSAHIP_DTVOL: case COSAHIP_BETAXI: case COSAHIP_BETAETA: case COSAHIP_BETAZETA: return L.n1tot * p.nsnap;
     case COSAHIP_XDOT: case COSAHIP_YDOT: case COSAHIP_ZDOT: return p.moving ? L.n2tot * p.nsnap_met : 0;
+    case COSAHIP_MUT: case COSAHIP_PRD: case COSAHIP_DIVVEL: case COSAHIP_DELPLS:
+      return p.kom ? L.n3tot * p.nsnap : 0;
+    case COSAHIP_DIST: return p.viscous ? L.n1tot : 0;
+    case COSAHIP_DTVOLT: return p.kom ? L.n1tot * p.nsnap : 0;
   }
+  if (field >= COSAHIP_XIDERI && field <= COSAHIP_ZETADERK) return p.viscous ? L.n0tot * 3 * p.nsnap_met : 0;
   return -1;
 }
 
@@ -680,7 +762,7 @@ static int rms(Solve* s, Level& L, double* out) {
   } else {
     CK(launch_rms(*s, L, L.f[COSAHIP_RES], L.f[COSAHIP_RHS], 1));
   }
-  CK(hipMemcpyAsync(s->hpinned, L.rms_part + (ll)L.ntile[TK_CELLS] * 8, 5 * sizeof(double), hipMemcpyDeviceToHost,
+  CK(hipMemcpyAsync(s->hpinned, L.rms_part + (ll)L.ntile[TK_CELLS] * 8, np * sizeof(double), hipMemcpyDeviceToHost,
                     s->stream));
   CK(hipStreamSynchronize(s->stream));
   double sum[8] = {0};
@@ -721,6 +803,26 @@ extern "C" int cosahip_cirs(cosahip_solve* h, int nl) {
   CK(launch_cirs(*s, *L));
   return COSAHIP_OK;
 }
+extern "C" int cosahip_tur_vis(cosahip_solve* h, int nl, int field) {
+  LEVEL_OR_FAIL(s, nl, L);
+  if (!s->par.kom) return fail(s, COSAHIP_ERR_STATE, "tur_vis: not a k-omega configuration");
+  double* q = qfield(*L, field);
+  if (!q) return fail(s, COSAHIP_ERR_ARG, "tur_vis: field must be Q or QOLD");
+  CK(launch_tur_vis(*s, *L, q));
+  return COSAHIP_OK;
+}
+extern "C" int cosahip_fixq(cosahip_solve* h, int nl) {
+  LEVEL_OR_FAIL(s, nl, L);
+  if (!s->par.kom) return fail(s, COSAHIP_ERR_STATE, "fixq: not a k-omega configuration");
+  CK(launch_fixq(*s, *L));
+  return COSAHIP_OK;
+}
+extern "C" int cosahip_turb_limits(cosahip_solve* h, int nl, double ar) {
+  LEVEL_OR_FAIL(s, nl, L);
+  if (!s->par.kom) return fail(s, COSAHIP_ERR_STATE, "turb_limits: not a k-omega configuration");
+  CK(launch_turb_limits(*s, *L, ar));
+  return COSAHIP_OK;
+}
 extern "C" int cosahip_update(cosahip_solve* h, int nl, double ar) {
   LEVEL_OR_FAIL(s, nl, L);
   CK(launch_update(*s, *L, ar));
@@ -757,6 +859,11 @@ extern "C" int cosahip_smooth(cosahip_solve* h, int nl, int from_fine, int to_co
         CK(launch_bc(*s, L, q));
         CK(launch_cut(*s, L, q));
       }
+      if (c.kom && mgit == 1 && iter == 1 && irk == 1 && from_fine && !c.prd_all_lvls) {
+        // first visit of a coarse level: mut from the restricted state
+        TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
+        CK(launch_tur_vis(*s, L, q));
+      }
       {
         TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
         CK(launch_prtest(*s, L, q, seq));
@@ -792,6 +899,10 @@ extern "C" int cosahip_smooth(cosahip_solve* h, int nl, int from_fine, int to_co
         CK(launch_add_interior(*s, L, L.f[COSAHIP_RES], L.f[COSAHIP_RHS], -1.0));
       }
       if (iter == niter + 1) {
+        if (c.kom) {
+          TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
+          CK(launch_turb_limits(*s, L, c.alphas[irk - 1]));
+        }
         returned_early = true;
         break;
       }
@@ -802,6 +913,11 @@ extern "C" int cosahip_smooth(cosahip_solve* h, int nl, int from_fine, int to_co
       {
         TimeScope t(*s, COSAHIP_FAM_UPDATE, cs);
         CK(launch_update(*s, L, c.alphas[irk - 1]));
+      }
+      if (c.kom) {
+        TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
+        if (c.ifixq == 1) CK(launch_fixq(*s, L));
+        if (irk == c.nstage && (nl == 1 || !c.prd_all_lvls)) CK(launch_tur_vis(*s, L, q));
       }
     }
     if (returned_early) break;
@@ -847,6 +963,12 @@ extern "C" int cosahip_restrict(cosahip_solve* h, int nl) {
   Level& L2 = s->lev[nl];
   TimeScope t(*s, COSAHIP_FAM_OTHER, L2.ncell * s->par.nsnap);
   CK(launch_rest1(*s, L1, L2, L1.f[COSAHIP_Q], L2.f[COSAHIP_Q]));
+  if (s->cfg.kom && !s->cfg.prd_all_lvls) {
+    // rest4 of the production, delpls and divergence (src/cosa.f90:2734-2739)
+    CK(launch_rest_scalar(*s, L1, L2, L1.f[COSAHIP_PRD], L2.f[COSAHIP_PRD]));
+    CK(launch_rest_scalar(*s, L1, L2, L1.f[COSAHIP_DELPLS], L2.f[COSAHIP_DELPLS]));
+    CK(launch_rest_scalar(*s, L1, L2, L1.f[COSAHIP_DIVVEL], L2.f[COSAHIP_DIVVEL]));
+  }
   CK(launch_rest2(*s, L1, L2, L2.f[COSAHIP_RHS], L1.f[COSAHIP_RES]));
   return COSAHIP_OK;
 }

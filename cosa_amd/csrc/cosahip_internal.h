@@ -41,6 +41,13 @@ struct Par {
   double qfst[5];
   double omega;
   double qmin1, qmin5;
+  // viscous flow and Wilcox k-omega
+  int kom, viscous, foturb, ilimt, turcmp, lim_prodtke, lim_prodomega, wallwilc, prd_all_lvls, nlevel;
+  int efx_setup_t, efx_all_t, eig_limit_t, eig_lim_all_t;
+  int entfx_t_w[3];
+  double epslimt, cflt, ratt, cntrpy_t, entfxctf_t, eig_cutoff_t;
+  double tkefar, omefar, reyno, pranl, prant, stemp, cdff, epst, cmu, sigk, sigw, bkw, bstrkw, gkw, prdlim, roughk;
+  double qmin6, qmin7;
 };
 
 __host__ __device__ inline ll n3of(const Blk& b) { return (ll)(b.imax + 3) * (b.jmax + 3) * (b.kmax + 3); }
@@ -75,6 +82,10 @@ enum TileKind {
   TK_PROL,        // 2x2x2 prolongation groups 0..imax-1 step 2, per block
   TK_HALO,        // prtest region: interior + face halo strips
   TK_FUSED,       // fused residual: (31 x 7)-cell (i,j) tiles x 32-plane k segments
+  TK_LINES_I_T,   // (jmax-1)(kmax-1) lines along i, TS per tile (thread per line)
+  TK_FACES_I,     // i faces imax x (jmax-1) x (kmax-1)
+  TK_FACES_J,     // j faces (imax-1) x jmax x (kmax-1)
+  TK_FACES_K,     // k faces (imax-1) x (jmax-1) x kmax
   TK_NKIND
 };
 
@@ -116,6 +127,11 @@ struct Level {
   CutRec* dcut = nullptr;
   int maxcut = 0;
   bool thin = false;          // a block has < 2 cells in some direction
+  // viscous / k-omega scratch
+  double* vflux = nullptr;    // face fluxes (imax, jmax, kmax, npde, 0:2N), off_0 offsets
+  double* tbeta[3] = {};      // CIRS coefficients of k, omega (P1, 0:2N)
+  double* h0tab = nullptr;    // per (block, snapshot, direction): h0l, h0r of the block's last face
+  double** dxd = nullptr;     // device table of the nine metric-derivative fields
 };
 
 struct TimedLaunch {

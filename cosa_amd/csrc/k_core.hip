@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) k_cons2prim(Par p, const Blk* __restrict_
   double u = qb[x + n3] / rho, v = qb[x + 2 * n3] / rho, w = qb[x + 3 * n3] / rho;
   for (int m = 5; m < p.npde; m++) qb[x + m * n3] = qb[x + m * n3] / rho;
   double rhoe = qb[x + 4 * n3];
-  double tke = 0.0;
+  double tke = p.kom ? qb[x + 5 * n3] : 0.0;
   qb[x + n3] = u;
   qb[x + 2 * n3] = v;
   qb[x + 3 * n3] = w;
@@ -108,9 +108,9 @@ __global__ void __launch_bounds__(256) k_prim2cons(Par p, const Blk* __restrict_
   const ll n3 = n3of(b), x = i3(b, i, j, k);
   double rho = qb[x];
   double rhou = qb[x + n3] * rho, rhov = qb[x + 2 * n3] * rho, rhow = qb[x + 3 * n3] * rho;
+  double tke = p.kom ? qb[x + 5 * n3] : 0.0;   // k saved before the scaling (work(:,1) of prim2cons_b)
   for (int m = 5; m < p.npde; m++) qb[x + m * n3] = qb[x + m * n3] * rho;
   double pp = qb[x + 4 * n3];
-  double tke = 0.0;
   qb[x + n3] = rhou;
   qb[x + 2 * n3] = rhov;
   qb[x + 3 * n3] = rhow;
@@ -181,19 +181,44 @@ __global__ void __launch_bounds__(256) k_sub_box(Par p, const Blk* __restrict__ 
 }
 
 // ---- update_b / update_bhb explicit (src/update.f90:568-624, 1359-1408) ----
+// With Wilcox k-omega: the point-implicit rho*omega, rho*k update first
+// (src/update.f90:596-634), reading rho, k, omega of the current state;
+// ttrm(:,2:3) of kom are the constants gkw, bkw.
 __global__ void __launch_bounds__(256) k_update(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
                                                 const double* __restrict__ res, double* __restrict__ q,
                                                 const double* __restrict__ qold, const double* __restrict__ dtvol,
-                                                double ar) {
+                                                double ar, const double* __restrict__ vol,
+                                                const double* __restrict__ dtvolt, const double* __restrict__ delpls,
+                                                const double* __restrict__ prd) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   CellIdx c;
   if (!cell_of(p, b, t.base + threadIdx.x, c)) return;
   const ll n3 = n3of(b), x = i3(b, c.i, c.j, c.k);
-  const double dt = dtvol[b.o1 * p.nsnap + n1of(b) * (ll)c.n + i1(b, c.i, c.j, c.k)];
+  const ll po = b.o1 * p.nsnap + n1of(b) * (ll)c.n + i1(b, c.i, c.j, c.k);
+  const double dt = dtvol[po];
   double* qb = qblk(p, b, q, c.n);
   const double* ob = qblk(p, b, qold, c.n);
   const double* rb = qblk(p, b, res, c.n);
+  if (p.kom) {
+    const double rho = qb[x];
+    const double ome = qb[x + 6 * n3] / qb[x];
+    const double tke = qb[x + 5 * n3] / qb[x];
+    const double dtt = dtvolt[po];
+    const double dtau = dtt * vol[b.o1 + i1(b, c.i, c.j, c.k)];
+    const double gav = p.gkw, bav = p.bkw;
+    const ll so = b.o3 * p.nsnap + n3 * (ll)c.n + x;
+    const double dl = delpls[so];
+    double rhomega = (ob[x + 6 * n3] - ar * rb[x + 6 * n3] * dtt + ar * dtau * rho * ome * (gav * dl + 2 * bav * ome)) /
+                     (1 + ar * dtau * (gav * dl + 2 * bav * ome));
+    const double rhomegamin = gav * p.cmu * rho * sqrt(prd[so]);
+    if (rhomega < rhomegamin) rhomega = rhomegamin;
+    qb[x + 6 * n3] = rhomega;
+    const double drhomega = rhomega - rho * ome;
+    qb[x + 5 * n3] = (ob[x + 5 * n3] - ar * rb[x + 5 * n3] * dtt + ar * dtau * rho * tke * (dl + p.bstrkw * ome) -
+                      ar * dtau * p.bstrkw * tke * drhomega) /
+                     (1 + ar * dtau * (dl + p.bstrkw * ome));
+  }
   for (int v = 0; v < 5; v++) qb[x + v * n3] = ob[x + v * n3] - ar * rb[x + v * n3] * dt;
 }
 
@@ -231,43 +256,45 @@ __global__ void __launch_bounds__(256) k_src_hb(Par p, const Blk* __restrict__ b
 __global__ void __launch_bounds__(256) k_rms_part(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
                                                   const double* __restrict__ res, const double* __restrict__ rhs,
                                                   int use_rhs, double* __restrict__ part) {
-  __shared__ double sh[5][256];
+  __shared__ double sh[7][256];
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   CellIdx c;
-  double s[5] = {0, 0, 0, 0, 0};
+  double s[7] = {0, 0, 0, 0, 0, 0, 0};
+  const int nv = p.npde;
   if (cell_of(p, b, t.base + threadIdx.x, c)) {
     const ll n3 = n3of(b), x = i3(b, c.i, c.j, c.k);
     const double* rb = qblk(p, b, res, c.n);
     const double* hb = use_rhs ? qblk(p, b, rhs, c.n) : nullptr;
-    for (int v = 0; v < 5; v++) {
+    for (int v = 0; v < nv; v++) {
       double d = use_rhs ? rb[x + v * n3] - hb[x + v * n3] : rb[x + v * n3];
       s[v] = d * d;
     }
   }
-  for (int v = 0; v < 5; v++) sh[v][threadIdx.x] = s[v];
+  for (int v = 0; v < nv; v++) sh[v][threadIdx.x] = s[v];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w)
-      for (int v = 0; v < 5; v++) sh[v][threadIdx.x] += sh[v][threadIdx.x + w];
+      for (int v = 0; v < nv; v++) sh[v][threadIdx.x] += sh[v][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x < 5) part[(ll)blockIdx.x * 8 + threadIdx.x] = sh[threadIdx.x][0];
+  if ((int)threadIdx.x < nv) part[(ll)blockIdx.x * 8 + threadIdx.x] = sh[threadIdx.x][0];
 }
 // fixed-order final reduction: one workgroup, strided partial sums then tree
-__global__ void __launch_bounds__(256) k_rms_final(const double* __restrict__ part, int nt, double* __restrict__ out) {
-  __shared__ double sh[5][256];
-  double s[5] = {0, 0, 0, 0, 0};
+__global__ void __launch_bounds__(256) k_rms_final(const double* __restrict__ part, int nt, int nv,
+                                                   double* __restrict__ out) {
+  __shared__ double sh[7][256];
+  double s[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int t = threadIdx.x; t < nt; t += 256)
-    for (int v = 0; v < 5; v++) s[v] += part[(ll)t * 8 + v];
-  for (int v = 0; v < 5; v++) sh[v][threadIdx.x] = s[v];
+    for (int v = 0; v < nv; v++) s[v] += part[(ll)t * 8 + v];
+  for (int v = 0; v < nv; v++) sh[v][threadIdx.x] = s[v];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w)
-      for (int v = 0; v < 5; v++) sh[v][threadIdx.x] += sh[v][threadIdx.x + w];
+      for (int v = 0; v < nv; v++) sh[v][threadIdx.x] += sh[v][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x < 5) out[threadIdx.x] = sh[threadIdx.x][0];
+  if ((int)threadIdx.x < nv) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // ---- bprtest (src/cosa.f90:960-1120): first offender in loop order ----
@@ -301,7 +328,8 @@ __device__ __forceinline__ int cycd(int d, int m) { return (d - 1 + m - 1) % 3 +
 
 __global__ void __launch_bounds__(256) k_bc(Par p, const Blk* __restrict__ blks, const PatchRec* __restrict__ pr,
                                             int first, double* __restrict__ q, const double* __restrict__ si,
-                                            const double* __restrict__ sj, const double* __restrict__ sk) {
+                                            const double* __restrict__ sj, const double* __restrict__ sk,
+                                            const double* __restrict__ dist) {
   const PatchRec rec = pr[first + blockIdx.y];
   const int item = blockIdx.x * blockDim.x + threadIdx.x;
   if (item >= rec.count * p.nsnap) return;
@@ -355,6 +383,56 @@ __global__ void __launch_bounds__(256) k_bc(Par p, const Blk* __restrict__ blks,
   const double* sm = s + b.o2 * p.lmet * p.nsnap_met + ((ll)nh * n2of(b) + i2(b, im_, jm_, km_)) * p.lmet;
   double hv[5];
   bool set = false;
+  if (type / 100 == 15) {   // bc_walvis_15 (src/bc.f90:2144-2492)
+    int in2, jn2, kn2;
+    bc_ijk(ic1, ic2, ic3, ibcn2, i2_, i3_, in2, jn2, kn2);
+    const ll xn2 = i3(b, in2, jn2, kn2);
+    double uw = 0.0, vw = 0.0, ww = 0.0;
+    if (p.moving) {
+      uw = sm[4];
+      vw = sm[5];
+      ww = sm[6];
+    }
+    const double p1 = qb[xn + 4 * n3], p2 = qb[xn2 + 4 * n3];
+    const double pw = 0.5 * (3 * p1 - p2);
+    const double rhow = pow(pw / p1, 1 / p.gamma) * qb[xn];
+    hv[0] = rhow;
+    hv[1] = uw;
+    hv[2] = vw;
+    hv[3] = ww;
+    hv[4] = pw;
+    for (int v = 0; v < 5; v++) {
+      qb[xb + v * n3] = hv[v];
+      qb[xb2 + v * n3] = hv[v];
+    }
+    if (p.kom) {
+      const double tw = pw * p.gamma / rhow;
+      const double muw = (p.stemp + 1) / (tw + p.stemp) * pow(tw, 1.5);
+      const double dn = dist[b.o1 + i1(b, in, jn, kn)];
+      double omew;
+      if (p.wallwilc) {
+        const double mor = p.machfs / p.reyno;
+        const double nx = sm[0], ny = sm[1], nz = sm[2];
+        const double uc = qb[xn + n3] - uw, vc = qb[xn + 2 * n3] - vw, wc = qb[xn + 3 * n3] - ww;
+        const double uorth = nx * uc + ny * vc + nz * wc;
+        const double a1 = uc - uorth * nx, a2 = vc - uorth * ny, a3 = wc - uorth * nz;
+        const double upar = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+        const double dupardn = upar / (dn + 1.e-12);
+        const double tauw = muw * fabs(dupardn);
+        double rkrpls = 1.0;
+        if (p.roughk > 0.0) rkrpls = fmax(1.0, sqrt(rhow * tauw / mor) * p.roughk / muw);
+        const double sr = rkrpls >= 25.0 ? 100.0 / rkrpls : 2500.0 / (rkrpls * rkrpls);
+        omew = fabs(dupardn) * sr;
+      } else {
+        omew = 60.0 * muw / (p.bkw * rhow * (dn * dn) + 1.e-24) * p.machfs / p.reyno;
+      }
+      qb[xb + 5 * n3] = 0.0;
+      qb[xb + 6 * n3] = omew;
+      qb[xb2 + 5 * n3] = 0.0;
+      qb[xb2 + 6 * n3] = omew;
+    }
+    return;
+  }
   if (type == 30) {   // bc_far0 (src/bc.f90:124-403)
     const double gamma = p.gamma;
     const double sinf = p.pinf / pow(p.rhoinf, gamma);
@@ -399,6 +477,16 @@ __global__ void __launch_bounds__(256) k_bc(Par p, const Blk* __restrict__ blks,
         set = true;
       }
     }
+    if (p.kom) {   // src/bc.f90:385-479: k, omega fixed on inflow, extrapolated on outflow
+      const double t6 = qdotn <= 0.0 ? p.tkefar : qb[xn + 5 * n3];
+      const double t7 = qdotn <= 0.0 ? p.omefar : qb[xn + 6 * n3];
+      if (qdotn <= 0.0 || qdotn > 0.0) {
+        qb[xb + 5 * n3] = t6;
+        qb[xb + 6 * n3] = t7;
+        qb[xb2 + 5 * n3] = t6;
+        qb[xb2 + 6 * n3] = t7;
+      }
+    }
   } else if (type == 14) {   // bc_walinv_14 (src/bc.f90:1457-1640)
     int in2, jn2, kn2;
     bc_ijk(ic1, ic2, ic3, ibcn2, i2_, i3_, in2, jn2, kn2);
@@ -421,6 +509,11 @@ __global__ void __launch_bounds__(256) k_bc(Par p, const Blk* __restrict__ blks,
     hv[3] = ww - unw * nz;
     hv[4] = pw;
     set = true;
+    if (p.kom)   // src/bc.f90:1634-1670: zeroth-order extrapolation of k, omega
+      for (int v = 5; v <= 6; v++) {
+        qb[xb + v * n3] = qb[xn + v * n3];
+        qb[xb2 + v * n3] = qb[xn + v * n3];
+      }
   }
   if (set)
     for (int v = 0; v < 5; v++) {
@@ -727,7 +820,8 @@ hipError_t launch_sub_box(Solve& s, Level& L, double* dst, const double* src) {
 hipError_t launch_update(Solve& s, Level& L, double ar) {
   if (L.ntile[TK_CELLS])
     hipLaunchKernelGGL(k_update, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_CELLS],
-                       L.f[COSAHIP_RES], L.f[COSAHIP_Q], L.f[COSAHIP_QOLD], L.f[COSAHIP_DTVOL], ar);
+                       L.f[COSAHIP_RES], L.f[COSAHIP_Q], L.f[COSAHIP_QOLD], L.f[COSAHIP_DTVOL], ar,
+                       L.f[COSAHIP_VOL], L.f[COSAHIP_DTVOLT], L.f[COSAHIP_DELPLS], L.f[COSAHIP_PRD]);
   return hipGetLastError();
 }
 hipError_t launch_src_hb(Solve& s, Level& L) {
@@ -744,7 +838,8 @@ hipError_t launch_rms(Solve& s, Level& L, const double* res, const double* rhs, 
   if (!nt) return hipSuccess;
   hipLaunchKernelGGL(k_rms_part, grid1(nt), dim3(256), 0, s.stream, s.par, L.dblk, L.tile[TK_CELLS], res, rhs,
                      use_rhs, L.rms_part);
-  hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(256), 0, s.stream, L.rms_part, nt, L.rms_part + (ll)nt * 8);
+  hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(256), 0, s.stream, L.rms_part, nt, s.par.npde,
+                     L.rms_part + (ll)nt * 8);
   return hipGetLastError();
 }
 hipError_t launch_prtest(Solve& s, Level& L, const double* q, int stage_seq) {
@@ -759,14 +854,14 @@ hipError_t launch_bc(Solve& s, Level& L, double* q) {
   if (!L.thin) {
     dim3 g((L.maxpatch * s.par.nsnap + TS - 1) / TS, np);
     hipLaunchKernelGGL(k_bc, g, dim3(TS), 0, s.stream, s.par, L.dblk, L.dpatch, 0, q, L.f[COSAHIP_SI],
-                       L.f[COSAHIP_SJ], L.f[COSAHIP_SK]);
+                       L.f[COSAHIP_SJ], L.f[COSAHIP_SK], L.f[COSAHIP_DIST]);
   } else {
     // a block one cell thick: patches read halos written by earlier
     // patches, so keep the reference's patch order (one launch per patch)
     for (int ip = 0; ip < np; ip++) {
       dim3 g((L.hpatch[ip].count * s.par.nsnap + TS - 1) / TS, 1);
       hipLaunchKernelGGL(k_bc, g, dim3(TS), 0, s.stream, s.par, L.dblk, L.dpatch, ip, q, L.f[COSAHIP_SI],
-                         L.f[COSAHIP_SJ], L.f[COSAHIP_SK]);
+                         L.f[COSAHIP_SJ], L.f[COSAHIP_SK], L.f[COSAHIP_DIST]);
     }
   }
   return hipGetLastError();
@@ -792,6 +887,39 @@ hipError_t launch_edges_corners(Solve& s, Level& L, double* q) {
   hipLaunchKernelGGL(k_edges, g, dim3(TS), 0, s.stream, s.par, L.dblk, q);
   dim3 g2((8 * s.par.nsnap + 63) / 64, L.nblocks);
   hipLaunchKernelGGL(k_corners, g2, dim3(64), 0, s.stream, s.par, L.dblk, q);
+  return hipGetLastError();
+}
+hipError_t launch_edges(Solve& s, Level& L, double* q) {
+  int mx = 0;
+  for (auto& b : L.hblk) mx = std::max(mx, 4 * ((b.imax - 1) + (b.jmax - 1) + (b.kmax - 1)));
+  dim3 g((mx * s.par.nsnap + TS - 1) / TS, L.nblocks);
+  hipLaunchKernelGGL(k_edges, g, dim3(TS), 0, s.stream, s.par, L.dblk, q);
+  return hipGetLastError();
+}
+// cut copies of a P3 array with ncomp variables per snapshot (mut: 1)
+hipError_t launch_cut_ncomp(Solve& s, Level& L, double* a, int ncomp) {
+  const int nc = (int)L.hcut.size();
+  if (!nc) return hipSuccess;
+  Par p = s.par;
+  p.npde = ncomp;
+  if (!L.thin) {
+    dim3 g((L.maxcut + TS - 1) / TS, nc);
+    hipLaunchKernelGGL(k_cut, g, dim3(TS), 0, s.stream, p, L.dblk, L.dcut, 0, a);
+  } else {
+    for (int ic = 0; ic < nc; ic++) {
+      dim3 g((L.hcut[ic].count + TS - 1) / TS, 1);
+      hipLaunchKernelGGL(k_cut, g, dim3(TS), 0, s.stream, p, L.dblk, L.dcut, ic, a);
+    }
+  }
+  return hipGetLastError();
+}
+// rest4_b (src/cosa.f90:3122-3164): rest1 of a scalar P3 field
+hipError_t launch_rest_scalar(Solve& s, Level& L1, Level& L2, const double* a1, double* a2) {
+  Par p = s.par;
+  p.npde = 1;
+  if (L1.ntile[TK_COARSE])
+    hipLaunchKernelGGL(k_rest, grid1(L1.ntile[TK_COARSE]), dim3(TS), 0, s.stream, p, L1.dblk, L2.dblk,
+                       L1.tile[TK_COARSE], 1, 0, a1, a2, L1.f[COSAHIP_VOL], L2.f[COSAHIP_VOL]);
   return hipGetLastError();
 }
 hipError_t launch_rest1(Solve& s, Level& L1, Level& L2, const double* q1, double* q2) {

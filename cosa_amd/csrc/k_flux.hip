@@ -15,13 +15,14 @@ namespace cosahip {
 
 // ------------------------------------------------------------------------
 // Roe flux-difference splitting for one face (mean-flow variables).
+template <int NV>
 __device__ __forceinline__ void roe_face(const Par& p, const double* qr, const double* ql, double kx, double ky,
-                                         double kz, double ds, double ubn, double* F) {
+                                         double kz, double ds, double ubn, double* F, double* h0lr = nullptr) {
   const double gamma = p.gamma;
   double rhor = qr[0], ur = qr[1], vr = qr[2], wr = qr[3], pr = qr[4];
   double unr = ur * kx + vr * ky + wr * kz - ubn;
   double q2r = ur * ur + vr * vr + wr * wr;
-  double tker = 0.0;
+  double tker = NV > 5 ? qr[5] : 0.0;
   double h0r = gamma / (gamma - 1) * pr / rhor + q2r / 2 + tker;
   double rh0r = rhor * h0r;
   double fr0 = rhor * unr;
@@ -32,8 +33,12 @@ __device__ __forceinline__ void roe_face(const Par& p, const double* qr, const d
   double rhol = ql[0], ul = ql[1], vl = ql[2], wl = ql[3], pl = ql[4];
   double unl = ul * kx + vl * ky + wl * kz - ubn;
   double q2l = ul * ul + vl * vl + wl * wl;
-  double tkel = 0.0;
+  double tkel = NV > 5 ? ql[5] : 0.0;
   double h0l = gamma / (gamma - 1) * pl / rhol + q2l / 2 + tkel;
+  if (h0lr) {
+    h0lr[0] = h0l;
+    h0lr[1] = h0r;
+  }
   double rh0l = rhol * h0l;
   double fl0 = rhol * unl;
   double fl1 = (rhol * ul * unl + pl * kx);
@@ -83,6 +88,58 @@ __device__ __forceinline__ void roe_face(const Par& p, const double* qr, const d
   F[2] = (fr2 + fl2 - df2) / 2 * ds;
   F[3] = (fr3 + fl3 - df3) / 2 * ds;
   F[4] = (fr4 + fl4 - df4) / 2 * ds;
+}
+
+// Turbulence part of roflux_b (src/flux.f90:250-383): k and omega upwinded
+// with the mean-flow waves.  h0l/h0r are those of the block's last face of
+// the mean-flow loop (the reference reuses the variables), from h0tab.
+__device__ __forceinline__ void roe_turb(const Par& p, const double* qr, const double* ql, double kx, double ky,
+                                         double kz, double ds, double ubn, double h0l, double h0r, double* F) {
+  const double gamma = p.gamma;
+  double rhor = qr[0], ur = qr[1], vr = qr[2], wr = qr[3], pr = qr[4], omer = qr[6];
+  double unr = ur * kx + vr * ky + wr * kz - ubn;
+  double tker = qr[5];
+  double fr1 = rhor * unr * tker, fr2 = rhor * unr * omer;
+  double rhol = ql[0], ul = ql[1], vl = ql[2], wl = ql[3], pl = ql[4], omel = ql[6];
+  double unl = ul * kx + vl * ky + wl * kz - ubn;
+  double tkel = ql[5];
+  double fl1 = rhol * unl * tkel, fl2 = rhol * unl * omel;
+  double drho = rhor - rhol, dp = pr - pl, dun = unr - unl, dtke = tker - tkel, dome = omer - omel;
+  double rhols = sqrt(rhol), rhors = sqrt(rhor);
+  double rho = rhols * rhors;
+  double denom = rhols + rhors;
+  double u = (rhols * ul + rhors * ur) / denom;
+  double v = (rhols * vl + rhors * vr) / denom;
+  double w = (rhols * wl + rhors * wr) / denom;
+  double un = u * kx + v * ky + w * kz;
+  double h0 = (rhols * h0l + rhors * h0r) / denom;
+  double tke = (rhols * tkel + rhors * tker) / denom;
+  double ome = (rhols * omel + rhors * omer) / denom;
+  double q2 = u * u + v * v + w * w;
+  double a2 = (gamma - 1) * (h0 - q2 / 2 - tke);
+  double a = sqrt(a2);
+  double rm123 = fabs(un - ubn), rm4 = fabs(un + a - ubn), rm5 = fabs(un - a - ubn);
+  if (p.efx_setup_t) {
+    double rdelta = p.cntrpy_t * (p.entfx_t_w[0] * p.entfxctf_t + p.entfx_t_w[1] * (sqrt(u * u + v * v + w * w) + a) +
+                                  p.entfx_t_w[2] * 0.2 * (fabs(un) + a + fabs(ubn)));
+    rdelta = fmax(rdelta, 1.e-12);
+    if (rm123 <= rdelta && p.efx_all_t) rm123 = 0.5 * (rm123 * rm123 + rdelta * rdelta) / rdelta;
+    if (rm4 <= rdelta) rm4 = 0.5 * (rm4 * rm4 + rdelta * rdelta) / rdelta;
+    if (rm5 <= rdelta) rm5 = 0.5 * (rm5 * rm5 + rdelta * rdelta) / rdelta;
+  }
+  if (p.eig_limit_t) {
+    if (rm123 <= p.eig_cutoff_t) rm123 = p.eig_cutoff_t;
+    if (p.eig_lim_all_t && rm4 <= p.eig_cutoff_t) rm4 = p.eig_cutoff_t;
+    if (p.eig_lim_all_t && rm5 <= p.eig_cutoff_t) rm5 = p.eig_cutoff_t;
+  }
+  double al1 = rm123 * (drho - dp / a2);
+  double al2 = rm123 * rho;
+  double al3 = rm4 * (dp / a2 + rho * dun / a) / 2;
+  double al4 = rm5 * (dp / a2 - rho * dun / a) / 2;
+  double df1 = (al1 + al3 + al4) * tke + al2 * dtke;
+  double df2 = (al1 + al3 + al4) * ome + al2 * dome;
+  F[5] = (fr1 + fl1 - df1) / 2 * ds;
+  F[6] = (fr2 + fl2 - df2) / 2 * ds;
 }
 
 // ------------------------------------------------------------------------
@@ -135,12 +192,15 @@ __device__ __forceinline__ void limit_face(int ilim, double eps, double rkap, do
 
 // Face states qp (right), qm (left) of face a from q(a-2..a+1) of one line.
 // fin/fout: the line's inner (a = 1) / outer (a = L) face is a strong BC.
+// k, omega (v >= 5) use the turbulence limiter and are first order with
+// foturb (muscl_b*, src/flux.f90:1221-1416).
+template <int NV>
 __device__ __forceinline__ void muscl_face(const Par& p, int nl, int a, int L, bool fin, bool fout, const double* qm2,
                                            const double* qm1, const double* q0, const double* qp1, double* QR,
                                            double* QL) {
   if (nl >= p.nl_crs) {
 #pragma unroll
-    for (int v = 0; v < 5; v++) {
+    for (int v = 0; v < NV; v++) {
       QR[v] = q0[v];
       QL[v] = qm1[v];
     }
@@ -149,7 +209,12 @@ __device__ __forceinline__ void muscl_face(const Par& p, int nl, int a, int L, b
     bool s0 = (a == 1 && fin) || (a == L && fout);
     bool sp1 = (a + 1 == 1 && fin) || (a + 1 == L && fout);
 #pragma unroll
-    for (int v = 0; v < 5; v++) {
+    for (int v = 0; v < NV; v++) {
+      if (v >= 5 && p.foturb) {
+        QR[v] = q0[v];
+        QL[v] = qm1[v];
+        continue;
+      }
       double dm1 = qm1[v] - qm2[v];
       double d0 = q0[v] - qm1[v];
       double dp1 = qp1[v] - q0[v];
@@ -157,18 +222,19 @@ __device__ __forceinline__ void muscl_face(const Par& p, int nl, int a, int L, b
       if (s0) d0 = (q0[v] - qm1[v]) * 2;
       if (sp1) dp1 = (qp1[v] - q0[v]) * 2;
       double dqp, dqm;
-      limit_face(p.ilimf, p.epslimf, p.rkap, dm1, d0, dp1, dqp, dqm);
+      if (v < 5) limit_face(p.ilimf, p.epslimf, p.rkap, dm1, d0, dp1, dqp, dqm);
+      else limit_face(p.ilimt, p.epslimt, p.rkap, dm1, d0, dp1, dqp, dqm);
       QR[v] = q0[v] + dqp;
       QL[v] = qm1[v] + dqm;
     }
   }
   if (a == 1 && fin) {
 #pragma unroll
-    for (int v = 0; v < 5; v++) QR[v] = QL[v] = qm1[v];
+    for (int v = 0; v < NV; v++) QR[v] = QL[v] = qm1[v];
   }
   if (a == L && fout) {
 #pragma unroll
-    for (int v = 0; v < 5; v++) QR[v] = QL[v] = q0[v];
+    for (int v = 0; v < NV; v++) QR[v] = QL[v] = q0[v];
   }
 }
 
@@ -185,11 +251,14 @@ __device__ __forceinline__ void face_metric(const Par& p, const double* s, doubl
 // ------------------------------------------------------------------------
 // i direction: one wavefront per row segment of ROWSEG cells; lane l
 // computes the flux of face i0+l and the cell flux difference takes the
-// neighbouring lane's face flux through a cross-lane shuffle.
+// neighbouring lane's face flux through a cross-lane shuffle.  NV = 7 adds
+// the k-omega transport (h0tab: stale h0l/h0r of roflux_b, see roe_turb).
+template <int NV>
 __global__ void __launch_bounds__(256) k_flux_i(Par p, int nl, const Blk* __restrict__ blks,
                                                 const Tile* __restrict__ tiles, const double* __restrict__ q,
                                                 const double* __restrict__ si, double* __restrict__ res,
-                                                const unsigned char* __restrict__ sbc, const ll* __restrict__ sbcoff) {
+                                                const unsigned char* __restrict__ sbc, const ll* __restrict__ sbcoff,
+                                                const double* __restrict__ h0tab) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int lane = threadIdx.x & 63;
@@ -207,43 +276,49 @@ __global__ void __launch_bounds__(256) k_flux_i(Par p, int nl, const Blk* __rest
   const ll n3 = n3of(b);
   const double* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
   double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
-  double F[5] = {0, 0, 0, 0, 0};
+  double F[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) F[v] = 0.0;
   if (a <= L) {
     const unsigned char* fl = sbc + sbcoff[t.block];
     const ll pl = (ll)(b.jmax + 1) * (b.kmax + 1);
     bool fin = fl[j + (ll)(b.jmax + 1) * k] != 0;
     bool fout = fl[pl + j + (ll)(b.jmax + 1) * k] != 0;
-    double w[4][5];
+    double w[4][NV];
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
-      for (int v = 0; v < 5; v++) w[m][v] = qb[i3(b, a - 2 + m, j, k) + n3 * v];
-    double QR[5], QL[5];
-    muscl_face(p, nl, a, L, fin, fout, w[0], w[1], w[2], w[3], QR, QL);
+      for (int v = 0; v < NV; v++) w[m][v] = qb[i3(b, a - 2 + m, j, k) + n3 * v];
+    double QR[NV], QL[NV];
+    muscl_face<NV>(p, nl, a, L, fin, fout, w[0], w[1], w[2], w[3], QR, QL);
     const int nh = n * p.hbmove;
     const double* s = si + b.o2 * p.lmet * p.nsnap_met + ((ll)nh * n2of(b) + i2(b, a, j, k)) * p.lmet;
     double kx, ky, kz, ds, ubn;
     face_metric(p, s, kx, ky, kz, ds, ubn);
-    roe_face(p, QR, QL, kx, ky, kz, ds, ubn, F);
+    roe_face<NV>(p, QR, QL, kx, ky, kz, ds, ubn, F);
+    if (NV > 5) {
+      const double* h = h0tab + ((ll)(t.block * p.nsnap + n) * 3 + 0) * 2;
+      roe_turb(p, QR, QL, kx, ky, kz, ds, ubn, h[0], h[1], F);
+    }
   }
-  double Fn[5];
+  double Fn[NV];
 #pragma unroll
-  for (int v = 0; v < 5; v++) Fn[v] = __shfl_down(F[v], 1, 64);
+  for (int v = 0; v < NV; v++) Fn[v] = __shfl_down(F[v], 1, 64);
   const int c = a;   // cell c = face a, difference F(c+1) - F(c)
   if (lane < ROWSEG && c <= im) {
 #pragma unroll
-    for (int v = 0; v < 5; v++) rb[i3(b, c, j, k) + n3 * v] = (0.0 + Fn[v]) - F[v];
+    for (int v = 0; v < NV; v++) rb[i3(b, c, j, k) + n3 * v] = (0.0 + Fn[v]) - F[v];
   }
 }
 
 // j or k direction: one thread per line, marching along the line with a
 // four-cell register window; res(c) += F(c+1) - F(c) in the reference order.
-template <int DIR>
+template <int DIR, int NV>
 __global__ void __launch_bounds__(256) k_flux_march(Par p, int nl, const Blk* __restrict__ blks,
                                                     const Tile* __restrict__ tiles, const double* __restrict__ q,
                                                     const double* __restrict__ sm, double* __restrict__ res,
                                                     const unsigned char* __restrict__ sbc,
-                                                    const ll* __restrict__ sbcoff) {
+                                                    const ll* __restrict__ sbcoff, const double* __restrict__ h0tab) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int item = t.base + threadIdx.x;
@@ -276,29 +351,36 @@ __global__ void __launch_bounds__(256) k_flux_march(Par p, int nl, const Blk* __
   }
   const int nh = n * p.hbmove;
   const double* sb = sm + b.o2 * p.lmet * p.nsnap_met + (ll)nh * n2of(b) * p.lmet;
-  double w0[5], w1[5], w2[5], w3[5];
+  double h0l = 0.0, h0r = 0.0;
+  if (NV > 5) {
+    const double* h = h0tab + ((ll)(t.block * p.nsnap + n) * 3 + (DIR - 1)) * 2;
+    h0l = h[0];
+    h0r = h[1];
+  }
+  double w0[NV], w1[NV], w2[NV], w3[NV];
 #pragma unroll
-  for (int v = 0; v < 5; v++) {
+  for (int v = 0; v < NV; v++) {
     w0[v] = qb[base - step + n3 * v];
     w1[v] = qb[base + n3 * v];
     w2[v] = qb[base + step + n3 * v];
     w3[v] = qb[base + 2 * step + n3 * v];
   }
-  double Fp[5];
+  double Fp[NV];
   for (int a = 1; a <= L; a++) {
-    double QR[5], QL[5], F[5];
-    muscl_face(p, nl, a, L, fin, fout, w0, w1, w2, w3, QR, QL);
+    double QR[NV], QL[NV], F[NV];
+    muscl_face<NV>(p, nl, a, L, fin, fout, w0, w1, w2, w3, QR, QL);
     const ll sidx = DIR == 2 ? i2(b, i, a, t2) : i2(b, i, t2, a);
     double kx, ky, kz, ds, ubn;
     face_metric(p, sb + sidx * p.lmet, kx, ky, kz, ds, ubn);
-    roe_face(p, QR, QL, kx, ky, kz, ds, ubn, F);
+    roe_face<NV>(p, QR, QL, kx, ky, kz, ds, ubn, F);
+    if (NV > 5) roe_turb(p, QR, QL, kx, ky, kz, ds, ubn, h0l, h0r, F);
     if (a >= 2) {
       const ll c = base + (ll)(a - 1) * step;
 #pragma unroll
-      for (int v = 0; v < 5; v++) rb[c + n3 * v] = (rb[c + n3 * v] + F[v]) - Fp[v];
+      for (int v = 0; v < NV; v++) rb[c + n3 * v] = (rb[c + n3 * v] + F[v]) - Fp[v];
     }
 #pragma unroll
-    for (int v = 0; v < 5; v++) {
+    for (int v = 0; v < NV; v++) {
       Fp[v] = F[v];
       w0[v] = w1[v];
       w1[v] = w2[v];
@@ -306,9 +388,63 @@ __global__ void __launch_bounds__(256) k_flux_march(Par p, int nl, const Blk* __
     }
     if (a + 2 <= L + 1) {
 #pragma unroll
-      for (int v = 0; v < 5; v++) w3[v] = qb[base + (ll)(a + 2) * step + n3 * v];
+      for (int v = 0; v < NV; v++) w3[v] = qb[base + (ll)(a + 2) * step + n3 * v];
     }
   }
+}
+
+// h0l / h0r of the last face (ic, jc, kc) of each block, snapshot and
+// direction: the values roflux_b's k-omega loop finds left over from the
+// mean-flow loop (src/flux.f90:121-141, 287-302).  One thread per
+// (block, snapshot, direction).
+__global__ void __launch_bounds__(64) k_h0_last(Par p, int nl, int nblocks, const Blk* __restrict__ blks,
+                                                const double* __restrict__ q, const double* __restrict__ si,
+                                                const double* __restrict__ sj, const double* __restrict__ sk,
+                                                const unsigned char* __restrict__ sbc, const ll* __restrict__ sbcoff,
+                                                double* __restrict__ h0tab) {
+  const int item = blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= nblocks * p.nsnap * 3) return;
+  const int dir = item % 3 + 1;
+  const int bn = item / 3;
+  const int blk = bn / p.nsnap, n = bn % p.nsnap;
+  const Blk b = blks[blk];
+  const int L = dir == 1 ? b.imax : (dir == 2 ? b.jmax : b.kmax);
+  // last face: a = L along dir, transverse indices at their maxima
+  const int i = dir == 1 ? L : b.imax - 1, j = dir == 2 ? L : b.jmax - 1, k = dir == 3 ? L : b.kmax - 1;
+  const ll n3 = n3of(b);
+  const double* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  const unsigned char* fl = sbc + sbcoff[blk];
+  bool fin, fout;
+  if (dir == 1) {
+    const ll pl = (ll)(b.jmax + 1) * (b.kmax + 1);
+    fin = fl[j + (ll)(b.jmax + 1) * k] != 0;
+    fout = fl[pl + j + (ll)(b.jmax + 1) * k] != 0;
+  } else if (dir == 2) {
+    const ll off = 2 * (ll)(b.jmax + 1) * (b.kmax + 1);
+    const ll pl = (ll)(b.imax + 1) * (b.kmax + 1);
+    fin = fl[off + i + (ll)(b.imax + 1) * k] != 0;
+    fout = fl[off + pl + i + (ll)(b.imax + 1) * k] != 0;
+  } else {
+    const ll off = 2 * (ll)(b.jmax + 1) * (b.kmax + 1) + 2 * (ll)(b.imax + 1) * (b.kmax + 1);
+    const ll pl = (ll)(b.imax + 1) * (b.jmax + 1);
+    fin = fl[off + i + (ll)(b.imax + 1) * j] != 0;
+    fout = fl[off + pl + i + (ll)(b.imax + 1) * j] != 0;
+  }
+  double w[4][7];
+  for (int m = 0; m < 4; m++) {
+    const int di = dir == 1 ? m - 2 : 0, dj = dir == 2 ? m - 2 : 0, dk = dir == 3 ? m - 2 : 0;
+    for (int v = 0; v < 7; v++) w[m][v] = qb[i3(b, i + di, j + dj, k + dk) + n3 * v];
+  }
+  double QR[7], QL[7], F[7], h[2];
+  muscl_face<7>(p, nl, L, L, fin, fout, w[0], w[1], w[2], w[3], QR, QL);
+  const double* sm = dir == 1 ? si : (dir == 2 ? sj : sk);
+  const int nh = n * p.hbmove;
+  const double* s = sm + b.o2 * p.lmet * p.nsnap_met + ((ll)nh * n2of(b) + i2(b, i, j, k)) * p.lmet;
+  double kx, ky, kz, ds, ubn;
+  face_metric(p, s, kx, ky, kz, ds, ubn);
+  roe_face<7>(p, QR, QL, kx, ky, kz, ds, ubn, F, h);
+  h0tab[(ll)item * 2] = h[0];
+  h0tab[(ll)item * 2 + 1] = h[1];
 }
 
 // ------------------------------------------------------------------------
@@ -379,9 +515,9 @@ __global__ void __launch_bounds__(256) k_residual_fused(Par p, int nl, const Blk
       w3[v] = a_p1;
     }
     double QR[5], QL[5], kx, ky, kz, ds, ubn;
-    muscl_face(p, nl, ks, b.kmax, kfin, kfout, w0, w1, w2, w3, QR, QL);
+    muscl_face<5>(p, nl, ks, b.kmax, kfin, kfout, w0, w1, w2, w3, QR, QL);
     face_metric(p, sk + mo + i2(b, ci, cj, ks) * p.lmet, kx, ky, kz, ds, ubn);
-    roe_face(p, QR, QL, kx, ky, kz, ds, ubn, Fkp);
+    roe_face<5>(p, QR, QL, kx, ky, kz, ds, ubn, Fkp);
     // shift to the window of plane ks: q(ks-1..ks+2)
 #pragma unroll
     for (int v = 0; v < 5; v++) {
@@ -424,9 +560,9 @@ __global__ void __launch_bounds__(256) k_residual_fused(Par p, int nl, const Blk
         z0[v] = pl[v][ty + 2][tx + 2];
         p1[v] = pl[v][ty + 2][tx + 3];
       }
-      muscl_face(p, nl, a, b.imax, fin, fout, m2, m1, z0, p1, QR, QL);
+      muscl_face<5>(p, nl, a, b.imax, fin, fout, m2, m1, z0, p1, QR, QL);
       face_metric(p, si + mo + i2(b, a, cj, k) * p.lmet, kx, ky, kz, ds, ubn);
-      roe_face(p, QR, QL, kx, ky, kz, ds, ubn, F);
+      roe_face<5>(p, QR, QL, kx, ky, kz, ds, ubn, F);
 #pragma unroll
       for (int v = 0; v < 5; v++) fi[v][ty][tx] = F[v];
     }
@@ -442,18 +578,18 @@ __global__ void __launch_bounds__(256) k_residual_fused(Par p, int nl, const Blk
         z0[v] = pl[v][ty + 2][tx + 2];
         p1[v] = pl[v][ty + 3][tx + 2];
       }
-      muscl_face(p, nl, a, b.jmax, fin, fout, m2, m1, z0, p1, QR, QL);
+      muscl_face<5>(p, nl, a, b.jmax, fin, fout, m2, m1, z0, p1, QR, QL);
       face_metric(p, sj + mo + i2(b, ci, a, k) * p.lmet, kx, ky, kz, ds, ubn);
-      roe_face(p, QR, QL, kx, ky, kz, ds, ubn, F);
+      roe_face<5>(p, QR, QL, kx, ky, kz, ds, ubn, F);
 #pragma unroll
       for (int v = 0; v < 5; v++) fj[v][ty][tx] = F[v];
     }
     double Fkn[5];
     if (cell) {   // k-face k+1 from the register window
       double QR[5], QL[5], kx, ky, kz, ds, ubn;
-      muscl_face(p, nl, k + 1, b.kmax, kfin, kfout, w0, w1, w2, w3, QR, QL);
+      muscl_face<5>(p, nl, k + 1, b.kmax, kfin, kfout, w0, w1, w2, w3, QR, QL);
       face_metric(p, sk + mo + i2(b, ci, cj, k + 1) * p.lmet, kx, ky, kz, ds, ubn);
-      roe_face(p, QR, QL, kx, ky, kz, ds, ubn, Fkn);
+      roe_face<5>(p, QR, QL, kx, ky, kz, ds, ubn, Fkn);
     }
     __syncthreads();
     // (c) accumulate in the reference order and store
@@ -489,7 +625,10 @@ __global__ void __launch_bounds__(256) k_lambda_deltat(Par p, const Blk* __restr
                                                        const double* __restrict__ ydot,
                                                        const double* __restrict__ zdot, double* __restrict__ dtvol,
                                                        double* __restrict__ bx, double* __restrict__ be,
-                                                       double* __restrict__ bz) {
+                                                       double* __restrict__ bz, const double* __restrict__ vol,
+                                                       const double* __restrict__ mut, double* __restrict__ dtvolt,
+                                                       double* __restrict__ tbx, double* __restrict__ tbe,
+                                                       double* __restrict__ tbz) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int item = t.base + threadIdx.x;
@@ -536,42 +675,81 @@ __global__ void __launch_bounds__(256) k_lambda_deltat(Par p, const Blk* __restr
   double lamj = fabs(u * etax + v * etay + w * etaz) + a * aeta;
   double lamk = fabs(u * zetax + v * zetay + w * zetaz) + a * azeta;
   const ll po = b.o1 * p.nsnap + (ll)n * n1of(b) + i1(b, i, j, k);
-  dtvol[po] = p.cfl / (lami + lamj + lamk);
-  if (p.irsop == 2) {
-    const double zeta = 0.4;
-    double phixi = 1 + pow(lamj / lami, zeta) + pow(lamk / lami, zeta);
-    double phieta = 1 + pow(lami / lamj, zeta) + pow(lamk / lamj, zeta);
-    double phizeta = 1 + pow(lami / lamk, zeta) + pow(lamj / lamk, zeta);
-    double tt;
-    tt = p.rat * lami * phixi / (lami + lamj + lamk);
-    bx[po] = fmax(0.0, (tt * tt - 1) / 4);
-    tt = p.rat * lamj * phieta / (lami + lamj + lamk);
-    be[po] = fmax(0.0, (tt * tt - 1) / 4);
-    tt = p.rat * lamk * phizeta / (lami + lamj + lamk);
-    bz[po] = fmax(0.0, (tt * tt - 1) / 4);
-  } else if (p.irsop == 3) {
-    double rji = lamj / lami, rki = lamk / lami, rij = lami / lamj, rkj = lamk / lamj, rik = lami / lamk,
-           rjk = lamj / lamk;
-    double tt;
-    tt = p.rat / (1 + p.psirs * (rji + rki));
-    bx[po] = fmax(0.0, 0.25 * (tt * tt - 1));
-    tt = p.rat / (1 + p.psirs * (rij + rkj));
-    be[po] = fmax(0.0, 0.25 * (tt * tt - 1));
-    tt = p.rat / (1 + p.psirs * (rik + rjk));
-    bz[po] = fmax(0.0, 0.25 * (tt * tt - 1));
+  double sum = lami + lamj + lamk;
+  if (p.viscous) {
+    // eig_ns viscous radii (src/cosa.f90:1350-1430, 4240-4297); mut only for
+    // k-omega; the +1 neighbour's temperature divides by the -1 neighbour's
+    // density, as in the reference
+    const double gamma = p.gamma;
+    const double t0 = gamma * pp / rho;
+    const double mu = (p.stemp + 1.0) / (t0 + p.stemp) * pow(t0, 1.5);
+    const double* mb = p.kom ? mut + b.o3 * p.nsnap + n3 * (ll)n : nullptr;
+    const double mutc = p.kom ? mb[c] : 0.0;
+    const double vc = vol[b.o1 + i1(b, i, j, k)];
+    const ll st[3] = {1, (ll)(b.imax + 3), (ll)(b.imax + 3) * (b.jmax + 3)};
+    const double ar_[3] = {axi, aeta, azeta};
+    double lv[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const ll cm = c - st[d], cp = c + st[d];
+      const double rhom = qb[cm], pm = qb[cm + 4 * n3];
+      const double tm = gamma * pm / rhom;
+      const double mum = (p.stemp + 1.0) / (tm + p.stemp) * pow(tm, 1.5);
+      const double mutm = p.kom ? mb[cm] : 0.0;
+      const double ppn = qb[cp + 4 * n3];
+      const double tp = gamma * ppn / rhom;
+      const double mup = (p.stemp + 1.0) / (tp + p.stemp) * pow(tp, 1.5);
+      const double mutp = p.kom ? mb[cp] : 0.0;
+      const double mud = (mum + 2 * mu + mup) / 4;
+      const double mutd = (mutm + 2 * mutc + mutp) / 4;
+      lv[d] = p.cdff * ar_[d] * ar_[d] * gamma * p.machfs / p.reyno * (mud / p.pranl + mutd / p.prant) / (rho * vc);
+    }
+    sum = sum + lv[0] + lv[1] + lv[2];
+  }
+  dtvol[po] = p.cfl / sum;
+  if (p.kom) dtvolt[po] = p.cflt / p.cfl * dtvol[po];
+  for (int pass = 0; pass < (p.kom ? 2 : 1); pass++) {
+    const double rr = pass == 0 ? p.rat : p.ratt;
+    double* ox = pass == 0 ? bx : tbx;
+    double* oe = pass == 0 ? be : tbe;
+    double* oz = pass == 0 ? bz : tbz;
+    if (p.irsop == 2) {
+      const double zeta = 0.4;
+      double phixi = 1 + pow(lamj / lami, zeta) + pow(lamk / lami, zeta);
+      double phieta = 1 + pow(lami / lamj, zeta) + pow(lamk / lamj, zeta);
+      double phizeta = 1 + pow(lami / lamk, zeta) + pow(lamj / lamk, zeta);
+      double tt;
+      tt = rr * lami * phixi / (lami + lamj + lamk);
+      ox[po] = fmax(0.0, (tt * tt - 1) / 4);
+      tt = rr * lamj * phieta / (lami + lamj + lamk);
+      oe[po] = fmax(0.0, (tt * tt - 1) / 4);
+      tt = rr * lamk * phizeta / (lami + lamj + lamk);
+      oz[po] = fmax(0.0, (tt * tt - 1) / 4);
+    } else if (p.irsop == 3) {
+      double rji = lamj / lami, rki = lamk / lami, rij = lami / lamj, rkj = lamk / lamj, rik = lami / lamk,
+             rjk = lamj / lamk;
+      double tt;
+      tt = rr / (1 + p.psirs * (rji + rki));
+      ox[po] = fmax(0.0, 0.25 * (tt * tt - 1));
+      tt = rr / (1 + p.psirs * (rij + rkj));
+      oe[po] = fmax(0.0, 0.25 * (tt * tt - 1));
+      tt = rr / (1 + p.psirs * (rik + rjk));
+      oz[po] = fmax(0.0, 0.25 * (tt * tt - 1));
+    }
   }
 }
 
 // ------------------------------------------------------------------------
-// CIRS: one thread per (line, snapshot) solving the tridiagonal system of
-// the five flow variables with the reference's Thomas recurrence; r(j) is
-// kept in a per-cell scratch, t(j) in place in res.  The first sweep (i)
-// scales res by dtvol, the last (k) divides it back (cirs_b order).
-template <int DIR>
+// CIRS: one thread per (line, snapshot) solving the tridiagonal systems of
+// NV variables v0..v0+NV-1 with the reference's Thomas recurrence; r(j) is
+// kept in a per-cell scratch, t(j) in place in res.  The first sweep
+// scales res by dt (dtvol, or dtvolt for k, omega), the last divides it
+// back (cirs_b order); epsc is the constant coefficient of irsop = 1.
+template <int DIR, int NV>
 __global__ void __launch_bounds__(256) k_cirs(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
                                               double* __restrict__ res, const double* __restrict__ dtvol,
                                               const double* __restrict__ beta, double* __restrict__ rsc, int first,
-                                              int last) {
+                                              int last, int v0, double epsc) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int item = t.base + threadIdx.x;
@@ -585,12 +763,11 @@ __global__ void __launch_bounds__(256) k_cirs(Par p, const Blk* __restrict__ blk
   const int t1 = rem % T1 + 1, t2 = rem / T1 + 1;
   const int L = DIR == 1 ? im : (DIR == 2 ? jm : km);
   const ll n3 = n3of(b);
-  double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n + n3 * v0;
   double* rs = rsc + b.o3 * p.nsnap + n3 * (ll)n;
   const double* dtb = dtvol + b.o1 * p.nsnap + n1of(b) * (ll)n;
   const double* bb = beta + b.o1 * p.nsnap + n1of(b) * (ll)n;
-  const double epsc = 0.25 * (p.rat * p.rat - 1.0);
-  double tp[5] = {0, 0, 0, 0, 0};
+  double tp[NV];
   double rp = 0.0;
   for (int a = 1; a <= L; a++) {
     const int i = DIR == 1 ? a : t1, j = DIR == 2 ? a : (DIR == 1 ? t1 : t2), k = DIR == 3 ? a : t2;
@@ -600,46 +777,46 @@ __global__ void __launch_bounds__(256) k_cirs(Par p, const Blk* __restrict__ blk
     const double cb = 1 + 2 * eps;
     const double cc = a == L ? 0.0 : -eps;
     const double dtv = first ? dtb[c1] : 1.0;
-    double x[5];
+    double x[NV];
 #pragma unroll
-    for (int v = 0; v < 5; v++) x[v] = first ? rb[c3 + n3 * v] * dtv : rb[c3 + n3 * v];
+    for (int v = 0; v < NV; v++) x[v] = first ? rb[c3 + n3 * v] * dtv : rb[c3 + n3 * v];
     double r;
     if (a == 1) {
       r = cc / cb;
 #pragma unroll
-      for (int v = 0; v < 5; v++) tp[v] = x[v] / cb;
+      for (int v = 0; v < NV; v++) tp[v] = x[v] / cb;
     } else {
       r = cc / (cb - rp * ca);
 #pragma unroll
-      for (int v = 0; v < 5; v++) tp[v] = (x[v] - tp[v] * ca) / (cb - rp * ca);
+      for (int v = 0; v < NV; v++) tp[v] = (x[v] - tp[v] * ca) / (cb - rp * ca);
     }
     rp = r;
     rs[c3] = r;
 #pragma unroll
-    for (int v = 0; v < 5; v++) rb[c3 + n3 * v] = tp[v];
+    for (int v = 0; v < NV; v++) rb[c3 + n3 * v] = tp[v];
   }
-  double xn[5];
+  double xn[NV];
   for (int a = L; a >= 1; a--) {
     const int i = DIR == 1 ? a : t1, j = DIR == 2 ? a : (DIR == 1 ? t1 : t2), k = DIR == 3 ? a : t2;
     const ll c3 = i3(b, i, j, k);
-    double xv[5];
+    double xv[NV];
     if (a == L) {
 #pragma unroll
-      for (int v = 0; v < 5; v++) xv[v] = rb[c3 + n3 * v];
+      for (int v = 0; v < NV; v++) xv[v] = rb[c3 + n3 * v];
     } else {
       const double r = rs[c3];
 #pragma unroll
-      for (int v = 0; v < 5; v++) xv[v] = rb[c3 + n3 * v] - xn[v] * r;
+      for (int v = 0; v < NV; v++) xv[v] = rb[c3 + n3 * v] - xn[v] * r;
     }
 #pragma unroll
-    for (int v = 0; v < 5; v++) xn[v] = xv[v];
+    for (int v = 0; v < NV; v++) xn[v] = xv[v];
     if (last) {
       const double dtv = dtb[i1(b, i, j, k)];
 #pragma unroll
-      for (int v = 0; v < 5; v++) rb[c3 + n3 * v] = xv[v] / dtv;
+      for (int v = 0; v < NV; v++) rb[c3 + n3 * v] = xv[v] / dtv;
     } else {
 #pragma unroll
-      for (int v = 0; v < 5; v++) rb[c3 + n3 * v] = xv[v];
+      for (int v = 0; v < NV; v++) rb[c3 + n3 * v] = xv[v];
     }
   }
 }
@@ -802,13 +979,15 @@ hipError_t launch_lambda_deltat(Solve& s, Level& L) {
   hipLaunchKernelGGL(k_lambda_deltat, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk,
                      L.tile[TK_CELLS], L.f[COSAHIP_Q], L.f[COSAHIP_SI], L.f[COSAHIP_SJ], L.f[COSAHIP_SK],
                      L.f[COSAHIP_XDOT], L.f[COSAHIP_YDOT], L.f[COSAHIP_ZDOT], L.f[COSAHIP_DTVOL],
-                     L.f[COSAHIP_BETAXI], L.f[COSAHIP_BETAETA], L.f[COSAHIP_BETAZETA]);
+                     L.f[COSAHIP_BETAXI], L.f[COSAHIP_BETAETA], L.f[COSAHIP_BETAZETA], L.f[COSAHIP_VOL],
+                     L.f[COSAHIP_MUT], L.f[COSAHIP_DTVOLT], L.tbeta[0], L.tbeta[1], L.tbeta[2]);
   return hipGetLastError();
 }
 
 hipError_t launch_residual(Solve& s, Level& L, const double* q, double* res) {
   hipError_t e = launch_zero_res_shell(s, L, res);
   if (e != hipSuccess) return e;
+  if (s.par.viscous || s.par.npde != 5) return launch_residual_viscous(s, L, q, res);
   if (!s.split_residual) {
     if (L.ntile[TK_FUSED])
       hipLaunchKernelGGL(k_residual_fused, grid1(L.ntile[TK_FUSED]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
@@ -817,14 +996,49 @@ hipError_t launch_residual(Solve& s, Level& L, const double* q, double* res) {
     return hipGetLastError();
   }
   if (L.ntile[TK_ROWS_I])
-    hipLaunchKernelGGL(k_flux_i, grid1(L.ntile[TK_ROWS_I]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
-                       L.tile[TK_ROWS_I], q, L.f[COSAHIP_SI], res, L.sbc, L.sbc_doff);
+    hipLaunchKernelGGL(k_flux_i<5>, grid1(L.ntile[TK_ROWS_I]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
+                       L.tile[TK_ROWS_I], q, L.f[COSAHIP_SI], res, L.sbc, L.sbc_doff, nullptr);
   if (L.ntile[TK_LINES_J])
-    hipLaunchKernelGGL(k_flux_march<2>, grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.nl, L.dblk,
-                       L.tile[TK_LINES_J], q, L.f[COSAHIP_SJ], res, L.sbc, L.sbc_doff);
+    hipLaunchKernelGGL((k_flux_march<2, 5>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.nl, L.dblk,
+                       L.tile[TK_LINES_J], q, L.f[COSAHIP_SJ], res, L.sbc, L.sbc_doff, nullptr);
   if (L.ntile[TK_LINES_K])
-    hipLaunchKernelGGL(k_flux_march<3>, grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.nl, L.dblk,
-                       L.tile[TK_LINES_K], q, L.f[COSAHIP_SK], res, L.sbc, L.sbc_doff);
+    hipLaunchKernelGGL((k_flux_march<3, 5>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.nl, L.dblk,
+                       L.tile[TK_LINES_K], q, L.f[COSAHIP_SK], res, L.sbc, L.sbc_doff, nullptr);
+  return hipGetLastError();
+}
+
+// Inviscid direction idir of the viscous / k-omega residual (one launch).
+hipError_t launch_flux_dir(Solve& s, Level& L, int idir, const double* q, double* res) {
+  const bool t = s.par.npde == 7;
+  if (idir == 1 && L.ntile[TK_ROWS_I]) {
+    if (t)
+      hipLaunchKernelGGL(k_flux_i<7>, grid1(L.ntile[TK_ROWS_I]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
+                         L.tile[TK_ROWS_I], q, L.f[COSAHIP_SI], res, L.sbc, L.sbc_doff, L.h0tab);
+    else
+      hipLaunchKernelGGL(k_flux_i<5>, grid1(L.ntile[TK_ROWS_I]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
+                         L.tile[TK_ROWS_I], q, L.f[COSAHIP_SI], res, L.sbc, L.sbc_doff, nullptr);
+  } else if (idir == 2 && L.ntile[TK_LINES_J]) {
+    if (t)
+      hipLaunchKernelGGL((k_flux_march<2, 7>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.nl,
+                         L.dblk, L.tile[TK_LINES_J], q, L.f[COSAHIP_SJ], res, L.sbc, L.sbc_doff, L.h0tab);
+    else
+      hipLaunchKernelGGL((k_flux_march<2, 5>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.nl,
+                         L.dblk, L.tile[TK_LINES_J], q, L.f[COSAHIP_SJ], res, L.sbc, L.sbc_doff, nullptr);
+  } else if (idir == 3 && L.ntile[TK_LINES_K]) {
+    if (t)
+      hipLaunchKernelGGL((k_flux_march<3, 7>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.nl,
+                         L.dblk, L.tile[TK_LINES_K], q, L.f[COSAHIP_SK], res, L.sbc, L.sbc_doff, L.h0tab);
+    else
+      hipLaunchKernelGGL((k_flux_march<3, 5>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.nl,
+                         L.dblk, L.tile[TK_LINES_K], q, L.f[COSAHIP_SK], res, L.sbc, L.sbc_doff, nullptr);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_h0_last(Solve& s, Level& L, const double* q) {
+  const int n = L.nblocks * s.par.nsnap * 3;
+  hipLaunchKernelGGL(k_h0_last, dim3((n + 63) / 64), dim3(64), 0, s.stream, s.par, L.nl, L.nblocks, L.dblk, q,
+                     L.f[COSAHIP_SI], L.f[COSAHIP_SJ], L.f[COSAHIP_SK], L.sbc, L.sbc_doff, L.h0tab);
   return hipGetLastError();
 }
 
@@ -835,12 +1049,27 @@ hipError_t launch_cirs(Solve& s, Level& L) {
   if (L.ntile[TK_LINES_I])
     hipLaunchKernelGGL(k_cirs_i, grid1(L.ntile[TK_LINES_I]), dim3(64), 0, s.stream, s.par, L.dblk,
                        L.tile[TK_LINES_I], res, dt, L.f[COSAHIP_BETAXI], L.cirs_r);
+  const double epsc = 0.25 * (s.par.rat * s.par.rat - 1.0);
   if (L.ntile[TK_LINES_J])
-    hipLaunchKernelGGL(k_cirs<2>, grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                       L.tile[TK_LINES_J], res, dt, L.f[COSAHIP_BETAETA], L.cirs_r, 0, 0);
+    hipLaunchKernelGGL((k_cirs<2, 5>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                       L.tile[TK_LINES_J], res, dt, L.f[COSAHIP_BETAETA], L.cirs_r, 0, 0, 0, epsc);
   if (L.ntile[TK_LINES_K])
-    hipLaunchKernelGGL(k_cirs<3>, grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                       L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, 0, 1);
+    hipLaunchKernelGGL((k_cirs<3, 5>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                       L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, 0, 1, 0, epsc);
+  if (s.par.kom) {
+    // k, omega: scaled by dtvolt; irsop 1 uses the flow's constant
+    // coefficient, irsop 2/3 the turbulence coefficients tbeta*
+    const double* dtt = L.f[COSAHIP_DTVOLT];
+    if (L.ntile[TK_LINES_I_T])
+      hipLaunchKernelGGL((k_cirs<1, 2>), grid1(L.ntile[TK_LINES_I_T]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                         L.tile[TK_LINES_I_T], res, dtt, L.tbeta[0], L.cirs_r, 1, 0, 5, epsc);
+    if (L.ntile[TK_LINES_J])
+      hipLaunchKernelGGL((k_cirs<2, 2>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                         L.tile[TK_LINES_J], res, dtt, L.tbeta[1], L.cirs_r, 0, 0, 5, epsc);
+    if (L.ntile[TK_LINES_K])
+      hipLaunchKernelGGL((k_cirs<3, 2>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                         L.tile[TK_LINES_K], res, dtt, L.tbeta[2], L.cirs_r, 0, 1, 5, epsc);
+  }
   return hipGetLastError();
 }
 

@@ -215,3 +215,82 @@ def cell_volumes(x: np.ndarray, y: np.ndarray, z: np.ndarray) -> np.ndarray:
     vol[0, :, :] = vol[1, :, :]
     vol[nk, :, :] = vol[nk - 1, :, :]
     return vol
+
+
+def metric_derivatives(x: np.ndarray, y: np.ndarray, z: np.ndarray):
+    """Metric derivatives (xi, eta, zeta)_(x, y, z) on the i, j and k faces
+    of one block: nine arrays [k, j, i, 3] over 1:kmax, 1:jmax, 1:imax (the
+    (3, imax, jmax, kmax) layout of xideri ... zetaderk, src/grid.f90:
+    3150-3583), zero outside each face family's range.  Computed as the
+    inverse of the face Jacobian [x_xi x_eta x_zeta]: the derivative normal
+    to the face from node central differences (one-sided at the block
+    ends) averaged over the face's four nodes, the tangential ones from the
+    face edges.  These are inputs the caller uploads; the hot path only
+    reads them."""
+    nk, nj, ni = x.shape
+    P = np.stack([x, y, z], axis=-1)                     # [k, j, i, 3]
+
+    def dnode(ax):
+        d = np.empty_like(P)
+        sl = [slice(None)] * 3
+        lo, hi, mid = list(sl), list(sl), list(sl)
+        n = P.shape[ax]
+        a = [slice(None)] * 4
+        b = [slice(None)] * 4
+        c = [slice(None)] * 4
+        a[ax], b[ax], c[ax] = slice(2, n), slice(0, n - 2), slice(1, n - 1)
+        d[tuple(c)] = (P[tuple(a)] - P[tuple(b)]) / 2
+        a[ax], b[ax], c[ax] = 1, 0, 0
+        d[tuple(c)] = P[tuple(a)] - P[tuple(b)]
+        a[ax], b[ax], c[ax] = n - 1, n - 2, n - 1
+        d[tuple(c)] = P[tuple(a)] - P[tuple(b)]
+        return d
+
+    dxi, deta, dzeta = dnode(2), dnode(1), dnode(0)     # node derivatives along i, j, k
+    out = []
+    for fam in range(3):
+        J = np.zeros((nk, nj, ni, 3, 3))
+        if fam == 0:      # i faces: nodes (i, j..j+1, k..k+1)
+            sl = (slice(0, nk - 1), slice(0, nj - 1), slice(0, ni))
+            avg = lambda A: 0.25 * (A[:-1, :-1, :] + A[:-1, 1:, :] + A[1:, :-1, :] + A[1:, 1:, :])
+            e_eta = 0.5 * ((P[:-1, 1:, :] - P[:-1, :-1, :]) + (P[1:, 1:, :] - P[1:, :-1, :]))
+            e_zeta = 0.5 * ((P[1:, :-1, :] - P[:-1, :-1, :]) + (P[1:, 1:, :] - P[:-1, 1:, :]))
+            cols = (avg(dxi), e_eta, e_zeta)
+        elif fam == 1:    # j faces: nodes (i..i+1, j, k..k+1)
+            sl = (slice(0, nk - 1), slice(0, nj), slice(0, ni - 1))
+            avg = lambda A: 0.25 * (A[:-1, :, :-1] + A[:-1, :, 1:] + A[1:, :, :-1] + A[1:, :, 1:])
+            e_xi = 0.5 * ((P[:-1, :, 1:] - P[:-1, :, :-1]) + (P[1:, :, 1:] - P[1:, :, :-1]))
+            e_zeta = 0.5 * ((P[1:, :, :-1] - P[:-1, :, :-1]) + (P[1:, :, 1:] - P[:-1, :, 1:]))
+            cols = (e_xi, avg(deta), e_zeta)
+        else:             # k faces: nodes (i..i+1, j..j+1, k)
+            sl = (slice(0, nk), slice(0, nj - 1), slice(0, ni - 1))
+            avg = lambda A: 0.25 * (A[:, :-1, :-1] + A[:, :-1, 1:] + A[:, 1:, :-1] + A[:, 1:, 1:])
+            e_xi = 0.5 * ((P[:, :-1, 1:] - P[:, :-1, :-1]) + (P[:, 1:, 1:] - P[:, 1:, :-1]))
+            e_eta = 0.5 * ((P[:, 1:, :-1] - P[:, :-1, :-1]) + (P[:, 1:, 1:] - P[:, :-1, 1:]))
+            cols = (e_xi, e_eta, avg(dzeta))
+        Jf = np.stack(cols, axis=-1)                      # [..., 3 (x,y,z), 3 (xi,eta,zeta)]
+        inv = np.linalg.inv(Jf)                           # rows: xi_(x,y,z), eta_(...), zeta_(...)
+        for r in range(3):
+            a = np.zeros((nk, nj, ni, 3))
+            a[sl] = inv[..., r, :]
+            out.append(a)
+    # order: xideri, etaderi, zetaderi, xiderj, etaderj, zetaderj, xiderk, etaderk, zetaderk
+    return out
+
+
+def cell_centres(x: np.ndarray, y: np.ndarray, z: np.ndarray) -> np.ndarray:
+    """Cell centres [k, j, i, 3] over 0:kmax, 0:jmax, 0:imax (P1 family);
+    halo cells take the centre of the adjacent interior cell."""
+    P = np.stack([x, y, z], axis=-1)
+    c = 0.125 * (P[:-1, :-1, :-1] + P[:-1, :-1, 1:] + P[:-1, 1:, :-1] + P[:-1, 1:, 1:] +
+                 P[1:, :-1, :-1] + P[1:, :-1, 1:] + P[1:, 1:, :-1] + P[1:, 1:, 1:])
+    nk, nj, ni = x.shape
+    out = np.zeros((nk + 1, nj + 1, ni + 1, 3))
+    out[1:nk, 1:nj, 1:ni] = c
+    out[:, :, 0] = out[:, :, 1]
+    out[:, :, ni] = out[:, :, ni - 1]
+    out[:, 0, :] = out[:, 1, :]
+    out[:, nj, :] = out[:, nj - 1, :]
+    out[0, :, :] = out[1, :, :]
+    out[nk, :, :] = out[nk - 1, :, :]
+    return out

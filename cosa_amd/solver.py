@@ -33,6 +33,8 @@ class Solve:
             self._define()
             if q is not None:
                 self.upload("q", q, 1)
+                if case.config["kom"]:
+                    self.upload("mut", case.turbulent_viscosity(q, 1), 1)
         except Exception:
             self.close()
             raise
@@ -56,10 +58,19 @@ class Solve:
                                                        _lib.iptr(keep[4]), L.ncuts, _lib.iptr(cut)),
                         f"define_level {L.nl}")
             g = c.geom[L.nl - 1]
-            for f in ("si", "sj", "sk", "vol", "xdot", "ydot", "zdot"):
+            for f in ("si", "sj", "sk", "vol", "xdot", "ydot", "zdot", "dist"):
                 if f in g:
                     self.upload(f, g[f], L.nl)
-            self.upload("q", c.freestream(L.nl), L.nl)
+            if "xd" in g:
+                for name, arr in zip(_lib.XD_NAMES, g["xd"]):
+                    self.upload(name, arr, L.nl)
+            q0 = c.freestream(L.nl)
+            self.upload("q", q0, L.nl)
+            if c.config["kom"]:
+                # the driver's mut after init2fs (src/cosa.f90:229-232); coarse
+                # levels get theirs on their first visit
+                self.upload("mut", c.turbulent_viscosity(q0, L.nl) if L.nl == 1 else
+                            np.zeros(c.size(L.nl, "p3s")), L.nl)
         d = c.hb_matrix()
         if d is not None:
             dd = np.ascontiguousarray(d.flatten(order="F"))

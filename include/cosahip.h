@@ -200,7 +200,11 @@ int cosahip_prtest(cosahip_solve* s, int nl, int iter);
 int cosahip_lambda_deltat(cosahip_solve* s, int nl);
 /* zero(res) on 0..imax (src/util.f90:238-369), then for idir=1..3:
  * muscl + roflux + resid (src/flux.f90:1183-1812, 2-387, 3472-3888;
- * src/cosa.f90:1123-1211).  Q must hold primitive variables with halos.   */
+ * src/cosa.f90:1123-1211) and, viscous, q_edges + q_face + q_der + rtst +
+ * vflux + resid (src/flux.f90:3889-4263, 2391-2801, 1813-2390, 2802-2948,
+ * 831-1182); k-omega adds the sources q_face + q_der_c + turprod + srckw
+ * (src/cosa.f90:1869-1902; src/src.f90:2-272).  Q must hold primitive
+ * variables with halos.                                                    */
 int cosahip_residual(cosahip_solve* s, int nl);
 /* src_hb (src/src.f90:530-613): res += omega*vol*D*q, q conservative.     */
 int cosahip_src_hb(cosahip_solve* s, int nl);
@@ -213,8 +217,20 @@ int cosahip_add_array(cosahip_solve* s, int nl, int dst, int src, double fact);
 int cosahip_copy_array(cosahip_solve* s, int nl, int dst, int src);
 /* cirs (src/cosa.f90:5206-5623), cutcirs = 0.                             */
 int cosahip_cirs(cosahip_solve* s, int nl);
-/* update (src/update.f90:2-100, 568-762, 1301-1408), explicit RK.         */
+/* update (src/update.f90:2-100, 568-762, 1301-1408), explicit RK; with
+ * k-omega the point-implicit rho*omega, rho*k update first.               */
 int cosahip_update(cosahip_solve* s, int nl, double ar);
+/* tur_vis + cutman_mut (src/update.f90:2272-2337; src/bc.f90:2713-2800):
+ * mut of level nl from `field` (Q or QOLD, primitive or conservative as the
+ * caller holds it), k-omega only.                                          */
+int cosahip_tur_vis(cosahip_solve* s, int nl, int field);
+/* fixq (src/update.f90:2191-2269): k, omega floors qmin(6:7) on Q.        */
+int cosahip_fixq(cosahip_solve* s, int nl);
+/* restriction-stage limits of k-omega (src/cosa.f90:1965-1973):
+ * lim_prod_mg (src/src.f90:674-757, when nlevel > 1, nl < nlevel and
+ * lim_prodtke) then lim_res_ome (src/update.f90:2527-2835) with stage
+ * coefficient ar; Q conservative.                                          */
+int cosahip_turb_limits(cosahip_solve* s, int nl, double ar);
 
 /* ---- drivers --------------------------------------------------------- */
 

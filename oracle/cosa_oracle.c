@@ -293,7 +293,7 @@ static void or_bc_far0(double* q, const double* si, const double* sj, const doub
             Q(q, g, ib2, jb2, kb2, v + 1, n) = hv[v];
           }
         if (kom) {
-          /* src/bc.f90:404-479: k, omega fixed on inflow, extrapolated on outflow */
+          /* src/bc.f90:385-479: k, omega fixed on inflow, extrapolated on outflow */
           double t6, t7;
           if (qdotn <= 0.0) {
             t6 = P.tkefar; t7 = P.omefar;
@@ -2139,6 +2139,8 @@ static ll off_bct(const or_level* L, int b) {
 #define PB(arr, L, b) ((arr) + off_of(L, b, 1) * DIM5)
 #define VB(arr, L, b) ((arr) + off_of(L, b, 1))
 #define FB(arr, L, b) ((arr) + off_of(L, b, 0) * NPDE * DIM5)
+#define SB(arr, L, b) ((arr) + off_of(L, b, 3) * DIM5)
+#define DB(arr, L, b) ((arr) + off_of(L, b, 0) * 3 * DIM5H)
 
 void or_cons2prim(or_case* c, int nl, double* q) {
   or_level* L = &c->lev[nl - 1];
@@ -2151,8 +2153,8 @@ void or_prim2cons(or_case* c, int nl, double* q) {
 void or_bc(or_case* c, int nl, double* q) {
   or_level* L = &c->lev[nl - 1];
   for (int b = 0; b < L->nblocks; b++)
-    or_bbc(QB(q, L, b), MB(L->si, L, b), MB(L->sj, L, b), MB(L->sk, L, b), L->bctopo + off_bct(L, b), L->nbcs[b],
-           L->imax[b], L->jmax[b], L->kmax[b]);
+    or_bbc_dist(QB(q, L, b), MB(L->si, L, b), MB(L->sj, L, b), MB(L->sk, L, b), L->dist ? VB(L->dist, L, b) : NULL,
+                L->bctopo + off_bct(L, b), L->nbcs[b], L->imax[b], L->jmax[b], L->kmax[b]);
 }
 void or_cutman_q(or_case* c, int nl, double* q) {
   or_level* L = &c->lev[nl - 1];
@@ -2181,6 +2183,19 @@ int or_prtest(or_case* c, int nl, int32_t* where) {
 }
 void or_lambda_deltat(or_case* c, int nl) {
   or_level* L = &c->lev[nl - 1];
+  if (P.viscous) {
+    for (int b = 0; b < L->nblocks; b++)
+      or_lambdab_v(QB(L->q, L, b), MB(L->si, L, b), MB(L->sj, L, b), MB(L->sk, L, b),
+                   L->xdot ? XB(L->xdot, L, b) : NULL, L->ydot ? XB(L->ydot, L, b) : NULL,
+                   L->zdot ? XB(L->zdot, L, b) : NULL, VB(L->vol, L, b), L->mut ? SB(L->mut, L, b) : NULL,
+                   QB(L->qp, L, b), QB(L->dq, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+    for (int b = 0; b < L->nblocks; b++)
+      or_bdeltat_v(PB(L->dtvol, L, b), L->dtvolt ? PB(L->dtvolt, L, b) : NULL, QB(L->qp, L, b), QB(L->dq, L, b),
+                   PB(L->betaxi, L, b), PB(L->betaeta, L, b), PB(L->betazeta, L, b),
+                   L->tbetaxi ? PB(L->tbetaxi, L, b) : NULL, L->tbetaeta ? PB(L->tbetaeta, L, b) : NULL,
+                   L->tbetazeta ? PB(L->tbetazeta, L, b) : NULL, L->imax[b], L->jmax[b], L->kmax[b]);
+    return;
+  }
   for (int b = 0; b < L->nblocks; b++)
     or_lambdab(QB(L->q, L, b), MB(L->si, L, b), MB(L->sj, L, b), MB(L->sk, L, b),
                L->xdot ? XB(L->xdot, L, b) : NULL, L->ydot ? XB(L->ydot, L, b) : NULL,
@@ -2202,9 +2217,59 @@ void or_zero_res(or_case* c, int nl) {
             for (int i = 0; i <= g.imax; i++) Q(r, g, i, j, k, v, n) = 0.0;
   }
 }
+/* viscous fluxes of direction idir (src/cosa.f90:1806-1821): q_face,
+ * q_der, rtst (k-omega), vflux, resid; scratch as in the reference: qf in
+ * dq, mutf in flux, derivatives in qp/qm/dqm, Reynolds stresses in dqp.    */
+static void or_viscous_dir(or_case* c, int nl, int idir) {
+  or_level* L = &c->lev[nl - 1];
+  int kom = is_kom();
+  const double* s = idir == 1 ? L->si : (idir == 2 ? L->sj : L->sk);
+  for (int b = 0; b < L->nblocks; b++) {
+    int im = L->imax[b], jm = L->jmax[b], km = L->kmax[b];
+    const int32_t* bt = L->bctopo + off_bct(L, b);
+    double* xd0 = L->xd[3 * (idir - 1)] ? DB(L->xd[3 * (idir - 1)], L, b) : NULL;
+    double* xd1 = L->xd[3 * (idir - 1) + 1] ? DB(L->xd[3 * (idir - 1) + 1], L, b) : NULL;
+    double* xd2 = L->xd[3 * (idir - 1) + 2] ? DB(L->xd[3 * (idir - 1) + 2], L, b) : NULL;
+    or_q_face_b(idir, bt, L->nbcs[b], QB(L->q, L, b), QB(L->dq, L, b), kom ? SB(L->mut, L, b) : NULL,
+                FB(L->flux, L, b), kom, im, jm, km);
+    or_q_der_b(idir, bt, L->nbcs[b], QB(L->q, L, b), QB(L->qp, L, b), QB(L->qm, L, b), QB(L->dqm, L, b), im, jm, km);
+    if (kom)
+      or_rtst_b(idir, QB(L->dq, L, b), QB(L->qp, L, b), QB(L->qm, L, b), QB(L->dqm, L, b), xd0, xd1, xd2,
+                QB(L->dqp, L, b), FB(L->flux, L, b), im, jm, km);
+    or_vflux_b(idir, QB(L->dq, L, b), QB(L->qp, L, b), QB(L->qm, L, b), QB(L->dqm, L, b), QB(L->dqp, L, b),
+               FB(L->flux, L, b), xd0, xd1, xd2, MB(s, L, b), im, jm, km);
+    or_bresid(idir, FB(L->flux, L, b), QB(L->res, L, b), im, jm, km);
+  }
+}
+
+/* k-omega sources (src/cosa.f90:1869-1902): cell-centre velocity
+ * differences from the face averages, turprod (finest level), srckw.     */
+static void or_komega_sources(or_case* c, int nl) {
+  or_level* L = &c->lev[nl - 1];
+  double* outs[3] = {L->qp, L->qm, L->dqm};
+  for (int idir = 1; idir <= 3; idir++)
+    for (int b = 0; b < L->nblocks; b++) {
+      or_q_face_b(idir, L->bctopo + off_bct(L, b), L->nbcs[b], QB(L->q, L, b), QB(L->dq, L, b), NULL, NULL, 0,
+                  L->imax[b], L->jmax[b], L->kmax[b]);
+      or_q_der_c_b(idir, QB(L->dq, L, b), QB(outs[idir - 1], L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+    }
+  if (nl == 1 || P.prd_all_lvls)
+    for (int b = 0; b < L->nblocks; b++) {
+      const double* xd[9];
+      for (int m = 0; m < 9; m++) xd[m] = DB(L->xd[m], L, b);
+      or_turprod_b(QB(L->q, L, b), QB(L->qp, L, b), QB(L->qm, L, b), QB(L->dqm, L, b), SB(L->prd, L, b),
+                   SB(L->divvel, L, b), SB(L->delpls, L, b), xd, L->imax[b], L->jmax[b], L->kmax[b]);
+    }
+  for (int b = 0; b < L->nblocks; b++)
+    or_srckw_b(QB(L->q, L, b), SB(L->mut, L, b), SB(L->prd, L, b), SB(L->divvel, L, b), QB(L->res, L, b),
+               VB(L->vol, L, b), nl, L->imax[b], L->jmax[b], L->kmax[b]);
+}
+
 void or_residual(or_case* c, int nl) {
   or_level* L = &c->lev[nl - 1];
   or_zero_res(c, nl);
+  if (P.viscous)
+    for (int b = 0; b < L->nblocks; b++) or_q_bedges(QB(L->q, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
   for (int idir = 1; idir <= 3; idir++) {
     for (int b = 0; b < L->nblocks; b++)
       or_muscl_b(idir, nl, L->bctopo + off_bct(L, b), L->nbcs[b], QB(L->q, L, b), QB(L->qp, L, b), QB(L->qm, L, b),
@@ -2215,7 +2280,9 @@ void or_residual(or_case* c, int nl) {
                   L->kmax[b]);
     for (int b = 0; b < L->nblocks; b++)
       or_bresid(idir, FB(L->flux, L, b), QB(L->res, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+    if (P.viscous) or_viscous_dir(c, nl, idir);
   }
+  if (is_kom()) or_komega_sources(c, nl);
 }
 void or_src_hb(or_case* c, int nl) {
   or_level* L = &c->lev[nl - 1];
@@ -2276,15 +2343,53 @@ void or_copy_array(or_case* c, int nl, double* dst, const double* src) {
 }
 void or_cirs(or_case* c, int nl) {
   or_level* L = &c->lev[nl - 1];
+  int kom = is_kom();
   for (int b = 0; b < L->nblocks; b++)
-    or_cirs_b(QB(L->res, L, b), PB(L->dtvol, L, b), PB(L->betaxi, L, b), PB(L->betaeta, L, b), PB(L->betazeta, L, b),
-              L->imax[b], L->jmax[b], L->kmax[b]);
+    or_cirs_kb(QB(L->res, L, b), PB(L->dtvol, L, b), PB(L->betaxi, L, b), PB(L->betaeta, L, b),
+               PB(L->betazeta, L, b), kom ? PB(L->dtvolt, L, b) : NULL, kom ? PB(L->tbetaxi, L, b) : NULL,
+               kom ? PB(L->tbetaeta, L, b) : NULL, kom ? PB(L->tbetazeta, L, b) : NULL, L->imax[b], L->jmax[b],
+               L->kmax[b]);
 }
 void or_update(or_case* c, int nl, double ar) {
   or_level* L = &c->lev[nl - 1];
+  for (int b = 0; b < L->nblocks; b++) {
+    if (is_kom())
+      or_update_kb(QB(L->res, L, b), QB(L->q, L, b), QB(L->qold, L, b), VB(L->vol, L, b), PB(L->dtvol, L, b),
+                   PB(L->dtvolt, L, b), SB(L->delpls, L, b), SB(L->prd, L, b), ar, L->imax[b], L->jmax[b],
+                   L->kmax[b]);
+    else
+      or_update_b(QB(L->res, L, b), QB(L->q, L, b), QB(L->qold, L, b), PB(L->dtvol, L, b), ar, L->imax[b],
+                  L->jmax[b], L->kmax[b]);
+  }
+}
+/* tur_vis + cutman_mut (src/update.f90:2272-2295; src/bc.f90:2713-2800) */
+void or_tur_vis(or_case* c, int nl) {
+  or_level* L = &c->lev[nl - 1];
   for (int b = 0; b < L->nblocks; b++)
-    or_update_b(QB(L->res, L, b), QB(L->q, L, b), QB(L->qold, L, b), PB(L->dtvol, L, b), ar, L->imax[b], L->jmax[b],
-                L->kmax[b]);
+    or_tur_vis_b(QB(L->q, L, b), SB(L->mut, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+  int save = NPDE;
+  NPDE = 1;   /* cut_q with one variable per snapshot */
+  for (int icut = 0; icut < L->ncuts; icut++) {
+    const int32_t* t = L->cutopo + 21 * icut;
+    int b1 = t[0] - 1, b2 = t[9] - 1;
+    or_cut_q(L->imax[b1], L->jmax[b1], L->kmax[b1], SB(L->mut, L, b1), L->imax[b2], L->jmax[b2], L->kmax[b2],
+             SB(L->mut, L, b2), t);
+  }
+  NPDE = save;
+}
+void or_fixq(or_case* c, int nl) {
+  or_level* L = &c->lev[nl - 1];
+  for (int b = 0; b < L->nblocks; b++) or_fixq_b(QB(L->q, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+}
+/* restriction stage limits of k-omega (src/cosa.f90:1965-1973) */
+void or_turb_limits(or_case* c, int nl, double ar) {
+  or_level* L = &c->lev[nl - 1];
+  if (P.nlevel > 1 && nl != P.nlevel && !P.prd_all_lvls && P.lim_prodtke)
+    for (int b = 0; b < L->nblocks; b++)
+      or_lim_prod_mg_b(QB(L->q, L, b), SB(L->prd, L, b), SB(L->mut, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+  for (int b = 0; b < L->nblocks; b++)
+    or_lim_res_ome_b(QB(L->res, L, b), QB(L->q, L, b), VB(L->vol, L, b), PB(L->dtvolt, L, b), SB(L->delpls, L, b),
+                     SB(L->prd, L, b), ar, L->imax[b], L->jmax[b], L->kmax[b]);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -2299,6 +2404,7 @@ int or_smooth(or_case* c, int nl, int from_fine, int to_coarse, int niter, int m
       or_cons2prim(c, nl, L->q);
       or_bc(c, nl, L->q);
       or_cutman_q(c, nl, L->q);
+      if (is_kom() && mgit == 1 && iter == 1 && irk == 1 && from_fine && !P.prd_all_lvls) or_tur_vis(c, nl);
       int bad = or_prtest(c, nl, where);
       if (bad) {
         if (where) where[5] = iter;
@@ -2317,9 +2423,16 @@ int or_smooth(or_case* c, int nl, int from_fine, int to_coarse, int niter, int m
       }
       if (from_fine && iter == 1 && irk == 1) or_add_array(c, nl, L->rhs, L->res, 1.0);
       if (nl > 1) or_add_array(c, nl, L->res, L->rhs, -1.0);
-      if (iter == niter + 1) return 0;
+      if (iter == niter + 1) {
+        if (is_kom()) or_turb_limits(c, nl, P.alphas[irk - 1]);
+        return 0;
+      }
       if (P.irsop > 0 && P.irsop <= 3) or_cirs(c, nl);
       or_update(c, nl, P.alphas[irk - 1]);
+      if (is_kom()) {
+        if (P.ifixq == 1) or_fixq(c, nl);
+        if (irk == P.nstage && (nl == 1 || !P.prd_all_lvls)) or_tur_vis(c, nl);
+      }
     }
     double errmax = RESRMS[1];
     for (int p = 2; p <= 5; p++) errmax = fmax(errmax, RESRMS[p]);
@@ -2337,6 +2450,14 @@ void or_restrict(or_case* c, int nl) {
   for (int b = 0; b < L1->nblocks; b++)
     or_rest1_b(L1->imax[b], L1->jmax[b], L1->kmax[b], L2->imax[b], L2->jmax[b], L2->kmax[b], QB(L1->q, L1, b),
                QB(L2->q, L2, b), VB(L1->vol, L1, b), VB(L2->vol, L2, b));
+  if (is_kom() && !P.prd_all_lvls) {
+    double* f1[3] = {L1->prd, L1->delpls, L1->divvel};
+    double* f2[3] = {L2->prd, L2->delpls, L2->divvel};
+    for (int m = 0; m < 3; m++)
+      for (int b = 0; b < L1->nblocks; b++)
+        or_rest4_b(L1->imax[b], L1->jmax[b], L1->kmax[b], L2->imax[b], L2->jmax[b], L2->kmax[b], SB(f1[m], L1, b),
+                   SB(f2[m], L2, b), VB(L1->vol, L1, b), VB(L2->vol, L2, b));
+  }
   for (int b = 0; b < L1->nblocks; b++)
     or_rest2_b(P.ho_rest, L1->imax[b], L1->jmax[b], L1->kmax[b], L2->imax[b], L2->jmax[b], L2->kmax[b],
                QB(L2->rhs, L2, b), QB(L1->res, L1, b));

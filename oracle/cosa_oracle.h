@@ -153,6 +153,9 @@ void or_add_array(or_case* c, int nl, double* dst, const double* src,
 void or_copy_array(or_case* c, int nl, double* dst, const double* src);
 void or_cirs(or_case* c, int nl);
 void or_update(or_case* c, int nl, double ar);
+void or_tur_vis(or_case* c, int nl);
+void or_fixq(or_case* c, int nl);
+void or_turb_limits(or_case* c, int nl, double ar);
 
 /* drivers */
 int or_smooth(or_case* c, int nl, int from_fine, int to_coarse, int niter,

@@ -34,6 +34,10 @@ def prim_to_cons(qp, imax, jmax, kmax, gamma=1.4, npde=5, nsnap=1):
     out[:, 2] = rho * v
     out[:, 3] = rho * w
     out[:, 4] = p / (gamma - 1) + 0.5 * rho * (u * u + v * v + w * w)
+    for v_ in range(5, npde):
+        out[:, v_] = rho * q[:, v_]
+    if npde > 5:
+        out[:, 4] += rho * q[:, 5]          # + rho k
     return out.reshape(-1).copy()
 
 

@@ -147,6 +147,8 @@ class OrCase:
             if c["viscous"] or c["kom"]:
                 for n in ("mut", "prd", "divvel", "delpls"):
                     a[n] = g[n].copy() if n in g else np.zeros(case.size(nl, "p3s"))
+                if c["kom"] and nl == 1:
+                    a["mut"] = case.turbulent_viscosity(a["q"], 1)
                 for n in ("dtvolt", "tbetaxi", "tbetaeta", "tbetazeta"):
                     a[n] = np.zeros(case.size(nl, "p1"))
                 a["dist"] = g["dist"].copy()

@@ -1,0 +1,165 @@
+"""Parity of the viscous / Wilcox k-omega path of libcosahip (HIP, gfx950)
+with the oracle, through the C ABI: every routine of a RANS stage in turn,
+then whole smooth() calls and multigrid cycles.  Same FP64 bar as
+test_gpu_parity.py, |gpu - cpu| <= 1e-12 (|cpu| + 1e-14 max|cpu|); the
+Sutherland law's t**1.5 and the wall density's (pw/p1)**(1/gamma) go
+through the device pow, everything else keeps the reference's operation
+order."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from cosa_amd import _lib
+from cosa_amd import case as cases
+from cosa_amd.config import kw_turbulence
+from oracle_bind import dp
+from parity_util import interior_mask, make_pair, parity
+
+pytestmark = pytest.mark.gpu
+
+TURB = dict(stemp=0.383, ifixq=1, lim_prodtke=1, prdlim=20.0, turcmp=1)
+
+
+def rans_cases():
+    out = []
+    out.append(cases.two_block_case(ni=13, nj=9, nk=9, nlevel=1, distort=0.1, name="rans_blocks", wall=1500,
+                                    irsop=2, cfl_irs=2.5, cfl=1.5, ratt=1.4, cflt=1.2,
+                                    **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB)))
+    out.append(cases.two_block_case(ni=13, nj=9, nk=9, nlevel=1, distort=0.1, name="laminar_blocks", wall=1500,
+                                    irsop=1, cfl_irs=2.5, cfl=1.5, viscous=1, reyno=500.0, stemp=0.383))
+    out.append(cases.naca_case(33, 13, 9, nlevel=1, name="rans_naca_menter", wall=1500, irsop=3, psirs=0.125,
+                               cfl_irs=2.0, cfl=1.0, ds_wall=2e-3, wallwilc=0,
+                               **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB)))
+    return out
+
+
+CASES = rans_cases()
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(params=CASES, ids=[c.name for c in CASES])
+def pair(request):
+    c = request.param
+    q0 = c.perturbed(amp=0.02, seed=11)
+    _say(f"case {c.name} ncell={c.ncell()} npde={c.npde}")
+    o, oc, g = make_pair(c, q0)
+    yield c, o, oc, g
+    g.close()
+
+
+def test_rans_stage_routines(pair):
+    """One RK stage of the viscous / k-omega path, routine by routine
+    (src/cosa.f90:1719-2016)."""
+    c, o, oc, g = pair
+    kom = bool(c.config["kom"])
+    A = oc[1]
+    cp = oc.ptr()
+    o.or_cons2prim(cp, 1, dp(A["q"]))
+    g.call("cons2prim", 1, _lib.Q)
+    parity(g.download("q"), A["q"], what="cons2prim")
+    o.or_bc(cp, 1, dp(A["q"]))
+    g.call("bc", 1, _lib.Q)
+    parity(g.download("q"), A["q"], what="bc (viscous walls)")
+    o.or_cutman_q(cp, 1, dp(A["q"]))
+    g.call("cutman_q", 1, _lib.Q)
+    parity(g.download("q"), A["q"], what="cutman_q")
+    if kom:
+        o.or_tur_vis(cp, 1)
+        g.call("tur_vis", 1, _lib.Q)
+        parity(g.download("mut"), A["mut"], what="tur_vis (primitive q)")
+    o.or_lambda_deltat(cp, 1)
+    g.call("lambda_deltat", 1)
+    parity(g.download("dtvol"), A["dtvol"], what="dtvol (viscous radii)")
+    if kom:
+        parity(g.download("dtvolt"), A["dtvolt"], what="dtvolt")
+    if c.config["irsop"] >= 2:
+        for f in ("betaxi", "betaeta", "betazeta"):
+            parity(g.download(f), A[f], what=f)
+    o.or_residual(cp, 1)
+    g.call("residual", 1)
+    m = interior_mask(c)
+    parity(g.download("res")[m], A["res"][m], what="residual (inviscid + viscous + sources)")
+    if kom:
+        ms = interior_mask(c, ncomp=1)
+        for f in ("prd", "divvel", "delpls"):
+            parity(g.download(f)[ms], A[f][ms], what=f)
+    o.or_prim2cons(cp, 1, dp(A["q"]))
+    g.call("prim2cons", 1, _lib.Q)
+    parity(g.download("q"), A["q"], what="prim2cons")
+    if kom:
+        o.or_turb_limits(cp, 1, ctypes.c_double(0.25))
+        g.call("turb_limits", 1, ctypes.c_double(0.25))
+        parity(g.download("res")[m], A["res"][m], what="lim_res_ome")
+    o.or_cirs(cp, 1)
+    g.call("cirs", 1)
+    parity(g.download("res")[m], A["res"][m], what="cirs (k-omega coefficients)")
+    o.or_copy_array(cp, 1, dp(A["qold"]), dp(A["q"]))
+    g.call("copy_array", 1, _lib.QOLD, _lib.Q)
+    o.or_update(cp, 1, ctypes.c_double(0.5))
+    g.call("update", 1, ctypes.c_double(0.5))
+    parity(g.download("q"), A["q"], what="update (point-implicit k-omega)")
+    if kom:
+        o.or_fixq(cp, 1)
+        g.call("fixq", 1)
+        parity(g.download("q"), A["q"], what="fixq")
+        o.or_tur_vis(cp, 1)
+        g.call("tur_vis", 1, _lib.Q)
+        parity(g.download("mut"), A["mut"], what="tur_vis (conservative q)")
+
+
+@pytest.mark.parametrize("niter", [1, 3])
+def test_rans_smooth_single_level(pair, niter):
+    c, o, oc, g = pair
+    done = (ctypes.c_int32 * 1)()
+    where = (ctypes.c_int32 * 8)()
+    assert o.or_smooth(oc.ptr(), 1, 0, 0, niter, 1, done, where) == 0
+    d, r = g.smooth(1, False, False, niter, 1)
+    parity(g.download("q"), oc[1]["q"], rtol=1e-11, what="smooth q")
+    npde = c.npde
+    r_o = np.array([o.or_get_resrms(i) for i in range(1, 9)])
+    parity(r[:npde + 1], r_o[:npde + 1], rtol=1e-12, what="smooth resrms")
+    if c.config["kom"]:
+        parity(g.download("mut"), oc[1]["mut"], rtol=1e-11, what="smooth mut")
+
+
+def _mg_case(name):
+    if name == "rans_blocks2":
+        return cases.two_block_case(ni=17, nj=9, nk=9, nlevel=2, distort=0.1, name="rans_blocks2", wall=1500,
+                                    irsop=1, cfl_irs=2.5, cfl=1.5, ncycle=4, nl_crs=2,
+                                    **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB))
+    if name == "laminar_blocks3":
+        return cases.two_block_case(ni=17, nj=9, nk=9, nlevel=3, distort=0.1, name="laminar_blocks3", wall=1500,
+                                    irsop=2, cfl_irs=2.5, cfl=1.5, ncycle=4, viscous=1, reyno=500.0, stemp=0.383)
+    return cases.naca_case(65, 17, 9, nlevel=3, name="rans_naca3", wall=1500, irsop=2, cfl_irs=2.0, cfl=1.0,
+                           ncycle=4, nl_crs=2, ds_wall=2e-3,
+                           **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB))
+
+
+@pytest.mark.parametrize("name", ["rans_blocks2", "laminar_blocks3", "rans_naca3"])
+def test_rans_multigrid_cycles(name):
+    """Whole V-cycles with the k-omega production restricted to the coarse
+    levels (rest4) and the restriction-stage limiters."""
+    c = _mg_case(name)
+    q0 = c.perturbed(amp=0.01, seed=5)
+    _say(f"mg case {name} levels={c.config['nlevel']} ncell={c.ncell()} npde={c.npde}")
+    o, oc, g = make_pair(c, q0)
+    try:
+        ncyc = c.config["ncycle"]
+        hist_o = np.zeros(8 * ncyc)
+        where = (ctypes.c_int32 * 8)()
+        n_o = o.or_mg(oc.ptr(), 1, 1, 1, 1, dp(hist_o), where)
+        assert n_o == ncyc
+        n_g = g.mg(ncyc)
+        assert n_g == ncyc
+        npde = c.npde
+        hist_g = np.array(g.history)
+        hist_o = hist_o.reshape(ncyc, 8)
+        parity(hist_g[:, :npde + 1].ravel(), hist_o[:, :npde + 1].ravel(), rtol=1e-10, what="resrms history")
+        for nl in range(1, c.config["nlevel"] + 1):
+            parity(g.download("q", nl), oc[nl]["q"], rtol=1e-10, what=f"q level {nl}")
+    finally:
+        g.close()
