@@ -1,12 +1,17 @@
 #!/usr/bin/env python3
 """Benchmark of the COSA pseudo-time residual/update path on one MI355X.
 
-Workload (default): the uneven multiblock configuration -- 96 independent
-bump-channel blocks whose node counts per direction are drawn from
-{17, 33, 65, 129} with numpy.random.default_rng(0) (24,285,184 cells), Euler
-at M = 0.7, freestream plus a 1% uniform perturbation (default_rng(1)),
+Workload (default, --workload wing4): steady RANS with the Wilcox k-omega
+model around a rectangular wing with a NACA 0012 section and aspect ratio 4
+-- 4 blocks of 257 x 65 x 65 nodes (4,194,304 cells), O-H topology, wall
+spacing 5e-6 chord -- at M = 0.3, Re = 3e6, alpha = 4 deg: point-implicit
 4-stage RK with central implicit residual smoothing (cirs_c), van Albada
-MUSCL, 3-level multigrid V-cycles, FP64.
+MUSCL, 3-level multigrid V-cycles, FP64, freestream plus a 1% uniform
+perturbation (default_rng(1)).
+
+--workload uneven96: 96 independent bump-channel blocks whose node counts
+per direction are drawn from {17, 33, 65, 129} with numpy.random
+.default_rng(0) (24,285,184 cells), Euler at M = 0.7, same scheme.
 
 One step = one multigrid cycle of mg() (src/cosa.f90:822-912) over all
 blocks.  The metric is cell-updates per second: one RK stage applied to one
@@ -33,9 +38,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per cell-update of each kernel family (FP64 Euler, static
-# grid; each required array read once and written once, halos not counted)
-BYTES_PER_CELL = {
+# algorithmic bytes per cell-update of each kernel family (FP64, static grid;
+# each required array read once and written once, halos not counted)
+BYTES_PER_CELL_EULER = {
     "residual": 176.0,   # q 5x8 R + face metrics 3x4x8 R + res 5x8 W
     "cirs": 248.0,       # res 5x8 R+W per sweep (3 sweeps) + dtvol 8 R
     "update": 128.0,     # qold 40 R + res 40 R + dtvol 8 R + q 40 W
@@ -44,25 +49,56 @@ BYTES_PER_CELL = {
     "lambda": 144.0,     # q 40 R + metrics 96 R + dtvol 8 W
     "other": None,
 }
-# HBM bytes per cell-update measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# (scaled by a plain-copy calibration: FETCH x2.00, WRITE x1.00) on the
-# uneven96 workload; profiles/residual_uneven96_fp64_hbm_traffic.md.  Re-measure
-# when the kernel changes.
-MEASURED_TRAFFIC_PER_CELL = {("uneven96", "residual"): 322.7}
+BYTES_PER_CELL_RANS = {  # Wilcox k-omega, npde = 7
+    # q 7x8 R + face metrics 96 R + metric derivatives 3 faces x 9 x 8 = 216 R
+    # + mut 8 R + vol 8 R + res 7x8 W + prd, divvel, delpls 24 W
+    "residual": 464.0,
+    "cirs": 352.0,       # res 7x8 R+W per sweep (3 sweeps) + dtvol, dtvolt 16 R
+    # qold 56 R + res 56 R + q 56 R+W + dtvol, dtvolt, vol, delpls, prd 40 R
+    "update": 264.0,
+    "halo": None,
+    "convert": 112.0,    # q 7x8 R + W
+    "lambda": 176.0,     # q 5x8 R + metrics 96 R + vol 8 + mut 8 R + dtvol, dtvolt 16 W + ... betas
+    "other": None,
+}
+# HBM bytes per cell-update of the residual pass measured with rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE (scaled by a plain-copy calibration: FETCH x2.00,
+# WRITE x1.00): profiles/residual_uneven96_fp64_hbm_traffic.md and
+# profiles/residual_wing4_rans_fp64_hbm_traffic.md.  Re-measure when the
+# kernels change.
+MEASURED_TRAFFIC_PER_CELL = {("uneven96", "residual"): 322.7, ("wing4", "residual"): 2693.1}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _wing_cfg():
+    from cosa_amd.config import kw_turbulence
+    return dict(irsop=1, cfl=2.0, cfl_irs=4.0, nl_crs=2, ncycle=100000, ds_wall=5e-6,
+                **kw_turbulence(0.3, 3.0e6, tu=1e-4, mutfar=1e-2, stemp=110.4 / 288.15, ifixq=1, lim_prodtke=1,
+                                prdlim=20.0))
+
+
+WING_CFG = None
+
+
 def make_case(workload: str):
     from cosa_amd import case as cases
+    global WING_CFG
+    WING_CFG = _wing_cfg()
     if workload == "uneven96":
         dims = cases.uneven_dims(96, seed=0)
         c = cases.channel_case(dims, nlevel=3, name="uneven96", machfs=0.7, irsop=1, cfl=2.0, cfl_irs=4.0,
                                ilimf=4, epslimf=1e-6, nl_crs=2, ncycle=100000)
         return c, dict(workload="uneven96", blocks=96, dims_seed=0, cells=c.ncell(), levels=3, physics="euler",
                        mach=0.7, rk_stages=4, irs="cirs_c", limiter="van_albada", precision="fp64")
+    if workload == "wing4":
+        c = cases.wing_case(257, 65, 65, nblocks=4, nlevel=3, name="wing4", **WING_CFG)
+        return c, dict(workload="wing4_rans_komega", blocks=4, block_nodes=[257, 65, 65], cells=c.ncell(), levels=3,
+                       physics="rans_wilcox_komega", mach=0.3, reynolds=3.0e6, alpha_deg=4.0, rk_stages=4,
+                       update="point_implicit_komega", irs="cirs_c", precision="fp64",
+                       grid="O-H wing, NACA0012 section, semi-span 2 (AR 4), wall spacing 5e-6 chord")
     if workload == "naca":
         c = cases.naca_case(193, 49, 9, nlevel=3, name="naca0012", irsop=1, cfl=2.0, cfl_irs=4.0, nl_crs=2,
                             ncycle=100000)
@@ -78,14 +114,20 @@ def cpu_baseline(case_full, budget_s: float):
     from oracle_bind import Oracle, OrCase, dp
     from cosa_amd.case import Case
     c0 = case_full
-    nb, cells = 0, 0
-    target = 1_500_000
-    for b in c0.blocks:
-        nb += 1
-        cells += (b.imax - 1) * (b.jmax - 1) * (b.kmax - 1)
-        if cells >= target:
-            break
-    sample = Case(dict(c0.config), c0.blocks[:nb], [], c0.xyz[:nb], name="sample")
+    if c0.config["kom"]:
+        # the same wing, 4 blocks of 65 x 17 x 17 nodes (65,536 cells)
+        from cosa_amd import case as cases
+        sample = cases.wing_case(65, 17, 17, nblocks=4, nlevel=3, name="sample", **WING_CFG)
+        nb = "4 (65x17x17 nodes each, the wing at 1/64 of the cells)"
+    else:
+        nb, cells = 0, 0
+        target = 1_500_000
+        for b in c0.blocks:
+            nb += 1
+            cells += (b.imax - 1) * (b.jmax - 1) * (b.kmax - 1)
+            if cells >= target:
+                break
+        sample = Case(dict(c0.config), c0.blocks[:nb], [], c0.xyz[:nb], name="sample")
     o = Oracle()
     o.set_config(sample.config)
     oc = OrCase(sample, q1=sample.perturbed(amp=0.01, seed=1))
@@ -111,11 +153,11 @@ def cpu_baseline(case_full, budget_s: float):
         ncyc += 1
         if rc:
             raise RuntimeError("oracle sample failed")
-        if time.perf_counter() - t0 > budget_s or ncyc >= 8:
+        if time.perf_counter() - t0 > budget_s or ncyc >= 400:
             break
     dt = time.perf_counter() - t0
     return {"value": round(per_cycle * ncyc / dt / 1e6, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{nb} of 96 blocks ({sample.ncell()} cells), {ncyc} multigrid cycles, {dt:.1f} s, "
+            "sample": f"{nb} of {len(c0.blocks)} blocks ({sample.ncell()} cells), {ncyc} multigrid cycles, {dt:.1f} s, "
                       f"oracle/liboracle.so (C restatement, gcc -O2, 1 thread)"}
 
 
@@ -139,7 +181,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="uneven96", choices=["uneven96", "naca"])
+    ap.add_argument("--workload", default="wing4", choices=["wing4", "uneven96", "naca"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -187,6 +229,7 @@ def main():
     log(f"elapsed {elapsed:.3f} s, {value:.1f} Mcell-updates/s, resrms(1) {res_hist[0]:.3e} -> {res_hist[-1]:.3e}")
 
     # roofline of the dominant family (most device time in the timed region)
+    BYTES_PER_CELL = BYTES_PER_CELL_RANS if case.config["kom"] else BYTES_PER_CELL_EULER
     fams = {}
     for name, d in fam.items():
         bpc = BYTES_PER_CELL.get(name)

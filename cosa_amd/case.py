@@ -162,7 +162,7 @@ class Case:
                 for n in range(nsm):
                     xd[m][(o0[b] * nsm + n * n0) * 3:(o0[b] * nsm + (n + 1) * n0) * 3] = flat
             cc = grids.cell_centres(x, y, z).reshape(-1, 3)
-            d = tree.query(cc)[0] if tree is not None else np.full(len(cc), 1.0e6)
+            d = tree.query(cc, workers=8)[0] if tree is not None else np.full(len(cc), 1.0e6)
             dist[o1[b]:o1[b] + L.points(1)[b]] = d
         return {"xd": xd, "dist": dist}
 
@@ -252,6 +252,35 @@ def naca_case(ni=193, nj=49, nk=5, nlevel=1, name="naca0012", pitch=None, wall=1
     ])
     cut = Cut(1, (1, 1, 1), (1, nj, nk), 1, (ni, 1, 1), (ni, nj, nk), (1, 2, 3))
     return Case(c, [b], [cut], [(x, y, z)], name=name, motion=pitch)
+
+
+def wing_case(ni=257, nj=65, nk=65, nblocks=4, nlevel=3, name="wing4", semispan=2.0, ds_wall=5e-6, r_far=20.0,
+              **cfg) -> Case:
+    """Rectangular wing with a NACA 0012 section (chord 1, semi-span
+    `semispan`: aspect ratio 2*semispan), O-H topology: every block is the
+    O-grid of the section (i around, j normal with geometric stretching from
+    ds_wall to r_far) extruded over 1/nblocks of the semi-span; blocks are
+    stacked spanwise and joined by 1:1 cuts, each block's O-grid seam is a
+    self-cut.  Viscous wall (1500) on j = 1, farfield (30) on j = jmax,
+    symmetry (80) at the root and farfield at the outer spanwise plane.
+    4 blocks of 257 x 65 x 65 nodes = 4,194,304 cells."""
+    cfg.setdefault("machfs", 0.3)
+    cfg.setdefault("alpha_deg", 4.0)
+    c = default_config(nlevel=nlevel, **cfg)
+    x, y, z0 = grids.naca0012_ogrid(ni, nj, nk, r_far=r_far, ds_wall=ds_wall, dz=semispan / nblocks / (nk - 1))
+    blocks, xyz, cuts = [], [], []
+    for b in range(nblocks):
+        patches = [Patch((1, 1, 1), (ni, 1, nk), 1500), Patch((1, nj, 1), (ni, nj, nk), 30)]
+        if b == 0:
+            patches.append(Patch((1, 1, 1), (ni, nj, 1), 80))
+        if b == nblocks - 1:
+            patches.append(Patch((1, 1, nk), (ni, nj, nk), 30))
+        blocks.append(Block(ni, nj, nk, patches))
+        xyz.append((x.copy(), y.copy(), z0 + b * semispan / nblocks))
+        cuts.append(Cut(b + 1, (1, 1, 1), (1, nj, nk), b + 1, (ni, 1, 1), (ni, nj, nk), (1, 2, 3)))
+        if b + 1 < nblocks:
+            cuts.append(Cut(b + 1, (1, 1, nk), (ni, nj, nk), b + 2, (1, 1, 1), (ni, nj, 1), (1, 2, 3)))
+    return Case(c, blocks, cuts, xyz, name=name)
 
 
 def channel_block(ni, nj, nk) -> Block:

@@ -830,10 +830,12 @@ __global__ void __launch_bounds__(256) k_cirs(Par p, const Blk* __restrict__ blk
 constexpr int CI = 8;     // cells per chunk
 constexpr int LPAD = 65;  // padded line stride in LDS (conflict-free lane access)
 
+template <int NV>
 __global__ void __launch_bounds__(64) k_cirs_i(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
                                                double* __restrict__ res, const double* __restrict__ dtvol,
-                                               const double* __restrict__ beta, double* __restrict__ rsc) {
-  __shared__ double xs[5][CI][LPAD];
+                                               const double* __restrict__ beta, double* __restrict__ rsc, int v0,
+                                               double epsc) {
+  __shared__ double xs[NV][CI][LPAD];
   __shared__ double rsh[CI][LPAD];
   __shared__ double dsh[CI][LPAD];
   __shared__ double bsh[CI][LPAD];
@@ -846,7 +848,6 @@ __global__ void __launch_bounds__(64) k_cirs_i(Par p, const Blk* __restrict__ bl
   const int nl_here = min(64, total - t.base);
   const int L = im;
   const ll n3 = n3of(b), n1 = n1of(b);
-  const double epsc = 0.25 * (p.rat * p.rat - 1.0);
   const bool cb_const = p.irsop == 1;
   // load slots: position ii = lane % CI of the chunk for lines lane/CI + 8 r
   const int ii_ld = lane % CI;
@@ -859,12 +860,12 @@ __global__ void __launch_bounds__(64) k_cirs_i(Par p, const Blk* __restrict__ bl
     const int gl = t.base + (ok[r] ? ln : 0);
     const int n = gl / nlines, rem = gl - (gl / nlines) * nlines;
     const int j = rem % jm + 1, k = rem / jm + 1;
-    off3[r] = b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n + i3(b, 0, j, k);
+    off3[r] = b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n + n3 * v0 + i3(b, 0, j, k);
     offr[r] = b.o3 * p.nsnap + n3 * (ll)n + i3(b, 0, j, k);
     off1[r] = b.o1 * p.nsnap + n1 * (ll)n + i1(b, 0, j, k);
   }
   const bool active = lane < nl_here;
-  double tp[5] = {0, 0, 0, 0, 0};
+  double tp[NV];
   double rp = 0.0;
   const int nchunk = (L + CI - 1) / CI;
   // forward elimination
@@ -878,7 +879,7 @@ __global__ void __launch_bounds__(64) k_cirs_i(Par p, const Blk* __restrict__ bl
         const int ln = lane / CI + 8 * r;
         const int i = i0 + ii_ld;
 #pragma unroll
-        for (int v = 0; v < 5; v++) xs[v][ii_ld][ln] = res[off3[r] + n3 * v + i];
+        for (int v = 0; v < NV; v++) xs[v][ii_ld][ln] = res[off3[r] + n3 * v + i];
         dsh[ii_ld][ln] = dtvol[off1[r] + i];
         if (!cb_const) bsh[ii_ld][ln] = beta[off1[r] + i];
       }
@@ -896,16 +897,16 @@ __global__ void __launch_bounds__(64) k_cirs_i(Par p, const Blk* __restrict__ bl
         if (a == 1) {
           r = cc / cb;
 #pragma unroll
-          for (int v = 0; v < 5; v++) tp[v] = (xs[v][ii][lane] * dtv) / cb;
+          for (int v = 0; v < NV; v++) tp[v] = (xs[v][ii][lane] * dtv) / cb;
         } else {
           r = cc / (cb - rp * ca);
 #pragma unroll
-          for (int v = 0; v < 5; v++) tp[v] = (xs[v][ii][lane] * dtv - tp[v] * ca) / (cb - rp * ca);
+          for (int v = 0; v < NV; v++) tp[v] = (xs[v][ii][lane] * dtv - tp[v] * ca) / (cb - rp * ca);
         }
         rp = r;
         rsh[ii][lane] = r;
 #pragma unroll
-        for (int v = 0; v < 5; v++) xs[v][ii][lane] = tp[v];
+        for (int v = 0; v < NV; v++) xs[v][ii][lane] = tp[v];
       }
     }
     __syncthreads();
@@ -916,14 +917,14 @@ __global__ void __launch_bounds__(64) k_cirs_i(Par p, const Blk* __restrict__ bl
         const int ln = lane / CI + 8 * r;
         const int i = i0 + ii_ld;
 #pragma unroll
-        for (int v = 0; v < 5; v++) res[off3[r] + n3 * v + i] = xs[v][ii_ld][ln];
+        for (int v = 0; v < NV; v++) res[off3[r] + n3 * v + i] = xs[v][ii_ld][ln];
         rsc[offr[r] + i] = rsh[ii_ld][ln];
       }
     }
     __syncthreads();
   }
   // back substitution
-  double xn[5] = {0, 0, 0, 0, 0};
+  double xn[NV];
   for (int c = nchunk - 1; c >= 0; c--) {
     const int i0 = 1 + c * CI;
     const int cnt = min(CI, L - i0 + 1);
@@ -934,7 +935,7 @@ __global__ void __launch_bounds__(64) k_cirs_i(Par p, const Blk* __restrict__ bl
         const int ln = lane / CI + 8 * r;
         const int i = i0 + ii_ld;
 #pragma unroll
-        for (int v = 0; v < 5; v++) xs[v][ii_ld][ln] = res[off3[r] + n3 * v + i];
+        for (int v = 0; v < NV; v++) xs[v][ii_ld][ln] = res[off3[r] + n3 * v + i];
         rsh[ii_ld][ln] = rsc[offr[r] + i];
       }
     }
@@ -944,14 +945,14 @@ __global__ void __launch_bounds__(64) k_cirs_i(Par p, const Blk* __restrict__ bl
         const int a = i0 + ii;
         if (a == L) {
 #pragma unroll
-          for (int v = 0; v < 5; v++) xn[v] = xs[v][ii][lane];
+          for (int v = 0; v < NV; v++) xn[v] = xs[v][ii][lane];
         } else {
           const double r = rsh[ii][lane];
 #pragma unroll
-          for (int v = 0; v < 5; v++) xn[v] = xs[v][ii][lane] - xn[v] * r;
+          for (int v = 0; v < NV; v++) xn[v] = xs[v][ii][lane] - xn[v] * r;
         }
 #pragma unroll
-        for (int v = 0; v < 5; v++) xs[v][ii][lane] = xn[v];
+        for (int v = 0; v < NV; v++) xs[v][ii][lane] = xn[v];
       }
     }
     __syncthreads();
@@ -962,7 +963,7 @@ __global__ void __launch_bounds__(64) k_cirs_i(Par p, const Blk* __restrict__ bl
         const int ln = lane / CI + 8 * r;
         const int i = i0 + ii_ld;
 #pragma unroll
-        for (int v = 0; v < 5; v++) res[off3[r] + n3 * v + i] = xs[v][ii_ld][ln];
+        for (int v = 0; v < NV; v++) res[off3[r] + n3 * v + i] = xs[v][ii_ld][ln];
       }
     }
     __syncthreads();
@@ -1046,10 +1047,10 @@ hipError_t launch_cirs(Solve& s, Level& L) {
   if (s.par.irsop < 1 || s.par.irsop > 3) return hipSuccess;
   double* res = L.f[COSAHIP_RES];
   const double* dt = L.f[COSAHIP_DTVOL];
-  if (L.ntile[TK_LINES_I])
-    hipLaunchKernelGGL(k_cirs_i, grid1(L.ntile[TK_LINES_I]), dim3(64), 0, s.stream, s.par, L.dblk,
-                       L.tile[TK_LINES_I], res, dt, L.f[COSAHIP_BETAXI], L.cirs_r);
   const double epsc = 0.25 * (s.par.rat * s.par.rat - 1.0);
+  if (L.ntile[TK_LINES_I])
+    hipLaunchKernelGGL(k_cirs_i<5>, grid1(L.ntile[TK_LINES_I]), dim3(64), 0, s.stream, s.par, L.dblk,
+                       L.tile[TK_LINES_I], res, dt, L.f[COSAHIP_BETAXI], L.cirs_r, 0, epsc);
   if (L.ntile[TK_LINES_J])
     hipLaunchKernelGGL((k_cirs<2, 5>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
                        L.tile[TK_LINES_J], res, dt, L.f[COSAHIP_BETAETA], L.cirs_r, 0, 0, 0, epsc);
@@ -1060,9 +1061,9 @@ hipError_t launch_cirs(Solve& s, Level& L) {
     // k, omega: scaled by dtvolt; irsop 1 uses the flow's constant
     // coefficient, irsop 2/3 the turbulence coefficients tbeta*
     const double* dtt = L.f[COSAHIP_DTVOLT];
-    if (L.ntile[TK_LINES_I_T])
-      hipLaunchKernelGGL((k_cirs<1, 2>), grid1(L.ntile[TK_LINES_I_T]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                         L.tile[TK_LINES_I_T], res, dtt, L.tbeta[0], L.cirs_r, 1, 0, 5, epsc);
+    if (L.ntile[TK_LINES_I])
+      hipLaunchKernelGGL(k_cirs_i<2>, grid1(L.ntile[TK_LINES_I]), dim3(64), 0, s.stream, s.par, L.dblk,
+                         L.tile[TK_LINES_I], res, dtt, L.tbeta[0], L.cirs_r, 5, epsc);
     if (L.ntile[TK_LINES_J])
       hipLaunchKernelGGL((k_cirs<2, 2>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
                          L.tile[TK_LINES_J], res, dtt, L.tbeta[1], L.cirs_r, 0, 0, 5, epsc);
