@@ -49,6 +49,18 @@ BYTES_PER_CELL_EULER = {
     "lambda": 144.0,     # q 40 R + metrics 96 R + dtvol 8 W
     "other": None,
 }
+# FP32-residual mode (Euler): the residual reads FP32 copies of q and the
+# face metrics and writes res in FP64; cons2prim also writes the FP32 copy
+BYTES_PER_CELL_FP32_RESIDUAL = {
+    "residual": 108.0,   # q 5x4 R + face metrics 3x4x4 R + res 5x8 W
+    "convert": 100.0,    # q 40 R + 40 W + FP32 copy 5x4 W
+}
+BYTES_PER_CELL_RANS_FP32_RESIDUAL = {
+    # q 7x4 R + face metrics 48 R + metric derivatives 108 R + mut 4 R + vol 8 R
+    # + res 7x8 W + prd, divvel, delpls 24 W
+    "residual": 276.0,
+    "convert": 140.0,    # q 7x8 R + W + FP32 copy 7x4 W
+}
 BYTES_PER_CELL_RANS = {  # Wilcox k-omega, npde = 7
     # q 7x8 R + face metrics 96 R + metric derivatives 3 faces x 9 x 8 = 216 R
     # + mut 8 R + vol 8 R + res 7x8 W + prd, divvel, delpls 24 W
@@ -66,7 +78,7 @@ BYTES_PER_CELL_RANS = {  # Wilcox k-omega, npde = 7
 # WRITE x1.00): profiles/residual_uneven96_fp64_hbm_traffic.md and
 # profiles/residual_wing4_rans_fp64_hbm_traffic.md.  Re-measure when the
 # kernels change.
-MEASURED_TRAFFIC_PER_CELL = {("uneven96", "residual"): 322.7, ("wing4", "residual"): 2693.1}
+MEASURED_TRAFFIC_PER_CELL = {("uneven96", "fp64", "residual"): 322.7, ("wing4", "fp64", "residual"): 2693.1}
 
 
 def log(msg):
@@ -184,6 +196,10 @@ def main():
     ap.add_argument("--workload", default="wing4", choices=["wing4", "uneven96", "naca"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "mixed"],
+                    help="fp32: FP32 residual / FP64 state (cosahip_set_precision mode 1); mixed: FP32 until "
+                         "resrms(1) < --tail-rms, then FP64 (mode 2)")
+    ap.add_argument("--tail-rms", type=float, default=1e-6)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy three times (scale for the HBM "
                          "byte counters FETCH_SIZE / WRITE_SIZE) and exit")
@@ -200,10 +216,15 @@ def main():
     from cosa_amd.solver import Solve
     t0 = time.time()
     case, cfg_desc = make_case(a.workload)
+    cfg_desc["precision"] = {"fp64": "fp64", "fp32": "fp32 residual / fp64 state",
+                             "mixed": f"fp32 residual until resrms(1) < {a.tail_rms:g}, then fp64"}[a.precision]
     log(f"case {case.name}: {case.ncell()} cells, built in {time.time() - t0:.1f} s")
     q0 = case.perturbed(amp=0.01, seed=1)
     s = Solve(case, device=0, q=q0)
     del q0
+    prec_mode = {"fp64": 0, "fp32": 1, "mixed": 2}[a.precision]
+    if prec_mode:
+        s.set_precision(prec_mode, a.tail_rms if prec_mode == 2 else 0.0)
     log("device state resident; warmup")
     mgit = 0
     for _ in range(a.warmup):
@@ -230,6 +251,9 @@ def main():
 
     # roofline of the dominant family (most device time in the timed region)
     BYTES_PER_CELL = BYTES_PER_CELL_RANS if case.config["kom"] else BYTES_PER_CELL_EULER
+    if prec_mode and s.precision_in_use() == 32:
+        BYTES_PER_CELL = dict(BYTES_PER_CELL, **(BYTES_PER_CELL_RANS_FP32_RESIDUAL if case.config["kom"]
+                                                 else BYTES_PER_CELL_FP32_RESIDUAL))
     fams = {}
     for name, d in fam.items():
         bpc = BYTES_PER_CELL.get(name)
@@ -242,7 +266,7 @@ def main():
     dom = max((n for n in fams if BYTES_PER_CELL.get(n)), key=lambda n: fams[n]["ms_total"])
     d = fam[dom]
     achieved = BYTES_PER_CELL[dom] * d["items"] / (d["ms"] / 1e3) / 1e9
-    tpc = MEASURED_TRAFFIC_PER_CELL.get((a.workload, dom))
+    tpc = MEASURED_TRAFFIC_PER_CELL.get((a.workload, a.precision, dom))
     traffic = round(tpc * d["items"] / max(d["launches"], 1)) if tpc else None
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -256,7 +280,8 @@ def main():
 
     out = {"metric": "Mcell-updates/s", "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(elapsed / a.steps * 1e3, 3),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if not prec_mode else "f32 residual / f64 state", "data": "synthetic",
            "config": cfg_desc, "roofline": roof, "cpu_baseline": cpu, "families": fams,
            "resrms1_first_last": [res_hist[0], res_hist[-1]]}
     print(json.dumps(out), flush=True)

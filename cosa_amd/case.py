@@ -78,6 +78,23 @@ class Case:
         t = 2.0 * math.pi * n / ((2 * self.config["nharms"] + 1) * om)
         return m["dtheta0"] * math.sin(om * t), m["dtheta0"] * om * math.cos(om * t)
 
+    def moment_centres(self, xyzmom=(0.25, 0.0, 0.0)) -> np.ndarray:
+        """(xmomr, ymomr, zmomr) of every snapshot: the moment reference
+        point carried with the pitching body as force_b computes it
+        (src/output.f90:4476-4486); the point itself for a fixed body."""
+        out = []
+        for n in range(self.nsnap):
+            xm, ym, zm = xyzmom
+            if self.config["moving"] and self.motion is not None:
+                th = self.snapshot_pose(n)[0]
+                xh, yh = self.motion.get("xh", 0.25), self.motion.get("yh", 0.0)
+                d1 = (xm - xh) * (math.cos(th) - 1) - (ym - yh) * math.sin(th)
+                d2 = (xm - xh) * math.sin(th) + (ym - yh) * (math.cos(th) - 1)
+                out += [xm + d1, ym + d2, zm]
+            else:
+                out += [xm, ym, zm]
+        return np.array(out)
+
     def _level_geometry(self, L: Level) -> Dict[str, np.ndarray]:
         lmet = self.config["lmet"]
         nsm = self.nsnap_met
@@ -89,6 +106,7 @@ class Case:
         sk = np.zeros_like(si)
         vol = np.zeros(self.size(L.nl, "vol"))
         vel = [np.zeros(self.size(L.nl, "x2")) for _ in range(3)] if moving else None
+        xyz2 = [np.zeros(self.size(L.nl, "x2")) for _ in range(3)]
         o2 = L.offsets(2)
         o1 = L.offsets(1)
         for b in range(L.nblocks):
@@ -105,14 +123,19 @@ class Case:
                         p2 = np.zeros((x.shape[0] + 2, x.shape[1] + 2, x.shape[2] + 2))
                         p2[1:-1, 1:-1, 1:-1] = src        # node (i,j,k) at P2 index (i,j,k)
                         dst[o2[b] * nsm + n * n2: o2[b] * nsm + (n + 1) * n2] = p2.reshape(-1)
-                elif n == 0:
-                    mets = grids.face_metrics(x, y, z, lmet)
+                    pose = (xn, yn, zn)
+                else:
+                    pose = (x, y, z)
+                    if n == 0:
+                        mets = grids.face_metrics(x, y, z, lmet)
+                for dst, src in zip(xyz2, pose):
+                    dst[o2[b] * nsm + n * n2: o2[b] * nsm + (n + 1) * n2] = node_coords_p2(src).reshape(-1)
                 for arr, m in zip((si, sj, sk), mets):
                     blk = arr[o2[b] * lmet * nsm: (o2[b] + n2) * lmet * nsm]
                     blk[n * n2 * lmet:(n + 1) * n2 * lmet] = m.reshape(-1)   # [k][j][i][lmet]
             v = grids.cell_volumes(x, y, z)
             vol[o1[b]: o1[b] + L.points(1)[b]] = v.reshape(-1)
-        out = {"si": si, "sj": sj, "sk": sk, "vol": vol}
+        out = {"si": si, "sj": sj, "sk": sk, "vol": vol, "x": xyz2[0], "y": xyz2[1], "z": xyz2[2]}
         if moving:
             out.update(xdot=vel[0], ydot=vel[1], zdot=vel[2])
         if self.config["viscous"]:
@@ -231,6 +254,22 @@ class Case:
 
 # ---------------------------------------------------------------------------
 # configurations
+
+def node_coords_p2(a: np.ndarray) -> np.ndarray:
+    """One coordinate of a block's nodes (kmax, jmax, imax) in the P2 layout
+    (0:imax+1, 0:jmax+1, 0:kmax+1) with the face halo nodes copied from the
+    boundary nodes as cordex_b does (src/grid.f90:23-78); edge and corner
+    halo nodes stay zero."""
+    p = np.zeros((a.shape[0] + 2, a.shape[1] + 2, a.shape[2] + 2))
+    p[1:-1, 1:-1, 1:-1] = a
+    p[1:-1, 0, 1:-1] = p[1:-1, 1, 1:-1]
+    p[1:-1, -1, 1:-1] = p[1:-1, -2, 1:-1]
+    p[1:-1, 1:-1, 0] = p[1:-1, 1:-1, 1]
+    p[1:-1, 1:-1, -1] = p[1:-1, 1:-1, -2]
+    p[0, 1:-1, 1:-1] = p[1, 1:-1, 1:-1]
+    p[-1, 1:-1, 1:-1] = p[-2, 1:-1, 1:-1]
+    return p
+
 
 def naca_case(ni=193, nj=49, nk=5, nlevel=1, name="naca0012", pitch=None, wall=14, r_far=20.0, ds_wall=1e-3,
               **cfg) -> Case:

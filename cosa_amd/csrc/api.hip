@@ -4,6 +4,7 @@
 // smooth() (src/cosa.f90:1594-2032), restrict() (2672-2754) and prolong()
 // (3167-3249).
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -29,7 +30,9 @@ static int fail(Solve* s, int code, const char* what, const char* detail = "") {
 // ---------------------------------------------------------------------------
 // small device helpers
 
-__global__ void k_fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = v;
+}
 
 // ---------------------------------------------------------------------------
 // instrumentation: HIP events recorded on the solve's stream around launches
@@ -79,9 +82,12 @@ static void drain_timing(Solve& s) {
 static void free_level(Level& L) {
   for (int f = 0; f < COSAHIP_NFIELDS; f++)
     if (L.f[f]) (void)hipFree(L.f[f]);
-  void* ptrs[] = {L.dblk, L.dq, L.work1, L.cirs_r, L.rms_part, L.sbc, L.sbc_doff, L.dpatch, L.dcut,
-                  L.vflux, L.tbeta[0], L.tbeta[1], L.tbeta[2], L.h0tab, L.dxd};
+  void* ptrs[] = {L.dblk, L.dq, L.work1, L.cirs_r, L.cirs_rt, L.rms_part, L.sbc, L.sbc_doff, L.dpatch, L.dcut,
+                  L.vflux, L.tbeta[0], L.tbeta[1], L.tbeta[2], L.h0tab, L.dxd, L.q32, L.m32[0], L.m32[1], L.m32[2],
+                  L.dxd32, L.mut32, L.dwall, L.fterms, L.fsums, L.fcoef, L.fgeom};
   for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (float* p : L.xd32)
     if (p) (void)hipFree(p);
   for (int k = 0; k < TK_NKIND; k++)
     if (L.tile[k]) (void)hipFree(L.tile[k]);
@@ -219,6 +225,13 @@ int cosahip_create(const cosahip_config* cfg, int device, cosahip_solve** out) {
   s->par = make_par(*cfg);
   const char* rk = getenv("COSAHIP_RESIDUAL");
   s->split_residual = rk && strcmp(rk, "split") == 0;
+  const char* gk = getenv("COSAHIP_GRAPH");
+  s->use_graph = !(gk && strcmp(gk, "0") == 0);
+  const char* ck = getenv("COSAHIP_CIRS_I");
+  s->cirs_i_lds = ck && strcmp(ck, "lds") == 0;
+  const char* vk = getenv("COSAHIP_VISCOUS");
+  s->fuse_viscous = vk && strcmp(vk, "fused") == 0;
+  s->viscous_faces = vk && strcmp(vk, "faces") == 0;
   if (device >= 0) {
     if (hipSetDevice(device) != hipSuccess) {
       delete s;
@@ -243,6 +256,7 @@ int cosahip_destroy(cosahip_solve* h) {
   (void)hipSetDevice(s->device);
   (void)hipStreamSynchronize(s->stream);
   drain_timing(*s);
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   for (hipEvent_t e : s->evpool) (void)hipEventDestroy(e);
   for (auto& L : s->lev) free_level(L);
   if (s->dflag) (void)hipFree(s->dflag);
@@ -304,8 +318,11 @@ static int build_tables(Solve* s, Level& L, const Level* Lc) {
     add_tiles(L.htile[TK_ROWS_I], b, ns * jm * km * nseg, 4);
     add_tiles(L.htile[TK_LINES_J], b, ns * im * km, TS);
     add_tiles(L.htile[TK_LINES_K], b, ns * im * jm, TS);
+    add_tiles(L.htile[TK_SEGS_J], b, ns * im * km * ((jm + MSEG - 1) / MSEG), TS);
+    add_tiles(L.htile[TK_SEGS_K], b, ns * im * jm * ((km + MSEG - 1) / MSEG), TS);
     add_tiles(L.htile[TK_LINES_I], b, ns * jm * km, 64);
     add_tiles(L.htile[TK_LINES_I_T], b, ns * jm * km, TS);
+    add_tiles(L.htile[TK_LINES_IV], b, ns * jm * km, 256 / s->par.npde);
     if (s->par.viscous) {
       add_tiles(L.htile[TK_FACES_I], b, ns * (im + 1) * jm * km, TS);
       add_tiles(L.htile[TK_FACES_J], b, ns * im * (jm + 1) * km, TS);
@@ -374,6 +391,34 @@ static bool cut_in_bounds(const Blk& b1, const Blk& b2, const int32_t* t, int n2
 }
 
 static ll field_size(const Solve* s, const Level& L, int field);
+
+// FP32 shadows of the level's primitive q and face metrics (FP32-residual mode)
+static int alloc_fp32(Solve* s, Level& L) {
+  if (L.q32) return COSAHIP_OK;
+  const ll nq = field_size(s, L, COSAHIP_Q), nm = field_size(s, L, COSAHIP_SI);
+  if (hipMalloc(&L.q32, nq * sizeof(float)) != hipSuccess) return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
+  for (int d = 0; d < 3; d++)
+    if (hipMalloc(&L.m32[d], nm * sizeof(float)) != hipSuccess)
+      return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
+  L.m32_ok = false;
+  if (s->par.viscous) {
+    const ll nx = field_size(s, L, COSAHIP_XIDERI);
+    for (int m = 0; m < 9; m++)
+      if (hipMalloc(&L.xd32[m], nx * sizeof(float)) != hipSuccess)
+        return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
+    std::vector<float*> t(L.xd32, L.xd32 + 9);
+    if (hipMalloc(&L.dxd32, 9 * sizeof(float*)) != hipSuccess)
+      return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
+    CK(hipMemcpy(L.dxd32, t.data(), 9 * sizeof(float*), hipMemcpyHostToDevice));
+    L.xd32_ok = false;
+  }
+  if (s->par.kom) {
+    if (hipMalloc(&L.mut32, field_size(s, L, COSAHIP_MUT) * sizeof(float)) != hipSuccess)
+      return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
+    L.mut32_ok = false;
+  }
+  return COSAHIP_OK;
+}
 static bool strong_bc(int t) {
   return t == 13 || t == 14 || t / 100 == 15 || t == 22 || t == 23 || t == 92 || t == 93 || t == 5;
 }
@@ -549,7 +594,8 @@ extern "C" int cosahip_define_level(cosahip_solve* h, int nl, int nblocks, const
   }
   const ll nq = L.n3tot * p.npde * p.nsnap;
   if (hipMalloc(&L.dq, nq * sizeof(double)) != hipSuccess || hipMalloc(&L.work1, nq * sizeof(double)) != hipSuccess ||
-      hipMalloc(&L.cirs_r, L.n3tot * p.nsnap * sizeof(double)) != hipSuccess)
+      hipMalloc(&L.cirs_r, L.n3tot * p.nsnap * sizeof(double)) != hipSuccess ||
+      (p.kom && hipMalloc(&L.cirs_rt, L.n3tot * p.nsnap * sizeof(double)) != hipSuccess))
     return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
   CK(hipMemset(L.dq, 0, nq * sizeof(double)));
   CK(hipMemset(L.work1, 0, nq * sizeof(double)));
@@ -560,6 +606,7 @@ extern "C" int cosahip_define_level(cosahip_solve* h, int nl, int nblocks, const
     if (rc) return rc;
   }
   CK(hipMalloc(&L.rms_part, ((ll)L.ntile[TK_CELLS] + 1) * 8 * sizeof(double)));
+  if (s->prec_mode) return alloc_fp32(s, L);
   return COSAHIP_OK;
 }
 
@@ -578,6 +625,7 @@ static ll field_size(const Solve* s, const Level& L, int field) {
       return p.kom ? L.n3tot * p.nsnap : 0;
     case COSAHIP_DIST: return p.viscous ? L.n1tot : 0;
     case COSAHIP_DTVOLT: return p.kom ? L.n1tot * p.nsnap : 0;
+    case COSAHIP_X: case COSAHIP_Y: case COSAHIP_Z: return L.n2tot * p.nsnap_met;
   }
   if (field >= COSAHIP_XIDERI && field <= COSAHIP_ZETADERK) return p.viscous ? L.n0tot * 3 * p.nsnap_met : 0;
   return -1;
@@ -605,7 +653,138 @@ extern "C" int cosahip_upload(cosahip_solve* h, int nl, int field, const double*
   CK(hipSetDevice(s->device));
   CK(hipStreamSynchronize(s->stream));
   CK(hipMemcpy(L->f[field], host, count * sizeof(double), hipMemcpyHostToDevice));
+  if (field == COSAHIP_SI || field == COSAHIP_SJ || field == COSAHIP_SK) L->m32_ok = false;
+  if (field >= COSAHIP_XIDERI && field <= COSAHIP_ZETADERK) L->xd32_ok = false;
+  if (field == COSAHIP_MUT) L->mut32_ok = false;
   return COSAHIP_OK;
+}
+
+// Wall runs of surfaces 1..nsurface (types 1500 + s - 1, and type 14 as
+// surface 1 when include_inviscid) in the order force_b visits them: blocks,
+// then patches, then (i3, i2).
+static int build_wall_runs(Solve* s, Level& L, int nsurface, int include_inviscid) {
+  const int key = nsurface * 2 + (include_inviscid ? 1 : 0);
+  if (L.force_key == key) return COSAHIP_OK;
+  const Par& p = s->par;
+  std::vector<WallRun> runs;
+  int nf = 0, off = 0;
+  for (int b = 0; b < L.nblocks; b++) {
+    for (int ib = 0; ib < L.nbcs[b]; ib++) {
+      const int32_t* t = L.bctopo.data() + 10 * (ll)(off + ib);
+      int surf = 0, visc = 0;
+      if (t[0] >= 1500 && t[0] < 1500 + nsurface) {
+        surf = t[0] - 1499;
+        visc = p.viscous ? 1 : 0;
+      } else if (include_inviscid && t[0] == 14) {
+        surf = 1;
+      }
+      if (!surf) continue;
+      const int idir = t[1];
+      if (idir < 1 || idir > 3) return fail(s, COSAHIP_ERR_ARG, "force: bad patch direction");
+      const int ic2 = idir % 3 + 1, ic3 = (idir + 1) % 3 + 1;
+      const int lo[3] = {t[3], t[5], t[7]}, hi[3] = {t[4], t[6], t[8]};
+      const int n2 = hi[ic2 - 1] - lo[ic2 - 1] + 1, n3 = hi[ic3 - 1] - lo[ic3 - 1] + 1;
+      if (n2 <= 0 || n3 <= 0) continue;
+      WallRun w;
+      w.block = b;
+      w.surface = surf;
+      w.visc = visc;
+      w.idir = idir;
+      w.inrout = t[2];
+      w.lo2 = lo[ic2 - 1];
+      w.n2 = n2;
+      w.lo3 = lo[ic3 - 1];
+      w.first = nf;
+      w.count = n2 * n3;
+      runs.push_back(w);
+      nf += n2 * n3;
+    }
+    off += L.nbcs[b];
+  }
+  void* old[] = {L.dwall, L.fterms, L.fsums, L.fcoef, L.fgeom};
+  for (void* q : old)
+    if (q) CK(hipFree(q));
+  L.dwall = nullptr;
+  L.fterms = L.fsums = L.fcoef = L.fgeom = nullptr;
+  L.force_key = -1;
+  const int nsnap = p.nsnap;
+  if (hipMalloc(&L.dwall, std::max<size_t>(runs.size(), 1) * sizeof(WallRun)) != hipSuccess ||
+      hipMalloc(&L.fterms, std::max<ll>((ll)nf * nsnap * 12, 1) * sizeof(double)) != hipSuccess ||
+      hipMalloc(&L.fsums, (ll)L.nblocks * nsnap * nsurface * 12 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&L.fcoef, (ll)nsnap * nsurface * 12 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&L.fgeom, (13 + 3 * (ll)nsnap) * sizeof(double)) != hipSuccess)
+    return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
+  if (!runs.empty()) CK(hipMemcpy(L.dwall, runs.data(), runs.size() * sizeof(WallRun), hipMemcpyHostToDevice));
+  L.hwall = runs;
+  L.nwall_faces = nf;
+  L.force_key = key;
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_force(cosahip_solve* h, int nl, int nsurface, int include_inviscid, const double* lngth,
+                             const double* momr, double* coef) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  Level* Lp = get_level(s, nl);
+  if (!Lp) return fail(s, COSAHIP_ERR_STATE, "force: level not defined");
+  CK(hipSetDevice(s->device));
+  Level& L = *Lp;
+  if (nsurface < 1 || nsurface > 100 || !lngth || !momr || !coef)
+    return fail(s, COSAHIP_ERR_ARG, "force: nsurface must be 1..100 and lngth, momr, coef non-null");
+  int rc = build_wall_runs(s, L, nsurface, include_inviscid);
+  if (rc) return rc;
+  const cosahip_config& c = s->cfg;
+  const int nsnap = s->par.nsnap;
+  // direction vectors of lift, drag and side force (src/output.f90:4451-4462)
+  const double alpha = c.alpha, beta = c.beta;
+  const double den = std::sqrt(std::sin(beta) * std::sin(beta) + std::cos(alpha) * std::cos(alpha) *
+                                                                     (std::cos(beta) * std::cos(beta)));
+  std::vector<double> g(13 + 3 * (size_t)nsnap);
+  g[0] = -std::sin(alpha) * std::cos(alpha) * (std::cos(beta) * std::cos(beta)) / den;
+  g[1] = den;
+  g[2] = std::sin(alpha) * std::sin(beta) * std::cos(beta) / den;
+  g[3] = std::cos(alpha) * std::cos(beta);
+  g[4] = std::sin(alpha) * std::cos(beta);
+  g[5] = std::sin(beta);
+  g[6] = -std::sin(beta) / den;
+  g[7] = 0.0;
+  g[8] = std::cos(alpha) * std::cos(beta) / den;
+  g[9] = lngth[0];
+  g[10] = lngth[1];
+  g[11] = lngth[2];
+  g[12] = c.machfs > 1e-14 ? c.machfs : 1.0;
+  for (int m = 0; m < 3 * nsnap; m++) g[13 + m] = momr[m];
+  CK(hipMemcpyAsync(L.fgeom, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  CK(launch_force(*s, L, nsurface, L.fgeom, L.fgeom + 13));
+  CK(hipMemcpyAsync(coef, L.fcoef, (size_t)12 * nsnap * nsurface * sizeof(double), hipMemcpyDeviceToHost,
+                    s->stream));
+  CK(hipStreamSynchronize(s->stream));
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_set_precision(cosahip_solve* h, int mode, double tail_rms) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  if (mode < 0 || mode > 2) return fail(s, COSAHIP_ERR_ARG, "set_precision: mode must be 0, 1 or 2");
+  if (mode == 2 && !(tail_rms > 0)) return fail(s, COSAHIP_ERR_ARG, "set_precision: mode 2 needs tail_rms > 0");
+  if (mode && s->split_residual && !s->par.viscous && s->par.npde == 5)
+    return fail(s, COSAHIP_ERR_ARG, "set_precision: the FP32 residual needs the fused residual kernel");
+  CK(hipSetDevice(s->device));
+  s->prec_mode = mode;
+  s->prec_tail_rms = mode == 2 ? tail_rms : 0;
+  s->prec_tail = false;
+  if (mode)
+    for (Level& L : s->lev)
+      if (L.nblocks) {
+        int rc = alloc_fp32(s, L);
+        if (rc) return rc;
+      }
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_precision_in_use(const cosahip_solve* h) {
+  const Solve* s = reinterpret_cast<const Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  return s->fp32() ? 32 : 64;
 }
 
 extern "C" int cosahip_download(cosahip_solve* h, int nl, int field, double* host, int64_t count) {
@@ -718,9 +897,12 @@ static int decode_prtest(Solve* s, int nl, unsigned long long key) {
 }
 
 static const unsigned long long NOFLAG = ~0ull;
+static const int RMS_ASYNC_SLOT = 16;   // pinned slots of the async cycle's norms
+static const int FLAG_ASYNC_SLOT = 32;  // and of its per-level positivity flags
 
-static int read_flag(Solve* s, unsigned long long* key) {
-  CK(hipMemcpyAsync(s->hpinned, s->dflag, sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream));
+// prtest first-offender key of level nl (one flag slot per level)
+static int read_flag(Solve* s, int nl, unsigned long long* key) {
+  CK(hipMemcpyAsync(s->hpinned, s->dflag + (nl - 1), sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream));
   CK(hipStreamSynchronize(s->stream));
   memcpy(key, s->hpinned, sizeof(*key));
   return COSAHIP_OK;
@@ -728,10 +910,10 @@ static int read_flag(Solve* s, unsigned long long* key) {
 
 extern "C" int cosahip_prtest(cosahip_solve* h, int nl, int iter) {
   LEVEL_OR_FAIL(s, nl, L);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, s->stream, s->dflag, NOFLAG);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->stream, s->dflag + (nl - 1), NOFLAG, 1);
   CK(launch_prtest(*s, *L, L->f[COSAHIP_Q], iter));
   unsigned long long key;
-  int rc = read_flag(s, &key);
+  int rc = read_flag(s, nl, &key);
   if (rc) return rc;
   if (key != NOFLAG) return decode_prtest(s, nl, key);
   return COSAHIP_OK;
@@ -743,6 +925,7 @@ extern "C" int cosahip_lambda_deltat(cosahip_solve* h, int nl) {
 }
 extern "C" int cosahip_residual(cosahip_solve* h, int nl) {
   LEVEL_OR_FAIL(s, nl, L);
+  if (s->fp32() && L->q32) CK(launch_to_f32(*s, L->f[COSAHIP_Q], L->q32, field_size(s, *L, COSAHIP_Q)));
   CK(launch_zero_res(*s, *L, L->f[COSAHIP_RES]));
   CK(launch_residual(*s, *L, L->f[COSAHIP_Q], L->f[COSAHIP_RES]));
   return COSAHIP_OK;
@@ -754,7 +937,8 @@ extern "C" int cosahip_src_hb(cosahip_solve* h, int nl) {
   return COSAHIP_OK;
 }
 
-static int rms(Solve* s, Level& L, double* out) {
+// resg_rms partial sums of level L, copied into pinned slot `slot` (no wait)
+static int rms_enqueue(Solve* s, Level& L, int slot) {
   const int np = s->par.npde;
   if (s->cfg.harbal) {
     CK(launch_copy_interior(*s, L, L.work1, L.f[COSAHIP_RES]));
@@ -762,11 +946,17 @@ static int rms(Solve* s, Level& L, double* out) {
   } else {
     CK(launch_rms(*s, L, L.f[COSAHIP_RES], L.f[COSAHIP_RHS], 1));
   }
-  CK(hipMemcpyAsync(s->hpinned, L.rms_part + (ll)L.ntile[TK_CELLS] * 8, np * sizeof(double), hipMemcpyDeviceToHost,
-                    s->stream));
-  CK(hipStreamSynchronize(s->stream));
+  CK(hipMemcpyAsync(s->hpinned + slot, L.rms_part + (ll)L.ntile[TK_CELLS] * 8, np * sizeof(double),
+                    hipMemcpyDeviceToHost, s->stream));
+  return COSAHIP_OK;
+}
+
+// resrms(1:npde+1) from the sums in pinned slot `slot` (after the stream
+// has reached the copy)
+static void rms_finish(Solve* s, Level& L, int slot, double* out) {
+  const int np = s->par.npde;
   double sum[8] = {0};
-  for (int v = 0; v < np; v++) sum[v] = s->hpinned[v];
+  for (int v = 0; v < np; v++) sum[v] = s->hpinned[slot + v];
   double tot = 0;
   for (int v = 0; v < np; v++) tot = tot + sum[v];
   const double nc = (double)L.ncell;
@@ -778,6 +968,13 @@ static int rms(Solve* s, Level& L, double* out) {
     for (int v = 0; v < np; v++) out[v] = sqrt(sum[v] / nc);
     out[np] = sqrt(tot / nc / np);
   }
+}
+
+static int rms(Solve* s, Level& L, double* out) {
+  int rc = rms_enqueue(s, L, 0);
+  if (rc) return rc;
+  CK(hipStreamSynchronize(s->stream));
+  rms_finish(s, L, 0, out);
   return COSAHIP_OK;
 }
 
@@ -832,15 +1029,18 @@ extern "C" int cosahip_update(cosahip_solve* h, int nl, double ar) {
 // ---------------------------------------------------------------------------
 // smooth (src/cosa.f90:1594-2032)
 
-extern "C" int cosahip_smooth(cosahip_solve* h, int nl, int from_fine, int to_coarse, int niter, int mgit,
-                              int32_t* done, double* resrms) {
-  LEVEL_OR_FAIL(s, nl, Lp);
-  Level& L = *Lp;
+// The stage loop of smooth.  async: the launch sequence of one V-cycle
+// enqueued without host synchronisation (cosahip_cycle_enqueue): the
+// level-1 norms go to a pinned slot and are read, with the convergence test
+// and the positivity flags, by cosahip_cycle_wait.
+static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter, int mgit, bool async,
+                       int32_t* done) {
+  Level& L = s->lev[nl - 1];
   const cosahip_config& c = s->cfg;
   if (done) *done = 0;
   double* q = L.f[COSAHIP_Q];
   const int niterp = to_coarse ? niter + 1 : niter;
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, s->stream, s->dflag, NOFLAG);
+  if (!async) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->stream, s->dflag + (nl - 1), NOFLAG, 1);
   int seq = 0;
   bool returned_early = false;
   for (int iter = 1; iter <= niterp && !returned_early; iter++) {
@@ -852,12 +1052,13 @@ extern "C" int cosahip_smooth(cosahip_solve* h, int nl, int from_fine, int to_co
     for (int irk = 1; irk <= c.nstage; irk++, seq++) {
       {
         TimeScope t(*s, COSAHIP_FAM_CONVERT, cs);
-        CK(launch_cons2prim(*s, L, q));
+        CK(launch_cons2prim(*s, L, q, true));
       }
       {
         TimeScope t(*s, COSAHIP_FAM_HALO, cs);
         CK(launch_bc(*s, L, q));
         CK(launch_cut(*s, L, q));
+        CK(launch_halo_f32(*s, L));
       }
       if (c.kom && mgit == 1 && iter == 1 && irk == 1 && from_fine && !c.prd_all_lvls) {
         // first visit of a coarse level: mut from the restricted state
@@ -884,11 +1085,16 @@ extern "C" int cosahip_smooth(cosahip_solve* h, int nl, int from_fine, int to_co
         TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
         CK(launch_src_hb(*s, L));
       }
-      if (nl == 1 && irk == 1 && iter != niter + 1 && (mgit == 1 || mgit % c.iprint == 0 || mgit == c.ncycle)) {
+      if (async && nl == 1 && irk == 1 && iter != niter + 1) {
+        int rc = rms_enqueue(s, L, RMS_ASYNC_SLOT);
+        if (rc) return rc;
+      } else if (nl == 1 && irk == 1 && iter != niter + 1 &&
+                 (mgit == 1 || mgit % c.iprint == 0 || mgit == c.ncycle)) {
         double rr[8];
         int rc = rms(s, L, rr);
         if (rc) return rc;
         for (int v = 0; v <= s->par.npde; v++) s->resrms[v + 1] = rr[v];
+        if (s->prec_mode == 2 && !s->prec_tail && s->resrms[1] < s->prec_tail_rms) s->prec_tail = true;
       }
       if (from_fine && iter == 1 && irk == 1) {
         TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
@@ -921,6 +1127,7 @@ extern "C" int cosahip_smooth(cosahip_solve* h, int nl, int from_fine, int to_co
       }
     }
     if (returned_early) break;
+    if (async) continue;   // convergence is tested by cosahip_cycle_wait
     double errmax = s->resrms[1];
     for (int v = 2; v <= 5; v++) errmax = std::max(errmax, s->resrms[v]);
     if (errmax < c.toler) {
@@ -928,8 +1135,18 @@ extern "C" int cosahip_smooth(cosahip_solve* h, int nl, int from_fine, int to_co
       break;
     }
   }
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_smooth(cosahip_solve* h, int nl, int from_fine, int to_coarse, int niter, int mgit,
+                              int32_t* done, double* resrms) {
+  LEVEL_OR_FAIL(s, nl, Lp);
+  (void)Lp;
+  if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "smooth: an enqueued cycle has not been waited for");
+  int rc = smooth_core(s, nl, from_fine, to_coarse, niter, mgit, false, done);
+  if (rc) return rc;
   unsigned long long key;
-  int rc = read_flag(s, &key);
+  rc = read_flag(s, nl, &key);
   if (rc) return rc;
   if (resrms)
     for (int v = 0; v < 8; v++) resrms[v] = s->resrms[v + 1];
@@ -953,12 +1170,7 @@ static int transfer_ready(Solve* s, int nl) {
   return COSAHIP_OK;
 }
 
-extern "C" int cosahip_restrict(cosahip_solve* h, int nl) {
-  Solve* s = reinterpret_cast<Solve*>(h);
-  if (!s) return COSAHIP_ERR_ARG;
-  int rc = transfer_ready(s, nl);
-  if (rc) return rc;
-  CK(hipSetDevice(s->device));
+static int restrict_core(Solve* s, int nl) {
   Level& L1 = s->lev[nl - 1];
   Level& L2 = s->lev[nl];
   TimeScope t(*s, COSAHIP_FAM_OTHER, L2.ncell * s->par.nsnap);
@@ -973,12 +1185,7 @@ extern "C" int cosahip_restrict(cosahip_solve* h, int nl) {
   return COSAHIP_OK;
 }
 
-extern "C" int cosahip_prolong(cosahip_solve* h, int nl) {
-  Solve* s = reinterpret_cast<Solve*>(h);
-  if (!s) return COSAHIP_ERR_ARG;
-  int rc = transfer_ready(s, nl);
-  if (rc) return rc;
-  CK(hipSetDevice(s->device));
+static int prolong_core(Solve* s, int nl) {
   Level& L1 = s->lev[nl - 1];
   Level& L2 = s->lev[nl];
   TimeScope t(*s, COSAHIP_FAM_OTHER, L1.ncell * s->par.nsnap);
@@ -993,6 +1200,124 @@ extern "C" int cosahip_prolong(cosahip_solve* h, int nl) {
   }
   CK(launch_sub_box(*s, L2, L2.f[COSAHIP_Q], L2.f[COSAHIP_QOLD]));
   CK(launch_prol(*s, L1, L2, L1.f[COSAHIP_Q], L2.f[COSAHIP_Q]));
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_restrict(cosahip_solve* h, int nl) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  int rc = transfer_ready(s, nl);
+  if (rc) return rc;
+  if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "restrict: an enqueued cycle has not been waited for");
+  CK(hipSetDevice(s->device));
+  return restrict_core(s, nl);
+}
+
+extern "C" int cosahip_prolong(cosahip_solve* h, int nl) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  int rc = transfer_ready(s, nl);
+  if (rc) return rc;
+  if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "prolong: an enqueued cycle has not been waited for");
+  CK(hipSetDevice(s->device));
+  return prolong_core(s, nl);
+}
+
+// ---------------------------------------------------------------------------
+// asynchronous multigrid cycles
+
+// One pass of mg's cycle loop (src/cosa.f90:822-912) with nstart = npre =
+// npost = ncrs = 1, as launches only: fine smooth, the coarse levels of the
+// V and the prolongations, then the per-level positivity flags are copied
+// to pinned memory.
+static int cycle_body(Solve* s, int mgit) {
+  const int nlevel = s->cfg.nlevel;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->stream, s->dflag, NOFLAG, nlevel);
+  int rc = smooth_core(s, 1, 0, nlevel > 1, 1, mgit, true, nullptr);
+  if (!rc && nlevel > 1) {
+    rc = restrict_core(s, 1);
+    if (!rc) rc = smooth_core(s, 2, 1, nlevel > 2, 1, mgit, true, nullptr);
+    if (!rc && nlevel > 2) {
+      rc = restrict_core(s, 2);
+      if (!rc) rc = smooth_core(s, 3, 1, 0, 1, mgit, true, nullptr);
+      if (!rc) rc = prolong_core(s, 2);
+      if (!rc) rc = smooth_core(s, 2, 0, 0, 1, mgit, true, nullptr);
+    }
+    if (!rc) rc = prolong_core(s, 1);
+  }
+  if (rc) return rc;
+  CK(hipMemcpyAsync(s->hpinned + FLAG_ASYNC_SLOT, s->dflag, nlevel * sizeof(unsigned long long),
+                    hipMemcpyDeviceToHost, s->stream));
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_cycle_enqueue(cosahip_solve* h, int mgit) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "cycle_enqueue: the previous cycle has not been waited for");
+  const int nlevel = s->cfg.nlevel;
+  if (nlevel < 1 || nlevel > 3 || mgit < 1) return fail(s, COSAHIP_ERR_ARG, "cycle_enqueue: 1..3 levels, mgit >= 1");
+  for (int nl = 1; nl <= nlevel; nl++)
+    if (!s->lev[nl - 1].nblocks) return fail(s, COSAHIP_ERR_STATE, "cycle_enqueue: every level must be defined");
+  for (int nl = 1; nl < nlevel; nl++) {
+    int rc = transfer_ready(s, nl);
+    if (rc) return rc;
+  }
+  CK(hipSetDevice(s->device));
+  // the first cycle differs (k-omega: mut of a coarse level on its first
+  // visit) and runs eagerly; later cycles replay one captured graph
+  const bool graph = s->use_graph && mgit >= 2 && !s->timing;
+  const int key = s->fp32() ? 32 : 64;
+  if (graph && s->gexec && s->gkey == key) {
+    CK(hipGraphLaunch(s->gexec, s->stream));
+  } else if (graph) {
+    if (s->gexec) {
+      CK(hipGraphExecDestroy(s->gexec));
+      s->gexec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    CK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
+    const int rc = cycle_body(s, mgit);
+    const hipError_t e = hipStreamEndCapture(s->stream, &g);
+    if (rc) {
+      if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    CK(e);
+    const hipError_t ei = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    CK(ei);
+    s->gkey = key;
+    CK(hipGraphLaunch(s->gexec, s->stream));
+  } else {
+    int rc = cycle_body(s, mgit);
+    if (rc) return rc;
+  }
+  s->cycle_pending = true;
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_cycle_wait(cosahip_solve* h, int32_t* done, double* resrms) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  if (!s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "cycle_wait: no cycle enqueued");
+  CK(hipSetDevice(s->device));
+  CK(hipStreamSynchronize(s->stream));
+  s->cycle_pending = false;
+  double rr[8];
+  rms_finish(s, s->lev[0], RMS_ASYNC_SLOT, rr);
+  for (int v = 0; v <= s->par.npde; v++) s->resrms[v + 1] = rr[v];
+  if (s->prec_mode == 2 && !s->prec_tail && s->resrms[1] < s->prec_tail_rms) s->prec_tail = true;
+  double errmax = s->resrms[1];
+  for (int v = 2; v <= 5; v++) errmax = std::max(errmax, s->resrms[v]);
+  if (done) *done = errmax < s->cfg.toler ? 1 : 0;
+  if (resrms)
+    for (int v = 0; v < 8; v++) resrms[v] = s->resrms[v + 1];
+  for (int nl = 1; nl <= s->cfg.nlevel; nl++) {
+    unsigned long long key;
+    memcpy(&key, s->hpinned + FLAG_ASYNC_SLOT + (nl - 1), sizeof(key));
+    if (key != NOFLAG) return decode_prtest(s, nl, key);
+  }
   return COSAHIP_OK;
 }
 

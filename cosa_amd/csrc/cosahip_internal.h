@@ -83,6 +83,9 @@ enum TileKind {
   TK_HALO,        // prtest region: interior + face halo strips
   TK_FUSED,       // fused residual: (31 x 7)-cell (i,j) tiles x 32-plane k segments
   TK_LINES_I_T,   // (jmax-1)(kmax-1) lines along i, TS per tile (thread per line)
+  TK_LINES_IV,    // (jmax-1)(kmax-1) lines along i, 256/npde per tile (thread per line and variable)
+  TK_SEGS_J,      // MSEG-cell segments of the (imax-1)(kmax-1) lines along j, i fastest
+  TK_SEGS_K,      // MSEG-cell segments of the (imax-1)(jmax-1) lines along k, i fastest
   TK_FACES_I,     // i faces imax x (jmax-1) x (kmax-1)
   TK_FACES_J,     // j faces (imax-1) x jmax x (kmax-1)
   TK_FACES_K,     // k faces (imax-1) x (jmax-1) x kmax
@@ -103,6 +106,17 @@ struct CutRec {
   int count, n2;   // (len(ic2)+1)*(len(ic3)+1), len(ic2)+1
 };
 
+// one wall BC patch of one block for the force integration: faces in the
+// reference's loop order (i3 outer, i2 inner)
+struct WallRun {
+  int block;      // 0-based
+  int surface;    // 1-based
+  int visc;       // 1: the viscous part is integrated too
+  int idir, inrout;
+  int lo2, n2, lo3;
+  int first, count;   // face range in the term array
+};
+
 struct Level {
   int nl = 0, nblocks = 0, ncuts = 0;
   std::vector<Blk> hblk;
@@ -113,6 +127,7 @@ struct Level {
   double* dq = nullptr;       // HB term (qhb) and scratch
   double* work1 = nullptr;    // copy of res for the HB norm
   double* cirs_r = nullptr;   // Thomas multipliers r(j), one per cell
+  double* cirs_rt = nullptr;  // the same for k, omega (k-omega)
   double* rms_part = nullptr; // per-tile partial sums of squares
   unsigned char* sbc = nullptr;  // strong-BC face flags (6 faces per block)
   std::vector<ll> sbc_off;    // per block: offset of its 6 face flag planes
@@ -132,6 +147,25 @@ struct Level {
   double* tbeta[3] = {};      // CIRS coefficients of k, omega (P1, 0:2N)
   double* h0tab = nullptr;    // per (block, snapshot, direction): h0l, h0r of the block's last face
   double** dxd = nullptr;     // device table of the nine metric-derivative fields
+  // FP32-residual mode: single-precision shadows read by the residual kernel
+  float* q32 = nullptr;       // primitive q, same layout as Q
+  float* m32[3] = {};         // si, sj, sk
+  bool m32_ok = false;        // m32 holds the current metrics
+  float* xd32[9] = {};        // metric derivatives (viscous)
+  float** dxd32 = nullptr;    // device table of xd32
+  bool xd32_ok = false;
+  float* mut32 = nullptr;     // eddy viscosity (k-omega)
+  bool mut32_ok = false;
+  // force integration: wall runs of the last (nsurface, include_inviscid)
+  // request, per-face terms, per-block sums, coefficients
+  int force_key = -1;
+  std::vector<struct WallRun> hwall;
+  struct WallRun* dwall = nullptr;
+  int nwall_faces = 0;
+  double* fterms = nullptr;   // (12, faces, 0:2N)
+  double* fsums = nullptr;    // (12, nsurface, 0:2N, blocks)
+  double* fcoef = nullptr;    // (12, 0:2N, nsurface)
+  double* fgeom = nullptr;    // direction vectors, lengths, Mach, moment centres
 };
 
 struct TimedLaunch {
@@ -156,6 +190,29 @@ struct Solve {
   // residual kernel choice: fused single pass (default) or one pass per
   // direction (COSAHIP_RESIDUAL=split in the environment at create time)
   bool split_residual = false;
+  // viscous face fluxes: flux and difference in one kernel per direction
+  // after the inviscid pass (default), fused into the inviscid line march
+  // (COSAHIP_VISCOUS=fused), or face fluxes stored and differenced by a
+  // second kernel (COSAHIP_VISCOUS=faces), all at create time
+  bool fuse_viscous = false;
+  bool viscous_faces = false;
+  // CIRS i-sweep: one thread per (line, variable) (default) or one thread per
+  // line with LDS-staged chunks (COSAHIP_CIRS_I=lds at create time)
+  bool cirs_i_lds = false;
+  // residual precision: 0 FP64, 1 FP32 residual / FP64 state, 2 FP32 until
+  // the level-1 density residual rms falls below prec_tail_rms, FP64 after
+  int prec_mode = 0;
+  double prec_tail_rms = 0;
+  bool prec_tail = false;
+  bool fp32() const { return prec_mode == 1 || (prec_mode == 2 && !prec_tail); }
+  // asynchronous multigrid cycles (cosahip_cycle_enqueue / _wait): a cycle
+  // is in flight; the cycle's launch sequence captured as a HIP graph
+  // (replayed while its key -- the residual precision -- is unchanged;
+  // COSAHIP_GRAPH=0 at create time runs every cycle eagerly)
+  bool cycle_pending = false;
+  bool use_graph = true;
+  hipGraphExec_t gexec = nullptr;
+  int gkey = -1;
   // instrumentation
   bool timing = false;
   std::vector<TimedLaunch> pending;
@@ -177,5 +234,6 @@ struct TimeScope {
 // tile sizes
 constexpr int TS = 256;        // items per tile, 1-D kernels
 constexpr int ROWSEG = 63;     // cells per i-row segment (64 faces)
+constexpr int MSEG = 16;       // cells per segment of a j / k line march
 
 }  // namespace cosahip

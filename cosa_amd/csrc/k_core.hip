@@ -79,7 +79,7 @@ __device__ __forceinline__ const double* qblk(const Par& p, const Blk& b, const 
 
 // ---- cons2prim_b (src/update.f90:2136-2188), interior ----
 __global__ void __launch_bounds__(256) k_cons2prim(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
-                                                   double* __restrict__ q) {
+                                                   double* __restrict__ q, float* __restrict__ q32) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   CellIdx c;
@@ -94,7 +94,36 @@ __global__ void __launch_bounds__(256) k_cons2prim(Par p, const Blk* __restrict_
   qb[x + n3] = u;
   qb[x + 2 * n3] = v;
   qb[x + 3 * n3] = w;
-  qb[x + 4 * n3] = (p.gamma - 1) * (rhoe - rho * (u * u + v * v + w * w) / 2 - rho * tke);
+  const double pp = (p.gamma - 1) * (rhoe - rho * (u * u + v * v + w * w) / 2 - rho * tke);
+  qb[x + 4 * n3] = pp;
+  if (q32) {   // FP32 shadow of the primitive state for the FP32-residual mode
+    float* sb = q32 + (qb - q);
+    sb[x] = (float)rho;
+    sb[x + n3] = (float)u;
+    sb[x + 2 * n3] = (float)v;
+    sb[x + 3 * n3] = (float)w;
+    sb[x + 4 * n3] = (float)pp;
+    for (int m = 5; m < p.npde; m++) sb[x + m * n3] = (float)qb[x + m * n3];
+  }
+}
+
+// FP32 shadow of the halo strips of q (the points of the prim2cons region
+// outside the interior), after the BC and cut fills.
+__global__ void __launch_bounds__(256) k_halo_f32(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
+                                                  const double* __restrict__ q, float* __restrict__ q32) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  int n, reg, loc, i, j, k;
+  if (!halo_of(p, b, t.base + threadIdx.x, n, reg, loc, i, j, k)) return;
+  if (reg == 0 && i >= 1 && i <= b.imax - 1) return;
+  const ll o = b.o3 * p.npde * p.nsnap + n3of(b) * (ll)p.npde * n;
+  const ll n3 = n3of(b), x = i3(b, i, j, k);
+  for (int m = 0; m < p.npde; m++) q32[o + x + m * n3] = (float)q[o + x + m * n3];
+}
+
+// dst = (float)src, n elements
+__global__ void __launch_bounds__(256) k_to_f32(const double* __restrict__ src, float* __restrict__ dst, ll n) {
+  for (ll x = (ll)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (ll)gridDim.x * blockDim.x) dst[x] = (float)src[x];
 }
 
 // ---- prim2cons_b (src/update.f90:1875-2084), interior + face halo strips ----
@@ -566,7 +595,8 @@ __global__ void __launch_bounds__(256) k_cut(Par p, const Blk* __restrict__ blks
 }
 
 // ---- q_bedges / q_bcorns (src/flux.f90:3914-4263, 3156-3471) ----
-__global__ void __launch_bounds__(256) k_edges(Par p, const Blk* __restrict__ blks, double* __restrict__ q) {
+__global__ void __launch_bounds__(256) k_edges(Par p, const Blk* __restrict__ blks, double* __restrict__ q,
+                                               float* __restrict__ q32) {
   const Blk b = blks[blockIdx.y];
   const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
   const int per = 4 * (km + jm + im);
@@ -596,7 +626,9 @@ __global__ void __launch_bounds__(256) k_edges(Par p, const Blk* __restrict__ bl
                 ((ys[3] - ys[1]) * (xs[2] - xs[1]) - (ys[2] - ys[1]) * (xs[3] - xs[1]));
     double aa = (q2 - q1 - bb * (ys[2] - ys[1])) / (xs[2] - xs[1]);
     double cc = q1 - aa * xs[1] - bb * ys[1];
-    qb[idx[0] + v * n3] = aa * xs[0] + bb * ys[0] + cc;
+    const double e = aa * xs[0] + bb * ys[0] + cc;
+    qb[idx[0] + v * n3] = e;
+    if (q32) q32[(qb - q) + idx[0] + v * n3] = (float)e;
   }
 }
 
@@ -777,9 +809,24 @@ __global__ void __launch_bounds__(256) k_prol(Par p, const Blk* __restrict__ b1s
 // ------------------------------------------------------------------------
 // launchers
 
-hipError_t launch_cons2prim(Solve& s, Level& L, double* q) {
+hipError_t launch_cons2prim(Solve& s, Level& L, double* q, bool shadow) {
+  float* q32 = shadow && s.fp32() && q == L.f[COSAHIP_Q] ? L.q32 : nullptr;
   if (L.ntile[TK_CELLS])
-    hipLaunchKernelGGL(k_cons2prim, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_CELLS], q);
+    hipLaunchKernelGGL(k_cons2prim, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_CELLS], q,
+                       q32);
+  return hipGetLastError();
+}
+hipError_t launch_halo_f32(Solve& s, Level& L) {
+  if (!s.fp32() || !L.q32) return hipSuccess;
+  if (L.ntile[TK_HALO])
+    hipLaunchKernelGGL(k_halo_f32, grid1(L.ntile[TK_HALO]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_HALO],
+                       L.f[COSAHIP_Q], L.q32);
+  return hipGetLastError();
+}
+hipError_t launch_to_f32(Solve& s, const double* src, float* dst, ll n) {
+  if (n <= 0) return hipSuccess;
+  const ll nb = std::min<ll>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nb), dim3(256), 0, s.stream, src, dst, n);
   return hipGetLastError();
 }
 hipError_t launch_prim2cons(Solve& s, Level& L, double* q) {
@@ -845,7 +892,7 @@ hipError_t launch_rms(Solve& s, Level& L, const double* res, const double* rhs, 
 hipError_t launch_prtest(Solve& s, Level& L, const double* q, int stage_seq) {
   if (L.ntile[TK_HALO])
     hipLaunchKernelGGL(k_prtest, grid1(L.ntile[TK_HALO]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_HALO], q,
-                       stage_seq, s.dflag);
+                       stage_seq, s.dflag + (L.nl - 1));
   return hipGetLastError();
 }
 hipError_t launch_bc(Solve& s, Level& L, double* q) {
@@ -884,16 +931,16 @@ hipError_t launch_edges_corners(Solve& s, Level& L, double* q) {
   int mx = 0;
   for (auto& b : L.hblk) mx = std::max(mx, 4 * ((b.imax - 1) + (b.jmax - 1) + (b.kmax - 1)));
   dim3 g((mx * s.par.nsnap + TS - 1) / TS, L.nblocks);
-  hipLaunchKernelGGL(k_edges, g, dim3(TS), 0, s.stream, s.par, L.dblk, q);
+  hipLaunchKernelGGL(k_edges, g, dim3(TS), 0, s.stream, s.par, L.dblk, q, nullptr);
   dim3 g2((8 * s.par.nsnap + 63) / 64, L.nblocks);
   hipLaunchKernelGGL(k_corners, g2, dim3(64), 0, s.stream, s.par, L.dblk, q);
   return hipGetLastError();
 }
-hipError_t launch_edges(Solve& s, Level& L, double* q) {
+hipError_t launch_edges(Solve& s, Level& L, double* q, float* q32) {
   int mx = 0;
   for (auto& b : L.hblk) mx = std::max(mx, 4 * ((b.imax - 1) + (b.jmax - 1) + (b.kmax - 1)));
   dim3 g((mx * s.par.nsnap + TS - 1) / TS, L.nblocks);
-  hipLaunchKernelGGL(k_edges, g, dim3(TS), 0, s.stream, s.par, L.dblk, q);
+  hipLaunchKernelGGL(k_edges, g, dim3(TS), 0, s.stream, s.par, L.dblk, q, q32);
   return hipGetLastError();
 }
 // cut copies of a P3 array with ncomp variables per snapshot (mut: 1)

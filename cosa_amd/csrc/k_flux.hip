@@ -10,78 +10,79 @@
 //   resid           : src/cosa.f90:1149-1211
 //   cirs_b + tridi  : src/cosa.f90:5238-5623, 2418-2443
 #include "kernels.h"
+#include "viscous.h"
 
 namespace cosahip {
 
 // ------------------------------------------------------------------------
 // Roe flux-difference splitting for one face (mean-flow variables).
-template <int NV>
-__device__ __forceinline__ void roe_face(const Par& p, const double* qr, const double* ql, double kx, double ky,
-                                         double kz, double ds, double ubn, double* F, double* h0lr = nullptr) {
-  const double gamma = p.gamma;
-  double rhor = qr[0], ur = qr[1], vr = qr[2], wr = qr[3], pr = qr[4];
-  double unr = ur * kx + vr * ky + wr * kz - ubn;
-  double q2r = ur * ur + vr * vr + wr * wr;
-  double tker = NV > 5 ? qr[5] : 0.0;
-  double h0r = gamma / (gamma - 1) * pr / rhor + q2r / 2 + tker;
-  double rh0r = rhor * h0r;
-  double fr0 = rhor * unr;
-  double fr1 = (rhor * ur * unr + pr * kx);
-  double fr2 = (rhor * vr * unr + pr * ky);
-  double fr3 = (rhor * wr * unr + pr * kz);
-  double fr4 = (rh0r * unr + pr * ubn);
-  double rhol = ql[0], ul = ql[1], vl = ql[2], wl = ql[3], pl = ql[4];
-  double unl = ul * kx + vl * ky + wl * kz - ubn;
-  double q2l = ul * ul + vl * vl + wl * wl;
-  double tkel = NV > 5 ? ql[5] : 0.0;
-  double h0l = gamma / (gamma - 1) * pl / rhol + q2l / 2 + tkel;
+template <int NV, typename T = double>
+__device__ __forceinline__ void roe_face(const Par& p, const T* qr, const T* ql, T kx, T ky, T kz, T ds, T ubn,
+                                         T* F, T* h0lr = nullptr) {
+  const T gamma = (T)p.gamma;
+  T rhor = qr[0], ur = qr[1], vr = qr[2], wr = qr[3], pr = qr[4];
+  T unr = ur * kx + vr * ky + wr * kz - ubn;
+  T q2r = ur * ur + vr * vr + wr * wr;
+  T tker = NV > 5 ? qr[5] : T(0);
+  T h0r = gamma / (gamma - 1) * pr / rhor + q2r / 2 + tker;
+  T rh0r = rhor * h0r;
+  T fr0 = rhor * unr;
+  T fr1 = (rhor * ur * unr + pr * kx);
+  T fr2 = (rhor * vr * unr + pr * ky);
+  T fr3 = (rhor * wr * unr + pr * kz);
+  T fr4 = (rh0r * unr + pr * ubn);
+  T rhol = ql[0], ul = ql[1], vl = ql[2], wl = ql[3], pl = ql[4];
+  T unl = ul * kx + vl * ky + wl * kz - ubn;
+  T q2l = ul * ul + vl * vl + wl * wl;
+  T tkel = NV > 5 ? ql[5] : T(0);
+  T h0l = gamma / (gamma - 1) * pl / rhol + q2l / 2 + tkel;
   if (h0lr) {
     h0lr[0] = h0l;
     h0lr[1] = h0r;
   }
-  double rh0l = rhol * h0l;
-  double fl0 = rhol * unl;
-  double fl1 = (rhol * ul * unl + pl * kx);
-  double fl2 = (rhol * vl * unl + pl * ky);
-  double fl3 = (rhol * wl * unl + pl * kz);
-  double fl4 = (rh0l * unl + pl * ubn);
-  double drho = rhor - rhol, dp = pr - pl, du = ur - ul, dv = vr - vl, dw = wr - wl, dun = unr - unl;
-  double dtke = tker - tkel;
-  double rhols = sqrt(rhol), rhors = sqrt(rhor);
-  double rho = rhols * rhors;
-  double denom = rhols + rhors;
-  double u = (rhols * ul + rhors * ur) / denom;
-  double v = (rhols * vl + rhors * vr) / denom;
-  double w = (rhols * wl + rhors * wr) / denom;
-  double un = u * kx + v * ky + w * kz;
-  double h0 = (rhols * h0l + rhors * h0r) / denom;
-  double tke = (rhols * tkel + rhors * tker) / denom;
-  double q2 = u * u + v * v + w * w;
-  double a2 = (gamma - 1) * (h0 - q2 / 2 - tke);
-  double a = sqrt(a2);
-  double rm123 = fabs(un - ubn), rm4 = fabs(un + a - ubn), rm5 = fabs(un - a - ubn);
+  T rh0l = rhol * h0l;
+  T fl0 = rhol * unl;
+  T fl1 = (rhol * ul * unl + pl * kx);
+  T fl2 = (rhol * vl * unl + pl * ky);
+  T fl3 = (rhol * wl * unl + pl * kz);
+  T fl4 = (rh0l * unl + pl * ubn);
+  T drho = rhor - rhol, dp = pr - pl, du = ur - ul, dv = vr - vl, dw = wr - wl, dun = unr - unl;
+  T dtke = tker - tkel;
+  T rhols = sqrt(rhol), rhors = sqrt(rhor);
+  T rho = rhols * rhors;
+  T denom = rhols + rhors;
+  T u = (rhols * ul + rhors * ur) / denom;
+  T v = (rhols * vl + rhors * vr) / denom;
+  T w = (rhols * wl + rhors * wr) / denom;
+  T un = u * kx + v * ky + w * kz;
+  T h0 = (rhols * h0l + rhors * h0r) / denom;
+  T tke = (rhols * tkel + rhors * tker) / denom;
+  T q2 = u * u + v * v + w * w;
+  T a2 = (gamma - 1) * (h0 - q2 / 2 - tke);
+  T a = sqrt(a2);
+  T rm123 = fabs(un - ubn), rm4 = fabs(un + a - ubn), rm5 = fabs(un - a - ubn);
   if (p.efx_setup_f) {
-    double rdelta = p.cntrpy_f * (p.entfx_f_w[0] * p.entfxctf_f + p.entfx_f_w[1] * (sqrt(u * u + v * v + w * w) + a) +
-                                  p.entfx_f_w[2] * 0.2 * (fabs(un) + a + fabs(ubn)));
-    rdelta = fmax(rdelta, 1.e-12);
-    if (rm123 <= rdelta && p.efx_all_f) rm123 = 0.5 * (rm123 * rm123 + rdelta * rdelta) / rdelta;
-    if (rm4 <= rdelta) rm4 = 0.5 * (rm4 * rm4 + rdelta * rdelta) / rdelta;
-    if (rm5 <= rdelta) rm5 = 0.5 * (rm5 * rm5 + rdelta * rdelta) / rdelta;
+    T rdelta = (T)p.cntrpy_f * (p.entfx_f_w[0] * (T)p.entfxctf_f + p.entfx_f_w[1] * (sqrt(u * u + v * v + w * w) + a) +
+                                  p.entfx_f_w[2] * T(0.2) * (fabs(un) + a + fabs(ubn)));
+    rdelta = fmax(rdelta, T(1.e-12));
+    if (rm123 <= rdelta && p.efx_all_f) rm123 = T(0.5) * (rm123 * rm123 + rdelta * rdelta) / rdelta;
+    if (rm4 <= rdelta) rm4 = T(0.5) * (rm4 * rm4 + rdelta * rdelta) / rdelta;
+    if (rm5 <= rdelta) rm5 = T(0.5) * (rm5 * rm5 + rdelta * rdelta) / rdelta;
   }
   if (p.eig_limit_f) {
-    if (rm123 <= p.eig_cutoff_f) rm123 = p.eig_cutoff_f;
-    if (p.eig_lim_all_f && rm4 <= p.eig_cutoff_f) rm4 = p.eig_cutoff_f;
-    if (p.eig_lim_all_f && rm5 <= p.eig_cutoff_f) rm5 = p.eig_cutoff_f;
+    if (rm123 <= (T)p.eig_cutoff_f) rm123 = (T)p.eig_cutoff_f;
+    if (p.eig_lim_all_f && rm4 <= (T)p.eig_cutoff_f) rm4 = (T)p.eig_cutoff_f;
+    if (p.eig_lim_all_f && rm5 <= (T)p.eig_cutoff_f) rm5 = (T)p.eig_cutoff_f;
   }
-  double al1 = rm123 * (drho - dp / a2);
-  double al2 = rm123 * rho;
-  double al3 = rm4 * (dp / a2 + rho * dun / a) / 2;
-  double al4 = rm5 * (dp / a2 - rho * dun / a) / 2;
-  double df0 = al1 + al3 + al4;
-  double df1 = al1 * u + al2 * (du - dun * kx) + al3 * (u + a * kx) + al4 * (u - a * kx);
-  double df2 = al1 * v + al2 * (dv - dun * ky) + al3 * (v + a * ky) + al4 * (v - a * ky);
-  double df3 = al1 * w + al2 * (dw - dun * kz) + al3 * (w + a * kz) + al4 * (w - a * kz);
-  double df4 = al1 * (q2 / 2 + tke) + al2 * (u * du + v * dv + w * dw - un * dun + dtke) + al3 * (h0 + a * un) +
+  T al1 = rm123 * (drho - dp / a2);
+  T al2 = rm123 * rho;
+  T al3 = rm4 * (dp / a2 + rho * dun / a) / 2;
+  T al4 = rm5 * (dp / a2 - rho * dun / a) / 2;
+  T df0 = al1 + al3 + al4;
+  T df1 = al1 * u + al2 * (du - dun * kx) + al3 * (u + a * kx) + al4 * (u - a * kx);
+  T df2 = al1 * v + al2 * (dv - dun * ky) + al3 * (v + a * ky) + al4 * (v - a * ky);
+  T df3 = al1 * w + al2 * (dw - dun * kz) + al3 * (w + a * kz) + al4 * (w - a * kz);
+  T df4 = al1 * (q2 / 2 + tke) + al2 * (u * du + v * dv + w * dw - un * dun + dtke) + al3 * (h0 + a * un) +
                al4 * (h0 - a * un);
   F[0] = (fr0 + fl0 - df0) / 2 * ds;
   F[1] = (fr1 + fl1 - df1) / 2 * ds;
@@ -93,51 +94,52 @@ __device__ __forceinline__ void roe_face(const Par& p, const double* qr, const d
 // Turbulence part of roflux_b (src/flux.f90:250-383): k and omega upwinded
 // with the mean-flow waves.  h0l/h0r are those of the block's last face of
 // the mean-flow loop (the reference reuses the variables), from h0tab.
-__device__ __forceinline__ void roe_turb(const Par& p, const double* qr, const double* ql, double kx, double ky,
-                                         double kz, double ds, double ubn, double h0l, double h0r, double* F) {
-  const double gamma = p.gamma;
-  double rhor = qr[0], ur = qr[1], vr = qr[2], wr = qr[3], pr = qr[4], omer = qr[6];
-  double unr = ur * kx + vr * ky + wr * kz - ubn;
-  double tker = qr[5];
-  double fr1 = rhor * unr * tker, fr2 = rhor * unr * omer;
-  double rhol = ql[0], ul = ql[1], vl = ql[2], wl = ql[3], pl = ql[4], omel = ql[6];
-  double unl = ul * kx + vl * ky + wl * kz - ubn;
-  double tkel = ql[5];
-  double fl1 = rhol * unl * tkel, fl2 = rhol * unl * omel;
-  double drho = rhor - rhol, dp = pr - pl, dun = unr - unl, dtke = tker - tkel, dome = omer - omel;
-  double rhols = sqrt(rhol), rhors = sqrt(rhor);
-  double rho = rhols * rhors;
-  double denom = rhols + rhors;
-  double u = (rhols * ul + rhors * ur) / denom;
-  double v = (rhols * vl + rhors * vr) / denom;
-  double w = (rhols * wl + rhors * wr) / denom;
-  double un = u * kx + v * ky + w * kz;
-  double h0 = (rhols * h0l + rhors * h0r) / denom;
-  double tke = (rhols * tkel + rhors * tker) / denom;
-  double ome = (rhols * omel + rhors * omer) / denom;
-  double q2 = u * u + v * v + w * w;
-  double a2 = (gamma - 1) * (h0 - q2 / 2 - tke);
-  double a = sqrt(a2);
-  double rm123 = fabs(un - ubn), rm4 = fabs(un + a - ubn), rm5 = fabs(un - a - ubn);
+template <typename T = double>
+__device__ __forceinline__ void roe_turb(const Par& p, const T* qr, const T* ql, T kx, T ky, T kz, T ds, T ubn,
+                                         T h0l, T h0r, T* F) {
+  const T gamma = (T)p.gamma;
+  T rhor = qr[0], ur = qr[1], vr = qr[2], wr = qr[3], pr = qr[4], omer = qr[6];
+  T unr = ur * kx + vr * ky + wr * kz - ubn;
+  T tker = qr[5];
+  T fr1 = rhor * unr * tker, fr2 = rhor * unr * omer;
+  T rhol = ql[0], ul = ql[1], vl = ql[2], wl = ql[3], pl = ql[4], omel = ql[6];
+  T unl = ul * kx + vl * ky + wl * kz - ubn;
+  T tkel = ql[5];
+  T fl1 = rhol * unl * tkel, fl2 = rhol * unl * omel;
+  T drho = rhor - rhol, dp = pr - pl, dun = unr - unl, dtke = tker - tkel, dome = omer - omel;
+  T rhols = sqrt(rhol), rhors = sqrt(rhor);
+  T rho = rhols * rhors;
+  T denom = rhols + rhors;
+  T u = (rhols * ul + rhors * ur) / denom;
+  T v = (rhols * vl + rhors * vr) / denom;
+  T w = (rhols * wl + rhors * wr) / denom;
+  T un = u * kx + v * ky + w * kz;
+  T h0 = (rhols * h0l + rhors * h0r) / denom;
+  T tke = (rhols * tkel + rhors * tker) / denom;
+  T ome = (rhols * omel + rhors * omer) / denom;
+  T q2 = u * u + v * v + w * w;
+  T a2 = (gamma - 1) * (h0 - q2 / 2 - tke);
+  T a = sqrt(a2);
+  T rm123 = fabs(un - ubn), rm4 = fabs(un + a - ubn), rm5 = fabs(un - a - ubn);
   if (p.efx_setup_t) {
-    double rdelta = p.cntrpy_t * (p.entfx_t_w[0] * p.entfxctf_t + p.entfx_t_w[1] * (sqrt(u * u + v * v + w * w) + a) +
-                                  p.entfx_t_w[2] * 0.2 * (fabs(un) + a + fabs(ubn)));
-    rdelta = fmax(rdelta, 1.e-12);
-    if (rm123 <= rdelta && p.efx_all_t) rm123 = 0.5 * (rm123 * rm123 + rdelta * rdelta) / rdelta;
-    if (rm4 <= rdelta) rm4 = 0.5 * (rm4 * rm4 + rdelta * rdelta) / rdelta;
-    if (rm5 <= rdelta) rm5 = 0.5 * (rm5 * rm5 + rdelta * rdelta) / rdelta;
+    T rdelta = (T)p.cntrpy_t * (p.entfx_t_w[0] * (T)p.entfxctf_t + p.entfx_t_w[1] * (sqrt(u * u + v * v + w * w) + a) +
+                                  p.entfx_t_w[2] * T(0.2) * (fabs(un) + a + fabs(ubn)));
+    rdelta = fmax(rdelta, T(1.e-12));
+    if (rm123 <= rdelta && p.efx_all_t) rm123 = T(0.5) * (rm123 * rm123 + rdelta * rdelta) / rdelta;
+    if (rm4 <= rdelta) rm4 = T(0.5) * (rm4 * rm4 + rdelta * rdelta) / rdelta;
+    if (rm5 <= rdelta) rm5 = T(0.5) * (rm5 * rm5 + rdelta * rdelta) / rdelta;
   }
   if (p.eig_limit_t) {
-    if (rm123 <= p.eig_cutoff_t) rm123 = p.eig_cutoff_t;
-    if (p.eig_lim_all_t && rm4 <= p.eig_cutoff_t) rm4 = p.eig_cutoff_t;
-    if (p.eig_lim_all_t && rm5 <= p.eig_cutoff_t) rm5 = p.eig_cutoff_t;
+    if (rm123 <= (T)p.eig_cutoff_t) rm123 = (T)p.eig_cutoff_t;
+    if (p.eig_lim_all_t && rm4 <= (T)p.eig_cutoff_t) rm4 = (T)p.eig_cutoff_t;
+    if (p.eig_lim_all_t && rm5 <= (T)p.eig_cutoff_t) rm5 = (T)p.eig_cutoff_t;
   }
-  double al1 = rm123 * (drho - dp / a2);
-  double al2 = rm123 * rho;
-  double al3 = rm4 * (dp / a2 + rho * dun / a) / 2;
-  double al4 = rm5 * (dp / a2 - rho * dun / a) / 2;
-  double df1 = (al1 + al3 + al4) * tke + al2 * dtke;
-  double df2 = (al1 + al3 + al4) * ome + al2 * dome;
+  T al1 = rm123 * (drho - dp / a2);
+  T al2 = rm123 * rho;
+  T al3 = rm4 * (dp / a2 + rho * dun / a) / 2;
+  T al4 = rm5 * (dp / a2 - rho * dun / a) / 2;
+  T df1 = (al1 + al3 + al4) * tke + al2 * dtke;
+  T df2 = (al1 + al3 + al4) * ome + al2 * dome;
   F[5] = (fr1 + fl1 - df1) / 2 * ds;
   F[6] = (fr2 + fl2 - df2) / 2 * ds;
 }
@@ -145,43 +147,43 @@ __device__ __forceinline__ void roe_turb(const Par& p, const double* qr, const d
 // ------------------------------------------------------------------------
 // Limited slopes of one variable at face a: dqp(a), dqm(a) from
 // dm1 = dq(a-1), d0 = dq(a), dp1 = dq(a+1)   (limit_i/j/k).
-__device__ __forceinline__ void limit_face(int ilim, double eps, double rkap, double dm1, double d0, double dp1,
-                                           double& dqp, double& dqm) {
+template <typename T = double>
+__device__ __forceinline__ void limit_face(int ilim, T eps, T rkap, T dm1, T d0, T dp1, T& dqp, T& dqm) {
   if (ilim == 0) {
-    double rkp = 1 + rkap, rkm = 1 - rkap;
+    T rkp = 1 + rkap, rkm = 1 - rkap;
     dqp = -(rkp * d0 + rkm * dp1) / 4;
     dqm = (rkm * dm1 + rkp * d0) / 4;
   } else if (ilim == 1) {
-    double s = (2 * dp1 * d0 + eps) / (dp1 * dp1 + d0 * d0 + eps);
-    double rkp = 1 + rkap * s, rkm = 1 - rkap * s;
+    T s = (2 * dp1 * d0 + eps) / (dp1 * dp1 + d0 * d0 + eps);
+    T rkp = 1 + rkap * s, rkm = 1 - rkap * s;
     dqp = -s * (rkp * d0 + rkm * dp1) / 4;
     s = (2 * d0 * dm1 + eps) / (d0 * d0 + dm1 * dm1 + eps);
     rkp = 1 + rkap * s;
     rkm = 1 - rkap * s;
     dqm = s * (rkm * dm1 + rkp * d0) / 4;
   } else if (ilim == 2) {
-    double rkp = 1 + rkap, rkm = 1 - rkap, b = (3.0 - rkap) / (1.0 - rkap);
+    T rkp = 1 + rkap, rkm = 1 - rkap, b = (T(3) - rkap) / (T(1) - rkap);
     // dqp(a): loop index a with dq(a), dq(a+1)
-    double dqmx = (copysign(1.0, d0) + copysign(1.0, dp1)) / 2 * fmin(fabs(d0), fabs(dp1) * b);
-    double dqpx = (copysign(1.0, dp1) + copysign(1.0, d0)) / 2 * fmin(fabs(dp1), fabs(d0) * b);
+    T dqmx = (copysign(T(1), d0) + copysign(T(1), dp1)) / 2 * fmin(fabs(d0), fabs(dp1) * b);
+    T dqpx = (copysign(T(1), dp1) + copysign(T(1), d0)) / 2 * fmin(fabs(dp1), fabs(d0) * b);
     dqp = -(rkp * dqmx + rkm * dqpx) / 4;
     // dqm(a): loop index a-1 with dq(a-1), dq(a)
-    dqmx = (copysign(1.0, dm1) + copysign(1.0, d0)) / 2 * fmin(fabs(dm1), fabs(d0) * b);
-    dqpx = (copysign(1.0, d0) + copysign(1.0, dm1)) / 2 * fmin(fabs(d0), fabs(dm1) * b);
+    dqmx = (copysign(T(1), dm1) + copysign(T(1), d0)) / 2 * fmin(fabs(dm1), fabs(d0) * b);
+    dqpx = (copysign(T(1), d0) + copysign(T(1), dm1)) / 2 * fmin(fabs(d0), fabs(dm1) * b);
     dqm = (rkm * dqmx + rkp * dqpx) / 4;
   } else if (ilim == 3) {
-    double epsq = eps * eps;
-    double d1 = dp1, d2 = d0;
-    double dqpx = (d2 * (d1 * d1 + 2 * epsq) + d1 * (2 * d2 * d2 + epsq)) / (2 * d2 * d2 - d2 * d1 + 2 * d1 * d1 + 3 * epsq);
+    T epsq = eps * eps;
+    T d1 = dp1, d2 = d0;
+    T dqpx = (d2 * (d1 * d1 + 2 * epsq) + d1 * (2 * d2 * d2 + epsq)) / (2 * d2 * d2 - d2 * d1 + 2 * d1 * d1 + 3 * epsq);
     dqp = -dqpx / 2;
     d1 = d0;
     d2 = dm1;
-    double dqmx = (d1 * (d2 * d2 + 2 * epsq) + d2 * (2 * d1 * d1 + epsq)) / (2 * d1 * d1 - d1 * d2 + 2 * d2 * d2 + 3 * epsq);
+    T dqmx = (d1 * (d2 * d2 + 2 * epsq) + d2 * (2 * d1 * d1 + epsq)) / (2 * d1 * d1 - d1 * d2 + 2 * d2 * d2 + 3 * epsq);
     dqm = dqmx / 2;
   } else {
-    double epsq = eps * eps;
-    double d1 = dp1, d2 = d0;
-    double dqmx = (d1 * (d2 * d2 + epsq) + d2 * (d1 * d1 + epsq)) / (d1 * d1 + d2 * d2 + 2 * epsq);
+    T epsq = eps * eps;
+    T d1 = dp1, d2 = d0;
+    T dqmx = (d1 * (d2 * d2 + epsq) + d2 * (d1 * d1 + epsq)) / (d1 * d1 + d2 * d2 + 2 * epsq);
     dqp = -dqmx / 2;
     d1 = d0;
     d2 = dm1;
@@ -194,10 +196,9 @@ __device__ __forceinline__ void limit_face(int ilim, double eps, double rkap, do
 // fin/fout: the line's inner (a = 1) / outer (a = L) face is a strong BC.
 // k, omega (v >= 5) use the turbulence limiter and are first order with
 // foturb (muscl_b*, src/flux.f90:1221-1416).
-template <int NV>
-__device__ __forceinline__ void muscl_face(const Par& p, int nl, int a, int L, bool fin, bool fout, const double* qm2,
-                                           const double* qm1, const double* q0, const double* qp1, double* QR,
-                                           double* QL) {
+template <int NV, typename T = double>
+__device__ __forceinline__ void muscl_face(const Par& p, int nl, int a, int L, bool fin, bool fout, const T* qm2,
+                                           const T* qm1, const T* q0, const T* qp1, T* QR, T* QL) {
   if (nl >= p.nl_crs) {
 #pragma unroll
     for (int v = 0; v < NV; v++) {
@@ -215,15 +216,15 @@ __device__ __forceinline__ void muscl_face(const Par& p, int nl, int a, int L, b
         QL[v] = qm1[v];
         continue;
       }
-      double dm1 = qm1[v] - qm2[v];
-      double d0 = q0[v] - qm1[v];
-      double dp1 = qp1[v] - q0[v];
+      T dm1 = qm1[v] - qm2[v];
+      T d0 = q0[v] - qm1[v];
+      T dp1 = qp1[v] - q0[v];
       if (sm1) dm1 = (qm1[v] - qm2[v]) * 2;
       if (s0) d0 = (q0[v] - qm1[v]) * 2;
       if (sp1) dp1 = (qp1[v] - q0[v]) * 2;
-      double dqp, dqm;
-      if (v < 5) limit_face(p.ilimf, p.epslimf, p.rkap, dm1, d0, dp1, dqp, dqm);
-      else limit_face(p.ilimt, p.epslimt, p.rkap, dm1, d0, dp1, dqp, dqm);
+      T dqp, dqm;
+      if (v < 5) limit_face<T>(p.ilimf, (T)p.epslimf, (T)p.rkap, dm1, d0, dp1, dqp, dqm);
+      else limit_face<T>(p.ilimt, (T)p.epslimt, (T)p.rkap, dm1, d0, dp1, dqp, dqm);
       QR[v] = q0[v] + dqp;
       QL[v] = qm1[v] + dqm;
     }
@@ -238,13 +239,13 @@ __device__ __forceinline__ void muscl_face(const Par& p, int nl, int a, int L, b
   }
 }
 
-__device__ __forceinline__ void face_metric(const Par& p, const double* s, double& kx, double& ky, double& kz,
-                                            double& ds, double& ubn) {
+template <typename T = double>
+__device__ __forceinline__ void face_metric(const Par& p, const T* s, T& kx, T& ky, T& kz, T& ds, T& ubn) {
   kx = s[0];
   ky = s[1];
   kz = s[2];
   ds = s[3];
-  ubn = 0.0;
+  ubn = T(0);
   if (p.moving) ubn = s[4] * kx + s[5] * ky + s[6] * kz;
 }
 
@@ -253,12 +254,14 @@ __device__ __forceinline__ void face_metric(const Par& p, const double* s, doubl
 // computes the flux of face i0+l and the cell flux difference takes the
 // neighbouring lane's face flux through a cross-lane shuffle.  NV = 7 adds
 // the k-omega transport (h0tab: stale h0l/h0r of roflux_b, see roe_turb).
-template <int NV>
+template <int NV, bool VIS, typename T>
 __global__ void __launch_bounds__(256) k_flux_i(Par p, int nl, const Blk* __restrict__ blks,
-                                                const Tile* __restrict__ tiles, const double* __restrict__ q,
-                                                const double* __restrict__ si, double* __restrict__ res,
+                                                const Tile* __restrict__ tiles, const T* __restrict__ q,
+                                                const T* __restrict__ si, double* __restrict__ res,
                                                 const unsigned char* __restrict__ sbc, const ll* __restrict__ sbcoff,
-                                                const double* __restrict__ h0tab) {
+                                                const double* __restrict__ h0tab, const T* __restrict__ mut,
+                                                const T* __restrict__ xd, const T* __restrict__ ed,
+                                                const T* __restrict__ zd) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int lane = threadIdx.x & 63;
@@ -274,64 +277,80 @@ __global__ void __launch_bounds__(256) k_flux_i(Par p, int nl, const Blk* __rest
   const int a = seg * ROWSEG + 1 + lane;   // face index
   const int L = b.imax;
   const ll n3 = n3of(b);
-  const double* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  const T* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
   double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
-  double F[NV];
+  T F[NV], Fv[NV];
 #pragma unroll
-  for (int v = 0; v < NV; v++) F[v] = 0.0;
+  for (int v = 0; v < NV; v++) F[v] = Fv[v] = 0;
   if (a <= L) {
     const unsigned char* fl = sbc + sbcoff[t.block];
     const ll pl = (ll)(b.jmax + 1) * (b.kmax + 1);
     bool fin = fl[j + (ll)(b.jmax + 1) * k] != 0;
     bool fout = fl[pl + j + (ll)(b.jmax + 1) * k] != 0;
-    double w[4][NV];
+    T w[4][NV];
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
       for (int v = 0; v < NV; v++) w[m][v] = qb[i3(b, a - 2 + m, j, k) + n3 * v];
-    double QR[NV], QL[NV];
-    muscl_face<NV>(p, nl, a, L, fin, fout, w[0], w[1], w[2], w[3], QR, QL);
+    T QR[NV], QL[NV];
+    muscl_face<NV, T>(p, nl, a, L, fin, fout, w[0], w[1], w[2], w[3], QR, QL);
     const int nh = n * p.hbmove;
-    const double* s = si + b.o2 * p.lmet * p.nsnap_met + ((ll)nh * n2of(b) + i2(b, a, j, k)) * p.lmet;
-    double kx, ky, kz, ds, ubn;
+    const T* s = si + b.o2 * p.lmet * p.nsnap_met + ((ll)nh * n2of(b) + i2(b, a, j, k)) * p.lmet;
+    T kx, ky, kz, ds, ubn;
     face_metric(p, s, kx, ky, kz, ds, ubn);
-    roe_face<NV>(p, QR, QL, kx, ky, kz, ds, ubn, F);
+    roe_face<NV, T>(p, QR, QL, kx, ky, kz, ds, ubn, F);
     if (NV > 5) {
       const double* h = h0tab + ((ll)(t.block * p.nsnap + n) * 3 + 0) * 2;
-      roe_turb(p, QR, QL, kx, ky, kz, ds, ubn, h[0], h[1], F);
+      roe_turb<T>(p, QR, QL, kx, ky, kz, ds, ubn, (T)h[0], (T)h[1], F);
     }
+    if (VIS) viscous_face<1, NV, T>(p, b, a, j, k, n, q, mut, si, xd, ed, zd, fl, Fv);
   }
-  double Fn[NV];
+  T Fn[NV], Fvn[NV];
 #pragma unroll
-  for (int v = 0; v < NV; v++) Fn[v] = __shfl_down(F[v], 1, 64);
+  for (int v = 0; v < NV; v++) {
+    Fn[v] = __shfl_down(F[v], 1, 64);
+    if (VIS) Fvn[v] = __shfl_down(Fv[v], 1, 64);
+  }
   const int c = a;   // cell c = face a, difference F(c+1) - F(c)
   if (lane < ROWSEG && c <= im) {
 #pragma unroll
-    for (int v = 0; v < NV; v++) rb[i3(b, c, j, k) + n3 * v] = (0.0 + Fn[v]) - F[v];
+    for (int v = 0; v < NV; v++) {
+      double r = (0.0 + (double)Fn[v]) - (double)F[v];
+      if (VIS) r = (r + Fvn[v]) - Fv[v];   // then the viscous resid of this direction
+      rb[i3(b, c, j, k) + n3 * v] = r;
+    }
   }
 }
 
-// j or k direction: one thread per line, marching along the line with a
-// four-cell register window; res(c) += F(c+1) - F(c) in the reference order.
-template <int DIR, int NV>
+// j or k direction: one thread per MSEG-cell segment of a line, marching
+// its faces with a four-cell register window; res(c) += F(c+1) - F(c) in
+// the reference order.  A face between two segments is computed by both
+// (identically); every cell is updated by exactly one segment.
+template <int DIR, int NV, bool VIS, typename T>
 __global__ void __launch_bounds__(256) k_flux_march(Par p, int nl, const Blk* __restrict__ blks,
-                                                    const Tile* __restrict__ tiles, const double* __restrict__ q,
-                                                    const double* __restrict__ sm, double* __restrict__ res,
+                                                    const Tile* __restrict__ tiles, const T* __restrict__ q,
+                                                    const T* __restrict__ sm, double* __restrict__ res,
                                                     const unsigned char* __restrict__ sbc,
-                                                    const ll* __restrict__ sbcoff, const double* __restrict__ h0tab) {
+                                                    const ll* __restrict__ sbcoff, const double* __restrict__ h0tab,
+                                                    const T* __restrict__ mut, const T* __restrict__ xd,
+                                                    const T* __restrict__ ed, const T* __restrict__ zd) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int item = t.base + threadIdx.x;
   const int im = b.imax - 1;
   const int t2n = DIR == 2 ? b.kmax - 1 : b.jmax - 1;
-  const int nlines = im * t2n;
-  if (item >= nlines * p.nsnap) return;
-  const int n = item / nlines;
-  const int rem = item - n * nlines;
-  const int i = rem % im + 1, t2 = rem / im + 1;
   const int L = DIR == 2 ? b.jmax : b.kmax;
+  const int nseg = (L - 1 + MSEG - 1) / MSEG;
+  const int nlines = im * t2n;
+  if (item >= nlines * nseg * p.nsnap) return;
+  const int n = item / (nlines * nseg);
+  int rem = item - n * nlines * nseg;
+  const int sg = rem / nlines;
+  rem -= sg * nlines;
+  const int i = rem % im + 1, t2 = rem / im + 1;
+  const int a0 = 1 + sg * MSEG, a1 = min(L, a0 + MSEG);   // faces a0..a1, cells a0..a1-1
   const ll n3 = n3of(b);
-  const double* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  const T* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
   double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
   const ll step = DIR == 2 ? (ll)(b.imax + 3) : (ll)(b.imax + 3) * (b.jmax + 3);
   // point index of a = 0 on this line
@@ -350,37 +369,46 @@ __global__ void __launch_bounds__(256) k_flux_march(Par p, int nl, const Blk* __
     fout = fl[off + pl + i + (ll)(b.imax + 1) * t2] != 0;
   }
   const int nh = n * p.hbmove;
-  const double* sb = sm + b.o2 * p.lmet * p.nsnap_met + (ll)nh * n2of(b) * p.lmet;
-  double h0l = 0.0, h0r = 0.0;
+  const T* sb = sm + b.o2 * p.lmet * p.nsnap_met + (ll)nh * n2of(b) * p.lmet;
+  T h0l = 0, h0r = 0;
   if (NV > 5) {
     const double* h = h0tab + ((ll)(t.block * p.nsnap + n) * 3 + (DIR - 1)) * 2;
-    h0l = h[0];
-    h0r = h[1];
+    h0l = (T)h[0];
+    h0r = (T)h[1];
   }
-  double w0[NV], w1[NV], w2[NV], w3[NV];
+  T w0[NV], w1[NV], w2[NV], w3[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) {
-    w0[v] = qb[base - step + n3 * v];
-    w1[v] = qb[base + n3 * v];
-    w2[v] = qb[base + step + n3 * v];
-    w3[v] = qb[base + 2 * step + n3 * v];
+    w0[v] = qb[base + (ll)(a0 - 2) * step + n3 * v];
+    w1[v] = qb[base + (ll)(a0 - 1) * step + n3 * v];
+    w2[v] = qb[base + (ll)a0 * step + n3 * v];
+    w3[v] = qb[base + (ll)(a0 + 1) * step + n3 * v];
   }
-  double Fp[NV];
-  for (int a = 1; a <= L; a++) {
-    double QR[NV], QL[NV], F[NV];
-    muscl_face<NV>(p, nl, a, L, fin, fout, w0, w1, w2, w3, QR, QL);
+  T Fp[NV], Fvp[NV];
+  for (int a = a0; a <= a1; a++) {
+    T QR[NV], QL[NV], F[NV], Fv[NV];
+    muscl_face<NV, T>(p, nl, a, L, fin, fout, w0, w1, w2, w3, QR, QL);
     const ll sidx = DIR == 2 ? i2(b, i, a, t2) : i2(b, i, t2, a);
-    double kx, ky, kz, ds, ubn;
+    T kx, ky, kz, ds, ubn;
     face_metric(p, sb + sidx * p.lmet, kx, ky, kz, ds, ubn);
-    roe_face<NV>(p, QR, QL, kx, ky, kz, ds, ubn, F);
-    if (NV > 5) roe_turb(p, QR, QL, kx, ky, kz, ds, ubn, h0l, h0r, F);
-    if (a >= 2) {
+    roe_face<NV, T>(p, QR, QL, kx, ky, kz, ds, ubn, F);
+    if (NV > 5) roe_turb<T>(p, QR, QL, kx, ky, kz, ds, ubn, h0l, h0r, F);
+    if (VIS) {
+      if (DIR == 2) viscous_face<2, NV, T>(p, b, i, a, t2, n, q, mut, sm, xd, ed, zd, fl, Fv);
+      else viscous_face<3, NV, T>(p, b, i, t2, a, n, q, mut, sm, xd, ed, zd, fl, Fv);
+    }
+    if (a > a0) {
       const ll c = base + (ll)(a - 1) * step;
 #pragma unroll
-      for (int v = 0; v < NV; v++) rb[c + n3 * v] = (rb[c + n3 * v] + F[v]) - Fp[v];
+      for (int v = 0; v < NV; v++) {
+        double r = (rb[c + n3 * v] + F[v]) - Fp[v];
+        if (VIS) r = (r + Fv[v]) - Fvp[v];   // then the viscous resid of this direction
+        rb[c + n3 * v] = r;
+      }
     }
 #pragma unroll
     for (int v = 0; v < NV; v++) {
+      if (VIS) Fvp[v] = Fv[v];
       Fp[v] = F[v];
       w0[v] = w1[v];
       w1[v] = w2[v];
@@ -397,9 +425,10 @@ __global__ void __launch_bounds__(256) k_flux_march(Par p, int nl, const Blk* __
 // direction: the values roflux_b's k-omega loop finds left over from the
 // mean-flow loop (src/flux.f90:121-141, 287-302).  One thread per
 // (block, snapshot, direction).
+template <typename T>
 __global__ void __launch_bounds__(64) k_h0_last(Par p, int nl, int nblocks, const Blk* __restrict__ blks,
-                                                const double* __restrict__ q, const double* __restrict__ si,
-                                                const double* __restrict__ sj, const double* __restrict__ sk,
+                                                const T* __restrict__ q, const T* __restrict__ si,
+                                                const T* __restrict__ sj, const T* __restrict__ sk,
                                                 const unsigned char* __restrict__ sbc, const ll* __restrict__ sbcoff,
                                                 double* __restrict__ h0tab) {
   const int item = blockIdx.x * blockDim.x + threadIdx.x;
@@ -412,7 +441,7 @@ __global__ void __launch_bounds__(64) k_h0_last(Par p, int nl, int nblocks, cons
   // last face: a = L along dir, transverse indices at their maxima
   const int i = dir == 1 ? L : b.imax - 1, j = dir == 2 ? L : b.jmax - 1, k = dir == 3 ? L : b.kmax - 1;
   const ll n3 = n3of(b);
-  const double* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  const T* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
   const unsigned char* fl = sbc + sbcoff[blk];
   bool fin, fout;
   if (dir == 1) {
@@ -430,19 +459,19 @@ __global__ void __launch_bounds__(64) k_h0_last(Par p, int nl, int nblocks, cons
     fin = fl[off + i + (ll)(b.imax + 1) * j] != 0;
     fout = fl[off + pl + i + (ll)(b.imax + 1) * j] != 0;
   }
-  double w[4][7];
+  T w[4][7];
   for (int m = 0; m < 4; m++) {
     const int di = dir == 1 ? m - 2 : 0, dj = dir == 2 ? m - 2 : 0, dk = dir == 3 ? m - 2 : 0;
     for (int v = 0; v < 7; v++) w[m][v] = qb[i3(b, i + di, j + dj, k + dk) + n3 * v];
   }
-  double QR[7], QL[7], F[7], h[2];
-  muscl_face<7>(p, nl, L, L, fin, fout, w[0], w[1], w[2], w[3], QR, QL);
-  const double* sm = dir == 1 ? si : (dir == 2 ? sj : sk);
+  T QR[7], QL[7], F[7], h[2];
+  muscl_face<7, T>(p, nl, L, L, fin, fout, w[0], w[1], w[2], w[3], QR, QL);
+  const T* sm = dir == 1 ? si : (dir == 2 ? sj : sk);
   const int nh = n * p.hbmove;
-  const double* s = sm + b.o2 * p.lmet * p.nsnap_met + ((ll)nh * n2of(b) + i2(b, i, j, k)) * p.lmet;
-  double kx, ky, kz, ds, ubn;
+  const T* s = sm + b.o2 * p.lmet * p.nsnap_met + ((ll)nh * n2of(b) + i2(b, i, j, k)) * p.lmet;
+  T kx, ky, kz, ds, ubn;
   face_metric(p, s, kx, ky, kz, ds, ubn);
-  roe_face<7>(p, QR, QL, kx, ky, kz, ds, ubn, F, h);
+  roe_face<7, T>(p, QR, QL, kx, ky, kz, ds, ubn, F, h);
   h0tab[(ll)item * 2] = h[0];
   h0tab[(ll)item * 2 + 1] = h[1];
 }
@@ -464,15 +493,16 @@ __global__ void __launch_bounds__(64) k_h0_last(Par p, int nl, int nblocks, cons
 constexpr int FTI = 31, FTJ = 7, KSEG = 32;
 constexpr int PW = FTI + 4, PH = FTJ + 4;
 
+template <typename T>
 __global__ void __launch_bounds__(256) k_residual_fused(Par p, int nl, const Blk* __restrict__ blks,
-                                                        const Tile* __restrict__ tiles, const double* __restrict__ q,
-                                                        const double* __restrict__ si, const double* __restrict__ sj,
-                                                        const double* __restrict__ sk, double* __restrict__ res,
+                                                        const Tile* __restrict__ tiles, const T* __restrict__ q,
+                                                        const T* __restrict__ si, const T* __restrict__ sj,
+                                                        const T* __restrict__ sk, double* __restrict__ res,
                                                         const unsigned char* __restrict__ sbc,
                                                         const ll* __restrict__ sbcoff) {
-  __shared__ double pl[5][PH][PW];
-  __shared__ double fi[5][FTJ][FTI + 1];
-  __shared__ double fj[5][FTJ + 1][FTI];
+  __shared__ T pl[5][PH][PW];
+  __shared__ T fi[5][FTJ][FTI + 1];
+  __shared__ T fj[5][FTJ + 1][FTI];
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -487,7 +517,7 @@ __global__ void __launch_bounds__(256) k_residual_fused(Par p, int nl, const Blk
   const int i0 = 1 + tx_i * FTI, j0 = 1 + ty_i * FTJ;
   const int ks = 1 + ks_i * KSEG, ke = min(km, ks + KSEG - 1);
   const ll n3 = n3of(b);
-  const double* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  const T* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
   double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
   const int nh = n * p.hbmove;
   const ll mo = b.o2 * p.lmet * p.nsnap_met + (ll)nh * n2of(b) * p.lmet;
@@ -499,7 +529,7 @@ __global__ void __launch_bounds__(256) k_residual_fused(Par p, int nl, const Blk
   const bool cell = tx < FTI && ty < FTJ && ci <= im && cj <= jm;
   const ll kstep = (ll)(b.imax + 3) * (b.jmax + 3);
   // k window of this thread's column: w0..w3 = q(k-1..k+2) at plane k
-  double w0[5], w1[5], w2[5], w3[5], Fkp[5];
+  T w0[5], w1[5], w2[5], w3[5], Fkp[5];
   bool kfin = false, kfout = false;
   if (cell) {
     kfin = fl[offK + ci + (ll)(b.imax + 1) * cj] != 0;
@@ -507,17 +537,17 @@ __global__ void __launch_bounds__(256) k_residual_fused(Par p, int nl, const Blk
     const ll c0 = i3(b, ci, cj, ks);
 #pragma unroll
     for (int v = 0; v < 5; v++) {
-      const double* qv = qb + n3 * v;
-      const double a_m2 = qv[c0 - 2 * kstep], a_m1 = qv[c0 - kstep], a_0 = qv[c0], a_p1 = qv[c0 + kstep];
+      const T* qv = qb + n3 * v;
+      const T a_m2 = qv[c0 - 2 * kstep], a_m1 = qv[c0 - kstep], a_0 = qv[c0], a_p1 = qv[c0 + kstep];
       w0[v] = a_m2;   // temporarily q(ks-2..ks+1) for face ks
       w1[v] = a_m1;
       w2[v] = a_0;
       w3[v] = a_p1;
     }
-    double QR[5], QL[5], kx, ky, kz, ds, ubn;
-    muscl_face<5>(p, nl, ks, b.kmax, kfin, kfout, w0, w1, w2, w3, QR, QL);
-    face_metric(p, sk + mo + i2(b, ci, cj, ks) * p.lmet, kx, ky, kz, ds, ubn);
-    roe_face<5>(p, QR, QL, kx, ky, kz, ds, ubn, Fkp);
+    T QR[5], QL[5], kx, ky, kz, ds, ubn;
+    muscl_face<5, T>(p, nl, ks, b.kmax, kfin, kfout, w0, w1, w2, w3, QR, QL);
+    face_metric<T>(p, sk + mo + i2(b, ci, cj, ks) * p.lmet, kx, ky, kz, ds, ubn);
+    roe_face<5, T>(p, QR, QL, kx, ky, kz, ds, ubn, Fkp);
     // shift to the window of plane ks: q(ks-1..ks+2)
 #pragma unroll
     for (int v = 0; v < 5; v++) {
@@ -552,7 +582,7 @@ __global__ void __launch_bounds__(256) k_residual_fused(Par p, int nl, const Blk
       const int a = ci;
       const bool fin = fl[cj + (ll)(b.jmax + 1) * k] != 0;
       const bool fout = fl[plI + cj + (ll)(b.jmax + 1) * k] != 0;
-      double m2[5], m1[5], z0[5], p1[5], QR[5], QL[5], F[5], kx, ky, kz, ds, ubn;
+      T m2[5], m1[5], z0[5], p1[5], QR[5], QL[5], F[5], kx, ky, kz, ds, ubn;
 #pragma unroll
       for (int v = 0; v < 5; v++) {
         m2[v] = pl[v][ty + 2][tx];
@@ -560,9 +590,9 @@ __global__ void __launch_bounds__(256) k_residual_fused(Par p, int nl, const Blk
         z0[v] = pl[v][ty + 2][tx + 2];
         p1[v] = pl[v][ty + 2][tx + 3];
       }
-      muscl_face<5>(p, nl, a, b.imax, fin, fout, m2, m1, z0, p1, QR, QL);
-      face_metric(p, si + mo + i2(b, a, cj, k) * p.lmet, kx, ky, kz, ds, ubn);
-      roe_face<5>(p, QR, QL, kx, ky, kz, ds, ubn, F);
+      muscl_face<5, T>(p, nl, a, b.imax, fin, fout, m2, m1, z0, p1, QR, QL);
+      face_metric<T>(p, si + mo + i2(b, a, cj, k) * p.lmet, kx, ky, kz, ds, ubn);
+      roe_face<5, T>(p, QR, QL, kx, ky, kz, ds, ubn, F);
 #pragma unroll
       for (int v = 0; v < 5; v++) fi[v][ty][tx] = F[v];
     }
@@ -570,7 +600,7 @@ __global__ void __launch_bounds__(256) k_residual_fused(Par p, int nl, const Blk
       const int a = cj;
       const bool fin = fl[offJ + ci + (ll)(b.imax + 1) * k] != 0;
       const bool fout = fl[offJ + plJ + ci + (ll)(b.imax + 1) * k] != 0;
-      double m2[5], m1[5], z0[5], p1[5], QR[5], QL[5], F[5], kx, ky, kz, ds, ubn;
+      T m2[5], m1[5], z0[5], p1[5], QR[5], QL[5], F[5], kx, ky, kz, ds, ubn;
 #pragma unroll
       for (int v = 0; v < 5; v++) {
         m2[v] = pl[v][ty][tx + 2];
@@ -578,18 +608,18 @@ __global__ void __launch_bounds__(256) k_residual_fused(Par p, int nl, const Blk
         z0[v] = pl[v][ty + 2][tx + 2];
         p1[v] = pl[v][ty + 3][tx + 2];
       }
-      muscl_face<5>(p, nl, a, b.jmax, fin, fout, m2, m1, z0, p1, QR, QL);
-      face_metric(p, sj + mo + i2(b, ci, a, k) * p.lmet, kx, ky, kz, ds, ubn);
-      roe_face<5>(p, QR, QL, kx, ky, kz, ds, ubn, F);
+      muscl_face<5, T>(p, nl, a, b.jmax, fin, fout, m2, m1, z0, p1, QR, QL);
+      face_metric<T>(p, sj + mo + i2(b, ci, a, k) * p.lmet, kx, ky, kz, ds, ubn);
+      roe_face<5, T>(p, QR, QL, kx, ky, kz, ds, ubn, F);
 #pragma unroll
       for (int v = 0; v < 5; v++) fj[v][ty][tx] = F[v];
     }
-    double Fkn[5];
+    T Fkn[5];
     if (cell) {   // k-face k+1 from the register window
-      double QR[5], QL[5], kx, ky, kz, ds, ubn;
-      muscl_face<5>(p, nl, k + 1, b.kmax, kfin, kfout, w0, w1, w2, w3, QR, QL);
-      face_metric(p, sk + mo + i2(b, ci, cj, k + 1) * p.lmet, kx, ky, kz, ds, ubn);
-      roe_face<5>(p, QR, QL, kx, ky, kz, ds, ubn, Fkn);
+      T QR[5], QL[5], kx, ky, kz, ds, ubn;
+      muscl_face<5, T>(p, nl, k + 1, b.kmax, kfin, kfout, w0, w1, w2, w3, QR, QL);
+      face_metric<T>(p, sk + mo + i2(b, ci, cj, k + 1) * p.lmet, kx, ky, kz, ds, ubn);
+      roe_face<5, T>(p, QR, QL, kx, ky, kz, ds, ubn, Fkn);
     }
     __syncthreads();
     // (c) accumulate in the reference order and store
@@ -597,10 +627,10 @@ __global__ void __launch_bounds__(256) k_residual_fused(Par p, int nl, const Blk
       const ll x = i3(b, ci, cj, k);
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        double r = (0.0 + fi[v][ty][tx + 1]) - fi[v][ty][tx];
+        T r = (T(0) + fi[v][ty][tx + 1]) - fi[v][ty][tx];
         r = (r + fj[v][ty + 1][tx]) - fj[v][ty][tx];
         r = (r + Fkn[v]) - Fkp[v];
-        rb[n3 * v + x] = r;
+        rb[n3 * v + x] = (double)r;
         Fkp[v] = Fkn[v];
       }
       if (k < ke) {
@@ -970,6 +1000,111 @@ __global__ void __launch_bounds__(64) k_cirs_i(Par p, const Blk* __restrict__ bl
   }
 }
 
+// CIRS i-sweep with one thread per (line, variable): a 256-thread
+// workgroup takes 256 / NV lines, NV = npde variables each, so k and omega
+// are smoothed in the same launch as the mean flow.  Mean-flow variables
+// use dtvol and beta (or the constant coefficient of irsop 1), k and omega
+// dtvolt and tbeta (irsop 1: the same constant).  Each thread walks its
+// line in chunks of CI cells whose loads are issued before the recurrence
+// consumes them.  The lanes of one line and group compute the same Thomas
+// multipliers and store them to the same address of rsc / rsct for the
+// back substitution.  Arithmetic as in k_cirs_i (tridi, src/cosa.f90:
+// 2418-2443).
+template <int NV>
+__global__ void __launch_bounds__(256) k_cirs_iv(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
+                                                 double* __restrict__ res, const double* __restrict__ dtvol,
+                                                 const double* __restrict__ beta, const double* __restrict__ dtvolt,
+                                                 const double* __restrict__ tbeta, double* __restrict__ rsc,
+                                                 double* __restrict__ rsct, double epsc) {
+  constexpr int LPT = 256 / NV;
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int ln = threadIdx.x / NV, v = threadIdx.x - (threadIdx.x / NV) * NV;
+  if (ln >= LPT) return;
+  const int jm = b.jmax - 1, km = b.kmax - 1;
+  const int nlines = jm * km;
+  const int gl = t.base + ln;
+  if (gl >= nlines * p.nsnap) return;
+  const int n = gl / nlines, rem = gl - (gl / nlines) * nlines;
+  const int j = rem % jm + 1, k = rem / jm + 1;
+  const ll n3 = n3of(b), n1 = n1of(b);
+  const bool turb = v >= 5;
+  const bool cconst = p.irsop == 1;
+  double* x = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n + n3 * v + i3(b, 0, j, k);
+  const ll o1 = b.o1 * p.nsnap + n1 * (ll)n + i1(b, 0, j, k);
+  const double* dt = (turb ? dtvolt : dtvol) + o1;
+  const double* be = cconst ? nullptr : (turb ? tbeta : beta) + o1;
+  double* rr = (turb ? rsct : rsc) + b.o3 * p.nsnap + n3 * (ll)n + i3(b, 0, j, k);
+  const int L = b.imax - 1;
+  double tp = 0.0, rp = 0.0;
+  for (int a0 = 1; a0 <= L; a0 += CI) {
+    double xv[CI], dv[CI], ev[CI], rv[CI];
+#pragma unroll
+    for (int u = 0; u < CI; u++) {
+      const int a = a0 + u;
+      if (a <= L) {
+        xv[u] = x[a];
+        dv[u] = dt[a];
+        ev[u] = cconst ? epsc : be[a];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CI; u++) {
+      const int a = a0 + u;
+      if (a > L) break;
+      const double eps = ev[u];
+      const double ca = a == 1 ? 0.0 : -eps;
+      const double cb = 1 + 2 * eps;
+      const double cc = a == L ? 0.0 : -eps;
+      double r;
+      if (a == 1) {
+        r = cc / cb;
+        tp = (xv[u] * dv[u]) / cb;
+      } else {
+        r = cc / (cb - rp * ca);
+        tp = (xv[u] * dv[u] - tp * ca) / (cb - rp * ca);
+      }
+      rp = r;
+      rv[u] = r;
+      xv[u] = tp;
+    }
+#pragma unroll
+    for (int u = 0; u < CI; u++) {
+      const int a = a0 + u;
+      if (a <= L) {
+        x[a] = xv[u];
+        rr[a] = rv[u];
+      }
+    }
+  }
+  double xn = 0.0;
+  const int nchunk = (L + CI - 1) / CI;
+  for (int c = nchunk - 1; c >= 0; c--) {
+    const int a0 = 1 + c * CI;
+    double xv[CI], rv[CI];
+#pragma unroll
+    for (int u = 0; u < CI; u++) {
+      const int a = a0 + u;
+      if (a <= L) {
+        xv[u] = x[a];
+        rv[u] = rr[a];
+      }
+    }
+#pragma unroll
+    for (int u = CI - 1; u >= 0; u--) {
+      const int a = a0 + u;
+      if (a > L) continue;
+      xn = a == L ? xv[u] : xv[u] - xn * rv[u];
+      xv[u] = xn;
+    }
+#pragma unroll
+    for (int u = 0; u < CI; u++) {
+      const int a = a0 + u;
+      if (a <= L) x[a] = xv[u];
+    }
+  }
+}
+
 // ------------------------------------------------------------------------
 // launchers
 
@@ -989,66 +1124,104 @@ hipError_t launch_residual(Solve& s, Level& L, const double* q, double* res) {
   hipError_t e = launch_zero_res_shell(s, L, res);
   if (e != hipSuccess) return e;
   if (s.par.viscous || s.par.npde != 5) return launch_residual_viscous(s, L, q, res);
+  if (s.fp32() && q == L.f[COSAHIP_Q] && L.q32) {
+    // FP32 residual: fluxes, limiters and differences in single precision
+    // from the FP32 shadows of q (kept by cons2prim and the halo pass) and
+    // of the metrics (converted once per upload); res is stored in FP64.
+    if (!L.m32_ok) {
+      const ll nm = L.n2tot * s.par.lmet * s.par.nsnap_met;
+      for (int d = 0; d < 3; d++) {
+        e = launch_to_f32(s, L.f[COSAHIP_SI + d], L.m32[d], nm);
+        if (e != hipSuccess) return e;
+      }
+      L.m32_ok = true;
+    }
+    if (L.ntile[TK_FUSED])
+      hipLaunchKernelGGL(k_residual_fused<float>, grid1(L.ntile[TK_FUSED]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
+                         L.tile[TK_FUSED], L.q32, L.m32[0], L.m32[1], L.m32[2], res, L.sbc, L.sbc_doff);
+    return hipGetLastError();
+  }
   if (!s.split_residual) {
     if (L.ntile[TK_FUSED])
-      hipLaunchKernelGGL(k_residual_fused, grid1(L.ntile[TK_FUSED]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
+      hipLaunchKernelGGL(k_residual_fused<double>, grid1(L.ntile[TK_FUSED]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
                          L.tile[TK_FUSED], q, L.f[COSAHIP_SI], L.f[COSAHIP_SJ], L.f[COSAHIP_SK], res, L.sbc,
                          L.sbc_doff);
     return hipGetLastError();
   }
   if (L.ntile[TK_ROWS_I])
-    hipLaunchKernelGGL(k_flux_i<5>, grid1(L.ntile[TK_ROWS_I]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
-                       L.tile[TK_ROWS_I], q, L.f[COSAHIP_SI], res, L.sbc, L.sbc_doff, nullptr);
-  if (L.ntile[TK_LINES_J])
-    hipLaunchKernelGGL((k_flux_march<2, 5>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.nl, L.dblk,
-                       L.tile[TK_LINES_J], q, L.f[COSAHIP_SJ], res, L.sbc, L.sbc_doff, nullptr);
-  if (L.ntile[TK_LINES_K])
-    hipLaunchKernelGGL((k_flux_march<3, 5>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.nl, L.dblk,
-                       L.tile[TK_LINES_K], q, L.f[COSAHIP_SK], res, L.sbc, L.sbc_doff, nullptr);
+    hipLaunchKernelGGL((k_flux_i<5, false, double>), grid1(L.ntile[TK_ROWS_I]), dim3(256), 0, s.stream, s.par, L.nl,
+                       L.dblk, L.tile[TK_ROWS_I], q, L.f[COSAHIP_SI], res, L.sbc, L.sbc_doff, nullptr, nullptr, nullptr,
+                       nullptr, nullptr);
+  if (L.ntile[TK_SEGS_J])
+    hipLaunchKernelGGL((k_flux_march<2, 5, false, double>), grid1(L.ntile[TK_SEGS_J]), dim3(TS), 0, s.stream, s.par, L.nl,
+                       L.dblk, L.tile[TK_SEGS_J], q, L.f[COSAHIP_SJ], res, L.sbc, L.sbc_doff, nullptr, nullptr,
+                       nullptr, nullptr, nullptr);
+  if (L.ntile[TK_SEGS_K])
+    hipLaunchKernelGGL((k_flux_march<3, 5, false, double>), grid1(L.ntile[TK_SEGS_K]), dim3(TS), 0, s.stream, s.par, L.nl,
+                       L.dblk, L.tile[TK_SEGS_K], q, L.f[COSAHIP_SK], res, L.sbc, L.sbc_doff, nullptr, nullptr,
+                       nullptr, nullptr, nullptr);
   return hipGetLastError();
 }
 
-// Inviscid direction idir of the viscous / k-omega residual (one launch).
-hipError_t launch_flux_dir(Solve& s, Level& L, int idir, const double* q, double* res) {
-  const bool t = s.par.npde == 7;
-  if (idir == 1 && L.ntile[TK_ROWS_I]) {
-    if (t)
-      hipLaunchKernelGGL(k_flux_i<7>, grid1(L.ntile[TK_ROWS_I]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
-                         L.tile[TK_ROWS_I], q, L.f[COSAHIP_SI], res, L.sbc, L.sbc_doff, L.h0tab);
-    else
-      hipLaunchKernelGGL(k_flux_i<5>, grid1(L.ntile[TK_ROWS_I]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
-                         L.tile[TK_ROWS_I], q, L.f[COSAHIP_SI], res, L.sbc, L.sbc_doff, nullptr);
-  } else if (idir == 2 && L.ntile[TK_LINES_J]) {
-    if (t)
-      hipLaunchKernelGGL((k_flux_march<2, 7>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.nl,
-                         L.dblk, L.tile[TK_LINES_J], q, L.f[COSAHIP_SJ], res, L.sbc, L.sbc_doff, L.h0tab);
-    else
-      hipLaunchKernelGGL((k_flux_march<2, 5>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.nl,
-                         L.dblk, L.tile[TK_LINES_J], q, L.f[COSAHIP_SJ], res, L.sbc, L.sbc_doff, nullptr);
-  } else if (idir == 3 && L.ntile[TK_LINES_K]) {
-    if (t)
-      hipLaunchKernelGGL((k_flux_march<3, 7>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.nl,
-                         L.dblk, L.tile[TK_LINES_K], q, L.f[COSAHIP_SK], res, L.sbc, L.sbc_doff, L.h0tab);
-    else
-      hipLaunchKernelGGL((k_flux_march<3, 5>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.nl,
-                         L.dblk, L.tile[TK_LINES_K], q, L.f[COSAHIP_SK], res, L.sbc, L.sbc_doff, nullptr);
-  }
-  return hipGetLastError();
+// Direction idir of the viscous / k-omega residual: inviscid Roe/MUSCL and,
+// viscous, the face diffusion fluxes in the same line march (one launch).
+template <int NV, bool VIS, typename T>
+static void flux_dir_t(Solve& s, Level& L, int idir, const ResIn<T>& in, double* res) {
+  const T* xd = in.xd[3 * (idir - 1)];
+  const T* ed = in.xd[3 * (idir - 1) + 1];
+  const T* zd = in.xd[3 * (idir - 1) + 2];
+  if (idir == 1 && L.ntile[TK_ROWS_I])
+    hipLaunchKernelGGL((k_flux_i<NV, VIS, T>), grid1(L.ntile[TK_ROWS_I]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
+                       L.tile[TK_ROWS_I], in.q, in.s[0], res, L.sbc, L.sbc_doff, L.h0tab, in.mut, xd, ed, zd);
+  else if (idir == 2 && L.ntile[TK_SEGS_J])
+    hipLaunchKernelGGL((k_flux_march<2, NV, VIS, T>), grid1(L.ntile[TK_SEGS_J]), dim3(TS), 0, s.stream, s.par, L.nl,
+                       L.dblk, L.tile[TK_SEGS_J], in.q, in.s[1], res, L.sbc, L.sbc_doff, L.h0tab, in.mut, xd, ed,
+                       zd);
+  else if (idir == 3 && L.ntile[TK_SEGS_K])
+    hipLaunchKernelGGL((k_flux_march<3, NV, VIS, T>), grid1(L.ntile[TK_SEGS_K]), dim3(TS), 0, s.stream, s.par, L.nl,
+                       L.dblk, L.tile[TK_SEGS_K], in.q, in.s[2], res, L.sbc, L.sbc_doff, L.h0tab, in.mut, xd, ed,
+                       zd);
 }
 
-hipError_t launch_h0_last(Solve& s, Level& L, const double* q) {
+template <typename T>
+hipError_t launch_flux_dir(Solve& s, Level& L, int idir, const ResIn<T>& in, double* res) {
+  const bool t = s.par.npde == 7, v = s.par.viscous != 0;
+  if (t && v) flux_dir_t<7, true, T>(s, L, idir, in, res);
+  else if (t) flux_dir_t<7, false, T>(s, L, idir, in, res);
+  else if (v) flux_dir_t<5, true, T>(s, L, idir, in, res);
+  else flux_dir_t<5, false, T>(s, L, idir, in, res);
+  return hipGetLastError();
+}
+template hipError_t launch_flux_dir<double>(Solve&, Level&, int, const ResIn<double>&, double*);
+template hipError_t launch_flux_dir<float>(Solve&, Level&, int, const ResIn<float>&, double*);
+
+template <typename T>
+hipError_t launch_h0_last(Solve& s, Level& L, const ResIn<T>& in) {
   const int n = L.nblocks * s.par.nsnap * 3;
-  hipLaunchKernelGGL(k_h0_last, dim3((n + 63) / 64), dim3(64), 0, s.stream, s.par, L.nl, L.nblocks, L.dblk, q,
-                     L.f[COSAHIP_SI], L.f[COSAHIP_SJ], L.f[COSAHIP_SK], L.sbc, L.sbc_doff, L.h0tab);
+  hipLaunchKernelGGL(k_h0_last<T>, dim3((n + 63) / 64), dim3(64), 0, s.stream, s.par, L.nl, L.nblocks, L.dblk, in.q,
+                     in.s[0], in.s[1], in.s[2], L.sbc, L.sbc_doff, L.h0tab);
   return hipGetLastError();
 }
+template hipError_t launch_h0_last<double>(Solve&, Level&, const ResIn<double>&);
+template hipError_t launch_h0_last<float>(Solve&, Level&, const ResIn<float>&);
 
 hipError_t launch_cirs(Solve& s, Level& L) {
   if (s.par.irsop < 1 || s.par.irsop > 3) return hipSuccess;
   double* res = L.f[COSAHIP_RES];
   const double* dt = L.f[COSAHIP_DTVOL];
   const double epsc = 0.25 * (s.par.rat * s.par.rat - 1.0);
-  if (L.ntile[TK_LINES_I])
+  const bool iv = !s.cirs_i_lds;
+  if (iv && L.ntile[TK_LINES_IV]) {
+    if (s.par.npde == 7)
+      hipLaunchKernelGGL(k_cirs_iv<7>, grid1(L.ntile[TK_LINES_IV]), dim3(256), 0, s.stream, s.par, L.dblk,
+                         L.tile[TK_LINES_IV], res, dt, L.f[COSAHIP_BETAXI], L.f[COSAHIP_DTVOLT], L.tbeta[0], L.cirs_r,
+                         L.cirs_rt, epsc);
+    else
+      hipLaunchKernelGGL(k_cirs_iv<5>, grid1(L.ntile[TK_LINES_IV]), dim3(256), 0, s.stream, s.par, L.dblk,
+                         L.tile[TK_LINES_IV], res, dt, L.f[COSAHIP_BETAXI], nullptr, nullptr, L.cirs_r, nullptr,
+                         epsc);
+  }
+  if (!iv && L.ntile[TK_LINES_I])
     hipLaunchKernelGGL(k_cirs_i<5>, grid1(L.ntile[TK_LINES_I]), dim3(64), 0, s.stream, s.par, L.dblk,
                        L.tile[TK_LINES_I], res, dt, L.f[COSAHIP_BETAXI], L.cirs_r, 0, epsc);
   if (L.ntile[TK_LINES_J])
@@ -1061,7 +1234,7 @@ hipError_t launch_cirs(Solve& s, Level& L) {
     // k, omega: scaled by dtvolt; irsop 1 uses the flow's constant
     // coefficient, irsop 2/3 the turbulence coefficients tbeta*
     const double* dtt = L.f[COSAHIP_DTVOLT];
-    if (L.ntile[TK_LINES_I])
+    if (!iv && L.ntile[TK_LINES_I])
       hipLaunchKernelGGL(k_cirs_i<2>, grid1(L.ntile[TK_LINES_I]), dim3(64), 0, s.stream, s.par, L.dblk,
                          L.tile[TK_LINES_I], res, dtt, L.tbeta[0], L.cirs_r, 5, epsc);
     if (L.ntile[TK_LINES_J])

@@ -14,31 +14,9 @@
 //   lim_res_ome     : src/update.f90:2527-2835
 //   lim_prod_mg     : src/src.f90:674-757
 #include "kernels.h"
+#include "viscous.h"
 
 namespace cosahip {
-
-__device__ __forceinline__ double suth(const Par& p, double t) {
-  return (p.stemp + 1) / (t + p.stemp) * pow(t, 1.5);
-}
-
-// strong-BC flag byte of the face of direction DIR at position a (1 or L)
-// and transverse (t1, t2) (lower-numbered transverse direction first)
-template <int DIR>
-__device__ __forceinline__ unsigned char face_flag(const Blk& b, const unsigned char* fl, int outer, int t1, int t2) {
-  if (DIR == 1) return fl[outer * (ll)(b.jmax + 1) * (b.kmax + 1) + t1 + (ll)(b.jmax + 1) * t2];
-  if (DIR == 2)
-    return fl[2 * (ll)(b.jmax + 1) * (b.kmax + 1) + outer * (ll)(b.imax + 1) * (b.kmax + 1) + t1 +
-              (ll)(b.imax + 1) * t2];
-  return fl[2 * (ll)(b.jmax + 1) * (b.kmax + 1) + 2 * (ll)(b.imax + 1) * (b.kmax + 1) +
-            outer * (ll)(b.imax + 1) * (b.jmax + 1) + t1 + (ll)(b.imax + 1) * t2];
-}
-
-// metric-derivative triple (3, imax, jmax, kmax, 0:2N*hbmove) at face (i,j,k)
-__device__ __forceinline__ const double* mder(const Par& p, const Blk& b, const double* a, int i, int j, int k,
-                                              int nh) {
-  return a + b.o0 * 3 * p.nsnap_met +
-         3 * ((ll)(i - 1) + (ll)b.imax * ((ll)(j - 1) + (ll)b.jmax * ((ll)(k - 1) + (ll)b.kmax * nh)));
-}
 
 // ------------------------------------------------------------------------
 // Viscous fluxes of the faces of direction DIR, one thread per face:
@@ -46,13 +24,13 @@ __device__ __forceinline__ const double* mder(const Par& p, const Blk& b, const 
 // stresses (rtst, k-omega) and the flux (vflux) into vflux(imax,jmax,kmax,
 // npde,0:2N).  Wall-like faces (strong-BC flag) take the halo value and
 // one-sided differences as bq_face / bq_der do.
-template <int DIR, int NV>
+template <int DIR, int NV, typename T>
 __global__ void __launch_bounds__(256) k_vflux(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
-                                               const double* __restrict__ q, const double* __restrict__ mut,
-                                               const double* __restrict__ sm, const double* __restrict__ xd,
-                                               const double* __restrict__ ed, const double* __restrict__ zd,
+                                               const T* __restrict__ q, const T* __restrict__ mut,
+                                               const T* __restrict__ sm, const T* __restrict__ xd,
+                                               const T* __restrict__ ed, const T* __restrict__ zd,
                                                const unsigned char* __restrict__ sbc,
-                                               const ll* __restrict__ sbcoff, double* __restrict__ vf) {
+                                               const ll* __restrict__ sbcoff, T* __restrict__ vf) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int ic = DIR == 1 ? b.imax : b.imax - 1, jc = DIR == 2 ? b.jmax : b.jmax - 1,
@@ -65,163 +43,9 @@ __global__ void __launch_bounds__(256) k_vflux(Par p, const Blk* __restrict__ bl
   const int i = rem % ic + 1;
   rem /= ic;
   const int j = rem % jc + 1, k = rem / jc + 1;
-  const ll n3 = n3of(b);
-  const double* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
-  const ll st[4] = {0, 1, (ll)(b.imax + 3), (ll)(b.imax + 3) * (b.jmax + 3)};
-  const ll cp = i3(b, i, j, k), cm = cp - st[DIR];
-  const int a = DIR == 1 ? i : (DIR == 2 ? j : k);
-  const int L = DIR == 1 ? b.imax : (DIR == 2 ? b.jmax : b.kmax);
-  const int t1 = DIR == 1 ? j : i, t2 = DIR == 3 ? j : k;
-  unsigned char flag = 0;
-  if (a == 1 || a == L) flag = face_flag<DIR>(b, sbc + sbcoff[t.block], a == L ? 1 : 0, t1, t2);
-  const bool wall = (flag & 1) != 0;
-  // transverse directions, lower-numbered first
-  const int d1 = DIR == 1 ? 2 : 1, d2 = DIR == 3 ? 2 : 3;
-  ll aux = 0, n1 = 0, n2 = 0;
-  double sgnm = 1.0;
-  if (wall) {
-    if (a == 1) {
-      aux = cm; n1 = cp; n2 = cp + st[DIR]; sgnm = 1.0;
-    } else {
-      aux = cp; n1 = cm; n2 = cm - st[DIR]; sgnm = -1.0;
-    }
-  }
-  double qf[NV], dd[3][NV];
-#pragma unroll
-  for (int v = 0; v < NV; v++) {
-    const double* qv = qb + n3 * v;
-    qf[v] = wall ? qv[aux] : (qv[cm] + qv[cp]) / 2;
-    for (int d = 1; d <= 3; d++) {
-      double val;
-      if (!wall) {
-        if (d == DIR) {
-          val = qv[cp] - qv[cm];
-        } else {
-          const ll tt = st[d];
-          const double q2 = (qv[cm] + qv[cp] + qv[cp + tt] + qv[cm + tt]) / 4;
-          const double q1 = (qv[cm - tt] + qv[cp - tt] + qv[cp] + qv[cm]) / 4;
-          val = q2 - q1;
-        }
-      } else {
-        if (d == DIR) {
-          val = sgnm / 3 * (-qv[aux] * 8 + qv[n1] * 9 - qv[n2]);
-        } else {
-          const ll tt = st[d];
-          const bool first = d == d1 ? (flag & 2) != 0 : (flag & 8) != 0;
-          const bool last = d == d1 ? (flag & 4) != 0 : (flag & 16) != 0;
-          const double qc = qv[aux], qp1 = qv[aux + tt], qm1 = qv[aux - tt];
-          double q2, q1;
-          if (first) {
-            q2 = (qc + qp1) / 2;
-            q1 = (qc * 3 - qp1) / 2;
-          } else if (last) {
-            q2 = (qc * 3 - qm1) / 2;
-            q1 = (qc + qm1) / 2;
-          } else {
-            q2 = (qc + qp1) / 2;
-            q1 = (qm1 + qc) / 2;
-          }
-          val = q2 - q1;
-        }
-      }
-      dd[d - 1][v] = val;
-    }
-  }
-  (void)d2;
-  double mutf = 0.0;
-  if (p.kom) {
-    const double* mb = mut + b.o3 * p.nsnap + n3 * (ll)n;
-    mutf = wall ? mb[aux] : (mb[cm] + mb[cp]) / 2;
-  }
-  const int nh = n * p.hbmove;
-  const double* x3 = mder(p, b, xd, i, j, k, nh);
-  const double* e3 = mder(p, b, ed, i, j, k, nh);
-  const double* z3 = mder(p, b, zd, i, j, k, nh);
-  const double xix = x3[0], xiy = x3[1], xiz = x3[2];
-  const double etax = e3[0], etay = e3[1], etaz = e3[2];
-  const double zetax = z3[0], zetay = z3[1], zetaz = z3[2];
-  const double* s = sm + b.o2 * p.lmet * p.nsnap_met + ((ll)nh * n2of(b) + i2(b, i, j, k)) * p.lmet;
-  const double kx = s[0], ky = s[1], kz = s[2], ds = s[3];
-  const double gamma = p.gamma;
-  const double rho = qf[0], u = qf[1], v = qf[2], w = qf[3], pr = qf[4];
-  const double tmp = gamma * pr / rho;
-  const double mu = suth(p, tmp);
-  const double uxi = dd[0][1], ueta = dd[1][1], uzeta = dd[2][1];
-  const double vxi = dd[0][2], veta = dd[1][2], vzeta = dd[2][2];
-  const double wxi = dd[0][3], weta = dd[1][3], wzeta = dd[2][3];
-  const double dudx = uxi * xix + ueta * etax + uzeta * zetax;
-  const double dudy = uxi * xiy + ueta * etay + uzeta * zetay;
-  const double dudz = uxi * xiz + ueta * etaz + uzeta * zetaz;
-  const double dvdx = vxi * xix + veta * etax + vzeta * zetax;
-  const double dvdy = vxi * xiy + veta * etay + vzeta * zetay;
-  const double dvdz = vxi * xiz + veta * etaz + vzeta * zetaz;
-  const double dwdx = wxi * xix + weta * etax + wzeta * zetax;
-  const double dwdy = wxi * xiy + weta * etay + wzeta * zetay;
-  const double dwdz = wxi * xiz + weta * etaz + wzeta * zetaz;
-  const double mr = p.machfs / p.reyno;
-  double F[NV];
-#pragma unroll
-  for (int m = 0; m < NV; m++) F[m] = 0.0;
-  if (NV == 7) {
-    // rtst: Reynolds stresses with mut at the face
-    const double tke = qf[5];
-    const double fact = 2.0 / 3.0;
-    const double divt = (dudx + dvdy + dwdz) * (double)p.turcmp;
-    const double ttxx = 2 * mutf * (dudx - divt / 3);
-    const double ttxy = mutf * (dudy + dvdx);
-    const double ttxz = mutf * (dudz + dwdx);
-    const double ttyy = 2 * mutf * (dvdy - divt / 3);
-    const double ttyz = mutf * (dvdz + dwdy);
-    const double ttzz = 2 * mutf * (dwdz - divt / 3);
-    const double r1 = mr * ttxx - fact * rho * tke;
-    const double r2 = mr * ttxy;
-    const double r3 = mr * ttxz;
-    const double r4 = mr * ttyy - fact * rho * tke;
-    const double r5 = mr * ttyz;
-    const double r6 = mr * ttzz - fact * rho * tke;
-    // vflux, k-omega branch
-    const double tkexi = dd[0][5], tkeeta = dd[1][5], tkezeta = dd[2][5];
-    const double omexi = dd[0][6], omeeta = dd[1][6];
-    const double dtkedx = tkexi * xix + tkeeta * etax + tkezeta * zetax;
-    const double dtkedy = tkexi * xiy + tkeeta * etay + tkezeta * zetay;
-    const double dtkedz = tkexi * xiz + tkeeta * etaz + tkezeta * zetaz;
-    const double domedx = omexi * xix + omeeta * etax + dd[2][6] * zetax;
-    const double domedy = omexi * xiy + omeeta * etay + dd[2][6] * zetay;
-    F[1] = -((r1 * kx + r2 * ky + r3 * kz) * ds);
-    F[2] = -((r2 * kx + r4 * ky + r5 * kz) * ds);
-    F[3] = -((r3 * kx + r5 * ky + r6 * kz) * ds);
-    const double ctke = (mu + p.sigk * mutf) * p.machfs / p.reyno;
-    const double come = (mu + p.sigw * mutf) * p.machfs / p.reyno;
-    F[4] = -(ctke * (dtkedx * kx + dtkedy * ky + dtkedz * kz) * ds);
-    F[5] = -(ctke * (dtkedx * kx + dtkedy * ky + dtkedz * kz) * ds);
-    // as in the reference: dk/dz in place of domega/dz
-    F[6] = -(come * (domedx * kx + domedy * ky + dtkedz * kz) * ds);
-  }
-  // laminar stresses and heat flux (common part of vflux_b)
-  const double txi = tmp * (dd[0][4] / pr - dd[0][0] / rho);
-  const double teta = tmp * (dd[1][4] / pr - dd[1][0] / rho);
-  const double tzeta = tmp * (dd[2][4] / pr - dd[2][0] / rho);
-  const double dtdx = txi * xix + teta * etax + tzeta * zetax;
-  const double dtdy = txi * xiy + teta * etay + tzeta * zetay;
-  const double dtdz = txi * xiz + teta * etaz + tzeta * zetaz;
-  const double divv = dudx + dvdy + dwdz;
-  const double txx = mr * 2 * mu * (dudx - divv / 3);
-  const double txy = mr * mu * (dudy + dvdx);
-  const double txz = mr * mu * (dudz + dwdx);
-  const double tyy = mr * 2 * mu * (dvdy - divv / 3);
-  const double tyz = mr * mu * (dvdz + dwdy);
-  const double tzz = mr * 2 * mu * (dwdz - divv / 3);
-  const double cq = mr * (mu / p.pranl + mutf / p.prant) / (gamma - 1);
-  const double qx = -(cq * dtdx), qy = -(cq * dtdy), qz = -(cq * dtdz);
-  F[0] = 0;
-  F[1] = F[1] - (txx * kx + txy * ky + txz * kz) * ds;
-  F[2] = F[2] - (txy * kx + tyy * ky + tyz * kz) * ds;
-  F[3] = F[3] - (txz * kx + tyz * ky + tzz * kz) * ds;
-  // as in the reference: the z term repeats the y-row stresses
-  F[4] = F[4] - ((u * txx + v * txy + w * txz) * kx + (u * txy + v * tyy + w * tyz) * ky +
-                 (u * txy + v * tyy + w * tyz) * kz) * ds +
-         (qx * kx + qy * ky + qz * kz) * ds;
-  double* fb = vf + b.o0 * p.npde * p.nsnap;
+  T F[NV];
+  viscous_face<DIR, NV, T>(p, b, i, j, k, n, q, mut, sm, xd, ed, zd, sbc + sbcoff[t.block], F);
+  T* fb = vf + b.o0 * p.npde * p.nsnap;
   const ll fo = (ll)(i - 1) + (ll)b.imax * ((ll)(j - 1) + (ll)b.jmax * (ll)(k - 1));
   const ll fst = (ll)b.imax * b.jmax * b.kmax;
 #pragma unroll
@@ -229,9 +53,9 @@ __global__ void __launch_bounds__(256) k_vflux(Par p, const Blk* __restrict__ bl
 }
 
 // res(c) += F(c+1) - F(c) of the viscous face fluxes of direction DIR
-template <int DIR, int NV>
+template <int DIR, int NV, typename T>
 __global__ void __launch_bounds__(256) k_vresid(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
-                                                const double* __restrict__ vf, double* __restrict__ res) {
+                                                const T* __restrict__ vf, double* __restrict__ res) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
@@ -245,7 +69,7 @@ __global__ void __launch_bounds__(256) k_vresid(Par p, const Blk* __restrict__ b
   const int j = rem % jm + 1, k = rem / jm + 1;
   const ll n3 = n3of(b);
   double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n + i3(b, i, j, k);
-  const double* fb = vf + b.o0 * p.npde * p.nsnap;
+  const T* fb = vf + b.o0 * p.npde * p.nsnap;
   const ll fst = (ll)b.imax * b.jmax * b.kmax;
   const ll f0 = (ll)(i - 1) + (ll)b.imax * ((ll)(j - 1) + (ll)b.jmax * (ll)(k - 1));
   const ll f1 = f0 + (DIR == 1 ? 1 : (DIR == 2 ? (ll)b.imax : (ll)b.imax * b.jmax));
@@ -256,16 +80,103 @@ __global__ void __launch_bounds__(256) k_vresid(Par p, const Blk* __restrict__ b
   }
 }
 
+// Viscous fluxes of direction 1 and their difference in one pass: one
+// wavefront per i-row segment, lane l computes the viscous flux of face
+// i0 + l and cell c takes F(c+1) from the neighbouring lane.  Same
+// arithmetic as k_vflux<1> followed by k_vresid<1>.
+template <int NV, typename T>
+__global__ void __launch_bounds__(256) k_vres_i(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
+                                                const T* __restrict__ q, const T* __restrict__ mut,
+                                                const T* __restrict__ sm, const T* __restrict__ xd,
+                                                const T* __restrict__ ed, const T* __restrict__ zd,
+                                                const unsigned char* __restrict__ sbc, const ll* __restrict__ sbcoff,
+                                                double* __restrict__ res) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int lane = threadIdx.x & 63;
+  const int item = t.base + (threadIdx.x >> 6);
+  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
+  const int nseg = (im + ROWSEG - 1) / ROWSEG;
+  const int nrows = jm * km;
+  if (item >= nseg * nrows * p.nsnap) return;   // whole wavefront exits
+  const int n = item / (nseg * nrows);
+  const int rem = item - n * nseg * nrows;
+  const int row = rem / nseg, seg = rem - (rem / nseg) * nseg;
+  const int j = row % jm + 1, k = row / jm + 1;
+  const int a = seg * ROWSEG + 1 + lane;   // face index
+  T F[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) F[v] = 0;
+  if (a <= b.imax) viscous_face<1, NV, T>(p, b, a, j, k, n, q, mut, sm, xd, ed, zd, sbc + sbcoff[t.block], F);
+  T Fn[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) Fn[v] = __shfl_down(F[v], 1, 64);
+  const int c = a;
+  if (lane < ROWSEG && c <= im) {
+    const ll n3 = n3of(b);
+    double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n + i3(b, c, j, k);
+#pragma unroll
+    for (int v = 0; v < NV; v++) rb[n3 * v] = rb[n3 * v] + (double)Fn[v] - (double)F[v];
+  }
+}
+
+// Viscous fluxes of direction DIR = 2 or 3 and their difference in one
+// pass: one thread per MSEG-cell segment of a line, marching its faces and
+// carrying the previous face's flux.  Same arithmetic as k_vflux<DIR> +
+// k_vresid<DIR>.
+template <int DIR, int NV, typename T>
+__global__ void __launch_bounds__(256) k_vres_march(Par p, const Blk* __restrict__ blks,
+                                                    const Tile* __restrict__ tiles, const T* __restrict__ q,
+                                                    const T* __restrict__ mut, const T* __restrict__ sm,
+                                                    const T* __restrict__ xd, const T* __restrict__ ed,
+                                                    const T* __restrict__ zd, const unsigned char* __restrict__ sbc,
+                                                    const ll* __restrict__ sbcoff, double* __restrict__ res) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int item = t.base + threadIdx.x;
+  const int im = b.imax - 1;
+  const int t2n = DIR == 2 ? b.kmax - 1 : b.jmax - 1;
+  const int L = DIR == 2 ? b.jmax : b.kmax;
+  const int nseg = (L - 1 + MSEG - 1) / MSEG;
+  const int nlines = im * t2n;
+  if (item >= nlines * nseg * p.nsnap) return;
+  const int n = item / (nlines * nseg);
+  int rem = item - n * nlines * nseg;
+  const int sg = rem / nlines;
+  rem -= sg * nlines;
+  const int i = rem % im + 1, t2 = rem / im + 1;
+  const int a0 = 1 + sg * MSEG, a1 = min(L, a0 + MSEG);   // faces a0..a1, cells a0..a1-1
+  const ll n3 = n3of(b);
+  double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  const ll step = DIR == 2 ? (ll)(b.imax + 3) : (ll)(b.imax + 3) * (b.jmax + 3);
+  const ll base = DIR == 2 ? i3(b, i, 0, t2) : i3(b, i, t2, 0);
+  const unsigned char* fl = sbc + sbcoff[t.block];
+  T Fp[NV];
+  for (int a = a0; a <= a1; a++) {
+    T F[NV];
+    if (DIR == 2) viscous_face<2, NV, T>(p, b, i, a, t2, n, q, mut, sm, xd, ed, zd, fl, F);
+    else viscous_face<3, NV, T>(p, b, i, t2, a, n, q, mut, sm, xd, ed, zd, fl, F);
+    if (a > a0) {
+      const ll c = base + (ll)(a - 1) * step;
+#pragma unroll
+      for (int v = 0; v < NV; v++) rb[c + n3 * v] = rb[c + n3 * v] + (double)F[v] - (double)Fp[v];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) Fp[v] = F[v];
+  }
+}
+
 // ------------------------------------------------------------------------
 // k-omega sources at cell centres: the velocity differences of the face
 // averages (q_face + q_der_c, three directions), turprod (finest level)
 // and srckw into res(6:7).  q primitive.
+template <typename T>
 __global__ void __launch_bounds__(256) k_turb_src(Par p, int nl, const Blk* __restrict__ blks,
-                                                  const Tile* __restrict__ tiles, const double* __restrict__ q,
-                                                  const double* __restrict__ mut, double* __restrict__ prd,
+                                                  const Tile* __restrict__ tiles, const T* __restrict__ q,
+                                                  const T* __restrict__ mut, double* __restrict__ prd,
                                                   double* __restrict__ divvel, double* __restrict__ delpls,
                                                   double* __restrict__ res, const double* __restrict__ vol,
-                                                  const double* const* __restrict__ xdp,
+                                                  const T* const* __restrict__ xdp,
                                                   const unsigned char* __restrict__ sbc,
                                                   const ll* __restrict__ sbcoff) {
   const Tile t = tiles[blockIdx.x];
@@ -280,14 +191,14 @@ __global__ void __launch_bounds__(256) k_turb_src(Par p, int nl, const Blk* __re
   rem /= im;
   const int j = rem % jm + 1, k = rem / jm + 1;
   const ll n3 = n3of(b);
-  const double* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  const T* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
   const ll c = i3(b, i, j, k);
   const ll st[4] = {0, 1, (ll)(b.imax + 3), (ll)(b.imax + 3) * (b.jmax + 3)};
   const unsigned char* fl = sbc + sbcoff[t.block];
   const ll so = b.o3 * p.nsnap + n3 * (ll)n + c;
   if (nl == 1 || p.prd_all_lvls) {
     // dq(d)[v] = qf(face c+1) - qf(face c), v = u, v, w
-    double dq[3][3];
+    T dq[3][3];
     for (int d = 1; d <= 3; d++) {
       const int a = d == 1 ? i : (d == 2 ? j : k);
       const int L = d == 1 ? b.imax : (d == 2 ? b.jmax : b.kmax);
@@ -305,60 +216,60 @@ __global__ void __launch_bounds__(256) k_turb_src(Par p, int nl, const Blk* __re
         whi = (f & 1) != 0;
       }
       for (int v = 1; v <= 3; v++) {
-        const double* qv = qb + n3 * v;
-        const double flo = wlo ? qv[c - st[d]] : (qv[c - st[d]] + qv[c]) / 2;
-        const double fhi = whi ? qv[c + st[d]] : (qv[c] + qv[c + st[d]]) / 2;
+        const T* qv = qb + n3 * v;
+        const T flo = wlo ? qv[c - st[d]] : (qv[c - st[d]] + qv[c]) / 2;
+        const T fhi = whi ? qv[c + st[d]] : (qv[c] + qv[c + st[d]]) / 2;
         dq[d - 1][v - 1] = fhi - flo;
       }
     }
     const int nh = n * p.hbmove;
     // xdp: xideri, etaderi, zetaderi, xiderj, etaderj, zetaderj, xiderk, etaderk, zetaderk
-    const double xix = (mder(p, b, xdp[0], i + 1, j, k, nh)[0] + mder(p, b, xdp[0], i, j, k, nh)[0]) / 2;
-    const double xiy = (mder(p, b, xdp[3], i, j + 1, k, nh)[1] + mder(p, b, xdp[3], i, j, k, nh)[1]) / 2;
-    const double xiz = (mder(p, b, xdp[6], i, j, k + 1, nh)[2] + mder(p, b, xdp[6], i, j, k, nh)[2]) / 2;
-    const double etax = (mder(p, b, xdp[1], i + 1, j, k, nh)[0] + mder(p, b, xdp[1], i, j, k, nh)[0]) / 2;
-    const double etay = (mder(p, b, xdp[4], i, j + 1, k, nh)[1] + mder(p, b, xdp[4], i, j, k, nh)[1]) / 2;
-    const double etaz = (mder(p, b, xdp[7], i, j, k + 1, nh)[2] + mder(p, b, xdp[7], i, j, k, nh)[2]) / 2;
-    const double zetax = (mder(p, b, xdp[2], i + 1, j, k, nh)[0] + mder(p, b, xdp[2], i, j, k, nh)[0]) / 2;
-    const double zetay = (mder(p, b, xdp[5], i, j + 1, k, nh)[1] + mder(p, b, xdp[5], i, j, k, nh)[1]) / 2;
-    const double zetaz = (mder(p, b, xdp[8], i, j, k + 1, nh)[2] + mder(p, b, xdp[8], i, j, k, nh)[2]) / 2;
-    const double uxi = dq[0][0], ueta = dq[1][0], uzeta = dq[2][0];
-    const double vxi = dq[0][1], veta = dq[1][1], vzeta = dq[2][1];
-    const double wxi = dq[0][2], weta = dq[1][2], wzeta = dq[2][2];
-    const double dudx = uxi * xix + ueta * etax + uzeta * zetax;
-    const double dudy = uxi * xiy + ueta * etay + uzeta * zetay;
-    const double dudz = uxi * xiz + ueta * etaz + uzeta * zetaz;
-    const double dvdx = vxi * xix + veta * etax + vzeta * zetax;
-    const double dvdy = vxi * xiy + veta * etay + vzeta * zetay;
-    const double dvdz = vxi * xiz + veta * etaz + vzeta * zetaz;
-    const double dwdx = wxi * xix + weta * etax + wzeta * zetax;
-    const double dwdy = wxi * xiy + weta * etay + wzeta * zetay;
-    const double dwdz = wxi * xiz + weta * etaz + wzeta * zetaz;
-    const double divv = (dudx + dvdy + dwdz) * (double)p.turcmp;
-    const double txx = dudx, txy = (dudy + dvdx) / 2, txz = (dudz + dwdx) / 2, tyy = dvdy, tyz = (dvdz + dwdy) / 2,
+    const T xix = (mder<T>(p, b, xdp[0], i + 1, j, k, nh)[0] + mder<T>(p, b, xdp[0], i, j, k, nh)[0]) / 2;
+    const T xiy = (mder<T>(p, b, xdp[3], i, j + 1, k, nh)[1] + mder<T>(p, b, xdp[3], i, j, k, nh)[1]) / 2;
+    const T xiz = (mder<T>(p, b, xdp[6], i, j, k + 1, nh)[2] + mder<T>(p, b, xdp[6], i, j, k, nh)[2]) / 2;
+    const T etax = (mder<T>(p, b, xdp[1], i + 1, j, k, nh)[0] + mder<T>(p, b, xdp[1], i, j, k, nh)[0]) / 2;
+    const T etay = (mder<T>(p, b, xdp[4], i, j + 1, k, nh)[1] + mder<T>(p, b, xdp[4], i, j, k, nh)[1]) / 2;
+    const T etaz = (mder<T>(p, b, xdp[7], i, j, k + 1, nh)[2] + mder<T>(p, b, xdp[7], i, j, k, nh)[2]) / 2;
+    const T zetax = (mder<T>(p, b, xdp[2], i + 1, j, k, nh)[0] + mder<T>(p, b, xdp[2], i, j, k, nh)[0]) / 2;
+    const T zetay = (mder<T>(p, b, xdp[5], i, j + 1, k, nh)[1] + mder<T>(p, b, xdp[5], i, j, k, nh)[1]) / 2;
+    const T zetaz = (mder<T>(p, b, xdp[8], i, j, k + 1, nh)[2] + mder<T>(p, b, xdp[8], i, j, k, nh)[2]) / 2;
+    const T uxi = dq[0][0], ueta = dq[1][0], uzeta = dq[2][0];
+    const T vxi = dq[0][1], veta = dq[1][1], vzeta = dq[2][1];
+    const T wxi = dq[0][2], weta = dq[1][2], wzeta = dq[2][2];
+    const T dudx = uxi * xix + ueta * etax + uzeta * zetax;
+    const T dudy = uxi * xiy + ueta * etay + uzeta * zetay;
+    const T dudz = uxi * xiz + ueta * etaz + uzeta * zetaz;
+    const T dvdx = vxi * xix + veta * etax + vzeta * zetax;
+    const T dvdy = vxi * xiy + veta * etay + vzeta * zetay;
+    const T dvdz = vxi * xiz + veta * etaz + vzeta * zetaz;
+    const T dwdx = wxi * xix + weta * etax + wzeta * zetax;
+    const T dwdy = wxi * xiy + weta * etay + wzeta * zetay;
+    const T dwdz = wxi * xiz + weta * etaz + wzeta * zetaz;
+    const T divv = (dudx + dvdy + dwdz) * (T)p.turcmp;
+    const T txx = dudx, txy = (dudy + dvdx) / 2, txz = (dudz + dwdx) / 2, tyy = dvdy, tyz = (dvdz + dwdy) / 2,
                  tzz = dwdz;
     prd[so] = 2 * (txx * txx + tyy * tyy + tzz * tzz + 2 * (txy * txy + txz * txz + tyz * tyz) - divv * divv / 3);
     divvel[so] = divv;
-    delpls[so] = 2.0 / 3.0 * fmax(divv, 0.0);
+    delpls[so] = T(2.0 / 3.0) * fmax(divv, T(0));
   }
   // srckw_b
-  const double fact = 2.0 / 3.0, mor = p.machfs / p.reyno;
-  const double rho = qb[c], tke = qb[c + 5 * n3], ome = qb[c + 6 * n3];
-  const double mu_t = mut[so];
-  const double divv = divvel[so];
-  double prod = prd[so], prodw = prd[so];
+  const T fact = T(2.0 / 3.0), mor = (T)(p.machfs / p.reyno);
+  const T rho = qb[c], tke = qb[c + 5 * n3], ome = qb[c + 6 * n3];
+  const T mu_t = mut[so];
+  const T divv = (T)divvel[so];
+  T prod = (T)prd[so], prodw = (T)prd[so];
   if (p.lim_prodtke && (nl == 1 || p.prd_all_lvls)) {
-    const double tstdis = p.prdlim * p.bstrkw * rho * tke * ome;
+    const T tstdis = (T)p.prdlim * (T)p.bstrkw * rho * tke * ome;
     if (mor * mu_t * prod > tstdis) {
       prod = tstdis / mor / mu_t;
       if (p.lim_prodomega) prodw = prod;
     }
   }
-  const double vv = vol[b.o1 + i1(b, i, j, k)];
+  const T vv = (T)vol[b.o1 + i1(b, i, j, k)];
   double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n + c;
-  rb[5 * n3] = rb[5 * n3] - vv * (mor * mu_t * prod - fact * divv * rho * tke - p.bstrkw * rho * tke * ome);
-  rb[6 * n3] = rb[6 * n3] - vv * (p.gkw * p.cmu * rho * prodw - p.gkw * fact * divv * rho * ome -
-                                  p.bkw * rho * (ome * ome));
+  rb[5 * n3] = rb[5 * n3] - vv * (mor * mu_t * prod - fact * divv * rho * tke - (T)p.bstrkw * rho * tke * ome);
+  rb[6 * n3] = rb[6 * n3] - vv * ((T)p.gkw * (T)p.cmu * rho * prodw - (T)p.gkw * fact * divv * rho * ome -
+                                  (T)p.bkw * rho * (ome * ome));
 }
 
 // tur_vis_b over 0..imax x 0..jmax x 0..kmax (q primitive or conservative,
@@ -457,57 +368,128 @@ __global__ void __launch_bounds__(256) k_turb_limits(Par p, int do_prod, double 
 
 static inline dim3 grid1(int n) { return dim3(n > 0 ? n : 1); }
 
-hipError_t launch_residual_viscous(Solve& s, Level& L, const double* q, double* res) {
+template <typename T>
+static hipError_t residual_viscous_t(Solve& s, Level& L, const ResIn<T>& in, double* res, T* vflux) {
   const Par& p = s.par;
   hipError_t e;
-  if (p.viscous) {
-    e = launch_edges(s, L, const_cast<double*>(q));
-    if (e != hipSuccess) return e;
-  }
   if (p.kom) {
-    e = launch_h0_last(s, L, q);
+    e = launch_h0_last<T>(s, L, in);
     if (e != hipSuccess) return e;
   }
-  const double* sm[3] = {L.f[COSAHIP_SI], L.f[COSAHIP_SJ], L.f[COSAHIP_SK]};
-  const int fk[3] = {TK_FACES_I, TK_FACES_J, TK_FACES_K};
-  for (int d = 1; d <= 3; d++) {
-    e = launch_flux_dir(s, L, d, q, res);
-    if (e != hipSuccess) return e;
-    if (!p.viscous) continue;
-    const double* xd = L.f[COSAHIP_XIDERI + 3 * (d - 1)];
-    const double* ed = L.f[COSAHIP_XIDERI + 3 * (d - 1) + 1];
-    const double* zd = L.f[COSAHIP_XIDERI + 3 * (d - 1) + 2];
-    const int nt = L.ntile[fk[d - 1]];
-    const Tile* tl = L.tile[fk[d - 1]];
-    const double* mu = L.f[COSAHIP_MUT];
-#define VF(D, NV)                                                                                                  \
-  hipLaunchKernelGGL((k_vflux<D, NV>), grid1(nt), dim3(TS), 0, s.stream, p, L.dblk, tl, q, mu, sm[D - 1], xd, ed, zd, \
-                     L.sbc, L.sbc_doff, L.vflux)
-#define VR(D, NV)                                                                                          \
-  hipLaunchKernelGGL((k_vresid<D, NV>), grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, p, L.dblk,        \
-                     L.tile[TK_CELLS], L.vflux, res)
-    if (nt) {
-      if (p.npde == 7) {
-        if (d == 1) { VF(1, 7); VR(1, 7); }
-        else if (d == 2) { VF(2, 7); VR(2, 7); }
-        else { VF(3, 7); VR(3, 7); }
-      } else {
-        if (d == 1) { VF(1, 5); VR(1, 5); }
-        else if (d == 2) { VF(2, 5); VR(2, 5); }
-        else { VF(3, 5); VR(3, 5); }
-      }
+  if (s.fuse_viscous || !p.viscous) {
+    for (int d = 1; d <= 3; d++) {
+      e = launch_flux_dir<T>(s, L, d, in, res);   // inviscid + viscous fluxes of direction d
+      if (e != hipSuccess) return e;
     }
-#undef VF
+  } else {
+    // per direction: inviscid line march, then the face-parallel viscous
+    // fluxes into vflux and their difference (faster than fusing the
+    // viscous face work into the line march, which runs out of registers)
+    const Par porig = s.par;
+    Par pi = porig;
+    pi.viscous = 0;
+    const int fk[3] = {TK_FACES_I, TK_FACES_J, TK_FACES_K};
+    for (int d = 1; d <= 3; d++) {
+      s.par = pi;
+      e = launch_flux_dir<T>(s, L, d, in, res);
+      s.par = porig;
+      if (e != hipSuccess) return e;
+      const T* xd = in.xd[3 * (d - 1)];
+      const T* ed = in.xd[3 * (d - 1) + 1];
+      const T* zd = in.xd[3 * (d - 1) + 2];
+      const int nt = L.ntile[fk[d - 1]];
+      const Tile* tl = L.tile[fk[d - 1]];
+#define VF(D, NV)                                                                                                 \
+  hipLaunchKernelGGL((k_vflux<D, NV, T>), grid1(nt), dim3(TS), 0, s.stream, p, L.dblk, tl, in.q, in.mut, in.s[D - 1], \
+                     xd, ed, zd, L.sbc, L.sbc_doff, vflux)
+#define VR(D, NV)                                                                                          \
+  hipLaunchKernelGGL((k_vresid<D, NV, T>), grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, p, L.dblk,     \
+                     L.tile[TK_CELLS], vflux, res)
+#define VI(NV)                                                                                                    \
+  hipLaunchKernelGGL((k_vres_i<NV, T>), grid1(L.ntile[TK_ROWS_I]), dim3(256), 0, s.stream, p, L.dblk,             \
+                     L.tile[TK_ROWS_I], in.q, in.mut, in.s[0], xd, ed, zd, L.sbc, L.sbc_doff, res)
+#define VM(D, NV, TK)                                                                                             \
+  hipLaunchKernelGGL((k_vres_march<D, NV, T>), grid1(L.ntile[TK]), dim3(TS), 0, s.stream, p, L.dblk, L.tile[TK],   \
+                     in.q, in.mut, in.s[D - 1], xd, ed, zd, L.sbc, L.sbc_doff, res)
+      if (s.viscous_faces) {
+        if (nt) {
+          if (p.npde == 7) {
+            if (d == 1) { VF(1, 7); VR(1, 7); }
+            else if (d == 2) { VF(2, 7); VR(2, 7); }
+            else { VF(3, 7); VR(3, 7); }
+          } else {
+            if (d == 1) { VF(1, 5); VR(1, 5); }
+            else if (d == 2) { VF(2, 5); VR(2, 5); }
+            else { VF(3, 5); VR(3, 5); }
+          }
+        }
+      } else if (p.npde == 7) {
+        if (d == 1 && L.ntile[TK_ROWS_I]) VI(7);
+        else if (d == 2 && L.ntile[TK_SEGS_J]) VM(2, 7, TK_SEGS_J);
+        else if (d == 3 && L.ntile[TK_SEGS_K]) VM(3, 7, TK_SEGS_K);
+      } else {
+        if (d == 1 && L.ntile[TK_ROWS_I]) VI(5);
+        else if (d == 2 && L.ntile[TK_SEGS_J]) VM(2, 5, TK_SEGS_J);
+        else if (d == 3 && L.ntile[TK_SEGS_K]) VM(3, 5, TK_SEGS_K);
+      }
 #undef VR
+#undef VI
+#undef VM
+    }
   }
   if (p.kom && L.ntile[TK_CELLS])
-    hipLaunchKernelGGL(k_turb_src, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, p, L.nl, L.dblk,
-                       L.tile[TK_CELLS], q, L.f[COSAHIP_MUT], L.f[COSAHIP_PRD], L.f[COSAHIP_DIVVEL],
-                       L.f[COSAHIP_DELPLS], res, L.f[COSAHIP_VOL], (const double* const*)L.dxd, L.sbc, L.sbc_doff);
+    hipLaunchKernelGGL(k_turb_src<T>, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, p, L.nl, L.dblk,
+                       L.tile[TK_CELLS], in.q, in.mut, L.f[COSAHIP_PRD], L.f[COSAHIP_DIVVEL], L.f[COSAHIP_DELPLS], res,
+                       L.f[COSAHIP_VOL], in.dxd, L.sbc, L.sbc_doff);
   return hipGetLastError();
 }
 
+hipError_t launch_residual_viscous(Solve& s, Level& L, const double* q, double* res) {
+  const Par& p = s.par;
+  const bool f32 = s.fp32() && q == L.f[COSAHIP_Q] && L.q32;
+  hipError_t e;
+  if (p.viscous) {
+    e = launch_edges(s, L, const_cast<double*>(q), f32 ? L.q32 : nullptr);
+    if (e != hipSuccess) return e;
+  }
+  if (!f32) {
+    ResIn<double> in{};
+    in.q = q;
+    for (int d = 0; d < 3; d++) in.s[d] = L.f[COSAHIP_SI + d];
+    in.mut = L.f[COSAHIP_MUT];
+    for (int m = 0; m < 9; m++) in.xd[m] = L.f[COSAHIP_XIDERI + m];
+    in.dxd = L.dxd;
+    return residual_viscous_t<double>(s, L, in, res, L.vflux);
+  }
+  // FP32 residual: single-precision copies of the inputs, converted when
+  // their FP64 source has changed (metrics on upload, mut after tur_vis)
+  if (!L.m32_ok) {
+    const ll nm = L.n2tot * p.lmet * p.nsnap_met;
+    for (int d = 0; d < 3; d++)
+      if ((e = launch_to_f32(s, L.f[COSAHIP_SI + d], L.m32[d], nm)) != hipSuccess) return e;
+    L.m32_ok = true;
+  }
+  if (p.viscous && !L.xd32_ok) {
+    const ll nx = L.n0tot * 3 * p.nsnap_met;
+    for (int m = 0; m < 9; m++)
+      if ((e = launch_to_f32(s, L.f[COSAHIP_XIDERI + m], L.xd32[m], nx)) != hipSuccess) return e;
+    L.xd32_ok = true;
+  }
+  if (p.kom && !L.mut32_ok) {
+    if ((e = launch_to_f32(s, L.f[COSAHIP_MUT], L.mut32, L.n3tot * p.nsnap)) != hipSuccess) return e;
+    L.mut32_ok = true;
+  }
+  ResIn<float> in{};
+  in.q = L.q32;
+  for (int d = 0; d < 3; d++) in.s[d] = L.m32[d];
+  in.mut = L.mut32;
+  for (int m = 0; m < 9; m++) in.xd[m] = L.xd32[m];
+  in.dxd = L.dxd32;
+  return residual_viscous_t<float>(s, L, in, res, reinterpret_cast<float*>(L.vflux));
+}
+
 hipError_t launch_tur_vis(Solve& s, Level& L, const double* q) {
+  L.mut32_ok = false;
   if (L.ntile[TK_CELLS_P1])
     hipLaunchKernelGGL(k_tur_vis, grid1(L.ntile[TK_CELLS_P1]), dim3(TS), 0, s.stream, s.par, L.dblk,
                        L.tile[TK_CELLS_P1], q, L.f[COSAHIP_MUT]);

@@ -6,7 +6,11 @@
 namespace cosahip {
 
 // ---- state conversions and array utilities (k_core.hip) ----
-hipError_t launch_cons2prim(Solve& s, Level& L, double* q);
+// shadow: also write the FP32 copy of the interior when the FP32-residual
+// mode is in effect and q is the level's Q
+hipError_t launch_cons2prim(Solve& s, Level& L, double* q, bool shadow = false);
+hipError_t launch_halo_f32(Solve& s, Level& L);
+hipError_t launch_to_f32(Solve& s, const double* src, float* dst, ll n);
 hipError_t launch_prim2cons(Solve& s, Level& L, double* q);
 hipError_t launch_copy_interior(Solve& s, Level& L, double* dst, const double* src);
 hipError_t launch_add_interior(Solve& s, Level& L, double* dst, const double* src, double fact);
@@ -23,21 +27,37 @@ hipError_t launch_rest1(Solve& s, Level& L1, Level& L2, const double* q1, double
 hipError_t launch_rest2(Solve& s, Level& L1, Level& L2, double* rhs2, const double* res1);
 hipError_t launch_sub_box(Solve& s, Level& L, double* dst, const double* src);  // dst -= src on 0..imax
 hipError_t launch_prol(Solve& s, Level& L1, Level& L2, double* q1, const double* q2);
-hipError_t launch_edges(Solve& s, Level& L, double* q);
+hipError_t launch_edges(Solve& s, Level& L, double* q, float* q32 = nullptr);
 hipError_t launch_cut_ncomp(Solve& s, Level& L, double* a, int ncomp);
 hipError_t launch_rest_scalar(Solve& s, Level& L1, Level& L2, const double* a1, double* a2);
+
+// inputs of the residual kernels in precision T: the level's FP64 fields,
+// or their FP32 copies in the FP32-residual mode
+template <typename T>
+struct ResIn {
+  const T* q;       // primitive state with halos
+  const T* s[3];    // si, sj, sk
+  const T* mut;     // eddy viscosity (k-omega)
+  const T* xd[9];   // metric derivatives xideri .. zetaderk
+  const T* const* dxd;   // device table of xd
+};
 
 // ---- spectral radii, residual, smoothing (k_flux.hip) ----
 hipError_t launch_lambda_deltat(Solve& s, Level& L);
 hipError_t launch_residual(Solve& s, Level& L, const double* q, double* res);
 hipError_t launch_cirs(Solve& s, Level& L);
-hipError_t launch_flux_dir(Solve& s, Level& L, int idir, const double* q, double* res);
-hipError_t launch_h0_last(Solve& s, Level& L, const double* q);
+template <typename T>
+hipError_t launch_flux_dir(Solve& s, Level& L, int idir, const ResIn<T>& in, double* res);
+template <typename T>
+hipError_t launch_h0_last(Solve& s, Level& L, const ResIn<T>& in);
 
 // ---- viscous fluxes and Wilcox k-omega (k_turb.hip) ----
 hipError_t launch_residual_viscous(Solve& s, Level& L, const double* q, double* res);
 hipError_t launch_tur_vis(Solve& s, Level& L, const double* q);
 hipError_t launch_fixq(Solve& s, Level& L);
 hipError_t launch_turb_limits(Solve& s, Level& L, double ar);
+
+// ---- forces (k_force.hip) ----
+hipError_t launch_force(Solve& s, Level& L, int nsurface, const double* geom_dev, const double* momr_dev);
 
 }  // namespace cosahip

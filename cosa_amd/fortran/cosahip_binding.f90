@@ -23,6 +23,7 @@ module cosahip_binding
   integer(c_int), parameter :: COSAHIP_ZETADERJ = 22, COSAHIP_XIDERK = 23, COSAHIP_ETADERK = 24
   integer(c_int), parameter :: COSAHIP_ZETADERK = 25, COSAHIP_PRD = 26, COSAHIP_DIVVEL = 27
   integer(c_int), parameter :: COSAHIP_DELPLS = 28, COSAHIP_DTVOLT = 29
+  integer(c_int), parameter :: COSAHIP_X = 30, COSAHIP_Y = 31, COSAHIP_Z = 32
 
   ! cosahip_config, member for member
   type, bind(C) :: cosahip_config
@@ -114,6 +115,24 @@ module cosahip_binding
       import :: c_int, c_ptr
       type(c_ptr), value :: solve
       integer(c_int), value :: nl
+    end function
+    integer(c_int) function cosahip_force(solve, nl, nsurface, include_inviscid, lngth, momr, coef) &
+        bind(C, name='cosahip_force')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: solve
+      integer(c_int), value :: nl, nsurface, include_inviscid
+      real(c_double), intent(in) :: lngth(3), momr(*)
+      real(c_double), intent(out) :: coef(*)
+    end function
+    integer(c_int) function cosahip_set_precision(solve, mode, tail_rms) bind(C, name='cosahip_set_precision')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: solve
+      integer(c_int), value :: mode
+      real(c_double), value :: tail_rms
+    end function
+    integer(c_int) function cosahip_precision_in_use(solve) bind(C, name='cosahip_precision_in_use')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: solve
     end function
     type(c_ptr) function cosahip_last_error_c(solve) bind(C, name='cosahip_last_error')
       import :: c_ptr

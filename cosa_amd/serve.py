@@ -1,0 +1,253 @@
+"""Continuous-batching serving of steady and harmonic-balance solves on one
+GPU (the north star's serving form; the reference runs one solve per
+process, src/cosa.f90:806-930).
+
+Requests arrive over time, each a NACA 0012 O-grid solve at its own Mach
+number, incidence, grid size and (optionally) harmonic-balance pitching.
+The server keeps every admitted solve resident in HBM (its own
+cosahip_solve and HIP stream), admits waiting requests while the estimated
+device footprint of the resident solves stays within a budget, and runs
+rounds: one V-cycle is enqueued for every resident solve (graph replays on
+separate streams, so their kernels overlap on the device), then each is
+waited for; a solve retires when its density residual rms has dropped by
+`drop` from its first cycle (or after `max_cycles`), its force coefficients
+are read and its device memory is freed.  Reported: time-to-converged per
+request (from arrival), p50 / p99, converged solves per second and the
+aggregate cell-update rate.
+
+    python -m cosa_amd.serve --requests 24 --precision mixed
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import case as cases
+
+# C5 grid sizes: O-grid nodes (i around the section, j normal); 5 k-nodes
+# (4 cells) so that both multigrid levels keep at least 2 cells in k
+GRIDS = {"small": (129, 33), "medium": (257, 65), "large": (513, 129)}
+NK = 5
+
+
+@dataclass
+class Request:
+    rid: int
+    grid: str = "small"
+    mach: float = 0.5
+    alpha_deg: float = 1.0
+    nharms: int = 0            # > 0: harmonic-balance pitching, dtheta0 = 1 deg
+    arrival: float = 0.0       # seconds after the server starts
+    drop: float = 1e-3         # retire when resrms(1) <= drop * first resrms(1)
+    max_cycles: int = 400
+    precision: int = 0         # cosahip_set_precision mode
+    tail_rms_factor: float = 1e-2   # mode 2: FP64 tail below this fraction of the first rms
+
+    def make_case(self) -> "cases.Case":
+        ni, nj = GRIDS[self.grid]
+        kw = dict(machfs=self.mach, alpha_deg=self.alpha_deg, irsop=1, cfl=2.0, cfl_irs=3.5, nl_crs=2,
+                  ncycle=self.max_cycles, toler=0.0)
+        pitch = None
+        if self.nharms:
+            kw.update(nharms=self.nharms, harbal=1, omega=0.2)
+            pitch = dict(dtheta0=math.radians(1.0), xh=0.25, yh=0.0)
+        return cases.naca_case(ni, nj, NK, nlevel=2, name=f"req{self.rid}", pitch=pitch, **kw)
+
+
+@dataclass
+class Result:
+    rid: int
+    cells: int
+    cycles: int
+    converged: bool
+    t_admit: float
+    t_done: float
+    latency: float             # t_done - arrival
+    resrms: List[float]        # resrms(1) per cycle
+    forces: np.ndarray         # (nsnap, 12) of surface 1 (type-14 walls)
+    updates: float = 0.0       # interior (cell, snapshot) RK-stage updates performed
+    q: Optional[np.ndarray] = None
+
+
+def device_bytes_estimate(c: "cases.Case") -> int:
+    """Device memory of a solve: the library's per-level fields, scratch and
+    tables, from the array extents (an upper estimate)."""
+    tot = 0
+    for L in c.levels:
+        nl = L.nl
+        p3 = c.size(nl, "p3")
+        tot += 8 * (6 * p3 + 6 * c.size(nl, "p3s") + 4 * c.size(nl, "met") + 9 * c.size(nl, "x2")
+                    + 6 * c.size(nl, "p1") + c.size(nl, "vol") + 9 * c.size(nl, "d0") + p3)
+    return tot + (8 << 20)
+
+
+@dataclass
+class Server:
+    budget_bytes: int = 8 << 30
+    max_active: int = 64
+    device: int = 0
+    keep_state: bool = False
+    results: List[Result] = field(default_factory=list)
+
+    def run(self, requests: List[Request], clock=time.perf_counter) -> List[Result]:
+        """Serve the requests.  Grids and metrics are built before the clock
+        starts (the client's set-up step, done on the host as the reference's
+        driver does); admission, upload, the cycles, retirement and the
+        force read-back are timed."""
+        from .solver import Solve
+        built = {r.rid: r.make_case() for r in requests}
+        waiting = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        active: Dict[int, dict] = {}
+        used = 0
+        t0 = clock()
+        while waiting or active:
+            now = clock() - t0
+            # admission: arrived requests, in order, while the footprint fits
+            while waiting and waiting[0].arrival <= now and len(active) < self.max_active:
+                r = waiting[0]
+                c = built[r.rid]
+                need = device_bytes_estimate(c)
+                if active and used + need > self.budget_bytes:
+                    break
+                waiting.pop(0)
+                s = Solve(c, device=self.device, q=c.freestream())
+                if r.precision:   # mode 2: threshold set after the first cycle
+                    s.set_precision(r.precision, 1e-300 if r.precision == 2 else 0.0)
+                active[r.rid] = dict(req=r, case=c, solve=s, mgit=0, need=need, t_admit=clock() - t0,
+                                     first=None)
+                used += need
+            if not active:
+                # idle until the next arrival
+                time.sleep(max(0.0, min(0.001, waiting[0].arrival - now)))
+                continue
+            for a in active.values():
+                a["mgit"] += 1
+                a["solve"].cycle_enqueue(a["mgit"])
+            retire = []
+            for rid, a in active.items():
+                s, r = a["solve"], a["req"]
+                s.cycle_wait()
+                r1 = float(s.history[-1][0])
+                if a["first"] is None:
+                    a["first"] = r1
+                    if r.precision == 2:   # FP64 tail once the rms is down by tail_rms_factor
+                        s.set_precision(2, r.tail_rms_factor * r1)
+                conv = r1 <= r.drop * a["first"]
+                if conv or a["mgit"] >= r.max_cycles:
+                    retire.append((rid, conv))
+            for rid, conv in retire:
+                a = active.pop(rid)
+                s, r, c = a["solve"], a["req"], a["case"]
+                t_done = clock() - t0
+                f = s.force(nsurface=1, include_inviscid=True)[0]
+                q = s.download("q") if self.keep_state else None
+                upd = s.cell_updates_per_cycle() * a["mgit"]
+                self.results.append(Result(rid=rid, cells=c.ncell() * c.nsnap, cycles=a["mgit"], converged=conv,
+                                           t_admit=a["t_admit"], t_done=t_done, latency=t_done - r.arrival,
+                                           resrms=[float(h[0]) for h in s.history], forces=f, updates=upd, q=q))
+                s.close()
+                used -= a["need"]
+        return self.results
+
+
+def summarize(results: List[Result], wall: float, cell_updates: float) -> dict:
+    lat = np.array([r.latency for r in results])
+    return {"requests": len(results), "converged": int(sum(r.converged for r in results)),
+            "wall_s": round(wall, 3), "solves_per_s": round(len(results) / wall, 3),
+            "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+            "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
+            "mcell_updates_per_s": round(cell_updates / wall / 1e6, 2)}
+
+
+def request_mix(n: int, seed: int = 0, rate: float = 50.0, precision: int = 0, drop: float = 1e-3,
+                grids=("small", "medium", "large"), hb_fraction: float = 0.25) -> List[Request]:
+    """n requests with Poisson arrivals at `rate` per second, grid sizes
+    cycling through `grids`, Mach in [0.3, 0.6], incidence in [0, 3] deg,
+    a fraction of them harmonic-balance pitching with N = 1."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        out.append(Request(rid=i, grid=grids[i % len(grids)], mach=float(rng.uniform(0.3, 0.6)),
+                           alpha_deg=float(rng.uniform(0.0, 3.0)), nharms=1 if rng.random() < hb_fraction else 0,
+                           arrival=t, drop=drop, precision=precision))
+    return out
+
+
+def cell_updates(results: List[Result]) -> float:
+    """Interior (cell, snapshot) RK-stage updates of the solves served."""
+    return float(sum(r.updates for r in results))
+
+
+def sequential(requests: List[Request], device: int = 0) -> List[Result]:
+    """The same requests solved one after another with the plain FP64
+    synchronous path (smooth / restrict / prolong per routine), for
+    comparison."""
+    from .solver import Solve
+    out = []
+    built = {r.rid: r.make_case() for r in requests}
+    t0 = time.perf_counter()
+    for r in sorted(requests, key=lambda r: r.arrival):
+        c = built[r.rid]
+        s = Solve(c, device=device, q=c.freestream())
+        ta = time.perf_counter() - t0
+        first = None
+        conv = False
+        mgit = 0
+        while mgit < r.max_cycles:
+            mgit += 1
+            s.cycle(mgit)
+            r1 = float(s.history[-1][0])
+            first = r1 if first is None else first
+            if r1 <= r.drop * first:
+                conv = True
+                break
+        td = time.perf_counter() - t0
+        out.append(Result(rid=r.rid, cells=c.ncell() * c.nsnap, cycles=mgit, converged=conv, t_admit=ta,
+                          t_done=td, latency=td - r.arrival, resrms=[float(h[0]) for h in s.history],
+                          forces=s.force(nsurface=1, include_inviscid=True)[0],
+                          updates=s.cell_updates_per_cycle() * mgit))
+        s.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--requests", type=int, default=24)
+    ap.add_argument("--rate", type=float, default=50.0, help="mean arrivals per second")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "mixed"], default="mixed")
+    ap.add_argument("--drop", type=float, default=1e-3)
+    ap.add_argument("--budget-gib", type=float, default=64.0)
+    ap.add_argument("--baseline", action="store_true", help="also run the requests sequentially in FP64")
+    a = ap.parse_args()
+    prec = {"fp64": 0, "fp32": 1, "mixed": 2}[a.precision]
+    reqs = request_mix(a.requests, rate=a.rate, precision=prec, drop=a.drop)
+    srv = Server(budget_bytes=int(a.budget_gib * (1 << 30)))
+    t0 = time.perf_counter()
+    res = srv.run(reqs)
+    wall = time.perf_counter() - t0
+    out = {"serving": summarize(res, wall, cell_updates(res)), "precision": a.precision,
+           "drop": a.drop, "rate_per_s": a.rate}
+    if a.baseline:
+        t0 = time.perf_counter()
+        base = sequential(reqs)
+        wb = time.perf_counter() - t0
+        out["sequential_fp64"] = summarize(base, wb, cell_updates(base))
+        cl = {r.rid: r.forces[0, 0] for r in base}
+        cd = {r.rid: r.forces[0, 1] for r in base}
+        out["max_abs_dCL_vs_fp64"] = float(max(abs(r.forces[0, 0] - cl[r.rid]) for r in res))
+        out["max_abs_dCD_vs_fp64"] = float(max(abs(r.forces[0, 1] - cd[r.rid]) for r in res))
+        out["speedup_solves_per_s"] = round(out["serving"]["solves_per_s"] / out["sequential_fp64"]["solves_per_s"],
+                                            2)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -58,7 +58,7 @@ class Solve:
                                                        _lib.iptr(keep[4]), L.ncuts, _lib.iptr(cut)),
                         f"define_level {L.nl}")
             g = c.geom[L.nl - 1]
-            for f in ("si", "sj", "sk", "vol", "xdot", "ydot", "zdot", "dist"):
+            for f in ("si", "sj", "sk", "vol", "xdot", "ydot", "zdot", "dist", "x", "y", "z"):
                 if f in g:
                     self.upload(f, g[f], L.nl)
             if "xd" in g:
@@ -127,6 +127,28 @@ class Solve:
     def prolong(self, nl: int):
         self._check(self.lib.cosahip_prolong(self.h, nl), "prolong")
 
+    def force(self, nsurface: int = 1, include_inviscid: bool = True, lngth=(1.0, 1.0, 1.0),
+              xyzmom=(0.25, 0.0, 0.0), nl: int = 1) -> np.ndarray:
+        """Force and moment coefficients (force_b): array (nsurface, nsnap,
+        12) of cl_p, cd_p, cz_p, cmx_p, cmy_p, cm_p, cl_v, cd_v, cz_v, cmx_v,
+        cmy_v, cm_v.  Q must be conservative with its halos filled (between
+        smooth calls it is)."""
+        ns = self.case.nsnap
+        momr = np.ascontiguousarray(self.case.moment_centres(xyzmom), np.float64)
+        lg = np.ascontiguousarray(lngth, np.float64)
+        out = np.zeros(12 * ns * nsurface)
+        self._check(self.lib.cosahip_force(self.h, nl, nsurface, int(include_inviscid), _lib.dptr(lg),
+                                           _lib.dptr(momr), _lib.dptr(out)), "force")
+        return out.reshape(nsurface, ns, 12)
+
+    def set_precision(self, mode: int, tail_rms: float = 0.0):
+        """0 FP64; 1 FP32 residual / FP64 state; 2 FP32 until the level-1
+        density residual rms drops below tail_rms, FP64 after."""
+        self._check(self.lib.cosahip_set_precision(self.h, int(mode), float(tail_rms)), "set_precision")
+
+    def precision_in_use(self) -> int:
+        return int(self.lib.cosahip_precision_in_use(self.h))
+
     # -- multigrid driver ------------------------------------------------------
     def cycle(self, mgit: int, nstart: int = 1, npre: int = 1, npost: int = 1, ncrs: int = 1) -> bool:
         """One pass of mg's cycle loop (src/cosa.f90:822-912): fine smooth,
@@ -148,6 +170,24 @@ class Solve:
             self.smooth(2, False, False, npost, mgit)
         self.prolong(1)
         return False
+
+    def cycle_enqueue(self, mgit: int):
+        """Put one V-cycle on the device without waiting (graph replay from
+        the second cycle on); pair with cycle_wait."""
+        self._check(self.lib.cosahip_cycle_enqueue(self.h, mgit), "cycle_enqueue")
+
+    def cycle_wait(self) -> bool:
+        """Wait for the enqueued cycle; appends its resrms(1:8) to history and
+        returns the convergence flag."""
+        done = (ctypes.c_int32 * 1)()
+        r = np.zeros(8)
+        self._check(self.lib.cosahip_cycle_wait(self.h, done, _lib.dptr(r)), "cycle_wait")
+        self.history.append(r)
+        return bool(done[0])
+
+    def cycle_async(self, mgit: int) -> bool:
+        self.cycle_enqueue(mgit)
+        return self.cycle_wait()
 
     def mg(self, ncycle: Optional[int] = None, nstart: int = 1, npre: int = 1, npost: int = 1, ncrs: int = 1,
            on_cycle=None) -> int:

@@ -141,7 +141,12 @@ enum {
   COSAHIP_DIVVEL = 27,  /* velocity divergence (P3 scalar, 0:2N)           */
   COSAHIP_DELPLS = 28,  /* 2/3 max(div u, 0) (P3 scalar, 0:2N)             */
   COSAHIP_DTVOLT = 29,  /* k-omega dt/vol (P1, 0:2N)                       */
-  COSAHIP_NFIELDS = 30
+  COSAHIP_X = 30,       /* node coordinates (P2 scalar, 0:2N*hbmove), halo nodes
+                           as cordex_b fills them (src/grid.f90:23-78); read
+                           only by cosahip_force                            */
+  COSAHIP_Y = 31,
+  COSAHIP_Z = 32,
+  COSAHIP_NFIELDS = 33
 };
 
 typedef struct cosahip_solve cosahip_solve;
@@ -244,6 +249,59 @@ int cosahip_smooth(cosahip_solve* s, int nl, int from_fine, int to_coarse,
 int cosahip_restrict(cosahip_solve* s, int nl);
 /* prolong (src/cosa.f90:3167-3249) correction from nl+1 onto nl.         */
 int cosahip_prolong(cosahip_solve* s, int nl);
+
+/* ---- asynchronous cycles (no counterpart: the reference's mg loop,
+ * src/cosa.f90:822-912, runs on the host one routine at a time) ---------
+ * cosahip_cycle_enqueue puts one whole V-cycle of mg (nstart = npre = npost
+ * = ncrs = 1: fine smooth, restrict, coarse smooths, prolongs) on the
+ * solve's stream and returns without waiting.  From the second cycle on the
+ * launch sequence is captured once as a HIP graph and replayed (recaptured
+ * when the residual precision changes; not while timing is enabled;
+ * COSAHIP_GRAPH=0 at create time disables it).  cosahip_cycle_wait waits
+ * for it and returns the cycle's level-1 resrms(1:8), *done = max(resrms(1:5))
+ * < toler, and COSAHIP_ERR_POSITIVITY (cosahip_error_info) when prtest
+ * failed on any level.  Differences from calling smooth/restrict/prolong:
+ * the norms are evaluated every cycle, the convergence test and the mode-2
+ * precision switch are applied after the whole cycle.  Solves on different
+ * streams run concurrently.                                              */
+int cosahip_cycle_enqueue(cosahip_solve* s, int mgit);
+int cosahip_cycle_wait(cosahip_solve* s, int32_t* done, double* resrms);
+
+/* ---- forces ------------------------------------------------------------
+ * force_b (src/output.f90:4388-4886) summed over the level's blocks as
+ * force does (src/output.f90:4290-4330), aircraft conventions: pressure and
+ * viscous force and moment coefficients of surfaces s = 1..nsurface, where
+ * surface s is the union of the BC patches of type 1500 + s - 1 (viscous
+ * walls; the viscous part only when the run is viscous).  With
+ * include_inviscid != 0 the type-14 walls also count as surface 1 (pressure
+ * part only): an extension without counterpart in the reference, whose
+ * force_b skips them and so reports zero forces for Euler runs.
+ * Reads Q (conservative, halos filled as after a completed stage), SI/SJ/SK,
+ * X/Y/Z, XDOT/YDOT/ZDOT (moving) and XIDERI/ETADERJ/ZETADERK (viscous).
+ * lngth[3]: the reference lengths lngth(1:3); momr[3 * (2N+1)]: the moment
+ * centre (xmomr, ymomr, zmomr) of each snapshot as force_b derives it from
+ * xmom and the motion.  coef(12, 0:2N, nsurface) receives cl_p, cd_p, cz_p,
+ * cmx_p, cmy_p, cm_p, cl_v, cd_v, cz_v, cmx_v, cmy_v, cm_v.                */
+int cosahip_force(cosahip_solve* s, int nl, int nsurface, int include_inviscid,
+                  const double* lngth, const double* momr, double* coef);
+
+/* ---- precision (no counterpart in the reference, which computes in
+ * real(kind=cosa_real), FP64, throughout: src/cosa_precision.f90:17) -----
+ * mode 0: FP64 everywhere (default, the parity mode).
+ * mode 1: the residual (MUSCL, limiters, Roe fluxes, viscous and
+ *         turbulent diffusion fluxes, k-omega sources) is evaluated in
+ *         FP32 from FP32 copies of the primitive state, face metrics,
+ *         metric derivatives and mut; res and its accumulation, the
+ *         conserved state, the stage update, CIRS and the residual norms
+ *         stay FP64.
+ * mode 2: as mode 1 until the level-1 density residual rms reported by
+ *         cosahip_smooth falls below tail_rms, FP64 from then on (FP64
+ *         defect-correction tail).  Calling again restarts the switch.
+ * COSAHIP_ERR_ARG for the per-direction inviscid variant
+ * (COSAHIP_RESIDUAL=split), which has no FP32 form.                      */
+int cosahip_set_precision(cosahip_solve* s, int mode, double tail_rms);
+/* 32 while the next residual is evaluated in FP32, else 64.              */
+int cosahip_precision_in_use(const cosahip_solve* s);
 
 /* ---- instrumentation (no counterpart in the reference, whose only timers
  * are whole-run wall clocks, src/cosa.f90:789-797) ----------------------

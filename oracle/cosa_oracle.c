@@ -1264,6 +1264,154 @@ void or_update_b(double* res, double* q, const double* qold, const double* dtvol
   (a)[((c) - 1) + 3 * ((ll)((i) - 1) + (ll)(g).imax * ((ll)((j) - 1) + (ll)(g).jmax *    \
                                                         ((ll)((k) - 1) + (ll)(g).kmax * (nh))))]
 
+/* ---------------------------------------------------------------------- */
+/* force_b  (src/output.f90:4388-4886), aircraft conventions              */
+void or_force_b(const double* q, const double* x, const double* y, const double* z, const double* xdot,
+                const double* ydot, const double* zdot, const double* si, const double* sj, const double* sk,
+                const double* xideri, const double* etaderj, const double* zetaderk, int nbcs,
+                const int32_t* bctopo, int nsurface, int inv14, const double* lngth, const double* momr,
+                double* coef, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  const double gamma = P.gamma;
+  const int ijkmax[3] = {imax, jmax, kmax};
+  const double mach = P.machfs > 1e-14 ? P.machfs : 1.0;
+  const double alpha = P.alpha, beta = P.beta;
+  /* src/output.f90:4451-4462 */
+  const double ssx = cos(alpha) * cos(beta), ssy = sin(alpha) * cos(beta), ssz = sin(beta);
+  const double den = sqrt(sin(beta) * sin(beta) + cos(alpha) * cos(alpha) * (cos(beta) * cos(beta)));
+  const double aax = -sin(alpha) * cos(alpha) * (cos(beta) * cos(beta)) / den;
+  const double aay = den;
+  const double aaz = sin(alpha) * sin(beta) * cos(beta) / den;
+  const double bbx = -sin(beta) / den, bby = 0.0, bbz = cos(alpha) * cos(beta) / den;
+  for (int n = 0; n < DIM5; n++) {
+    const int nh = n * P.hbmove;
+    const double xmomr = momr[3 * n], ymomr = momr[3 * n + 1], zmomr = momr[3 * n + 2];
+    for (int isurface = 1; isurface <= nsurface; isurface++) {
+      double fp[6] = {0, 0, 0, 0, 0, 0}, fv[6] = {0, 0, 0, 0, 0, 0};
+      for (int ib = 0; ib < nbcs; ib++) {
+        const int32_t* t = bctopo + 10 * ib;
+        const int visc_wall = t[0] == 1500 + (isurface - 1);
+        const int inv_wall = inv14 && isurface == 1 && t[0] == 14;
+        if (!visc_wall && !inv_wall) continue;
+        const int idir = t[1], inrout = t[2];
+        const int istrt[3] = {t[3], t[5], t[7]}, iend[3] = {t[4], t[6], t[8]};
+        int ibcpt, ibcw, ibcm;
+        if (inrout == 1) {
+          ibcpt = 0;
+          ibcm = 1;
+          ibcw = 0;
+        } else {
+          ibcpt = ijkmax[idir - 1];
+          ibcm = ijkmax[idir - 1];
+          ibcw = ijkmax[idir - 1];
+        }
+        const int sgn = inrout == 1 ? 1 : -1; /* (-1)**(1+1/inrout) */
+        const int ioff1 = idir == 1 ? sgn : 0, joff1 = idir == 2 ? sgn : 0, koff1 = idir == 3 ? sgn : 0;
+        const int ioff2 = 2 * ioff1, joff2 = 2 * joff1, koff2 = 2 * koff1;
+        const int ic1 = CYC(idir, 1), ic2 = CYC(idir, 2), ic3 = CYC(idir, 3);
+        const double* sm = idir == 1 ? si : (idir == 2 ? sj : sk);
+        const double* ed = idir == 1 ? xideri : (idir == 2 ? etaderj : zetaderk);
+        for (int pass = 0; pass < 2; pass++) {
+          if (pass == 1 && !(P.viscous && visc_wall)) break;
+          for (int i3 = istrt[ic3 - 1]; i3 <= iend[ic3 - 1]; i3++)
+            for (int i2 = istrt[ic2 - 1]; i2 <= iend[ic2 - 1]; i2++) {
+              const int ibc = ibcpt * KRD(ic1, 1) + i2 * KRD(ic2, 1) + i3 * KRD(ic3, 1);
+              const int jbc = ibcpt * KRD(ic1, 2) + i2 * KRD(ic2, 2) + i3 * KRD(ic3, 2);
+              const int kbc = ibcpt * KRD(ic1, 3) + i2 * KRD(ic2, 3) + i3 * KRD(ic3, 3);
+              const int im = ibcm * KRD(ic1, 1) + i2 * KRD(ic2, 1) + i3 * KRD(ic3, 1);
+              const int jm = ibcm * KRD(ic1, 2) + i2 * KRD(ic2, 2) + i3 * KRD(ic3, 2);
+              const int km = ibcm * KRD(ic1, 3) + i2 * KRD(ic2, 3) + i3 * KRD(ic3, 3);
+              const int iw = ibcw * KRD(ic1, 1) + i2 * KRD(ic2, 1) + i3 * KRD(ic3, 1);
+              const int jw = ibcw * KRD(ic1, 2) + i2 * KRD(ic2, 2) + i3 * KRD(ic3, 2);
+              const int kw = ibcw * KRD(ic1, 3) + i2 * KRD(ic2, 3) + i3 * KRD(ic3, 3);
+              const double nx = M(sm, g, 1, im, jm, km, nh), ny = M(sm, g, 2, im, jm, km, nh);
+              const double nz = M(sm, g, 3, im, jm, km, nh), ds = M(sm, g, 4, im, jm, km, nh);
+              const double kx = sgn * nx, ky = sgn * ny, kz = sgn * nz;
+              const double r1 = Q(q, g, ibc, jbc, kbc, 1, n), r2 = Q(q, g, ibc, jbc, kbc, 2, n);
+              const double r3 = Q(q, g, ibc, jbc, kbc, 3, n), r4 = Q(q, g, ibc, jbc, kbc, 4, n);
+              const double pw = (gamma - 1) * (Q(q, g, ibc, jbc, kbc, 5, n) - r2 * r2 / r1 / 2 - r3 * r3 / r1 / 2 -
+                                               r4 * r4 / r1 / 2);
+              const double xarm = X2(x, g, iw, jw, kw, nh) - xmomr;
+              const double yarm = X2(y, g, iw, jw, kw, nh) - ymomr;
+              const double zarm = X2(z, g, iw, jw, kw, nh) - zmomr;
+              if (pass == 0) { /* src/output.f90:4672-4684 */
+                fp[0] = fp[0] - pw * kx * ds;
+                fp[1] = fp[1] - pw * ky * ds;
+                fp[2] = fp[2] - pw * kz * ds;
+                fp[3] = fp[3] + (ky * zarm - kz * yarm) * pw * ds;
+                fp[4] = fp[4] + (kz * xarm - kx * zarm) * pw * ds;
+                fp[5] = fp[5] + (kx * yarm - ky * xarm) * pw * ds;
+                continue;
+              }
+              /* viscous force, src/output.f90:4727-4850 */
+              const double etax = MD(ed, g, 1, im, jm, km, nh), etay = MD(ed, g, 2, im, jm, km, nh);
+              const double etaz = MD(ed, g, 3, im, jm, km, nh);
+              const double rhow = r1;
+              const double tw = pw * gamma / rhow;
+              const double muw = (P.stemp + 1) / (tw + P.stemp) * pow(tw, 1.5);
+              const int a1 = ibc + ioff1, b1 = jbc + joff1, c1 = kbc + koff1;
+              const int a2 = ibc + ioff2, b2 = jbc + joff2, c2 = kbc + koff2;
+              double uw = Q(q, g, ibc, jbc, kbc, 2, n) / Q(q, g, ibc, jbc, kbc, 1, n);
+              double u1 = Q(q, g, a1, b1, c1, 2, n) / Q(q, g, a1, b1, c1, 1, n);
+              double u2 = Q(q, g, a2, b2, c2, 2, n) / Q(q, g, a2, b2, c2, 1, n);
+              double vw = Q(q, g, ibc, jbc, kbc, 3, n) / Q(q, g, ibc, jbc, kbc, 1, n);
+              double v1 = Q(q, g, a1, b1, c1, 3, n) / Q(q, g, a1, b1, c1, 1, n);
+              double v2 = Q(q, g, a2, b2, c2, 3, n) / Q(q, g, a2, b2, c2, 1, n);
+              double ww = Q(q, g, ibc, jbc, kbc, 4, n) / Q(q, g, ibc, jbc, kbc, 1, n);
+              double w1 = Q(q, g, a1, b1, c1, 4, n) / Q(q, g, a1, b1, c1, 1, n);
+              double w2 = Q(q, g, a2, b2, c2, 4, n) / Q(q, g, a2, b2, c2, 1, n);
+              if (P.moving) {
+                uw = uw - X2(xdot, g, ibc, jbc, kbc, nh);
+                u1 = u1 - X2(xdot, g, a1, b1, c1, nh);
+                u2 = u2 - X2(xdot, g, a2, b2, c2, nh);
+                vw = vw - X2(ydot, g, ibc, jbc, kbc, nh);
+                v1 = v1 - X2(ydot, g, a1, b1, c1, nh);
+                v2 = v2 - X2(ydot, g, a2, b2, c2, nh);
+                ww = ww - X2(zdot, g, ibc, jbc, kbc, nh);
+                w1 = w1 - X2(zdot, g, a1, b1, c1, nh);
+                w2 = w2 - X2(zdot, g, a2, b2, c2, nh);
+              }
+              const double ueta = sgn * (-8 * uw + 9 * u1 - u2) / 3;
+              const double veta = sgn * (-8 * vw + 9 * v1 - v2) / 3;
+              const double weta = sgn * (-8 * ww + 9 * w1 - w2) / 3;
+              const double dudx = ueta * etax, dudy = ueta * etay, dudz = ueta * etaz;
+              const double dvdx = veta * etax, dvdy = veta * etay, dvdz = veta * etaz;
+              const double dwdx = weta * etax, dwdy = weta * etay, dwdz = weta * etaz;
+              const double divv = dudx + dvdy + dwdz;
+              const double txx = 2 * muw * (dudx - divv / 3);
+              const double txy = muw * (dudy + dvdx);
+              const double txz = muw * (dudz + dwdx);
+              const double tyy = 2 * muw * (dvdy - divv / 3);
+              const double tyz = muw * (dvdz + dwdy);
+              const double tzz = 2 * muw * (dwdz - divv / 3);
+              const double mr = P.machfs / P.reyno;
+              fv[0] = fv[0] + mr * ds * (txx * kx + txy * ky + txz * kz);
+              fv[1] = fv[1] + mr * ds * (txy * kx + tyy * ky + tyz * kz);
+              fv[2] = fv[2] + mr * ds * (txz * kx + tyz * ky + tzz * kz);
+              fv[3] = fv[3] + mr * ds * (-(txy * kx + tyy * ky + tyz * kz) * zarm + (txz * kx + tyz * ky + tzz * kz) * yarm);
+              fv[4] = fv[4] + mr * ds * (-(txz * kx + tyz * ky + tzz * kz) * xarm + (txx * kx + txy * ky + txz * kz) * zarm);
+              fv[5] = fv[5] + mr * ds * (-(txx * kx + txy * ky + txz * kz) * yarm + (txy * kx + tyy * ky + tyz * kz) * xarm);
+            }
+        }
+      }
+      /* src/output.f90:4857-4880 */
+      double* c = coef + 12 * ((ll)n + (ll)DIM5 * (isurface - 1));
+      c[0] = (fp[0] * aax + fp[1] * aay + fp[2] * aaz) * 2 / (lngth[0] * lngth[1] * (mach * mach));
+      c[1] = (fp[0] * ssx + fp[1] * ssy + fp[2] * ssz) * 2 / (lngth[0] * lngth[1] * (mach * mach));
+      c[2] = (fp[0] * bbx + fp[1] * bby + fp[2] * bbz) * 2 / (lngth[0] * lngth[1] * (mach * mach));
+      c[3] = fp[3] * 2 / (lngth[0] * lngth[1] * lngth[2] * (mach * mach));
+      c[4] = fp[4] * 2 / (lngth[0] * lngth[1] * lngth[2] * (mach * mach));
+      c[5] = fp[5] * 2 / (lngth[0] * lngth[1] * lngth[2] * (mach * mach));
+      c[6] = (fv[0] * aax + fv[1] * aay + fv[2] * aaz) * 2 / (lngth[0] * lngth[1] * (mach * mach));
+      c[7] = (fv[0] * ssx + fv[1] * ssy + fv[2] * ssz) * 2 / (lngth[0] * lngth[1] * (mach * mach));
+      c[8] = (fv[0] * bbx + fv[1] * bby + fv[2] * bbz) * 2 / (lngth[0] * lngth[1] * (mach * mach));
+      c[9] = fv[3] * 2 / (lngth[0] * lngth[1] * lngth[2] * (mach * mach));
+      c[10] = fv[4] * 2 / (lngth[0] * lngth[1] * lngth[2] * (mach * mach));
+      c[11] = fv[5] * 2 / (lngth[0] * lngth[1] * lngth[2] * (mach * mach));
+    }
+  }
+}
+
 /* the strong (wall-like) BC types of muscl_b and bq_face/bq_der
  * (src/flux.f90:1320, 1898, 2471)                                         */
 static int wall_like(int t) {

@@ -132,6 +132,19 @@ void or_update_kb(double* res, double* q, const double* qold, const double* vol,
                   int kmax);
 void or_lim_res_ome_b(double* res, const double* q, const double* vol, const double* dtvolt, const double* delpls,
                       const double* prd, double ar, int imax, int jmax, int kmax);
+/* force_b (src/output.f90:4388-4886) on one block: pressure and viscous
+ * force / moment coefficients of surfaces s = 1..nsurface (BC patches of
+ * type 1500 + s - 1; inv14 != 0 also counts type-14 walls as surface 1,
+ * pressure part only).  q conservative, x/y/z node coordinates and
+ * xdot/ydot/zdot grid velocities (0:imax+1,...,0:2N*hbmove), momr the
+ * moment centre (xmomr, ymomr, zmomr) per snapshot, lngth(1:3).
+ * coef(12, 0:2N, nsurface): cl_p cd_p cz_p cmx_p cmy_p cm_p cl_v cd_v cz_v
+ * cmx_v cmy_v cm_v.                                                       */
+void or_force_b(const double* q, const double* x, const double* y, const double* z, const double* xdot,
+                const double* ydot, const double* zdot, const double* si, const double* sj, const double* sk,
+                const double* xideri, const double* etaderj, const double* zetaderk, int nbcs,
+                const int32_t* bctopo, int nsurface, int inv14, const double* lngth, const double* momr,
+                double* coef, int imax, int jmax, int kmax);
 void or_rest4_b(int imax1, int jmax1, int kmax1, int imax2, int jmax2, int kmax2, const double* prd1, double* prd2,
                 const double* vol1, const double* vol2);
 void or_q_bcorns(double* q, int imax, int jmax, int kmax);

@@ -383,4 +383,38 @@ contains
     call rest4_b(i1, j1, k1, i2, j2, k2, nharms, prd1, prd2, vol1, vol2)
   end subroutine
 
+  ! force_b (src/output.f90:4388-4886) for aircraft; pitching about (xh, yh)
+  ! with angles dth(0:2N) when ipitch /= 0.  coef(12, 0:2N, nsurf).
+  subroutine ref_force_b(q, x, y, z, xdot, ydot, zdot, si, sj, sk, xideri, etaderj, zetaderk, nbcs, bctopo, &
+                         nsurf, lng, xyzmom, ipitch, xhh, yhh, dth, coef, imax, jmax, kmax) bind(C, name='ref_force_b')
+    real(c_double) :: q(*), x(*), y(*), z(*), xdot(*), ydot(*), zdot(*), si(*), sj(*), sk(*)
+    real(c_double) :: xideri(*), etaderj(*), zetaderk(*)
+    integer(c_int) :: bctopo(*)
+    integer(c_int), value :: nbcs, nsurf, ipitch, imax, jmax, kmax
+    real(c_double), value :: xhh, yhh
+    real(c_double) :: lng(3), xyzmom(3), dth(*)
+    real(c_double) :: coef(12, 0:2*nharms, nsurf)
+    real(c_double), allocatable :: c(:, :, :)
+    integer :: m, n, is
+    allocate(c(0:2*nharms, nsurf, 12))
+    nsurface = nsurf
+    lngth = lng
+    xmom = xyzmom(1); ymom = xyzmom(2); zmom = xyzmom(3)
+    pitching = ipitch /= 0; plunging = .false.; plupitching = .false.; rotating = .false.
+    xh = xhh; yh = yhh
+    dtheta(0:2*nharms) = dth(1:2*nharms+1)
+    call force_b(q, x, y, z, xdot, ydot, zdot, si, sj, sk, xideri, etaderj, zetaderk, nbcs, bctopo, &
+                 c(:, :, 1), c(:, :, 2), c(:, :, 3), c(:, :, 4), c(:, :, 5), c(:, :, 6), &
+                 c(:, :, 7), c(:, :, 8), c(:, :, 9), c(:, :, 10), c(:, :, 11), c(:, :, 12), &
+                 imax, jmax, kmax, npde, nharms, lmet)
+    do is = 1, nsurf
+      do n = 0, 2*nharms
+        do m = 1, 12
+          coef(m, n, is) = c(n, is, m)
+        end do
+      end do
+    end do
+    deallocate(c)
+  end subroutine
+
 end module ref_shim

@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
 
-from blockgen import all_faces_patches, bctopo_of, box_block, n3, prim_to_cons, rand_prim  # noqa: E402
+from blockgen import Patch, all_faces_patches, bctopo_of, box_block, n3, prim_to_cons, rand_prim  # noqa: E402
 from cosa_amd.config import default_config, hb_operator  # noqa: E402
 from oracle_bind import Ref, dp, ip  # noqa: E402
 
@@ -76,6 +76,30 @@ def cases():
                     dict(q1=rng.uniform(0.5, 1.5, n3(*f) * 5), q2=rng.uniform(-0.1, 0.1, n3(*c) * 5),
                          v1=rng.uniform(0.5, 1.5, (f[0] + 1) * (f[1] + 1) * (f[2] + 1)))))
     out.append(("edges_corners", cfg(), dict(q=rng.uniform(0.5, 1.5, n3(*d) * 5))))
+    # force_b: viscous walls of two surfaces on a pitching harmonic-balance
+    # block (moving metrics, lmet = 7, 3 snapshots), its own generator
+    fr = np.random.default_rng(77)
+    ns = 3
+    n2 = (d[0] + 2) * (d[1] + 2) * (d[2] + 2)
+    n0 = d[0] * d[1] * d[2]
+    mets = []
+    for _ in range(3):
+        sm = fr.uniform(-1, 1, (ns * n2, 7))
+        sm[:, 3] = fr.uniform(0.01, 0.2, ns * n2)
+        mets.append(sm.reshape(-1).copy())
+    fp = [Patch((1, 1, 1), (1, d[1], d[2]), 1500), Patch((d[0], 1, 1), (d[0], d[1], d[2]), 1501),
+          Patch((1, 1, 1), (d[0], 1, d[2]), 1500), Patch((1, d[1], 1), (d[0], d[1], d[2]), 14),
+          Patch((1, 1, 1), (d[0], d[1], 1), 1501), Patch((1, 1, d[2]), (d[0], d[1], d[2]), 80)]
+    out.append(("force_b_pitch_hb1", cfg(nharms=1, moving=1, hbmove=1, lmet=7, viscous=1, reyno=2.0e5,
+                                        stemp=0.383, machfs=0.45, alpha_deg=3.0, beta_deg=1.5),
+                dict(q=prim_to_cons(rand_prim(fr, *d, nsnap=ns), *d, nsnap=ns),
+                     x=fr.uniform(-1, 1, n2 * ns), y=fr.uniform(-1, 1, n2 * ns), z=fr.uniform(-1, 1, n2 * ns),
+                     xdot=fr.uniform(-0.05, 0.05, n2 * ns), ydot=fr.uniform(-0.05, 0.05, n2 * ns),
+                     zdot=fr.uniform(-0.05, 0.05, n2 * ns), si=mets[0], sj=mets[1], sk=mets[2],
+                     xideri=fr.uniform(-3, 3, n0 * 3 * ns), etaderj=fr.uniform(-3, 3, n0 * 3 * ns),
+                     zetaderk=fr.uniform(-3, 3, n0 * 3 * ns), bt=bctopo_of(fp),
+                     lngth=np.array([1.3, 0.9, 0.7]), xyzmom=np.array([0.25, 0.1, -0.05]),
+                     pitch=np.array([0.3, 0.05, 0.05, -0.03, 0.02]))))
     return out
 
 
@@ -167,6 +191,26 @@ def run(lib, kind, name, c, a, oracle):
         q1 = a["q1"].copy()
         (lib.or_prol_b if oracle else lib.ref_prol_b)(bln, *f, *cc, dp(q1), dp(a["q2"]), dp(a["v1"]))
         return dict(q1=q1)
+    if name.startswith("force_b"):
+        import math
+        xh, yh, dth = a["pitch"][0], a["pitch"][1], a["pitch"][2:]
+        xm = a["xyzmom"]
+        coef = np.zeros(12 * len(dth) * 2)
+        arrs = [dp(a[k]) for k in ("q", "x", "y", "z", "xdot", "ydot", "zdot", "si", "sj", "sk", "xideri",
+                                   "etaderj", "zetaderk")]
+        nbcs = a["bt"].size // 10
+        if oracle:
+            momr = []
+            for t in dth:   # moment centre carried with the pitching body (src/output.f90:4476-4486)
+                d1 = (xm[0] - xh) * (math.cos(t) - 1) - (xm[1] - yh) * math.sin(t)
+                d2 = (xm[0] - xh) * math.sin(t) + (xm[1] - yh) * (math.cos(t) - 1)
+                momr += [xm[0] + d1, xm[1] + d2, xm[2]]
+            momr = np.array(momr)
+            lib.or_force_b(*arrs, nbcs, ip(a["bt"]), 2, 0, dp(a["lngth"]), dp(momr), dp(coef), *d)
+        else:
+            lib.ref_force_b(*arrs, nbcs, ip(a["bt"]), 2, dp(a["lngth"]), dp(a["xyzmom"]), 1, ctypes.c_double(xh),
+                            ctypes.c_double(yh), dp(np.ascontiguousarray(dth)), dp(coef), *d)
+        return dict(coef=coef)
     if name == "edges_corners":
         q = a["q"].copy()
         if oracle:

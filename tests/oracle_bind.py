@@ -164,3 +164,42 @@ class OrCase:
 
     def __getitem__(self, nl):
         return self.arrays[nl - 1]
+
+
+def or_force(o: "Oracle", case, q: np.ndarray, nsurface: int = 1, inv14: bool = True, lngth=(1.0, 1.0, 1.0),
+             xyzmom=(0.25, 0.0, 0.0), nl: int = 1) -> np.ndarray:
+    """force (src/output.f90:4290-4330) on the oracle: or_force_b per block
+    of level nl, the block coefficients summed in block order.  q is the
+    level's conservative state; returns (nsurface, nsnap, 12)."""
+    L = case.levels[nl - 1]
+    g = case.geom[nl - 1]
+    c = case.config
+    nsn, nsm, npde, lmet = case.nsnap, case.nsnap_met, case.npde, c["lmet"]
+    o3, o2, o0 = L.offsets(3), L.offsets(2), L.offsets(0)
+    p3, p2, p0 = L.points(3), L.points(2), L.points(0)
+    zero2 = np.zeros(case.size(nl, "x2"))
+    zero0 = np.zeros(case.size(nl, "d0"))
+    xd = g.get("xd")
+    der = (xd[0], xd[4], xd[8]) if xd is not None else (zero0, zero0, zero0)
+    vel = (g.get("xdot", zero2), g.get("ydot", zero2), g.get("zdot", zero2))
+    momr = np.ascontiguousarray(case.moment_centres(xyzmom), np.float64)
+    lg = np.ascontiguousarray(lngth, np.float64)
+    total = np.zeros((nsurface, nsn, 12))
+    bt_off = 0
+    for b in range(L.nblocks):
+        def sl(a, off, size):
+            return np.ascontiguousarray(a[off:off + size])
+        qb = sl(q, o3[b] * npde * nsn, p3[b] * npde * nsn)
+        xyz = [sl(g[k], o2[b] * nsm, p2[b] * nsm) for k in ("x", "y", "z")]
+        v = [sl(a, o2[b] * nsm, p2[b] * nsm) for a in vel]
+        m = [sl(g[k], o2[b] * lmet * nsm, p2[b] * lmet * nsm) for k in ("si", "sj", "sk")]
+        d = [sl(a, o0[b] * 3 * nsm, p0[b] * 3 * nsm) for a in der]
+        nb = int(L.nbcs[b])
+        bt = np.ascontiguousarray(L.bctopo[bt_off:bt_off + 10 * nb], np.int32)
+        bt_off += 10 * nb
+        cb = np.zeros(12 * nsn * nsurface)
+        o.or_force_b(dp(qb), *[dp(a) for a in xyz], *[dp(a) for a in v], *[dp(a) for a in m],
+                     *[dp(a) for a in d], nb, ip(bt), nsurface, int(inv14), dp(lg), dp(momr), dp(cb),
+                     int(L.imax[b]), int(L.jmax[b]), int(L.kmax[b]))
+        total = total + cb.reshape(nsurface, nsn, 12)
+    return total

@@ -15,6 +15,7 @@ import ctypes
 import numpy as np
 
 from cosa_amd import case as cases
+from cosa_amd.config import kw_turbulence
 from oracle_bind import Oracle, OrCase, dp
 
 RTOL = 1e-12     # per-value relative bar (FP64 GPU mode)
@@ -82,4 +83,23 @@ def small_cases():
     c = cases.naca_case(33, 13, 5, nlevel=1, name="naca_pitch_hb1", irsop=2, cfl_irs=3.0, cfl=2.0, nharms=1,
                         harbal=1, omega=0.6, pitch=dict(dtheta0=0.05, xh=0.25, yh=0.0))
     out.append(c)
+    return out
+
+
+TURB = dict(stemp=0.383, ifixq=1, lim_prodtke=1, prdlim=20.0, turcmp=1)
+
+
+def rans_cases():
+    """Viscous parity cases: Wilcox k-omega on the distorted two-block case
+    (viscous walls 1500), laminar Navier-Stokes on the same grid, k-omega on
+    the NACA O-grid with Menter's wall omega."""
+    out = []
+    out.append(cases.two_block_case(ni=13, nj=9, nk=9, nlevel=1, distort=0.1, name="rans_blocks", wall=1500,
+                                    irsop=2, cfl_irs=2.5, cfl=1.5, ratt=1.4, cflt=1.2,
+                                    **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB)))
+    out.append(cases.two_block_case(ni=13, nj=9, nk=9, nlevel=1, distort=0.1, name="laminar_blocks", wall=1500,
+                                    irsop=1, cfl_irs=2.5, cfl=1.5, viscous=1, reyno=500.0, stemp=0.383))
+    out.append(cases.naca_case(33, 13, 9, nlevel=1, name="rans_naca_menter", wall=1500, irsop=3, psirs=0.125,
+                               cfl_irs=2.0, cfl=1.0, ds_wall=2e-3, wallwilc=0,
+                               **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB)))
     return out

@@ -1,0 +1,192 @@
+"""FP32-residual mode (cosahip_set_precision) against the FP64 oracle.
+
+Mode 1 evaluates the residual (MUSCL, limiters, Roe fluxes, viscous and
+turbulent diffusion fluxes, k-omega sources) in FP32 from FP32 copies of
+the primitive state, face metrics, metric derivatives and mut, and keeps
+res, the conserved state, the update, CIRS and the norms in FP64.  Mode 2 switches to FP64 residuals once the density residual rms
+has fallen below a threshold (defect-correction tail).
+
+Stated errors (asserted below, measured values are printed):
+  - one residual evaluation, per equation:
+        max |r32 - r64| <= 1e-4 * max |r64|        (the FP64 oracle's r)
+  - mode 2 after the FP64 tail converges to the FP64 fixed point:
+        max |q_mixed - q_fp64| <= 1e-9 * max |q|   (interior cells) at the same
+    cycle count,
+    both runs with the density residual rms down by more than 1e9
+  - mode 1 alone stalls at an FP32 residual floor; the state it reaches
+    differs from the FP64 solution by less than 1e-5 * max |q|, and its
+    lift and drag coefficients (pressure, type-14 walls) by less than 1e-5;
+    mode 2's by less than 1e-9.
+  - k-omega, 20 iterations from the same state: max |q32 - q64| <= 1e-5 *
+    max |q|, and every resrms within 1e-3 of its FP64 value (relative).
+"""
+import numpy as np
+import pytest
+
+from cosa_amd import _lib
+from cosa_amd import case as cases
+from oracle_bind import dp
+from parity_util import interior_mask, make_pair, rans_cases, small_cases
+
+pytestmark = pytest.mark.gpu
+
+CASES = small_cases() + rans_cases()
+R_BOUND = 1e-4
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("c", CASES, ids=[c.name for c in CASES])
+def test_fp32_residual_error(c):
+    q0 = c.perturbed(amp=0.02, seed=11)
+    _say(f"case {c.name} ncell={c.ncell()} nsnap={c.nsnap}")
+    o, oc, g = make_pair(c, q0)
+    try:
+        g.set_precision(1)
+        assert g.precision_in_use() == 32
+        A = oc[1]
+        cp = oc.ptr()
+        for name in ("cons2prim", "bc", "cutman_q"):
+            getattr(o, "or_" + name)(cp, 1, dp(A["q"]))
+            g.call(name, 1, _lib.Q)
+        if c.config["kom"]:
+            o.or_tur_vis(cp, 1)
+            g.call("tur_vis", 1, _lib.Q)
+        o.or_residual(cp, 1)
+        g.call("residual", 1)
+        m = interior_mask(c)
+        rg, rc = g.download("res"), A["res"]
+        assert np.all(rg[~m] == 0)
+        n3 = rg.size // (c.npde * c.nsnap)
+        # per equation over all blocks and snapshots: the level array is
+        # blocks x snapshots x equations x points
+        worst = 0.0
+        L = c.levels[0]
+        eq_err = np.zeros(c.npde)
+        eq_max = np.zeros(c.npde)
+        off = 0
+        for b in range(L.nblocks):
+            nb = (int(L.imax[b]) + 3) * (int(L.jmax[b]) + 3) * (int(L.kmax[b]) + 3)
+            for n in range(c.nsnap):
+                for v in range(c.npde):
+                    sl = slice(off, off + nb)
+                    mm = m[sl]
+                    eq_err[v] = max(eq_err[v], float(np.max(np.abs(rg[sl][mm] - rc[sl][mm]))))
+                    eq_max[v] = max(eq_max[v], float(np.max(np.abs(rc[sl][mm]))))
+                    off += nb
+        assert off == rg.size, (off, rg.size, n3)
+        rel = eq_err / np.maximum(eq_max, 1e-300)
+        worst = float(rel.max())
+        _say(f"  FP32 residual: max|r32-r64|/max|r64| per equation = {np.array2string(rel, precision=2)}")
+        assert worst <= R_BOUND, rel
+        assert worst > 0, "FP32 path gave FP64-identical results: the FP32 kernel did not run"
+        # back to FP64: the same call is bit-identical to the oracle again
+        g.set_precision(0)
+        assert g.precision_in_use() == 64
+        g.call("residual", 1)
+        assert np.array_equal(g.download("res")[m], rc[m]) or np.allclose(g.download("res")[m], rc[m], rtol=1e-12,
+                                                                            atol=1e-14 * np.abs(rc).max())
+    finally:
+        g.close()
+
+
+def _conv_case():
+    # converges from 1.7e-4 to 5e-15 in 1500 cycles (FP64, oracle)
+    return cases.two_block_case(ni=17, nj=9, nk=9, nlevel=2, distort=0.1, name="two_block_conv", irsop=1,
+                                cfl_irs=4.0, cfl=2.5, ncycle=100000, toler=0.0)
+
+
+def _run(mode, ncyc, tail=0.0):
+    from cosa_amd.solver import Solve
+    c = _conv_case()
+    q0 = c.perturbed(amp=0.01, seed=3)
+    g = Solve(c, device=0, q=q0)
+    try:
+        if mode:
+            g.set_precision(mode, tail)
+        switched = None
+        for mgit in range(1, ncyc + 1):
+            g.cycle(mgit)
+            if mode == 2 and switched is None and g.precision_in_use() == 64:
+                switched = mgit
+        hist = np.array([h[0] for h in g.history])
+        f = g.force(nsurface=1, include_inviscid=True)[0, 0]
+        return g.download("q"), hist, switched, g.precision_in_use(), f
+    finally:
+        g.close()
+
+
+def test_fp32_modes_converge():
+    ncyc = 1500
+    q64, h64, _, p64, f64 = _run(0, ncyc)
+    q32, h32, _, p32, f32 = _run(1, ncyc)
+    tail = 1e-5 * h64[0]
+    qmx, hmx, switched, pmx, fmx = _run(2, ncyc, tail)
+    # interior cells only: halo edge and corner points are never written by
+    # the stage and keep whatever the initial state held there
+    mk = interior_mask(_conv_case())
+    qs = float(np.abs(q64[mk]).max())
+    d32 = float(np.abs(q32 - q64)[mk].max()) / qs
+    dmx = float(np.abs(qmx - q64)[mk].max()) / qs
+    _say(f"  resrms(1): first {h64[0]:.3e}; FP64 last {h64[-1]:.3e} min {h64.min():.3e}")
+    _say(f"  FP32 mode: last {h32[-1]:.3e} min {h32.min():.3e}; max|q32-q64|/max|q| = {d32:.3e}")
+    _say(f"  mixed: switched at cycle {switched}, last {hmx[-1]:.3e}; max|q-q64|/max|q| = {dmx:.3e}")
+    for k in (1, 10, 50, 100, 200, 400, 800, 1200, ncyc):
+        _say(f"    cycle {k:5d}: fp64 {h64[k - 1]:.3e} fp32 {h32[k - 1]:.3e} mixed {hmx[k - 1]:.3e}")
+    assert p64 == 64 and p32 == 32 and pmx == 64
+    assert switched is not None
+    assert h64[-1] < 1e-9 * h64[0] and hmx[-1] < 1e-9 * h64[0]
+    _say(f"  CL_p, CD_p, CM_p: fp64 {f64[[0, 1, 5]]} fp32 {f32[[0, 1, 5]]} mixed {fmx[[0, 1, 5]]}")
+    _say(f"  |dCL|, |dCD| vs fp64: fp32 {abs(f32[0] - f64[0]):.2e} {abs(f32[1] - f64[1]):.2e}; "
+         f"mixed {abs(fmx[0] - f64[0]):.2e} {abs(fmx[1] - f64[1]):.2e}")
+    assert dmx <= 1e-9
+    assert d32 <= 1e-5
+    assert abs(f32[0] - f64[0]) <= 1e-5 and abs(f32[1] - f64[1]) <= 1e-5
+    assert abs(fmx[0] - f64[0]) <= 1e-9 and abs(fmx[1] - f64[1]) <= 1e-9
+
+
+def test_fp32_rans_smooth():
+    """k-omega: 20 pseudo-time iterations in FP32-residual mode stay within
+    1e-5 * max|q| of the FP64 iterations from the same state."""
+    from cosa_amd.solver import Solve
+    c = rans_cases()[2]
+    q0 = c.perturbed(amp=0.01, seed=4)
+    out = []
+    for mode in (0, 1):
+        g = Solve(c, device=0, q=q0)
+        try:
+            g.set_precision(mode)
+            _, r = g.smooth(1, False, False, 20, 1)
+            out.append((g.download("q"), r))
+        finally:
+            g.close()
+    (q64, r64), (q32, r32) = out
+    d = float(np.abs(q32 - q64).max()) / float(np.abs(q64).max())
+    _say(f"  RANS 20 iterations: max|q32-q64|/max|q| = {d:.3e}; resrms64 {r64[:7]} resrms32 {r32[:7]}")
+    assert d <= 1e-5
+    assert np.all(np.abs(r32[:7] - r64[:7]) <= 1e-3 * np.abs(r64[:7]) + 1e-300)
+
+
+def test_precision_rejects_unsupported(monkeypatch):
+    from cosa_amd._lib import CosahipError
+    from cosa_amd.solver import Solve
+    c = cases.naca_case(33, 13, 5, nlevel=1, name="naca_small", irsop=1, cfl_irs=3.0, cfl=2.0)
+    g = Solve(c, device=0, q=c.perturbed(amp=0.01, seed=1))
+    try:
+        with pytest.raises(CosahipError):
+            g.set_precision(3)
+        with pytest.raises(CosahipError):
+            g.set_precision(2, 0.0)
+        assert g.precision_in_use() == 64
+    finally:
+        g.close()
+    monkeypatch.setenv("COSAHIP_RESIDUAL", "split")   # per-direction inviscid kernels: FP64 only
+    g = Solve(c, device=0, q=c.perturbed(amp=0.01, seed=1))
+    try:
+        with pytest.raises(CosahipError):
+            g.set_precision(1)
+        assert g.precision_in_use() == 64
+    finally:
+        g.close()

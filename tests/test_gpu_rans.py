@@ -14,25 +14,9 @@ from cosa_amd import _lib
 from cosa_amd import case as cases
 from cosa_amd.config import kw_turbulence
 from oracle_bind import dp
-from parity_util import interior_mask, make_pair, parity
+from parity_util import TURB, interior_mask, make_pair, parity, rans_cases
 
 pytestmark = pytest.mark.gpu
-
-TURB = dict(stemp=0.383, ifixq=1, lim_prodtke=1, prdlim=20.0, turcmp=1)
-
-
-def rans_cases():
-    out = []
-    out.append(cases.two_block_case(ni=13, nj=9, nk=9, nlevel=1, distort=0.1, name="rans_blocks", wall=1500,
-                                    irsop=2, cfl_irs=2.5, cfl=1.5, ratt=1.4, cflt=1.2,
-                                    **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB)))
-    out.append(cases.two_block_case(ni=13, nj=9, nk=9, nlevel=1, distort=0.1, name="laminar_blocks", wall=1500,
-                                    irsop=1, cfl_irs=2.5, cfl=1.5, viscous=1, reyno=500.0, stemp=0.383))
-    out.append(cases.naca_case(33, 13, 9, nlevel=1, name="rans_naca_menter", wall=1500, irsop=3, psirs=0.125,
-                               cfl_irs=2.0, cfl=1.0, ds_wall=2e-3, wallwilc=0,
-                               **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB)))
-    return out
-
 
 CASES = rans_cases()
 
@@ -163,3 +147,54 @@ def test_rans_multigrid_cycles(name):
             parity(g.download("q", nl), oc[nl]["q"], rtol=1e-10, what=f"q level {nl}")
     finally:
         g.close()
+
+
+def test_viscous_kernel_variants_agree(monkeypatch):
+    """The three arrangements of the viscous face work give bit-identical
+    residuals (same operands, same accumulation order): flux and difference
+    in one kernel per direction (default), fused into the inviscid line march
+    (COSAHIP_VISCOUS=fused), face fluxes stored then differenced
+    (COSAHIP_VISCOUS=faces)."""
+    from cosa_amd.solver import Solve
+    c = CASES[0]
+    q0 = c.perturbed(amp=0.02, seed=13)
+    out = []
+    for mode in ("default", "fused", "faces"):
+        if mode == "default":
+            monkeypatch.delenv("COSAHIP_VISCOUS", raising=False)
+        else:
+            monkeypatch.setenv("COSAHIP_VISCOUS", mode)
+        g = Solve(c, device=0, q=q0)
+        try:
+            for f in ("cons2prim", "bc", "cutman_q"):
+                g.call(f, 1, _lib.Q)
+            g.call("lambda_deltat", 1)
+            g.call("residual", 1)
+            out.append(g.download("res"))
+        finally:
+            g.close()
+    m = interior_mask(c)
+    assert np.array_equal(out[0][m], out[1][m])
+    assert np.array_equal(out[0][m], out[2][m])
+
+
+def test_cirs_i_variants_agree(monkeypatch):
+    """CIRS i-sweep: one thread per (line, variable) with k and omega in the
+    same launch (default) and one thread per line with LDS-staged chunks
+    (COSAHIP_CIRS_I=lds) give bit-identical results."""
+    from cosa_amd.solver import Solve
+    for c in (CASES[0], CASES[2]):
+        q0 = c.perturbed(amp=0.02, seed=17)
+        res = []
+        for mode in ("default", "lds"):
+            if mode == "default":
+                monkeypatch.delenv("COSAHIP_CIRS_I", raising=False)
+            else:
+                monkeypatch.setenv("COSAHIP_CIRS_I", mode)
+            g = Solve(c, device=0, q=q0)
+            try:
+                g.smooth(1, False, False, 2, 1)
+                res.append(g.download("q"))
+            finally:
+                g.close()
+        assert np.array_equal(res[0], res[1]), c.name
