@@ -141,6 +141,20 @@ class Solve:
                                            _lib.dptr(momr), _lib.dptr(out)), "force")
         return out.reshape(nsurface, ns, 12)
 
+    def write_restart(self, path: str, nl: int = 1):
+        """The reference's restart file (writerest) of the resident state."""
+        from . import restart
+        kom = bool(self.case.config["kom"])
+        restart.write_restart(path, self.case, self.download("q", nl), self.download("mut", nl) if kom else None, nl)
+
+    def read_restart(self, path: str, nl: int = 1):
+        """Load a restart file (readrest) into the resident state."""
+        from . import restart
+        q, mut = restart.read_restart(path, self.case, nl)
+        self.upload("q", q, nl)
+        if mut is not None:
+            self.upload("mut", mut, nl)
+
     def set_precision(self, mode: int, tail_rms: float = 0.0):
         """0 FP64; 1 FP32 residual / FP64 state; 2 FP32 until the level-1
         density residual rms drops below tail_rms, FP64 after."""

@@ -6,10 +6,10 @@ driver oracle/ref_shim.f90.  TEST INFRASTRUCTURE ONLY.
 Recipe (the reference's own Makefile is not used):
   * cosa_precision, common_variables, cosa_variables, util, flux, src and
     update .f90 compile unmodified with amdflang;
-  * bc.f90, cosa.f90 and output.f90 do not compile as whole files in this
-    image: bc.f90's periodic-cut routines need the parallelutils module,
-    cosa.f90's main program and output.f90's writers need the un-vendored
-    CGNS library.  The subroutines this path
+  * bc.f90, cosa.f90, output.f90 and input.f90 do not compile as whole
+    files in this image: bc.f90's periodic-cut routines need the
+    parallelutils module, cosa.f90's main program and the readers and
+    writers of input.f90 / output.f90 need the un-vendored CGNS library.  The subroutines this path
     needs are self-contained; they are compiled verbatim, selected by name
     into a temporary file under oracle/_ref/ that is deleted right after
     compilation (nothing from the reference is kept or committed);
@@ -37,7 +37,8 @@ BY_NAME = {
     "cosa.f90": ["bresid", "lambdab", "eig_eul", "eig_ns", "eig_eul_p", "eig_ns_p", "bdeltat", "cirs_b",
                  "tridi", "resg_brms", "resg_brms_hb", "rest1_b", "rest2_ho_b", "rest2_lo_b", "prol_lin_b",
                  "prol_bln_b", "rest4_b"],
-    "output.f90": ["force_b"],
+    "output.f90": ["force_b", "write_brest"],
+    "input.f90": ["read_brest"],
 }
 FFLAGS = ["-cpp", "-fdefault-real-8", "-fdefault-double-8", "-O2", "-ffp-contract=off", "-fPIC"]
 

@@ -417,4 +417,40 @@ contains
     deallocate(c)
   end subroutine
 
+  ! writerest / readrest (src/output.f90:3481-3529, src/input.f90:4620-4663)
+  ! over nblocks blocks stored one after another (q: npde*(2N+1) P3 arrays per
+  ! block, mut: 2N+1), calling write_brest / read_brest per block
+  subroutine ref_restart(mode, path, plen, q, mut, nblocks, imax, jmax, kmax) bind(C, name='ref_restart')
+    integer(c_int), value :: mode, plen, nblocks
+    character(kind=c_char) :: path(plen)
+    real(c_double) :: q(*), mut(*)
+    integer(c_int) :: imax(nblocks), jmax(nblocks), kmax(nblocks)
+    character(len=plen) :: fname
+    real(c_double) :: q1(1)
+    integer :: i, ib, iq, imt, n3, fid
+    do i = 1, plen
+      fname(i:i) = path(i)
+    end do
+    dualt = .false.; dual_prop_start = .false.
+    fid = 61
+    if (mode == 0) then
+      open(fid, file=fname, form='unformatted', status='replace')
+    else
+      open(fid, file=fname, form='unformatted', status='old')
+    end if
+    iq = 1
+    imt = 1
+    do ib = 1, nblocks
+      if (mode == 0) then
+        call write_brest(q(iq), q1, mut(imt), imax(ib), jmax(ib), kmax(ib), npde, nharms, fid)
+      else
+        call read_brest(q(iq), q1, mut(imt), imax(ib), jmax(ib), kmax(ib), npde, nharms, fid)
+      end if
+      n3 = (imax(ib) + 3) * (jmax(ib) + 3) * (kmax(ib) + 3)
+      iq = iq + n3 * npde * dim5
+      imt = imt + n3 * dim5
+    end do
+    close(fid)
+  end subroutine
+
 end module ref_shim

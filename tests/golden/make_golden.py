@@ -229,10 +229,26 @@ def set_cfg(lib, c):
     lib.set_config(c, d)
 
 
+def restart_fixture(ref, store):
+    """The reference's restart file (write_brest per block) of the k-omega HB
+    two-block case of tests/test_restart.py, as bytes."""
+    import tempfile
+    sys.path.insert(0, os.path.dirname(HERE))
+    from test_restart import _arrays, _case, _ref_file
+    c = _case(True, 1)
+    ref.set_config(c.config, c.hb_matrix())
+    q, mut = _arrays(c, 14)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "restart")
+        _ref_file(ref, c, p, q, mut, 0)
+        store["restart_kom_hb1/out/bytes"] = np.frombuffer(open(p, "rb").read(), np.uint8)
+
+
 def main():
     ref = Ref()
     store = {}
     names = []
+    restart_fixture(ref, store)
     for name, c, a in cases():
         set_cfg(ref, c)
         outs = run(ref, None, name, c, a, oracle=False)
