@@ -119,34 +119,40 @@ def make_case(workload: str):
     raise SystemExit(f"unknown workload {workload}")
 
 
-def cpu_baseline(case_full, budget_s: float):
-    """Time the CPU restatement (oracle/liboracle.so, 1 thread) on the first
-    blocks of the same workload, for whole multigrid cycles."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_bind import Oracle, OrCase, dp
-    from cosa_amd.case import Case
-    c0 = case_full
-    if c0.config["kom"]:
-        # the same wing, 4 blocks of 65 x 17 x 17 nodes (65,536 cells)
-        from cosa_amd import case as cases
-        sample = cases.wing_case(65, 17, 17, nblocks=4, nlevel=3, name="sample", **WING_CFG)
-        nb = "4 (65x17x17 nodes each, the wing at 1/64 of the cells)"
-    else:
+def _cpu_sample_case(workload: str):
+    """The CPU sample of a workload: the wing at 1/64 of the cells, or the
+    first uneven channel blocks up to 1.5 M cells."""
+    from cosa_amd import case as cases
+    if workload == "wing4":
+        c = cases.wing_case(65, 17, 17, nblocks=4, nlevel=3, name="sample", **_wing_cfg())
+        return c, "the wing at 1/64 of the cells: 4 blocks of 65x17x17 nodes"
+    if workload == "uneven96":
+        dims = cases.uneven_dims(96, seed=0)
         nb, cells = 0, 0
-        target = 1_500_000
-        for b in c0.blocks:
+        for d in dims:
             nb += 1
-            cells += (b.imax - 1) * (b.jmax - 1) * (b.kmax - 1)
-            if cells >= target:
+            cells += (d[0] - 1) * (d[1] - 1) * (d[2] - 1)
+            if cells >= 1_500_000:
                 break
-        sample = Case(dict(c0.config), c0.blocks[:nb], [], c0.xyz[:nb], name="sample")
+        c = cases.channel_case(dims[:nb], nlevel=3, name="sample", machfs=0.7, irsop=1, cfl=2.0, cfl_irs=4.0,
+                               ilimf=4, epslimf=1e-6, nl_crs=2, ncycle=100000)
+        return c, f"the first {nb} of the 96 uneven blocks"
+    c, _ = make_case(workload)
+    return c, "the whole case"
+
+
+def _cpu_sample_run(workload: str, budget_s: float):
+    """One single-threaded oracle process: whole multigrid cycles of the
+    sample for about budget_s; returns (cell-updates, seconds, cycles,
+    cells, description)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import Oracle, OrCase
+    sample, desc = _cpu_sample_case(workload)
     o = Oracle()
     o.set_config(sample.config)
     oc = OrCase(sample, q1=sample.perturbed(amp=0.01, seed=1))
-    from cosa_amd.solver import Solve  # noqa: F401  (cell-update count helper below)
-    ns, st = 1, sample.config["nstage"]
+    st = sample.config["nstage"]
     per_cycle = st * (sample.ncell(1) + 2 * sample.ncell(2) + sample.ncell(3))
-    hist = np.zeros(8 * 64)
     where = (ctypes.c_int32 * 8)()
     done = (ctypes.c_int32 * 1)()
     t0 = time.perf_counter()
@@ -168,9 +174,27 @@ def cpu_baseline(case_full, budget_s: float):
         if time.perf_counter() - t0 > budget_s or ncyc >= 400:
             break
     dt = time.perf_counter() - t0
-    return {"value": round(per_cycle * ncyc / dt / 1e6, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{nb} of {len(c0.blocks)} blocks ({sample.ncell()} cells), {ncyc} multigrid cycles, {dt:.1f} s, "
-                      f"oracle/liboracle.so (C restatement, gcc -O2, 1 thread)"}
+    return per_cycle * ncyc, dt, ncyc, sample.ncell(), desc
+
+
+def cpu_baseline(workload: str, budget_s: float, workers: int):
+    """The CPU restatement (oracle/liboracle.so, gcc -O2, one thread per
+    process) timed on this host's cores: `workers` independent copies of the
+    sample run at once in fresh processes (the throughput form of the
+    reference's single-rank solves), aggregate cell-updates per second."""
+    import multiprocessing as mp
+    t0 = time.perf_counter()
+    if workers > 1:
+        with mp.get_context("spawn").Pool(workers) as pool:
+            runs = pool.starmap(_cpu_sample_run, [(workload, budget_s)] * workers)
+    else:
+        runs = [_cpu_sample_run(workload, budget_s)]
+    wall = time.perf_counter() - t0
+    rate = sum(u / dt for u, dt, *_ in runs)
+    u, dt, ncyc, cells, desc = runs[0]
+    return {"value": round(rate / 1e6, 4), "unit": "Mcell-updates/s", "cores": workers, "kind": "port",
+            "sample": f"{workers} concurrent single-threaded copies of {desc} ({cells} cells), {ncyc} multigrid "
+                      f"cycles each in {dt:.1f} s ({wall:.0f} s wall), oracle/liboracle.so (C restatement, gcc -O2)"}
 
 
 def copy_calibration():
@@ -196,6 +220,8 @@ def main():
     ap.add_argument("--workload", default="wing4", choices=["wing4", "uneven96", "naca"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1),
+                    help="concurrent single-threaded oracle processes for cpu_baseline (the host's cores, at most 16)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "mixed"],
                     help="fp32: FP32 residual / FP64 state (cosahip_set_precision mode 1); mixed: FP32 until "
                          "resrms(1) < --tail-rms, then FP64 (mode 2)")
@@ -276,7 +302,7 @@ def main():
     cpu = None
     if not a.no_cpu:
         log("cpu baseline (oracle, 1 thread)")
-        cpu = cpu_baseline(case, a.cpu_seconds)
+        cpu = cpu_baseline(a.workload, a.cpu_seconds, a.cpu_workers)
 
     out = {"metric": "Mcell-updates/s", "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(elapsed / a.steps * 1e3, 3),

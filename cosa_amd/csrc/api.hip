@@ -195,7 +195,7 @@ static int check_config(Solve* s, const cosahip_config& c) {
   if (c.lmet != 4 && c.lmet != 7) return fail(s, COSAHIP_ERR_ARG, "lmet must be 4 or 7");
   if (c.moving && c.lmet != 7) return fail(s, COSAHIP_ERR_ARG, "moving grids need lmet = 7");
   if (c.nlevel < 1 || c.nlevel > 4) return fail(s, COSAHIP_ERR_ARG, "nlevel out of range 1..4");
-  if (c.irsop < 0 || c.irsop > 3) return fail(s, COSAHIP_ERR_UNSUPPORTED, "irsop 5/6 (viscous CIRS) not built");
+  if (c.irsop < 0 || c.irsop == 4 || c.irsop > 6) return fail(s, COSAHIP_ERR_ARG, "irsop must be 0, 1, 2, 3, 5 or 6");
   if (c.irsop > 0 && c.cutcirs != 0) return fail(s, COSAHIP_ERR_UNSUPPORTED, "cutcirs = 1 (ccirs) not built");
   if (c.ilimf < 0 || c.ilimf > 4) return fail(s, COSAHIP_ERR_ARG, "ilimf out of range 0..4");
   if (c.nstage < 1 || c.nstage > 5) return fail(s, COSAHIP_ERR_ARG, "nstage out of range 1..5");
@@ -228,10 +228,10 @@ int cosahip_create(const cosahip_config* cfg, int device, cosahip_solve** out) {
   const char* gk = getenv("COSAHIP_GRAPH");
   s->use_graph = !(gk && strcmp(gk, "0") == 0);
   const char* ck = getenv("COSAHIP_CIRS_I");
-  s->cirs_i_lds = ck && strcmp(ck, "lds") == 0;
+  s->cirs_i_mode = ck && strcmp(ck, "lds") == 0 ? 1 : (ck && strcmp(ck, "iv") == 0 ? 2 : 0);
   const char* vk = getenv("COSAHIP_VISCOUS");
   s->fuse_viscous = vk && strcmp(vk, "fused") == 0;
-  s->viscous_faces = vk && strcmp(vk, "faces") == 0;
+  s->viscous_faces = !(vk && (strcmp(vk, "march") == 0 || strcmp(vk, "fused") == 0));
   if (device >= 0) {
     if (hipSetDevice(device) != hipSuccess) {
       delete s;

@@ -190,15 +190,16 @@ struct Solve {
   // residual kernel choice: fused single pass (default) or one pass per
   // direction (COSAHIP_RESIDUAL=split in the environment at create time)
   bool split_residual = false;
-  // viscous face fluxes: flux and difference in one kernel per direction
-  // after the inviscid pass (default), fused into the inviscid line march
-  // (COSAHIP_VISCOUS=fused), or face fluxes stored and differenced by a
-  // second kernel (COSAHIP_VISCOUS=faces), all at create time
+  // viscous face fluxes after the inviscid pass of each direction: face
+  // fluxes stored then differenced (default, measured fastest), flux and
+  // difference in one marching kernel (COSAHIP_VISCOUS=march), or fused into
+  // the inviscid line march (COSAHIP_VISCOUS=fused), all at create time
   bool fuse_viscous = false;
-  bool viscous_faces = false;
-  // CIRS i-sweep: one thread per (line, variable) (default) or one thread per
-  // line with LDS-staged chunks (COSAHIP_CIRS_I=lds at create time)
-  bool cirs_i_lds = false;
+  bool viscous_faces = true;
+  // CIRS i-sweep: one thread per (line, variable) on levels with fewer than
+  // CIRS_IV_MAX_LINES lines along i, one thread per line with LDS-staged
+  // chunks on larger ones; COSAHIP_CIRS_I=lds|iv at create time forces one
+  int cirs_i_mode = 0;   // 0 auto, 1 lds, 2 iv
   // residual precision: 0 FP64, 1 FP32 residual / FP64 state, 2 FP32 until
   // the level-1 density residual rms falls below prec_tail_rms, FP64 after
   int prec_mode = 0;
@@ -235,5 +236,6 @@ struct TimeScope {
 constexpr int TS = 256;        // items per tile, 1-D kernels
 constexpr int ROWSEG = 63;     // cells per i-row segment (64 faces)
 constexpr int MSEG = 16;       // cells per segment of a j / k line march
+constexpr ll CIRS_IV_MAX_LINES = 262144;   // i-lines (x snapshots) of a level below which k_cirs_iv is used
 
 }  // namespace cosahip

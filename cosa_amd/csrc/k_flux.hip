@@ -648,6 +648,7 @@ __global__ void __launch_bounds__(256) k_residual_fused(Par p, int nl, const Blk
 
 // ------------------------------------------------------------------------
 // lambda (inviscid eig_eul) + deltat, one thread per (cell, snapshot)
+template <bool VIS>
 __global__ void __launch_bounds__(256) k_lambda_deltat(Par p, const Blk* __restrict__ blks,
                                                        const Tile* __restrict__ tiles, const double* __restrict__ q,
                                                        const double* __restrict__ si, const double* __restrict__ sj,
@@ -706,7 +707,8 @@ __global__ void __launch_bounds__(256) k_lambda_deltat(Par p, const Blk* __restr
   double lamk = fabs(u * zetax + v * zetay + w * zetaz) + a * azeta;
   const ll po = b.o1 * p.nsnap + (ll)n * n1of(b) + i1(b, i, j, k);
   double sum = lami + lamj + lamk;
-  if (p.viscous) {
+  double lv[3] = {0.0, 0.0, 0.0};
+  if (VIS) {
     // eig_ns viscous radii (src/cosa.f90:1350-1430, 4240-4297); mut only for
     // k-omega; the +1 neighbour's temperature divides by the -1 neighbour's
     // density, as in the reference
@@ -718,7 +720,6 @@ __global__ void __launch_bounds__(256) k_lambda_deltat(Par p, const Blk* __restr
     const double vc = vol[b.o1 + i1(b, i, j, k)];
     const ll st[3] = {1, (ll)(b.imax + 3), (ll)(b.imax + 3) * (b.jmax + 3)};
     const double ar_[3] = {axi, aeta, azeta};
-    double lv[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       const ll cm = c - st[d], cp = c + st[d];
@@ -734,7 +735,9 @@ __global__ void __launch_bounds__(256) k_lambda_deltat(Par p, const Blk* __restr
       const double mutd = (mutm + 2 * mutc + mutp) / 4;
       lv[d] = p.cdff * ar_[d] * ar_[d] * gamma * p.machfs / p.reyno * (mud / p.pranl + mutd / p.prant) / (rho * vc);
     }
-    sum = sum + lv[0] + lv[1] + lv[2];
+    // irsop 5/6 take the viscous radii into the smoothing, not the time
+    // step (src/cosa.f90:4831-4890)
+    if (p.irsop != 5 && p.irsop != 6) sum = sum + lv[0] + lv[1] + lv[2];
   }
   dtvol[po] = p.cfl / sum;
   if (p.kom) dtvolt[po] = p.cflt / p.cfl * dtvol[po];
@@ -743,7 +746,7 @@ __global__ void __launch_bounds__(256) k_lambda_deltat(Par p, const Blk* __restr
     double* ox = pass == 0 ? bx : tbx;
     double* oe = pass == 0 ? be : tbe;
     double* oz = pass == 0 ? bz : tbz;
-    if (p.irsop == 2) {
+    if (p.irsop == 2 || p.irsop == 5) {
       const double zeta = 0.4;
       double phixi = 1 + pow(lamj / lami, zeta) + pow(lamk / lami, zeta);
       double phieta = 1 + pow(lami / lamj, zeta) + pow(lamk / lamj, zeta);
@@ -755,7 +758,13 @@ __global__ void __launch_bounds__(256) k_lambda_deltat(Par p, const Blk* __restr
       oe[po] = fmax(0.0, (tt * tt - 1) / 4);
       tt = rr * lamk * phizeta / (lami + lamj + lamk);
       oz[po] = fmax(0.0, (tt * tt - 1) / 4);
-    } else if (p.irsop == 3) {
+      if (p.irsop == 5) {   // cirs_v1v: at least c1/4 lambda_v / sum(lambda) (src/cosa.f90:4981-5060)
+        const double c1 = 40.0;
+        ox[po] = fmax(ox[po], c1 / 4 * lv[0] / (lami + lamj + lamk));
+        oe[po] = fmax(oe[po], c1 / 4 * lv[1] / (lami + lamj + lamk));
+        oz[po] = fmax(oz[po], c1 / 4 * lv[2] / (lami + lamj + lamk));
+      }
+    } else if (p.irsop == 3 || p.irsop == 6) {
       double rji = lamj / lami, rki = lamk / lami, rij = lami / lamj, rkj = lamk / lamj, rik = lami / lamk,
              rjk = lamj / lamk;
       double tt;
@@ -765,97 +774,28 @@ __global__ void __launch_bounds__(256) k_lambda_deltat(Par p, const Blk* __restr
       oe[po] = fmax(0.0, 0.25 * (tt * tt - 1));
       tt = rr / (1 + p.psirs * (rik + rjk));
       oz[po] = fmax(0.0, 0.25 * (tt * tt - 1));
+      if (p.irsop == 6) {   // cirs_v2v (src/cosa.f90:4896-4975)
+        const double c1 = 40.0;
+        ox[po] = fmax(ox[po], 0.25 * c1 * lv[0] / (lami + lamj + lamk));
+        oe[po] = fmax(oe[po], 0.25 * c1 * lv[1] / (lami + lamj + lamk));
+        oz[po] = fmax(oz[po], 0.25 * c1 * lv[2] / (lami + lamj + lamk));
+      }
     }
   }
 }
 
 // ------------------------------------------------------------------------
-// CIRS: one thread per (line, snapshot) solving the tridiagonal systems of
-// NV variables v0..v0+NV-1 with the reference's Thomas recurrence; r(j) is
-// kept in a per-cell scratch, t(j) in place in res.  The first sweep
-// scales res by dt (dtvol, or dtvolt for k, omega), the last divides it
-// back (cirs_b order); epsc is the constant coefficient of irsop = 1.
-template <int DIR, int NV>
-__global__ void __launch_bounds__(256) k_cirs(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
-                                              double* __restrict__ res, const double* __restrict__ dtvol,
-                                              const double* __restrict__ beta, double* __restrict__ rsc, int first,
-                                              int last, int v0, double epsc) {
-  const Tile t = tiles[blockIdx.x];
-  const Blk b = blks[t.block];
-  const int item = t.base + threadIdx.x;
-  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
-  const int T1 = DIR == 1 ? jm : im;
-  const int T2 = DIR == 3 ? jm : km;
-  const int nlines = T1 * T2;
-  if (item >= nlines * p.nsnap) return;
-  const int n = item / nlines;
-  const int rem = item - n * nlines;
-  const int t1 = rem % T1 + 1, t2 = rem / T1 + 1;
-  const int L = DIR == 1 ? im : (DIR == 2 ? jm : km);
-  const ll n3 = n3of(b);
-  double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n + n3 * v0;
-  double* rs = rsc + b.o3 * p.nsnap + n3 * (ll)n;
-  const double* dtb = dtvol + b.o1 * p.nsnap + n1of(b) * (ll)n;
-  const double* bb = beta + b.o1 * p.nsnap + n1of(b) * (ll)n;
-  double tp[NV];
-  double rp = 0.0;
-  for (int a = 1; a <= L; a++) {
-    const int i = DIR == 1 ? a : t1, j = DIR == 2 ? a : (DIR == 1 ? t1 : t2), k = DIR == 3 ? a : t2;
-    const ll c3 = i3(b, i, j, k), c1 = i1(b, i, j, k);
-    const double eps = p.irsop == 1 ? epsc : bb[c1];
-    const double ca = a == 1 ? 0.0 : -eps;
-    const double cb = 1 + 2 * eps;
-    const double cc = a == L ? 0.0 : -eps;
-    const double dtv = first ? dtb[c1] : 1.0;
-    double x[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) x[v] = first ? rb[c3 + n3 * v] * dtv : rb[c3 + n3 * v];
-    double r;
-    if (a == 1) {
-      r = cc / cb;
-#pragma unroll
-      for (int v = 0; v < NV; v++) tp[v] = x[v] / cb;
-    } else {
-      r = cc / (cb - rp * ca);
-#pragma unroll
-      for (int v = 0; v < NV; v++) tp[v] = (x[v] - tp[v] * ca) / (cb - rp * ca);
-    }
-    rp = r;
-    rs[c3] = r;
-#pragma unroll
-    for (int v = 0; v < NV; v++) rb[c3 + n3 * v] = tp[v];
-  }
-  double xn[NV];
-  for (int a = L; a >= 1; a--) {
-    const int i = DIR == 1 ? a : t1, j = DIR == 2 ? a : (DIR == 1 ? t1 : t2), k = DIR == 3 ? a : t2;
-    const ll c3 = i3(b, i, j, k);
-    double xv[NV];
-    if (a == L) {
-#pragma unroll
-      for (int v = 0; v < NV; v++) xv[v] = rb[c3 + n3 * v];
-    } else {
-      const double r = rs[c3];
-#pragma unroll
-      for (int v = 0; v < NV; v++) xv[v] = rb[c3 + n3 * v] - xn[v] * r;
-    }
-#pragma unroll
-    for (int v = 0; v < NV; v++) xn[v] = xv[v];
-    if (last) {
-      const double dtv = dtb[i1(b, i, j, k)];
-#pragma unroll
-      for (int v = 0; v < NV; v++) rb[c3 + n3 * v] = xv[v] / dtv;
-    } else {
-#pragma unroll
-      for (int v = 0; v < NV; v++) rb[c3 + n3 * v] = xv[v];
-    }
-  }
-}
+// CIRS (cirs_b, src/cosa.f90:5238-5623): tridiagonal line solves along i,
+// j, k with the reference's Thomas recurrence (tridi); the multipliers r(j)
+// go to a per-cell scratch, t(j) stays in place in res.  The i sweep scales
+// res by the time step (dtvol, or dtvolt for k, omega), the k sweep divides
+// it back; epsc is the constant coefficient of irsop = 1.
 
 // CIRS along i, where each line is contiguous in memory: one wavefront per
 // 64 lines.  Chunks of CI cells of every line (five variables, dtvol and the
 // coefficients) are staged through LDS with coalesced loads and stores; the
 // per-line Thomas recurrence then runs from LDS with the same arithmetic as
-// k_cirs (forward pass r, t; backward pass x).  This is always the first
+// k_cirs_jk (forward pass r, t; backward pass x).  This is always the first
 // sweep: res is scaled by dtvol on load.
 constexpr int CI = 8;     // cells per chunk
 constexpr int LPAD = 65;  // padded line stride in LDS (conflict-free lane access)
@@ -1000,6 +940,130 @@ __global__ void __launch_bounds__(64) k_cirs_i(Par p, const Blk* __restrict__ bl
   }
 }
 
+// CIRS j / k sweep of the mean flow (NM variables: dtvol, beta, rsc) and,
+// k-omega, of k and omega (NT variables: dtvolt, tbeta, rsct) in one pass:
+// one thread per (line, snapshot), each group with its own coefficients,
+// Thomas multipliers and time step, each variable with the operations of
+// tridi (src/cosa.f90:2418-2443).  first: scale by the time step on the way
+// in; last: divide by it on the way out.
+template <int DIR, int NM, int NT>
+__global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
+                                                 double* __restrict__ res, const double* __restrict__ dtvol,
+                                                 const double* __restrict__ beta, double* __restrict__ rsc,
+                                                 const double* __restrict__ dtvolt, const double* __restrict__ tbeta,
+                                                 double* __restrict__ rsct, int first, int last, double epsc) {
+  constexpr int NV = NM + NT;
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int item = t.base + threadIdx.x;
+  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
+  const int T1 = im;
+  const int T2 = DIR == 3 ? jm : km;
+  const int nlines = T1 * T2;
+  if (item >= nlines * p.nsnap) return;
+  const int n = item / nlines;
+  const int rem = item - n * nlines;
+  const int t1 = rem % T1 + 1, t2 = rem / T1 + 1;
+  const int L = DIR == 2 ? jm : km;
+  const ll n3 = n3of(b);
+  const bool cconst = p.irsop == 1;
+  double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  double* rs = rsc + b.o3 * p.nsnap + n3 * (ll)n;
+  double* rst = NT ? rsct + b.o3 * p.nsnap + n3 * (ll)n : nullptr;
+  const ll o1 = b.o1 * p.nsnap + n1of(b) * (ll)n;
+  const double* dtb = dtvol + o1;
+  const double* dtt = NT ? dtvolt + o1 : nullptr;
+  const double* bb = cconst ? nullptr : beta + o1;
+  const double* bbt = (cconst || !NT) ? nullptr : tbeta + o1;
+  double tp[NV];
+  double rp = 0.0, rpt = 0.0;
+  for (int a = 1; a <= L; a++) {
+    const int i = t1, j = DIR == 2 ? a : t2, k = DIR == 3 ? a : t2;
+    const ll c3 = i3(b, i, j, k), c1 = i1(b, i, j, k);
+    const double eps = cconst ? epsc : bb[c1];
+    const double ca = a == 1 ? 0.0 : -eps, cb = 1 + 2 * eps, cc = a == L ? 0.0 : -eps;
+    const double dtv = first ? dtb[c1] : 1.0;
+    double r;
+    if (a == 1) {
+      r = cc / cb;
+#pragma unroll
+      for (int v = 0; v < NM; v++) {
+        const double x = first ? rb[c3 + n3 * v] * dtv : rb[c3 + n3 * v];
+        tp[v] = x / cb;
+      }
+    } else {
+      r = cc / (cb - rp * ca);
+#pragma unroll
+      for (int v = 0; v < NM; v++) {
+        const double x = first ? rb[c3 + n3 * v] * dtv : rb[c3 + n3 * v];
+        tp[v] = (x - tp[v] * ca) / (cb - rp * ca);
+      }
+    }
+    rp = r;
+    rs[c3] = r;
+    if (NT) {
+      const double epst = cconst ? epsc : bbt[c1];
+      const double cat = a == 1 ? 0.0 : -epst, cbt = 1 + 2 * epst, cct = a == L ? 0.0 : -epst;
+      const double dtvt = first ? dtt[c1] : 1.0;
+      double rt;
+      if (a == 1) {
+        rt = cct / cbt;
+#pragma unroll
+        for (int v = NM; v < NV; v++) {
+          const double x = first ? rb[c3 + n3 * v] * dtvt : rb[c3 + n3 * v];
+          tp[v] = x / cbt;
+        }
+      } else {
+        rt = cct / (cbt - rpt * cat);
+#pragma unroll
+        for (int v = NM; v < NV; v++) {
+          const double x = first ? rb[c3 + n3 * v] * dtvt : rb[c3 + n3 * v];
+          tp[v] = (x - tp[v] * cat) / (cbt - rpt * cat);
+        }
+      }
+      rpt = rt;
+      rst[c3] = rt;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) rb[c3 + n3 * v] = tp[v];
+  }
+  double xn[NV];
+  for (int a = L; a >= 1; a--) {
+    const int i = t1, j = DIR == 2 ? a : t2, k = DIR == 3 ? a : t2;
+    const ll c3 = i3(b, i, j, k);
+    double xv[NV];
+    if (a == L) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) xv[v] = rb[c3 + n3 * v];
+    } else {
+      const double r = rs[c3];
+#pragma unroll
+      for (int v = 0; v < NM; v++) xv[v] = rb[c3 + n3 * v] - xn[v] * r;
+      if (NT) {
+        const double rt = rst[c3];
+#pragma unroll
+        for (int v = NM; v < NV; v++) xv[v] = rb[c3 + n3 * v] - xn[v] * rt;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) xn[v] = xv[v];
+    if (last) {
+      const ll c1 = i1(b, i, j, k);
+      const double dtv = dtb[c1];
+#pragma unroll
+      for (int v = 0; v < NM; v++) rb[c3 + n3 * v] = xv[v] / dtv;
+      if (NT) {
+        const double dtvt = dtt[c1];
+#pragma unroll
+        for (int v = NM; v < NV; v++) rb[c3 + n3 * v] = xv[v] / dtvt;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; v++) rb[c3 + n3 * v] = xv[v];
+    }
+  }
+}
+
 // CIRS i-sweep with one thread per (line, variable): a 256-thread
 // workgroup takes 256 / NV lines, NV = npde variables each, so k and omega
 // are smoothed in the same launch as the mean flow.  Mean-flow variables
@@ -1112,7 +1176,8 @@ static inline dim3 grid1(int n) { return dim3(n > 0 ? n : 1); }
 
 hipError_t launch_lambda_deltat(Solve& s, Level& L) {
   if (!L.ntile[TK_CELLS]) return hipSuccess;
-  hipLaunchKernelGGL(k_lambda_deltat, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk,
+  hipLaunchKernelGGL(s.par.viscous ? k_lambda_deltat<true> : k_lambda_deltat<false>, grid1(L.ntile[TK_CELLS]),
+                     dim3(TS), 0, s.stream, s.par, L.dblk,
                      L.tile[TK_CELLS], L.f[COSAHIP_Q], L.f[COSAHIP_SI], L.f[COSAHIP_SJ], L.f[COSAHIP_SK],
                      L.f[COSAHIP_XDOT], L.f[COSAHIP_YDOT], L.f[COSAHIP_ZDOT], L.f[COSAHIP_DTVOL],
                      L.f[COSAHIP_BETAXI], L.f[COSAHIP_BETAETA], L.f[COSAHIP_BETAZETA], L.f[COSAHIP_VOL],
@@ -1206,11 +1271,13 @@ template hipError_t launch_h0_last<double>(Solve&, Level&, const ResIn<double>&)
 template hipError_t launch_h0_last<float>(Solve&, Level&, const ResIn<float>&);
 
 hipError_t launch_cirs(Solve& s, Level& L) {
-  if (s.par.irsop < 1 || s.par.irsop > 3) return hipSuccess;
+  if (s.par.irsop < 1 || s.par.irsop == 4 || s.par.irsop > 6) return hipSuccess;
   double* res = L.f[COSAHIP_RES];
   const double* dt = L.f[COSAHIP_DTVOL];
   const double epsc = 0.25 * (s.par.rat * s.par.rat - 1.0);
-  const bool iv = !s.cirs_i_lds;
+  ll ilines = 0;
+  for (const Blk& b : L.hblk) ilines += (ll)(b.jmax - 1) * (b.kmax - 1) * s.par.nsnap;
+  const bool iv = s.cirs_i_mode == 2 || (s.cirs_i_mode == 0 && ilines < CIRS_IV_MAX_LINES);
   if (iv && L.ntile[TK_LINES_IV]) {
     if (s.par.npde == 7)
       hipLaunchKernelGGL(k_cirs_iv<7>, grid1(L.ntile[TK_LINES_IV]), dim3(256), 0, s.stream, s.par, L.dblk,
@@ -1224,25 +1291,29 @@ hipError_t launch_cirs(Solve& s, Level& L) {
   if (!iv && L.ntile[TK_LINES_I])
     hipLaunchKernelGGL(k_cirs_i<5>, grid1(L.ntile[TK_LINES_I]), dim3(64), 0, s.stream, s.par, L.dblk,
                        L.tile[TK_LINES_I], res, dt, L.f[COSAHIP_BETAXI], L.cirs_r, 0, epsc);
-  if (L.ntile[TK_LINES_J])
-    hipLaunchKernelGGL((k_cirs<2, 5>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                       L.tile[TK_LINES_J], res, dt, L.f[COSAHIP_BETAETA], L.cirs_r, 0, 0, 0, epsc);
-  if (L.ntile[TK_LINES_K])
-    hipLaunchKernelGGL((k_cirs<3, 5>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                       L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, 0, 1, 0, epsc);
+  if (s.par.kom && !iv && L.ntile[TK_LINES_I])
+    hipLaunchKernelGGL(k_cirs_i<2>, grid1(L.ntile[TK_LINES_I]), dim3(64), 0, s.stream, s.par, L.dblk,
+                       L.tile[TK_LINES_I], res, L.f[COSAHIP_DTVOLT], L.tbeta[0], L.cirs_r, 5, epsc);
+  // j and k sweeps: mean flow and k-omega in one launch per direction
+  const double* dtt = L.f[COSAHIP_DTVOLT];
   if (s.par.kom) {
-    // k, omega: scaled by dtvolt; irsop 1 uses the flow's constant
-    // coefficient, irsop 2/3 the turbulence coefficients tbeta*
-    const double* dtt = L.f[COSAHIP_DTVOLT];
-    if (!iv && L.ntile[TK_LINES_I])
-      hipLaunchKernelGGL(k_cirs_i<2>, grid1(L.ntile[TK_LINES_I]), dim3(64), 0, s.stream, s.par, L.dblk,
-                         L.tile[TK_LINES_I], res, dtt, L.tbeta[0], L.cirs_r, 5, epsc);
     if (L.ntile[TK_LINES_J])
-      hipLaunchKernelGGL((k_cirs<2, 2>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                         L.tile[TK_LINES_J], res, dtt, L.tbeta[1], L.cirs_r, 0, 0, 5, epsc);
+      hipLaunchKernelGGL((k_cirs_jk<2, 5, 2>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                         L.tile[TK_LINES_J], res, dt, L.f[COSAHIP_BETAETA], L.cirs_r, dtt, L.tbeta[1], L.cirs_rt, 0, 0,
+                         epsc);
     if (L.ntile[TK_LINES_K])
-      hipLaunchKernelGGL((k_cirs<3, 2>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                         L.tile[TK_LINES_K], res, dtt, L.tbeta[2], L.cirs_r, 0, 1, 5, epsc);
+      hipLaunchKernelGGL((k_cirs_jk<3, 5, 2>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                         L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, dtt, L.tbeta[2], L.cirs_rt, 0, 1,
+                         epsc);
+  } else {
+    if (L.ntile[TK_LINES_J])
+      hipLaunchKernelGGL((k_cirs_jk<2, 5, 0>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                         L.tile[TK_LINES_J], res, dt, L.f[COSAHIP_BETAETA], L.cirs_r, nullptr, nullptr, nullptr, 0, 0,
+                         epsc);
+    if (L.ntile[TK_LINES_K])
+      hipLaunchKernelGGL((k_cirs_jk<3, 5, 0>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                         L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, nullptr, nullptr, nullptr, 0, 1,
+                         epsc);
   }
   return hipGetLastError();
 }

@@ -650,7 +650,8 @@ void or_bdeltat_v(double* dtvol, double* dtvolt, const double* lambdas, const do
       for (int j = 1; j <= jmax - 1; j++)
         for (int i = 1; i <= imax - 1; i++) {
           double sum = Q(lambdas, g, i, j, k, 1, n) + Q(lambdas, g, i, j, k, 2, n) + Q(lambdas, g, i, j, k, 3, n);
-          if (lamv)
+          /* irsop 5/6: viscous radii in the smoothing only (src/cosa.f90:4831-4890) */
+          if (lamv && P.irsop != 5 && P.irsop != 6)
             sum = sum + Q(lamv, g, i, j, k, 1, n) + Q(lamv, g, i, j, k, 2, n) + Q(lamv, g, i, j, k, 3, n);
           P1(dtvol, g, i, j, k, n) = P.cfl / sum;
         }
@@ -664,7 +665,7 @@ void or_bdeltat_v(double* dtvol, double* dtvolt, const double* lambdas, const do
     double* bx_ = pass == 0 ? betaxi : tbetaxi;
     double* by_ = pass == 0 ? betaeta : tbetaeta;
     double* bz_ = pass == 0 ? betazeta : tbetazeta;
-    if (P.irsop == 2) {
+    if (P.irsop == 2 || P.irsop == 5) {
       for (int n = 0; n <= 2 * NH; n++)
         for (int k = 1; k <= kmax - 1; k++)
           for (int j = 1; j <= jmax - 1; j++)
@@ -681,8 +682,16 @@ void or_bdeltat_v(double* dtvol, double* dtvolt, const double* lambdas, const do
               P1(by_, g, i, j, k, n) = fmax(0.0, (t * t - 1) / 4);
               t = rr * lamk * phizeta / (lami + lamj + lamk);
               P1(bz_, g, i, j, k, n) = fmax(0.0, (t * t - 1) / 4);
+              if (P.irsop == 5) { /* cirs_v1v, src/cosa.f90:4981-5060 */
+                const double c1 = 40.0;
+                double lv1 = lamv ? Q(lamv, g, i, j, k, 1, n) : 0.0, lv2 = lamv ? Q(lamv, g, i, j, k, 2, n) : 0.0,
+                       lv3 = lamv ? Q(lamv, g, i, j, k, 3, n) : 0.0;
+                P1(bx_, g, i, j, k, n) = fmax(P1(bx_, g, i, j, k, n), c1 / 4 * lv1 / (lami + lamj + lamk));
+                P1(by_, g, i, j, k, n) = fmax(P1(by_, g, i, j, k, n), c1 / 4 * lv2 / (lami + lamj + lamk));
+                P1(bz_, g, i, j, k, n) = fmax(P1(bz_, g, i, j, k, n), c1 / 4 * lv3 / (lami + lamj + lamk));
+              }
             }
-    } else if (P.irsop == 3) {
+    } else if (P.irsop == 3 || P.irsop == 6) {
       for (int n = 0; n <= 2 * NH; n++)
         for (int k = 1; k <= kmax - 1; k++)
           for (int j = 1; j <= jmax - 1; j++)
@@ -698,6 +707,14 @@ void or_bdeltat_v(double* dtvol, double* dtvolt, const double* lambdas, const do
               P1(by_, g, i, j, k, n) = fmax(0.0, 0.25 * (t * t - 1));
               t = rr / (1 + P.psirs * (rik + rjk));
               P1(bz_, g, i, j, k, n) = fmax(0.0, 0.25 * (t * t - 1));
+              if (P.irsop == 6) { /* cirs_v2v, src/cosa.f90:4896-4975 */
+                const double c1 = 40.0;
+                double lv1 = lamv ? Q(lamv, g, i, j, k, 1, n) : 0.0, lv2 = lamv ? Q(lamv, g, i, j, k, 2, n) : 0.0,
+                       lv3 = lamv ? Q(lamv, g, i, j, k, 3, n) : 0.0;
+                P1(bx_, g, i, j, k, n) = fmax(P1(bx_, g, i, j, k, n), 0.25 * c1 * lv1 / (lami + lamj + lamk));
+                P1(by_, g, i, j, k, n) = fmax(P1(by_, g, i, j, k, n), 0.25 * c1 * lv2 / (lami + lamj + lamk));
+                P1(bz_, g, i, j, k, n) = fmax(P1(bz_, g, i, j, k, n), 0.25 * c1 * lv3 / (lami + lamj + lamk));
+              }
             }
     }
   }
@@ -715,7 +732,7 @@ void or_bdeltat(double* dtvol, const double* lambdas, double* betaxi, double* be
         for (int i = 1; i <= imax - 1; i++)
           P1(dtvol, g, i, j, k, n) = P.cfl / (Q(lambdas, g, i, j, k, 1, n) + Q(lambdas, g, i, j, k, 2, n) +
                                               Q(lambdas, g, i, j, k, 3, n));
-  if (P.irsop == 2) {
+  if (P.irsop == 2 || P.irsop == 5) {
     for (int n = 0; n <= 2 * NH; n++)
       for (int k = 1; k <= kmax - 1; k++)
         for (int j = 1; j <= jmax - 1; j++)
@@ -736,7 +753,7 @@ void or_bdeltat(double* dtvol, const double* lambdas, double* betaxi, double* be
             bz = (t * t - 1) / 4;
             P1(betazeta, g, i, j, k, n) = fmax(0.0, bz);
           }
-  } else if (P.irsop == 3) {
+  } else if (P.irsop == 3 || P.irsop == 6) {
     for (int n = 0; n <= 2 * NH; n++)
       for (int k = 1; k <= kmax - 1; k++)
         for (int j = 1; j <= jmax - 1; j++)
@@ -2575,7 +2592,7 @@ int or_smooth(or_case* c, int nl, int from_fine, int to_coarse, int niter, int m
         if (is_kom()) or_turb_limits(c, nl, P.alphas[irk - 1]);
         return 0;
       }
-      if (P.irsop > 0 && P.irsop <= 3) or_cirs(c, nl);
+      if ((P.irsop > 0 && P.irsop <= 3) || P.irsop == 5 || P.irsop == 6) or_cirs(c, nl);
       or_update(c, nl, P.alphas[irk - 1]);
       if (is_kom()) {
         if (P.ifixq == 1) or_fixq(c, nl);

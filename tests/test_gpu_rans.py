@@ -151,15 +151,15 @@ def test_rans_multigrid_cycles(name):
 
 def test_viscous_kernel_variants_agree(monkeypatch):
     """The three arrangements of the viscous face work give bit-identical
-    residuals (same operands, same accumulation order): flux and difference
-    in one kernel per direction (default), fused into the inviscid line march
-    (COSAHIP_VISCOUS=fused), face fluxes stored then differenced
-    (COSAHIP_VISCOUS=faces)."""
+    residuals (same operands, same accumulation order): face fluxes stored
+    then differenced (default), flux and difference in one marching kernel
+    (COSAHIP_VISCOUS=march), fused into the inviscid line march
+    (COSAHIP_VISCOUS=fused)."""
     from cosa_amd.solver import Solve
     c = CASES[0]
     q0 = c.perturbed(amp=0.02, seed=13)
     out = []
-    for mode in ("default", "fused", "faces"):
+    for mode in ("default", "fused", "march"):
         if mode == "default":
             monkeypatch.delenv("COSAHIP_VISCOUS", raising=False)
         else:
@@ -180,17 +180,15 @@ def test_viscous_kernel_variants_agree(monkeypatch):
 
 def test_cirs_i_variants_agree(monkeypatch):
     """CIRS i-sweep: one thread per (line, variable) with k and omega in the
-    same launch (default) and one thread per line with LDS-staged chunks
-    (COSAHIP_CIRS_I=lds) give bit-identical results."""
+    same launch (COSAHIP_CIRS_I=iv, the default on small levels) and one
+    thread per line with LDS-staged chunks (COSAHIP_CIRS_I=lds) give
+    bit-identical results."""
     from cosa_amd.solver import Solve
     for c in (CASES[0], CASES[2]):
         q0 = c.perturbed(amp=0.02, seed=17)
         res = []
-        for mode in ("default", "lds"):
-            if mode == "default":
-                monkeypatch.delenv("COSAHIP_CIRS_I", raising=False)
-            else:
-                monkeypatch.setenv("COSAHIP_CIRS_I", mode)
+        for mode in ("iv", "lds"):
+            monkeypatch.setenv("COSAHIP_CIRS_I", mode)
             g = Solve(c, device=0, q=q0)
             try:
                 g.smooth(1, False, False, 2, 1)
@@ -198,3 +196,27 @@ def test_cirs_i_variants_agree(monkeypatch):
             finally:
                 g.close()
         assert np.array_equal(res[0], res[1]), c.name
+
+
+@pytest.mark.parametrize("irsop", [5, 6])
+def test_viscous_cirs_irsop(irsop):
+    """cirs_v1v / cirs_v2v (irsop 5 / 6): viscous radii enter the smoothing
+    coefficients instead of the time step; three smooth iterations against
+    the oracle."""
+    c = cases.two_block_case(ni=13, nj=9, nk=9, nlevel=1, distort=0.1, name=f"rans_irsop{irsop}", wall=1500,
+                             irsop=irsop, psirs=0.125, cfl_irs=2.5, cfl=1.5, ratt=1.4, cflt=1.2,
+                             **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB))
+    q0 = c.perturbed(amp=0.02, seed=19)
+    _say(f"case {c.name} ncell={c.ncell()}")
+    o, oc, g = make_pair(c, q0)
+    try:
+        done = (ctypes.c_int32 * 1)()
+        where = (ctypes.c_int32 * 8)()
+        assert o.or_smooth(oc.ptr(), 1, 0, 0, 3, 1, done, where) == 0
+        _, r = g.smooth(1, False, False, 3, 1)
+        parity(g.download("q"), oc[1]["q"], rtol=1e-11, what="smooth q")
+        r_o = np.array([o.or_get_resrms(i) for i in range(1, 9)])
+        parity(r[:8], r_o[:8], rtol=1e-12, what="smooth resrms")
+        parity(g.download("betaxi"), oc[1]["betaxi"], what="betaxi")
+    finally:
+        g.close()

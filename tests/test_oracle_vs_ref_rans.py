@@ -105,7 +105,7 @@ def test_bbc_kom(libs, wallwilc, roughk, types):
 
 
 @pytest.mark.parametrize("kom", [True, False])
-@pytest.mark.parametrize("irsop", [1, 2, 3])
+@pytest.mark.parametrize("irsop", [1, 2, 3, 5, 6])
 def test_lambda_deltat_viscous(libs, kom, irsop):
     o, r = libs
     setcfg(libs, kom=kom, irsop=irsop, cfl=3.0, cflt=1.5, rat=1.7, ratt=1.3, psirs=0.125, cdff=2.0)
@@ -133,6 +133,8 @@ def test_lambda_deltat_viscous(libs, kom, irsop):
     r.ref_bdeltat(dp(rr[0]), dp(rr[1]), dp(lam_r), dp(lv_r), *[dp(x) for x in rr[2:]], *DIMS)
     for k in range(8):
         same(oo[k], rr[k], f"deltat array {k}")
+    if irsop in (5, 6):   # the viscous floor of the smoothing is active somewhere
+        assert np.any(oo[2] > 0)
 
 
 @pytest.mark.parametrize("idir", [1, 2, 3])
