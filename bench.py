@@ -239,6 +239,7 @@ def main():
         copy_calibration()
         return
 
+    from cosa_amd.config import komega
     from cosa_amd.solver import Solve
     t0 = time.time()
     case, cfg_desc = make_case(a.workload)
@@ -276,9 +277,9 @@ def main():
     log(f"elapsed {elapsed:.3f} s, {value:.1f} Mcell-updates/s, resrms(1) {res_hist[0]:.3e} -> {res_hist[-1]:.3e}")
 
     # roofline of the dominant family (most device time in the timed region)
-    BYTES_PER_CELL = BYTES_PER_CELL_RANS if case.config["kom"] else BYTES_PER_CELL_EULER
+    BYTES_PER_CELL = BYTES_PER_CELL_RANS if komega(case.config) else BYTES_PER_CELL_EULER
     if prec_mode and s.precision_in_use() == 32:
-        BYTES_PER_CELL = dict(BYTES_PER_CELL, **(BYTES_PER_CELL_RANS_FP32_RESIDUAL if case.config["kom"]
+        BYTES_PER_CELL = dict(BYTES_PER_CELL, **(BYTES_PER_CELL_RANS_FP32_RESIDUAL if komega(case.config)
                                                  else BYTES_PER_CELL_FP32_RESIDUAL))
     fams = {}
     for name, d in fam.items():

@@ -17,12 +17,12 @@ LIB_PATH = os.path.join(HERE, "libcosahip.so")
 
 # field ids (enum in include/cosahip.h)
 (Q, QOLD, RES, RHS, SI, SJ, SK, VOL, DTVOL, BETAXI, BETAETA, BETAZETA, XDOT, YDOT, ZDOT, MUT, DIST, XIDERI, ETADERI,
- ZETADERI, XIDERJ, ETADERJ, ZETADERJ, XIDERK, ETADERK, ZETADERK, PRD, DIVVEL, DELPLS, DTVOLT, X, Y, Z) = range(33)
+ ZETADERI, XIDERJ, ETADERJ, ZETADERJ, XIDERK, ETADERK, ZETADERK, PRD, DIVVEL, DELPLS, DTVOLT, X, Y, Z, TTRM) = range(34)
 FIELD_IDS = dict(q=Q, qold=QOLD, res=RES, rhs=RHS, si=SI, sj=SJ, sk=SK, vol=VOL, dtvol=DTVOL, betaxi=BETAXI,
                  betaeta=BETAETA, betazeta=BETAZETA, xdot=XDOT, ydot=YDOT, zdot=ZDOT, mut=MUT, dist=DIST,
                  xideri=XIDERI, etaderi=ETADERI, zetaderi=ZETADERI, xiderj=XIDERJ, etaderj=ETADERJ,
                  zetaderj=ZETADERJ, xiderk=XIDERK, etaderk=ETADERK, zetaderk=ZETADERK, prd=PRD, divvel=DIVVEL,
-                 delpls=DELPLS, dtvolt=DTVOLT, x=X, y=Y, z=Z)
+                 delpls=DELPLS, dtvolt=DTVOLT, x=X, y=Y, z=Z, ttrm=TTRM)
 # the nine metric-derivative fields in the order of the reference's arrays
 XD_NAMES = ["xideri", "etaderi", "zetaderi", "xiderj", "etaderj", "zetaderj", "xiderk", "etaderk", "zetaderk"]
 
@@ -90,6 +90,7 @@ def load() -> ctypes.CDLL:
         "cosahip_force": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP], ctypes.c_int),
         "cosahip_cycle_enqueue": ([_VP, ctypes.c_int], ctypes.c_int),
         "cosahip_cycle_wait": ([_VP, _IP, _DP], ctypes.c_int),
+        "cosahip_cycle_ready": ([_VP], ctypes.c_int),
         "cosahip_precision_in_use": ([_VP], ctypes.c_int),
         "cosahip_build_info": ([], ctypes.c_char_p),
         "cosahip_timing_enable": ([_VP, ctypes.c_int], ctypes.c_int),
@@ -114,6 +115,7 @@ def exported_symbols():
             "cosahip_residual", "cosahip_src_hb", "cosahip_resg_rms", "cosahip_add_array", "cosahip_copy_array",
             "cosahip_cirs", "cosahip_update", "cosahip_tur_vis", "cosahip_fixq", "cosahip_turb_limits",
             "cosahip_smooth", "cosahip_restrict", "cosahip_prolong", "cosahip_cycle_enqueue", "cosahip_cycle_wait",
+            "cosahip_cycle_ready",
             "cosahip_force", "cosahip_set_precision",
             "cosahip_precision_in_use",
             "cosahip_build_info", "cosahip_timing_enable", "cosahip_timing_read", "cosahip_timing_reset"]

@@ -34,7 +34,7 @@ _TURB_DBL_FIELDS = ["reyno", "pranl", "prant", "stemp", "cdff", "epst", "cmu", "
 class CosahipConfig(ctypes.Structure):
     _fields_ = ([(n, ctypes.c_int32) for n in _INT_FIELDS]
                 + [("entfx_f_w", ctypes.c_int32 * 3), ("entfx_t_w", ctypes.c_int32 * 3),
-                   ("bln_prol", ctypes.c_int32), ("ho_rest", ctypes.c_int32), ("pad0", ctypes.c_int32)]
+                   ("bln_prol", ctypes.c_int32), ("ho_rest", ctypes.c_int32), ("rkimp", ctypes.c_int32)]
                 + [(n, ctypes.c_double) for n in _DBL_FIELDS]
                 + [("qmin", ctypes.c_double * 7), ("alphas", ctypes.c_double * 5),
                    ("tkefar", ctypes.c_double), ("omefar", ctypes.c_double)]
@@ -49,7 +49,7 @@ def default_config(**kw: Any) -> Dict[str, Any]:
         npde=5, nharms=0, hbmove=0, lmet=4, nlevel=1, nl_crs=4, nstage=4, irsop=1, cutcirs=0, ilimf=4, ilimt=4,
         iupdt=1, iprint=1, ncycle=100, kom=0, kom_bsl=0, kom_sst=0, viscous=0, moving=0, harbal=0, foturb=0,
         efx_setup_f=0, efx_all_f=0, eig_limit_f=0, eig_lim_all_f=0, efx_setup_t=0, efx_all_t=0, eig_limit_t=0,
-        eig_lim_all_t=0, entfx_f_w=(0, 0, 0), entfx_t_w=(0, 0, 0), bln_prol=1, ho_rest=0,
+        eig_lim_all_t=0, entfx_f_w=(0, 0, 0), entfx_t_w=(0, 0, 0), bln_prol=1, ho_rest=0, rkimp=0,
         gamma=gamma, machfs=0.5, alpha=0.0, beta=0.0, rhoinf=1.0, pinf=1.0 / gamma, ainf=1.0,
         cfl=2.0, cflt=2.0, rat=1.0, ratt=1.0, psirs=0.0, rkap=0.333333, epslimf=1e-6, epslimt=1e-6,
         cntrpy_f=0.0, cntrpy_t=0.0, entfxctf_f=0.0, entfxctf_t=0.0, eig_cutoff_f=0.0, eig_cutoff_t=0.0,
@@ -80,6 +80,11 @@ def default_config(**kw: Any) -> Dict[str, Any]:
     return c
 
 
+def komega(c: Dict[str, Any]) -> bool:
+    """Any of the k-omega models (Wilcox, BSL, SST): npde = 7."""
+    return bool(c["kom"] or c["kom_bsl"] or c["kom_sst"])
+
+
 def to_struct(c: Dict[str, Any]) -> CosahipConfig:
     s = CosahipConfig()
     for n in _INT_FIELDS:
@@ -91,6 +96,7 @@ def to_struct(c: Dict[str, Any]) -> CosahipConfig:
         s.entfx_t_w[i] = int(c["entfx_t_w"][i])
     s.bln_prol = int(c["bln_prol"])
     s.ho_rest = int(c["ho_rest"])
+    s.rkimp = int(c["rkimp"])
     for i in range(7):
         s.qmin[i] = float(c["qmin"][i])
     for i in range(5):
@@ -106,14 +112,17 @@ def to_struct(c: Dict[str, Any]) -> CosahipConfig:
     return s
 
 
-def kw_turbulence(machfs: float, reyno: float, tu: float = 1e-4, mutfar: float = 1e-2, **kw) -> Dict[str, Any]:
-    """Wilcox k-omega settings of a viscous run (the 'komega' branch of
-    input, src/input.f90:2695-2720): farfield k from the turbulence
-    intensity squared tu, omega from mut/mu at the farfield mutfar."""
+def kw_turbulence(machfs: float, reyno: float, tu: float = 1e-4, mutfar: float = 1e-2, model: str = "komega",
+                  **kw) -> Dict[str, Any]:
+    """k-omega settings of a viscous run (the 'komega', 'bsl' and 'sst'
+    branches of input, src/input.f90:2580-2610, 2695-2720): farfield k from
+    the turbulence intensity squared tu, omega from mut/mu at the farfield
+    mutfar."""
     tkefar = tu * machfs ** 2
     rhoinf = kw.get("rhoinf", 1.0)
     cmu = kw.get("cmu", 1.0)
-    out = dict(npde=7, kom=1, viscous=1, machfs=machfs, reyno=reyno, tkefar=tkefar,
+    out = dict(npde=7, kom=int(model == "komega"), kom_bsl=int(model == "bsl"), kom_sst=int(model == "sst"),
+               viscous=1, machfs=machfs, reyno=reyno, tkefar=tkefar,
                omefar=cmu * rhoinf * tkefar / mutfar * reyno / machfs)
     out.update(kw)
     return out

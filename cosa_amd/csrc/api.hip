@@ -84,7 +84,7 @@ static void free_level(Level& L) {
     if (L.f[f]) (void)hipFree(L.f[f]);
   void* ptrs[] = {L.dblk, L.dq, L.work1, L.cirs_r, L.cirs_rt, L.rms_part, L.sbc, L.sbc_doff, L.dpatch, L.dcut,
                   L.vflux, L.tbeta[0], L.tbeta[1], L.tbeta[2], L.h0tab, L.dxd, L.q32, L.m32[0], L.m32[1], L.m32[2],
-                  L.dxd32, L.mut32, L.dwall, L.fterms, L.fsums, L.fcoef, L.fgeom};
+                  L.dxd32, L.mut32, L.dwall, L.fterms, L.fsums, L.fcoef, L.fgeom, L.dqc};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (float* p : L.xd32)
@@ -135,7 +135,20 @@ static Par make_par(const cosahip_config& c) {
   p.omega = c.omega;
   p.qmin1 = c.qmin[0];
   p.qmin5 = c.qmin[4];
-  p.kom = c.kom;
+  p.kom = c.kom || c.kom_bsl || c.kom_sst;
+  p.sst = c.kom_sst ? 2 : (c.kom_bsl ? 1 : 0);
+  {
+    const double kappa = 0.41, bstr = 0.09;
+    p.sst_a1 = 0.31;
+    p.sigk1 = c.kom_sst ? 0.85 : 0.5;
+    p.sigk2 = 1.0;
+    p.sigw1 = 0.5;
+    p.sigw2 = 0.856;
+    p.b1 = 3.0 / 40.0;
+    p.b2 = 0.0828;
+    p.g1 = p.b1 / bstr - p.sigw1 * (kappa * kappa) / sqrt(bstr);
+    p.g2 = p.b2 / bstr - p.sigw2 * (kappa * kappa) / sqrt(bstr);
+  }
   p.viscous = c.viscous;
   p.foturb = c.foturb;
   p.ilimt = c.ilimt;
@@ -177,16 +190,21 @@ static Par make_par(const cosahip_config& c) {
   return p;
 }
 
+// any of the k-omega models (Wilcox, BSL, SST)
+static bool komega(const cosahip_config& c) { return c.kom || c.kom_bsl || c.kom_sst; }
+
 static int check_config(Solve* s, const cosahip_config& c) {
-  if (c.kom_bsl || c.kom_sst)
-    return fail(s, COSAHIP_ERR_UNSUPPORTED, "k-omega BSL/SST are not built (Wilcox k-omega and laminar are)");
-  if (c.kom && !c.viscous) return fail(s, COSAHIP_ERR_ARG, "k-omega requires viscous");
-  if (c.npde != (c.kom ? 7 : 5)) return fail(s, COSAHIP_ERR_ARG, "npde must be 5 (Euler, laminar) or 7 (k-omega)");
-  if (c.kom && c.nharms > 0)
-    return fail(s, COSAHIP_ERR_UNSUPPORTED, "k-omega with harmonic balance (lim_res_ome_bhb) is not built");
-  if (c.kom && (c.prd_all_lvls || c.lim_corr[0] || c.lim_corr[1] || c.lim_corr[2]))
+  if ((c.kom != 0) + (c.kom_bsl != 0) + (c.kom_sst != 0) > 1)
+    return fail(s, COSAHIP_ERR_ARG, "at most one of kom, kom_bsl, kom_sst");
+  if (komega(c) && !c.viscous) return fail(s, COSAHIP_ERR_ARG, "k-omega requires viscous");
+  if (c.npde != (komega(c) ? 7 : 5)) return fail(s, COSAHIP_ERR_ARG, "npde must be 5 (Euler, laminar) or 7 (k-omega)");
+  if (c.rkimp && !c.harbal)
+    return fail(s, COSAHIP_ERR_UNSUPPORTED, "rkimp without harmonic balance (dual time stepping) is not built");
+  if (c.rkimp && rkim_group(2 * c.nharms + 1) < 1)
+    return fail(s, COSAHIP_ERR_UNSUPPORTED, "rkimp: 2N+1 snapshots exceed the per-cell LDS system (N <= 40)");
+  if (komega(c) && (c.prd_all_lvls || c.lim_corr[0] || c.lim_corr[1] || c.lim_corr[2]))
     return fail(s, COSAHIP_ERR_UNSUPPORTED, "k-omega prd_all_lvls / lim_corr options are not built");
-  if (c.kom && c.ho_rest) return fail(s, COSAHIP_ERR_UNSUPPORTED, "k-omega with ho_rest (bc_res) is not built");
+  if (komega(c) && c.ho_rest) return fail(s, COSAHIP_ERR_UNSUPPORTED, "k-omega with ho_rest (bc_res) is not built");
   if (c.ilimt < 0 || c.ilimt > 4) return fail(s, COSAHIP_ERR_ARG, "ilimt out of range 0..4");
   if (c.viscous && (c.reyno <= 0 || c.pranl <= 0 || c.prant <= 0))
     return fail(s, COSAHIP_ERR_ARG, "viscous runs need reyno, pranl, prant > 0");
@@ -588,6 +606,12 @@ extern "C" int cosahip_define_level(cosahip_solve* h, int nl, int nblocks, const
     if (hipMalloc(&L.h0tab, (ll)nblocks * p.nsnap * 6 * sizeof(double)) != hipSuccess)
       return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
   }
+  if (p.sst == 2) {
+    const ll nd = L.n3tot * 9 * p.nsnap;
+    if (hipMalloc(&L.dqc, nd * sizeof(double)) != hipSuccess)
+      return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
+    CK(hipMemset(L.dqc, 0, nd * sizeof(double)));
+  }
   if (p.viscous) {
     std::vector<double*> xdp(L.f + COSAHIP_XIDERI, L.f + COSAHIP_XIDERI + 9);
     CK(upload_vec(&L.dxd, xdp));
@@ -626,6 +650,7 @@ static ll field_size(const Solve* s, const Level& L, int field) {
     case COSAHIP_DIST: return p.viscous ? L.n1tot : 0;
     case COSAHIP_DTVOLT: return p.kom ? L.n1tot * p.nsnap : 0;
     case COSAHIP_X: case COSAHIP_Y: case COSAHIP_Z: return L.n2tot * p.nsnap_met;
+    case COSAHIP_TTRM: return p.sst ? L.n3tot * 3 * p.nsnap : 0;
   }
   if (field >= COSAHIP_XIDERI && field <= COSAHIP_ZETADERK) return p.viscous ? L.n0tot * 3 * p.nsnap_met : 0;
   return -1;
@@ -937,11 +962,16 @@ extern "C" int cosahip_src_hb(cosahip_solve* h, int nl) {
   return COSAHIP_OK;
 }
 
+// rkim without IRS on one grid level: src_hb is not added to the residual
+// (src/cosa.f90:1916-1921); the norm adds the HB term itself
+static bool rkim_plain(const cosahip_config& c) { return c.harbal && c.rkimp && c.irsop == 0 && c.nlevel == 1; }
+
 // resg_rms partial sums of level L, copied into pinned slot `slot` (no wait)
 static int rms_enqueue(Solve* s, Level& L, int slot) {
   const int np = s->par.npde;
   if (s->cfg.harbal) {
     CK(launch_copy_interior(*s, L, L.work1, L.f[COSAHIP_RES]));
+    if (rkim_plain(s->cfg)) CK(launch_rms_hb_term(*s, L, L.work1));
     CK(launch_rms(*s, L, L.work1, nullptr, 0));
   } else {
     CK(launch_rms(*s, L, L.f[COSAHIP_RES], L.f[COSAHIP_RHS], 1));
@@ -1060,10 +1090,11 @@ static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter
         CK(launch_cut(*s, L, q));
         CK(launch_halo_f32(*s, L));
       }
-      if (c.kom && mgit == 1 && iter == 1 && irk == 1 && from_fine && !c.prd_all_lvls) {
+      if (komega(c) && mgit == 1 && iter == 1 && irk == 1 && from_fine && !c.prd_all_lvls) {
         // first visit of a coarse level: mut from the restricted state
         TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
-        CK(launch_tur_vis(*s, L, q));
+        if (c.kom_sst) CK(launch_sst_first_mut(*s, L));
+        else CK(launch_tur_vis(*s, L, q));
       }
       {
         TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
@@ -1081,7 +1112,7 @@ static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter
         TimeScope t(*s, COSAHIP_FAM_CONVERT, cs);
         CK(launch_prim2cons(*s, L, q));
       }
-      if (c.harbal) {
+      if (c.harbal && !rkim_plain(c)) {
         TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
         CK(launch_src_hb(*s, L));
       }
@@ -1105,7 +1136,7 @@ static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter
         CK(launch_add_interior(*s, L, L.f[COSAHIP_RES], L.f[COSAHIP_RHS], -1.0));
       }
       if (iter == niter + 1) {
-        if (c.kom) {
+        if (komega(c)) {
           TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
           CK(launch_turb_limits(*s, L, c.alphas[irk - 1]));
         }
@@ -1120,7 +1151,7 @@ static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter
         TimeScope t(*s, COSAHIP_FAM_UPDATE, cs);
         CK(launch_update(*s, L, c.alphas[irk - 1]));
       }
-      if (c.kom) {
+      if (komega(c)) {
         TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
         if (c.ifixq == 1) CK(launch_fixq(*s, L));
         if (irk == c.nstage && (nl == 1 || !c.prd_all_lvls)) CK(launch_tur_vis(*s, L, q));
@@ -1175,7 +1206,7 @@ static int restrict_core(Solve* s, int nl) {
   Level& L2 = s->lev[nl];
   TimeScope t(*s, COSAHIP_FAM_OTHER, L2.ncell * s->par.nsnap);
   CK(launch_rest1(*s, L1, L2, L1.f[COSAHIP_Q], L2.f[COSAHIP_Q]));
-  if (s->cfg.kom && !s->cfg.prd_all_lvls) {
+  if (komega(s->cfg) && !s->cfg.prd_all_lvls) {
     // rest4 of the production, delpls and divergence (src/cosa.f90:2734-2739)
     CK(launch_rest_scalar(*s, L1, L2, L1.f[COSAHIP_PRD], L2.f[COSAHIP_PRD]));
     CK(launch_rest_scalar(*s, L1, L2, L1.f[COSAHIP_DELPLS], L2.f[COSAHIP_DELPLS]));
@@ -1295,6 +1326,18 @@ extern "C" int cosahip_cycle_enqueue(cosahip_solve* h, int mgit) {
   }
   s->cycle_pending = true;
   return COSAHIP_OK;
+}
+
+extern "C" int cosahip_cycle_ready(cosahip_solve* h) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  if (!s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "cycle_ready: no cycle enqueued");
+  CK(hipSetDevice(s->device));
+  const hipError_t e = hipStreamQuery(s->stream);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  CK(e);
+  return 0;
 }
 
 extern "C" int cosahip_cycle_wait(cosahip_solve* h, int32_t* done, double* resrms) {

@@ -48,6 +48,10 @@ struct Par {
   double epslimt, cflt, ratt, cntrpy_t, entfxctf_t, eig_cutoff_t;
   double tkefar, omefar, reyno, pranl, prant, stemp, cdff, epst, cmu, sigk, sigw, bkw, bstrkw, gkw, prdlim, roughk;
   double qmin6, qmin7;
+  // k-omega model: kom above is set for every model; sst = 0 Wilcox, 1 BSL,
+  // 2 SST, with the BSL/SST constants of input (src/input.f90:2523-2541)
+  int sst;
+  double sst_a1, sigk1, sigk2, sigw1, sigw2, b1, b2, g1, g2;
 };
 
 __host__ __device__ inline ll n3of(const Blk& b) { return (ll)(b.imax + 3) * (b.jmax + 3) * (b.kmax + 3); }
@@ -126,6 +130,7 @@ struct Level {
   double* f[COSAHIP_NFIELDS] = {};
   double* dq = nullptr;       // HB term (qhb) and scratch
   double* work1 = nullptr;    // copy of res for the HB norm
+  double* dqc = nullptr;      // SST: (P3, 9, nsnap) cell-centre velocity differences of the stage
   double* cirs_r = nullptr;   // Thomas multipliers r(j), one per cell
   double* cirs_rt = nullptr;  // the same for k, omega (k-omega)
   double* rms_part = nullptr; // per-tile partial sums of squares

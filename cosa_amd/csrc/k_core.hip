@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(256) k_update(Par p, const Blk* __restrict__ b
                                                 const double* __restrict__ qold, const double* __restrict__ dtvol,
                                                 double ar, const double* __restrict__ vol,
                                                 const double* __restrict__ dtvolt, const double* __restrict__ delpls,
-                                                const double* __restrict__ prd) {
+                                                const double* __restrict__ prd, const double* __restrict__ ttrm) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   CellIdx c;
@@ -235,8 +235,10 @@ __global__ void __launch_bounds__(256) k_update(Par p, const Blk* __restrict__ b
     const double tke = qb[x + 5 * n3] / qb[x];
     const double dtt = dtvolt[po];
     const double dtau = dtt * vol[b.o1 + i1(b, c.i, c.j, c.k)];
-    const double gav = p.gkw, bav = p.bkw;
     const ll so = b.o3 * p.nsnap + n3 * (ll)c.n + x;
+    // ttrm(2:3): the constants gkw, bkw of Wilcox k-omega, blended for BSL/SST
+    const double* tb = p.sst ? ttrm + b.o3 * 3 * p.nsnap + n3 * 3 * (ll)c.n + x : nullptr;
+    const double gav = tb ? tb[n3] : p.gkw, bav = tb ? tb[2 * n3] : p.bkw;
     const double dl = delpls[so];
     double rhomega = (ob[x + 6 * n3] - ar * rb[x + 6 * n3] * dtt + ar * dtau * rho * ome * (gav * dl + 2 * bav * ome)) /
                      (1 + ar * dtau * (gav * dl + 2 * bav * ome));
@@ -278,6 +280,263 @@ __global__ void __launch_bounds__(256) k_src_hb(Par p, const Blk* __restrict__ b
       for (int nj = 0; nj < ns; nj++) acc = acc + p.omega * vv * dhb[ni + ns * nj] * qb[stride_n * nj + n3 * v];
       hb[stride_n * ni + n3 * v] = acc;
       rb[stride_n * ni + n3 * v] = rb[stride_n * ni + n3 * v] + acc;
+    }
+}
+
+// ---- harmonic-balance implicit stage, rkim ----------------------------------
+// Per interior cell the 2N+1 snapshots are coupled by the spectral operator:
+// (I d + ar omega dtau D) q = rhs, solved with an LU per cell.  The system
+// of each thread lives in LDS, thread-interleaved (element e of thread t at
+// [e * cb + t]), so one work-group of cb threads holds cb systems.
+
+// DGETRF's elimination (partial pivoting on the first largest |a|, whole
+// rows swapped, multipliers times the reciprocal pivot, rank-1 updates in
+// column order) with the swaps applied to the right-hand sides as they
+// happen, then DGETRS's forward and back substitution by columns
+// (src/update.f90:1460-1478 call LAPACK for this).  a: m x m, b: m x nrhs,
+// column-major with stride cb, both already offset by the thread.
+__device__ void lds_lu_solve(double* a, double* bb, int m, int nrhs, int cb) {
+#define A_(r, c) a[((r) + m * (c)) * cb]
+#define B_(r, c) bb[((r) + m * (c)) * cb]
+  for (int j = 0; j < m; j++) {
+    int pr = j;
+    double amax = fabs(A_(j, j));
+    for (int i = j + 1; i < m; i++) {
+      const double v = fabs(A_(i, j));
+      if (v > amax) {
+        amax = v;
+        pr = i;
+      }
+    }
+    if (A_(pr, j) != 0.0) {
+      if (pr != j) {
+        for (int l = 0; l < m; l++) {
+          const double t = A_(j, l);
+          A_(j, l) = A_(pr, l);
+          A_(pr, l) = t;
+        }
+        for (int l = 0; l < nrhs; l++) {
+          const double t = B_(j, l);
+          B_(j, l) = B_(pr, l);
+          B_(pr, l) = t;
+        }
+      }
+      const double piv = A_(j, j);
+      if (fabs(piv) >= 2.2250738585072014e-308) {
+        const double r = 1.0 / piv;
+        for (int i = j + 1; i < m; i++) A_(i, j) = A_(i, j) * r;
+      } else {
+        for (int i = j + 1; i < m; i++) A_(i, j) = A_(i, j) / piv;
+      }
+    }
+    for (int l = j + 1; l < m; l++) {
+      const double u = A_(j, l);
+      if (u != 0.0)
+        for (int i = j + 1; i < m; i++) A_(i, l) = A_(i, l) - A_(i, j) * u;
+    }
+  }
+  for (int c = 0; c < nrhs; c++) {
+    for (int k = 0; k < m; k++) {
+      const double x = B_(k, c);
+      if (x != 0.0)
+        for (int i = k + 1; i < m; i++) B_(i, c) = B_(i, c) - x * A_(i, k);
+    }
+    for (int k = m - 1; k >= 0; k--)
+      if (B_(k, c) != 0.0) {
+        const double x = B_(k, c) / A_(k, k);
+        B_(k, c) = x;
+        for (int i = 0; i < k; i++) B_(i, c) = B_(i, c) - x * A_(i, k);
+      }
+  }
+}
+
+// cell (i, j, k) of work item `item` of the single-snapshot items of a block
+__device__ __forceinline__ bool cell_item(const Blk& b, int item, int& i, int& j, int& k) {
+  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
+  if (item >= im * jm * km) return false;
+  i = item % im + 1;
+  const int rem = item / im;
+  j = rem % jm + 1;
+  k = rem / jm + 1;
+  return true;
+}
+
+// update_bhb, rkim branch (src/update.f90:1410-1841): rho*omega (floored at
+// rhomegamin), rho*k, then the mean flow with 5 right-hand sides.  plain:
+// no IRS and one grid level, the right-hand side is the explicit stage;
+// otherwise the HB term qhb0 kept by src_hb is taken out of the smoothed
+// residual.  qhb0 is overwritten with the right-hand sides, as in the
+// reference.  Work-group: cb threads, tiles of TS single-snapshot items split
+// in `sub` = TS / cb parts.
+__global__ void __launch_bounds__(64) k_update_rkim(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
+                                                    int sub, int cb, const double* __restrict__ res,
+                                                    double* __restrict__ q, const double* __restrict__ qold,
+                                                    const double* __restrict__ dtvol, double ar,
+                                                    const double* __restrict__ vol, const double* __restrict__ dtvolt,
+                                                    const double* __restrict__ delpls, const double* __restrict__ prd,
+                                                    double* __restrict__ qhb0, const double* __restrict__ dhb,
+                                                    int plain, const double* __restrict__ ttrm) {
+  extern __shared__ double lds[];
+  const Tile t = tiles[blockIdx.x / sub];
+  const Blk b = blks[t.block];
+  int i, j, k;
+  if (!cell_item(b, t.base + (blockIdx.x % sub) * cb + threadIdx.x, i, j, k)) return;
+  const int m = p.nsnap;
+  double* a = lds + threadIdx.x;
+  double* bb = lds + (ll)m * m * cb + threadIdx.x;
+  const ll n3 = n3of(b), n1 = n1of(b), x = i3(b, i, j, k), x1 = i1(b, i, j, k);
+  const ll sq = n3 * p.npde;
+  double* qb = q + b.o3 * p.npde * p.nsnap + x;
+  const double* ob = qold + b.o3 * p.npde * p.nsnap + x;
+  const double* rb = res + b.o3 * p.npde * p.nsnap + x;
+  double* hb = qhb0 + b.o3 * p.npde * p.nsnap + x;
+  const double* dtb = dtvol + b.o1 * p.nsnap + x1;
+  const double vv = vol[b.o1 + x1];
+  if (p.kom) {
+    const double* dttb = dtvolt + b.o1 * p.nsnap + x1;
+    const double* dlb = delpls + b.o3 * p.nsnap + x;
+    const double* pb = prd + b.o3 * p.nsnap + x;
+    // ttrm(2:3) per snapshot: gkw, bkw (Wilcox) or blended (BSL/SST)
+    const double* tb = p.sst ? ttrm + b.o3 * 3 * p.nsnap + x : nullptr;
+#define GAV(n) (tb ? tb[n3 * (3 * (ll)(n) + 1)] : p.gkw)
+#define BAV(n) (tb ? tb[n3 * (3 * (ll)(n) + 2)] : p.bkw)
+    // rho*omega: rhs in column 0, the old rho*omega in column 1
+    for (int ni = 0; ni < m; ni++) {
+      const double gav = GAV(ni), bav = BAV(ni);
+      const double ome = qb[sq * ni + 6 * n3] / qb[sq * ni];
+      const double dtt = dttb[n1 * ni];
+      const double dtau = dtt * vv;
+      const double dps = dlb[n3 * ni];
+      const double q7 = qb[sq * ni + 6 * n3];
+      const double rhs = plain ? ob[sq * ni + 6 * n3] - ar * rb[sq * ni + 6 * n3] * dtt +
+                                     ar * dtau * q7 * (gav * dps + 2 * bav * ome)
+                               : ob[sq * ni + 6 * n3] + ar * dtt * (hb[sq * ni + 6 * n3] - rb[sq * ni + 6 * n3]) +
+                                     ar * dtau * q7 * (gav * dps + 2 * bav * ome);
+      hb[sq * ni + 6 * n3] = rhs;
+      B_(ni, 0) = rhs;
+      B_(ni, 1) = q7;
+      const double dg = 1 + ar * dtau * (gav * dps + 2 * bav * ome);
+      for (int nj = 0; nj < m; nj++) A_(ni, nj) = (ni == nj ? 1.0 : 0.0) * dg + ar * p.omega * dtau * dhb[ni + m * nj];
+    }
+    lds_lu_solve(a, bb, m, 1, cb);
+    for (int ni = 0; ni < m; ni++) {
+      const double rho = qb[sq * ni];
+      const double rhomegamin = GAV(ni) * p.cmu * rho * sqrt(pb[n3 * ni]);
+      double w = B_(ni, 0);
+      if (w < rhomegamin) w = rhomegamin;
+      qb[sq * ni + 6 * n3] = w;
+      B_(ni, 2) = w - B_(ni, 1);   // d(rho*omega)
+    }
+    // rho*k
+    for (int ni = 0; ni < m; ni++) {
+      const double rho = qb[sq * ni];
+      const double ome = B_(ni, 1) / qb[sq * ni];
+      const double dtt = dttb[n1 * ni];
+      const double dtau = dtt * vv;
+      const double dps = dlb[n3 * ni];
+      const double drho = B_(ni, 2);
+      const double q6 = qb[sq * ni + 5 * n3];
+      const double rhs =
+          plain ? (ob[sq * ni + 5 * n3] - ar * rb[sq * ni + 5 * n3] * dtt +
+                   ar * dtau * q6 * (dps + p.bstrkw * ome - p.bstrkw * drho / rho))
+                : (ob[sq * ni + 5 * n3] + ar * dtt * (hb[sq * ni + 5 * n3] - rb[sq * ni + 5 * n3]) +
+                   ar * dtau * q6 * (dps + p.bstrkw * ome - p.bstrkw * drho / rho));
+      hb[sq * ni + 5 * n3] = rhs;
+      B_(ni, 0) = rhs;
+      const double dg = 1 + ar * dtau * (dps + p.bstrkw * ome);
+      for (int nj = 0; nj < m; nj++) A_(ni, nj) = (ni == nj ? 1.0 : 0.0) * dg + ar * p.omega * dtau * dhb[ni + m * nj];
+    }
+    lds_lu_solve(a, bb, m, 1, cb);
+    for (int ni = 0; ni < m; ni++) qb[sq * ni + 5 * n3] = B_(ni, 0);
+  }
+  // mean flow
+  for (int ni = 0; ni < m; ni++) {
+    const double dt = dtb[n1 * ni];
+    for (int v = 0; v < 5; v++) {
+      const ll o = sq * ni + n3 * v;
+      const double rhs = plain ? ob[o] - ar * rb[o] * dt : ob[o] + (hb[o] - rb[o]) * ar * dt;
+      hb[o] = rhs;
+      B_(ni, v) = rhs;
+    }
+    const double dtau = dt * vv;
+    for (int nj = 0; nj < m; nj++) A_(ni, nj) = (ni == nj ? 1.0 : 0.0) + ar * p.omega * dtau * dhb[ni + m * nj];
+  }
+  lds_lu_solve(a, bb, m, 5, cb);
+  for (int ni = 0; ni < m; ni++)
+    for (int v = 0; v < 5; v++) qb[sq * ni + n3 * v] = B_(ni, v);
+}
+
+// lim_res_ome_bhb, rkim branch (src/update.f90:2925-2990): the omega
+// residual limited against the coupled solve of res7 * dtvolt.
+__global__ void __launch_bounds__(64) k_lim_res_ome_rkim(Par p, const Blk* __restrict__ blks,
+                                                         const Tile* __restrict__ tiles, int sub, int cb,
+                                                         double* __restrict__ res, const double* __restrict__ q,
+                                                         const double* __restrict__ qold,
+                                                         const double* __restrict__ vol,
+                                                         const double* __restrict__ dtvolt,
+                                                         const double* __restrict__ delpls,
+                                                         const double* __restrict__ prd,
+                                                         const double* __restrict__ dhb,
+                                                         const double* __restrict__ ttrm) {
+  extern __shared__ double lds[];
+  const Tile t = tiles[blockIdx.x / sub];
+  const Blk b = blks[t.block];
+  int i, j, k;
+  if (!cell_item(b, t.base + (blockIdx.x % sub) * cb + threadIdx.x, i, j, k)) return;
+  const int m = p.nsnap;
+  double* a = lds + threadIdx.x;
+  double* bb = lds + (ll)m * m * cb + threadIdx.x;
+  const ll n3 = n3of(b), n1 = n1of(b), x = i3(b, i, j, k), x1 = i1(b, i, j, k);
+  const ll sq = n3 * p.npde;
+  const double* qb = q + b.o3 * p.npde * p.nsnap + x;
+  const double* ob = qold + b.o3 * p.npde * p.nsnap + x;
+  double* rb = res + b.o3 * p.npde * p.nsnap + x;
+  const double* dttb = dtvolt + b.o1 * p.nsnap + x1;
+  const double* dlb = delpls + b.o3 * p.nsnap + x;
+  const double* pb = prd + b.o3 * p.nsnap + x;
+  const double vv = vol[b.o1 + x1];
+  const double* tb = p.sst ? ttrm + b.o3 * 3 * p.nsnap + x : nullptr;
+  for (int ni = 0; ni < m; ni++) {
+    const double gav = GAV(ni), bav = BAV(ni);
+    const double dtau = dttb[n1 * ni] * vv;
+    const double ome = qb[sq * ni + 6 * n3] / qb[sq * ni];
+    const double dg = 1 + dtau * (gav * dlb[n3 * ni] + 2 * bav * ome);
+    for (int nj = 0; nj < m; nj++) A_(ni, nj) = (ni == nj ? 1.0 : 0.0) * dg + p.omega * dtau * dhb[ni + m * nj];
+    B_(ni, 0) = rb[sq * ni + 6 * n3] * dttb[n1 * ni];
+  }
+  lds_lu_solve(a, bb, m, 1, cb);
+  for (int n = 0; n < m; n++) {
+    const double rho = qb[sq * n];
+    const double rhomegamin = GAV(n) * p.cmu * rho * sqrt(pb[n3 * n]);
+    const double rhomeganew = ob[sq * n + 6 * n3] - B_(n, 0);
+    rb[sq * n + 6 * n3] = rb[sq * n + 6 * n3] - fmax(0.0, (rhomegamin - rhomeganew)) / dttb[n1 * n];
+  }
+#undef A_
+#undef B_
+#undef GAV
+#undef BAV
+}
+
+// resg_brms_hb with rkim, no IRS, one level (src/cosa.f90:4657-4674): the
+// HB term added to the copy of the residual, snapshot by snapshot in order.
+__global__ void __launch_bounds__(256) k_rms_hb_term(Par p, const Blk* __restrict__ blks,
+                                                     const Tile* __restrict__ tiles, double* __restrict__ work1,
+                                                     const double* __restrict__ q, const double* __restrict__ vol,
+                                                     const double* __restrict__ dhb) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  int i, j, k;
+  if (!cell_item(b, t.base + threadIdx.x, i, j, k)) return;
+  const int m = p.nsnap;
+  const ll n3 = n3of(b), x = i3(b, i, j, k), sq = n3 * p.npde;
+  const double* qb = q + b.o3 * p.npde * p.nsnap + x;
+  double* wb = work1 + b.o3 * p.npde * p.nsnap + x;
+  const double vv = vol[b.o1 + i1(b, i, j, k)];
+  for (int ni = 0; ni < m; ni++)
+    for (int v = 0; v < p.npde; v++) {
+      double acc = wb[sq * ni + n3 * v];
+      for (int nj = 0; nj < m; nj++) acc = acc + p.omega * dhb[ni + m * nj] * qb[sq * nj + n3 * v] * vv;
+      wb[sq * ni + n3 * v] = acc;
     }
 }
 
@@ -864,11 +1123,50 @@ hipError_t launch_sub_box(Solve& s, Level& L, double* dst, const double* src) {
                        L.tile[TK_CELLS_P1], dst, src);
   return hipGetLastError();
 }
+// threads per work-group of the rkim kernels: systems of m x m + m x 5
+// doubles per thread within 64 KiB of LDS
+int rkim_group(int m) {
+  const ll per = (ll)m * m + 5LL * m;
+  for (int cb = 64; cb >= 1; cb >>= 1)
+    if ((ll)cb * per * 8 <= 65536) return cb;
+  return 0;
+}
+static hipError_t launch_update_rkim(Solve& s, Level& L, double ar) {
+  const int m = s.par.nsnap, cb = rkim_group(m);
+  const int sub = TS / cb;
+  const int plain = s.cfg.irsop == 0 && s.cfg.nlevel == 1;
+  const size_t lds = (size_t)cb * ((size_t)m * m + 5 * (size_t)m) * sizeof(double);
+  if (L.ntile[TK_CELLS])
+    hipLaunchKernelGGL(k_update_rkim, dim3(L.ntile[TK_CELLS] * sub), dim3(cb), lds, s.stream, s.par, L.dblk,
+                       L.tile[TK_CELLS], sub, cb, L.f[COSAHIP_RES], L.f[COSAHIP_Q], L.f[COSAHIP_QOLD],
+                       L.f[COSAHIP_DTVOL], ar, L.f[COSAHIP_VOL], L.f[COSAHIP_DTVOLT], L.f[COSAHIP_DELPLS],
+                       L.f[COSAHIP_PRD], L.dq, s.ddhb, plain, L.f[COSAHIP_TTRM]);
+  return hipGetLastError();
+}
+hipError_t launch_lim_res_ome_rkim(Solve& s, Level& L) {
+  const int m = s.par.nsnap, cb = rkim_group(m);
+  const int sub = TS / cb;
+  const size_t lds = (size_t)cb * ((size_t)m * m + 5 * (size_t)m) * sizeof(double);
+  if (L.ntile[TK_CELLS])
+    hipLaunchKernelGGL(k_lim_res_ome_rkim, dim3(L.ntile[TK_CELLS] * sub), dim3(cb), lds, s.stream, s.par, L.dblk,
+                       L.tile[TK_CELLS], sub, cb, L.f[COSAHIP_RES], L.f[COSAHIP_Q], L.f[COSAHIP_QOLD],
+                       L.f[COSAHIP_VOL], L.f[COSAHIP_DTVOLT], L.f[COSAHIP_DELPLS], L.f[COSAHIP_PRD], s.ddhb,
+                       L.f[COSAHIP_TTRM]);
+  return hipGetLastError();
+}
+hipError_t launch_rms_hb_term(Solve& s, Level& L, double* work1) {
+  if (L.ntile[TK_CELLS])
+    hipLaunchKernelGGL(k_rms_hb_term, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                       L.tile[TK_CELLS], work1, L.f[COSAHIP_Q], L.f[COSAHIP_VOL], s.ddhb);
+  return hipGetLastError();
+}
 hipError_t launch_update(Solve& s, Level& L, double ar) {
+  if (s.cfg.harbal && s.cfg.rkimp) return launch_update_rkim(s, L, ar);
   if (L.ntile[TK_CELLS])
     hipLaunchKernelGGL(k_update, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_CELLS],
                        L.f[COSAHIP_RES], L.f[COSAHIP_Q], L.f[COSAHIP_QOLD], L.f[COSAHIP_DTVOL], ar,
-                       L.f[COSAHIP_VOL], L.f[COSAHIP_DTVOLT], L.f[COSAHIP_DELPLS], L.f[COSAHIP_PRD]);
+                       L.f[COSAHIP_VOL], L.f[COSAHIP_DTVOLT], L.f[COSAHIP_DELPLS], L.f[COSAHIP_PRD],
+                       L.f[COSAHIP_TTRM]);
   return hipGetLastError();
 }
 hipError_t launch_src_hb(Solve& s, Level& L) {

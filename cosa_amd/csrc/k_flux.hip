@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(256) k_flux_i(Par p, int nl, const Blk* __rest
                                                 const unsigned char* __restrict__ sbc, const ll* __restrict__ sbcoff,
                                                 const double* __restrict__ h0tab, const T* __restrict__ mut,
                                                 const T* __restrict__ xd, const T* __restrict__ ed,
-                                                const T* __restrict__ zd) {
+                                                const T* __restrict__ zd, const double* __restrict__ dist) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int lane = threadIdx.x & 63;
@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(256) k_flux_i(Par p, int nl, const Blk* __rest
       const double* h = h0tab + ((ll)(t.block * p.nsnap + n) * 3 + 0) * 2;
       roe_turb<T>(p, QR, QL, kx, ky, kz, ds, ubn, (T)h[0], (T)h[1], F);
     }
-    if (VIS) viscous_face<1, NV, T>(p, b, a, j, k, n, q, mut, si, xd, ed, zd, fl, Fv);
+    if (VIS) viscous_face<1, NV, T>(p, b, a, j, k, n, q, mut, si, xd, ed, zd, fl, dist, Fv);
   }
   T Fn[NV], Fvn[NV];
 #pragma unroll
@@ -333,7 +333,8 @@ __global__ void __launch_bounds__(256) k_flux_march(Par p, int nl, const Blk* __
                                                     const unsigned char* __restrict__ sbc,
                                                     const ll* __restrict__ sbcoff, const double* __restrict__ h0tab,
                                                     const T* __restrict__ mut, const T* __restrict__ xd,
-                                                    const T* __restrict__ ed, const T* __restrict__ zd) {
+                                                    const T* __restrict__ ed, const T* __restrict__ zd,
+                                                    const T* __restrict__ vpre, const double* __restrict__ dist) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int item = t.base + threadIdx.x;
@@ -394,11 +395,16 @@ __global__ void __launch_bounds__(256) k_flux_march(Par p, int nl, const Blk* __
     roe_face<NV, T>(p, QR, QL, kx, ky, kz, ds, ubn, F);
     if (NV > 5) roe_turb<T>(p, QR, QL, kx, ky, kz, ds, ubn, h0l, h0r, F);
     if (VIS) {
-      if (DIR == 2) viscous_face<2, NV, T>(p, b, i, a, t2, n, q, mut, sm, xd, ed, zd, fl, Fv);
-      else viscous_face<3, NV, T>(p, b, i, t2, a, n, q, mut, sm, xd, ed, zd, fl, Fv);
+      if (DIR == 2) viscous_face<2, NV, T>(p, b, i, a, t2, n, q, mut, sm, xd, ed, zd, fl, dist, Fv);
+      else viscous_face<3, NV, T>(p, b, i, t2, a, n, q, mut, sm, xd, ed, zd, fl, dist, Fv);
     }
     if (a > a0) {
       const ll c = base + (ll)(a - 1) * step;
+      // vpre: first the viscous flux difference of the previous direction
+      if (vpre) {
+        if (DIR == 2) add_vdiff<NV, T>(p, b, vpre, 1, i, a - 1, t2, n, rb + c, n3);
+        else add_vdiff<NV, T>(p, b, vpre, 2, i, t2, a - 1, n, rb + c, n3);
+      }
 #pragma unroll
       for (int v = 0; v < NV; v++) {
         double r = (rb[c + n3 * v] + F[v]) - Fp[v];
@@ -1216,49 +1222,50 @@ hipError_t launch_residual(Solve& s, Level& L, const double* q, double* res) {
   if (L.ntile[TK_ROWS_I])
     hipLaunchKernelGGL((k_flux_i<5, false, double>), grid1(L.ntile[TK_ROWS_I]), dim3(256), 0, s.stream, s.par, L.nl,
                        L.dblk, L.tile[TK_ROWS_I], q, L.f[COSAHIP_SI], res, L.sbc, L.sbc_doff, nullptr, nullptr, nullptr,
-                       nullptr, nullptr);
+                       nullptr, nullptr, nullptr);
   if (L.ntile[TK_SEGS_J])
     hipLaunchKernelGGL((k_flux_march<2, 5, false, double>), grid1(L.ntile[TK_SEGS_J]), dim3(TS), 0, s.stream, s.par, L.nl,
                        L.dblk, L.tile[TK_SEGS_J], q, L.f[COSAHIP_SJ], res, L.sbc, L.sbc_doff, nullptr, nullptr,
-                       nullptr, nullptr, nullptr);
+                       nullptr, nullptr, nullptr, nullptr, nullptr);
   if (L.ntile[TK_SEGS_K])
     hipLaunchKernelGGL((k_flux_march<3, 5, false, double>), grid1(L.ntile[TK_SEGS_K]), dim3(TS), 0, s.stream, s.par, L.nl,
                        L.dblk, L.tile[TK_SEGS_K], q, L.f[COSAHIP_SK], res, L.sbc, L.sbc_doff, nullptr, nullptr,
-                       nullptr, nullptr, nullptr);
+                       nullptr, nullptr, nullptr, nullptr, nullptr);
   return hipGetLastError();
 }
 
 // Direction idir of the viscous / k-omega residual: inviscid Roe/MUSCL and,
 // viscous, the face diffusion fluxes in the same line march (one launch).
 template <int NV, bool VIS, typename T>
-static void flux_dir_t(Solve& s, Level& L, int idir, const ResIn<T>& in, double* res) {
+static void flux_dir_t(Solve& s, Level& L, int idir, const ResIn<T>& in, double* res, const T* vpre) {
   const T* xd = in.xd[3 * (idir - 1)];
   const T* ed = in.xd[3 * (idir - 1) + 1];
   const T* zd = in.xd[3 * (idir - 1) + 2];
   if (idir == 1 && L.ntile[TK_ROWS_I])
     hipLaunchKernelGGL((k_flux_i<NV, VIS, T>), grid1(L.ntile[TK_ROWS_I]), dim3(256), 0, s.stream, s.par, L.nl, L.dblk,
-                       L.tile[TK_ROWS_I], in.q, in.s[0], res, L.sbc, L.sbc_doff, L.h0tab, in.mut, xd, ed, zd);
+                       L.tile[TK_ROWS_I], in.q, in.s[0], res, L.sbc, L.sbc_doff, L.h0tab, in.mut, xd, ed, zd,
+                       L.f[COSAHIP_DIST]);
   else if (idir == 2 && L.ntile[TK_SEGS_J])
     hipLaunchKernelGGL((k_flux_march<2, NV, VIS, T>), grid1(L.ntile[TK_SEGS_J]), dim3(TS), 0, s.stream, s.par, L.nl,
                        L.dblk, L.tile[TK_SEGS_J], in.q, in.s[1], res, L.sbc, L.sbc_doff, L.h0tab, in.mut, xd, ed,
-                       zd);
+                       zd, vpre, L.f[COSAHIP_DIST]);
   else if (idir == 3 && L.ntile[TK_SEGS_K])
     hipLaunchKernelGGL((k_flux_march<3, NV, VIS, T>), grid1(L.ntile[TK_SEGS_K]), dim3(TS), 0, s.stream, s.par, L.nl,
                        L.dblk, L.tile[TK_SEGS_K], in.q, in.s[2], res, L.sbc, L.sbc_doff, L.h0tab, in.mut, xd, ed,
-                       zd);
+                       zd, vpre, L.f[COSAHIP_DIST]);
 }
 
 template <typename T>
-hipError_t launch_flux_dir(Solve& s, Level& L, int idir, const ResIn<T>& in, double* res) {
+hipError_t launch_flux_dir(Solve& s, Level& L, int idir, const ResIn<T>& in, double* res, const T* vpre) {
   const bool t = s.par.npde == 7, v = s.par.viscous != 0;
-  if (t && v) flux_dir_t<7, true, T>(s, L, idir, in, res);
-  else if (t) flux_dir_t<7, false, T>(s, L, idir, in, res);
-  else if (v) flux_dir_t<5, true, T>(s, L, idir, in, res);
-  else flux_dir_t<5, false, T>(s, L, idir, in, res);
+  if (t && v) flux_dir_t<7, true, T>(s, L, idir, in, res, vpre);
+  else if (t) flux_dir_t<7, false, T>(s, L, idir, in, res, vpre);
+  else if (v) flux_dir_t<5, true, T>(s, L, idir, in, res, vpre);
+  else flux_dir_t<5, false, T>(s, L, idir, in, res, vpre);
   return hipGetLastError();
 }
-template hipError_t launch_flux_dir<double>(Solve&, Level&, int, const ResIn<double>&, double*);
-template hipError_t launch_flux_dir<float>(Solve&, Level&, int, const ResIn<float>&, double*);
+template hipError_t launch_flux_dir<double>(Solve&, Level&, int, const ResIn<double>&, double*, const double*);
+template hipError_t launch_flux_dir<float>(Solve&, Level&, int, const ResIn<float>&, double*, const float*);
 
 template <typename T>
 hipError_t launch_h0_last(Solve& s, Level& L, const ResIn<T>& in) {

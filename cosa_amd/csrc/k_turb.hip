@@ -30,7 +30,8 @@ __global__ void __launch_bounds__(256) k_vflux(Par p, const Blk* __restrict__ bl
                                                const T* __restrict__ sm, const T* __restrict__ xd,
                                                const T* __restrict__ ed, const T* __restrict__ zd,
                                                const unsigned char* __restrict__ sbc,
-                                               const ll* __restrict__ sbcoff, T* __restrict__ vf) {
+                                               const ll* __restrict__ sbcoff, T* __restrict__ vf,
+                                               const double* __restrict__ dist) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int ic = DIR == 1 ? b.imax : b.imax - 1, jc = DIR == 2 ? b.jmax : b.jmax - 1,
@@ -44,7 +45,7 @@ __global__ void __launch_bounds__(256) k_vflux(Par p, const Blk* __restrict__ bl
   rem /= ic;
   const int j = rem % jc + 1, k = rem / jc + 1;
   T F[NV];
-  viscous_face<DIR, NV, T>(p, b, i, j, k, n, q, mut, sm, xd, ed, zd, sbc + sbcoff[t.block], F);
+  viscous_face<DIR, NV, T>(p, b, i, j, k, n, q, mut, sm, xd, ed, zd, sbc + sbcoff[t.block], dist, F);
   T* fb = vf + b.o0 * p.npde * p.nsnap;
   const ll fo = (ll)(i - 1) + (ll)b.imax * ((ll)(j - 1) + (ll)b.jmax * (ll)(k - 1));
   const ll fst = (ll)b.imax * b.jmax * b.kmax;
@@ -90,7 +91,7 @@ __global__ void __launch_bounds__(256) k_vres_i(Par p, const Blk* __restrict__ b
                                                 const T* __restrict__ sm, const T* __restrict__ xd,
                                                 const T* __restrict__ ed, const T* __restrict__ zd,
                                                 const unsigned char* __restrict__ sbc, const ll* __restrict__ sbcoff,
-                                                double* __restrict__ res) {
+                                                double* __restrict__ res, const double* __restrict__ dist) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int lane = threadIdx.x & 63;
@@ -107,7 +108,8 @@ __global__ void __launch_bounds__(256) k_vres_i(Par p, const Blk* __restrict__ b
   T F[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) F[v] = 0;
-  if (a <= b.imax) viscous_face<1, NV, T>(p, b, a, j, k, n, q, mut, sm, xd, ed, zd, sbc + sbcoff[t.block], F);
+  if (a <= b.imax)
+    viscous_face<1, NV, T>(p, b, a, j, k, n, q, mut, sm, xd, ed, zd, sbc + sbcoff[t.block], dist, F);
   T Fn[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) Fn[v] = __shfl_down(F[v], 1, 64);
@@ -130,7 +132,8 @@ __global__ void __launch_bounds__(256) k_vres_march(Par p, const Blk* __restrict
                                                     const T* __restrict__ mut, const T* __restrict__ sm,
                                                     const T* __restrict__ xd, const T* __restrict__ ed,
                                                     const T* __restrict__ zd, const unsigned char* __restrict__ sbc,
-                                                    const ll* __restrict__ sbcoff, double* __restrict__ res) {
+                                                    const ll* __restrict__ sbcoff, double* __restrict__ res,
+                                                    const double* __restrict__ dist) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int item = t.base + threadIdx.x;
@@ -154,8 +157,8 @@ __global__ void __launch_bounds__(256) k_vres_march(Par p, const Blk* __restrict
   T Fp[NV];
   for (int a = a0; a <= a1; a++) {
     T F[NV];
-    if (DIR == 2) viscous_face<2, NV, T>(p, b, i, a, t2, n, q, mut, sm, xd, ed, zd, fl, F);
-    else viscous_face<3, NV, T>(p, b, i, t2, a, n, q, mut, sm, xd, ed, zd, fl, F);
+    if (DIR == 2) viscous_face<2, NV, T>(p, b, i, a, t2, n, q, mut, sm, xd, ed, zd, fl, dist, F);
+    else viscous_face<3, NV, T>(p, b, i, t2, a, n, q, mut, sm, xd, ed, zd, fl, dist, F);
     if (a > a0) {
       const ll c = base + (ll)(a - 1) * step;
 #pragma unroll
@@ -167,9 +170,12 @@ __global__ void __launch_bounds__(256) k_vres_march(Par p, const Blk* __restrict
 }
 
 // ------------------------------------------------------------------------
-// k-omega sources at cell centres: the velocity differences of the face
-// averages (q_face + q_der_c, three directions), turprod (finest level)
-// and srckw into res(6:7).  q primitive.
+// k-omega sources at cell centres: the differences of the face averages
+// (q_face + q_der_c, three directions; u, v, w, and for BSL/SST k and
+// omega), turprod (finest level), then srckw (Wilcox) or update_ttrm +
+// srckw_sst (BSL/SST, src/cosa.f90:1894-1900) into res(6:7).  SST keeps
+// the velocity differences (dqc) for tur_vis_sst after the update.
+// q primitive.
 template <typename T>
 __global__ void __launch_bounds__(256) k_turb_src(Par p, int nl, const Blk* __restrict__ blks,
                                                   const Tile* __restrict__ tiles, const T* __restrict__ q,
@@ -178,7 +184,9 @@ __global__ void __launch_bounds__(256) k_turb_src(Par p, int nl, const Blk* __re
                                                   double* __restrict__ res, const double* __restrict__ vol,
                                                   const T* const* __restrict__ xdp,
                                                   const unsigned char* __restrict__ sbc,
-                                                  const ll* __restrict__ sbcoff) {
+                                                  const ll* __restrict__ sbcoff, const T* __restrict__ vpre,
+                                                  double* __restrict__ ttrm, double* __restrict__ dqc,
+                                                  const double* __restrict__ dist) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
@@ -196,9 +204,12 @@ __global__ void __launch_bounds__(256) k_turb_src(Par p, int nl, const Blk* __re
   const ll st[4] = {0, 1, (ll)(b.imax + 3), (ll)(b.imax + 3) * (b.jmax + 3)};
   const unsigned char* fl = sbc + sbcoff[t.block];
   const ll so = b.o3 * p.nsnap + n3 * (ll)n + c;
-  if (nl == 1 || p.prd_all_lvls) {
-    // dq(d)[v] = qf(face c+1) - qf(face c), v = u, v, w
-    T dq[3][3];
+  const bool prodlvl = nl == 1 || p.prd_all_lvls;
+  T cdif = 0, gav = (T)p.gkw, bav = (T)p.bkw;
+  if (prodlvl || p.sst) {
+    // dq[d][s] = qf(face c+1) - qf(face c), s = u, v, w, k, omega
+    T dq[3][5];
+    const int nvar = p.sst ? 5 : 3;
     for (int d = 1; d <= 3; d++) {
       const int a = d == 1 ? i : (d == 2 ? j : k);
       const int L = d == 1 ? b.imax : (d == 2 ? b.jmax : b.kmax);
@@ -215,11 +226,11 @@ __global__ void __launch_bounds__(256) k_turb_src(Par p, int nl, const Blk* __re
                                        : (d == 2 ? face_flag<2>(b, fl, 1, t1, t2) : face_flag<3>(b, fl, 1, t1, t2));
         whi = (f & 1) != 0;
       }
-      for (int v = 1; v <= 3; v++) {
-        const T* qv = qb + n3 * v;
+      for (int sv = 0; sv < nvar; sv++) {
+        const T* qv = qb + n3 * (sv < 3 ? sv + 1 : sv + 2);
         const T flo = wlo ? qv[c - st[d]] : (qv[c - st[d]] + qv[c]) / 2;
         const T fhi = whi ? qv[c + st[d]] : (qv[c] + qv[c + st[d]]) / 2;
-        dq[d - 1][v - 1] = fhi - flo;
+        dq[d - 1][sv] = fhi - flo;
       }
     }
     const int nh = n * p.hbmove;
@@ -233,32 +244,70 @@ __global__ void __launch_bounds__(256) k_turb_src(Par p, int nl, const Blk* __re
     const T zetax = (mder<T>(p, b, xdp[2], i + 1, j, k, nh)[0] + mder<T>(p, b, xdp[2], i, j, k, nh)[0]) / 2;
     const T zetay = (mder<T>(p, b, xdp[5], i, j + 1, k, nh)[1] + mder<T>(p, b, xdp[5], i, j, k, nh)[1]) / 2;
     const T zetaz = (mder<T>(p, b, xdp[8], i, j, k + 1, nh)[2] + mder<T>(p, b, xdp[8], i, j, k, nh)[2]) / 2;
-    const T uxi = dq[0][0], ueta = dq[1][0], uzeta = dq[2][0];
-    const T vxi = dq[0][1], veta = dq[1][1], vzeta = dq[2][1];
-    const T wxi = dq[0][2], weta = dq[1][2], wzeta = dq[2][2];
-    const T dudx = uxi * xix + ueta * etax + uzeta * zetax;
-    const T dudy = uxi * xiy + ueta * etay + uzeta * zetay;
-    const T dudz = uxi * xiz + ueta * etaz + uzeta * zetaz;
-    const T dvdx = vxi * xix + veta * etax + vzeta * zetax;
-    const T dvdy = vxi * xiy + veta * etay + vzeta * zetay;
-    const T dvdz = vxi * xiz + veta * etaz + vzeta * zetaz;
-    const T dwdx = wxi * xix + weta * etax + wzeta * zetax;
-    const T dwdy = wxi * xiy + weta * etay + wzeta * zetay;
-    const T dwdz = wxi * xiz + weta * etaz + wzeta * zetaz;
-    const T divv = (dudx + dvdy + dwdz) * (T)p.turcmp;
-    const T txx = dudx, txy = (dudy + dvdx) / 2, txz = (dudz + dwdx) / 2, tyy = dvdy, tyz = (dvdz + dwdy) / 2,
-                 tzz = dwdz;
-    prd[so] = 2 * (txx * txx + tyy * tyy + tzz * tzz + 2 * (txy * txy + txz * txz + tyz * tyz) - divv * divv / 3);
-    divvel[so] = divv;
-    delpls[so] = T(2.0 / 3.0) * fmax(divv, T(0));
+    if (prodlvl) {
+      const T uxi = dq[0][0], ueta = dq[1][0], uzeta = dq[2][0];
+      const T vxi = dq[0][1], veta = dq[1][1], vzeta = dq[2][1];
+      const T wxi = dq[0][2], weta = dq[1][2], wzeta = dq[2][2];
+      const T dudx = uxi * xix + ueta * etax + uzeta * zetax;
+      const T dudy = uxi * xiy + ueta * etay + uzeta * zetay;
+      const T dudz = uxi * xiz + ueta * etaz + uzeta * zetaz;
+      const T dvdx = vxi * xix + veta * etax + vzeta * zetax;
+      const T dvdy = vxi * xiy + veta * etay + vzeta * zetay;
+      const T dvdz = vxi * xiz + veta * etaz + vzeta * zetaz;
+      const T dwdx = wxi * xix + weta * etax + wzeta * zetax;
+      const T dwdy = wxi * xiy + weta * etay + wzeta * zetay;
+      const T dwdz = wxi * xiz + weta * etaz + wzeta * zetaz;
+      const T divv = (dudx + dvdy + dwdz) * (T)p.turcmp;
+      const T txx = dudx, txy = (dudy + dvdx) / 2, txz = (dudz + dwdx) / 2, tyy = dvdy, tyz = (dvdz + dwdy) / 2,
+              tzz = dwdz;
+      prd[so] = 2 * (txx * txx + tyy * tyy + tzz * tzz + 2 * (txy * txy + txz * txz + tyz * tyz) - divv * divv / 3);
+      divvel[so] = divv;
+      delpls[so] = T(2.0 / 3.0) * fmax(divv, T(0));
+    }
+    if (p.sst == 2) {
+      // u, v, w differences along xi, eta, zeta for tur_vis_sst
+      double* db = dqc + b.o3 * 9 * p.nsnap + n3 * 9 * (ll)n + c;
+#pragma unroll
+      for (int sv = 0; sv < 3; sv++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) db[n3 * (3 * sv + d)] = (double)dq[d][sv];
+    }
+    if (p.sst) {
+      // update_ttrm_b (src/update.f90:3083-3155): F1 blending
+      const T rho = qb[c], tke = qb[c + 5 * n3], ome = qb[c + 6 * n3], pr = qb[c + 4 * n3];
+      const T tt = (T)p.gamma * pr / rho;
+      const T mu = suth<T>(p, tt);
+      const T dtkedx = dq[0][3] * xix + dq[1][3] * etax + dq[2][3] * zetax;
+      const T dtkedy = dq[0][3] * xiy + dq[1][3] * etay + dq[2][3] * zetay;
+      const T dtkedz = dq[0][3] * xiz + dq[1][3] * etaz + dq[2][3] * zetaz;
+      const T domedx = dq[0][4] * xix + dq[1][4] * etax + dq[2][4] * zetax;
+      const T domedy = dq[0][4] * xiy + dq[1][4] * etay + dq[2][4] * zetay;
+      const T domedz = dq[0][4] * xiz + dq[1][4] * etaz + dq[2][4] * zetaz;
+      const T de = (T)dist[b.o1 + i1(b, i, j, k)] + (T)p.epst;
+      T arg1 = fmax(sqrt(tke) / (T)0.09 / ome / de, (T)500 * mu / rho / ome / (de * de) * (T)p.machfs / (T)p.reyno);
+      const T cdt = 2 * rho * (T)p.sigw2 / ome * (dtkedx * domedx + dtkedy * domedy + dtkedz * domedz);
+      const T acdt = fmax(cdt, (T)1e-20);
+      arg1 = fmin(arg1, 4 * rho * (T)p.sigw2 * tke / acdt / (de * de));
+      arg1 = arg1 * arg1 * arg1 * arg1;   // arg1**4 as ((a a) a) a
+      arg1 = fmin(arg1, (T)1e2);
+      const T e2 = exp(2 * arg1);
+      const T f1 = (e2 - 1) / (e2 + 1);
+      cdif = cdt * (1 - f1);
+      gav = f1 * (T)p.g1 + (1 - f1) * (T)p.g2;
+      bav = f1 * (T)p.b1 + (1 - f1) * (T)p.b2;
+      double* tb = ttrm + b.o3 * 3 * p.nsnap + n3 * 3 * (ll)n + c;
+      tb[0] = (double)cdif;
+      tb[n3] = (double)gav;
+      tb[2 * n3] = (double)bav;
+    }
   }
-  // srckw_b
+  // srckw_b / srckw_sst_b
   const T fact = T(2.0 / 3.0), mor = (T)(p.machfs / p.reyno);
   const T rho = qb[c], tke = qb[c + 5 * n3], ome = qb[c + 6 * n3];
   const T mu_t = mut[so];
   const T divv = (T)divvel[so];
   T prod = (T)prd[so], prodw = (T)prd[so];
-  if (p.lim_prodtke && (nl == 1 || p.prd_all_lvls)) {
+  if (p.lim_prodtke && prodlvl) {
     const T tstdis = (T)p.prdlim * (T)p.bstrkw * rho * tke * ome;
     if (mor * mu_t * prod > tstdis) {
       prod = tstdis / mor / mu_t;
@@ -267,9 +316,155 @@ __global__ void __launch_bounds__(256) k_turb_src(Par p, int nl, const Blk* __re
   }
   const T vv = (T)vol[b.o1 + i1(b, i, j, k)];
   double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n + c;
+  // vpre: first the difference of the direction-3 viscous face fluxes
+  if (vpre) add_vdiff<7, T>(p, b, vpre, 3, i, j, k, n, rb, n3);
   rb[5 * n3] = rb[5 * n3] - vv * (mor * mu_t * prod - fact * divv * rho * tke - (T)p.bstrkw * rho * tke * ome);
-  rb[6 * n3] = rb[6 * n3] - vv * ((T)p.gkw * (T)p.cmu * rho * prodw - (T)p.gkw * fact * divv * rho * ome -
-                                  (T)p.bkw * rho * (ome * ome));
+  if (p.sst)
+    rb[6 * n3] = rb[6 * n3] - vv * (gav * (T)p.cmu * rho * prodw - gav * fact * divv * rho * ome -
+                                    bav * rho * (ome * ome) + cdif);
+  else
+    rb[6 * n3] = rb[6 * n3] - vv * ((T)p.gkw * (T)p.cmu * rho * prodw - (T)p.gkw * fact * divv * rho * ome -
+                                    (T)p.bkw * rho * (ome * ome));
+}
+
+// tur_vis_sst_b interior (src/update.f90:2422-2486): SST eddy viscosity
+// with the F2 limiter on the strain rate, from the velocity differences dqc
+// of the stage; q conservative
+__global__ void __launch_bounds__(256) k_tur_vis_sst(Par p, const Blk* __restrict__ blks,
+                                                     const Tile* __restrict__ tiles, const double* __restrict__ q,
+                                                     double* __restrict__ mut, const double* __restrict__ dqc,
+                                                     const double* const* __restrict__ xdp,
+                                                     const double* __restrict__ dist) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
+  const int nc = im * jm * km;
+  const int item = t.base + threadIdx.x;
+  if (item >= nc * p.nsnap) return;
+  const int n = item / nc;
+  int rem = item - n * nc;
+  const int i = rem % im + 1;
+  rem /= im;
+  const int j = rem % jm + 1, k = rem / jm + 1;
+  const ll n3 = n3of(b), c = i3(b, i, j, k);
+  const double* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n + c;
+  const double* db = dqc + b.o3 * 9 * p.nsnap + n3 * 9 * (ll)n + c;
+  const int nh = n * p.hbmove;
+  const double q1 = qb[0], q2 = qb[n3], q3 = qb[2 * n3], q4 = qb[3 * n3];
+  const double pr = (p.gamma - 1) * (qb[4 * n3] - (q2 * q2 + q3 * q3 + q4 * q4) / q1 / 2 - qb[5 * n3]);
+  const double rho = q1;
+  const double tke = qb[5 * n3] / rho;
+  const double ome = qb[6 * n3] / rho;
+  const double tt = p.gamma * pr / rho;
+  const double mu = suth<double>(p, tt);
+  const double xix = (mder<double>(p, b, xdp[0], i + 1, j, k, nh)[0] + mder<double>(p, b, xdp[0], i, j, k, nh)[0]) / 2;
+  const double xiy = (mder<double>(p, b, xdp[3], i, j + 1, k, nh)[1] + mder<double>(p, b, xdp[3], i, j, k, nh)[1]) / 2;
+  const double xiz = (mder<double>(p, b, xdp[6], i, j, k + 1, nh)[2] + mder<double>(p, b, xdp[6], i, j, k, nh)[2]) / 2;
+  const double etax = (mder<double>(p, b, xdp[1], i + 1, j, k, nh)[0] + mder<double>(p, b, xdp[1], i, j, k, nh)[0]) / 2;
+  const double etay = (mder<double>(p, b, xdp[4], i, j + 1, k, nh)[1] + mder<double>(p, b, xdp[4], i, j, k, nh)[1]) / 2;
+  const double etaz = (mder<double>(p, b, xdp[7], i, j, k + 1, nh)[2] + mder<double>(p, b, xdp[7], i, j, k, nh)[2]) / 2;
+  const double zetax =
+      (mder<double>(p, b, xdp[2], i + 1, j, k, nh)[0] + mder<double>(p, b, xdp[2], i, j, k, nh)[0]) / 2;
+  const double zetay =
+      (mder<double>(p, b, xdp[5], i, j + 1, k, nh)[1] + mder<double>(p, b, xdp[5], i, j, k, nh)[1]) / 2;
+  const double zetaz =
+      (mder<double>(p, b, xdp[8], i, j, k + 1, nh)[2] + mder<double>(p, b, xdp[8], i, j, k, nh)[2]) / 2;
+  const double uxi = db[0], ueta = db[n3], uzeta = db[2 * n3];
+  const double vxi = db[3 * n3], veta = db[4 * n3], vzeta = db[5 * n3];
+  const double wxi = db[6 * n3], weta = db[7 * n3], wzeta = db[8 * n3];
+  const double dudx = uxi * xix + ueta * etax + uzeta * zetax;
+  const double dudy = uxi * xiy + ueta * etay + uzeta * zetay;
+  const double dudz = uxi * xiz + ueta * etaz + uzeta * zetaz;
+  const double dvdx = vxi * xix + veta * etax + vzeta * zetax;
+  const double dvdy = vxi * xiy + veta * etay + vzeta * zetay;
+  const double dvdz = vxi * xiz + veta * etaz + vzeta * zetaz;
+  const double dwdx = wxi * xix + weta * etax + wzeta * zetax;
+  const double dwdy = wxi * xiy + weta * etay + wzeta * zetay;
+  const double dwdz = wxi * xiz + weta * etaz + wzeta * zetaz;
+  const double sxx = dudx, sxy = (dudy + dvdx) / 2, sxz = (dudz + dwdx) / 2, syy = dvdy, syz = (dvdz + dwdy) / 2,
+               szz = dwdz;
+  const double sv = sxx * sxx + syy * syy + szz * szz + 2 * (sxy * sxy + sxz * sxz + syz * syz);
+  const double de = dist[b.o1 + i1(b, i, j, k)] + p.epst;
+  double arg2 = fmax(2 * sqrt(tke) / 0.09 / ome / de, 500 * mu / rho / ome / (de * de) * p.machfs / p.reyno);
+  arg2 = arg2 * arg2;
+  arg2 = fmin(arg2, 1e2);
+  const double e2 = exp(2 * arg2);
+  const double f2 = (e2 - 1) / (e2 + 1);
+  const double fct = sqrt(2 * sv);
+  mut[b.o3 * p.nsnap + n3 * (ll)n + c] = p.sst_a1 * rho * tke / fmax(p.sst_a1 * ome, fct * f2) * p.reyno / p.machfs;
+}
+
+// tur_vis_sst_b halo faces (src/update.f90:2488-2520): mut of the face
+// halos copied from the first interior layer (no edges)
+__global__ void __launch_bounds__(256) k_mut_faces(Par p, const Blk* __restrict__ blks,
+                                                   const Tile* __restrict__ tiles, double* __restrict__ mut) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
+  const int nc = im * jm * km;
+  const int item = t.base + threadIdx.x;
+  if (item >= nc * p.nsnap) return;
+  const int n = item / nc;
+  int rem = item - n * nc;
+  const int i = rem % im + 1;
+  rem /= im;
+  const int j = rem % jm + 1, k = rem / jm + 1;
+  const ll n3 = n3of(b);
+  double* mb = mut + b.o3 * p.nsnap + n3 * (ll)n;
+  const double v = mb[i3(b, i, j, k)];
+  if (k == 1) mb[i3(b, i, j, 0)] = v;
+  if (k == km) mb[i3(b, i, j, b.kmax)] = v;
+  if (j == 1) mb[i3(b, i, 0, k)] = v;
+  if (j == jm) mb[i3(b, i, b.jmax, k)] = v;
+  if (i == 1) mb[i3(b, 0, j, k)] = v;
+  if (i == im) mb[i3(b, b.imax, j, k)] = v;
+}
+
+// q_face + q_der_c of u, v, w (src/flux.f90:2391-2801, 2979-3131) into
+// dqc: the first visit of a coarse level computes SST mut from them
+// (src/cosa.f90:1739-1750); q primitive
+__global__ void __launch_bounds__(256) k_vel_diffs(Par p, const Blk* __restrict__ blks,
+                                                   const Tile* __restrict__ tiles, const double* __restrict__ q,
+                                                   double* __restrict__ dqc, const unsigned char* __restrict__ sbc,
+                                                   const ll* __restrict__ sbcoff) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
+  const int nc = im * jm * km;
+  const int item = t.base + threadIdx.x;
+  if (item >= nc * p.nsnap) return;
+  const int n = item / nc;
+  int rem = item - n * nc;
+  const int i = rem % im + 1;
+  rem /= im;
+  const int j = rem % jm + 1, k = rem / jm + 1;
+  const ll n3 = n3of(b), c = i3(b, i, j, k);
+  const double* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  const ll st[4] = {0, 1, (ll)(b.imax + 3), (ll)(b.imax + 3) * (b.jmax + 3)};
+  const unsigned char* fl = sbc + sbcoff[t.block];
+  double* db = dqc + b.o3 * 9 * p.nsnap + n3 * 9 * (ll)n + c;
+  for (int d = 1; d <= 3; d++) {
+    const int a = d == 1 ? i : (d == 2 ? j : k);
+    const int L = d == 1 ? b.imax : (d == 2 ? b.jmax : b.kmax);
+    const int t1 = d == 1 ? j : i, t2 = d == 3 ? j : k;
+    bool wlo = false, whi = false;
+    if (a == 1) {
+      const unsigned char f = d == 1 ? face_flag<1>(b, fl, 0, t1, t2)
+                                     : (d == 2 ? face_flag<2>(b, fl, 0, t1, t2) : face_flag<3>(b, fl, 0, t1, t2));
+      wlo = (f & 1) != 0;
+    }
+    if (a + 1 == L) {
+      const unsigned char f = d == 1 ? face_flag<1>(b, fl, 1, t1, t2)
+                                     : (d == 2 ? face_flag<2>(b, fl, 1, t1, t2) : face_flag<3>(b, fl, 1, t1, t2));
+      whi = (f & 1) != 0;
+    }
+    for (int sv = 0; sv < 3; sv++) {
+      const double* qv = qb + n3 * (sv + 1);
+      const double flo = wlo ? qv[c - st[d]] : (qv[c - st[d]] + qv[c]) / 2;
+      const double fhi = whi ? qv[c + st[d]] : (qv[c] + qv[c + st[d]]) / 2;
+      db[n3 * (3 * sv + d - 1)] = fhi - flo;
+    }
+  }
 }
 
 // tur_vis_b over 0..imax x 0..jmax x 0..kmax (q primitive or conservative,
@@ -324,14 +519,17 @@ __global__ void __launch_bounds__(256) k_fixq(Par p, const Blk* __restrict__ blk
   }
 }
 
-// lim_prod_mg_b then lim_res_ome_b (restriction stage), interior, one
-// snapshot, q conservative
-__global__ void __launch_bounds__(256) k_turb_limits(Par p, int do_prod, double ar, const Blk* __restrict__ blks,
-                                                     const Tile* __restrict__ tiles, const double* __restrict__ q,
+// lim_prod_mg_b then lim_res_ome (restriction stage), interior, every
+// snapshot, q conservative.  lim: 1 lim_res_ome_b (steady), 2 the explicit
+// branch of lim_res_ome_bhb (src/update.f90:2880-2905: no RK coefficient,
+// from qold), 0 none (the rkim branch runs as its own kernel).
+__global__ void __launch_bounds__(256) k_turb_limits(Par p, int do_prod, int lim, double ar,
+                                                     const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
+                                                     const double* __restrict__ q, const double* __restrict__ qold,
                                                      double* __restrict__ res, const double* __restrict__ vol,
                                                      const double* __restrict__ dtvolt,
                                                      const double* __restrict__ delpls, double* __restrict__ prd,
-                                                     const double* __restrict__ mut) {
+                                                     const double* __restrict__ mut, const double* __restrict__ ttrm) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
@@ -342,25 +540,38 @@ __global__ void __launch_bounds__(256) k_turb_limits(Par p, int do_prod, double 
   const int i = rem % im + 1;
   rem /= im;
   const int j = rem % jm + 1, k = rem / jm + 1;
-  const ll n3 = n3of(b);
-  const double* qb = q + b.o3 * p.npde * p.nsnap;
-  double* rb = res + b.o3 * p.npde * p.nsnap;
-  const ll c = i3(b, i, j, k), so = b.o3 * p.nsnap + c;
+  const ll n3 = n3of(b), n1 = n1of(b);
+  const ll c = i3(b, i, j, k);
   const double mor = p.machfs / p.reyno;
-  if (do_prod) {
-    const double tstdis = p.prdlim * p.bstrkw * qb[c] * qb[c + 5 * n3] * qb[c + 6 * n3];
-    if (mor * mut[so] * prd[so] > tstdis) prd[so] = tstdis / mor / mut[so];
+  const double vv = vol[b.o1 + i1(b, i, j, k)];
+  for (int n = 0; n < p.nsnap; n++) {
+    // ttrm(2:3): gkw, bkw (Wilcox) or blended (BSL/SST)
+    const double* tb = p.sst ? ttrm + b.o3 * 3 * p.nsnap + n3 * 3 * (ll)n + c : nullptr;
+    const double gav = tb ? tb[n3] : p.gkw, bav = tb ? tb[2 * n3] : p.bkw;
+    const double* qb = q + b.o3 * p.npde * p.nsnap + n3 * p.npde * n;
+    double* rb = res + b.o3 * p.npde * p.nsnap + n3 * p.npde * n;
+    const ll so = b.o3 * p.nsnap + n3 * n + c;
+    if (do_prod) {
+      const double tstdis = p.prdlim * p.bstrkw * qb[c] * qb[c + 5 * n3] * qb[c + 6 * n3];
+      if (mor * mut[so] * prd[so] > tstdis) prd[so] = tstdis / mor / mut[so];
+    }
+    const double dtt = dtvolt[b.o1 * p.nsnap + n1 * n + i1(b, i, j, k)];
+    const double dtau = dtt * vv;
+    const double rho = qb[c];
+    const double ome = qb[c + 6 * n3] / qb[c];
+    const double rhomegamin = gav * p.cmu * rho * sqrt(prd[so]);
+    if (lim == 1) {
+      const double den1 = 1 + ar * dtau * (gav * delpls[so] + 2 * bav * ome);
+      const double rhomega = qb[c + 6 * n3] - ar * rb[c + 6 * n3] * dtt / den1;
+      if (rhomega < rhomegamin) rb[c + 6 * n3] = den1 * (qb[c + 6 * n3] - rhomegamin) / dtt / ar;
+    } else if (lim == 2) {
+      const double* ob = qold + b.o3 * p.npde * p.nsnap + n3 * p.npde * n;
+      const double rhomega =
+          ob[c + 6 * n3] - rb[c + 6 * n3] * dtt / (1 + dtau * (gav * delpls[so] + 2 * bav * ome));
+      rb[c + 6 * n3] = rb[c + 6 * n3] -
+                       (1 + dtau * (gav * delpls[so] + 2 * bav * ome)) * fmax(0.0, (rhomegamin - rhomega) / dtt);
+    }
   }
-  const double gav = p.gkw, bav = p.bkw;
-  const ll po = b.o1 * p.nsnap + i1(b, i, j, k);
-  const double dtt = dtvolt[po];
-  const double dtau = dtt * vol[b.o1 + i1(b, i, j, k)];
-  const double rho = qb[c];
-  const double ome = qb[c + 6 * n3] / qb[c];
-  const double den1 = 1 + ar * dtau * (gav * delpls[so] + 2 * bav * ome);
-  const double rhomega = qb[c + 6 * n3] - ar * rb[c + 6 * n3] * dtt / den1;
-  const double rhomegamin = gav * p.cmu * rho * sqrt(prd[so]);
-  if (rhomega < rhomegamin) rb[c + 6 * n3] = den1 * (qb[c + 6 * n3] - rhomegamin) / dtt / ar;
 }
 
 // ------------------------------------------------------------------------
@@ -382,16 +593,20 @@ static hipError_t residual_viscous_t(Solve& s, Level& L, const ResIn<T>& in, dou
       if (e != hipSuccess) return e;
     }
   } else {
-    // per direction: inviscid line march, then the face-parallel viscous
-    // fluxes into vflux and their difference (faster than fusing the
-    // viscous face work into the line march, which runs out of registers)
+    // per direction: inviscid line march, then the viscous face fluxes.
+    // Default (viscous_faces): k_vflux stores the face fluxes of direction d
+    // and their difference is added to res by the next kernel that reads and
+    // writes res -- the inviscid march of direction d + 1, or for d = 3 the
+    // k-omega sources (a difference kernel for laminar runs) -- in the
+    // reference's order.  Otherwise one marching kernel per direction does
+    // flux and difference.
     const Par porig = s.par;
     Par pi = porig;
     pi.viscous = 0;
     const int fk[3] = {TK_FACES_I, TK_FACES_J, TK_FACES_K};
     for (int d = 1; d <= 3; d++) {
       s.par = pi;
-      e = launch_flux_dir<T>(s, L, d, in, res);
+      e = launch_flux_dir<T>(s, L, d, in, res, (s.viscous_faces && d > 1) ? vflux : nullptr);
       s.par = porig;
       if (e != hipSuccess) return e;
       const T* xd = in.xd[3 * (d - 1)];
@@ -401,26 +616,23 @@ static hipError_t residual_viscous_t(Solve& s, Level& L, const ResIn<T>& in, dou
       const Tile* tl = L.tile[fk[d - 1]];
 #define VF(D, NV)                                                                                                 \
   hipLaunchKernelGGL((k_vflux<D, NV, T>), grid1(nt), dim3(TS), 0, s.stream, p, L.dblk, tl, in.q, in.mut, in.s[D - 1], \
-                     xd, ed, zd, L.sbc, L.sbc_doff, vflux)
-#define VR(D, NV)                                                                                          \
-  hipLaunchKernelGGL((k_vresid<D, NV, T>), grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, p, L.dblk,     \
-                     L.tile[TK_CELLS], vflux, res)
+                     xd, ed, zd, L.sbc, L.sbc_doff, vflux, L.f[COSAHIP_DIST])
 #define VI(NV)                                                                                                    \
   hipLaunchKernelGGL((k_vres_i<NV, T>), grid1(L.ntile[TK_ROWS_I]), dim3(256), 0, s.stream, p, L.dblk,             \
-                     L.tile[TK_ROWS_I], in.q, in.mut, in.s[0], xd, ed, zd, L.sbc, L.sbc_doff, res)
+                     L.tile[TK_ROWS_I], in.q, in.mut, in.s[0], xd, ed, zd, L.sbc, L.sbc_doff, res, L.f[COSAHIP_DIST])
 #define VM(D, NV, TK)                                                                                             \
   hipLaunchKernelGGL((k_vres_march<D, NV, T>), grid1(L.ntile[TK]), dim3(TS), 0, s.stream, p, L.dblk, L.tile[TK],   \
-                     in.q, in.mut, in.s[D - 1], xd, ed, zd, L.sbc, L.sbc_doff, res)
+                     in.q, in.mut, in.s[D - 1], xd, ed, zd, L.sbc, L.sbc_doff, res, L.f[COSAHIP_DIST])
       if (s.viscous_faces) {
         if (nt) {
           if (p.npde == 7) {
-            if (d == 1) { VF(1, 7); VR(1, 7); }
-            else if (d == 2) { VF(2, 7); VR(2, 7); }
-            else { VF(3, 7); VR(3, 7); }
+            if (d == 1) VF(1, 7);
+            else if (d == 2) VF(2, 7);
+            else VF(3, 7);
           } else {
-            if (d == 1) { VF(1, 5); VR(1, 5); }
-            else if (d == 2) { VF(2, 5); VR(2, 5); }
-            else { VF(3, 5); VR(3, 5); }
+            if (d == 1) VF(1, 5);
+            else if (d == 2) VF(2, 5);
+            else VF(3, 5);
           }
         }
       } else if (p.npde == 7) {
@@ -432,15 +644,20 @@ static hipError_t residual_viscous_t(Solve& s, Level& L, const ResIn<T>& in, dou
         else if (d == 2 && L.ntile[TK_SEGS_J]) VM(2, 5, TK_SEGS_J);
         else if (d == 3 && L.ntile[TK_SEGS_K]) VM(3, 5, TK_SEGS_K);
       }
-#undef VR
+#undef VF
 #undef VI
 #undef VM
     }
+    if (s.viscous_faces && !p.kom && L.ntile[TK_CELLS])   // laminar: difference of the direction-3 fluxes
+      hipLaunchKernelGGL((k_vresid<3, 5, T>), grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, p, L.dblk,
+                         L.tile[TK_CELLS], vflux, res);
   }
+  const bool vpre3 = p.viscous && !s.fuse_viscous && s.viscous_faces;
   if (p.kom && L.ntile[TK_CELLS])
     hipLaunchKernelGGL(k_turb_src<T>, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, p, L.nl, L.dblk,
                        L.tile[TK_CELLS], in.q, in.mut, L.f[COSAHIP_PRD], L.f[COSAHIP_DIVVEL], L.f[COSAHIP_DELPLS], res,
-                       L.f[COSAHIP_VOL], in.dxd, L.sbc, L.sbc_doff);
+                       L.f[COSAHIP_VOL], in.dxd, L.sbc, L.sbc_doff, vpre3 ? vflux : nullptr, L.f[COSAHIP_TTRM],
+                       L.dqc, L.f[COSAHIP_DIST]);
   return hipGetLastError();
 }
 
@@ -488,8 +705,46 @@ hipError_t launch_residual_viscous(Solve& s, Level& L, const double* q, double* 
   return residual_viscous_t<float>(s, L, in, res, reinterpret_cast<float*>(L.vflux));
 }
 
+__global__ void __launch_bounds__(256) k_zero_n(double* __restrict__ a, ll n) {
+  const ll x = (ll)blockIdx.x * 256 + threadIdx.x;
+  if (x < n) a[x] = 0.0;
+}
+
+// SST: tur_vis_sst from the velocity differences dqc of the stage's
+// residual, halo faces, cut
+static hipError_t launch_tur_vis_sst(Solve& s, Level& L, const double* q) {
+  if (L.ntile[TK_CELLS]) {
+    hipLaunchKernelGGL(k_tur_vis_sst, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                       L.tile[TK_CELLS], q, L.f[COSAHIP_MUT], L.dqc, (const double* const*)L.dxd,
+                       L.f[COSAHIP_DIST]);
+    hipLaunchKernelGGL(k_mut_faces, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_CELLS],
+                       L.f[COSAHIP_MUT]);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_cut_ncomp(s, L, L.f[COSAHIP_MUT], 1);
+}
+
+hipError_t launch_sst_first_mut(Solve& s, Level& L) {
+  L.mut32_ok = false;
+  const ll nq = L.n3tot * s.par.npde * s.par.nsnap;
+  double* res = L.f[COSAHIP_RES];
+  hipError_t e = hipMemcpyAsync(res, L.f[COSAHIP_Q], nq * sizeof(double), hipMemcpyDeviceToDevice, s.stream);
+  if (e != hipSuccess) return e;
+  if (L.ntile[TK_CELLS])
+    hipLaunchKernelGGL(k_vel_diffs, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_CELLS],
+                       res, L.dqc, L.sbc, L.sbc_doff);
+  e = launch_prim2cons(s, L, res);
+  if (e != hipSuccess) return e;
+  e = launch_tur_vis_sst(s, L, res);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_zero_n, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s.stream, res, nq);
+  return hipGetLastError();
+}
+
 hipError_t launch_tur_vis(Solve& s, Level& L, const double* q) {
   L.mut32_ok = false;
+  if (s.par.sst == 2) return launch_tur_vis_sst(s, L, q);
   if (L.ntile[TK_CELLS_P1])
     hipLaunchKernelGGL(k_tur_vis, grid1(L.ntile[TK_CELLS_P1]), dim3(TS), 0, s.stream, s.par, L.dblk,
                        L.tile[TK_CELLS_P1], q, L.f[COSAHIP_MUT]);
@@ -508,10 +763,13 @@ hipError_t launch_fixq(Solve& s, Level& L) {
 hipError_t launch_turb_limits(Solve& s, Level& L, double ar) {
   const Par& p = s.par;
   const int do_prod = p.nlevel > 1 && L.nl != p.nlevel && !p.prd_all_lvls && p.lim_prodtke;
+  const int lim = !s.cfg.harbal ? 1 : (s.cfg.rkimp ? 0 : 2);
   if (L.ntile[TK_CELLS])
-    hipLaunchKernelGGL(k_turb_limits, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, p, do_prod, ar, L.dblk,
-                       L.tile[TK_CELLS], L.f[COSAHIP_Q], L.f[COSAHIP_RES], L.f[COSAHIP_VOL], L.f[COSAHIP_DTVOLT],
-                       L.f[COSAHIP_DELPLS], L.f[COSAHIP_PRD], L.f[COSAHIP_MUT]);
+    hipLaunchKernelGGL(k_turb_limits, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, p, do_prod, lim, ar, L.dblk,
+                       L.tile[TK_CELLS], L.f[COSAHIP_Q], L.f[COSAHIP_QOLD], L.f[COSAHIP_RES], L.f[COSAHIP_VOL],
+                       L.f[COSAHIP_DTVOLT], L.f[COSAHIP_DELPLS], L.f[COSAHIP_PRD], L.f[COSAHIP_MUT],
+                       L.f[COSAHIP_TTRM]);
+  if (s.cfg.harbal && s.cfg.rkimp) return launch_lim_res_ome_rkim(s, L);
   return hipGetLastError();
 }
 

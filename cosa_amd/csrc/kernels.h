@@ -17,6 +17,9 @@ hipError_t launch_add_interior(Solve& s, Level& L, double* dst, const double* sr
 hipError_t launch_zero_res(Solve& s, Level& L, double* res);        // 0..imax box
 hipError_t launch_zero_res_shell(Solve& s, Level& L, double* res);  // first halo layer only
 hipError_t launch_update(Solve& s, Level& L, double ar);
+int rkim_group(int m);
+hipError_t launch_lim_res_ome_rkim(Solve& s, Level& L);
+hipError_t launch_rms_hb_term(Solve& s, Level& L, double* work1);
 hipError_t launch_src_hb(Solve& s, Level& L);
 hipError_t launch_rms(Solve& s, Level& L, const double* res, const double* rhs, int use_rhs);
 hipError_t launch_prtest(Solve& s, Level& L, const double* q, int stage_seq);
@@ -46,13 +49,16 @@ struct ResIn {
 hipError_t launch_lambda_deltat(Solve& s, Level& L);
 hipError_t launch_residual(Solve& s, Level& L, const double* q, double* res);
 hipError_t launch_cirs(Solve& s, Level& L);
+// vpre: viscous face fluxes of direction idir - 1 whose difference is added
+// to res first (nullptr: none)
 template <typename T>
-hipError_t launch_flux_dir(Solve& s, Level& L, int idir, const ResIn<T>& in, double* res);
+hipError_t launch_flux_dir(Solve& s, Level& L, int idir, const ResIn<T>& in, double* res, const T* vpre = nullptr);
 template <typename T>
 hipError_t launch_h0_last(Solve& s, Level& L, const ResIn<T>& in);
 
 // ---- viscous fluxes and Wilcox k-omega (k_turb.hip) ----
 hipError_t launch_residual_viscous(Solve& s, Level& L, const double* q, double* res);
+hipError_t launch_sst_first_mut(Solve& s, Level& L);
 hipError_t launch_tur_vis(Solve& s, Level& L, const double* q);
 hipError_t launch_fixq(Solve& s, Level& L);
 hipError_t launch_turb_limits(Solve& s, Level& L, double ar);

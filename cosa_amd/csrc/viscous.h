@@ -23,6 +23,25 @@ __device__ __forceinline__ unsigned char face_flag(const Blk& b, const unsigned 
             outer * (ll)(b.imax + 1) * (b.jmax + 1) + t1 + (ll)(b.imax + 1) * t2];
 }
 
+// res(c) = (res(c) + vf(c + e_d)) - vf(c) for the NV variables of cell
+// (i,j,k), snapshot n: the difference of the viscous face fluxes of
+// direction d stored by k_vflux (vflux(imax,jmax,kmax,npde,0:2N), off_0),
+// i.e. resid of the viscous fluxes (src/cosa.f90:1123-1211), applied inside
+// the kernel that next reads and writes res.
+template <int NV, typename T>
+__device__ __forceinline__ void add_vdiff(const Par& p, const Blk& b, const T* __restrict__ vf, int d, int i, int j,
+                                          int k, int n, double* __restrict__ rc, ll n3) {
+  const T* fb = vf + b.o0 * p.npde * p.nsnap;
+  const ll fst = (ll)b.imax * b.jmax * b.kmax;
+  const ll f0 = (ll)(i - 1) + (ll)b.imax * ((ll)(j - 1) + (ll)b.jmax * (ll)(k - 1));
+  const ll f1 = f0 + (d == 1 ? 1 : (d == 2 ? (ll)b.imax : (ll)b.imax * b.jmax));
+#pragma unroll
+  for (int m = 0; m < NV; m++) {
+    const ll o = fst * (m + (ll)p.npde * n);
+    rc[n3 * m] = rc[n3 * m] + (double)fb[f1 + o] - (double)fb[f0 + o];
+  }
+}
+
 // metric-derivative triple (3, imax, jmax, kmax, 0:2N*hbmove) at face (i,j,k)
 template <typename T>
 __device__ __forceinline__ const T* mder(const Par& p, const Blk& b, const T* a, int i, int j, int k, int nh) {
@@ -39,7 +58,8 @@ __device__ __forceinline__ void viscous_face(const Par& p, const Blk& b, int i, 
                                              const T* __restrict__ q, const T* __restrict__ mut,
                                              const T* __restrict__ sm, const T* __restrict__ xd,
                                              const T* __restrict__ ed, const T* __restrict__ zd,
-                                             const unsigned char* __restrict__ fl, T* F) {
+                                             const unsigned char* __restrict__ fl, const double* __restrict__ dist,
+                                             T* F) {
   const ll n3 = n3of(b);
   const T* qb = q + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
   const ll st[4] = {0, 1, (ll)(b.imax + 3), (ll)(b.imax + 3) * (b.jmax + 3)};
@@ -104,7 +124,7 @@ __device__ __forceinline__ void viscous_face(const Par& p, const Blk& b, int i, 
   }
   (void)d2;
   T mutf = 0;
-  if (p.kom) {
+  if (NV == 7) {
     const T* mb = mut + b.o3 * p.nsnap + n3 * (ll)n;
     mutf = wall ? mb[aux] : (mb[cm] + mb[cp]) / 2;
   }
@@ -164,8 +184,30 @@ __device__ __forceinline__ void viscous_face(const Par& p, const Blk& b, int i, 
     F[1] = -((r1 * kx + r2 * ky + r3 * kz) * ds);
     F[2] = -((r2 * kx + r4 * ky + r5 * kz) * ds);
     F[3] = -((r3 * kx + r5 * ky + r6 * kz) * ds);
-    const T ctke = (mu + (T)p.sigk * mutf) * (T)p.machfs / (T)p.reyno;
-    const T come = (mu + (T)p.sigw * mutf) * (T)p.machfs / (T)p.reyno;
+    T sk = (T)p.sigk, sw = (T)p.sigw;
+    if (p.sst) {
+      // BSL/SST: sigma_k, sigma_omega blended with F1 of the face
+      // (src/flux.f90:1035-1046); the face wall distance of facedist_b
+      // (src/grid.f90:2834-2884) from the cell-centre distance
+      const double* db = dist + b.o1;
+      const ll ihi = i1(b, i, j, k);
+      const ll ilo = DIR == 1 ? i1(b, i - 1, j, k) : (DIR == 2 ? i1(b, i, j - 1, k) : i1(b, i, j, k - 1));
+      const T fd = a == 1 ? (T)db[ilo] : (a == L ? (T)db[ihi] : (T)((db[ilo] + db[ihi]) / 2));
+      const T ome = qf[6];
+      const T domedz = omexi * xiz + omeeta * etaz + dd[2][6] * zetaz;
+      const T de = fd + (T)p.epst;
+      T arg1 = fmax(sqrt(tke) / (T)0.09 / ome / de, (T)500 * mu / rho / ome / (de * de) * (T)p.machfs / (T)p.reyno);
+      const T cdt = 2 * rho * (T)p.sigw2 / ome * (dtkedx * domedx + dtkedy * domedy + dtkedz * domedz);
+      const T acdt = fmax(cdt, (T)1e-20);
+      arg1 = fmin(arg1, 4 * rho * (T)p.sigw2 * tke / acdt / (de * de));
+      arg1 = fmin(arg1 * arg1 * arg1 * arg1, (T)1e2);   // arg1**4 as ((a a) a) a
+      const T e2 = exp(2 * arg1);
+      const T f1 = (e2 - 1) / (e2 + 1);
+      sk = f1 * (T)p.sigk1 + (1 - f1) * (T)p.sigk2;
+      sw = f1 * (T)p.sigw1 + (1 - f1) * (T)p.sigw2;
+    }
+    const T ctke = (mu + sk * mutf) * (T)p.machfs / (T)p.reyno;
+    const T come = (mu + sw * mutf) * (T)p.machfs / (T)p.reyno;
     F[4] = -(ctke * (dtkedx * kx + dtkedy * ky + dtkedz * kz) * ds);
     F[5] = -(ctke * (dtkedx * kx + dtkedy * ky + dtkedz * kz) * ds);
     // as in the reference: dk/dz in place of domega/dz

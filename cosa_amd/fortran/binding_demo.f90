@@ -27,7 +27,7 @@ program binding_demo
   cfg%kom = 0; cfg%kom_bsl = 0; cfg%kom_sst = 0; cfg%viscous = 0; cfg%moving = 0; cfg%harbal = 0
   cfg%foturb = 0; cfg%efx_setup_f = 0; cfg%efx_all_f = 0; cfg%eig_limit_f = 0; cfg%eig_lim_all_f = 0
   cfg%efx_setup_t = 0; cfg%efx_all_t = 0; cfg%eig_limit_t = 0; cfg%eig_lim_all_t = 0
-  cfg%entfx_f_w = 0; cfg%entfx_t_w = 0; cfg%bln_prol = 1; cfg%ho_rest = 0; cfg%pad0 = 0
+  cfg%entfx_f_w = 0; cfg%entfx_t_w = 0; cfg%bln_prol = 1; cfg%ho_rest = 0; cfg%rkimp = 0
   cfg%gamma = gam; cfg%machfs = mach; cfg%alpha = al; cfg%beta = 0.d0
   cfg%rhoinf = 1.d0; cfg%pinf = 1.d0 / gam; cfg%ainf = 1.d0
   cfg%cfl = 4.d0; cfg%cflt = 4.d0; cfg%rat = 2.d0; cfg%ratt = 2.d0; cfg%psirs = 0.d0; cfg%rkap = 0.333333d0

@@ -23,7 +23,7 @@ module cosahip_binding
   integer(c_int), parameter :: COSAHIP_ZETADERJ = 22, COSAHIP_XIDERK = 23, COSAHIP_ETADERK = 24
   integer(c_int), parameter :: COSAHIP_ZETADERK = 25, COSAHIP_PRD = 26, COSAHIP_DIVVEL = 27
   integer(c_int), parameter :: COSAHIP_DELPLS = 28, COSAHIP_DTVOLT = 29
-  integer(c_int), parameter :: COSAHIP_X = 30, COSAHIP_Y = 31, COSAHIP_Z = 32
+  integer(c_int), parameter :: COSAHIP_X = 30, COSAHIP_Y = 31, COSAHIP_Z = 32, COSAHIP_TTRM = 33
 
   ! cosahip_config, member for member
   type, bind(C) :: cosahip_config
@@ -34,7 +34,7 @@ module cosahip_binding
     integer(c_int32_t) :: efx_setup_f, efx_all_f, eig_limit_f, eig_lim_all_f
     integer(c_int32_t) :: efx_setup_t, efx_all_t, eig_limit_t, eig_lim_all_t
     integer(c_int32_t) :: entfx_f_w(3), entfx_t_w(3)
-    integer(c_int32_t) :: bln_prol, ho_rest, pad0
+    integer(c_int32_t) :: bln_prol, ho_rest, rkimp
     real(c_double) :: gamma, machfs, alpha, beta
     real(c_double) :: rhoinf, pinf, ainf
     real(c_double) :: cfl, cflt, rat, ratt, psirs, rkap

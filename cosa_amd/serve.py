@@ -6,10 +6,11 @@ Requests arrive over time, each a NACA 0012 O-grid solve at its own Mach
 number, incidence, grid size and (optionally) harmonic-balance pitching.
 The server keeps every admitted solve resident in HBM (its own
 cosahip_solve and HIP stream), admits waiting requests while the estimated
-device footprint of the resident solves stays within a budget, and runs
-rounds: one V-cycle is enqueued for every resident solve (graph replays on
-separate streams, so their kernels overlap on the device), then each is
-waited for; a solve retires when its density residual rms has dropped by
+device footprint of the resident solves stays within a budget, and
+keeps one V-cycle of every resident solve in flight (graph replays on
+separate streams, so their kernels overlap on the device): a finished cycle
+is read and the next one enqueued at once; a solve retires when its density
+residual rms has dropped by
 `drop` from its first cycle (or after `max_cycles`), its force coefficients
 are read and its device memory is freed.  Reported: time-to-converged per
 request (from arrival), p50 / p99, converged solves per second and the
@@ -120,19 +121,29 @@ class Server:
                 if r.precision:   # mode 2: threshold set after the first cycle
                     s.set_precision(r.precision, 1e-300 if r.precision == 2 else 0.0)
                 active[r.rid] = dict(req=r, case=c, solve=s, mgit=0, need=need, t_admit=clock() - t0,
-                                     first=None)
+                                     first=None, pending=False)
                 used += need
             if not active:
                 # idle until the next arrival
                 time.sleep(max(0.0, min(0.001, waiting[0].arrival - now)))
                 continue
-            for a in active.values():
-                a["mgit"] += 1
-                a["solve"].cycle_enqueue(a["mgit"])
+            # every resident solve keeps one cycle in flight: a finished cycle
+            # is read, tested and followed at once by the next (no lockstep
+            # rounds, so short and long cycles of different grids overlap)
+            progressed = False
             retire = []
             for rid, a in active.items():
                 s, r = a["solve"], a["req"]
+                if not a["pending"]:
+                    a["mgit"] += 1
+                    s.cycle_enqueue(a["mgit"])
+                    a["pending"] = progressed = True
+                    continue
+                if not s.cycle_ready():
+                    continue
                 s.cycle_wait()
+                a["pending"] = False
+                progressed = True
                 r1 = float(s.history[-1][0])
                 if a["first"] is None:
                     a["first"] = r1
@@ -141,6 +152,12 @@ class Server:
                 conv = r1 <= r.drop * a["first"]
                 if conv or a["mgit"] >= r.max_cycles:
                     retire.append((rid, conv))
+                else:
+                    a["mgit"] += 1
+                    s.cycle_enqueue(a["mgit"])
+                    a["pending"] = True
+            if not progressed and not retire:
+                time.sleep(2e-5)
             for rid, conv in retire:
                 a = active.pop(rid)
                 s, r, c = a["solve"], a["req"], a["case"]

@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _lib
 from .case import Case
-from .config import to_struct
+from .config import komega, to_struct
 
 
 class Solve:
@@ -33,7 +33,7 @@ class Solve:
             self._define()
             if q is not None:
                 self.upload("q", q, 1)
-                if case.config["kom"]:
+                if komega(case.config):
                     self.upload("mut", case.turbulent_viscosity(q, 1), 1)
         except Exception:
             self.close()
@@ -66,7 +66,7 @@ class Solve:
                     self.upload(name, arr, L.nl)
             q0 = c.freestream(L.nl)
             self.upload("q", q0, L.nl)
-            if c.config["kom"]:
+            if komega(c.config):
                 # the driver's mut after init2fs (src/cosa.f90:229-232); coarse
                 # levels get theirs on their first visit
                 self.upload("mut", c.turbulent_viscosity(q0, L.nl) if L.nl == 1 else
@@ -144,7 +144,7 @@ class Solve:
     def write_restart(self, path: str, nl: int = 1):
         """The reference's restart file (writerest) of the resident state."""
         from . import restart
-        kom = bool(self.case.config["kom"])
+        kom = komega(self.case.config)
         restart.write_restart(path, self.case, self.download("q", nl), self.download("mut", nl) if kom else None, nl)
 
     def read_restart(self, path: str, nl: int = 1):
@@ -198,6 +198,13 @@ class Solve:
         self._check(self.lib.cosahip_cycle_wait(self.h, done, _lib.dptr(r)), "cycle_wait")
         self.history.append(r)
         return bool(done[0])
+
+    def cycle_ready(self) -> bool:
+        """True when the enqueued cycle has finished (cycle_wait will not block)."""
+        r = self.lib.cosahip_cycle_ready(self.h)
+        if r < 0:
+            self._check(r, "cycle_ready")
+        return r == 1
 
     def cycle_async(self, mgit: int) -> bool:
         self.cycle_enqueue(mgit)

@@ -71,7 +71,8 @@ typedef struct cosahip_config {
   int32_t entfx_f_w[3], entfx_t_w[3];
   int32_t bln_prol;    /* 1: bilinear prolongation, 0: linear          */
   int32_t ho_rest;     /* 1: 64-weight residual restriction             */
-  int32_t pad0;
+  int32_t rkimp;       /* 1: HB implicit RK (rkim): the spectral term solved
+                          per cell with an LU of (2N+1)x(2N+1) (update_bhb) */
   double gamma, machfs, alpha, beta;       /* alpha, beta in radians       */
   double rhoinf, pinf, ainf;
   double cfl, cflt, rat, ratt, psirs, rkap;
@@ -146,7 +147,10 @@ enum {
                            only by cosahip_force                            */
   COSAHIP_Y = 31,
   COSAHIP_Z = 32,
-  COSAHIP_NFIELDS = 33
+  COSAHIP_TTRM = 33,    /* BSL/SST blended coefficients (P3, 3: cross diffusion,
+                           gamma, beta; 0:2N), refreshed by every residual;
+                           download only                                     */
+  COSAHIP_NFIELDS = 34
 };
 
 typedef struct cosahip_solve cosahip_solve;
@@ -266,6 +270,10 @@ int cosahip_prolong(cosahip_solve* s, int nl);
  * streams run concurrently.                                              */
 int cosahip_cycle_enqueue(cosahip_solve* s, int mgit);
 int cosahip_cycle_wait(cosahip_solve* s, int32_t* done, double* resrms);
+/* 1 when the enqueued cycle has finished on the device (cosahip_cycle_wait
+ * then returns without blocking), 0 while it runs, < 0 on error; no
+ * counterpart in the reference (a serving loop polls its solves with it). */
+int cosahip_cycle_ready(cosahip_solve* s);
 
 /* ---- forces ------------------------------------------------------------
  * force_b (src/output.f90:4388-4886) summed over the level's blocks as

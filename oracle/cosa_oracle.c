@@ -65,6 +65,19 @@ static int cyc[3][3] = {{1, 2, 3}, {2, 3, 1}, {3, 1, 2}}; /* cyc(idir,m) */
 #define KRD(a, b) ((a) == (b) ? 1 : 0)
 
 static int is_kom(void) { return P.kom || P.kom_bsl || P.kom_sst; }
+static int is_sst2(void) { return P.kom_bsl || P.kom_sst; }
+
+/* BSL / SST constants, fixed by input (src/input.f90:2523-2541) */
+static double SST_A1 = 0.31, SIGK1 = 0.5, SIGK2 = 1.0, SIGW1 = 0.5, SIGW2 = 0.856, B1 = 3.0 / 40.0, B2 = 0.0828,
+              G1 = 0.0, G2 = 0.0;
+static void set_sst_constants(void) {
+  const double kappa = 0.41, bstr = 0.09;
+  SIGK1 = P.kom_sst ? 0.85 : 0.5;
+  G1 = B1 / bstr - SIGW1 * (kappa * kappa) / sqrt(bstr);
+  G2 = B2 / bstr - SIGW2 * (kappa * kappa) / sqrt(bstr);
+}
+/* ttrm(-1:imax+1,-1:jmax+1,-1:kmax+1,3,0:2N): cross diffusion, gamma, beta */
+#define T3(a, g, i, j, k, m, n) (a)[I3(g, i, j, k) + (g).n3 * ((ll)((m) - 1) + 3LL * (n))]
 
 void or_set_config(const cosahip_config* cfg) {
   P = *cfg;
@@ -73,6 +86,7 @@ void or_set_config(const cosahip_config* cfg) {
   DIM5 = 2 * NH + 1;
   DIM5H = 2 * NH * P.hbmove + 1;
   LMET = P.lmet;
+  set_sst_constants();
 }
 
 void or_set_dhb(int nsnap, const double* dhb) {
@@ -1719,8 +1733,8 @@ void or_rtst_b(int idir, const double* qf, const double* dqxi, const double* dqe
  * Kept as in the reference: the omega diffusion uses dk/dz in place of
  * domega/dz, and the energy flux's z term repeats the y-row stresses.    */
 void or_vflux_b(int idir, const double* qf, const double* dqxi, const double* dqeta, const double* dqzeta,
-                const double* tau1, double* flux, const double* xider, const double* etader, const double* zetader,
-                const double* sijk, int imax, int jmax, int kmax) {
+                const double* tau1, double* flux, const double* fdist, const double* xider, const double* etader,
+                const double* zetader, const double* sijk, int imax, int jmax, int kmax) {
   bg g = mkbg(imax, jmax, kmax);
   int ic, jc, kc;
   face_ranges(idir, g, &ic, &jc, &kc);
@@ -1757,8 +1771,26 @@ void or_vflux_b(int idir, const double* qf, const double* dqxi, const double* dq
             F0(flux, g, i, j, k, 2, n) = -((txx * kx + txy * ky + txz * kz) * ds);
             F0(flux, g, i, j, k, 3, n) = -((txy * kx + tyy * ky + tyz * kz) * ds);
             F0(flux, g, i, j, k, 4, n) = -((txz * kx + tyz * ky + tzz * kz) * ds);
-            double ctke = (mu + P.sigk * mut) * P.machfs / P.reyno;
-            double come = (mu + P.sigw * mut) * P.machfs / P.reyno;
+            double sk = P.sigk, sw = P.sigw;
+            if (is_sst2()) {
+              /* BSL/SST: sigma_k, sigma_omega blended with F1 of the face
+               * (src/flux.f90:1035-1046), fdist the face wall distance */
+              const double fd = fdist[(ll)(i - 1) + (ll)imax * ((ll)(j - 1) + (ll)jmax * (ll)(k - 1))];
+              const double tke = Q(qf, g, i, j, k, 6, n), ome = Q(qf, g, i, j, k, 7, n);
+              double arg1 = fmax(sqrt(tke) / 0.09 / ome / (fd + P.epst),
+                                 500 * mu / rho / ome / ((fd + P.epst) * (fd + P.epst)) * P.machfs / P.reyno);
+              const double dmz = omexi * xiz + omeeta * etaz + omezeta * zetaz;
+              const double cdt = 2 * rho * SIGW2 / ome * (dtkedx * domedx + dtkedy * domedy + dtkedz * dmz);
+              const double acdt = fmax(cdt, 1e-20);
+              arg1 = fmin(arg1, 4 * rho * SIGW2 * tke / acdt / ((fd + P.epst) * (fd + P.epst)));
+              /* arg1**4 as the reference build evaluates it: ((a a) a) a */
+              arg1 = fmin(arg1 * arg1 * arg1 * arg1, 1e2);
+              const double f1 = (exp(2 * arg1) - 1) / (exp(2 * arg1) + 1);
+              sk = f1 * SIGK1 + (1 - f1) * SIGK2;
+              sw = f1 * SIGW1 + (1 - f1) * SIGW2;
+            }
+            double ctke = (mu + sk * mut) * P.machfs / P.reyno;
+            double come = (mu + sw * mut) * P.machfs / P.reyno;
             F0(flux, g, i, j, k, 5, n) = -(ctke * (dtkedx * kx + dtkedy * ky + dtkedz * kz) * ds);
             F0(flux, g, i, j, k, 6, n) = -(ctke * (dtkedx * kx + dtkedy * ky + dtkedz * kz) * ds);
             F0(flux, g, i, j, k, 7, n) = -(come * (domedx * kx + domedy * ky + dtkedz * kz) * ds);
@@ -1834,11 +1866,13 @@ void or_q_der_c_b(int idir, const double* qf, double* dqc, int imax, int jmax, i
   bg g = mkbg(imax, jmax, kmax);
   int di = idir == 1, dj = idir == 2, dk = idir == 3;
   for (int n = 0; n <= 2 * NH; n++)
-    for (int ipde = 2; ipde <= 4; ipde++)
+    for (int ipde = 2; ipde <= 7; ipde++) {
+      if (ipde == 5 || (ipde > 5 && !is_sst2())) continue;   /* k, omega for BSL/SST */
       for (int k = 1; k <= kmax - 1; k++)
         for (int j = 1; j <= jmax - 1; j++)
           for (int i = 1; i <= imax - 1; i++)
             Q(dqc, g, i, j, k, ipde, n) = Q(qf, g, i + di, j + dj, k + dk, ipde, n) - Q(qf, g, i, j, k, ipde, n);
+    }
 }
 
 /* turprod_b (src/src.f90:40-175): P_d (second, low-round-off form),
@@ -1965,13 +1999,195 @@ void or_tur_vis_b(const double* q, double* mut, int imax, int jmax, int kmax) {
         }
 }
 
+/* the cell-centre metric derivatives of turprod_b / update_ttrm_b /
+ * tur_vis_sst_b: averages of the two face values along each direction   */
+static void cell_metrics(const double* const* xd, bg g, int i, int j, int k, int nh, double m[9]) {
+  m[0] = (MD(xd[0], g, 1, i + 1, j, k, nh) + MD(xd[0], g, 1, i, j, k, nh)) / 2; /* xix   */
+  m[1] = (MD(xd[3], g, 2, i, j + 1, k, nh) + MD(xd[3], g, 2, i, j, k, nh)) / 2; /* xiy   */
+  m[2] = (MD(xd[6], g, 3, i, j, k + 1, nh) + MD(xd[6], g, 3, i, j, k, nh)) / 2; /* xiz   */
+  m[3] = (MD(xd[1], g, 1, i + 1, j, k, nh) + MD(xd[1], g, 1, i, j, k, nh)) / 2; /* etax  */
+  m[4] = (MD(xd[4], g, 2, i, j + 1, k, nh) + MD(xd[4], g, 2, i, j, k, nh)) / 2; /* etay  */
+  m[5] = (MD(xd[7], g, 3, i, j, k + 1, nh) + MD(xd[7], g, 3, i, j, k, nh)) / 2; /* etaz  */
+  m[6] = (MD(xd[2], g, 1, i + 1, j, k, nh) + MD(xd[2], g, 1, i, j, k, nh)) / 2; /* zetax */
+  m[7] = (MD(xd[5], g, 2, i, j + 1, k, nh) + MD(xd[5], g, 2, i, j, k, nh)) / 2; /* zetay */
+  m[8] = (MD(xd[8], g, 3, i, j, k + 1, nh) + MD(xd[8], g, 3, i, j, k, nh)) / 2; /* zetaz */
+}
+
+/* facedist_b (src/grid.f90:2834-2884): wall distance of the idir faces,
+ * face array (imax, jmax, kmax), from the cell-centre distance.           */
+void or_facedist_b(int idir, const double* dist, double* fd, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+#define FD(i, j, k) fd[(ll)((i) - 1) + (ll)imax * ((ll)((j) - 1) + (ll)jmax * (ll)((k) - 1))]
+  if (idir == 1) {
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++) {
+        FD(1, j, k) = V1(dist, g, 0, j, k);
+        for (int i = 2; i <= imax - 1; i++) FD(i, j, k) = (V1(dist, g, i - 1, j, k) + V1(dist, g, i, j, k)) / 2;
+        FD(imax, j, k) = V1(dist, g, imax, j, k);
+      }
+  } else if (idir == 2) {
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int i = 1; i <= imax - 1; i++) {
+        FD(i, 1, k) = V1(dist, g, i, 0, k);
+        for (int j = 2; j <= jmax - 1; j++) FD(i, j, k) = (V1(dist, g, i, j - 1, k) + V1(dist, g, i, j, k)) / 2;
+        FD(i, jmax, k) = V1(dist, g, i, jmax, k);
+      }
+  } else {
+    for (int j = 1; j <= jmax - 1; j++)
+      for (int i = 1; i <= imax - 1; i++) {
+        FD(i, j, 1) = V1(dist, g, i, j, 0);
+        for (int k = 2; k <= kmax - 1; k++) FD(i, j, k) = (V1(dist, g, i, j, k - 1) + V1(dist, g, i, j, k)) / 2;
+        FD(i, j, kmax) = V1(dist, g, i, j, kmax);
+      }
+  }
+#undef FD
+}
+
+/* update_ttrm_b (src/update.f90:3036-3161): Wilcox -- ttrm = (0, gkw, bkw);
+ * BSL/SST -- F1 from the wall distance, k and omega gradients (cell-centre
+ * differences dq of the face averages), ttrm = (cross diffusion (1-F1),
+ * blended gamma, blended beta).  q primitive.                           */
+void or_update_ttrm_b(const double* q, const double* dqxi, const double* dqeta, const double* dqzeta,
+                      const double* const* xd, double* ttrm, const double* dist, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  for (int n = 0; n <= 2 * NH; n++) {
+    int nh = n * P.hbmove;
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          if (P.kom) {
+            T3(ttrm, g, i, j, k, 1, n) = 0.0;
+            T3(ttrm, g, i, j, k, 2, n) = P.gkw;
+            T3(ttrm, g, i, j, k, 3, n) = P.bkw;
+            continue;
+          }
+          double rho = Q(q, g, i, j, k, 1, n), tke = Q(q, g, i, j, k, 6, n), ome = Q(q, g, i, j, k, 7, n);
+          double p = Q(q, g, i, j, k, 5, n);
+          double t = P.gamma * p / rho;
+          double mu = suth(t);
+          double m[9];
+          cell_metrics(xd, g, i, j, k, nh, m);
+          double tkexi = Q(dqxi, g, i, j, k, 6, n), tkeeta = Q(dqeta, g, i, j, k, 6, n),
+                 tkezeta = Q(dqzeta, g, i, j, k, 6, n);
+          double omexi = Q(dqxi, g, i, j, k, 7, n), omeeta = Q(dqeta, g, i, j, k, 7, n),
+                 omezeta = Q(dqzeta, g, i, j, k, 7, n);
+          double dtkedx = tkexi * m[0] + tkeeta * m[3] + tkezeta * m[6];
+          double dtkedy = tkexi * m[1] + tkeeta * m[4] + tkezeta * m[7];
+          double dtkedz = tkexi * m[2] + tkeeta * m[5] + tkezeta * m[8];
+          double domedx = omexi * m[0] + omeeta * m[3] + omezeta * m[6];
+          double domedy = omexi * m[1] + omeeta * m[4] + omezeta * m[7];
+          double domedz = omexi * m[2] + omeeta * m[5] + omezeta * m[8];
+          double d = V1(dist, g, i, j, k) + P.epst;
+          double arg1 = fmax(sqrt(tke) / 0.09 / ome / d, 500 * mu / rho / ome / (d * d) * P.machfs / P.reyno);
+          double cdt = 2 * rho * SIGW2 / ome * (dtkedx * domedx + dtkedy * domedy + dtkedz * domedz);
+          double acdt = fmax(cdt, 1e-20);
+          arg1 = fmin(arg1, 4 * rho * SIGW2 * tke / acdt / (d * d));
+          arg1 = arg1 * arg1 * arg1 * arg1;   /* arg1**4: ((a a) a) a */
+          arg1 = fmin(arg1, 1e2);
+          double f1 = (exp(2 * arg1) - 1) / (exp(2 * arg1) + 1);
+          T3(ttrm, g, i, j, k, 1, n) = cdt * (1 - f1);
+          T3(ttrm, g, i, j, k, 2, n) = f1 * G1 + (1 - f1) * G2;
+          T3(ttrm, g, i, j, k, 3, n) = f1 * B1 + (1 - f1) * B2;
+        }
+  }
+}
+
+/* srckw_sst_b (src/src.f90:306-378): BSL/SST sources with the blended
+ * coefficients and cross diffusion of ttrm; q primitive.                 */
+void or_srckw_sst_b(const double* q, const double* prd, const double* divvel, double* res, const double* mutg,
+                    const double* ttrm, const double* vol, int nl, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  const double fact = 2.0 / 3.0, mor = P.machfs / P.reyno;
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          double rho = Q(q, g, i, j, k, 1, n), tke = Q(q, g, i, j, k, 6, n), ome = Q(q, g, i, j, k, 7, n);
+          double mut = S1(mutg, g, i, j, k, n);
+          double divv = S1(divvel, g, i, j, k, n);
+          double prod = S1(prd, g, i, j, k, n), prodw = S1(prd, g, i, j, k, n);
+          double cdif = T3(ttrm, g, i, j, k, 1, n), gav = T3(ttrm, g, i, j, k, 2, n), bav = T3(ttrm, g, i, j, k, 3, n);
+          if (P.lim_prodtke && (nl == 1 || P.prd_all_lvls)) {
+            double tstdis = P.prdlim * P.bstrkw * rho * tke * ome;
+            if (mor * mut * prod > tstdis) {
+              prod = tstdis / mor / mut;
+              if (P.lim_prodomega) prodw = prod;
+            }
+          }
+          double v = V1(vol, g, i, j, k);
+          Q(res, g, i, j, k, 6, n) =
+              Q(res, g, i, j, k, 6, n) - v * (mor * mut * prod - fact * divv * rho * tke - P.bstrkw * rho * tke * ome);
+          Q(res, g, i, j, k, 7, n) = Q(res, g, i, j, k, 7, n) - v * (gav * P.cmu * rho * prodw -
+                                                                     gav * fact * divv * rho * ome -
+                                                                     bav * rho * (ome * ome) + cdif);
+        }
+}
+
+/* tur_vis_sst_b (src/update.f90:2375-2524): SST eddy viscosity with the F2
+ * limiter on the strain rate; q conservative; velocity differences dq of
+ * the cell-centre face averages; halo faces copied from the first interior
+ * layer (k, then j, then i faces; no edges).                             */
+void or_tur_vis_sst_b(const double* q, double* mut, const double* dqxi, const double* dqeta, const double* dqzeta,
+                      const double* const* xd, const double* dist, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  for (int n = 0; n <= 2 * NH; n++) {
+    int nh = n * P.hbmove;
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          double q1 = Q(q, g, i, j, k, 1, n), q2 = Q(q, g, i, j, k, 2, n), q3 = Q(q, g, i, j, k, 3, n),
+                 q4 = Q(q, g, i, j, k, 4, n);
+          double p = (P.gamma - 1) * (Q(q, g, i, j, k, 5, n) - (q2 * q2 + q3 * q3 + q4 * q4) / q1 / 2 -
+                                      Q(q, g, i, j, k, 6, n));
+          double rho = q1;
+          double tke = Q(q, g, i, j, k, 6, n) / rho;
+          double ome = Q(q, g, i, j, k, 7, n) / rho;
+          double t = P.gamma * p / rho;
+          double mu = suth(t);
+          double m[9];
+          cell_metrics(xd, g, i, j, k, nh, m);
+          double uxi = Q(dqxi, g, i, j, k, 2, n), ueta = Q(dqeta, g, i, j, k, 2, n), uzeta = Q(dqzeta, g, i, j, k, 2, n);
+          double vxi = Q(dqxi, g, i, j, k, 3, n), veta = Q(dqeta, g, i, j, k, 3, n), vzeta = Q(dqzeta, g, i, j, k, 3, n);
+          double wxi = Q(dqxi, g, i, j, k, 4, n), weta = Q(dqeta, g, i, j, k, 4, n), wzeta = Q(dqzeta, g, i, j, k, 4, n);
+          double dudx = uxi * m[0] + ueta * m[3] + uzeta * m[6];
+          double dudy = uxi * m[1] + ueta * m[4] + uzeta * m[7];
+          double dudz = uxi * m[2] + ueta * m[5] + uzeta * m[8];
+          double dvdx = vxi * m[0] + veta * m[3] + vzeta * m[6];
+          double dvdy = vxi * m[1] + veta * m[4] + vzeta * m[7];
+          double dvdz = vxi * m[2] + veta * m[5] + vzeta * m[8];
+          double dwdx = wxi * m[0] + weta * m[3] + wzeta * m[6];
+          double dwdy = wxi * m[1] + weta * m[4] + wzeta * m[7];
+          double dwdz = wxi * m[2] + weta * m[5] + wzeta * m[8];
+          double sxx = dudx, sxy = (dudy + dvdx) / 2, sxz = (dudz + dwdx) / 2, syy = dvdy, syz = (dvdz + dwdy) / 2,
+                 szz = dwdz;
+          double sv = sxx * sxx + syy * syy + szz * szz + 2 * (sxy * sxy + sxz * sxz + syz * syz);
+          double d = V1(dist, g, i, j, k) + P.epst;
+          double arg2 = fmax(2 * sqrt(tke) / 0.09 / ome / d, 500 * mu / rho / ome / (d * d) * P.machfs / P.reyno);
+          arg2 = arg2 * arg2;
+          arg2 = fmin(arg2, 1e2);
+          double f2 = (exp(2 * arg2) - 1) / (exp(2 * arg2) + 1);
+          double fact = sqrt(2 * sv);
+          S1(mut, g, i, j, k, n) = SST_A1 * rho * tke / fmax(SST_A1 * ome, fact * f2) * P.reyno / P.machfs;
+        }
+    for (int k = 0; k <= kmax; k += kmax)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) S1(mut, g, i, j, k, n) = S1(mut, g, i, j, k == 0 ? 1 : kmax - 1, n);
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 0; j <= jmax; j += jmax)
+        for (int i = 1; i <= imax - 1; i++) S1(mut, g, i, j, k, n) = S1(mut, g, i, j == 0 ? 1 : jmax - 1, k, n);
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 0; i <= imax; i += imax) S1(mut, g, i, j, k, n) = S1(mut, g, i == 0 ? 1 : imax - 1, j, k, n);
+  }
+}
+
 /* update_b / update_bhb explicit branch with the Wilcox k-omega point-
  * implicit rho*omega, rho*k update (src/update.f90:596-634, 1327-1366);
  * ttrm(:,:,:,2:3) of kom are the constants gkw, bkw (update_ttrm_b,
  * src/update.f90:3069-3081).                                              */
 void or_update_kb(double* res, double* q, const double* qold, const double* vol, const double* dtvol,
                   const double* dtvolt, const double* delpls, const double* prd, double ar, int imax, int jmax,
-                  int kmax) {
+                  int kmax, const double* ttrm) {
   bg g = mkbg(imax, jmax, kmax);
   for (int n = 0; n <= 2 * NH; n++) {
     if (is_kom())
@@ -1983,7 +2199,7 @@ void or_update_kb(double* res, double* q, const double* qold, const double* vol,
             double tke = Q(q, g, i, j, k, 6, n) / Q(q, g, i, j, k, 1, n);
             double dtt = P1(dtvolt, g, i, j, k, n);
             double dtau = dtt * V1(vol, g, i, j, k);
-            double gav = P.gkw, bav = P.bkw;
+            double gav = ttrm ? T3(ttrm, g, i, j, k, 2, n) : P.gkw, bav = ttrm ? T3(ttrm, g, i, j, k, 3, n) : P.bkw;
             double dl = S1(delpls, g, i, j, k, n);
             double rhomega = (Q(qold, g, i, j, k, 7, n) - ar * Q(res, g, i, j, k, 7, n) * dtt +
                               ar * dtau * rho * ome * (gav * dl + 2 * bav * ome)) /
@@ -2009,12 +2225,12 @@ void or_update_kb(double* res, double* q, const double* qold, const double* vol,
 /* lim_res_ome_b (src/update.f90:2585-2835), explicit branch without
  * low-speed preconditioning; q conservative.                              */
 void or_lim_res_ome_b(double* res, const double* q, const double* vol, const double* dtvolt, const double* delpls,
-                      const double* prd, double ar, int imax, int jmax, int kmax) {
+                      const double* prd, double ar, int imax, int jmax, int kmax, const double* ttrm) {
   bg g = mkbg(imax, jmax, kmax);
   for (int k = 1; k <= kmax - 1; k++)
     for (int j = 1; j <= jmax - 1; j++)
       for (int i = 1; i <= imax - 1; i++) {
-        double gav = P.gkw, bav = P.bkw;
+        double gav = ttrm ? T3(ttrm, g, i, j, k, 2, 0) : P.gkw, bav = ttrm ? T3(ttrm, g, i, j, k, 3, 0) : P.bkw;
         double dtt = P1(dtvolt, g, i, j, k, 0);
         double dtau = dtt * V1(vol, g, i, j, k);
         double rho = Q(q, g, i, j, k, 1, 0);
@@ -2024,6 +2240,265 @@ void or_lim_res_ome_b(double* res, const double* q, const double* vol, const dou
         double rhomegamin = gav * P.cmu * rho * sqrt(S1(prd, g, i, j, k, 0));
         if (rhomega < rhomegamin) Q(res, g, i, j, k, 7, 0) = den1 * (Q(q, g, i, j, k, 7, 0) - rhomegamin) / dtt / ar;
       }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Dense LU with partial pivoting and its solve, as the reference's calls
+ * DGETRF / DGETRS('N') compute them (src/update.f90:1460-1478): LAPACK's
+ * right-looking elimination -- pivot = first row of largest |a| in the
+ * column, rows swapped whole, multipliers a(i,j) * (1/a(j,j)) when
+ * |a(j,j)| >= the smallest normal number, rank-1 update a(i,l) - a(i,j) a(j,l)
+ * in column order -- then row interchanges on b, forward substitution with
+ * the unit lower factor and back substitution with the upper one, both by
+ * columns.  a is m x m column-major, b is m x nrhs column-major.  LAPACK is
+ * not part of the reference tree (an external library of the build); this is
+ * its published algorithm.  Returns info as DGETRF: 0, or j+1 for the first
+ * zero pivot.                                                              */
+int or_lu_factor(double* a, int m, int* ipiv) {
+  int info = 0;
+  for (int j = 0; j < m; j++) {
+    int p = j;
+    double amax = fabs(a[j + (ll)m * j]);
+    for (int i = j + 1; i < m; i++)
+      if (fabs(a[i + (ll)m * j]) > amax) {
+        amax = fabs(a[i + (ll)m * j]);
+        p = i;
+      }
+    ipiv[j] = p;
+    if (a[p + (ll)m * j] != 0.0) {
+      if (p != j)
+        for (int l = 0; l < m; l++) {
+          double t = a[j + (ll)m * l];
+          a[j + (ll)m * l] = a[p + (ll)m * l];
+          a[p + (ll)m * l] = t;
+        }
+      const double piv = a[j + (ll)m * j];
+      if (fabs(piv) >= 2.2250738585072014e-308) {
+        const double r = 1.0 / piv;
+        for (int i = j + 1; i < m; i++) a[i + (ll)m * j] = a[i + (ll)m * j] * r;
+      } else {
+        for (int i = j + 1; i < m; i++) a[i + (ll)m * j] = a[i + (ll)m * j] / piv;
+      }
+    } else if (info == 0) {
+      info = j + 1;
+    }
+    for (int l = j + 1; l < m; l++) {
+      const double u = a[j + (ll)m * l];
+      if (u != 0.0)
+        for (int i = j + 1; i < m; i++) a[i + (ll)m * l] = a[i + (ll)m * l] - a[i + (ll)m * j] * u;
+    }
+  }
+  return info;
+}
+void or_lu_solve(const double* a, int m, const int* ipiv, double* b, int nrhs) {
+  for (int c = 0; c < nrhs; c++) {
+    double* x = b + (ll)m * c;
+    for (int j = 0; j < m; j++)
+      if (ipiv[j] != j) {
+        double t = x[j];
+        x[j] = x[ipiv[j]];
+        x[ipiv[j]] = t;
+      }
+    for (int k = 0; k < m; k++)
+      if (x[k] != 0.0)
+        for (int i = k + 1; i < m; i++) x[i] = x[i] - x[k] * a[i + (ll)m * k];
+    for (int k = m - 1; k >= 0; k--)
+      if (x[k] != 0.0) {
+        x[k] = x[k] / a[k + (ll)m * k];
+        for (int i = 0; i < k; i++) x[i] = x[i] - x[k] * a[i + (ll)m * k];
+      }
+  }
+}
+/* m x m system with nrhs right-hand sides, solved in place in b */
+int or_lu_solve_dense(int m, int nrhs, double* a, double* b) {
+  int ipiv[131];
+  if (m < 1 || m > 131) return -1;
+  int info = or_lu_factor(a, m, ipiv);
+  or_lu_solve(a, m, ipiv, b, nrhs);
+  return info;
+}
+
+/* update_bhb, implicit branch rkim (src/update.f90:1410-1841): per cell the
+ * 2N+1 snapshots are coupled through the spectral operator and solved with
+ * an LU, first rho*omega (limited at rhomegamin), then rho*k, then the mean
+ * flow (5 right-hand sides).  Without IRS and with one grid level the
+ * right-hand side is the explicit stage (src_hb was not added to res,
+ * src/cosa.f90:1916-1921); otherwise the HB term qhb0 (kept by src_hb) is
+ * taken back out of the smoothed residual.  gav, bav = ttrm(2:3) = gkw, bkw
+ * for Wilcox k-omega.  qhb0 is overwritten, as in the reference.          */
+void or_update_bhb_rkim(double* res, double* q, const double* qold, const double* vol, const double* dtvol,
+                        const double* dtvolt, double* qhb0, const double* delpls, const double* prd, double ar,
+                        int imax, int jmax, int kmax, const double* ttrm) {
+  bg g = mkbg(imax, jmax, kmax);
+  const int m = DIM5;
+  const int plain = P.irsop == 0 && P.nlevel == 1;
+  double* a = (double*)malloc(sizeof(double) * (size_t)m * m);
+  double* w = (double*)malloc(sizeof(double) * (size_t)m * 5);
+  double* old7 = (double*)malloc(sizeof(double) * (size_t)m);
+  double* drho = (double*)malloc(sizeof(double) * (size_t)m);
+  int ipiv[131];
+  for (int k = 1; k <= kmax - 1; k++)
+    for (int j = 1; j <= jmax - 1; j++)
+      for (int i = 1; i <= imax - 1; i++) {
+        const double vv = V1(vol, g, i, j, k);
+        if (is_kom()) {
+          /* rho*omega */
+          for (int ni = 0; ni < m; ni++) {
+            const double gav = ttrm ? T3(ttrm, g, i, j, k, 2, ni) : P.gkw;
+            const double bav = ttrm ? T3(ttrm, g, i, j, k, 3, ni) : P.bkw;
+            double ome = Q(q, g, i, j, k, 7, ni) / Q(q, g, i, j, k, 1, ni);
+            double dtt = P1(dtvolt, g, i, j, k, ni);
+            double dtau = dtt * vv;
+            double dps = S1(delpls, g, i, j, k, ni);
+            if (plain)
+              Q(qhb0, g, i, j, k, 7, ni) = Q(qold, g, i, j, k, 7, ni) - ar * Q(res, g, i, j, k, 7, ni) * dtt +
+                                           ar * dtau * Q(q, g, i, j, k, 7, ni) * (gav * dps + 2 * bav * ome);
+            else
+              Q(qhb0, g, i, j, k, 7, ni) = Q(qold, g, i, j, k, 7, ni) +
+                                           ar * dtt * (Q(qhb0, g, i, j, k, 7, ni) - Q(res, g, i, j, k, 7, ni)) +
+                                           ar * dtau * Q(q, g, i, j, k, 7, ni) * (gav * dps + 2 * bav * ome);
+            old7[ni] = Q(q, g, i, j, k, 7, ni);
+            for (int nj = 0; nj < m; nj++)
+              a[ni + (ll)m * nj] = (double)KRD(ni, nj) * (1 + ar * dtau * (gav * dps + 2 * bav * ome)) +
+                                   ar * P.omega * dtau * D(ni, nj);
+          }
+          or_lu_factor(a, m, ipiv);
+          for (int ni = 0; ni < m; ni++) w[ni] = Q(qhb0, g, i, j, k, 7, ni);
+          or_lu_solve(a, m, ipiv, w, 1);
+          for (int ni = 0; ni < m; ni++) {
+            const double gav = ttrm ? T3(ttrm, g, i, j, k, 2, ni) : P.gkw;
+            double rho = Q(q, g, i, j, k, 1, ni);
+            double rhomegamin = gav * P.cmu * rho * sqrt(S1(prd, g, i, j, k, ni));
+            if (w[ni] < rhomegamin) w[ni] = rhomegamin;
+            Q(q, g, i, j, k, 7, ni) = w[ni];
+            drho[ni] = Q(q, g, i, j, k, 7, ni) - old7[ni];
+          }
+          /* rho*k */
+          for (int ni = 0; ni < m; ni++) {
+            double rho = Q(q, g, i, j, k, 1, ni);
+            double ome = old7[ni] / Q(q, g, i, j, k, 1, ni);
+            double dtt = P1(dtvolt, g, i, j, k, ni);
+            double dtau = dtt * vv;
+            double dps = S1(delpls, g, i, j, k, ni);
+            if (plain)
+              Q(qhb0, g, i, j, k, 6, ni) =
+                  (Q(qold, g, i, j, k, 6, ni) - ar * Q(res, g, i, j, k, 6, ni) * dtt +
+                   ar * dtau * Q(q, g, i, j, k, 6, ni) * (dps + P.bstrkw * ome - P.bstrkw * drho[ni] / rho));
+            else
+              Q(qhb0, g, i, j, k, 6, ni) =
+                  (Q(qold, g, i, j, k, 6, ni) + ar * dtt * (Q(qhb0, g, i, j, k, 6, ni) - Q(res, g, i, j, k, 6, ni)) +
+                   ar * dtau * Q(q, g, i, j, k, 6, ni) * (dps + P.bstrkw * ome - P.bstrkw * drho[ni] / rho));
+            for (int nj = 0; nj < m; nj++)
+              a[ni + (ll)m * nj] =
+                  (double)KRD(ni, nj) * (1 + ar * dtau * (dps + P.bstrkw * ome)) + ar * P.omega * dtau * D(ni, nj);
+          }
+          or_lu_factor(a, m, ipiv);
+          for (int ni = 0; ni < m; ni++) w[ni] = Q(qhb0, g, i, j, k, 6, ni);
+          or_lu_solve(a, m, ipiv, w, 1);
+          for (int ni = 0; ni < m; ni++) Q(q, g, i, j, k, 6, ni) = w[ni];
+        }
+        /* mean flow */
+        for (int ni = 0; ni < m; ni++) {
+          double dt = P1(dtvol, g, i, j, k, ni);
+          for (int v = 1; v <= 5; v++) {
+            if (plain)
+              Q(qhb0, g, i, j, k, v, ni) = Q(qold, g, i, j, k, v, ni) - ar * Q(res, g, i, j, k, v, ni) * dt;
+            else
+              Q(qhb0, g, i, j, k, v, ni) =
+                  Q(qold, g, i, j, k, v, ni) + (Q(qhb0, g, i, j, k, v, ni) - Q(res, g, i, j, k, v, ni)) * ar * dt;
+          }
+          double dtau = dt * vv;
+          for (int nj = 0; nj < m; nj++)
+            a[ni + (ll)m * nj] = (double)KRD(ni, nj) + ar * P.omega * dtau * D(ni, nj);
+        }
+        or_lu_factor(a, m, ipiv);
+        for (int ni = 0; ni < m; ni++)
+          for (int v = 1; v <= 5; v++) w[ni + (ll)m * (v - 1)] = Q(qhb0, g, i, j, k, v, ni);
+        or_lu_solve(a, m, ipiv, w, 5);
+        for (int ni = 0; ni < m; ni++)
+          for (int v = 1; v <= 5; v++) Q(q, g, i, j, k, v, ni) = w[ni + (ll)m * (v - 1)];
+      }
+  free(a);
+  free(w);
+  free(old7);
+  free(drho);
+}
+
+/* lim_res_ome_bhb (src/update.f90:2836-2998) without low-speed
+ * preconditioning: the omega residual limited so that the stage would not
+ * take rho*omega below rhomegamin; explicit (per snapshot, no RK
+ * coefficient, from qold) or rkim (per cell, the snapshots coupled as in
+ * the update, right-hand side res*dtvolt).                                */
+void or_lim_res_ome_bhb(double* res, const double* q, const double* qold, const double* vol, const double* dtvolt,
+                        const double* delpls, const double* prd, int imax, int jmax, int kmax, const double* ttrm) {
+  bg g = mkbg(imax, jmax, kmax);
+  const int m = DIM5;
+#define GAV(n) (ttrm ? T3(ttrm, g, i, j, k, 2, n) : P.gkw)
+#define BAV(n) (ttrm ? T3(ttrm, g, i, j, k, 3, n) : P.bkw)
+  if (!P.rkimp) {
+    for (int n = 0; n < m; n++)
+      for (int k = 1; k <= kmax - 1; k++)
+        for (int j = 1; j <= jmax - 1; j++)
+          for (int i = 1; i <= imax - 1; i++) {
+            double rho = Q(q, g, i, j, k, 1, n);
+            double ome = Q(q, g, i, j, k, 7, n) / Q(q, g, i, j, k, 1, n);
+            double dtt = P1(dtvolt, g, i, j, k, n);
+            double dtau = dtt * V1(vol, g, i, j, k);
+            double dl = S1(delpls, g, i, j, k, n);
+            const double gav = GAV(n), bav = BAV(n);
+            double rhomega = Q(qold, g, i, j, k, 7, n) - Q(res, g, i, j, k, 7, n) * dtt / (1 + dtau * (gav * dl + 2 * bav * ome));
+            double rhomegamin = gav * P.cmu * rho * sqrt(S1(prd, g, i, j, k, n));
+            double res_ome = Q(res, g, i, j, k, 7, n) -
+                             (1 + dtau * (gav * dl + 2 * bav * ome)) * fmax(0.0, (rhomegamin - rhomega) / dtt);
+            Q(res, g, i, j, k, 7, n) = res_ome;
+          }
+    return;
+  }
+  double* a = (double*)malloc(sizeof(double) * (size_t)m * m);
+  double* w = (double*)malloc(sizeof(double) * (size_t)m);
+  int ipiv[131];
+  for (int k = 1; k <= kmax - 1; k++)
+    for (int j = 1; j <= jmax - 1; j++)
+      for (int i = 1; i <= imax - 1; i++) {
+        for (int ni = 0; ni < m; ni++) {
+          double dtau = P1(dtvolt, g, i, j, k, ni) * V1(vol, g, i, j, k);
+          double ome = Q(q, g, i, j, k, 7, ni) / Q(q, g, i, j, k, 1, ni);
+          double dps = S1(delpls, g, i, j, k, ni);
+          const double gav = GAV(ni), bav = BAV(ni);
+          for (int nj = 0; nj < m; nj++)
+            a[ni + (ll)m * nj] =
+                (double)KRD(ni, nj) * (1 + dtau * (gav * dps + 2 * bav * ome)) + P.omega * dtau * D(ni, nj);
+        }
+        or_lu_factor(a, m, ipiv);
+        for (int n = 0; n < m; n++) w[n] = Q(res, g, i, j, k, 7, n) * P1(dtvolt, g, i, j, k, n);
+        or_lu_solve(a, m, ipiv, w, 1);
+        for (int n = 0; n < m; n++) {
+          double rho = Q(q, g, i, j, k, 1, n);
+          double rhomegamin = GAV(n) * P.cmu * rho * sqrt(S1(prd, g, i, j, k, n));
+          double rhomeganew = Q(qold, g, i, j, k, 7, n) - w[n];
+          double res_ome = Q(res, g, i, j, k, 7, n) - fmax(0.0, (rhomegamin - rhomeganew)) / P1(dtvolt, g, i, j, k, n);
+          Q(res, g, i, j, k, 7, n) = res_ome;
+        }
+      }
+  free(a);
+  free(w);
+}
+#undef GAV
+#undef BAV
+
+/* resg_brms_hb with rkim, no IRS and one grid level (src/cosa.f90:4657-
+ * 4674): the HB term, which src_hb did not add to res, is added to the copy
+ * work1 before the sums: work1 += omega dhb(ni,nj) q(nj) vol in nj order. */
+void or_rms_hb_term(double* work1, const double* q, const double* vol, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  for (int ni = 0; ni < DIM5; ni++)
+    for (int nj = 0; nj < DIM5; nj++)
+      for (int v = 1; v <= NPDE; v++)
+        for (int k = 1; k <= kmax - 1; k++)
+          for (int j = 1; j <= jmax - 1; j++)
+            for (int i = 1; i <= imax - 1; i++)
+              Q(work1, g, i, j, k, v, ni) =
+                  Q(work1, g, i, j, k, v, ni) + P.omega * D(ni, nj) * Q(q, g, i, j, k, v, nj) * V1(vol, g, i, j, k);
 }
 
 /* rest4_b (src/cosa.f90:3122-3164): volume-weighted restriction of a
@@ -2306,6 +2781,8 @@ static ll off_bct(const or_level* L, int b) {
 #define FB(arr, L, b) ((arr) + off_of(L, b, 0) * NPDE * DIM5)
 #define SB(arr, L, b) ((arr) + off_of(L, b, 3) * DIM5)
 #define DB(arr, L, b) ((arr) + off_of(L, b, 0) * 3 * DIM5H)
+/* ttrm of block b, or NULL: Wilcox k-omega uses the constants gkw, bkw */
+#define TTB(L, b) (is_sst2() && (L)->ttrm ? (L)->ttrm + off_of(L, b, 3) * 3 * DIM5 : (const double*)NULL)
 
 void or_cons2prim(or_case* c, int nl, double* q) {
   or_level* L = &c->lev[nl - 1];
@@ -2401,8 +2878,14 @@ static void or_viscous_dir(or_case* c, int nl, int idir) {
     if (kom)
       or_rtst_b(idir, QB(L->dq, L, b), QB(L->qp, L, b), QB(L->qm, L, b), QB(L->dqm, L, b), xd0, xd1, xd2,
                 QB(L->dqp, L, b), FB(L->flux, L, b), im, jm, km);
+    double* fd = NULL;
+    if (is_sst2()) {
+      fd = (double*)malloc(sizeof(double) * (size_t)im * jm * km);
+      or_facedist_b(idir, VB(L->dist, L, b), fd, im, jm, km);
+    }
     or_vflux_b(idir, QB(L->dq, L, b), QB(L->qp, L, b), QB(L->qm, L, b), QB(L->dqm, L, b), QB(L->dqp, L, b),
-               FB(L->flux, L, b), xd0, xd1, xd2, MB(s, L, b), im, jm, km);
+               FB(L->flux, L, b), fd, xd0, xd1, xd2, MB(s, L, b), im, jm, km);
+    free(fd);
     or_bresid(idir, FB(L->flux, L, b), QB(L->res, L, b), im, jm, km);
   }
 }
@@ -2425,9 +2908,21 @@ static void or_komega_sources(or_case* c, int nl) {
       or_turprod_b(QB(L->q, L, b), QB(L->qp, L, b), QB(L->qm, L, b), QB(L->dqm, L, b), SB(L->prd, L, b),
                    SB(L->divvel, L, b), SB(L->delpls, L, b), xd, L->imax[b], L->jmax[b], L->kmax[b]);
     }
-  for (int b = 0; b < L->nblocks; b++)
-    or_srckw_b(QB(L->q, L, b), SB(L->mut, L, b), SB(L->prd, L, b), SB(L->divvel, L, b), QB(L->res, L, b),
-               VB(L->vol, L, b), nl, L->imax[b], L->jmax[b], L->kmax[b]);
+  if (P.kom)
+    for (int b = 0; b < L->nblocks; b++)
+      or_srckw_b(QB(L->q, L, b), SB(L->mut, L, b), SB(L->prd, L, b), SB(L->divvel, L, b), QB(L->res, L, b),
+                 VB(L->vol, L, b), nl, L->imax[b], L->jmax[b], L->kmax[b]);
+  else
+    for (int b = 0; b < L->nblocks; b++) {
+      /* update_ttrm then srckw_sst (src/cosa.f90:1896-1900) */
+      const double* xd[9];
+      for (int m = 0; m < 9; m++) xd[m] = DB(L->xd[m], L, b);
+      double* tt = L->ttrm + off_of(L, b, 3) * 3 * DIM5;
+      or_update_ttrm_b(QB(L->q, L, b), QB(L->qp, L, b), QB(L->qm, L, b), QB(L->dqm, L, b), xd, tt,
+                       VB(L->dist, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+      or_srckw_sst_b(QB(L->q, L, b), SB(L->prd, L, b), SB(L->divvel, L, b), QB(L->res, L, b), SB(L->mut, L, b), tt,
+                     VB(L->vol, L, b), nl, L->imax[b], L->jmax[b], L->kmax[b]);
+    }
 }
 
 void or_residual(or_case* c, int nl) {
@@ -2518,20 +3013,23 @@ void or_cirs(or_case* c, int nl) {
 void or_update(or_case* c, int nl, double ar) {
   or_level* L = &c->lev[nl - 1];
   for (int b = 0; b < L->nblocks; b++) {
-    if (is_kom())
+    if (P.harbal && P.rkimp)
+      or_update_bhb_rkim(QB(L->res, L, b), QB(L->q, L, b), QB(L->qold, L, b), VB(L->vol, L, b), PB(L->dtvol, L, b),
+                         PB(L->dtvolt, L, b), QB(L->dq, L, b), SB(L->delpls, L, b), SB(L->prd, L, b), ar,
+                         L->imax[b], L->jmax[b], L->kmax[b], TTB(L, b));
+    else if (is_kom())
       or_update_kb(QB(L->res, L, b), QB(L->q, L, b), QB(L->qold, L, b), VB(L->vol, L, b), PB(L->dtvol, L, b),
                    PB(L->dtvolt, L, b), SB(L->delpls, L, b), SB(L->prd, L, b), ar, L->imax[b], L->jmax[b],
-                   L->kmax[b]);
+                   L->kmax[b], TTB(L, b));
     else
       or_update_b(QB(L->res, L, b), QB(L->q, L, b), QB(L->qold, L, b), PB(L->dtvol, L, b), ar, L->imax[b],
                   L->jmax[b], L->kmax[b]);
   }
 }
 /* tur_vis + cutman_mut (src/update.f90:2272-2295; src/bc.f90:2713-2800) */
-void or_tur_vis(or_case* c, int nl) {
+/* cutman_mut (src/bc.f90:2713-2800): cut_q on the one-variable mut */
+static void or_cutman_mut(or_case* c, int nl) {
   or_level* L = &c->lev[nl - 1];
-  for (int b = 0; b < L->nblocks; b++)
-    or_tur_vis_b(QB(L->q, L, b), SB(L->mut, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
   int save = NPDE;
   NPDE = 1;   /* cut_q with one variable per snapshot */
   for (int icut = 0; icut < L->ncuts; icut++) {
@@ -2541,6 +3039,53 @@ void or_tur_vis(or_case* c, int nl) {
              SB(L->mut, L, b2), t);
   }
   NPDE = save;
+}
+
+/* SST mut on the first visit of a coarse level (src/cosa.f90:1739-1757):
+ * the cell-centre differences of a primitive copy of q (in res), that copy
+ * made conservative, tur_vis_sst, res cleared over the whole box.        */
+static void or_sst_first_mut(or_case* c, int nl) {
+  or_level* L = &c->lev[nl - 1];
+  double* outs[3] = {L->qp, L->qm, L->dqm};
+  for (int b = 0; b < L->nblocks; b++) {
+    bg g = mkbg(L->imax[b], L->jmax[b], L->kmax[b]);
+    memcpy(QB(L->res, L, b), QB(L->q, L, b), sizeof(double) * (size_t)g.n3 * NPDE * DIM5);
+  }
+  for (int idir = 1; idir <= 3; idir++)
+    for (int b = 0; b < L->nblocks; b++) {
+      or_q_face_b(idir, L->bctopo + off_bct(L, b), L->nbcs[b], QB(L->res, L, b), QB(L->dq, L, b), NULL, NULL, 0,
+                  L->imax[b], L->jmax[b], L->kmax[b]);
+      or_q_der_c_b(idir, QB(L->dq, L, b), QB(outs[idir - 1], L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+    }
+  or_prim2cons(c, nl, L->res);
+  for (int b = 0; b < L->nblocks; b++) {
+    const double* xd[9];
+    for (int m = 0; m < 9; m++) xd[m] = DB(L->xd[m], L, b);
+    or_tur_vis_sst_b(QB(L->res, L, b), SB(L->mut, L, b), QB(L->qp, L, b), QB(L->qm, L, b), QB(L->dqm, L, b), xd,
+                     VB(L->dist, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+  }
+  for (int b = 0; b < L->nblocks; b++) {
+    bg g = mkbg(L->imax[b], L->jmax[b], L->kmax[b]);
+    memset(QB(L->res, L, b), 0, sizeof(double) * (size_t)g.n3 * NPDE * DIM5);
+  }
+  or_cutman_mut(c, nl);
+}
+
+void or_tur_vis(or_case* c, int nl) {
+  or_level* L = &c->lev[nl - 1];
+  for (int b = 0; b < L->nblocks; b++) {
+    if (P.kom_sst) {
+      /* tur_vis_sst with the differences q_der_c left in qp, qm, dqm by the
+       * stage's residual (src/cosa.f90:2008-2011) */
+      const double* xd[9];
+      for (int m = 0; m < 9; m++) xd[m] = DB(L->xd[m], L, b);
+      or_tur_vis_sst_b(QB(L->q, L, b), SB(L->mut, L, b), QB(L->qp, L, b), QB(L->qm, L, b), QB(L->dqm, L, b), xd,
+                       VB(L->dist, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+    } else {
+      or_tur_vis_b(QB(L->q, L, b), SB(L->mut, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
+    }
+  }
+  or_cutman_mut(c, nl);
 }
 void or_fixq(or_case* c, int nl) {
   or_level* L = &c->lev[nl - 1];
@@ -2552,9 +3097,14 @@ void or_turb_limits(or_case* c, int nl, double ar) {
   if (P.nlevel > 1 && nl != P.nlevel && !P.prd_all_lvls && P.lim_prodtke)
     for (int b = 0; b < L->nblocks; b++)
       or_lim_prod_mg_b(QB(L->q, L, b), SB(L->prd, L, b), SB(L->mut, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
-  for (int b = 0; b < L->nblocks; b++)
-    or_lim_res_ome_b(QB(L->res, L, b), QB(L->q, L, b), VB(L->vol, L, b), PB(L->dtvolt, L, b), SB(L->delpls, L, b),
-                     SB(L->prd, L, b), ar, L->imax[b], L->jmax[b], L->kmax[b]);
+  for (int b = 0; b < L->nblocks; b++) {
+    if (P.harbal)
+      or_lim_res_ome_bhb(QB(L->res, L, b), QB(L->q, L, b), QB(L->qold, L, b), VB(L->vol, L, b), PB(L->dtvolt, L, b),
+                         SB(L->delpls, L, b), SB(L->prd, L, b), L->imax[b], L->jmax[b], L->kmax[b], TTB(L, b));
+    else
+      or_lim_res_ome_b(QB(L->res, L, b), QB(L->q, L, b), VB(L->vol, L, b), PB(L->dtvolt, L, b), SB(L->delpls, L, b),
+                       SB(L->prd, L, b), ar, L->imax[b], L->jmax[b], L->kmax[b], TTB(L, b));
+  }
 }
 
 /* ---------------------------------------------------------------------- */
@@ -2569,7 +3119,10 @@ int or_smooth(or_case* c, int nl, int from_fine, int to_coarse, int niter, int m
       or_cons2prim(c, nl, L->q);
       or_bc(c, nl, L->q);
       or_cutman_q(c, nl, L->q);
-      if (is_kom() && mgit == 1 && iter == 1 && irk == 1 && from_fine && !P.prd_all_lvls) or_tur_vis(c, nl);
+      if (is_kom() && mgit == 1 && iter == 1 && irk == 1 && from_fine && !P.prd_all_lvls) {
+        if (P.kom_sst) or_sst_first_mut(c, nl);
+        else or_tur_vis(c, nl);
+      }
       int bad = or_prtest(c, nl, where);
       if (bad) {
         if (where) where[5] = iter;
@@ -2578,10 +3131,14 @@ int or_smooth(or_case* c, int nl, int from_fine, int to_coarse, int niter, int m
       if (iter % P.iupdt == 0 && irk == 1) or_lambda_deltat(c, nl);
       or_residual(c, nl);
       or_prim2cons(c, nl, L->q);
-      if (P.harbal) or_src_hb(c, nl);
+      const int rkim_plain = P.rkimp && P.irsop == 0 && P.nlevel == 1;
+      if (P.harbal && !rkim_plain) or_src_hb(c, nl);
       if (nl == 1 && irk == 1 && iter != niter + 1 &&
           (mgit == 1 || mgit % P.iprint == 0 || mgit == P.ncycle)) {
         if (P.harbal) or_copy_array(c, nl, L->work1, L->res);
+        if (P.harbal && rkim_plain)
+          for (int b = 0; b < L->nblocks; b++)
+            or_rms_hb_term(QB(L->work1, L, b), QB(L->q, L, b), VB(L->vol, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
         double rr[8];
         or_resg_rms(c, nl, rr);
         for (int p = 0; p <= NPDE; p++) RESRMS[p + 1] = rr[p];

@@ -38,6 +38,7 @@ typedef struct or_level {
   /* viscous / k-omega (NULL otherwise) */
   double *mut, *dist, *prd, *divvel, *delpls, *dtvolt, *tbetaxi, *tbetaeta, *tbetazeta;
   double* xd[9];   /* xideri, etaderi, zetaderi, xiderj, ..., zetaderk */
+  double* ttrm;    /* (P3, 3, 0:2N): cross diffusion, gamma, beta (k-omega) */
 } or_level;
 
 typedef struct or_case {
@@ -117,8 +118,8 @@ void or_rtst_b(int idir, const double* qf, const double* dqxi, const double* dqe
                const double* xider, const double* etader, const double* zetader, double* tau1, const double* mutf,
                int imax, int jmax, int kmax);
 void or_vflux_b(int idir, const double* qf, const double* dqxi, const double* dqeta, const double* dqzeta,
-                const double* tau1, double* flux, const double* xider, const double* etader, const double* zetader,
-                const double* sijk, int imax, int jmax, int kmax);
+                const double* tau1, double* flux, const double* fdist, const double* xider, const double* etader,
+                const double* zetader, const double* sijk, int imax, int jmax, int kmax);
 void or_q_der_c_b(int idir, const double* qf, double* dqc, int imax, int jmax, int kmax);
 void or_turprod_b(const double* q, const double* dqxi, const double* dqeta, const double* dqzeta, double* prd,
                   double* divvel, double* delpls, const double* const* xd, int imax, int jmax, int kmax);
@@ -127,11 +128,27 @@ void or_srckw_b(const double* q, const double* mutg, const double* prd, const do
 void or_lim_prod_mg_b(const double* q, double* prd, const double* mutg, int imax, int jmax, int kmax);
 void or_fixq_b(double* q, int imax, int jmax, int kmax);
 void or_tur_vis_b(const double* q, double* mut, int imax, int jmax, int kmax);
+void or_facedist_b(int idir, const double* dist, double* fd, int imax, int jmax, int kmax);
+void or_update_ttrm_b(const double* q, const double* dqxi, const double* dqeta, const double* dqzeta,
+                      const double* const* xd, double* ttrm, const double* dist, int imax, int jmax, int kmax);
+void or_srckw_sst_b(const double* q, const double* prd, const double* divvel, double* res, const double* mutg,
+                    const double* ttrm, const double* vol, int nl, int imax, int jmax, int kmax);
+void or_tur_vis_sst_b(const double* q, double* mut, const double* dqxi, const double* dqeta, const double* dqzeta,
+                      const double* const* xd, const double* dist, int imax, int jmax, int kmax);
 void or_update_kb(double* res, double* q, const double* qold, const double* vol, const double* dtvol,
                   const double* dtvolt, const double* delpls, const double* prd, double ar, int imax, int jmax,
-                  int kmax);
+                  int kmax, const double* ttrm);
 void or_lim_res_ome_b(double* res, const double* q, const double* vol, const double* dtvolt, const double* delpls,
-                      const double* prd, double ar, int imax, int jmax, int kmax);
+                      const double* prd, double ar, int imax, int jmax, int kmax, const double* ttrm);
+int or_lu_factor(double* a, int m, int* ipiv);
+void or_lu_solve(const double* a, int m, const int* ipiv, double* b, int nrhs);
+int or_lu_solve_dense(int m, int nrhs, double* a, double* b);
+void or_update_bhb_rkim(double* res, double* q, const double* qold, const double* vol, const double* dtvol,
+                        const double* dtvolt, double* qhb0, const double* delpls, const double* prd, double ar,
+                        int imax, int jmax, int kmax, const double* ttrm);
+void or_lim_res_ome_bhb(double* res, const double* q, const double* qold, const double* vol, const double* dtvolt,
+                        const double* delpls, const double* prd, int imax, int jmax, int kmax, const double* ttrm);
+void or_rms_hb_term(double* work1, const double* q, const double* vol, int imax, int jmax, int kmax);
 /* force_b (src/output.f90:4388-4886) on one block: pressure and viscous
  * force / moment coefficients of surfaces s = 1..nsurface (BC patches of
  * type 1500 + s - 1; inv14 != 0 also counts type-14 walls as surface 1,

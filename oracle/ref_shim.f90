@@ -24,7 +24,7 @@ module ref_shim
     integer(c_int32_t) :: efx_setup_f, efx_all_f, eig_limit_f, eig_lim_all_f
     integer(c_int32_t) :: efx_setup_t, efx_all_t, eig_limit_t, eig_lim_all_t
     integer(c_int32_t) :: entfx_f_w(3), entfx_t_w(3)
-    integer(c_int32_t) :: bln_prol, ho_rest, pad0
+    integer(c_int32_t) :: bln_prol, ho_rest, rkimp
     real(c_double) :: gamma, machfs, alpha, beta
     real(c_double) :: rhoinf, pinf, ainf
     real(c_double) :: cfl, cflt, rat, ratt, psirs, rkap
@@ -86,6 +86,11 @@ contains
     reyno = c%reyno; pranl = c%pranl; prant = c%prant; stemp = c%stemp; cdff = c%cdff
     epst = c%epst; cmu = c%cmu; sigk = c%sigk; sigw = c%sigw; bkw = c%bkw; bstrkw = c%bstrkw
     gkw = c%gkw; prdlim = c%prdlim; roughk = c%roughk
+    ! BSL / SST constants as input sets them
+    bstr = 0.09d0; a1 = 0.31d0
+    sigk1_bsl = 0.5d0; sigk1_sst = 0.85d0; sigk2 = 1.0d0
+    sigw1 = 0.5d0; b1 = 3.d0 / 40.d0; g1 = b1/bstr - sigw1*0.41d0**2/dsqrt(bstr)
+    sigw2 = 0.856d0; b2 = 0.0828d0; g2 = b2/bstr - sigw2*0.41d0**2/dsqrt(bstr)
   end subroutine
 
   subroutine ref_set_dhb(nsnap, d) bind(C, name='ref_set_dhb')
@@ -375,6 +380,33 @@ contains
     real(c_double) :: dummy(1)
     call lim_res_ome_b(res, q, vol, dtvolt, dummy, dummy, dummy, delpls, prd, ttrm, dummy, ar, &
                        imax, jmax, kmax, npde)
+  end subroutine
+
+  subroutine ref_srckw_sst_b(q, prd, divvel, res, mut, ttrm, vol, nl, imax, jmax, kmax) &
+      bind(C, name='ref_srckw_sst_b')
+    real(c_double) :: q(*), prd(*), divvel(*), res(*), mut(*), ttrm(*), vol(*)
+    integer(c_int), value :: nl, imax, jmax, kmax
+    call srckw_sst_b(q, prd, divvel, res, mut, ttrm, vol, nl, imax, jmax, kmax, npde, nharms)
+  end subroutine
+
+  subroutine ref_tur_vis_sst_b(q, mut, dqxi, dqeta, dqzeta, xdi, edi, zdi, xdj, edj, zdj, xdk, edk, zdk, &
+                               dist, imax, jmax, kmax) bind(C, name='ref_tur_vis_sst_b')
+    real(c_double) :: q(*), mut(*), dqxi(*), dqeta(*), dqzeta(*), dist(*)
+    real(c_double) :: xdi(*), edi(*), zdi(*), xdj(*), edj(*), zdj(*), xdk(*), edk(*), zdk(*)
+    integer(c_int), value :: imax, jmax, kmax
+    call tur_vis_sst_b(q, mut, dqxi, dqeta, dqzeta, xdi, edi, zdi, xdj, edj, zdj, xdk, edk, zdk, dist, &
+                       npde, imax, jmax, kmax, nharms)
+  end subroutine
+
+  ! explicit branch only: rkimp stays .false. here (its branch calls LAPACK,
+  ! which is not linked into this library)
+  subroutine ref_lim_res_ome_bhb(res, q, qold, vol, dtvolt, delpls, prd, ttrm, imax, jmax, kmax) &
+      bind(C, name='ref_lim_res_ome_bhb')
+    real(c_double) :: res(*), q(*), qold(*), vol(*), dtvolt(*), delpls(*), prd(*), ttrm(*)
+    integer(c_int), value :: imax, jmax, kmax
+    real(c_double) :: dummy(1)
+    call lim_res_ome_bhb(res, q, qold, vol, dtvolt, dummy, dummy, dummy, delpls, prd, ttrm, dummy, &
+                         imax, jmax, kmax, npde, nharms)
   end subroutine
 
   subroutine ref_rest4_b(i1, j1, k1, i2, j2, k2, prd1, prd2, vol1, vol2) bind(C, name='ref_rest4_b')

@@ -14,7 +14,7 @@ import subprocess
 
 import numpy as np
 
-from cosa_amd.config import CosahipConfig, to_struct
+from cosa_amd.config import CosahipConfig, komega, to_struct
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
@@ -50,7 +50,7 @@ class OrLevel(ctypes.Structure):
         (n, DP) for n in ("q", "qold", "qp", "qm", "dq", "dqp", "dqm", "flux", "res", "rhs", "work1", "si", "sj",
                           "sk", "vol", "dtvol", "betaxi", "betaeta", "betazeta", "xdot", "ydot", "zdot", "pr")] + [
         (n, DP) for n in ("mut", "dist", "prd", "divvel", "delpls", "dtvolt", "tbetaxi", "tbetaeta", "tbetazeta")] + [
-        ("xd", DP * 9)]
+        ("xd", DP * 9), ("ttrm", DP)]
 
 
 class OrCaseS(ctypes.Structure):
@@ -144,10 +144,10 @@ class OrCase:
             for n in self.FIELDS:
                 setattr(lev, n, dp(a[n]))
             c = case.config
-            if c["viscous"] or c["kom"]:
+            if c["viscous"] or komega(c):
                 for n in ("mut", "prd", "divvel", "delpls"):
                     a[n] = g[n].copy() if n in g else np.zeros(case.size(nl, "p3s"))
-                if c["kom"] and nl == 1:
+                if komega(c) and nl == 1:
                     a["mut"] = case.turbulent_viscosity(a["q"], 1)
                 for n in ("dtvolt", "tbetaxi", "tbetaeta", "tbetazeta"):
                     a[n] = np.zeros(case.size(nl, "p1"))
@@ -157,6 +157,8 @@ class OrCase:
                     setattr(lev, n, dp(a[n]))
                 for m in range(9):
                     lev.xd[m] = dp(a["xd"][m])
+                a["ttrm"] = np.zeros(case.size(nl, "p3s") * 3)
+                lev.ttrm = dp(a["ttrm"])
             self.arrays.append(a)
 
     def ptr(self):

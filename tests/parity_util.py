@@ -92,7 +92,8 @@ TURB = dict(stemp=0.383, ifixq=1, lim_prodtke=1, prdlim=20.0, turcmp=1)
 def rans_cases():
     """Viscous parity cases: Wilcox k-omega on the distorted two-block case
     (viscous walls 1500), laminar Navier-Stokes on the same grid, k-omega on
-    the NACA O-grid with Menter's wall omega."""
+    the NACA O-grid with Menter's wall omega, Menter BSL on the two-block
+    case and SST on the O-grid."""
     out = []
     out.append(cases.two_block_case(ni=13, nj=9, nk=9, nlevel=1, distort=0.1, name="rans_blocks", wall=1500,
                                     irsop=2, cfl_irs=2.5, cfl=1.5, ratt=1.4, cflt=1.2,
@@ -102,4 +103,10 @@ def rans_cases():
     out.append(cases.naca_case(33, 13, 9, nlevel=1, name="rans_naca_menter", wall=1500, irsop=3, psirs=0.125,
                                cfl_irs=2.0, cfl=1.0, ds_wall=2e-3, wallwilc=0,
                                **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB)))
+    # Menter BSL and SST (blended coefficients, cross diffusion, SST eddy viscosity)
+    out.append(cases.two_block_case(ni=13, nj=9, nk=9, nlevel=1, distort=0.1, name="bsl_blocks", wall=1500,
+                                    irsop=1, cfl_irs=2.5, cfl=1.5,
+                                    **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, model="bsl", **TURB)))
+    out.append(cases.naca_case(33, 13, 9, nlevel=1, name="sst_naca", wall=1500, irsop=2, cfl_irs=2.0, cfl=1.0,
+                               ds_wall=2e-3, **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, model="sst", **TURB)))
     return out

@@ -25,6 +25,7 @@ import pytest
 
 from cosa_amd import _lib
 from cosa_amd import case as cases
+from cosa_amd.config import komega
 from oracle_bind import dp
 from parity_util import interior_mask, make_pair, rans_cases, small_cases
 
@@ -51,7 +52,7 @@ def test_fp32_residual_error(c):
         for name in ("cons2prim", "bc", "cutman_q"):
             getattr(o, "or_" + name)(cp, 1, dp(A["q"]))
             g.call(name, 1, _lib.Q)
-        if c.config["kom"]:
+        if komega(c.config):
             o.or_tur_vis(cp, 1)
             g.call("tur_vis", 1, _lib.Q)
         o.or_residual(cp, 1)

@@ -12,7 +12,7 @@ import pytest
 
 from cosa_amd import _lib
 from cosa_amd import case as cases
-from cosa_amd.config import kw_turbulence
+from cosa_amd.config import komega, kw_turbulence
 from oracle_bind import dp
 from parity_util import TURB, interior_mask, make_pair, parity, rans_cases
 
@@ -39,7 +39,7 @@ def test_rans_stage_routines(pair):
     """One RK stage of the viscous / k-omega path, routine by routine
     (src/cosa.f90:1719-2016)."""
     c, o, oc, g = pair
-    kom = bool(c.config["kom"])
+    kom = komega(c.config)
     A = oc[1]
     cp = oc.ptr()
     o.or_cons2prim(cp, 1, dp(A["q"]))
@@ -106,7 +106,7 @@ def test_rans_smooth_single_level(pair, niter):
     npde = c.npde
     r_o = np.array([o.or_get_resrms(i) for i in range(1, 9)])
     parity(r[:npde + 1], r_o[:npde + 1], rtol=1e-12, what="smooth resrms")
-    if c.config["kom"]:
+    if komega(c.config):
         parity(g.download("mut"), oc[1]["mut"], rtol=1e-11, what="smooth mut")
 
 
@@ -115,6 +115,15 @@ def _mg_case(name):
         return cases.two_block_case(ni=17, nj=9, nk=9, nlevel=2, distort=0.1, name="rans_blocks2", wall=1500,
                                     irsop=1, cfl_irs=2.5, cfl=1.5, ncycle=4, nl_crs=2,
                                     **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB))
+    if name == "sst_blocks2":    # SST mut on the coarse level's first visit
+        return cases.two_block_case(ni=17, nj=9, nk=9, nlevel=2, distort=0.1, name="sst_blocks2", wall=1500,
+                                    irsop=1, cfl_irs=2.5, cfl=1.5, ncycle=4, nl_crs=2,
+                                    **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, model="sst", **TURB))
+    if name == "bsl_hb_rkim2":   # BSL with harmonic balance and the implicit stage
+        return cases.two_block_case(ni=17, nj=9, nk=9, nlevel=2, distort=0.1, name="bsl_hb_rkim2", wall=1500,
+                                    irsop=1, cfl_irs=2.5, cfl=1.5, ncycle=3, nl_crs=2, nharms=1, harbal=1,
+                                    omega=0.8, rkimp=1,
+                                    **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, model="bsl", **TURB))
     if name == "laminar_blocks3":
         return cases.two_block_case(ni=17, nj=9, nk=9, nlevel=3, distort=0.1, name="laminar_blocks3", wall=1500,
                                     irsop=2, cfl_irs=2.5, cfl=1.5, ncycle=4, viscous=1, reyno=500.0, stemp=0.383)
@@ -123,7 +132,7 @@ def _mg_case(name):
                            **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB))
 
 
-@pytest.mark.parametrize("name", ["rans_blocks2", "laminar_blocks3", "rans_naca3"])
+@pytest.mark.parametrize("name", ["rans_blocks2", "laminar_blocks3", "rans_naca3", "sst_blocks2", "bsl_hb_rkim2"])
 def test_rans_multigrid_cycles(name):
     """Whole V-cycles with the k-omega production restricted to the coarse
     levels (rest4) and the restriction-stage limiters."""

@@ -230,7 +230,7 @@ def test_rtst_vflux_resid(libs, idir, kom, turcmp):
         same(to, tr, "rtst")
     fo, fr = mutf.copy(), mutf.copy()
     dist = np.zeros(n0(DIMS))
-    o.or_vflux_b(idir, dp(qf), *[dp(x) for x in d], dp(to), dp(fo), *[dp(x) for x in xd], dp(s), *DIMS)
+    o.or_vflux_b(idir, dp(qf), *[dp(x) for x in d], dp(to), dp(fo), dp(dist), *[dp(x) for x in xd], dp(s), *DIMS)
     r.ref_vflux_b(idir, dp(qf), *[dp(x) for x in d], dp(tr), dp(fr), dp(dist), *[dp(x) for x in xd], dp(s), *DIMS)
     same(fo, fr, "vflux")
     fv = fo.reshape(npde, -1)
@@ -332,7 +332,7 @@ def test_update_kom(libs, nh):
     r.ref_update_ttrm_b(dp(q), dp(z), dp(z), dp(z), *[dp(zz) for _ in range(9)], dp(ttrm), dp(vol), *DIMS)
     a, b = q.copy(), q.copy()
     o.or_update_kb(dp(res), dp(a), dp(qold), dp(vol), dp(dtv), dp(dtvt), dp(delpls), dp(prd), ctypes_double(0.5),
-                   *DIMS)
+                   *DIMS, None)
     r.ref_update(dp(res), dp(b), dp(qold), dp(vol), dp(dtv), dp(dtvt), dp(z), dp(z), dp(delpls), dp(prd), dp(ttrm),
                  ctypes_double(0.5), *DIMS)
     same(a, b, "update kom")
@@ -359,7 +359,7 @@ def test_lim_res_ome(libs):
     zz = np.zeros(n0(DIMS) * 3)
     r.ref_update_ttrm_b(dp(q), dp(z), dp(z), dp(z), *[dp(zz) for _ in range(9)], dp(ttrm), dp(vol), *DIMS)
     a, b = res.copy(), res.copy()
-    o.or_lim_res_ome_b(dp(a), dp(q), dp(vol), dp(dtvt), dp(delpls), dp(prd), ctypes_double(0.25), *DIMS)
+    o.or_lim_res_ome_b(dp(a), dp(q), dp(vol), dp(dtvt), dp(delpls), dp(prd), ctypes_double(0.25), *DIMS, None)
     r.ref_lim_res_ome_b(dp(b), dp(q), dp(vol), dp(dtvt), dp(delpls), dp(prd), dp(ttrm), ctypes_double(0.25), *DIMS)
     same(a, b, "lim_res_ome")
     assert not np.array_equal(a, res)
@@ -391,3 +391,133 @@ def test_rest4(libs):
     o.or_rest4_b(*f, *c, dp(p1), dp(a), dp(v1), dp(v2))
     r.ref_rest4_b(*f, *c, dp(p1), dp(b), dp(v1), dp(v2))
     same(a, b, "rest4")
+
+
+# ---------------------------------------------------------------------------
+# Menter BSL / SST (src/src.f90:306-378, src/update.f90:2375-2524, 3036-3161,
+# src/flux.f90:987-1061, 2949-3131)
+
+def setcfg_sst(libs, model, **kw):
+    o, r = libs
+    base = kw_turbulence(0.4, 2.0e5, tu=1e-3, mutfar=0.1)
+    base.update(kom=0, kom_bsl=int(model == "bsl"), kom_sst=int(model == "sst"))
+    base.update(kw)
+    c = default_config(stemp=0.383, **base)
+    o.set_config(c)
+    r.set_config(c)
+    return c
+
+
+def sst_inputs(seed):
+    rng = np.random.default_rng(seed)
+    q = prim7(rng, DIMS)
+    d = [rng.uniform(-0.3, 0.3, q.size) for _ in range(3)]
+    xd = mder(rng, DIMS)
+    # wall distances from near-wall to far: both F1 / F2 branches are taken
+    dist = 10.0 ** rng.uniform(-5, 0, n1(DIMS))
+    return rng, q, d, xd, dist
+
+
+@pytest.mark.parametrize("idir", [1, 2, 3])
+def test_q_der_c_sst(libs, idir):
+    o, r = libs
+    setcfg_sst(libs, "sst")
+    rng = np.random.default_rng(160 + idir)
+    qf = prim7(rng, DIMS)
+    a, b = np.zeros_like(qf), np.zeros_like(qf)
+    o.or_q_der_c_b(idir, dp(qf), dp(a), *DIMS)
+    r.ref_q_der_c_b(idir, dp(qf), dp(b), *DIMS)
+    same(a, b, "q_der_c (k, omega too)")
+    assert np.count_nonzero(a.reshape(7, -1)[5:]) > 0
+
+
+@pytest.mark.parametrize("model", ["bsl", "sst"])
+def test_update_ttrm_srckw_sst(libs, model):
+    o, r = libs
+    setcfg_sst(libs, model, lim_prodtke=1, lim_prodomega=1, prdlim=0.5)
+    rng, q, d, xd, dist = sst_inputs(170)
+    N3 = n3(*DIMS)
+    to, tr = np.zeros(N3 * 3), np.zeros(N3 * 3)
+    o.or_update_ttrm_b(dp(q), *[dp(x) for x in d], ctypes_ptr_array(xd), dp(to), dp(dist), *DIMS)
+    r.ref_update_ttrm_b(dp(q), *[dp(x) for x in d], *[dp(x) for x in xd], dp(tr), dp(dist), *DIMS)
+    same(to, tr, f"update_ttrm {model}")
+    t = to.reshape(3, -1)
+    inner = np.zeros((DIMS[2] + 3, DIMS[1] + 3, DIMS[0] + 3), bool)
+    inner[2:DIMS[2] + 1, 2:DIMS[1] + 1, 2:DIMS[0] + 1] = True
+    g = t[1][inner.ravel()]
+    # F1 spans the range: blended gamma strictly between the inner and outer values somewhere
+    assert g.max() > g.min()
+    prd = rng.uniform(0, 4.0, N3)
+    div = rng.uniform(-0.5, 0.5, N3)
+    mut = rng.uniform(0, 200, N3)
+    vol = rng.uniform(0.5, 1.5, n1(DIMS))
+    res0 = rng.uniform(-1, 1, q.size)
+    for nl in (1, 2):
+        a, b = res0.copy(), res0.copy()
+        o.or_srckw_sst_b(dp(q), dp(prd), dp(div), dp(a), dp(mut), dp(to), dp(vol), nl, *DIMS)
+        r.ref_srckw_sst_b(dp(q), dp(prd), dp(div), dp(b), dp(mut), dp(tr), dp(vol), nl, *DIMS)
+        same(a, b, f"srckw_sst nl={nl}")
+        assert not np.array_equal(a, res0)
+
+
+def test_tur_vis_sst(libs):
+    o, r = libs
+    setcfg_sst(libs, "sst")
+    rng, q, d, xd, dist = sst_inputs(180)
+    qc = prim_to_cons(q, *DIMS, npde=7)
+    N3 = n3(*DIMS)
+    mo, mr = np.zeros(N3), np.zeros(N3)
+    o.or_tur_vis_sst_b(dp(qc), dp(mo), *[dp(x) for x in d], ctypes_ptr_array(xd), dp(dist), *DIMS)
+    r.ref_tur_vis_sst_b(dp(qc), dp(mr), *[dp(x) for x in d], *[dp(x) for x in xd], dp(dist), *DIMS)
+    same(mo, mr, "tur_vis_sst")
+    assert np.count_nonzero(mo) > 0
+
+
+@pytest.mark.parametrize("idir", [1, 2, 3])
+@pytest.mark.parametrize("model", ["bsl", "sst"])
+def test_vflux_sst(libs, idir, model):
+    o, r = libs
+    setcfg_sst(libs, model)
+    rng = np.random.default_rng(190 + idir)
+    si, sj, sk, vol = box_block(*DIMS, seed=8)
+    s = (si, sj, sk)[idir - 1]
+    qf = prim7(rng, DIMS)
+    d = [rng.uniform(-0.05, 0.05, qf.size) for _ in range(3)]
+    xd = mder(rng, DIMS)[3 * (idir - 1):3 * idir]
+    mutf = np.zeros(n0(DIMS) * 7)
+    mutf[:n0(DIMS)] = rng.uniform(0, 30, n0(DIMS))
+    tau = np.zeros_like(qf)
+    r.ref_rtst_b(idir, dp(qf), *[dp(x) for x in d], *[dp(x) for x in xd], dp(tau), dp(mutf), *DIMS)
+    fdist = 10.0 ** rng.uniform(-5, 0, n0(DIMS))
+    fo, fr = mutf.copy(), mutf.copy()
+    o.or_vflux_b(idir, dp(qf), *[dp(x) for x in d], dp(tau), dp(fo), dp(fdist), *[dp(x) for x in xd], dp(s), *DIMS)
+    r.ref_vflux_b(idir, dp(qf), *[dp(x) for x in d], dp(tau), dp(fr), dp(fdist), *[dp(x) for x in xd], dp(s),
+                  *DIMS)
+    same(fo, fr, f"vflux {model}")
+
+
+def test_facedist_closed_form(libs):
+    """facedist_b: with a wall distance linear in the index the face values
+    are the exact midpoints, the boundary faces take the halo cell."""
+    o, _ = libs
+    setcfg_sst(libs, "sst")
+    im, jm, km = DIMS
+    i = np.arange(im + 1)[None, None, :]
+    j = np.arange(jm + 1)[None, :, None]
+    k = np.arange(km + 1)[:, None, None]
+    dist = np.ascontiguousarray((0.25 * i + 0.5 * j + 1.0 * k + 0.125).astype(np.float64).reshape(-1))
+    for idir, step in ((1, 0.25), (2, 0.5), (3, 1.0)):
+        fd = np.zeros(im * jm * km)
+        o.or_facedist_b(idir, dp(dist), dp(fd), *DIMS)
+        f = fd.reshape(km, jm, im)
+        ii, jj, kk = np.meshgrid(np.arange(1, im + 1), np.arange(1, jm + 1), np.arange(1, km + 1), indexing="ij")
+        ii, jj, kk = ii.transpose(2, 1, 0), jj.transpose(2, 1, 0), kk.transpose(2, 1, 0)
+        a = {1: ii, 2: jj, 3: kk}[idir]
+        L = {1: im, 2: jm, 3: km}[idir]
+        want = 0.25 * ii + 0.5 * jj + 1.0 * kk + 0.125 - step / 2
+        want = np.where(a == 1, want - step / 2, want)          # dist of cell 0
+        want = np.where(a == L, want + step / 2, want)          # dist of cell L
+        m = (ii < im) | (idir == 1)
+        m &= (jj < jm) | (idir == 2)
+        m &= (kk < km) | (idir == 3)
+        np.testing.assert_array_equal(f[m], want[m])

@@ -10,7 +10,7 @@ import pytest
 
 from cosa_amd import case as cases
 from cosa_amd import restart
-from cosa_amd.config import kw_turbulence
+from cosa_amd.config import komega, kw_turbulence
 from oracle_bind import Ref, ref_available
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -27,7 +27,7 @@ def _case(kom, nharms):
 def _arrays(c, seed):
     rng = np.random.default_rng(seed)
     q = rng.uniform(-2, 2, c.size(1, "p3"))
-    mut = rng.uniform(0, 5, c.size(1, "p3s")) if c.config["kom"] else None
+    mut = rng.uniform(0, 5, c.size(1, "p3s")) if komega(c.config) else None
     return q, mut
 
 
