@@ -40,7 +40,9 @@ class CosahipConfig(ctypes.Structure):
                    ("tkefar", ctypes.c_double), ("omefar", ctypes.c_double)]
                 + [(n, ctypes.c_int32) for n in _TURB_INT_FIELDS]
                 + [("lim_corr", ctypes.c_int32 * 3), ("pad1", ctypes.c_int32)]
-                + [(n, ctypes.c_double) for n in _TURB_DBL_FIELDS])
+                + [(n, ctypes.c_double) for n in _TURB_DBL_FIELDS]
+                + [("ramping", ctypes.c_int32), ("ncycr", ctypes.c_int32 * 4), ("pad2", ctypes.c_int32),
+                   ("vcfli", ctypes.c_double * 3), ("vcflit", ctypes.c_double * 3)])
 
 
 def default_config(**kw: Any) -> Dict[str, Any]:
@@ -61,6 +63,8 @@ def default_config(**kw: Any) -> Dict[str, Any]:
         reyno=1.0e6, pranl=0.72, prant=0.9, stemp=110.4 / 288.15, cdff=1.0, epst=1e-12, cmu=1.0, sigk=0.5,
         sigw=0.5, bkw=3.0 / 40.0, bstrkw=0.09, gkw=(3.0 / 40.0) / 0.09 - 0.5 * 0.41 ** 2 / math.sqrt(0.09),
         prdlim=20.0, roughk=0.0,
+        # CFL ramping (k-omega): off
+        ramping=0, ncycr=(0, 0, 0, 0), vcfli=(0.0, 0.0, 0.0), vcflit=(0.0, 0.0, 0.0),
     )
     for k, v in kw.items():
         if k not in c and k not in ("alpha_deg", "beta_deg", "cfl_irs"):
@@ -109,6 +113,12 @@ def to_struct(c: Dict[str, Any]) -> CosahipConfig:
         s.lim_corr[i] = int(c["lim_corr"][i])
     for n in _TURB_DBL_FIELDS:
         setattr(s, n, float(c[n]))
+    s.ramping = int(c["ramping"])
+    for i in range(4):
+        s.ncycr[i] = int(c["ncycr"][i])
+    for i in range(3):
+        s.vcfli[i] = float(c["vcfli"][i])
+        s.vcflit[i] = float(c["vcflit"][i])
     return s
 
 

@@ -206,6 +206,9 @@ static int check_config(Solve* s, const cosahip_config& c) {
     return fail(s, COSAHIP_ERR_UNSUPPORTED, "k-omega prd_all_lvls / lim_corr options are not built");
   if (komega(c) && c.ho_rest) return fail(s, COSAHIP_ERR_UNSUPPORTED, "k-omega with ho_rest (bc_res) is not built");
   if (c.ilimt < 0 || c.ilimt > 4) return fail(s, COSAHIP_ERR_ARG, "ilimt out of range 0..4");
+  if (c.ramping < 0 || c.ramping > 2) return fail(s, COSAHIP_ERR_ARG, "ramping must be 0, 1 or 2");
+  if (c.ramping && c.irsop == 0)
+    return fail(s, COSAHIP_ERR_ARG, "CFL ramping cannot be used without IRS (src/input.f90:4217-4220)");
   if (c.viscous && (c.reyno <= 0 || c.pranl <= 0 || c.prant <= 0))
     return fail(s, COSAHIP_ERR_ARG, "viscous runs need reyno, pranl, prant > 0");
   if (c.nharms < 0 || c.nharms > 65) return fail(s, COSAHIP_ERR_ARG, "nharms out of range 0..65");
@@ -214,7 +217,7 @@ static int check_config(Solve* s, const cosahip_config& c) {
   if (c.moving && c.lmet != 7) return fail(s, COSAHIP_ERR_ARG, "moving grids need lmet = 7");
   if (c.nlevel < 1 || c.nlevel > 4) return fail(s, COSAHIP_ERR_ARG, "nlevel out of range 1..4");
   if (c.irsop < 0 || c.irsop == 4 || c.irsop > 6) return fail(s, COSAHIP_ERR_ARG, "irsop must be 0, 1, 2, 3, 5 or 6");
-  if (c.irsop > 0 && c.cutcirs != 0) return fail(s, COSAHIP_ERR_UNSUPPORTED, "cutcirs = 1 (ccirs) not built");
+  if (c.cutcirs < 0 || c.cutcirs > 1) return fail(s, COSAHIP_ERR_ARG, "cutcirs must be 0 or 1");
   if (c.ilimf < 0 || c.ilimf > 4) return fail(s, COSAHIP_ERR_ARG, "ilimf out of range 0..4");
   if (c.nstage < 1 || c.nstage > 5) return fail(s, COSAHIP_ERR_ARG, "nstage out of range 1..5");
   if (c.iupdt < 1 || c.iprint < 1) return fail(s, COSAHIP_ERR_ARG, "iupdt and iprint must be >= 1");
@@ -943,8 +946,59 @@ extern "C" int cosahip_prtest(cosahip_solve* h, int nl, int iter) {
   if (key != NOFLAG) return decode_prtest(s, nl, key);
   return COSAHIP_OK;
 }
+// CFL ramping of k-omega runs (bdeltat, src/cosa.f90:4760-4820): cfl and
+// cflt of the solve from the cycle counter, kept between calls as the
+// reference keeps its globals; ramping1 refreshes at cycle 1 and every
+// 100th, ramping2 at every call.  Same arithmetic as the reference.
+static void cfl_ramp(Solve* s, int mgit) {
+  const cosahip_config& c = s->cfg;
+  Par& p = s->par;
+  if (!komega(c)) return;
+  const int* nc = c.ncycr;
+  if (c.ramping == 1 && (mgit == 1 || mgit / 100 * 100 == mgit)) {
+    const double sqrt3 = sqrt(3.0);
+    const double s33 = sqrt3 * sqrt3 * sqrt3;
+    const double dcfli1 = (c.vcfli[1] - c.vcfli[0]) * s33, dcfli2 = c.vcfli[2] - c.vcfli[1];
+    const double dcflit1 = (c.vcflit[1] - c.vcflit[0]) * s33, dcflit2 = c.vcflit[2] - c.vcflit[1];
+    if (mgit <= nc[1]) {
+      p.cfl = c.vcfli[0];
+      p.cflt = c.vcflit[0];
+    } else if (mgit <= nc[2]) {
+      const double x = (double)(mgit - nc[1]) / (nc[2] - nc[1]) / sqrt3;
+      p.cfl = c.vcfli[0] + dcfli1 * (x * x * x);
+      p.cflt = c.vcflit[0] + dcflit1 * (x * x * x);
+    } else if (mgit <= nc[3]) {
+      p.cfl = c.vcfli[1] + dcfli2 * ((double)(mgit - nc[2]) / (nc[3] - nc[2]));
+      p.cflt = c.vcflit[1] + dcflit2 * ((double)(mgit - nc[2]) / (nc[3] - nc[2]));
+    } else {
+      p.cfl = c.vcfli[2];
+      p.cflt = c.vcflit[2];
+    }
+  } else if (c.ramping == 2) {
+    const double dcfli1 = c.vcfli[1] - c.vcfli[0], dcfli2 = c.vcfli[2] - c.vcfli[1];
+    const double dcflit1 = c.vcflit[1] - c.vcflit[0], dcflit2 = c.vcflit[2] - c.vcflit[1];
+    if (mgit <= nc[0]) {
+      p.cfl = c.vcfli[0];
+      p.cflt = c.vcflit[0];
+    } else if (mgit <= nc[1]) {
+      p.cfl = c.vcfli[0] + dcfli1 * ((double)(mgit - nc[0]) / (nc[1] - nc[0]));
+      p.cflt = c.vcflit[0] + dcflit1 * ((double)(mgit - nc[0]) / (nc[1] - nc[0]));
+    } else if (mgit <= nc[2]) {
+      p.cfl = c.vcfli[1];
+      p.cflt = c.vcflit[1];
+    } else if (mgit <= nc[3]) {
+      p.cfl = c.vcfli[1] + dcfli2 * ((double)(mgit - nc[2]) / (nc[3] - nc[2]));
+      p.cflt = c.vcflit[1] + dcflit2 * ((double)(mgit - nc[2]) / (nc[3] - nc[2]));
+    } else {
+      p.cfl = c.vcfli[2];
+      p.cflt = c.vcflit[2];
+    }
+  }
+}
+
 extern "C" int cosahip_lambda_deltat(cosahip_solve* h, int nl) {
   LEVEL_OR_FAIL(s, nl, L);
+  cfl_ramp(s, s->mgit);
   CK(launch_lambda_deltat(*s, *L));
   return COSAHIP_OK;
 }
@@ -1070,6 +1124,7 @@ static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter
   if (done) *done = 0;
   double* q = L.f[COSAHIP_Q];
   const int niterp = to_coarse ? niter + 1 : niter;
+  s->mgit = mgit;
   if (!async) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->stream, s->dflag + (nl - 1), NOFLAG, 1);
   int seq = 0;
   bool returned_early = false;
@@ -1102,6 +1157,7 @@ static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter
       }
       if (iter % c.iupdt == 0 && irk == 1) {
         TimeScope t(*s, COSAHIP_FAM_LAMBDA, cs);
+        cfl_ramp(s, mgit);
         CK(launch_lambda_deltat(*s, L));
       }
       {
@@ -1297,7 +1353,8 @@ extern "C" int cosahip_cycle_enqueue(cosahip_solve* h, int mgit) {
   CK(hipSetDevice(s->device));
   // the first cycle differs (k-omega: mut of a coarse level on its first
   // visit) and runs eagerly; later cycles replay one captured graph
-  const bool graph = s->use_graph && mgit >= 2 && !s->timing;
+  // (a CFL ramp changes the kernels' arguments from cycle to cycle: no graph)
+  const bool graph = s->use_graph && mgit >= 2 && !s->timing && !s->cfg.ramping;
   const int key = s->fp32() ? 32 : 64;
   if (graph && s->gexec && s->gkey == key) {
     CK(hipGraphLaunch(s->gexec, s->stream));

@@ -216,6 +216,7 @@ struct Solve {
   // (replayed while its key -- the residual precision -- is unchanged;
   // COSAHIP_GRAPH=0 at create time runs every cycle eagerly)
   bool cycle_pending = false;
+  int mgit = 1;                 // cycle counter of the last smooth (CFL ramping)
   bool use_graph = true;
   hipGraphExec_t gexec = nullptr;
   int gkey = -1;

@@ -1277,8 +1277,141 @@ hipError_t launch_h0_last(Solve& s, Level& L, const ResIn<T>& in) {
 template hipError_t launch_h0_last<double>(Solve&, Level&, const ResIn<double>&);
 template hipError_t launch_h0_last<float>(Solve&, Level&, const ResIn<float>&);
 
+// ccirs_b sweep of direction DIR (src/cosa.f90:2067-2415, cutcirs = 1):
+// thread per (line, snapshot), the variables in turn.  The xi sweep's
+// right-hand side is ores (the scaled residual before smoothing), eta and
+// zeta use the current res; both line ends add eps * res(halo) (the
+// neighbour residual exchanged by cutman_q).  Thomas arithmetic of tridi
+// (src/cosa.f90:2418-2443): multipliers in rsc, forward values in res.
+template <int DIR>
+__global__ void __launch_bounds__(256) k_ccirs(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
+                                               double* __restrict__ res, const double* __restrict__ ores,
+                                               const double* __restrict__ beta, const double* __restrict__ tbeta,
+                                               double* __restrict__ rsc, double epsc) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int item = t.base + threadIdx.x;
+  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
+  const int T1 = DIR == 1 ? jm : im;
+  const int T2 = DIR == 3 ? jm : km;
+  const int nlines = T1 * T2;
+  if (item >= nlines * p.nsnap) return;
+  const int n = item / nlines;
+  const int rem = item - n * nlines;
+  const int t1 = rem % T1 + 1, t2 = rem / T1 + 1;
+  const int L = DIR == 1 ? im : (DIR == 2 ? jm : km);
+  const int hi = DIR == 1 ? b.imax : (DIR == 2 ? b.jmax : b.kmax);
+  const ll n3 = n3of(b);
+  double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  const double* ob = ores + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  double* rs = rsc + b.o3 * p.nsnap + n3 * (ll)n;
+  const ll o1 = b.o1 * p.nsnap + n1of(b) * (ll)n;
+  auto cell = [&](int a, ll& c3, ll& c1) {
+    const int i = DIR == 1 ? a : t1, j = DIR == 2 ? a : (DIR == 1 ? t1 : t2), k = DIR == 3 ? a : t2;
+    c3 = i3(b, i, j, k);
+    c1 = i1(b, i, j, k);
+  };
+  ll lo3, hi3, tmp;
+  {
+    const int i = DIR == 1 ? 0 : t1, j = DIR == 2 ? 0 : (DIR == 1 ? t1 : t2), k = DIR == 3 ? 0 : t2;
+    lo3 = i3(b, i, j, k);
+    const int ih = DIR == 1 ? hi : t1, jh = DIR == 2 ? hi : (DIR == 1 ? t1 : t2), kh = DIR == 3 ? hi : t2;
+    hi3 = i3(b, ih, jh, kh);
+  }
+  (void)tmp;
+  for (int v = 0; v < p.npde; v++) {
+    const bool turb = v >= 5;
+    const double* bb = p.irsop == 1 ? nullptr : (turb ? tbeta : beta) + o1;
+    const ll ov = n3 * v;
+    double tp = 0.0, rp = 0.0;
+    for (int a = 1; a <= L; a++) {
+      ll c3, c1;
+      cell(a, c3, c1);
+      const double eps = bb ? bb[c1] : epsc;
+      const double ca = -eps, cb = 1 + 2 * eps, cc = a == L ? 0.0 : -eps;
+      double rhs = DIR == 1 ? ob[c3 + ov] : rb[c3 + ov];
+      if (a == L) rhs = rhs + eps * rb[hi3 + ov];
+      else if (a == 1) rhs = rhs + eps * rb[lo3 + ov];
+      double r, tc;
+      if (a == 1) {
+        r = cc / cb;
+        tc = rhs / cb;
+      } else {
+        r = cc / (cb - rp * ca);
+        tc = (rhs - tp * ca) / (cb - rp * ca);
+      }
+      rs[c3] = r;
+      rb[c3 + ov] = tc;
+      tp = tc;
+      rp = r;
+    }
+    double xn = 0.0;
+    for (int a = L; a >= 1; a--) {
+      ll c3, c1;
+      cell(a, c3, c1);
+      xn = a == L ? rb[c3 + ov] : rb[c3 + ov] - xn * rs[c3];
+      rb[c3 + ov] = xn;
+    }
+  }
+}
+
+// scalef_b / scaleb_b (src/cosa.f90:5652-5773) on the interior
+__global__ void __launch_bounds__(256) k_scale_dt(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
+                                                  double* __restrict__ res, const double* __restrict__ dtvol,
+                                                  const double* __restrict__ dtvolt, int back) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
+  const int nc = im * jm * km;
+  const int item = t.base + threadIdx.x;
+  if (item >= nc * p.nsnap) return;
+  const int n = item / nc;
+  int rem = item - n * nc;
+  const int i = rem % im + 1;
+  rem /= im;
+  const int j = rem % jm + 1, k = rem / jm + 1;
+  const ll n3 = n3of(b), x = i3(b, i, j, k);
+  const ll po = b.o1 * p.nsnap + n1of(b) * (ll)n + i1(b, i, j, k);
+  double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n + x;
+  for (int v = 0; v < p.npde; v++) {
+    const double d = v < 5 ? dtvol[po] : dtvolt[po];
+    rb[n3 * v] = back ? rb[n3 * v] / d : rb[n3 * v] * d;
+  }
+}
+
+// CIRS with cut boundaries (src/cosa.f90:1984-1992): scalef, ores = res,
+// two passes of cutman_q on res and the three ccirs sweeps, scaleb
+static hipError_t launch_ccirs(Solve& s, Level& L) {
+  double* res = L.f[COSAHIP_RES];
+  const double epsc = 0.25 * (s.par.rat * s.par.rat - 1.0);
+  const ll nq = L.n3tot * s.par.npde * s.par.nsnap;
+  if (L.ntile[TK_CELLS])
+    hipLaunchKernelGGL(k_scale_dt, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_CELLS],
+                       res, L.f[COSAHIP_DTVOL], L.f[COSAHIP_DTVOLT], 0);
+  hipError_t e = hipMemcpyAsync(L.work1, res, nq * sizeof(double), hipMemcpyDeviceToDevice, s.stream);
+  if (e != hipSuccess) return e;
+  for (int it = 0; it < 2; it++) {
+    e = launch_cut(s, L, res);
+    if (e != hipSuccess) return e;
+    if (L.ntile[TK_LINES_I_T])
+      hipLaunchKernelGGL(k_ccirs<1>, grid1(L.ntile[TK_LINES_I_T]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                         L.tile[TK_LINES_I_T], res, L.work1, L.f[COSAHIP_BETAXI], L.tbeta[0], L.cirs_r, epsc);
+    if (L.ntile[TK_LINES_J])
+      hipLaunchKernelGGL(k_ccirs<2>, grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                         L.tile[TK_LINES_J], res, L.work1, L.f[COSAHIP_BETAETA], L.tbeta[1], L.cirs_r, epsc);
+    if (L.ntile[TK_LINES_K])
+      hipLaunchKernelGGL(k_ccirs<3>, grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                         L.tile[TK_LINES_K], res, L.work1, L.f[COSAHIP_BETAZETA], L.tbeta[2], L.cirs_r, epsc);
+  }
+  if (L.ntile[TK_CELLS])
+    hipLaunchKernelGGL(k_scale_dt, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_CELLS],
+                       res, L.f[COSAHIP_DTVOL], L.f[COSAHIP_DTVOLT], 1);
+  return hipGetLastError();
+}
+
 hipError_t launch_cirs(Solve& s, Level& L) {
   if (s.par.irsop < 1 || s.par.irsop == 4 || s.par.irsop > 6) return hipSuccess;
+  if (s.cfg.cutcirs == 1) return launch_ccirs(s, L);
   double* res = L.f[COSAHIP_RES];
   const double* dt = L.f[COSAHIP_DTVOL];
   const double epsc = 0.25 * (s.par.rat * s.par.rat - 1.0);

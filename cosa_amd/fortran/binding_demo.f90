@@ -42,6 +42,7 @@ program binding_demo
   cfg%reyno = 1.d6; cfg%pranl = 0.72d0; cfg%prant = 0.9d0; cfg%stemp = 0.383d0; cfg%cdff = 1.d0
   cfg%epst = 1.d-12; cfg%cmu = 1.d0; cfg%sigk = 0.5d0; cfg%sigw = 0.5d0; cfg%bkw = 0.075d0
   cfg%bstrkw = 0.09d0; cfg%gkw = 0.5527d0; cfg%prdlim = 20.d0; cfg%roughk = 0.d0
+  cfg%ramping = 0; cfg%ncycr = 0; cfg%pad2 = 0; cfg%vcfli = 0.d0; cfg%vcflit = 0.d0
 
   rc = cosahip_create(cfg, 0, s)
   if (rc /= COSAHIP_OK) stop 'cosahip_create failed'

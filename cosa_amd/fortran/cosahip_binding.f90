@@ -48,6 +48,8 @@ module cosahip_binding
     integer(c_int32_t) :: lim_corr(3), pad1
     real(c_double) :: reyno, pranl, prant, stemp, cdff, epst, cmu, sigk, sigw, bkw, bstrkw, gkw
     real(c_double) :: prdlim, roughk
+    integer(c_int32_t) :: ramping, ncycr(4), pad2
+    real(c_double) :: vcfli(3), vcflit(3)
   end type cosahip_config
 
   interface

@@ -23,6 +23,7 @@ from __future__ import annotations
 import argparse
 import json
 import math
+import os
 import time
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
@@ -52,11 +53,13 @@ class Request:
 
     def make_case(self) -> "cases.Case":
         ni, nj = GRIDS[self.grid]
-        kw = dict(machfs=self.mach, alpha_deg=self.alpha_deg, irsop=1, cfl=2.0, cfl_irs=3.5, nl_crs=2,
+        # CIRS with variable coefficients (cirs_v2), the fastest-converging
+        # smoothing of these O-grids
+        kw = dict(machfs=self.mach, alpha_deg=self.alpha_deg, irsop=3, psirs=0.125, cfl=2.0, cfl_irs=5.0, nl_crs=2,
                   ncycle=self.max_cycles, toler=0.0)
         pitch = None
         if self.nharms:
-            kw.update(nharms=self.nharms, harbal=1, omega=0.2)
+            kw.update(nharms=self.nharms, harbal=1, omega=0.2, irsop=1, cfl_irs=3.5)
             pitch = dict(dtheta0=math.radians(1.0), xh=0.25, yh=0.0)
         return cases.naca_case(ni, nj, NK, nlevel=2, name=f"req{self.rid}", pitch=pitch, **kw)
 
@@ -182,19 +185,20 @@ def summarize(results: List[Result], wall: float, cell_updates: float) -> dict:
             "mcell_updates_per_s": round(cell_updates / wall / 1e6, 2)}
 
 
-def request_mix(n: int, seed: int = 0, rate: float = 50.0, precision: int = 0, drop: float = 1e-3,
-                grids=("small", "medium", "large"), hb_fraction: float = 0.25) -> List[Request]:
+def request_mix(n: int, seed: int = 0, rate: float = 4.0, precision: int = 0, drop: float = 1e-2,
+                grids=("small", "medium", "large"), hb_fraction: float = 0.1, max_cycles: int = 600,
+                mach=(0.3, 0.5), alpha_deg=(-2.0, 4.0)) -> List[Request]:
     """n requests with Poisson arrivals at `rate` per second, grid sizes
-    cycling through `grids`, Mach in [0.3, 0.6], incidence in [0, 3] deg,
-    a fraction of them harmonic-balance pitching with N = 1."""
+    cycling through `grids`, Mach and incidence uniform in the given
+    ranges, a fraction of them harmonic-balance pitching with N = 2."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
     for i in range(n):
         t += float(rng.exponential(1.0 / rate))
-        out.append(Request(rid=i, grid=grids[i % len(grids)], mach=float(rng.uniform(0.3, 0.6)),
-                           alpha_deg=float(rng.uniform(0.0, 3.0)), nharms=1 if rng.random() < hb_fraction else 0,
-                           arrival=t, drop=drop, precision=precision))
+        out.append(Request(rid=i, grid=grids[i % len(grids)], mach=float(rng.uniform(*mach)),
+                           alpha_deg=float(rng.uniform(*alpha_deg)), nharms=2 if rng.random() < hb_fraction else 0,
+                           arrival=t, drop=drop, precision=precision, max_cycles=max_cycles))
     return out
 
 
@@ -238,14 +242,20 @@ def sequential(requests: List[Request], device: int = 0) -> List[Result]:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--requests", type=int, default=24)
-    ap.add_argument("--rate", type=float, default=50.0, help="mean arrivals per second")
+    ap.add_argument("--rate", type=float, default=4.0, help="mean arrivals per second")
     ap.add_argument("--precision", choices=["fp64", "fp32", "mixed"], default="mixed")
-    ap.add_argument("--drop", type=float, default=1e-3)
+    ap.add_argument("--drop", type=float, default=1e-2)
+    ap.add_argument("--max-cycles", type=int, default=600)
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES of this process (HIP's default 4 lets only 4 streams run at once)")
     ap.add_argument("--budget-gib", type=float, default=64.0)
     ap.add_argument("--baseline", action="store_true", help="also run the requests sequentially in FP64")
     a = ap.parse_args()
+    # before the HIP runtime starts (first solve): hardware queues for the
+    # concurrent streams of the resident solves
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(max(a.hw_queues, 1), 32)))
     prec = {"fp64": 0, "fp32": 1, "mixed": 2}[a.precision]
-    reqs = request_mix(a.requests, rate=a.rate, precision=prec, drop=a.drop)
+    reqs = request_mix(a.requests, rate=a.rate, precision=prec, drop=a.drop, max_cycles=a.max_cycles)
     srv = Server(budget_bytes=int(a.budget_gib * (1 << 30)))
     t0 = time.perf_counter()
     res = srv.run(reqs)

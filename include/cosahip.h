@@ -101,6 +101,14 @@ typedef struct cosahip_config {
   double epst, cmu, sigk, sigw, bkw, bstrkw, gkw;   /* k-omega constants     */
   double prdlim;         /* production limiter factor                       */
   double roughk;         /* wall roughness (Wilcox wall, 0: smooth)          */
+  /* ---- CFL ramping of k-omega runs (bdeltat, src/cosa.f90:4760-4820;
+   *      input src/input.f90:3304-3345) ---- */
+  int32_t ramping;       /* 0 none, 1 ramping1 (cubic, refreshed at cycle 1 and
+                            every 100th), 2 ramping2 (piecewise linear)    */
+  int32_t ncycr[4];      /* ncycr(0:3): cycles where the ramp segments end   */
+  int32_t pad2;
+  double vcfli[3];       /* CFL at the marks (with IRS), vcfli(1:3)           */
+  double vcflit[3];      /* turbulence CFL at the marks, vcflit(1:3)          */
 } cosahip_config;
 
 /* ---- device fields (per level; reference layout and extents) ----------

@@ -35,7 +35,7 @@ BY_NAME = {
     "bc.f90": ["bbc", "bc_far0", "bc_far1", "bc_insub", "bc_outsub", "bc_walinv_14", "bc_walvis_15",
                "bc_extrapol", "bc_sym"],
     "cosa.f90": ["bresid", "lambdab", "eig_eul", "eig_ns", "eig_eul_p", "eig_ns_p", "bdeltat", "cirs_b",
-                 "tridi", "resg_brms", "resg_brms_hb", "rest1_b", "rest2_ho_b", "rest2_lo_b", "prol_lin_b",
+                 "tridi", "ccirs_b", "scalef_b", "scaleb_b", "resg_brms", "resg_brms_hb", "rest1_b", "rest2_ho_b", "rest2_lo_b", "prol_lin_b",
                  "prol_bln_b", "rest4_b"],
     "output.f90": ["force_b", "write_brest"],
     "input.f90": ["read_brest"],

@@ -649,6 +649,57 @@ void or_lambdab_v(const double* q, const double* si, const double* sj, const dou
   }
 }
 
+/* CFL ramping of k-omega runs (bdeltat, src/cosa.f90:4760-4820): cfl and
+ * cflt (kept, as the reference's globals) from the cycle counter MGIT;
+ * ramping1 refreshes at cycle 1 and every 100th, ramping2 at every call. */
+static int MGIT = 1;
+void or_set_mgit(int m) { MGIT = m; }
+static void cfl_ramp(void) {
+  if (!is_kom()) return;
+  const int mgit = MGIT;
+  const int* nc = P.ncycr; /* ncycr(0:3) */
+  if (P.ramping == 1 && (mgit == 1 || mgit / 100 * 100 == mgit)) {
+    const double sqrt3 = sqrt(3.0);
+    const double s33 = sqrt3 * sqrt3 * sqrt3;
+    const double dcfli1 = (P.vcfli[1] - P.vcfli[0]) * s33, dcfli2 = P.vcfli[2] - P.vcfli[1];
+    const double dcflit1 = (P.vcflit[1] - P.vcflit[0]) * s33, dcflit2 = P.vcflit[2] - P.vcflit[1];
+    if (mgit <= nc[1]) {
+      P.cfl = P.vcfli[0];
+      P.cflt = P.vcflit[0];
+    } else if (mgit > nc[1] && mgit <= nc[2]) {
+      const double x = (double)(mgit - nc[1]) / (nc[2] - nc[1]) / sqrt3;
+      P.cfl = P.vcfli[0] + dcfli1 * (x * x * x);
+      P.cflt = P.vcflit[0] + dcflit1 * (x * x * x);
+    } else if (mgit > nc[2] && mgit <= nc[3]) {
+      P.cfl = P.vcfli[1] + dcfli2 * ((double)(mgit - nc[2]) / (nc[3] - nc[2]));
+      P.cflt = P.vcflit[1] + dcflit2 * ((double)(mgit - nc[2]) / (nc[3] - nc[2]));
+    } else if (mgit > nc[3]) {
+      P.cfl = P.vcfli[2];
+      P.cflt = P.vcflit[2];
+    }
+  } else if (P.ramping == 2) {
+    const double dcfli1 = P.vcfli[1] - P.vcfli[0], dcfli2 = P.vcfli[2] - P.vcfli[1];
+    const double dcflit1 = P.vcflit[1] - P.vcflit[0], dcflit2 = P.vcflit[2] - P.vcflit[1];
+    if (mgit <= nc[0]) {
+      P.cfl = P.vcfli[0];
+      P.cflt = P.vcflit[0];
+    } else if (mgit > nc[0] && mgit <= nc[1]) {
+      P.cfl = P.vcfli[0] + dcfli1 * ((double)(mgit - nc[0]) / (nc[1] - nc[0]));
+      P.cflt = P.vcflit[0] + dcflit1 * ((double)(mgit - nc[0]) / (nc[1] - nc[0]));
+    } else if (mgit > nc[1] && mgit <= nc[2]) {
+      P.cfl = P.vcfli[1];
+      P.cflt = P.vcflit[1];
+    } else if (mgit > nc[2] && mgit <= nc[3]) {
+      P.cfl = P.vcfli[1] + dcfli2 * ((double)(mgit - nc[2]) / (nc[3] - nc[2]));
+      P.cflt = P.vcflit[1] + dcflit2 * ((double)(mgit - nc[2]) / (nc[3] - nc[2]));
+    } else if (mgit > nc[3]) {
+      P.cfl = P.vcfli[2];
+      P.cflt = P.vcflit[2];
+    }
+  }
+}
+double or_get_cfl(int turb) { return turb ? P.cflt : P.cfl; }
+
 /* bdeltat (src/cosa.f90:4733-5203) without CFL ramping: lamv (viscous
  * radii, NULL when inviscid) join the sum of dtvol; k-omega adds
  * dtvolt = cflt/cfl * dtvol and the turbulence CIRS coefficients tbeta*
@@ -656,6 +707,7 @@ void or_lambdab_v(const double* q, const double* si, const double* sj, const dou
 void or_bdeltat_v(double* dtvol, double* dtvolt, const double* lambdas, const double* lamv, double* betaxi,
                   double* betaeta, double* betazeta, double* tbetaxi, double* tbetaeta, double* tbetazeta, int imax,
                   int jmax, int kmax) {
+  cfl_ramp();
   bg g = mkbg(imax, jmax, kmax);
   const double zeta = 0.4;
   int kom = is_kom();
@@ -1186,6 +1238,86 @@ void or_tridi(const double* a, const double* b, const double* c, double* x, int 
   for (int j = dim - 1; j >= 1; j--) x[j - 1] = t[j] - x[j] * r[j];
   free(r);
   free(t);
+}
+
+/* ccirs_b (src/cosa.f90:2067-2415): CIRS with cut boundaries (cutcirs = 1).
+ * The xi sweep takes the scaled residual before smoothing, ores, as its
+ * right-hand side, eta and zeta the current res; every line adds
+ * eps * res(halo) at both ends (the cut-exchanged neighbour residual, zero
+ * elsewhere).  irsop 1: one constant eps for all variables; irsop >= 2:
+ * beta (mean flow) and tbeta (k-omega) per cell.  res is already scaled by
+ * dtvol (scalef).                                                          */
+void or_ccirs_b(double* res, const double* ores, const double* betaxi, const double* betaeta, const double* betazeta,
+                const double* tbetaxi, const double* tbetaeta, const double* tbetazeta, int imax, int jmax,
+                int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  const int im = imax - 1, jm = jmax - 1, km = kmax - 1;
+  int nmax = imax > jmax ? imax : jmax;
+  if (kmax > nmax) nmax = kmax;
+  double *wa = (double*)malloc(sizeof(double) * nmax), *wb = (double*)malloc(sizeof(double) * nmax),
+         *wc = (double*)malloc(sizeof(double) * nmax), *wx = (double*)malloc(sizeof(double) * nmax);
+  const int kom = is_kom();
+  const double* bset[2][4] = {{NULL, betaxi, betaeta, betazeta}, {NULL, tbetaxi, tbetaeta, tbetazeta}};
+  const int nsets = P.irsop == 1 ? 1 : (kom ? 2 : 1);
+  for (int set = 0; set < nsets; set++)
+    for (int idir = 1; idir <= 3; idir++) {
+      const int v0 = set == 0 ? 1 : 6;
+      const int v1 = set == 1 ? NPDE : (P.irsop == 1 ? NPDE : 5);
+      const int L = idir == 1 ? im : (idir == 2 ? jm : km);
+      const int T1 = idir == 1 ? jm : im;
+      const int T2 = idir == 3 ? jm : km;
+      const int hi = idir == 1 ? imax : (idir == 2 ? jmax : kmax);
+      for (int n = 0; n <= 2 * NH; n++)
+        for (int ipde = v0; ipde <= v1; ipde++)
+          for (int t2 = 1; t2 <= T2; t2++)
+            for (int t1 = 1; t1 <= T1; t1++) {
+              for (int a = 1; a <= L; a++) {
+                const int i = idir == 1 ? a : t1, j = idir == 2 ? a : (idir == 1 ? t1 : t2), k = idir == 3 ? a : t2;
+                const double eps =
+                    P.irsop == 1 ? 0.25 * (P.rat * P.rat - 1.0) : P1(bset[set][idir], g, i, j, k, n);
+                wa[a - 1] = a == 1 ? 0.0 : -eps;
+                wb[a - 1] = 1 + 2 * eps;
+                wc[a - 1] = a == L ? 0.0 : -eps;
+                const double rhs = idir == 1 ? Q(ores, g, i, j, k, ipde, n) : Q(res, g, i, j, k, ipde, n);
+                /* the last cell's assignment comes after the first's in the
+                 * reference, so a one-cell line takes the upper halo term */
+                if (a == L) {
+                  const int i0 = idir == 1 ? hi : i, j0 = idir == 2 ? hi : j, k0 = idir == 3 ? hi : k;
+                  wx[a - 1] = rhs + eps * Q(res, g, i0, j0, k0, ipde, n);
+                } else if (a == 1) {
+                  const int i0 = idir == 1 ? 0 : i, j0 = idir == 2 ? 0 : j, k0 = idir == 3 ? 0 : k;
+                  wx[a - 1] = rhs + eps * Q(res, g, i0, j0, k0, ipde, n);
+                } else {
+                  wx[a - 1] = rhs;
+                }
+              }
+              or_tridi(wa, wb, wc, wx, L);
+              for (int a = 1; a <= L; a++) {
+                const int i = idir == 1 ? a : t1, j = idir == 2 ? a : (idir == 1 ? t1 : t2), k = idir == 3 ? a : t2;
+                Q(res, g, i, j, k, ipde, n) = wx[a - 1];
+              }
+            }
+    }
+  free(wa);
+  free(wb);
+  free(wc);
+  free(wx);
+}
+
+/* scalef_b / scaleb_b (src/cosa.f90:5652-5773): res times / over dtvol
+ * (dtvolt for k and omega) on the interior.                              */
+void or_scale_b(double* res, const double* dtvol, const double* dtvolt, int back, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  for (int n = 0; n <= 2 * NH; n++)
+    for (int ipde = 1; ipde <= NPDE; ipde++) {
+      if (ipde > 5 && !is_kom()) break;
+      const double* d = ipde > 5 ? dtvolt : dtvol;
+      for (int k = 1; k <= kmax - 1; k++)
+        for (int j = 1; j <= jmax - 1; j++)
+          for (int i = 1; i <= imax - 1; i++)
+            Q(res, g, i, j, k, ipde, n) =
+                back ? Q(res, g, i, j, k, ipde, n) / P1(d, g, i, j, k, n) : Q(res, g, i, j, k, ipde, n) * P1(d, g, i, j, k, n);
+    }
 }
 
 void or_cirs_b(double* res, const double* dtvol, const double* betaxi, const double* betaeta, const double* betazeta,
@@ -3004,6 +3136,28 @@ void or_copy_array(or_case* c, int nl, double* dst, const double* src) {
 void or_cirs(or_case* c, int nl) {
   or_level* L = &c->lev[nl - 1];
   int kom = is_kom();
+  if (P.cutcirs == 1) {
+    /* src/cosa.f90:1984-1992: scalef, ores = res, two passes of cutman_q on
+     * res and ccirs, scaleb */
+    for (int b = 0; b < L->nblocks; b++)
+      or_scale_b(QB(L->res, L, b), PB(L->dtvol, L, b), kom ? PB(L->dtvolt, L, b) : NULL, 0, L->imax[b], L->jmax[b],
+                 L->kmax[b]);
+    for (int b = 0; b < L->nblocks; b++) {
+      bg g = mkbg(L->imax[b], L->jmax[b], L->kmax[b]);
+      memcpy(QB(L->work1, L, b), QB(L->res, L, b), sizeof(double) * (size_t)g.n3 * NPDE * DIM5);
+    }
+    for (int it = 0; it < 2; it++) {
+      or_cutman_q(c, nl, L->res);
+      for (int b = 0; b < L->nblocks; b++)
+        or_ccirs_b(QB(L->res, L, b), QB(L->work1, L, b), PB(L->betaxi, L, b), PB(L->betaeta, L, b),
+                   PB(L->betazeta, L, b), kom ? PB(L->tbetaxi, L, b) : NULL, kom ? PB(L->tbetaeta, L, b) : NULL,
+                   kom ? PB(L->tbetazeta, L, b) : NULL, L->imax[b], L->jmax[b], L->kmax[b]);
+    }
+    for (int b = 0; b < L->nblocks; b++)
+      or_scale_b(QB(L->res, L, b), PB(L->dtvol, L, b), kom ? PB(L->dtvolt, L, b) : NULL, 1, L->imax[b], L->jmax[b],
+                 L->kmax[b]);
+    return;
+  }
   for (int b = 0; b < L->nblocks; b++)
     or_cirs_kb(QB(L->res, L, b), PB(L->dtvol, L, b), PB(L->betaxi, L, b), PB(L->betaeta, L, b),
                PB(L->betazeta, L, b), kom ? PB(L->dtvolt, L, b) : NULL, kom ? PB(L->tbetaxi, L, b) : NULL,
@@ -3111,6 +3265,7 @@ void or_turb_limits(or_case* c, int nl, double ar) {
 /* smooth (src/cosa.f90:1594-2032), Euler / HB explicit path               */
 int or_smooth(or_case* c, int nl, int from_fine, int to_coarse, int niter, int mgit, int32_t* done, int32_t* where) {
   or_level* L = &c->lev[nl - 1];
+  MGIT = mgit;
   *done = 0;
   int niterp = to_coarse ? niter + 1 : niter;
   for (int iter = 1; iter <= niterp; iter++) {

@@ -52,6 +52,8 @@ void or_set_config(const cosahip_config* cfg);
 void or_set_dhb(int nsnap, const double* dhb);
 double or_get_resrms(int i);            /* resrms(i), i = 1..8            */
 void or_set_resrms(int i, double v);
+void or_set_mgit(int m);                /* cycle counter of CFL ramping   */
+double or_get_cfl(int turb);            /* cfl (0) / cflt (1) in use      */
 
 /* per-block restatements (explicit-shape dummies of the reference) */
 void or_cons2prim_b(double* q, int imax, int jmax, int kmax, int xtnd);
@@ -140,6 +142,10 @@ void or_update_kb(double* res, double* q, const double* qold, const double* vol,
                   int kmax, const double* ttrm);
 void or_lim_res_ome_b(double* res, const double* q, const double* vol, const double* dtvolt, const double* delpls,
                       const double* prd, double ar, int imax, int jmax, int kmax, const double* ttrm);
+void or_ccirs_b(double* res, const double* ores, const double* betaxi, const double* betaeta, const double* betazeta,
+                const double* tbetaxi, const double* tbetaeta, const double* tbetazeta, int imax, int jmax,
+                int kmax);
+void or_scale_b(double* res, const double* dtvol, const double* dtvolt, int back, int imax, int jmax, int kmax);
 int or_lu_factor(double* a, int m, int* ipiv);
 void or_lu_solve(const double* a, int m, const int* ipiv, double* b, int nrhs);
 int or_lu_solve_dense(int m, int nrhs, double* a, double* b);

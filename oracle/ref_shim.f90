@@ -38,6 +38,8 @@ module ref_shim
     integer(c_int32_t) :: lim_corr(3), pad1
     real(c_double) :: reyno, pranl, prant, stemp, cdff, epst, cmu, sigk, sigw, bkw, bstrkw, gkw
     real(c_double) :: prdlim, roughk
+    integer(c_int32_t) :: ramping, ncycr(4), pad2
+    real(c_double) :: vcfli(3), vcflit(3)
   end type cfg_t
 
 contains
@@ -78,7 +80,7 @@ contains
     aircraft = .true.; vawt = .false.; hawt = .false.; rotating = .false.
     tpitch = .false.; lomach = .false.; rkimp = .false.; rgkuns = .false.
     dualt = .false.; relframe = .false.; unsteady = harbal
-    ramping1 = .false.; ramping2 = .false.; mgit = 1
+    mgit = 1
     lim_corr = c%lim_corr /= 0
     turcmp = c%turcmp; lim_prodtke = c%lim_prodtke /= 0; lim_prodomega = c%lim_prodomega /= 0
     wallwilc = c%wallwilc /= 0; wallment = .not. wallwilc; ifixq = c%ifixq
@@ -86,11 +88,19 @@ contains
     reyno = c%reyno; pranl = c%pranl; prant = c%prant; stemp = c%stemp; cdff = c%cdff
     epst = c%epst; cmu = c%cmu; sigk = c%sigk; sigw = c%sigw; bkw = c%bkw; bstrkw = c%bstrkw
     gkw = c%gkw; prdlim = c%prdlim; roughk = c%roughk
+    ramping1 = c%ramping == 1; ramping2 = c%ramping == 2
+    ncycr(0:3) = c%ncycr; vcfli(1:3) = c%vcfli; vcflit(1:3) = c%vcflit
     ! BSL / SST constants as input sets them
     bstr = 0.09d0; a1 = 0.31d0
     sigk1_bsl = 0.5d0; sigk1_sst = 0.85d0; sigk2 = 1.0d0
     sigw1 = 0.5d0; b1 = 3.d0 / 40.d0; g1 = b1/bstr - sigw1*0.41d0**2/dsqrt(bstr)
     sigw2 = 0.856d0; b2 = 0.0828d0; g2 = b2/bstr - sigw2*0.41d0**2/dsqrt(bstr)
+  end subroutine
+
+  ! the multigrid cycle counter of common_variables (CFL ramping reads it)
+  subroutine ref_set_mgit(m) bind(C, name='ref_set_mgit')
+    integer(c_int), value :: m
+    mgit = m
   end subroutine
 
   subroutine ref_set_dhb(nsnap, d) bind(C, name='ref_set_dhb')
@@ -194,6 +204,27 @@ contains
     integer(c_int), value :: imax, jmax, kmax
     call cirs_b(res, dtvol, bx, be, bz, dtvolt, tbx, tbe, tbz, work, imax, jmax, kmax, npde, nharms, &
                 max(imax, jmax, kmax))
+  end subroutine
+
+  subroutine ref_ccirs_b(res, ores, bx, be, bz, tbx, tbe, tbz, imax, jmax, kmax) bind(C, name='ref_ccirs_b')
+    real(c_double) :: res(*), ores(*), bx(*), be(*), bz(*), tbx(*), tbe(*), tbz(*)
+    integer(c_int), value :: imax, jmax, kmax
+    real(c_double), allocatable :: work(:)
+    integer :: ijkmax
+    ijkmax = max(imax, jmax, kmax)
+    allocate(work((ijkmax - 1) * max(npde, 4)))
+    call ccirs_b(res, ores, bx, be, bz, tbx, tbe, tbz, work, imax, jmax, kmax, npde, nharms, ijkmax)
+    deallocate(work)
+  end subroutine
+
+  subroutine ref_scale_b(res, dtvol, dtvolt, back, imax, jmax, kmax) bind(C, name='ref_scale_b')
+    real(c_double) :: res(*), dtvol(*), dtvolt(*)
+    integer(c_int), value :: back, imax, jmax, kmax
+    if (back == 0) then
+      call scalef_b(res, dtvol, dtvolt, imax, jmax, kmax, npde, nharms)
+    else
+      call scaleb_b(res, dtvol, dtvolt, imax, jmax, kmax, npde, nharms)
+    end if
   end subroutine
 
   subroutine ref_tridi(a, b, c, x, dimx, n) bind(C, name='ref_tridi')

@@ -38,8 +38,9 @@ def test_library_exports_every_symbol():
 
 
 def test_config_struct_layout():
-    # 38 int32 members then 37 doubles (include/cosahip.h)
-    assert ctypes.sizeof(CosahipConfig) == 38 * 4 + 37 * 8 + 10 * 4 + 14 * 8
+    # 38 int32 members then 37 doubles, the viscous block (10 int32, 14
+    # doubles), the CFL-ramp block (6 int32, 6 doubles) (include/cosahip.h)
+    assert ctypes.sizeof(CosahipConfig) == 38 * 4 + 37 * 8 + 10 * 4 + 14 * 8 + 6 * 4 + 6 * 8
     assert CosahipConfig.gamma.offset == 152
 
 

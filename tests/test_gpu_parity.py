@@ -113,11 +113,18 @@ def _mg_case(name):
     if name == "naca3_pitch_hb2":
         return cases.naca_case(65, 17, 9, nlevel=3, name="naca3_pitch", irsop=2, cfl_irs=3.0, cfl=2.0, nl_crs=2,
                                ncycle=5, nharms=2, harbal=1, omega=0.6, pitch=dict(dtheta0=0.05, xh=0.25, yh=0.0))
+    if name == "blocks2_ccirs":      # CIRS with cut boundaries (cutcirs = 1), irsop 2
+        return cases.two_block_case(ni=17, nj=9, nk=9, nlevel=2, distort=0.1, name="blocks2_ccirs", irsop=2,
+                                    cfl_irs=3.0, cfl=2.0, ncycle=5, cutcirs=1)
+    if name == "naca2_ccirs":        # ccirs with irsop 1 and the O-grid's seam cut
+        return cases.naca_case(65, 17, 9, nlevel=2, name="naca2_ccirs", irsop=1, cfl_irs=3.5, cfl=2.0, nl_crs=2,
+                               ncycle=5, cutcirs=1)
     return cases.two_block_case(ni=17, nj=9, nk=9, nlevel=2, distort=0.1, name="blocks2", irsop=3, psirs=0.125,
                                 cfl_irs=3.0, cfl=2.0, ncycle=5, nharms=1, harbal=1, omega=0.8)
 
 
-@pytest.mark.parametrize("name", ["naca3", "blocks3_lin_ho", "blocks2_hb1", "naca3_pitch_hb2"])
+@pytest.mark.parametrize("name", ["naca3", "blocks3_lin_ho", "blocks2_hb1", "naca3_pitch_hb2", "blocks2_ccirs",
+                                  "naca2_ccirs"])
 def test_multigrid_cycles(name):
     """Whole V-cycles (mg, src/cosa.f90:806-930): resrms history per cycle
     and the final state."""

@@ -124,6 +124,15 @@ def _mg_case(name):
                                     irsop=1, cfl_irs=2.5, cfl=1.5, ncycle=3, nl_crs=2, nharms=1, harbal=1,
                                     omega=0.8, rkimp=1,
                                     **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, model="bsl", **TURB))
+    if name == "rans_ramp2":     # CFL ramping (ramping2) across the cycles
+        return cases.two_block_case(ni=17, nj=9, nk=9, nlevel=2, distort=0.1, name="rans_ramp2", wall=1500,
+                                    irsop=1, cfl_irs=2.5, cfl=1.5, ncycle=6, nl_crs=2, ramping=2,
+                                    ncycr=(1, 2, 4, 5), vcfli=(0.8, 1.6, 2.5), vcflit=(0.5, 1.0, 1.5),
+                                    **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB))
+    if name == "rans_ccirs2":    # k-omega with CIRS on cut boundaries, irsop 2 (tbeta for k, omega)
+        return cases.two_block_case(ni=17, nj=9, nk=9, nlevel=2, distort=0.1, name="rans_ccirs2", wall=1500,
+                                    irsop=2, cfl_irs=2.5, cfl=1.5, ncycle=4, nl_crs=2, cutcirs=1,
+                                    **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB))
     if name == "laminar_blocks3":
         return cases.two_block_case(ni=17, nj=9, nk=9, nlevel=3, distort=0.1, name="laminar_blocks3", wall=1500,
                                     irsop=2, cfl_irs=2.5, cfl=1.5, ncycle=4, viscous=1, reyno=500.0, stemp=0.383)
@@ -132,7 +141,8 @@ def _mg_case(name):
                            **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB))
 
 
-@pytest.mark.parametrize("name", ["rans_blocks2", "laminar_blocks3", "rans_naca3", "sst_blocks2", "bsl_hb_rkim2"])
+@pytest.mark.parametrize("name", ["rans_blocks2", "laminar_blocks3", "rans_naca3", "sst_blocks2", "bsl_hb_rkim2",
+                                  "rans_ramp2", "rans_ccirs2"])
 def test_rans_multigrid_cycles(name):
     """Whole V-cycles with the k-omega production restricted to the coarse
     levels (rest4) and the restriction-stage limiters."""

@@ -521,3 +521,73 @@ def test_facedist_closed_form(libs):
         m &= (jj < jm) | (idir == 2)
         m &= (kk < km) | (idir == 3)
         np.testing.assert_array_equal(f[m], want[m])
+
+
+@pytest.mark.parametrize("ramping", [1, 2])
+def test_cfl_ramping(libs, ramping):
+    """bdeltat with CFL ramping (src/cosa.f90:4760-4820) over a cycle
+    schedule crossing every ramp segment: dtvol / dtvolt of the oracle and
+    the compiled reference agree cycle by cycle, and the ramp reaches the
+    end values."""
+    o, r = libs
+    ncycr = (3, 8, 15, 24)
+    vcfli, vcflit = (0.5, 2.0, 4.0), (0.3, 1.0, 2.5)
+    setcfg(libs, kom=True, irsop=1, cfl=7.0, cflt=7.0, rat=1.7, ratt=1.3, cdff=1.0, ramping=ramping, ncycr=ncycr,
+           vcfli=vcfli, vcflit=vcflit)
+    rng = np.random.default_rng(44)
+    si, sj, sk, vol = box_block(*DIMS, seed=6)
+    q = prim7(rng, DIMS)
+    N3 = n3(*DIMS)
+    mut = rng.uniform(0.0, 50.0, N3)
+    xd = np.zeros((DIMS[0] + 2) * (DIMS[1] + 2) * (DIMS[2] + 2))
+    lam, lv = np.zeros(N3 * 7), np.zeros(N3 * 7)
+    o.or_lambdab_v(dp(q), dp(si), dp(sj), dp(sk), dp(xd), dp(xd), dp(xd), dp(vol), dp(mut), dp(lam), dp(lv), *DIMS)
+    o.or_get_cfl.restype = __import__("ctypes").c_double
+    seen = []
+    for mgit in (1, 2, 3, 4, 6, 8, 9, 12, 15, 16, 20, 24, 25, 99, 100, 101):
+        o.or_set_mgit(mgit)
+        r.ref_set_mgit(mgit)
+        oo = [np.zeros(n1(DIMS)) for _ in range(8)]
+        rr = [np.zeros(n1(DIMS)) for _ in range(8)]
+        o.or_bdeltat_v(*[dp(x) for x in oo[:2]], dp(lam), dp(lv), *[dp(x) for x in oo[2:]], *DIMS)
+        r.ref_bdeltat(dp(rr[0]), dp(rr[1]), dp(lam), dp(lv), *[dp(x) for x in rr[2:]], *DIMS)
+        same(oo[0], rr[0], f"dtvol cycle {mgit}")
+        same(oo[1], rr[1], f"dtvolt cycle {mgit}")
+        seen.append((mgit, o.or_get_cfl(0), o.or_get_cfl(1)))
+    cfl = dict((m, c) for m, c, _ in seen)
+    assert cfl[1] == vcfli[0] and cfl[25] == vcfli[2] or ramping == 1
+    if ramping == 2:
+        assert cfl[3] == vcfli[0] and cfl[9] == vcfli[1] and cfl[25] == vcfli[2]
+        assert vcfli[0] < cfl[6] < vcfli[1] and vcfli[1] < cfl[20] < vcfli[2]
+    else:   # ramping1 refreshes at cycle 1 and every 100th only
+        assert cfl[1] == vcfli[0] and cfl[99] == vcfli[0] and cfl[100] == vcfli[2] and cfl[101] == vcfli[2]
+
+
+@pytest.mark.parametrize("kom", [False, True])
+@pytest.mark.parametrize("irsop", [1, 2, 3])
+def test_ccirs(libs, kom, irsop):
+    """ccirs_b (cutcirs = 1, src/cosa.f90:2067-2415) and scalef/scaleb: the
+    xi sweep from ores, every line with the halo residual as Dirichlet data
+    at both ends."""
+    o, r = libs
+    if kom:
+        setcfg(libs, irsop=irsop, cutcirs=1, rat=1.8, ratt=1.4)
+    else:
+        setcfg(libs, kom=False, irsop=irsop, cutcirs=1, rat=1.8)
+    npde = 7 if kom else 5
+    rng = np.random.default_rng(300 + irsop + 10 * kom)
+    N3 = n3(*DIMS)
+    res = rng.uniform(-1, 1, N3 * npde)          # halos hold neighbour residuals
+    ores = rng.uniform(-1, 1, N3 * npde)
+    b = [rng.uniform(0.0, 2.0, n1(DIMS)) for _ in range(6)]
+    dtv, dtvt = rng.uniform(0.1, 1.0, n1(DIMS)), rng.uniform(0.1, 1.0, n1(DIMS))
+    a1, a2 = res.copy(), res.copy()
+    o.or_ccirs_b(dp(a1), dp(ores), *[dp(x) for x in b], *DIMS)
+    r.ref_ccirs_b(dp(a2), dp(ores), *[dp(x) for x in b], *DIMS)
+    same(a1, a2, "ccirs")
+    assert not np.array_equal(a1, res)
+    for back in (0, 1):
+        s1, s2 = res.copy(), res.copy()
+        o.or_scale_b(dp(s1), dp(dtv), dp(dtvt), back, *DIMS)
+        r.ref_scale_b(dp(s2), dp(dtv), dp(dtvt), back, *DIMS)
+        same(s1, s2, f"scale back={back}")
