@@ -91,6 +91,7 @@ def load() -> ctypes.CDLL:
         "cosahip_cycle_enqueue": ([_VP, ctypes.c_int], ctypes.c_int),
         "cosahip_cycle_wait": ([_VP, _IP, _DP], ctypes.c_int),
         "cosahip_cycle_ready": ([_VP], ctypes.c_int),
+        "cosahip_autotune": ([_VP, ctypes.c_int, _IP], ctypes.c_int),
         "cosahip_precision_in_use": ([_VP], ctypes.c_int),
         "cosahip_build_info": ([], ctypes.c_char_p),
         "cosahip_timing_enable": ([_VP, ctypes.c_int], ctypes.c_int),
@@ -115,7 +116,7 @@ def exported_symbols():
             "cosahip_residual", "cosahip_src_hb", "cosahip_resg_rms", "cosahip_add_array", "cosahip_copy_array",
             "cosahip_cirs", "cosahip_update", "cosahip_tur_vis", "cosahip_fixq", "cosahip_turb_limits",
             "cosahip_smooth", "cosahip_restrict", "cosahip_prolong", "cosahip_cycle_enqueue", "cosahip_cycle_wait",
-            "cosahip_cycle_ready",
+            "cosahip_cycle_ready", "cosahip_autotune",
             "cosahip_force", "cosahip_set_precision",
             "cosahip_precision_in_use",
             "cosahip_build_info", "cosahip_timing_enable", "cosahip_timing_read", "cosahip_timing_reset"]

@@ -1385,6 +1385,68 @@ extern "C" int cosahip_cycle_enqueue(cosahip_solve* h, int mgit) {
   return COSAHIP_OK;
 }
 
+extern "C" int cosahip_autotune(cosahip_solve* h, int reps, int32_t* choice) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "autotune: an enqueued cycle has not been waited for");
+  if (reps < 1 || reps > 100) return fail(s, COSAHIP_ERR_ARG, "autotune: reps out of range 1..100");
+  CK(hipSetDevice(s->device));
+  for (int m = 0; m < 4; m++)
+    if (choice) choice[m] = 0;
+  hipEvent_t e0, e1;
+  CK(hipEventCreate(&e0));
+  CK(hipEventCreate(&e1));
+  auto timed = [&](auto&& fn, float& ms) -> int {
+    if (fn() != hipSuccess) return fail(s, COSAHIP_ERR_DEVICE, "autotune: launch failed");   // warm-up
+    CK(hipEventRecord(e0, s->stream));
+    for (int r = 0; r < reps; r++)
+      if (fn() != hipSuccess) return fail(s, COSAHIP_ERR_DEVICE, "autotune: launch failed");
+    CK(hipEventRecord(e1, s->stream));
+    CK(hipEventSynchronize(e1));
+    CK(hipEventElapsedTime(&ms, e0, e1));
+    return COSAHIP_OK;
+  };
+  int rc = COSAHIP_OK;
+  for (int nl = 1; nl <= (int)s->lev.size() && nl <= 3 && rc == COSAHIP_OK; nl++) {
+    Level& L = s->lev[nl - 1];
+    if (!L.nblocks || s->par.irsop < 1 || s->par.irsop == 4 || s->par.irsop > 6 || s->cfg.cutcirs) continue;
+    // the sweeps run on a copy of the residual (work1), the state is untouched
+    const ll nq = L.n3tot * s->par.npde * s->par.nsnap;
+    CK(hipMemcpyAsync(L.work1, L.f[COSAHIP_RES], nq * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    std::swap(L.f[COSAHIP_RES], L.work1);
+    float t[3] = {0, 0, 0};
+    for (int mode = 1; mode <= 2 && rc == COSAHIP_OK; mode++) {
+      L.cirs_mode = mode;
+      rc = timed([&]() { return launch_cirs(*s, L); }, t[mode]);
+    }
+    std::swap(L.f[COSAHIP_RES], L.work1);
+    L.cirs_mode = t[1] <= t[2] ? 1 : 2;
+    if (choice) choice[nl - 1] = L.cirs_mode;
+  }
+  if (rc == COSAHIP_OK && s->par.viscous && !s->fuse_viscous && getenv("COSAHIP_VISCOUS") == nullptr &&
+      !s->lev.empty() && s->lev[0].nblocks) {
+    Level& L = s->lev[0];
+    float t[2] = {0, 0};
+    const bool keep = s->viscous_faces;
+    for (int f = 0; f <= 1 && rc == COSAHIP_OK; f++) {
+      s->viscous_faces = f == 1;
+      rc = timed([&]() { return launch_residual(*s, L, L.f[COSAHIP_Q], L.work1); }, t[f]);
+    }
+    s->viscous_faces = rc == COSAHIP_OK ? (t[1] <= t[0]) : keep;
+    if (choice) choice[3] = s->viscous_faces ? 1 : 0;
+  } else if (choice) {
+    choice[3] = s->viscous_faces ? 1 : 0;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // the launch sequence changed: a captured cycle graph is rebuilt
+  if (s->gexec) {
+    (void)hipGraphExecDestroy(s->gexec);
+    s->gexec = nullptr;
+  }
+  return rc;
+}
+
 extern "C" int cosahip_cycle_ready(cosahip_solve* h) {
   Solve* s = reinterpret_cast<Solve*>(h);
   if (!s) return COSAHIP_ERR_ARG;

@@ -130,7 +130,8 @@ struct Level {
   double* f[COSAHIP_NFIELDS] = {};
   double* dq = nullptr;       // HB term (qhb) and scratch
   double* work1 = nullptr;    // copy of res for the HB norm
-  double* dqc = nullptr;      // SST: (P3, 9, nsnap) cell-centre velocity differences of the stage
+  double* dqc = nullptr;
+  int cirs_mode = 0;          // CIRS i-sweep picked by cosahip_autotune: 0 none, 1 lds, 2 iv      // SST: (P3, 9, nsnap) cell-centre velocity differences of the stage
   double* cirs_r = nullptr;   // Thomas multipliers r(j), one per cell
   double* cirs_rt = nullptr;  // the same for k, omega (k-omega)
   double* rms_part = nullptr; // per-tile partial sums of squares

@@ -946,6 +946,25 @@ __global__ void __launch_bounds__(64) k_cirs_i(Par p, const Blk* __restrict__ bl
   }
 }
 
+// CIRS multipliers of the constant coefficient (irsop 1) in LDS: tridi's
+// r(a) = c(a) / (b(a) - r(a-1) a(a)) depend only on the position along the
+// line, and all lines of a work-group (one block) have the same length, so
+// thread 0 computes them once instead of every line storing and reloading
+// its own (same arithmetic, same values)
+constexpr int RTAB_MAX = 1024;
+__device__ __forceinline__ void fill_rtab(double* rtab, int L, double epsc) {
+  if (threadIdx.x == 0) {
+    double rp = 0.0;
+    for (int a = 1; a <= L; a++) {
+      const double ca = a == 1 ? 0.0 : -epsc, cb = 1 + 2 * epsc, cc = a == L ? 0.0 : -epsc;
+      const double r = a == 1 ? cc / cb : cc / (cb - rp * ca);
+      rtab[a] = r;
+      rp = r;
+    }
+  }
+  __syncthreads();
+}
+
 // CIRS j / k sweep of the mean flow (NM variables: dtvol, beta, rsc) and,
 // k-omega, of k and omega (NT variables: dtvolt, tbeta, rsct) in one pass:
 // one thread per (line, snapshot), each group with its own coefficients,
@@ -959,6 +978,7 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
                                                  const double* __restrict__ dtvolt, const double* __restrict__ tbeta,
                                                  double* __restrict__ rsct, int first, int last, double epsc) {
   constexpr int NV = NM + NT;
+  __shared__ double rtab[RTAB_MAX + 1];
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int item = t.base + threadIdx.x;
@@ -966,6 +986,8 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
   const int T1 = im;
   const int T2 = DIR == 3 ? jm : km;
   const int nlines = T1 * T2;
+  const bool tab = p.irsop == 1 && (DIR == 2 ? jm : km) <= RTAB_MAX;
+  if (tab) fill_rtab(rtab, DIR == 2 ? jm : km, epsc);
   if (item >= nlines * p.nsnap) return;
   const int n = item / nlines;
   const int rem = item - n * nlines;
@@ -991,14 +1013,14 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
     const double dtv = first ? dtb[c1] : 1.0;
     double r;
     if (a == 1) {
-      r = cc / cb;
+      r = tab ? rtab[1] : cc / cb;
 #pragma unroll
       for (int v = 0; v < NM; v++) {
         const double x = first ? rb[c3 + n3 * v] * dtv : rb[c3 + n3 * v];
         tp[v] = x / cb;
       }
     } else {
-      r = cc / (cb - rp * ca);
+      r = tab ? rtab[a] : cc / (cb - rp * ca);
 #pragma unroll
       for (int v = 0; v < NM; v++) {
         const double x = first ? rb[c3 + n3 * v] * dtv : rb[c3 + n3 * v];
@@ -1006,21 +1028,21 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
       }
     }
     rp = r;
-    rs[c3] = r;
+    if (!tab) rs[c3] = r;
     if (NT) {
       const double epst = cconst ? epsc : bbt[c1];
       const double cat = a == 1 ? 0.0 : -epst, cbt = 1 + 2 * epst, cct = a == L ? 0.0 : -epst;
       const double dtvt = first ? dtt[c1] : 1.0;
       double rt;
       if (a == 1) {
-        rt = cct / cbt;
+        rt = tab ? rtab[1] : cct / cbt;
 #pragma unroll
         for (int v = NM; v < NV; v++) {
           const double x = first ? rb[c3 + n3 * v] * dtvt : rb[c3 + n3 * v];
           tp[v] = x / cbt;
         }
       } else {
-        rt = cct / (cbt - rpt * cat);
+        rt = tab ? rtab[a] : cct / (cbt - rpt * cat);
 #pragma unroll
         for (int v = NM; v < NV; v++) {
           const double x = first ? rb[c3 + n3 * v] * dtvt : rb[c3 + n3 * v];
@@ -1028,7 +1050,7 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
         }
       }
       rpt = rt;
-      rst[c3] = rt;
+      if (!tab) rst[c3] = rt;
     }
 #pragma unroll
     for (int v = 0; v < NV; v++) rb[c3 + n3 * v] = tp[v];
@@ -1042,11 +1064,11 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
 #pragma unroll
       for (int v = 0; v < NV; v++) xv[v] = rb[c3 + n3 * v];
     } else {
-      const double r = rs[c3];
+      const double r = tab ? rtab[a] : rs[c3];
 #pragma unroll
       for (int v = 0; v < NM; v++) xv[v] = rb[c3 + n3 * v] - xn[v] * r;
       if (NT) {
-        const double rt = rst[c3];
+        const double rt = tab ? rtab[a] : rst[c3];
 #pragma unroll
         for (int v = NM; v < NV; v++) xv[v] = rb[c3 + n3 * v] - xn[v] * rt;
       }
@@ -1080,6 +1102,10 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
 // multipliers and store them to the same address of rsc / rsct for the
 // back substitution.  Arithmetic as in k_cirs_i (tridi, src/cosa.f90:
 // 2418-2443).
+// With the constant coefficient of irsop 1 the multipliers depend only on
+// the position along the line: every line of a work-group (one block) has
+// the same, so they are computed once into LDS (RTAB_MAX cells at most)
+// instead of being stored and reloaded per line and variable.
 template <int NV>
 __global__ void __launch_bounds__(256) k_cirs_iv(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
                                                  double* __restrict__ res, const double* __restrict__ dtvol,
@@ -1087,8 +1113,13 @@ __global__ void __launch_bounds__(256) k_cirs_iv(Par p, const Blk* __restrict__ 
                                                  const double* __restrict__ tbeta, double* __restrict__ rsc,
                                                  double* __restrict__ rsct, double epsc) {
   constexpr int LPT = 256 / NV;
+  __shared__ double rtab[RTAB_MAX + 1];
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
+  const int L = b.imax - 1;
+  const bool cconst = p.irsop == 1;
+  const bool tab = cconst && L <= RTAB_MAX;
+  if (tab) fill_rtab(rtab, L, epsc);
   const int ln = threadIdx.x / NV, v = threadIdx.x - (threadIdx.x / NV) * NV;
   if (ln >= LPT) return;
   const int jm = b.jmax - 1, km = b.kmax - 1;
@@ -1099,13 +1130,11 @@ __global__ void __launch_bounds__(256) k_cirs_iv(Par p, const Blk* __restrict__ 
   const int j = rem % jm + 1, k = rem / jm + 1;
   const ll n3 = n3of(b), n1 = n1of(b);
   const bool turb = v >= 5;
-  const bool cconst = p.irsop == 1;
   double* x = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n + n3 * v + i3(b, 0, j, k);
   const ll o1 = b.o1 * p.nsnap + n1 * (ll)n + i1(b, 0, j, k);
   const double* dt = (turb ? dtvolt : dtvol) + o1;
   const double* be = cconst ? nullptr : (turb ? tbeta : beta) + o1;
   double* rr = (turb ? rsct : rsc) + b.o3 * p.nsnap + n3 * (ll)n + i3(b, 0, j, k);
-  const int L = b.imax - 1;
   double tp = 0.0, rp = 0.0;
   for (int a0 = 1; a0 <= L; a0 += CI) {
     double xv[CI], dv[CI], ev[CI], rv[CI];
@@ -1128,10 +1157,10 @@ __global__ void __launch_bounds__(256) k_cirs_iv(Par p, const Blk* __restrict__ 
       const double cc = a == L ? 0.0 : -eps;
       double r;
       if (a == 1) {
-        r = cc / cb;
+        r = tab ? rtab[1] : cc / cb;
         tp = (xv[u] * dv[u]) / cb;
       } else {
-        r = cc / (cb - rp * ca);
+        r = tab ? rtab[a] : cc / (cb - rp * ca);
         tp = (xv[u] * dv[u] - tp * ca) / (cb - rp * ca);
       }
       rp = r;
@@ -1143,7 +1172,7 @@ __global__ void __launch_bounds__(256) k_cirs_iv(Par p, const Blk* __restrict__ 
       const int a = a0 + u;
       if (a <= L) {
         x[a] = xv[u];
-        rr[a] = rv[u];
+        if (!tab) rr[a] = rv[u];
       }
     }
   }
@@ -1157,7 +1186,7 @@ __global__ void __launch_bounds__(256) k_cirs_iv(Par p, const Blk* __restrict__ 
       const int a = a0 + u;
       if (a <= L) {
         xv[u] = x[a];
-        rv[u] = rr[a];
+        rv[u] = tab ? rtab[a] : rr[a];
       }
     }
 #pragma unroll
@@ -1417,7 +1446,8 @@ hipError_t launch_cirs(Solve& s, Level& L) {
   const double epsc = 0.25 * (s.par.rat * s.par.rat - 1.0);
   ll ilines = 0;
   for (const Blk& b : L.hblk) ilines += (ll)(b.jmax - 1) * (b.kmax - 1) * s.par.nsnap;
-  const bool iv = s.cirs_i_mode == 2 || (s.cirs_i_mode == 0 && ilines < CIRS_IV_MAX_LINES);
+  const int mode = s.cirs_i_mode ? s.cirs_i_mode : L.cirs_mode;
+  const bool iv = mode == 2 || (mode == 0 && ilines < CIRS_IV_MAX_LINES);
   if (iv && L.ntile[TK_LINES_IV]) {
     if (s.par.npde == 7)
       hipLaunchKernelGGL(k_cirs_iv<7>, grid1(L.ntile[TK_LINES_IV]), dim3(256), 0, s.stream, s.par, L.dblk,

@@ -290,7 +290,8 @@ __global__ void __launch_bounds__(256) k_turb_src(Par p, int nl, const Blk* __re
       arg1 = fmin(arg1, 4 * rho * (T)p.sigw2 * tke / acdt / (de * de));
       arg1 = arg1 * arg1 * arg1 * arg1;   // arg1**4 as ((a a) a) a
       arg1 = fmin(arg1, (T)1e2);
-      const T e2 = exp(2 * arg1);
+      // FP32: exp(2 arg1) overflows above arg1 = 44; F1 is 1 to float precision from arg1 = 9 on
+      const T e2 = sizeof(T) == 4 ? exp(fmin(2 * arg1, (T)80)) : exp(2 * arg1);
       const T f1 = (e2 - 1) / (e2 + 1);
       cdif = cdt * (1 - f1);
       gav = f1 * (T)p.g1 + (1 - f1) * (T)p.g2;

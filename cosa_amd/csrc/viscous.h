@@ -201,7 +201,8 @@ __device__ __forceinline__ void viscous_face(const Par& p, const Blk& b, int i, 
       const T acdt = fmax(cdt, (T)1e-20);
       arg1 = fmin(arg1, 4 * rho * (T)p.sigw2 * tke / acdt / (de * de));
       arg1 = fmin(arg1 * arg1 * arg1 * arg1, (T)1e2);   // arg1**4 as ((a a) a) a
-      const T e2 = exp(2 * arg1);
+      // FP32: exp(2 arg1) overflows above arg1 = 44; F1 is 1 to float precision from arg1 = 9 on
+      const T e2 = sizeof(T) == 4 ? exp(fmin(2 * arg1, (T)80)) : exp(2 * arg1);
       const T f1 = (e2 - 1) / (e2 + 1);
       sk = f1 * (T)p.sigk1 + (1 - f1) * (T)p.sigk2;
       sw = f1 * (T)p.sigw1 + (1 - f1) * (T)p.sigw2;

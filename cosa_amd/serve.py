@@ -49,7 +49,7 @@ class Request:
     drop: float = 1e-3         # retire when resrms(1) <= drop * first resrms(1)
     max_cycles: int = 400
     precision: int = 0         # cosahip_set_precision mode
-    tail_rms_factor: float = 1e-2   # mode 2: FP64 tail below this fraction of the first rms
+    tail_rms_factor: float = 1e-1   # mode 2: FP64 tail below this fraction of the first rms
 
     def make_case(self) -> "cases.Case":
         ni, nj = GRIDS[self.grid]

@@ -199,6 +199,14 @@ class Solve:
         self.history.append(r)
         return bool(done[0])
 
+    def autotune(self, reps: int = 3) -> Dict[str, object]:
+        """Time the interchangeable kernel arrangements on the current state
+        and keep the fastest (cosahip_autotune); results are unchanged."""
+        ch = (ctypes.c_int32 * 4)()
+        self._check(self.lib.cosahip_autotune(self.h, reps, ch), "autotune")
+        names = {0: None, 1: "lds", 2: "per_variable"}
+        return {"cirs_i_sweep": [names[ch[m]] for m in range(3)], "viscous_faces_stored": bool(ch[3])}
+
     def cycle_ready(self) -> bool:
         """True when the enqueued cycle has finished (cycle_wait will not block)."""
         r = self.lib.cosahip_cycle_ready(self.h)

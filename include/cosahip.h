@@ -283,6 +283,18 @@ int cosahip_cycle_wait(cosahip_solve* s, int32_t* done, double* resrms);
  * counterpart in the reference (a serving loop polls its solves with it). */
 int cosahip_cycle_ready(cosahip_solve* s);
 
+/* ---- autotuner (no counterpart in the reference) -------------------------
+ * Times the interchangeable kernel arrangements on the solve's current
+ * state and keeps the fastest: per level the CIRS i-sweep (one thread per
+ * line with LDS-staged chunks, or one thread per line and variable), and
+ * for viscous runs the viscous face fluxes (stored then differenced, or
+ * flux and difference in one line march).  The arrangements give
+ * bit-identical results; the state is not changed (timing runs on a copy of
+ * the residual).  choice[0..2]: CIRS i-sweep of levels 1..3 (1 LDS, 2 per
+ * variable, 0 level absent), choice[3]: 1 stored face fluxes, 0 march.
+ * Environment switches (COSAHIP_CIRS_I, COSAHIP_VISCOUS) take precedence. */
+int cosahip_autotune(cosahip_solve* s, int reps, int32_t* choice);
+
 /* ---- forces ------------------------------------------------------------
  * force_b (src/output.f90:4388-4886) summed over the level's blocks as
  * force does (src/output.f90:4290-4330), aircraft conventions: pressure and

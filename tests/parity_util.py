@@ -23,8 +23,11 @@ FLOOR = 1e-14    # absolute floor as a fraction of the field's max magnitude
 
 
 def parity(gpu: np.ndarray, cpu: np.ndarray, rtol: float = RTOL, floor: float = FLOOR, what: str = "") -> float:
-    """Assert the FP64 parity bar; return the worst normalised error."""
+    """Assert the FP64 parity bar; return the worst normalised error.  NaN
+    in either array fails (it would pass every comparison silently)."""
     assert gpu.shape == cpu.shape, what
+    assert not np.isnan(cpu).any(), f"{what}: NaN in the oracle's values"
+    assert not np.isnan(gpu).any(), f"{what}: NaN in the GPU values ({int(np.isnan(gpu).sum())})"
     ndiff = int(np.count_nonzero(gpu != cpu))
     if ndiff:
         ulp = np.abs(gpu.view(np.int64) - cpu.view(np.int64))

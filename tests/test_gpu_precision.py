@@ -59,6 +59,7 @@ def test_fp32_residual_error(c):
         g.call("residual", 1)
         m = interior_mask(c)
         rg, rc = g.download("res"), A["res"]
+        assert not np.isnan(rg).any() and not np.isnan(rc).any()
         assert np.all(rg[~m] == 0)
         n3 = rg.size // (c.npde * c.nsnap)
         # per equation over all blocks and snapshots: the level array is

@@ -239,3 +239,32 @@ def test_viscous_cirs_irsop(irsop):
         parity(g.download("betaxi"), oc[1]["betaxi"], what="betaxi")
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("name", ["rans_blocks2", "laminar_blocks3"])
+def test_autotune_keeps_results(name):
+    """cosahip_autotune times the interchangeable arrangements (CIRS
+    i-sweep per level, viscous face fluxes) on copies and keeps the fastest:
+    the state it leaves and every later cycle are bit-identical to an
+    untuned solve."""
+    from cosa_amd.solver import Solve
+    c = _mg_case(name)
+    q0 = c.perturbed(amp=0.01, seed=9)
+    a = Solve(c, device=0, q=q0)
+    b = Solve(c, device=0, q=q0)
+    try:
+        a.cycle(1)
+        b.cycle(1)
+        tuned = b.autotune(2)
+        _say(f"autotune {name}: {tuned}")
+        for nl in range(1, c.config["nlevel"] + 1):
+            assert np.array_equal(a.download("q", nl), b.download("q", nl))
+        for mgit in (2, 3):
+            a.cycle(mgit)
+            b.cycle(mgit)
+        for nl in range(1, c.config["nlevel"] + 1):
+            assert np.array_equal(a.download("q", nl), b.download("q", nl)), nl
+        assert np.array_equal(np.array(a.history), np.array(b.history))
+    finally:
+        a.close()
+        b.close()
