@@ -92,6 +92,11 @@ def load() -> ctypes.CDLL:
         "cosahip_cycle_wait": ([_VP, _IP, _DP], ctypes.c_int),
         "cosahip_cycle_ready": ([_VP], ctypes.c_int),
         "cosahip_autotune": ([_VP, ctypes.c_int, _IP], ctypes.c_int),
+        "cosahip_set_block_freestream": ([_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+                                         ctypes.c_int),
+        "cosahip_set_block_rms": ([_VP, ctypes.c_int], ctypes.c_int),
+        "cosahip_block_rms": ([_VP, _DP], ctypes.c_int),
+        "cosahip_force_blocks": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP], ctypes.c_int),
         "cosahip_precision_in_use": ([_VP], ctypes.c_int),
         "cosahip_build_info": ([], ctypes.c_char_p),
         "cosahip_timing_enable": ([_VP, ctypes.c_int], ctypes.c_int),
@@ -116,7 +121,8 @@ def exported_symbols():
             "cosahip_residual", "cosahip_src_hb", "cosahip_resg_rms", "cosahip_add_array", "cosahip_copy_array",
             "cosahip_cirs", "cosahip_update", "cosahip_tur_vis", "cosahip_fixq", "cosahip_turb_limits",
             "cosahip_smooth", "cosahip_restrict", "cosahip_prolong", "cosahip_cycle_enqueue", "cosahip_cycle_wait",
-            "cosahip_cycle_ready", "cosahip_autotune",
+            "cosahip_cycle_ready", "cosahip_autotune", "cosahip_set_block_freestream", "cosahip_set_block_rms",
+            "cosahip_block_rms", "cosahip_force_blocks",
             "cosahip_force", "cosahip_set_precision",
             "cosahip_precision_in_use",
             "cosahip_build_info", "cosahip_timing_enable", "cosahip_timing_read", "cosahip_timing_reset"]

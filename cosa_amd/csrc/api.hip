@@ -283,6 +283,8 @@ int cosahip_destroy(cosahip_solve* h) {
   if (s->dflag) (void)hipFree(s->dflag);
   if (s->ddhb) (void)hipFree(s->ddhb);
   if (s->hpinned) (void)hipHostFree(s->hpinned);
+  if (s->hpart) (void)hipHostFree(s->hpart);
+  if (s->dqfst) (void)hipFree(s->dqfst);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return COSAHIP_OK;
@@ -452,6 +454,24 @@ extern "C" int cosahip_define_level(cosahip_solve* h, int nl, int nblocks, const
   if (nl < 1 || nl > (int)s->lev.size()) return fail(s, COSAHIP_ERR_ARG, "level out of range");
   if (nblocks < 1 || !imax || !jmax || !kmax || !nbcs) return fail(s, COSAHIP_ERR_ARG, "bad block arrays");
   CK(hipSetDevice(s->device));
+  if (nl > 1 && s->lev[0].nblocks && s->lev[0].nblocks != nblocks)
+    return fail(s, COSAHIP_ERR_ARG, "every level has the blocks of level 1");
+  if (nl == 1) {
+    // per-block freestream, initially the configuration's for every block
+    s->bqfst.assign(5 * (size_t)nblocks, 0.0);
+    s->bflow.assign(3 * (size_t)nblocks, 0.0);
+    for (int b = 0; b < nblocks; b++) {
+      for (int v = 0; v < 5; v++) s->bqfst[5 * b + v] = s->par.qfst[v];
+      s->bflow[3 * b] = s->cfg.machfs;
+      s->bflow[3 * b + 1] = s->cfg.alpha;
+      s->bflow[3 * b + 2] = s->cfg.beta;
+    }
+    if (s->dqfst) (void)hipFree(s->dqfst);
+    s->dqfst = nullptr;
+    if (hipMalloc(&s->dqfst, 5 * (size_t)nblocks * sizeof(double)) != hipSuccess)
+      return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
+    CK(hipMemcpy(s->dqfst, s->bqfst.data(), 5 * (size_t)nblocks * sizeof(double), hipMemcpyHostToDevice));
+  }
   Level& L = s->lev[nl - 1];
   free_level(L);
   L.nl = nl;
@@ -789,6 +809,137 @@ extern "C" int cosahip_force(cosahip_solve* h, int nl, int nsurface, int include
   return COSAHIP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// blocks as independent solves (serving: requests of one grid batched as the
+// blocks of one solve; no counterpart in the reference)
+
+extern "C" int cosahip_set_block_freestream(cosahip_solve* h, int block, double machfs, double alpha,
+                                            double beta) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  const int nb = (int)s->bqfst.size() / 5;
+  if (block < 1 || block > nb) return fail(s, COSAHIP_ERR_ARG, "set_block_freestream: block out of range");
+  if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "set_block_freestream: a cycle is in flight");
+  CK(hipSetDevice(s->device));
+  const cosahip_config& c = s->cfg;
+  double* f = s->bqfst.data() + 5 * (block - 1);
+  // as the configuration's freestream (bc_far0, src/bc.f90:145-152)
+  f[0] = c.rhoinf;
+  f[1] = machfs * cos(alpha) * cos(beta);
+  f[2] = machfs * sin(alpha) * cos(beta);
+  f[3] = machfs * sin(beta);
+  f[4] = c.pinf;
+  s->bflow[3 * (block - 1)] = machfs;
+  s->bflow[3 * (block - 1) + 1] = alpha;
+  s->bflow[3 * (block - 1) + 2] = beta;
+  CK(hipStreamSynchronize(s->stream));
+  CK(hipMemcpy(s->dqfst + 5 * (block - 1), f, 5 * sizeof(double), hipMemcpyHostToDevice));
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_set_block_rms(cosahip_solve* h, int on) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  if (!s->lev[0].nblocks) return fail(s, COSAHIP_ERR_STATE, "set_block_rms: level 1 not defined");
+  CK(hipSetDevice(s->device));
+  const ll n = (ll)s->lev[0].ntile[TK_CELLS] * 8;
+  if (on && !s->hpart) {
+    if (hipHostMalloc((void**)&s->hpart, std::max<ll>(n, 8) * sizeof(double)) != hipSuccess)
+      return fail(s, COSAHIP_ERR_NOMEM, "pinned allocation failed");
+    s->hpart_n = n;
+    memset(s->hpart, 0, std::max<ll>(n, 8) * sizeof(double));
+  }
+  s->block_rms = on != 0;
+  return COSAHIP_OK;
+}
+
+// resrms(1:npde+1) of every block from the level-1 partial sums of the last
+// norm evaluation (tiles never span blocks; each block's tiles summed in
+// order), as resg_rms / resg_rms_hb with the block's own cell count
+extern "C" int cosahip_block_rms(cosahip_solve* h, double* out) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s || !out) return COSAHIP_ERR_ARG;
+  if (!s->block_rms || !s->hpart) return fail(s, COSAHIP_ERR_STATE, "block_rms: not enabled");
+  if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "block_rms: a cycle is in flight");
+  CK(hipStreamSynchronize(s->stream));
+  const Level& L = s->lev[0];
+  const int np = s->par.npde;
+  std::vector<double> sum((size_t)L.nblocks * 8, 0.0);
+  for (int t = 0; t < L.ntile[TK_CELLS]; t++) {
+    const int b = L.htile[TK_CELLS][t].block;
+    for (int v = 0; v < np; v++) sum[8 * (size_t)b + v] = sum[8 * (size_t)b + v] + s->hpart[(ll)t * 8 + v];
+  }
+  for (int b = 0; b < L.nblocks; b++) {
+    const Blk& B = L.hblk[b];
+    const double nc = (double)((ll)(B.imax - 1) * (B.jmax - 1) * (B.kmax - 1));
+    double tot = 0;
+    for (int v = 0; v < np; v++) tot = tot + sum[8 * (size_t)b + v];
+    double* o = out + 8 * (size_t)b;
+    if (s->cfg.harbal) {
+      const double d5 = s->par.nsnap;
+      for (int v = 0; v < np; v++) o[v] = sqrt(sum[8 * (size_t)b + v] / nc / d5);
+      o[np] = sqrt(tot / nc / d5 / np);
+    } else {
+      for (int v = 0; v < np; v++) o[v] = sqrt(sum[8 * (size_t)b + v] / nc);
+      o[np] = sqrt(tot / nc / np);
+    }
+    for (int v = np + 1; v < 8; v++) o[v] = 0.0;
+  }
+  return COSAHIP_OK;
+}
+
+// force coefficients of every block with the block's own Mach number and
+// angles: force_b's sums per block (k_force_terms, k_force_sums), then the
+// projection of k_force_coef without the block sum (one block's share of
+// cosahip_force, the same arithmetic)
+extern "C" int cosahip_force_blocks(cosahip_solve* h, int nl, int nsurface, int include_inviscid,
+                                    const double* lngth, const double* momr, double* coef) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  Level* Lp = get_level(s, nl);
+  if (!Lp) return fail(s, COSAHIP_ERR_STATE, "force_blocks: level not defined");
+  if (nsurface < 1 || nsurface > 100 || !lngth || !momr || !coef)
+    return fail(s, COSAHIP_ERR_ARG, "force_blocks: nsurface must be 1..100 and lngth, momr, coef non-null");
+  Level& L = *Lp;
+  const int nsnap = s->par.nsnap;
+  std::vector<double> tmp((size_t)12 * nsnap * nsurface);
+  int rc = cosahip_force(h, nl, nsurface, include_inviscid, lngth, momr, tmp.data());   // fills L.fsums
+  if (rc) return rc;
+  std::vector<double> S((size_t)L.nblocks * nsnap * nsurface * 12);
+  CK(hipMemcpy(S.data(), L.fsums, S.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int b = 0; b < L.nblocks; b++) {
+    const double mach0 = s->bflow[3 * b], alpha = s->bflow[3 * b + 1], beta = s->bflow[3 * b + 2];
+    const double den = std::sqrt(std::sin(beta) * std::sin(beta) + std::cos(alpha) * std::cos(alpha) *
+                                                                       (std::cos(beta) * std::cos(beta)));
+    const double aax = -std::sin(alpha) * std::cos(alpha) * (std::cos(beta) * std::cos(beta)) / den, aay = den,
+                 aaz = std::sin(alpha) * std::sin(beta) * std::cos(beta) / den;
+    const double ssx = std::cos(alpha) * std::cos(beta), ssy = std::sin(alpha) * std::cos(beta),
+                 ssz = std::sin(beta);
+    const double bbx = -std::sin(beta) / den, bby = 0.0, bbz = std::cos(alpha) * std::cos(beta) / den;
+    const double l1 = lngth[0], l2 = lngth[1], l3 = lngth[2];
+    const double mach = mach0 > 1e-14 ? mach0 : 1.0;
+    for (int sf = 0; sf < nsurface; sf++)
+      for (int n = 0; n < nsnap; n++) {
+        const double* fp = S.data() + (((size_t)b * nsnap + n) * nsurface + sf) * 12;
+        const double* fv = fp + 6;
+        double* o = coef + (((size_t)b * nsurface + sf) * nsnap + n) * 12;
+        const double tot0 = 0.0;
+        o[0] = tot0 + (fp[0] * aax + fp[1] * aay + fp[2] * aaz) * 2 / (l1 * l2 * (mach * mach));
+        o[1] = tot0 + (fp[0] * ssx + fp[1] * ssy + fp[2] * ssz) * 2 / (l1 * l2 * (mach * mach));
+        o[2] = tot0 + (fp[0] * bbx + fp[1] * bby + fp[2] * bbz) * 2 / (l1 * l2 * (mach * mach));
+        o[3] = tot0 + fp[3] * 2 / (l1 * l2 * l3 * (mach * mach));
+        o[4] = tot0 + fp[4] * 2 / (l1 * l2 * l3 * (mach * mach));
+        o[5] = tot0 + fp[5] * 2 / (l1 * l2 * l3 * (mach * mach));
+        o[6] = tot0 + (fv[0] * aax + fv[1] * aay + fv[2] * aaz) * 2 / (l1 * l2 * (mach * mach));
+        o[7] = tot0 + (fv[0] * ssx + fv[1] * ssy + fv[2] * ssz) * 2 / (l1 * l2 * (mach * mach));
+        o[8] = tot0 + (fv[0] * bbx + fv[1] * bby + fv[2] * bbz) * 2 / (l1 * l2 * (mach * mach));
+        o[9] = tot0 + fv[3] * 2 / (l1 * l2 * l3 * (mach * mach));
+        o[10] = tot0 + fv[4] * 2 / (l1 * l2 * l3 * (mach * mach));
+        o[11] = tot0 + fv[5] * 2 / (l1 * l2 * l3 * (mach * mach));
+      }
+  }
+  return COSAHIP_OK;
+}
+
 extern "C" int cosahip_set_precision(cosahip_solve* h, int mode, double tail_rms) {
   Solve* s = reinterpret_cast<Solve*>(h);
   if (!s) return COSAHIP_ERR_ARG;
@@ -1032,6 +1183,9 @@ static int rms_enqueue(Solve* s, Level& L, int slot) {
   }
   CK(hipMemcpyAsync(s->hpinned + slot, L.rms_part + (ll)L.ntile[TK_CELLS] * 8, np * sizeof(double),
                     hipMemcpyDeviceToHost, s->stream));
+  if (s->block_rms && L.nl == 1 && s->hpart && s->hpart_n >= (ll)L.ntile[TK_CELLS] * 8)
+    CK(hipMemcpyAsync(s->hpart, L.rms_part, (ll)L.ntile[TK_CELLS] * 8 * sizeof(double), hipMemcpyDeviceToHost,
+                      s->stream));
   return COSAHIP_OK;
 }
 
@@ -1428,9 +1582,12 @@ extern "C" int cosahip_autotune(cosahip_solve* h, int reps, int32_t* choice) {
     Level& L = s->lev[0];
     float t[2] = {0, 0};
     const bool keep = s->viscous_faces;
+    // on copies: the residual also fills the edge halos of the state it reads
+    const ll nq = L.n3tot * s->par.npde * s->par.nsnap;
+    CK(hipMemcpyAsync(L.dq, L.f[COSAHIP_Q], nq * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
     for (int f = 0; f <= 1 && rc == COSAHIP_OK; f++) {
       s->viscous_faces = f == 1;
-      rc = timed([&]() { return launch_residual(*s, L, L.f[COSAHIP_Q], L.work1); }, t[f]);
+      rc = timed([&]() { return launch_residual(*s, L, L.dq, L.work1); }, t[f]);
     }
     s->viscous_faces = rc == COSAHIP_OK ? (t[1] <= t[0]) : keep;
     if (choice) choice[3] = s->viscous_faces ? 1 : 0;

@@ -190,6 +190,14 @@ struct Solve {
   double* ddhb = nullptr;
   unsigned long long* dflag = nullptr;   // prtest first offender key
   double* hpinned = nullptr;             // small pinned buffer for results
+  // per-block freestream (independent solves batched as blocks of one
+  // solve): qfst(1:5) of bc_far0 per block on the device, and the Mach
+  // number and angles per block for the force coefficients
+  double* dqfst = nullptr;
+  std::vector<double> bqfst, bflow;      // 5 and 3 (machfs, alpha, beta) per block
+  bool block_rms = false;                // copy every tile's norm partials with the level-1 norms
+  double* hpart = nullptr;               // pinned: the partials (ntile x 8)
+  ll hpart_n = 0;
   double resrms[9] = {};                 // resrms(1:8), index 0 unused
   std::string err;
   int32_t errinfo[8] = {};

@@ -617,7 +617,7 @@ __device__ __forceinline__ int cycd(int d, int m) { return (d - 1 + m - 1) % 3 +
 __global__ void __launch_bounds__(256) k_bc(Par p, const Blk* __restrict__ blks, const PatchRec* __restrict__ pr,
                                             int first, double* __restrict__ q, const double* __restrict__ si,
                                             const double* __restrict__ sj, const double* __restrict__ sk,
-                                            const double* __restrict__ dist) {
+                                            const double* __restrict__ dist, const double* __restrict__ qfb) {
   const PatchRec rec = pr[first + blockIdx.y];
   const int item = blockIdx.x * blockDim.x + threadIdx.x;
   if (item >= rec.count * p.nsnap) return;
@@ -722,11 +722,13 @@ __global__ void __launch_bounds__(256) k_bc(Par p, const Blk* __restrict__ blks,
     return;
   }
   if (type == 30) {   // bc_far0 (src/bc.f90:124-403)
+    // freestream of the patch's block (blocks may be independent solves)
+    const double* qfst = qfb + 5 * rec.block;
     const double gamma = p.gamma;
     const double sinf = p.pinf / pow(p.rhoinf, gamma);
     double sx = sm[0] * sgnm, sy = sm[1] * sgnm, sz = sm[2] * sgnm;
     double qext = qb[xn + n3] * sx + qb[xn + 2 * n3] * sy + qb[xn + 3 * n3] * sz;
-    double qinf = p.qfst[1] * sx + p.qfst[2] * sy + p.qfst[3] * sz;
+    double qinf = qfst[1] * sx + qfst[2] * sy + qfst[3] * sz;
     double rhoext = qb[xn];
     double pext = qb[xn + 4 * n3];
     double aext = sqrt(gamma * pext / rhoext);
@@ -738,15 +740,15 @@ __global__ void __launch_bounds__(256) k_bc(Par p, const Blk* __restrict__ blks,
     double sext = pext / pow(rhoext, gamma);
     if (qdotn <= 0.0) {
       if (fabs(mach) >= 1.0) {
-        for (int v = 0; v < 5; v++) hv[v] = p.qfst[v];
+        for (int v = 0; v < 5; v++) hv[v] = qfst[v];
         set = true;
       } else if (fabs(mach) < 1.0) {
         double rhobc = pow(a * a / (gamma * sinf), 1.0 / (gamma - 1));
         double pb = rhobc * a * a / gamma;
         hv[0] = rhobc;
-        hv[1] = p.qfst[1] + (qdotn - qinf) * sx;
-        hv[2] = p.qfst[2] + (qdotn - qinf) * sy;
-        hv[3] = p.qfst[3] + (qdotn - qinf) * sz;
+        hv[1] = qfst[1] + (qdotn - qinf) * sx;
+        hv[2] = qfst[2] + (qdotn - qinf) * sy;
+        hv[3] = qfst[3] + (qdotn - qinf) * sz;
         hv[4] = pb;
         set = true;
       }
@@ -1199,14 +1201,14 @@ hipError_t launch_bc(Solve& s, Level& L, double* q) {
   if (!L.thin) {
     dim3 g((L.maxpatch * s.par.nsnap + TS - 1) / TS, np);
     hipLaunchKernelGGL(k_bc, g, dim3(TS), 0, s.stream, s.par, L.dblk, L.dpatch, 0, q, L.f[COSAHIP_SI],
-                       L.f[COSAHIP_SJ], L.f[COSAHIP_SK], L.f[COSAHIP_DIST]);
+                       L.f[COSAHIP_SJ], L.f[COSAHIP_SK], L.f[COSAHIP_DIST], s.dqfst);
   } else {
     // a block one cell thick: patches read halos written by earlier
     // patches, so keep the reference's patch order (one launch per patch)
     for (int ip = 0; ip < np; ip++) {
       dim3 g((L.hpatch[ip].count * s.par.nsnap + TS - 1) / TS, 1);
       hipLaunchKernelGGL(k_bc, g, dim3(TS), 0, s.stream, s.par, L.dblk, L.dpatch, ip, q, L.f[COSAHIP_SI],
-                         L.f[COSAHIP_SJ], L.f[COSAHIP_SK], L.f[COSAHIP_DIST]);
+                         L.f[COSAHIP_SJ], L.f[COSAHIP_SK], L.f[COSAHIP_DIST], s.dqfst);
     }
   }
   return hipGetLastError();

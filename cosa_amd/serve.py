@@ -176,6 +176,141 @@ class Server:
         return self.results
 
 
+# slots of one batch solve per grid size (requests of one grid run as the
+# blocks of one solve, so small grids share their launches)
+BATCH_SLOTS = {"small": 16, "medium": 8, "large": 2}
+
+
+def batch_case(req: Request, nslots: int) -> "cases.Case":
+    """`nslots` copies of the request's grid as the blocks of one case, each
+    with its own seam self-cut and no cut between copies: every block
+    evolves as the grid solved alone."""
+    one = req.make_case()
+    b = one.blocks[0]
+    cut = one.cuts[0]
+    cuts = [cases.Cut(k + 1, cut.istrt1, cut.iend1, k + 1, cut.istrt2, cut.iend2, cut.iord) for k in range(nslots)]
+    return cases.Case(dict(one.config), [b] * nslots, cuts, [one.xyz[0]] * nslots, name=f"batch_{req.grid}",
+                      motion=one.motion)
+
+
+def block_slice(c: "cases.Case", block: int, nl: int = 1) -> slice:
+    """The P3 slice (all variables and snapshots) of block `block` (0-based)."""
+    L = c.levels[nl - 1]
+    o3, p3 = L.offsets(3), L.points(3)
+    w = c.npde * c.nsnap
+    return slice(o3[block] * w, (o3[block] + p3[block]) * w)
+
+
+@dataclass
+class BatchServer:
+    """Continuous batching at block granularity: the requests of one grid
+    size and harmonic count share a batch solve of BATCH_SLOTS[grid] blocks;
+    an arriving request takes a free slot (its block gets the request's
+    freestream and initial state between cycles), every batch keeps one
+    cycle in flight, and a slot retires at its own residual drop (per-block
+    norms) with its own force coefficients, freeing the slot for the next
+    request.  FP64 or FP32-residual (precision 1) per batch."""
+    device: int = 0
+    keep_state: bool = False
+    slots: Dict[str, int] = field(default_factory=lambda: dict(BATCH_SLOTS))
+    results: List[Result] = field(default_factory=list)
+
+    def run(self, requests: List[Request], clock=time.perf_counter) -> List[Result]:
+        from .solver import Solve
+        keyof = lambda r: (r.grid, r.nharms, r.precision)  # noqa: E731
+        built = {}      # key -> batch case (built before the clock, the client's set-up)
+        init = {}       # rid -> the request's initial block state
+        for r in requests:
+            k = keyof(r)
+            if k not in built:
+                built[k] = batch_case(r, self.slots[r.grid])
+            one = r.make_case()
+            init[r.rid] = [one.freestream(nl) for nl in range(1, one.config["nlevel"] + 1)]
+        waiting = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        batches: Dict[tuple, dict] = {}
+        t0 = clock()
+        while waiting or any(any(sl is not None for sl in b["slot"]) for b in batches.values()):
+            now = clock() - t0
+            # admission into free slots; a batch is created on its first request
+            admitted = {}
+            while waiting and waiting[0].arrival <= now:
+                r = waiting[0]
+                k = keyof(r)
+                b = batches.get(k)
+                if b is None:
+                    c = built[k]
+                    sv = Solve(c, device=self.device, q=c.freestream())
+                    sv.set_block_rms(True)
+                    if r.precision == 1:
+                        sv.set_precision(1)
+                    b = batches[k] = dict(case=c, solve=sv, slot=[None] * c.levels[0].nblocks, pending=False,
+                                          mgit=0, q=None)
+                if b["pending"] or None not in b["slot"]:
+                    break        # wait for the batch's cycle / a free slot (arrival order kept)
+                waiting.pop(0)
+                i = b["slot"].index(None)
+                admitted.setdefault(k, []).append((i, r))
+                b["slot"][i] = dict(req=r, first=None, cycles=0, t_admit=clock() - t0)
+            for k, lst in admitted.items():
+                # the slot's block on every level starts from the request's
+                # freestream, as a solve of its own grid does
+                b = batches[k]
+                sv, c = b["solve"], b["case"]
+                for i, r in lst:
+                    sv.set_block_freestream(i + 1, r.mach, r.alpha_deg)
+                for nl in range(1, c.config["nlevel"] + 1):
+                    q = sv.download("q", nl)
+                    for i, r in lst:
+                        q[block_slice(c, i, nl)] = init[r.rid][nl - 1]
+                    sv.upload("q", q, nl)
+            progressed = False
+            for k, b in batches.items():
+                sv, c = b["solve"], b["case"]
+                if not any(sl is not None for sl in b["slot"]):
+                    continue
+                if not b["pending"]:
+                    b["mgit"] += 1
+                    sv.cycle_enqueue(b["mgit"])
+                    b["pending"] = progressed = True
+                    continue
+                if not sv.cycle_ready():
+                    continue
+                sv.cycle_wait()
+                b["pending"] = False
+                progressed = True
+                rms = sv.block_rms()
+                done = []
+                for i, sl in enumerate(b["slot"]):
+                    if sl is None:
+                        continue
+                    sl["cycles"] += 1
+                    r1 = float(rms[i, 0])
+                    if sl["first"] is None:
+                        sl["first"] = r1
+                    conv = r1 <= sl["req"].drop * sl["first"]
+                    if conv or sl["cycles"] >= sl["req"].max_cycles:
+                        done.append((i, conv))
+                if done:
+                    f = sv.force_blocks(nsurface=1, include_inviscid=True)
+                    q = sv.download("q") if self.keep_state else None
+                    t_done = clock() - t0
+                    nc = c.ncell() // c.levels[0].nblocks
+                    for i, conv in done:
+                        sl = b["slot"][i]
+                        r = sl["req"]
+                        self.results.append(Result(
+                            rid=r.rid, cells=nc * c.nsnap, cycles=sl["cycles"], converged=conv, t_admit=sl["t_admit"],
+                            t_done=t_done, latency=t_done - r.arrival, resrms=[], forces=f[i, 0],
+                            updates=sv.cell_updates_per_cycle() / c.levels[0].nblocks * sl["cycles"],
+                            q=q[block_slice(c, i)].copy() if q is not None else None))
+                        b["slot"][i] = None
+            if not progressed:
+                time.sleep(2e-5)
+        for b in batches.values():
+            b["solve"].close()
+        return self.results
+
+
 def summarize(results: List[Result], wall: float, cell_updates: float) -> dict:
     lat = np.array([r.latency for r in results])
     return {"requests": len(results), "converged": int(sum(r.converged for r in results)),
@@ -246,6 +381,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32", "mixed"], default="mixed")
     ap.add_argument("--drop", type=float, default=1e-2)
     ap.add_argument("--max-cycles", type=int, default=600)
+    ap.add_argument("--batch", action="store_true",
+                    help="serve requests of one grid as the blocks of shared batch solves (FP64 / FP32 residual)")
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="GPU_MAX_HW_QUEUES of this process (HIP's default 4 lets only 4 streams run at once)")
     ap.add_argument("--budget-gib", type=float, default=64.0)
@@ -256,11 +393,15 @@ def main():
     os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(max(a.hw_queues, 1), 32)))
     prec = {"fp64": 0, "fp32": 1, "mixed": 2}[a.precision]
     reqs = request_mix(a.requests, rate=a.rate, precision=prec, drop=a.drop, max_cycles=a.max_cycles)
-    srv = Server(budget_bytes=int(a.budget_gib * (1 << 30)))
+    if a.batch:
+        for r in reqs:    # batches run one precision throughout: mixed is served as FP32 residual
+            r.precision = 1 if prec else 0
+    srv = BatchServer() if a.batch else Server(budget_bytes=int(a.budget_gib * (1 << 30)))
     t0 = time.perf_counter()
     res = srv.run(reqs)
     wall = time.perf_counter() - t0
-    out = {"serving": summarize(res, wall, cell_updates(res)), "precision": a.precision,
+    out = {"serving": summarize(res, wall, cell_updates(res)), "mode": "batch" if a.batch else "per-solve streams",
+           "precision": a.precision,
            "drop": a.drop, "rate_per_s": a.rate}
     if a.baseline:
         t0 = time.perf_counter()

@@ -207,6 +207,36 @@ class Solve:
         names = {0: None, 1: "lds", 2: "per_variable"}
         return {"cirs_i_sweep": [names[ch[m]] for m in range(3)], "viscous_faces_stored": bool(ch[3])}
 
+    # -- blocks as independent solves (serving batches) -----------------------
+    def set_block_freestream(self, block: int, machfs: float, alpha_deg: float, beta_deg: float = 0.0):
+        """Freestream of block `block` (1-based) for bc_far0 and its forces;
+        angles in degrees converted as the configuration converts them."""
+        import math
+        self._check(self.lib.cosahip_set_block_freestream(self.h, block, machfs, alpha_deg * math.pi / 180.0,
+                                                          beta_deg * math.pi / 180.0), "set_block_freestream")
+
+    def set_block_rms(self, on: bool = True):
+        self._check(self.lib.cosahip_set_block_rms(self.h, int(on)), "set_block_rms")
+
+    def block_rms(self) -> np.ndarray:
+        """resrms(1:8) per block of the last level-1 norms, (nblocks, 8)."""
+        nb = self.case.levels[0].nblocks
+        out = np.zeros(8 * nb)
+        self._check(self.lib.cosahip_block_rms(self.h, _lib.dptr(out)), "block_rms")
+        return out.reshape(nb, 8)
+
+    def force_blocks(self, nsurface: int = 1, include_inviscid: bool = True, lngth=(1.0, 1.0, 1.0),
+                     xyzmom=(0.25, 0.0, 0.0), nl: int = 1) -> np.ndarray:
+        """cosahip_force per block with the block's own freestream:
+        (nblocks, nsurface, nsnap, 12)."""
+        nb, ns = self.case.levels[nl - 1].nblocks, self.case.nsnap
+        lg = np.ascontiguousarray(lngth, np.float64)
+        momr = np.ascontiguousarray(self.case.moment_centres(xyzmom), np.float64)
+        out = np.zeros(nb * nsurface * ns * 12)
+        self._check(self.lib.cosahip_force_blocks(self.h, nl, nsurface, int(include_inviscid), _lib.dptr(lg),
+                                                  _lib.dptr(momr), _lib.dptr(out)), "force_blocks")
+        return out.reshape(nb, nsurface, ns, 12)
+
     def cycle_ready(self) -> bool:
         """True when the enqueued cycle has finished (cycle_wait will not block)."""
         r = self.lib.cosahip_cycle_ready(self.h)

@@ -295,6 +295,24 @@ int cosahip_cycle_ready(cosahip_solve* s);
  * Environment switches (COSAHIP_CIRS_I, COSAHIP_VISCOUS) take precedence. */
 int cosahip_autotune(cosahip_solve* s, int reps, int32_t* choice);
 
+/* ---- blocks as independent solves (no counterpart in the reference) -----
+ * For serving, requests of one grid run as the blocks of one solve: every
+ * block has its own freestream (bc_far0's qfst, set from the Mach number and
+ * angles as the configuration's), its own norms and its own force
+ * coefficients.  Blocks without cuts between them evolve exactly as the
+ * same grid solved alone.  Block numbers are 1-based.
+ *   cosahip_set_block_freestream  between cycles
+ *   cosahip_set_block_rms(on)     keep every tile's norm partials of level 1
+ *   cosahip_block_rms(out)        resrms(1:8) per block of the last norm
+ *                                 evaluation, out[8 * nblocks]
+ *   cosahip_force_blocks(...)     cosahip_force per block with the block's
+ *                                 Mach number and angles, coef[nblocks][nsurface][2N+1][12] */
+int cosahip_set_block_freestream(cosahip_solve* s, int block, double machfs, double alpha, double beta);
+int cosahip_set_block_rms(cosahip_solve* s, int on);
+int cosahip_block_rms(cosahip_solve* s, double* out);
+int cosahip_force_blocks(cosahip_solve* s, int nl, int nsurface, int include_inviscid, const double* lngth,
+                         const double* momr, double* coef);
+
 /* ---- forces ------------------------------------------------------------
  * force_b (src/output.f90:4388-4886) summed over the level's blocks as
  * force does (src/output.f90:4290-4330), aircraft conventions: pressure and
