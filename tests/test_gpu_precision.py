@@ -19,6 +19,9 @@ Stated errors (asserted below, measured values are printed):
     mode 2's by less than 1e-9.
   - k-omega, 20 iterations from the same state: max |q32 - q64| <= 1e-5 *
     max |q|, and every resrms within 1e-3 of its FP64 value (relative).
+  - harmonic balance (pitching aerofoil, N = 2), after the same cycles: the
+    mean and the first two cosine/sine harmonics of CL and CM within 1e-4 of
+    FP64's in mode 1, within 1e-5 in mode 2.
 """
 import numpy as np
 import pytest
@@ -192,3 +195,58 @@ def test_precision_rejects_unsupported(monkeypatch):
         assert g.precision_in_use() == 64
     finally:
         g.close()
+
+
+def test_fp32_hb_harmonics():
+    """Harmonic balance (NACA 0012 pitching by 0.02 rad about the quarter
+    chord, M 0.4, N = 2, two levels): the harmonics of CL and CM over the
+    snapshots -- mean, first and second cosine and sine amplitudes -- after
+    1500 cycles from freestream in FP64, FP32 residual (mode 1) and mixed
+    (mode 2, FP64 once resrms(1) is 1e-1 of its first value: mode 1's floor
+    on this case is about 3e-2 of it).  The FP64 run
+    drops resrms(1) by more than 1e3; mode 1 stays within 1e-4 of its
+    harmonics, mode 2 within 1e-5."""
+    from cosa_amd.solver import Solve
+    c = cases.naca_case(65, 17, 5, nlevel=2, name="naca2_pitch_hb2", irsop=3, psirs=0.125, cfl_irs=4.0, cfl=2.0,
+                        nl_crs=2, machfs=0.4, ncycle=100000, toler=0.0, nharms=2, harbal=1, omega=0.6,
+                        pitch=dict(dtheta0=0.02, xh=0.25, yh=0.0))
+    ncyc = 1500
+    ns = c.nsnap
+    t = np.arange(ns) * 2 * np.pi / ns        # omega t_n of the snapshots
+
+    def harmonics(x):
+        out = [x.mean()]
+        for k in (1, 2):
+            out += [2 / ns * np.sum(x * np.cos(k * t)), 2 / ns * np.sum(x * np.sin(k * t))]
+        return np.array(out)
+
+    res = {}
+    tail = 0.0
+    for mode in (0, 1, 2):
+        g = Solve(c, device=0, q=c.freestream())
+        try:
+            if mode:
+                g.set_precision(mode, tail)
+            for mgit in range(1, ncyc + 1):
+                g.cycle(mgit)
+            f = g.force(nsurface=1, include_inviscid=True)[0]      # (nsnap, 12)
+            h = np.array([x[0] for x in g.history])
+            assert np.all(np.isfinite(f)) and np.all(np.isfinite(h))
+            res[mode] = (harmonics(f[:, 0]), harmonics(f[:, 5]), h, g.precision_in_use())
+        finally:
+            g.close()
+        if mode == 0:
+            tail = 1e-1 * res[0][2][0]
+    cl64, cm64, h64, _ = res[0]
+    _say(f"  resrms(1) FP64 {h64[0]:.3e} -> {h64[-1]:.3e} in {ncyc} cycles")
+    _say(f"  CL harmonics (mean, a1, b1, a2, b2) FP64 {np.array2string(cl64, precision=6)}")
+    _say(f"  CM harmonics (mean, a1, b1, a2, b2) FP64 {np.array2string(cm64, precision=6)}")
+    err = {}
+    for mode in (1, 2):
+        cl, cm, h, p = res[mode]
+        err[mode] = max(np.abs(cl - cl64).max(), np.abs(cm - cm64).max())
+        _say(f"  mode {mode} (in use FP{p}): max |dCL_k| {np.abs(cl - cl64).max():.2e}, "
+             f"max |dCM_k| {np.abs(cm - cm64).max():.2e}, resrms(1) last {h[-1]:.3e}")
+    assert h64[-1] < 1e-3 * h64[0]
+    assert res[1][3] == 32 and res[2][3] == 64
+    assert err[1] <= 1e-4 and err[2] <= 1e-5
