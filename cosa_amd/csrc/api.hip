@@ -84,7 +84,7 @@ static void free_level(Level& L) {
     if (L.f[f]) (void)hipFree(L.f[f]);
   void* ptrs[] = {L.dblk, L.dq, L.work1, L.cirs_r, L.cirs_rt, L.rms_part, L.sbc, L.sbc_doff, L.dpatch, L.dcut,
                   L.vflux, L.tbeta[0], L.tbeta[1], L.tbeta[2], L.h0tab, L.dxd, L.q32, L.m32[0], L.m32[1], L.m32[2],
-                  L.dxd32, L.mut32, L.dwall, L.fterms, L.fsums, L.fcoef, L.fgeom, L.dqc};
+                  L.dxd32, L.mut32, L.dwall, L.fterms, L.fsums, L.fcoef, L.fgeom, L.dqc, L.il_rtab};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (float* p : L.xd32)
@@ -249,7 +249,7 @@ int cosahip_create(const cosahip_config* cfg, int device, cosahip_solve** out) {
   const char* gk = getenv("COSAHIP_GRAPH");
   s->use_graph = !(gk && strcmp(gk, "0") == 0);
   const char* ck = getenv("COSAHIP_CIRS_I");
-  s->cirs_i_mode = ck && strcmp(ck, "lds") == 0 ? 1 : (ck && strcmp(ck, "iv") == 0 ? 2 : 0);
+  s->cirs_i_mode = !ck ? 0 : strcmp(ck, "lds") == 0 ? 1 : strcmp(ck, "iv") == 0 ? 2 : strcmp(ck, "line") == 0 ? 3 : 0;
   const char* vk = getenv("COSAHIP_VISCOUS");
   s->fuse_viscous = vk && strcmp(vk, "fused") == 0;
   s->viscous_faces = !(vk && (strcmp(vk, "march") == 0 || strcmp(vk, "fused") == 0));
@@ -330,9 +330,60 @@ static void add_tiles(std::vector<Tile>& v, int block, ll items, int per) {
   for (ll b0 = 0; b0 < items; b0 += per) v.push_back(Tile{block, (int)b0});
 }
 
+// k_cirs_il (CIRS i-sweep, constant coefficient of irsop 1): how many whole
+// lines of all npde variables one work-group keeps in LDS, within
+// IL_LDS_BUDGET bytes (two tables, lines, per-row offsets), so that four
+// work-groups share a CU.
+constexpr int IL_LDS_BUDGET = 40960;
+
+static void plan_cirs_il(Solve* s, Level& L) {
+  L.il_lpw = 0;
+  L.il_stride = 0;
+  if (s->par.irsop != 1 || !L.nblocks) return;
+  int lmax = 1;
+  for (const Blk& B : L.hblk) lmax = std::max(lmax, B.imax - 1);
+  const int S = lmax | 1;
+  const int nv = s->par.npde;
+  const int lpw = std::min((IL_LDS_BUDGET - 16 * S) / (nv * (8 * S + 16)), 256 / nv);
+  if (lpw < 1) return;
+  L.il_lpw = lpw;
+  L.il_stride = S;
+}
+
+// Per block, the Thomas multipliers r(a) = c(a) / (b(a) - r(a-1) a(a)) of
+// the constant coefficient epsc along an i-line (tridi, src/cosa.f90:
+// 2418-2443), the recurrence of fill_rtab in k_flux.hip evaluated on the
+// host with the same operations (correctly rounded divisions, no
+// contraction), so the values are the same; and the reciprocals 1 / d(a)
+// of the forward-pass divisors d(1) = b, d(a) = b - r(a-1) a(a), rounded
+// to nearest.  Layout: block b at 2 il_stride b, r then 1/d.
+static int upload_cirs_il_table(Solve* s, Level& L) {
+  if (L.il_rtab) (void)hipFree(L.il_rtab);
+  L.il_rtab = nullptr;
+  if (!L.il_lpw) return COSAHIP_OK;
+  const double epsc = 0.25 * (s->par.rat * s->par.rat - 1.0);
+  const size_t S = L.il_stride;
+  std::vector<double> tab((size_t)L.nblocks * 2 * S, 0.0);
+  for (int b = 0; b < L.nblocks; b++) {
+    const int n = L.hblk[b].imax - 1;
+    double rp = 0.0;
+    for (int a = 1; a <= n; a++) {
+      const double ca = a == 1 ? 0.0 : -epsc, cb = 1 + 2 * epsc, cc = a == n ? 0.0 : -epsc;
+      const double d = a == 1 ? cb : cb - rp * ca;
+      const double r = a == 1 ? cc / cb : cc / (cb - rp * ca);
+      tab[2 * S * b + a - 1] = r;
+      tab[2 * S * b + S + a - 1] = 1.0 / d;
+      rp = r;
+    }
+  }
+  CK(upload_vec(&L.il_rtab, tab));
+  return COSAHIP_OK;
+}
+
 static int build_tables(Solve* s, Level& L, const Level* Lc) {
   const int ns = s->par.nsnap;
   for (int k = 0; k < TK_NKIND; k++) L.htile[k].clear();
+  plan_cirs_il(s, L);
   for (int b = 0; b < L.nblocks; b++) {
     const Blk& B = L.hblk[b];
     const ll im = B.imax - 1, jm = B.jmax - 1, km = B.kmax - 1;
@@ -346,6 +397,7 @@ static int build_tables(Solve* s, Level& L, const Level* Lc) {
     add_tiles(L.htile[TK_LINES_I], b, ns * jm * km, 64);
     add_tiles(L.htile[TK_LINES_I_T], b, ns * jm * km, TS);
     add_tiles(L.htile[TK_LINES_IV], b, ns * jm * km, 256 / s->par.npde);
+    if (L.il_lpw) add_tiles(L.htile[TK_LINES_IL], b, ns * jm * km, L.il_lpw);
     if (s->par.viscous) {
       add_tiles(L.htile[TK_FACES_I], b, ns * (im + 1) * jm * km, TS);
       add_tiles(L.htile[TK_FACES_J], b, ns * im * (jm + 1) * km, TS);
@@ -366,7 +418,7 @@ static int build_tables(Solve* s, Level& L, const Level* Lc) {
     L.ntile[k] = (int)L.htile[k].size();
     CK(upload_vec(&L.tile[k], L.htile[k]));
   }
-  return COSAHIP_OK;
+  return upload_cirs_il_table(s, L);
 }
 
 // Host replica of k_cut's index arithmetic: every halo target and interior
@@ -1568,13 +1620,16 @@ extern "C" int cosahip_autotune(cosahip_solve* h, int reps, int32_t* choice) {
     const ll nq = L.n3tot * s->par.npde * s->par.nsnap;
     CK(hipMemcpyAsync(L.work1, L.f[COSAHIP_RES], nq * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
     std::swap(L.f[COSAHIP_RES], L.work1);
-    float t[3] = {0, 0, 0};
-    for (int mode = 1; mode <= 2 && rc == COSAHIP_OK; mode++) {
+    float t[4] = {0, 0, 0, 0};
+    const int last = L.ntile[TK_LINES_IL] ? 3 : 2;
+    for (int mode = 1; mode <= last && rc == COSAHIP_OK; mode++) {
       L.cirs_mode = mode;
       rc = timed([&]() { return launch_cirs(*s, L); }, t[mode]);
     }
     std::swap(L.f[COSAHIP_RES], L.work1);
-    L.cirs_mode = t[1] <= t[2] ? 1 : 2;
+    L.cirs_mode = 1;
+    for (int mode = 2; mode <= last; mode++)
+      if (t[mode] < t[L.cirs_mode]) L.cirs_mode = mode;
     if (choice) choice[nl - 1] = L.cirs_mode;
   }
   if (rc == COSAHIP_OK && s->par.viscous && !s->fuse_viscous && getenv("COSAHIP_VISCOUS") == nullptr &&

@@ -87,6 +87,7 @@ enum TileKind {
   TK_HALO,        // prtest region: interior + face halo strips
   TK_FUSED,       // fused residual: (31 x 7)-cell (i,j) tiles x 32-plane k segments
   TK_LINES_I_T,   // (jmax-1)(kmax-1) lines along i, TS per tile (thread per line)
+  TK_LINES_IL,    // (jmax-1)(kmax-1) lines along i, il_lpw per tile (whole lines resident in LDS)
   TK_LINES_IV,    // (jmax-1)(kmax-1) lines along i, 256/npde per tile (thread per line and variable)
   TK_SEGS_J,      // MSEG-cell segments of the (imax-1)(kmax-1) lines along j, i fastest
   TK_SEGS_K,      // MSEG-cell segments of the (imax-1)(jmax-1) lines along k, i fastest
@@ -130,8 +131,11 @@ struct Level {
   double* f[COSAHIP_NFIELDS] = {};
   double* dq = nullptr;       // HB term (qhb) and scratch
   double* work1 = nullptr;    // copy of res for the HB norm
-  double* dqc = nullptr;
-  int cirs_mode = 0;          // CIRS i-sweep picked by cosahip_autotune: 0 none, 1 lds, 2 iv      // SST: (P3, 9, nsnap) cell-centre velocity differences of the stage
+  double* dqc = nullptr;      // SST: (P3, 9, nsnap) cell-centre velocity differences of the stage
+  int cirs_mode = 0;          // CIRS i-sweep picked by cosahip_autotune: 0 none, 1 lds, 2 iv, 3 line
+  int il_lpw = 0;             // k_cirs_il: lines per work-group (0: lines too long for its LDS budget)
+  int il_stride = 0;          // k_cirs_il: odd LDS row stride >= the longest i-line of the level
+  double* il_rtab = nullptr;  // k_cirs_il: multipliers and reciprocal divisors, 2 il_stride per block
   double* cirs_r = nullptr;   // Thomas multipliers r(j), one per cell
   double* cirs_rt = nullptr;  // the same for k, omega (k-omega)
   double* rms_part = nullptr; // per-tile partial sums of squares
@@ -213,7 +217,7 @@ struct Solve {
   // CIRS i-sweep: one thread per (line, variable) on levels with fewer than
   // CIRS_IV_MAX_LINES lines along i, one thread per line with LDS-staged
   // chunks on larger ones; COSAHIP_CIRS_I=lds|iv at create time forces one
-  int cirs_i_mode = 0;   // 0 auto, 1 lds, 2 iv
+  int cirs_i_mode = 0;   // 0 auto, 1 lds, 2 iv, 3 line
   // residual precision: 0 FP64, 1 FP32 residual / FP64 state, 2 FP32 until
   // the level-1 density residual rms falls below prec_tail_rms, FP64 after
   int prec_mode = 0;

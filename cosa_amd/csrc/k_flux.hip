@@ -1204,6 +1204,99 @@ __global__ void __launch_bounds__(256) k_cirs_iv(Par p, const Blk* __restrict__ 
   }
 }
 
+// One line of k_cirs_il: the forward and backward pass of the Thomas
+// recurrence over x (already scaled by the time step) in LDS.  Each forward
+// divisor d(a) = b - r(a-1) a(a) is known before the pass, so t(a) =
+// (x(a) - t(a-1) a(a)) / d(a) is formed from its reciprocal y(a) = RN(1/d)
+// as q = RN(n y), t = RN(q + (n - q d) y) with two fused multiply-adds:
+// with y correctly rounded and q faithful, the correction gives the
+// correctly rounded quotient (Markstein's theorem; P. Markstein, IBM J. Res.
+// Dev. 34 (1990) 111-119), i.e. the same bits as n / d, with five
+// dependent operations instead of a full division.  The restrict-qualified
+// arguments let the LDS reads of the next cells and table entries run ahead
+// of the stores of the recurrence.
+__device__ __forceinline__ void il_thomas(double* __restrict__ x, const double* __restrict__ r,
+                                          const double* __restrict__ y, int L, double ca, double cb) {
+  double tp;
+  {
+    const double n = x[0], q = n * y[0];
+    tp = fma(fma(-q, cb, n), y[0], q);
+  }
+  x[0] = tp;
+  for (int a = 1; a < L; a++) {
+    const double d = cb - r[a - 1] * ca;
+    const double n = x[a] - tp * ca;
+    const double q = n * y[a];
+    tp = fma(fma(-q, d, n), y[a], q);
+    x[a] = tp;
+  }
+  double xn = x[L - 1];
+  for (int a = L - 2; a >= 0; a--) {
+    xn = x[a] - xn * r[a];
+    x[a] = xn;
+  }
+}
+
+// CIRS i-sweep with whole lines resident in LDS (constant coefficient of
+// irsop 1): a work-group takes lpw lines of one block with all NV
+// variables.  The lines are read once with coalesced loads (row by row,
+// consecutive threads on consecutive cells), already multiplied by the time
+// step (dtvol, dtvolt for k and omega); one thread per (line, variable)
+// then runs il_thomas from LDS with the block's tables (multipliers r, the
+// values of fill_rtab, and the reciprocal divisors); the lines are written
+// back once.  Global traffic per cell is one read and one write of res plus
+// the time step, against two of each for the per-line sweeps.  Row stride S
+// is odd, so the lanes of the recurrence (S doubles apart) fall in distinct
+// LDS banks.  Results are those of k_cirs_iv (tridi, src/cosa.f90:
+// 2418-2443), bit for bit.
+template <int NV>
+__global__ void __launch_bounds__(256) k_cirs_il(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
+                                                 double* __restrict__ res, const double* __restrict__ dtvol,
+                                                 const double* __restrict__ dtvolt, const double* __restrict__ tabs,
+                                                 int lpw, int S, double epsc) {
+  extern __shared__ double sm[];
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int L = b.imax - 1;
+  const int jm = b.jmax - 1, km = b.kmax - 1;
+  const int nlines = jm * km;
+  const int nl_here = min(lpw, nlines * p.nsnap - t.base);
+  const int nrow = NV * nl_here;
+  double* rt = sm;                              // r(a), a = 1..L at rt[a - 1]
+  double* yt = sm + S;                          // 1 / d(a) at yt[a - 1]
+  double* xs = sm + 2 * S;                      // row r = line * NV + variable
+  ll* base3 = reinterpret_cast<ll*>(xs + (ll)NV * lpw * S);
+  ll* base1 = base3 + NV * lpw;
+  const ll n3 = n3of(b), n1 = n1of(b);
+  for (int r = threadIdx.x; r < nrow; r += blockDim.x) {
+    const int ln = r / NV, v = r - (r / NV) * NV;
+    const int gl = t.base + ln;
+    const int n = gl / nlines, rem = gl - (gl / nlines) * nlines;
+    const int j = rem % jm + 1, k = rem / jm + 1;
+    base3[r] = b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n + n3 * v + i3(b, 0, j, k);
+    base1[r] = b.o1 * p.nsnap + n1 * (ll)n + i1(b, 0, j, k);
+  }
+  const double* tb = tabs + 2 * (ll)S * t.block;
+  for (int a = threadIdx.x; a < L; a += blockDim.x) {
+    rt[a] = tb[a];
+    yt[a] = tb[S + a];
+  }
+  __syncthreads();
+  const int tot = nrow * L;
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int r = e / L, a = e - (e / L) * L + 1;
+    const double* dt = r % NV >= 5 ? dtvolt : dtvol;
+    xs[(ll)r * S + a - 1] = res[base3[r] + a] * dt[base1[r] + a];
+  }
+  __syncthreads();
+  if (threadIdx.x < nrow) il_thomas(xs + (ll)threadIdx.x * S, rt, yt, L, -epsc, 1 + 2 * epsc);
+  __syncthreads();
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int r = e / L, a = e - (e / L) * L + 1;
+    res[base3[r] + a] = xs[(ll)r * S + a - 1];
+  }
+}
+
 // ------------------------------------------------------------------------
 // launchers
 
@@ -1447,8 +1540,18 @@ hipError_t launch_cirs(Solve& s, Level& L) {
   ll ilines = 0;
   for (const Blk& b : L.hblk) ilines += (ll)(b.jmax - 1) * (b.kmax - 1) * s.par.nsnap;
   const int mode = s.cirs_i_mode ? s.cirs_i_mode : L.cirs_mode;
-  const bool iv = mode == 2 || (mode == 0 && ilines < CIRS_IV_MAX_LINES);
-  if (iv && L.ntile[TK_LINES_IV]) {
+  const bool line = mode == 3 && L.ntile[TK_LINES_IL];
+  const bool iv = line || mode == 2 || (mode == 0 && ilines < CIRS_IV_MAX_LINES);
+  if (line) {
+    const int S = L.il_stride, lpw = L.il_lpw, nv = s.par.npde;
+    const size_t lds = sizeof(double) * (2 * (size_t)S + (size_t)nv * lpw * S) + 2 * sizeof(ll) * nv * lpw;
+    if (nv == 7)
+      hipLaunchKernelGGL(k_cirs_il<7>, grid1(L.ntile[TK_LINES_IL]), dim3(256), lds, s.stream, s.par, L.dblk,
+                         L.tile[TK_LINES_IL], res, dt, L.f[COSAHIP_DTVOLT], L.il_rtab, lpw, S, epsc);
+    else
+      hipLaunchKernelGGL(k_cirs_il<5>, grid1(L.ntile[TK_LINES_IL]), dim3(256), lds, s.stream, s.par, L.dblk,
+                         L.tile[TK_LINES_IL], res, dt, nullptr, L.il_rtab, lpw, S, epsc);
+  } else if (iv && L.ntile[TK_LINES_IV]) {
     if (s.par.npde == 7)
       hipLaunchKernelGGL(k_cirs_iv<7>, grid1(L.ntile[TK_LINES_IV]), dim3(256), 0, s.stream, s.par, L.dblk,
                          L.tile[TK_LINES_IV], res, dt, L.f[COSAHIP_BETAXI], L.f[COSAHIP_DTVOLT], L.tbeta[0], L.cirs_r,

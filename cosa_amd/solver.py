@@ -204,7 +204,7 @@ class Solve:
         and keep the fastest (cosahip_autotune); results are unchanged."""
         ch = (ctypes.c_int32 * 4)()
         self._check(self.lib.cosahip_autotune(self.h, reps, ch), "autotune")
-        names = {0: None, 1: "lds", 2: "per_variable"}
+        names = {0: None, 1: "lds", 2: "per_variable", 3: "line_resident"}
         return {"cirs_i_sweep": [names[ch[m]] for m in range(3)], "viscous_faces_stored": bool(ch[3])}
 
     # -- blocks as independent solves (serving batches) -----------------------

@@ -217,6 +217,39 @@ def test_cirs_i_variants_agree(monkeypatch):
         assert np.array_equal(res[0], res[1]), c.name
 
 
+def _irsop1_cases():
+    by_name = {c.name: c for c in CASES}
+    return [by_name["bsl_blocks"], by_name["laminar_blocks"],
+            cases.two_block_case(ni=17, nj=9, nk=9, nlevel=2, distort=0.1, name="euler_hb1_blocks2", irsop=1,
+                                 cfl_irs=3.0, cfl=2.0, nharms=1, harbal=1, omega=0.8),
+            cases.naca_case(65, 17, 9, nlevel=3, name="rans_naca3_irs1", wall=1500, irsop=1, cfl_irs=2.0, cfl=1.0,
+                            nl_crs=2, **kw_turbulence(0.5, 1e5, tu=1e-3, mutfar=0.1, **TURB))]
+
+
+@pytest.mark.parametrize("c", _irsop1_cases(), ids=lambda c: c.name)
+def test_cirs_i_line_resident_matches(c, monkeypatch):
+    """CIRS i-sweep with whole lines resident in LDS (COSAHIP_CIRS_I=line,
+    irsop 1) against one thread per (line, variable) (iv): two multigrid
+    cycles give bit-identical states on every level and identical norms,
+    mean flow and k-omega, one and three snapshots, one and three levels."""
+    from cosa_amd.solver import Solve
+    q0 = c.perturbed(amp=0.02, seed=23)
+    _say(f"case {c.name} ncell={c.ncell()} nsnap={c.nsnap} npde={c.npde} levels={c.config['nlevel']}")
+    out = []
+    for mode in ("iv", "line"):
+        monkeypatch.setenv("COSAHIP_CIRS_I", mode)
+        g = Solve(c, device=0, q=q0)
+        try:
+            for mgit in (1, 2):
+                g.cycle(mgit)
+            out.append(([g.download("q", nl) for nl in range(1, c.config["nlevel"] + 1)], list(g.history)))
+        finally:
+            g.close()
+    for nl, (qa, qb) in enumerate(zip(out[0][0], out[1][0]), 1):
+        assert np.array_equal(qa, qb), f"level {nl}: max |d| {np.abs(qa - qb).max()}"
+    assert np.array_equal(np.array(out[0][1]), np.array(out[1][1]))
+
+
 @pytest.mark.parametrize("irsop", [5, 6])
 def test_viscous_cirs_irsop(irsop):
     """cirs_v1v / cirs_v2v (irsop 5 / 6): viscous radii enter the smoothing
