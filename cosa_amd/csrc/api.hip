@@ -84,7 +84,8 @@ static void free_level(Level& L) {
     if (L.f[f]) (void)hipFree(L.f[f]);
   void* ptrs[] = {L.dblk, L.dq, L.work1, L.cirs_r, L.cirs_rt, L.rms_part, L.sbc, L.sbc_doff, L.dpatch, L.dcut,
                   L.vflux, L.tbeta[0], L.tbeta[1], L.tbeta[2], L.h0tab, L.dxd, L.q32, L.m32[0], L.m32[1], L.m32[2],
-                  L.dxd32, L.mut32, L.dwall, L.fterms, L.fsums, L.fcoef, L.fgeom, L.dqc, L.il_rtab};
+                  L.dxd32, L.mut32, L.dwall, L.fterms, L.fsums, L.fcoef, L.fgeom, L.dqc, L.il_rtab, L.cmet,
+                  L.cmet32};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (float* p : L.xd32)
@@ -681,6 +682,11 @@ extern "C" int cosahip_define_level(cosahip_solve* h, int nl, int nblocks, const
     if (hipMalloc(&L.h0tab, (ll)nblocks * p.nsnap * 6 * sizeof(double)) != hipSuccess)
       return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
   }
+  if (p.kom && p.viscous) {
+    const ll nc = L.n3tot * 9 * p.nsnap_met;
+    if (hipMalloc(&L.cmet, nc * sizeof(double)) != hipSuccess || hipMalloc(&L.cmet32, nc * sizeof(float)) != hipSuccess)
+      return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
+  }
   if (p.sst == 2) {
     const ll nd = L.n3tot * 9 * p.nsnap;
     if (hipMalloc(&L.dqc, nd * sizeof(double)) != hipSuccess)
@@ -754,8 +760,15 @@ extern "C" int cosahip_upload(cosahip_solve* h, int nl, int field, const double*
   CK(hipStreamSynchronize(s->stream));
   CK(hipMemcpy(L->f[field], host, count * sizeof(double), hipMemcpyHostToDevice));
   if (field == COSAHIP_SI || field == COSAHIP_SJ || field == COSAHIP_SK) L->m32_ok = false;
-  if (field >= COSAHIP_XIDERI && field <= COSAHIP_ZETADERK) L->xd32_ok = false;
+  if (field >= COSAHIP_XIDERI && field <= COSAHIP_ZETADERK) L->xd32_ok = L->cmet_ok = L->cmet32_ok = false;
   if (field == COSAHIP_MUT) L->mut32_ok = false;
+  // a captured cycle replays the conversions of the state it was captured
+  // in: new metrics or mut need a fresh capture
+  if (s->gexec && ((field >= COSAHIP_SI && field <= COSAHIP_SK) ||
+                   (field >= COSAHIP_XIDERI && field <= COSAHIP_ZETADERK) || field == COSAHIP_MUT)) {
+    CK(hipGraphExecDestroy(s->gexec));
+    s->gexec = nullptr;
+  }
   return COSAHIP_OK;
 }
 

@@ -164,6 +164,11 @@ struct Level {
   float* xd32[9] = {};        // metric derivatives (viscous)
   float** dxd32 = nullptr;    // device table of xd32
   bool xd32_ok = false;
+  // k-omega sources: cell-centre averages of the nine metric derivatives
+  // (xi_x .. zeta_z), (P3, 9, nsnap_met), computed once per metric upload
+  double* cmet = nullptr;
+  float* cmet32 = nullptr;
+  bool cmet_ok = false, cmet32_ok = false;
   float* mut32 = nullptr;     // eddy viscosity (k-omega)
   bool mut32_ok = false;
   // force integration: wall runs of the last (nsurface, include_inviscid)

@@ -170,6 +170,41 @@ __global__ void __launch_bounds__(256) k_vres_march(Par p, const Blk* __restrict
 }
 
 // ------------------------------------------------------------------------
+// Cell-centre averages of the metric derivatives used by the k-omega
+// sources (src/cosa.f90 srckw_b / update_ttrm_b: xix = (xideri(1,i+1) +
+// xideri(1,i)) / 2, xiy from xiderj(2, j, j+1), xiz from xiderk(3, k, k+1),
+// and the same for eta and zeta), stored (P3, 9, nsnap_met) per block and
+// metric snapshot: xix, xiy, xiz, etax, etay, etaz, zetax, zetay, zetaz.
+// The same operations as when k_turb_src formed them itself, once per
+// metric upload instead of every stage, and read as nine coalesced values
+// instead of gathers from nine triples of which one component is used.
+template <typename T>
+__global__ void __launch_bounds__(256) k_cell_metrics(Par p, const Blk* __restrict__ blks,
+                                                      const Tile* __restrict__ tiles, const T* const* __restrict__ xdp,
+                                                      T* __restrict__ cmet) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
+  const int nc = im * jm * km;
+  const int item = t.base + threadIdx.x;
+  if (item >= nc * p.nsnap) return;
+  const int n = item / nc;
+  if (n >= p.nsnap_met) return;
+  int rem = item - n * nc;
+  const int i = rem % im + 1;
+  rem /= im;
+  const int j = rem % jm + 1, k = rem / jm + 1;
+  const ll n3 = n3of(b);
+  const int nh = n;   // nsnap_met > 1 only with hbmove
+  T* cm = cmet + b.o3 * 9 * p.nsnap_met + n3 * 9 * (ll)nh + i3(b, i, j, k);
+  for (int e = 0; e < 3; e++) {      // xi, eta, zeta
+    cm[(3 * e + 0) * n3] = (mder<T>(p, b, xdp[e], i + 1, j, k, nh)[0] + mder<T>(p, b, xdp[e], i, j, k, nh)[0]) / 2;
+    cm[(3 * e + 1) * n3] = (mder<T>(p, b, xdp[3 + e], i, j + 1, k, nh)[1] + mder<T>(p, b, xdp[3 + e], i, j, k, nh)[1]) / 2;
+    cm[(3 * e + 2) * n3] = (mder<T>(p, b, xdp[6 + e], i, j, k + 1, nh)[2] + mder<T>(p, b, xdp[6 + e], i, j, k, nh)[2]) / 2;
+  }
+}
+
+// ------------------------------------------------------------------------
 // k-omega sources at cell centres: the differences of the face averages
 // (q_face + q_der_c, three directions; u, v, w, and for BSL/SST k and
 // omega), turprod (finest level), then srckw (Wilcox) or update_ttrm +
@@ -186,7 +221,7 @@ __global__ void __launch_bounds__(256) k_turb_src(Par p, int nl, const Blk* __re
                                                   const unsigned char* __restrict__ sbc,
                                                   const ll* __restrict__ sbcoff, const T* __restrict__ vpre,
                                                   double* __restrict__ ttrm, double* __restrict__ dqc,
-                                                  const double* __restrict__ dist) {
+                                                  const double* __restrict__ dist, const T* __restrict__ cmet) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
@@ -233,17 +268,11 @@ __global__ void __launch_bounds__(256) k_turb_src(Par p, int nl, const Blk* __re
         dq[d - 1][sv] = fhi - flo;
       }
     }
-    const int nh = n * p.hbmove;
-    // xdp: xideri, etaderi, zetaderi, xiderj, etaderj, zetaderj, xiderk, etaderk, zetaderk
-    const T xix = (mder<T>(p, b, xdp[0], i + 1, j, k, nh)[0] + mder<T>(p, b, xdp[0], i, j, k, nh)[0]) / 2;
-    const T xiy = (mder<T>(p, b, xdp[3], i, j + 1, k, nh)[1] + mder<T>(p, b, xdp[3], i, j, k, nh)[1]) / 2;
-    const T xiz = (mder<T>(p, b, xdp[6], i, j, k + 1, nh)[2] + mder<T>(p, b, xdp[6], i, j, k, nh)[2]) / 2;
-    const T etax = (mder<T>(p, b, xdp[1], i + 1, j, k, nh)[0] + mder<T>(p, b, xdp[1], i, j, k, nh)[0]) / 2;
-    const T etay = (mder<T>(p, b, xdp[4], i, j + 1, k, nh)[1] + mder<T>(p, b, xdp[4], i, j, k, nh)[1]) / 2;
-    const T etaz = (mder<T>(p, b, xdp[7], i, j, k + 1, nh)[2] + mder<T>(p, b, xdp[7], i, j, k, nh)[2]) / 2;
-    const T zetax = (mder<T>(p, b, xdp[2], i + 1, j, k, nh)[0] + mder<T>(p, b, xdp[2], i, j, k, nh)[0]) / 2;
-    const T zetay = (mder<T>(p, b, xdp[5], i, j + 1, k, nh)[1] + mder<T>(p, b, xdp[5], i, j, k, nh)[1]) / 2;
-    const T zetaz = (mder<T>(p, b, xdp[8], i, j, k + 1, nh)[2] + mder<T>(p, b, xdp[8], i, j, k, nh)[2]) / 2;
+    // cell-centre metric derivatives, averaged once per grid by k_cell_metrics
+    const T* cm = cmet + b.o3 * 9 * p.nsnap_met + n3 * 9 * (ll)(n * p.hbmove) + c;
+    const T xix = cm[0], xiy = cm[n3], xiz = cm[2 * n3];
+    const T etax = cm[3 * n3], etay = cm[4 * n3], etaz = cm[5 * n3];
+    const T zetax = cm[6 * n3], zetay = cm[7 * n3], zetaz = cm[8 * n3];
     if (prodlvl) {
       const T uxi = dq[0][0], ueta = dq[1][0], uzeta = dq[2][0];
       const T vxi = dq[0][1], veta = dq[1][1], vzeta = dq[2][1];
@@ -654,11 +683,18 @@ static hipError_t residual_viscous_t(Solve& s, Level& L, const ResIn<T>& in, dou
                          L.tile[TK_CELLS], vflux, res);
   }
   const bool vpre3 = p.viscous && !s.fuse_viscous && s.viscous_faces;
+  T* cmet = sizeof(T) == 4 ? reinterpret_cast<T*>(L.cmet32) : reinterpret_cast<T*>(L.cmet);
+  bool& cmet_ok = sizeof(T) == 4 ? L.cmet32_ok : L.cmet_ok;
+  if (p.kom && L.ntile[TK_CELLS] && !cmet_ok) {
+    hipLaunchKernelGGL(k_cell_metrics<T>, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, p, L.dblk, L.tile[TK_CELLS],
+                       in.dxd, cmet);
+    cmet_ok = true;
+  }
   if (p.kom && L.ntile[TK_CELLS])
     hipLaunchKernelGGL(k_turb_src<T>, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, p, L.nl, L.dblk,
                        L.tile[TK_CELLS], in.q, in.mut, L.f[COSAHIP_PRD], L.f[COSAHIP_DIVVEL], L.f[COSAHIP_DELPLS], res,
                        L.f[COSAHIP_VOL], in.dxd, L.sbc, L.sbc_doff, vpre3 ? vflux : nullptr, L.f[COSAHIP_TTRM],
-                       L.dqc, L.f[COSAHIP_DIST]);
+                       L.dqc, L.f[COSAHIP_DIST], cmet);
   return hipGetLastError();
 }
 
