@@ -1209,7 +1209,8 @@ __global__ void __launch_bounds__(256) k_cirs_iv(Par p, const Blk* __restrict__ 
 // divisor d(a) = b - r(a-1) a(a) is known before the pass, so t(a) =
 // (x(a) - t(a-1) a(a)) / d(a) is formed from its reciprocal y(a) = RN(1/d)
 // as q = RN(n y), t = RN(q + (n - q d) y) with two fused multiply-adds:
-// with y correctly rounded and q faithful, the correction gives the
+// with y correctly rounded, q faithful and no underflow (the residuals
+// here are far above the subnormal range), the correction gives the
 // correctly rounded quotient (Markstein's theorem; P. Markstein, IBM J. Res.
 // Dev. 34 (1990) 111-119), i.e. the same bits as n / d, with five
 // dependent operations instead of a full division.  The restrict-qualified
