@@ -251,6 +251,8 @@ int cosahip_create(const cosahip_config* cfg, int device, cosahip_solve** out) {
   s->use_graph = !(gk && strcmp(gk, "0") == 0);
   const char* ck = getenv("COSAHIP_CIRS_I");
   s->cirs_i_mode = !ck ? 0 : strcmp(ck, "lds") == 0 ? 1 : strcmp(ck, "iv") == 0 ? 2 : strcmp(ck, "line") == 0 ? 3 : 0;
+  const char* lk = getenv("COSAHIP_CIRS_IL_LDS");
+  if (lk) s->il_lds = std::min(65536, std::max(4096, atoi(lk)));
   const char* vk = getenv("COSAHIP_VISCOUS");
   s->fuse_viscous = vk && strcmp(vk, "fused") == 0;
   s->viscous_faces = !(vk && (strcmp(vk, "march") == 0 || strcmp(vk, "fused") == 0));
@@ -333,9 +335,8 @@ static void add_tiles(std::vector<Tile>& v, int block, ll items, int per) {
 
 // k_cirs_il (CIRS i-sweep, constant coefficient of irsop 1): how many whole
 // lines of all npde variables one work-group keeps in LDS, within
-// IL_LDS_BUDGET bytes (two tables, lines, per-row offsets), so that four
-// work-groups share a CU.
-constexpr int IL_LDS_BUDGET = 40960;
+// s->il_lds bytes (two tables, lines, per-row offsets; default 40 KB, so that
+// four work-groups share a CU; at most 64 KB).
 
 static void plan_cirs_il(Solve* s, Level& L) {
   L.il_lpw = 0;
@@ -345,7 +346,7 @@ static void plan_cirs_il(Solve* s, Level& L) {
   for (const Blk& B : L.hblk) lmax = std::max(lmax, B.imax - 1);
   const int S = lmax | 1;
   const int nv = s->par.npde;
-  const int lpw = std::min((IL_LDS_BUDGET - 16 * S) / (nv * (8 * S + 16)), 256 / nv);
+  const int lpw = std::min((s->il_lds - 16 * S) / (nv * (8 * S + 16)), 256 / nv);
   if (lpw < 1) return;
   L.il_lpw = lpw;
   L.il_stride = S;

@@ -223,6 +223,7 @@ struct Solve {
   // CIRS_IV_MAX_LINES lines along i, one thread per line with LDS-staged
   // chunks on larger ones; COSAHIP_CIRS_I=lds|iv at create time forces one
   int cirs_i_mode = 0;   // 0 auto, 1 lds, 2 iv, 3 line
+  int il_lds = 40960;    // LDS bytes per work-group of the line-resident CIRS i-sweep
   // residual precision: 0 FP64, 1 FP32 residual / FP64 state, 2 FP32 until
   // the level-1 density residual rms falls below prec_tail_rms, FP64 after
   int prec_mode = 0;
