@@ -4,6 +4,7 @@
 // smooth() (src/cosa.f90:1594-2032), restrict() (2672-2754) and prolong()
 // (3167-3249).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -505,6 +506,7 @@ extern "C" int cosahip_define_level(cosahip_solve* h, int nl, int nblocks, const
                                     const int32_t* cutopo) {
   Solve* s = reinterpret_cast<Solve*>(h);
   if (!s) return COSAHIP_ERR_ARG;
+  if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "define_level: an enqueued cycle has not been waited for");
   if (nl < 1 || nl > (int)s->lev.size()) return fail(s, COSAHIP_ERR_ARG, "level out of range");
   if (nblocks < 1 || !imax || !jmax || !kmax || !nbcs) return fail(s, COSAHIP_ERR_ARG, "bad block arrays");
   CK(hipSetDevice(s->device));
@@ -527,6 +529,12 @@ extern "C" int cosahip_define_level(cosahip_solve* h, int nl, int nblocks, const
     CK(hipMemcpy(s->dqfst, s->bqfst.data(), 5 * (size_t)nblocks * sizeof(double), hipMemcpyHostToDevice));
   }
   Level& L = s->lev[nl - 1];
+  // a captured cycle graph points at the buffers about to be freed
+  if (s->gexec) {
+    CK(hipStreamSynchronize(s->stream));
+    CK(hipGraphExecDestroy(s->gexec));
+    s->gexec = nullptr;
+  }
   free_level(L);
   L.nl = nl;
   L.nblocks = nblocks;
@@ -886,8 +894,13 @@ extern "C" int cosahip_set_block_freestream(cosahip_solve* h, int block, double 
   const int nb = (int)s->bqfst.size() / 5;
   if (block < 1 || block > nb) return fail(s, COSAHIP_ERR_ARG, "set_block_freestream: block out of range");
   if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "set_block_freestream: a cycle is in flight");
-  CK(hipSetDevice(s->device));
   const cosahip_config& c = s->cfg;
+  // viscous terms, the wall omega, mut and the viscous forces scale with the
+  // configuration's Mach/Reynolds ratio (one value for the whole solve)
+  if (s->par.viscous && machfs != c.machfs)
+    return fail(s, COSAHIP_ERR_UNSUPPORTED,
+                "set_block_freestream: a viscous solve keeps one Mach number (viscous scaling M/Re is per solve)");
+  CK(hipSetDevice(s->device));
   double* f = s->bqfst.data() + 5 * (block - 1);
   // as the configuration's freestream (bc_far0, src/bc.f90:145-152)
   f[0] = c.rhoinf;
@@ -1583,18 +1596,38 @@ extern "C" int cosahip_cycle_enqueue(cosahip_solve* h, int mgit) {
       CK(hipGraphExecDestroy(s->gexec));
       s->gexec = nullptr;
     }
+    // the lazy conversions (FP32 shadows, cell-centre metric averages) mark
+    // themselves done when their launch is recorded; if the capture fails
+    // they never ran, so their marks are restored
+    std::vector<std::array<bool, 5>> marks;
+    for (const Level& L : s->lev) marks.push_back({L.m32_ok, L.xd32_ok, L.cmet_ok, L.cmet32_ok, L.mut32_ok});
+    auto unmark = [&]() {
+      for (size_t m = 0; m < s->lev.size(); m++) {
+        Level& L = s->lev[m];
+        L.m32_ok = marks[m][0];
+        L.xd32_ok = marks[m][1];
+        L.cmet_ok = marks[m][2];
+        L.cmet32_ok = marks[m][3];
+        L.mut32_ok = marks[m][4];
+      }
+    };
     hipGraph_t g = nullptr;
     CK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
     const int rc = cycle_body(s, mgit);
     const hipError_t e = hipStreamEndCapture(s->stream, &g);
-    if (rc) {
+    if (rc || e != hipSuccess) {
+      unmark();
       if (e == hipSuccess && g) (void)hipGraphDestroy(g);
-      return rc;
+      if (rc) return rc;
+      CK(e);
     }
-    CK(e);
     const hipError_t ei = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    CK(ei);
+    if (ei != hipSuccess) {
+      s->gexec = nullptr;
+      unmark();
+      CK(ei);
+    }
     s->gkey = key;
     CK(hipGraphLaunch(s->gexec, s->stream));
   } else {

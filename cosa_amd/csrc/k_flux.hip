@@ -1213,7 +1213,10 @@ __global__ void __launch_bounds__(256) k_cirs_iv(Par p, const Blk* __restrict__ 
 // here are far above the subnormal range), the correction gives the
 // correctly rounded quotient (Markstein's theorem; P. Markstein, IBM J. Res.
 // Dev. 34 (1990) 111-119), i.e. the same bits as n / d, with five
-// dependent operations instead of a full division.  The restrict-qualified
+// dependent operations instead of a full division.  An exact zero n is the
+// one case where the correction changes the sign (-0 + +0 rounds to +0):
+// there q = n y already is the signed zero n / d, so it is taken as is.
+// The restrict-qualified
 // arguments let the LDS reads of the next cells and table entries run ahead
 // of the stores of the recurrence.
 __device__ __forceinline__ void il_thomas(double* __restrict__ x, const double* __restrict__ r,
@@ -1221,14 +1224,14 @@ __device__ __forceinline__ void il_thomas(double* __restrict__ x, const double* 
   double tp;
   {
     const double n = x[0], q = n * y[0];
-    tp = fma(fma(-q, cb, n), y[0], q);
+    tp = n == 0.0 ? q : fma(fma(-q, cb, n), y[0], q);
   }
   x[0] = tp;
   for (int a = 1; a < L; a++) {
     const double d = cb - r[a - 1] * ca;
     const double n = x[a] - tp * ca;
     const double q = n * y[a];
-    tp = fma(fma(-q, d, n), y[a], q);
+    tp = n == 0.0 ? q : fma(fma(-q, d, n), y[a], q);
     x[a] = tp;
   }
   double xn = x[L - 1];

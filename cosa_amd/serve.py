@@ -312,12 +312,21 @@ class BatchServer:
 
 
 def summarize(results: List[Result], wall: float, cell_updates: float) -> dict:
-    lat = np.array([r.latency for r in results])
-    return {"requests": len(results), "converged": int(sum(r.converged for r in results)),
-            "wall_s": round(wall, 3), "solves_per_s": round(len(results) / wall, 3),
-            "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
-            "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
-            "mcell_updates_per_s": round(cell_updates / wall / 1e6, 2)}
+    """Converged solves per second and p50 / p99 time-to-converged count only
+    the requests that reached their drop; the ones retired at max_cycles are
+    reported separately (count and their own retirement latencies)."""
+    conv = [r for r in results if r.converged]
+    unconv = [r for r in results if not r.converged]
+    lat = np.array([r.latency for r in conv])
+    out = {"requests": len(results), "converged": len(conv), "unconverged_retired": len(unconv),
+           "wall_s": round(wall, 3), "solves_per_s": round(len(conv) / wall, 3),
+           "latency_p50_s": round(float(np.percentile(lat, 50)), 4) if conv else None,
+           "latency_p99_s": round(float(np.percentile(lat, 99)), 4) if conv else None,
+           "mcell_updates_per_s": round(cell_updates / wall / 1e6, 2)}
+    if unconv:
+        ul = np.array([r.latency for r in unconv])
+        out["unconverged_latency_p50_s"] = round(float(np.percentile(ul, 50)), 4)
+    return out
 
 
 def request_mix(n: int, seed: int = 0, rate: float = 4.0, precision: int = 0, drop: float = 1e-2,

@@ -58,7 +58,7 @@ typedef struct cosahip_config {
   int32_t nl_crs;      /* first-order MUSCL on levels nl >= nl_crs         */
   int32_t nstage;      /* RK stages (4)                                    */
   int32_t irsop;       /* 0 no_irs, 1 cirs_c, 2 cirs_v1, 3 cirs_v2         */
-  int32_t cutcirs;     /* 0 only (ccirs not built)                         */
+  int32_t cutcirs;     /* 0 cirs, 1 ccirs: cut-coupled CIRS (src/cosa.f90:2035-2415, 5238-5773) */
   int32_t ilimf;       /* flow limiter 0..4                                */
   int32_t ilimt;       /* turbulence limiter 0..4                          */
   int32_t iupdt;       /* local time step refresh period                   */
@@ -232,7 +232,9 @@ int cosahip_resg_rms(cosahip_solve* s, int nl, double* resrms);
 int cosahip_add_array(cosahip_solve* s, int nl, int dst, int src, double fact);
 /* copy_array interior (src/util.f90:2-135): dst = src.                   */
 int cosahip_copy_array(cosahip_solve* s, int nl, int dst, int src);
-/* cirs (src/cosa.f90:5206-5623), cutcirs = 0.                             */
+/* cirs (src/cosa.f90:5206-5623) for cutcirs = 0; for cutcirs = 1 ccirs
+ * (src/cosa.f90:2035-2415) with scalef/scaleb (5626-5773): two passes with
+ * the cut-exchanged halo residuals as Dirichlet data.                     */
 int cosahip_cirs(cosahip_solve* s, int nl);
 /* update (src/update.f90:2-100, 568-762, 1301-1408), explicit RK; with
  * k-omega the point-implicit rho*omega, rho*k update first.               */
@@ -290,9 +292,14 @@ int cosahip_cycle_ready(cosahip_solve* s);
  * for viscous runs the viscous face fluxes (stored then differenced, or
  * flux and difference in one line march).  The arrangements give
  * bit-identical results; the state is not changed (timing runs on a copy of
- * the residual).  choice[0..2]: CIRS i-sweep of levels 1..3 (1 LDS, 2 per
- * variable, 0 level absent), choice[3]: 1 stored face fluxes, 0 march.
- * Environment switches (COSAHIP_CIRS_I, COSAHIP_VISCOUS) take precedence. */
+ * the residual).  choice[0..2]: CIRS i-sweep of levels 1..3 (1 one thread
+ * per line with LDS-staged chunks, 2 one thread per line and variable,
+ * 3 whole lines resident in LDS -- irsop 1 only --, 0 level absent),
+ * choice[3]: 1 stored face fluxes, 0 march.
+ * Environment switches at create time take precedence:
+ * COSAHIP_CIRS_I=lds|iv|line forces the i-sweep, COSAHIP_CIRS_IL_LDS sets
+ * the LDS bytes per work-group of the line-resident sweep (4096..65536,
+ * default 40960), COSAHIP_VISCOUS=march|fused the viscous arrangement.    */
 int cosahip_autotune(cosahip_solve* s, int reps, int32_t* choice);
 
 /* ---- blocks as independent solves (no counterpart in the reference) -----

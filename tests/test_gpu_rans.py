@@ -245,9 +245,10 @@ def test_cirs_i_line_resident_matches(c, monkeypatch):
             out.append(([g.download("q", nl) for nl in range(1, c.config["nlevel"] + 1)], list(g.history)))
         finally:
             g.close()
+    # bitwise: signed zeros must agree too
     for nl, (qa, qb) in enumerate(zip(out[0][0], out[1][0]), 1):
-        assert np.array_equal(qa, qb), f"level {nl}: max |d| {np.abs(qa - qb).max()}"
-    assert np.array_equal(np.array(out[0][1]), np.array(out[1][1]))
+        assert np.array_equal(qa.view(np.uint64), qb.view(np.uint64)), f"level {nl}: max |d| {np.abs(qa - qb).max()}"
+    assert np.array_equal(np.array(out[0][1]).view(np.uint64), np.array(out[1][1]).view(np.uint64))
 
 
 @pytest.mark.parametrize("irsop", [5, 6])
