@@ -6,9 +6,10 @@ driver oracle/ref_shim.f90.  TEST INFRASTRUCTURE ONLY.
 Recipe (the reference's own Makefile is not used):
   * cosa_precision, common_variables, cosa_variables, util, flux, src and
     update .f90 compile unmodified with amdflang;
-  * bc.f90, cosa.f90, output.f90 and input.f90 do not compile as whole
-    files in this image: bc.f90's periodic-cut routines need the
-    parallelutils module, cosa.f90's main program and the readers and
+  * bc.f90, cosa.f90, output.f90, input.f90 and parallelutils.f90 do not
+    compile as whole files in this image: bc.f90's periodic-cut routines and
+    cutman_q need the parallelutils module (MPI plumbing), cosa.f90's main
+    program and the readers and
     writers of input.f90 / output.f90 need the un-vendored CGNS library.  The subroutines this path
     needs are self-contained; they are compiled verbatim, selected by name
     into a temporary file under oracle/_ref/ that is deleted right after
@@ -33,7 +34,10 @@ OUT = os.path.join(HERE, "_ref")
 WHOLE = ["cosa_precision", "common_variables", "cosa_variables", "util", "flux", "src", "update"]
 BY_NAME = {
     "bc.f90": ["bbc", "bc_far0", "bc_far1", "bc_insub", "bc_outsub", "bc_walinv_14", "bc_walvis_15",
-               "bc_extrapol", "bc_sym"],
+               "bc_extrapol", "bc_sym", "cut_q"],
+    # the block-ownership queries cut_q calls (external subroutines after the
+    # parallelutils module in the same file; they need only cosa_variables)
+    "parallelutils.f90": ["myblock", "getownerid"],
     "cosa.f90": ["bresid", "lambdab", "eig_eul", "eig_ns", "eig_eul_p", "eig_ns_p", "bdeltat", "cirs_b",
                  "tridi", "ccirs_b", "scalef_b", "scaleb_b", "resg_brms", "resg_brms_hb", "rest1_b", "rest2_ho_b", "rest2_lo_b", "prol_lin_b",
                  "prol_bln_b", "rest4_b"],

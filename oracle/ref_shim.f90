@@ -227,6 +227,26 @@ contains
     end if
   end subroutine
 
+  ! cut_q (src/bc.f90:1894-2141), the branch where both blocks belong to
+  ! this process (lowernblock..uppernblock cover them): q1's two halo
+  ! layers from q2's two interior layers; the message arrays of the other
+  ! branches are sized but unused
+  subroutine ref_cut_q(imax1, jmax1, kmax1, q1, imax2, jmax2, kmax2, q2, cutopo) bind(C, name='ref_cut_q')
+    integer(c_int32_t), value :: imax1, jmax1, kmax1, imax2, jmax2, kmax2
+    real(c_double) :: q1(*), q2(*)
+    integer(c_int32_t) :: cutopo(21)
+    integer(kind=cosa_int) :: rreq(1), sreq(1), nrecv(1), nsend(1), myr(0:0), mys(0:0)
+    integer(kind=cosa_int), allocatable :: rind(:,:,:)
+    real(kind=cosa_real), allocatable :: sarr(:,:,:), rarr(:,:,:)
+    allocate(rind(1,8,1), sarr(npde*(2*nharms+1),1,1), rarr(npde*(2*nharms+1),1,1))
+    lowernblock = 1
+    uppernblock = max(cutopo(1), cutopo(10))
+    nrecv = 0; nsend = 0; myr = 1; mys = 1
+    call cut_q(imax1, jmax1, kmax1, q1, imax2, jmax2, kmax2, q2, npde, nharms, cutopo, rarr, rreq, nrecv, &
+         rind, sarr, sreq, nsend, 1, 1, 1, 1, myr, mys)
+    deallocate(rind, sarr, rarr)
+  end subroutine
+
   subroutine ref_tridi(a, b, c, x, dimx, n) bind(C, name='ref_tridi')
     real(c_double) :: a(*), b(*), c(*), x(*)
     integer(c_int), value :: dimx, n

@@ -100,6 +100,20 @@ def cases():
                      zetaderk=fr.uniform(-3, 3, n0 * 3 * ns), bt=bctopo_of(fp),
                      lngth=np.array([1.3, 0.9, 0.7]), xyzmom=np.array([0.25, 0.1, -0.05]),
                      pitch=np.array([0.3, 0.05, 0.05, -0.03, 0.02]))))
+    # cut_q: a 1:1 cut in several orientations (index map and range
+    # directions on both sides), two snapshots
+    sys.path.insert(0, os.path.dirname(HERE))
+    from test_oracle_cut import cut_record
+    cr = np.random.default_rng(55)
+    d1, d2 = (6, 7, 5), (8, 5, 6)
+    for k, (i1, s1, i2, s2, sw, g1, g2) in enumerate([(1, 1, 1, 0, 0, (1, 1), (1, 1)), (2, 0, 3, 1, 1, (1, -1), (-1, 1)),
+                                                       (3, 1, 2, 1, 0, (-1, -1), (1, -1)), (1, 0, 2, 0, 1, (1, 1), (-1, -1)),
+                                                       (2, 1, 1, 1, 1, (-1, 1), (1, 1)), (3, 0, 3, 0, 0, (1, -1), (1, -1))]):
+        rec = cut_record(cr, d1, d2, i1, s1, i2, s2, sw, g1, g2)
+        # every value distinct, so any wrong source or target shows
+        out.append((f"cut_q_{k}", cfg(nharms=1), dict(rec=rec, d1=np.array(d1, np.int32), d2=np.array(d2, np.int32),
+                                                      q1=-1.0 - np.arange(n3(*d1) * 15.0),
+                                                      q2=0.5 + np.arange(n3(*d2) * 15.0))))
     return out
 
 
@@ -211,6 +225,11 @@ def run(lib, kind, name, c, a, oracle):
             lib.ref_force_b(*arrs, nbcs, ip(a["bt"]), 2, dp(a["lngth"]), dp(a["xyzmom"]), 1, ctypes.c_double(xh),
                             ctypes.c_double(yh), dp(np.ascontiguousarray(dth)), dp(coef), *d)
         return dict(coef=coef)
+    if name.startswith("cut_q"):
+        q1 = a["q1"].copy()
+        d1, d2 = (int(v) for v in a["d1"]), (int(v) for v in a["d2"])
+        (lib.or_cut_q if oracle else lib.ref_cut_q)(*d1, dp(q1), *d2, dp(a["q2"]), ip(a["rec"]))
+        return dict(q1=q1)
     if name == "edges_corners":
         q = a["q"].copy()
         if oracle:

@@ -1,0 +1,181 @@
+"""Pin the oracle's orchestration of a pseudo-time iteration (or_smooth,
+src/cosa.f90:1594-2032) and of the multigrid cycle (or_mg,
+src/cosa.f90:806-930; FAS forcing src/cosa.f90:1951-1960, restrict and
+prolong 2672-3249) by what the mathematics fixes, since no reference run
+covers them:
+
+* one smooth iteration is the Runge-Kutta scheme the routines define:
+  stage k evaluates the residual R at the latest stage state, smooths it
+  (CIRS), and updates from the iteration's start state,
+  q(k) = q(0) - alpha_k dtvol S(R(q(k-1))), with dtvol frozen at stage 1;
+  the residual, smoothing and time step come from the per-routine oracle
+  functions that are pinned bit for bit against COSA's compiled routines
+  (test_oracle_vs_ref.py) and the update is formed here;
+* the converged state is the zero of the fine-grid residual, so it does not
+  depend on the number of multigrid levels, the CFL number or the number of
+  RK stages; a wrong FAS sign, a forcing left out or a correction taken
+  against the wrong state changes it (or stops convergence);
+* at that state one V-cycle changes nothing, every smoothing option
+  (irsop 1, 2) keeps it, and the coarse-grid correction is zero.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from cosa_amd import case as cases
+from oracle_bind import Oracle, OrCase, dp
+
+ALPHA5 = (0.25, 0.166666, 0.375, 0.5, 1.0)   # ar5 (src/input.f90:2409)
+
+
+def channel(nlevel=1, ncycle=100, dims=(17, 9, 9), **kw):
+    cfg = dict(irsop=0, cfl=1.0, nl_crs=2, ilimf=0, ncycle=ncycle, toler=0.0)
+    cfg.update(kw)
+    return cases.channel_case([dims], nlevel=nlevel, name="channel", machfs=0.5, **cfg)
+
+
+def interior(c, nl=1):
+    """(P3 interior mask over npde x block, P1 index of each P3 interior point)."""
+    L = c.levels[nl - 1]
+    im, jm, km = int(L.imax[0]), int(L.jmax[0]), int(L.kmax[0])
+    m3 = np.zeros((c.npde, km + 3, jm + 3, im + 3), bool)
+    m3[:, 2:km + 1, 2:jm + 1, 2:im + 1] = True
+    p1 = np.arange((km + 1) * (jm + 1) * (im + 1)).reshape(km + 1, jm + 1, im + 1)[1:km, 1:jm, 1:im]
+    return m3.reshape(-1), np.tile(p1.reshape(-1), c.npde)
+
+
+def _where():
+    return (ctypes.c_int32 * 8)(), (ctypes.c_int32 * 1)()
+
+
+@pytest.mark.parametrize("nstage,irsop", [(1, 0), (4, 0), (4, 1), (4, 2), (5, 1)])
+def test_smooth_is_the_rk_scheme_of_its_routines(nstage, irsop):
+    kw = dict(nstage=nstage, irsop=irsop, cfl=1.2 if irsop == 0 else 2.0)
+    if irsop:
+        kw.update(cfl_irs=3.5, psirs=0.125)
+    if nstage == 5:
+        kw["alphas"] = ALPHA5
+    c = channel(**kw)
+    c.config["ilimf"] = 4
+    c.config["epslimf"] = 1e-2
+    q0 = c.perturbed(amp=0.02, seed=3)
+    o = Oracle()
+    o.set_config(c.config)
+    # the oracle's smooth: one iteration
+    B = OrCase(c, q1=q0.copy())
+    where, done = _where()
+    assert o.or_smooth(B.ptr(), 1, 0, 0, 1, 1, done, where) == 0
+    # the RK scheme built from the pinned routines
+    A = OrCase(c, q1=q0.copy())
+    a = A[1]
+    cp = A.ptr()
+    m3, p1 = interior(c)
+    alphas = c.config["alphas"]
+    for k in range(nstage):
+        o.or_cons2prim(cp, 1, dp(a["q"]))
+        o.or_bc(cp, 1, dp(a["q"]))
+        o.or_cutman_q(cp, 1, dp(a["q"]))
+        if k == 0:
+            o.or_lambda_deltat(cp, 1)
+            dtvol = a["dtvol"].copy()
+        o.or_residual(cp, 1)
+        o.or_prim2cons(cp, 1, dp(a["q"]))
+        if irsop:
+            o.or_cirs(cp, 1)
+        a["q"][m3] = q0[m3] - alphas[k] * a["res"][m3] * dtvol[p1]
+    d = np.abs(B[1]["q"][m3] - a["q"][m3])
+    scale = np.abs(a["q"][m3]).max()
+    print(f"nstage {nstage} irsop {irsop}: max |smooth - RK| = {d.max():.3e} (state scale {scale:.2f}), "
+          f"{int(np.count_nonzero(d))} of {d.size} values differ")
+    assert d.max() <= 1e-14 * scale
+    # the iteration moved the state (the check is not vacuous)
+    assert np.abs(a["q"][m3] - q0[m3]).max() > 1e-4
+
+
+def _converge(c, q0, max_cycles):
+    o = Oracle()
+    o.set_config(c.config)
+    oc = OrCase(c, q1=q0.copy())
+    c.config["ncycle"] = max_cycles
+    o.set_config(c.config)
+    hist = np.zeros(8 * max_cycles)
+    where = (ctypes.c_int32 * 8)()
+    n = o.or_mg(oc.ptr(), 1, 1, 1, 1, dp(hist), where)
+    assert n > 0, f"or_mg failed ({n})"
+    h = hist.reshape(-1, 8)[:n, 0]
+    return oc, h
+
+
+@pytest.fixture(scope="module")
+def fixed_point():
+    """The channel converged with three levels (second order on the fine
+    level, first order below, unlimited MUSCL, no IRS, CFL 1)."""
+    c = channel(nlevel=3, toler=3e-14)
+    q0 = c.perturbed(amp=0.01, seed=1)
+    oc, h = _converge(c, q0, 2500)
+    assert h[-1] < 3e-14, h[-1]
+    return c, q0, oc[1]["q"].copy(), h
+
+
+@pytest.mark.parametrize("variant", ["2 levels", "3 levels cfl 0.8", "2 levels 5 stages",
+                                     "3 levels linear prolongation, 64-weight restriction"])
+def test_converged_state_independent_of_multigrid_cfl_stages(fixed_point, variant):
+    c3, q0, qstar, h3 = fixed_point
+    kw = dict(toler=3e-14)
+    if variant == "2 levels":
+        c = channel(nlevel=2, **kw)
+    elif variant == "3 levels cfl 0.8":
+        c = channel(nlevel=3, cfl=0.8, **kw)
+    elif variant == "2 levels 5 stages":
+        c = channel(nlevel=2, nstage=5, alphas=ALPHA5, **kw)
+    else:
+        c = channel(nlevel=3, bln_prol=0, ho_rest=1, **kw)
+    oc, h = _converge(c, q0, 3000)
+    assert h[-1] < 3e-14, (variant, h[-1], len(h))
+    m3, _ = interior(c)
+    d = np.abs(oc[1]["q"][m3] - qstar[m3]).max() / np.abs(qstar[m3]).max()
+    moved = np.abs(q0[m3] - qstar[m3]).max() / np.abs(qstar[m3]).max()
+    print(f"{variant}: {len(h)} cycles (3 levels: {len(h3)}), max |q - q*| / max|q*| = {d:.2e} "
+          f"(start {moved:.2e})")
+    assert d < 1e-10
+    assert moved > 1e-3
+
+
+@pytest.mark.parametrize("irsop", [0, 1, 2])
+def test_fixed_point_kept_and_no_coarse_correction(fixed_point, irsop):
+    """From q*, V-cycles with any smoothing leave the fine state where it
+    is; after restriction and the coarse smoothing the coarse state equals
+    the restricted fine state (zero correction)."""
+    c3, q0, qstar, _ = fixed_point
+    kw = dict(toler=0.0, irsop=irsop)
+    if irsop:
+        kw.update(cfl=2.0, cfl_irs=3.5, psirs=0.125)
+    c = channel(nlevel=3, **kw)
+    o = Oracle()
+    o.set_config(c.config)
+    oc = OrCase(c, q1=qstar.copy())
+    cp = oc.ptr()
+    where, done = _where()
+    m3, _ = interior(c)
+    m3c, _ = interior(c, 2)
+    scale = np.abs(qstar[m3]).max()
+    for mgit in range(1, 6):
+        assert o.or_smooth(cp, 1, 0, 1, 1, mgit, done, where) == 0
+        o.or_restrict(cp, 1)
+        q2r = oc[2]["q"].copy()
+        assert o.or_smooth(cp, 2, 1, 1, 1, mgit, done, where) == 0
+        dc = np.abs(oc[2]["q"][m3c] - q2r[m3c]).max() / scale
+        assert dc < 1e-12, (mgit, dc)
+        o.or_restrict(cp, 2)
+        assert o.or_smooth(cp, 3, 1, 0, 1, mgit, done, where) == 0
+        o.or_prolong(cp, 2)
+        assert o.or_smooth(cp, 2, 0, 0, 1, mgit, done, where) == 0
+        o.or_prolong(cp, 1)
+    d = np.abs(oc[1]["q"][m3] - qstar[m3]).max() / scale
+    r = o.or_get_resrms(1)
+    print(f"irsop {irsop}: 5 V-cycles from q*: max |q - q*| / max|q*| = {d:.2e}, resrms(1) = {r:.2e}")
+    # q* holds the fixed point to its convergence level (resrms 3e-14): the
+    # cycles may only move it within the tolerance of the variant tests
+    assert d < 1e-10
+    assert r < 1e-12
