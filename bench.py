@@ -1,27 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the COSA pseudo-time residual/update path on one MI355X.
 
-Workload (default, --workload wing4): steady RANS with the Wilcox k-omega
-model around a rectangular wing with a NACA 0012 section and aspect ratio 4
--- 4 blocks of 257 x 65 x 65 nodes (4,194,304 cells), O-H topology, wall
-spacing 5e-6 chord -- at M = 0.3, Re = 3e6, alpha = 4 deg: point-implicit
-4-stage RK with central implicit residual smoothing (cirs_c), van Albada
-MUSCL, 3-level multigrid V-cycles, FP64, freestream plus a 1% uniform
-perturbation (default_rng(1)).
-
---workload uneven96: 96 independent bump-channel blocks whose node counts
-per direction are drawn from {17, 33, 65, 129} with numpy.random
-.default_rng(0) (24,285,184 cells), Euler at M = 0.7, same scheme.
+Headline workload (default, --workload uneven96): 96 independent
+bump-channel blocks whose node counts per direction are drawn from
+{17, 33, 65, 129} with numpy.random.default_rng(0) (24,285,184 cells), Euler
+at M = 0.7, freestream plus a 1% uniform perturbation (default_rng(1)),
+4-stage RK, cirs_c, van Albada MUSCL, 3-level multigrid V-cycles, FP64 --
+the largest configuration that fits one GPU.  The same JSON line carries the
+RANS wing (4 blocks of 257 x 65 x 65 nodes, Wilcox k-omega, M 0.3, Re 3e6,
+alpha 4 deg, 3 levels) under "extra_workloads" (--no-extras leaves it out).
 
 One step = one multigrid cycle of mg() (src/cosa.f90:822-912) over all
-blocks.  The metric is cell-updates per second: one RK stage applied to one
-interior cell, every level counted.  Inputs are resident in HBM before the
-timed region; the timed region is bracketed by device synchronisation.
+blocks, enqueued as one launch sequence and, from the second cycle on,
+replayed as one HIP graph (cosahip_cycle_enqueue / cosahip_cycle_wait).  The
+metric is cell-updates per second: one RK stage applied to one interior cell,
+every level counted.  Inputs are resident in HBM before the timed region;
+the timed region is bracketed by device synchronisation.  A second timed
+region runs the same cycles synchronously with HIP events around every
+kernel family on the library's stream ("sync_path", "families") for the
+roofline of the dominant kernel.
 
-Prints ONE JSON line with the driver's contract plus ``roofline`` (dominant
-kernel family, achieved algorithmic GB/s from HIP events recorded on the
-library's stream over the timed region) and ``cpu_baseline`` (the CPU
-restatement timed on this host's cores on a bounded sample).
+cpu_baseline: COSA's own compiled routines (oracle/_ref/libcosaref_o3.so,
+amdflang -O3, no fast-math) driven in smooth()/mg() order by
+oracle/ref_mg.py, one block per process as the reference's MPI ranks own
+whole blocks, on this host's cores, for a bounded time.
 """
 from __future__ import annotations
 
@@ -92,21 +94,20 @@ def _wing_cfg():
                                 prdlim=20.0))
 
 
-WING_CFG = None
+def _uneven_cfg():
+    return dict(machfs=0.7, irsop=1, cfl=2.0, cfl_irs=4.0, ilimf=4, epslimf=1e-6, nl_crs=2, ncycle=100000)
 
 
 def make_case(workload: str):
     from cosa_amd import case as cases
-    global WING_CFG
-    WING_CFG = _wing_cfg()
     if workload == "uneven96":
         dims = cases.uneven_dims(96, seed=0)
-        c = cases.channel_case(dims, nlevel=3, name="uneven96", machfs=0.7, irsop=1, cfl=2.0, cfl_irs=4.0,
-                               ilimf=4, epslimf=1e-6, nl_crs=2, ncycle=100000)
+        c = cases.channel_case(dims, nlevel=3, name="uneven96", **_uneven_cfg())
         return c, dict(workload="uneven96", blocks=96, dims_seed=0, cells=c.ncell(), levels=3, physics="euler",
-                       mach=0.7, rk_stages=4, irs="cirs_c", limiter="van_albada", precision="fp64")
+                       mach=0.7, rk_stages=4, irs="cirs_c", limiter="van_albada", precision="fp64",
+                       grid="bump channels, node counts per direction from {17,33,65,129}")
     if workload == "wing4":
-        c = cases.wing_case(257, 65, 65, nblocks=4, nlevel=3, name="wing4", **WING_CFG)
+        c = cases.wing_case(257, 65, 65, nblocks=4, nlevel=3, name="wing4", **_wing_cfg())
         return c, dict(workload="wing4_rans_komega", blocks=4, block_nodes=[257, 65, 65], cells=c.ncell(), levels=3,
                        physics="rans_wilcox_komega", mach=0.3, reynolds=3.0e6, alpha_deg=4.0, rk_stages=4,
                        update="point_implicit_komega", irs="cirs_c", precision="fp64",
@@ -119,82 +120,60 @@ def make_case(workload: str):
     raise SystemExit(f"unknown workload {workload}")
 
 
-def _cpu_sample_case(workload: str):
-    """The CPU sample of a workload: the wing at 1/64 of the cells, or the
-    first uneven channel blocks up to 1.5 M cells."""
+# ---------------------------------------------------------------------------
+# CPU baseline: the reference's compiled routines
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _ref_block_run(dims, budget_s, lib):
+    """One process = one reference rank owning one block: whole multigrid
+    cycles of the block for about budget_s; returns (cell-updates, s, cycles)."""
     from cosa_amd import case as cases
-    if workload == "wing4":
-        c = cases.wing_case(65, 17, 17, nblocks=4, nlevel=3, name="sample", **_wing_cfg())
-        return c, "the wing at 1/64 of the cells: 4 blocks of 65x17x17 nodes"
-    if workload == "uneven96":
-        dims = cases.uneven_dims(96, seed=0)
-        nb, cells = 0, 0
-        for d in dims:
-            nb += 1
-            cells += (d[0] - 1) * (d[1] - 1) * (d[2] - 1)
-            if cells >= 1_500_000:
-                break
-        c = cases.channel_case(dims[:nb], nlevel=3, name="sample", machfs=0.7, irsop=1, cfl=2.0, cfl_irs=4.0,
-                               ilimf=4, epslimf=1e-6, nl_crs=2, ncycle=100000)
-        return c, f"the first {nb} of the 96 uneven blocks"
-    c, _ = make_case(workload)
-    return c, "the whole case"
-
-
-def _cpu_sample_run(workload: str, budget_s: float):
-    """One single-threaded oracle process: whole multigrid cycles of the
-    sample for about budget_s; returns (cell-updates, seconds, cycles,
-    cells, description)."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_bind import Oracle, OrCase
-    sample, desc = _cpu_sample_case(workload)
-    o = Oracle()
-    o.set_config(sample.config)
-    oc = OrCase(sample, q1=sample.perturbed(amp=0.01, seed=1))
-    st = sample.config["nstage"]
-    per_cycle = st * (sample.ncell(1) + 2 * sample.ncell(2) + sample.ncell(3))
-    where = (ctypes.c_int32 * 8)()
-    done = (ctypes.c_int32 * 1)()
+    from oracle.ref_mg import RefMG
+    c = cases.channel_case([dims], nlevel=3, name="sample", **_uneven_cfg())
+    r = RefMG(c, c.perturbed(amp=0.01, seed=1), lib=lib)
+    per = r.cell_updates_per_cycle()
     t0 = time.perf_counter()
-    ncyc = 0
-    cp = oc.ptr()
+    n = 0
     while True:
-        mgit = ncyc + 1
-        rc = o.or_smooth(cp, 1, 0, 1, 1, mgit, done, where)
-        o.or_restrict(cp, 1)
-        rc |= o.or_smooth(cp, 2, 1, 1, 1, mgit, done, where)
-        o.or_restrict(cp, 2)
-        rc |= o.or_smooth(cp, 3, 1, 0, 1, mgit, done, where)
-        o.or_prolong(cp, 2)
-        rc |= o.or_smooth(cp, 2, 0, 0, 1, mgit, done, where)
-        o.or_prolong(cp, 1)
-        ncyc += 1
-        if rc:
-            raise RuntimeError("oracle sample failed")
-        if time.perf_counter() - t0 > budget_s or ncyc >= 400:
+        n += 1
+        r.cycle(n)
+        if time.perf_counter() - t0 > budget_s or n >= 200:
             break
-    dt = time.perf_counter() - t0
-    return per_cycle * ncyc, dt, ncyc, sample.ncell(), desc
+    return per * n, time.perf_counter() - t0, n
 
 
-def cpu_baseline(workload: str, budget_s: float, workers: int):
-    """The CPU restatement (oracle/liboracle.so, gcc -O2, one thread per
-    process) timed on this host's cores: `workers` independent copies of the
-    sample run at once in fresh processes (the throughput form of the
-    reference's single-rank solves), aggregate cell-updates per second."""
+def cpu_baseline(budget_s: float, workers: int):
+    """COSA's compiled routines timed on this host: `workers` processes, each
+    one of the first `workers` blocks of the uneven set (the reference runs
+    one MPI rank per block), aggregate cell-updates per second."""
     import multiprocessing as mp
+    from cosa_amd import case as cases
+    from oracle.ref_mg import TIMING_LIB
+    if not os.path.exists(TIMING_LIB):
+        return None
+    dims = cases.uneven_dims(96, seed=0)[:workers]
     t0 = time.perf_counter()
-    if workers > 1:
-        with mp.get_context("spawn").Pool(workers) as pool:
-            runs = pool.starmap(_cpu_sample_run, [(workload, budget_s)] * workers)
-    else:
-        runs = [_cpu_sample_run(workload, budget_s)]
+    with mp.get_context("spawn").Pool(workers) as pool:
+        runs = pool.starmap(_ref_block_run, [(d, budget_s, TIMING_LIB) for d in dims])
     wall = time.perf_counter() - t0
-    rate = sum(u / dt for u, dt, *_ in runs)
-    u, dt, ncyc, cells, desc = runs[0]
-    return {"value": round(rate / 1e6, 4), "unit": "Mcell-updates/s", "cores": workers, "kind": "port",
-            "sample": f"{workers} concurrent single-threaded copies of {desc} ({cells} cells), {ncyc} multigrid "
-                      f"cycles each in {dt:.1f} s ({wall:.0f} s wall), oracle/liboracle.so (C restatement, gcc -O2)"}
+    rate = sum(u / dt for u, dt, _ in runs)
+    cells = sum((a - 1) * (b - 1) * (c - 1) for a, b, c in dims)
+    return {"value": round(rate / 1e6, 4), "unit": "Mcell-updates/s", "cores": workers, "kind": "reference",
+            "sample": f"the first {workers} of the 96 uneven blocks ({cells} cells, 3 levels), one block per "
+                      f"process as one reference rank, {min(n for *_, n in runs)}-{max(n for *_, n in runs)} "
+                      f"multigrid cycles each in ~{budget_s:.0f} s ({wall:.0f} s wall); COSA's compiled kernels "
+                      f"(amdflang -O3 -fdefault-real-8, no fast-math) in smooth/mg order (oracle/ref_mg.py); "
+                      f"CPU: {_cpu_model()}"}
 
 
 def copy_calibration():
@@ -212,17 +191,106 @@ def copy_calibration():
                       "launches": 3}), flush=True)
 
 
+# ---------------------------------------------------------------------------
+# GPU measurement
+
+def measure(workload: str, steps: int, warmup: int, precision: str, tail_rms: float, autotune: bool,
+            sync_region: bool = True):
+    """Graph-replayed cycles (value), then the synchronous path with kernel
+    family timing (roofline).  Returns a dict."""
+    from cosa_amd.config import komega
+    from cosa_amd.solver import Solve
+    t0 = time.time()
+    case, cfg_desc = make_case(workload)
+    cfg_desc["precision"] = {"fp64": "fp64", "fp32": "fp32 residual / fp64 state",
+                             "mixed": f"fp32 residual until resrms(1) < {tail_rms:g}, then fp64"}[precision]
+    log(f"case {case.name}: {case.ncell()} cells, built in {time.time() - t0:.1f} s")
+    q0 = case.perturbed(amp=0.01, seed=1)
+    s = Solve(case, device=0, q=q0)
+    del q0
+    prec_mode = {"fp64": 0, "fp32": 1, "mixed": 2}[precision]
+    if prec_mode:
+        s.set_precision(prec_mode, tail_rms if prec_mode == 2 else 0.0)
+    log("device state resident; warmup")
+    mgit = 0
+    tuned = None
+    for w in range(max(warmup, 2)):
+        mgit += 1
+        s.cycle_async(mgit)
+        if w == 0 and autotune:
+            # after one cycle the level states are realistic: pick the kernel arrangements
+            tuned = s.autotune(3)
+            log(f"autotune: {tuned}")
+    s.synchronize()
+    log(f"timed region (graph-replayed cycles): {steps} cycles")
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        mgit += 1
+        s.cycle_async(mgit)
+    s.synchronize()
+    t2 = time.perf_counter()
+    elapsed = t2 - t1
+    per_cycle = s.cell_updates_per_cycle()
+    value = per_cycle * steps / elapsed / 1e6
+    res_hist = [float(h[0]) for h in s.history]
+    log(f"elapsed {elapsed:.3f} s, {value:.1f} Mcell-updates/s, resrms(1) {res_hist[0]:.3e} -> {res_hist[-1]:.3e}")
+    out = dict(value=value, ms_per_step=elapsed / steps * 1e3, config=dict(cfg_desc, autotune=tuned),
+               resrms1_first_last=[res_hist[0], res_hist[-1]], cycles_run=mgit)
+    if sync_region:
+        # the synchronous path with HIP events around each kernel family
+        s.timing_reset()
+        s.timing(True)
+        s.synchronize()
+        log(f"timed region (synchronous, kernel families timed): {steps} cycles")
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            mgit += 1
+            s.cycle(mgit)
+        s.synchronize()
+        t2 = time.perf_counter()
+        fam = s.timing_read()
+        s.timing(False)
+        out["sync_path"] = {"value": round(per_cycle * steps / (t2 - t1) / 1e6, 2),
+                            "ms_per_step": round((t2 - t1) / steps * 1e3, 3)}
+        bpc_tab = BYTES_PER_CELL_RANS if komega(case.config) else BYTES_PER_CELL_EULER
+        if prec_mode and s.precision_in_use() == 32:
+            bpc_tab = dict(bpc_tab, **(BYTES_PER_CELL_RANS_FP32_RESIDUAL if komega(case.config)
+                                       else BYTES_PER_CELL_FP32_RESIDUAL))
+        fams = {}
+        for name, d in fam.items():
+            bpc = bpc_tab.get(name)
+            if d["ms"] > 0 and bpc:
+                gbs = bpc * d["items"] / (d["ms"] / 1e3) / 1e9
+                fams[name] = {"ms_total": round(d["ms"], 3), "launches": d["launches"], "bytes_per_cell": bpc,
+                              "achieved_GBs": round(gbs, 1)}
+            elif d["ms"] > 0:
+                fams[name] = {"ms_total": round(d["ms"], 3), "launches": d["launches"]}
+        dom = max((n for n in fams if bpc_tab.get(n)), key=lambda n: fams[n]["ms_total"])
+        d = fam[dom]
+        achieved = bpc_tab[dom] * d["items"] / (d["ms"] / 1e3) / 1e9
+        tpc = MEASURED_TRAFFIC_PER_CELL.get((workload, precision, dom))
+        traffic = round(tpc * d["items"] / max(d["launches"], 1)) if tpc else None
+        out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                           "bytes_per_cell_update": bpc_tab[dom],
+                           "avg_launch_ms": round(d["ms"] / max(d["launches"], 1), 4)}
+        out["families"] = fams
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="wing4", choices=["wing4", "uneven96", "naca"])
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--workload", default="uneven96", choices=["uneven96", "wing4", "naca"])
+    ap.add_argument("--no-extras", action="store_true", help="leave out the wing workload of extra_workloads")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default kernel arrangements")
     ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1),
-                    help="concurrent single-threaded oracle processes for cpu_baseline (the host's cores, at most 16)")
+                    help="concurrent single-block reference processes for cpu_baseline (at most 16)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "mixed"],
                     help="fp32: FP32 residual / FP64 state (cosahip_set_precision mode 1); mixed: FP32 until "
                          "resrms(1) < --tail-rms, then FP64 (mode 2)")
@@ -235,88 +303,31 @@ def main():
         print("bench.py: this build runs the path on one GPU only (one GPU's share of the blocks); "
               "--gpus must be 1", file=sys.stderr)
         sys.exit(2)
-
     if a.copy_calibration:
         copy_calibration()
         return
 
-    from cosa_amd.config import komega
-    from cosa_amd.solver import Solve
-    t0 = time.time()
-    case, cfg_desc = make_case(a.workload)
-    cfg_desc["precision"] = {"fp64": "fp64", "fp32": "fp32 residual / fp64 state",
-                             "mixed": f"fp32 residual until resrms(1) < {a.tail_rms:g}, then fp64"}[a.precision]
-    log(f"case {case.name}: {case.ncell()} cells, built in {time.time() - t0:.1f} s")
-    q0 = case.perturbed(amp=0.01, seed=1)
-    s = Solve(case, device=0, q=q0)
-    del q0
-    prec_mode = {"fp64": 0, "fp32": 1, "mixed": 2}[a.precision]
-    if prec_mode:
-        s.set_precision(prec_mode, a.tail_rms if prec_mode == 2 else 0.0)
-    log("device state resident; warmup")
-    mgit = 0
-    tuned = None
-    for w in range(a.warmup):
-        mgit += 1
-        s.cycle(mgit)
-        if w == 0 and not a.no_autotune:
-            # after one cycle the level states are realistic: pick the kernel arrangements
-            tuned = s.autotune(3)
-            log(f"autotune: {tuned}")
-    s.synchronize()
-    s.timing_reset()
-    s.timing(True)
-    s.synchronize()
-    log(f"timed region: {a.steps} cycles")
-    t1 = time.perf_counter()
-    for _ in range(a.steps):
-        mgit += 1
-        s.cycle(mgit)
-    s.synchronize()
-    t2 = time.perf_counter()
-    fam = s.timing_read()
-    s.timing(False)
-    elapsed = t2 - t1
-    per_cycle = s.cell_updates_per_cycle()
-    value = per_cycle * a.steps / elapsed / 1e6
-    res_hist = [float(h[0]) for h in s.history]
-    log(f"elapsed {elapsed:.3f} s, {value:.1f} Mcell-updates/s, resrms(1) {res_hist[0]:.3e} -> {res_hist[-1]:.3e}")
-
-    # roofline of the dominant family (most device time in the timed region)
-    BYTES_PER_CELL = BYTES_PER_CELL_RANS if komega(case.config) else BYTES_PER_CELL_EULER
-    if prec_mode and s.precision_in_use() == 32:
-        BYTES_PER_CELL = dict(BYTES_PER_CELL, **(BYTES_PER_CELL_RANS_FP32_RESIDUAL if komega(case.config)
-                                                 else BYTES_PER_CELL_FP32_RESIDUAL))
-    fams = {}
-    for name, d in fam.items():
-        bpc = BYTES_PER_CELL.get(name)
-        if d["ms"] > 0 and bpc:
-            gbs = bpc * d["items"] / (d["ms"] / 1e3) / 1e9
-            fams[name] = {"ms_total": round(d["ms"], 3), "launches": d["launches"], "bytes_per_cell": bpc,
-                          "achieved_GBs": round(gbs, 1)}
-        elif d["ms"] > 0:
-            fams[name] = {"ms_total": round(d["ms"], 3), "launches": d["launches"]}
-    dom = max((n for n in fams if BYTES_PER_CELL.get(n)), key=lambda n: fams[n]["ms_total"])
-    d = fam[dom]
-    achieved = BYTES_PER_CELL[dom] * d["items"] / (d["ms"] / 1e3) / 1e9
-    tpc = MEASURED_TRAFFIC_PER_CELL.get((a.workload, a.precision, dom))
-    traffic = round(tpc * d["items"] / max(d["launches"], 1)) if tpc else None
-    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "bytes_per_cell_update": BYTES_PER_CELL[dom], "avg_launch_ms": round(d["ms"] / max(d["launches"], 1), 4)}
-    s.close()
-
+    head = measure(a.workload, a.steps, a.warmup, a.precision, a.tail_rms, not a.no_autotune)
+    extras = {}
+    if not a.no_extras and a.workload == "uneven96":
+        w = measure("wing4", a.steps, a.warmup, a.precision, a.tail_rms, not a.no_autotune)
+        extras["wing4_rans_komega"] = {"value": round(w["value"], 2), "unit": "Mcell-updates/s",
+                                       "ms_per_step": round(w["ms_per_step"], 3), "config": w["config"],
+                                       "sync_path": w.get("sync_path"), "roofline": w.get("roofline")}
     cpu = None
     if not a.no_cpu:
-        log("cpu baseline (oracle, 1 thread)")
-        cpu = cpu_baseline(a.workload, a.cpu_seconds, a.cpu_workers)
+        log("cpu baseline (the reference's compiled routines)")
+        cpu = cpu_baseline(a.cpu_seconds, max(1, min(16, a.cpu_workers)))
 
-    out = {"metric": "Mcell-updates/s", "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
-           "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(elapsed / a.steps * 1e3, 3),
+    out = {"metric": "Mcell-updates/s", "value": round(head["value"], 2), "unit": "Mcell-updates/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(head["ms_per_step"], 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64" if not prec_mode else "f32 residual / f64 state", "data": "synthetic",
-           "config": dict(cfg_desc, autotune=tuned), "roofline": roof, "cpu_baseline": cpu, "families": fams,
-           "resrms1_first_last": [res_hist[0], res_hist[-1]]}
+           "dtype": "f64" if a.precision == "fp64" else "f32 residual / f64 state", "data": "synthetic",
+           "config": head["config"], "roofline": head.get("roofline"), "cpu_baseline": cpu,
+           "sync_path": head.get("sync_path"), "families": head.get("families"),
+           "resrms1_first_last": head["resrms1_first_last"]}
+    if extras:
+        out["extra_workloads"] = extras
     print(json.dumps(out), flush=True)
 
 

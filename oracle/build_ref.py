@@ -45,6 +45,9 @@ BY_NAME = {
     "input.f90": ["read_brest"],
 }
 FFLAGS = ["-cpp", "-fdefault-real-8", "-fdefault-double-8", "-O2", "-ffp-contract=off", "-fPIC"]
+# the CPU baseline's build: the reference's optimisation level (-O3) with the
+# compiler's default floating-point contraction, without -ffast-math
+FFLAGS_TIMING = ["-cpp", "-fdefault-real-8", "-fdefault-double-8", "-O3", "-fPIC"]
 
 
 def extract(path, names):
@@ -83,11 +86,19 @@ def main():
     if not os.path.isdir(src) or not os.path.exists(fc):
         print("build_ref: reference sources or amdflang absent; oracle/_ref not built")
         return 2
+    build_lib(src, fc, FFLAGS, "libcosaref.so", a.quiet)
+    build_lib(src, fc, FFLAGS_TIMING, "libcosaref_o3.so", a.quiet)
+    return 0
+
+
+def build_lib(src, fc, flags, name, quiet):
+    """Compile the reference routines and the shim with `flags` into
+    oracle/_ref/<name>."""
     os.makedirs(OUT, exist_ok=True)
     objs = []
     for m in WHOLE:
         o = os.path.join(OUT, m + ".o")
-        run([fc] + FFLAGS + ["-module-dir", OUT, "-c", os.path.join(src, m + ".f90"), "-o", o], a.quiet)
+        run([fc] + flags + ["-module-dir", OUT, "-c", os.path.join(src, m + ".f90"), "-o", o], quiet)
         objs.append(o)
     tmp = os.path.join(OUT, "tmp_src")
     os.makedirs(tmp, exist_ok=True)
@@ -97,22 +108,21 @@ def main():
             with open(part, "w") as f:
                 f.write(extract(os.path.join(src, fname), names))
             o = os.path.join(OUT, "sel_" + fname.replace(".f90", ".o"))
-            run([fc] + FFLAGS + ["-I", OUT, "-c", part, "-o", o], a.quiet)
+            run([fc] + flags + ["-I", OUT, "-c", part, "-o", o], quiet)
             objs.append(o)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     o = os.path.join(OUT, "ref_shim.o")
-    run([fc] + FFLAGS + ["-I", OUT, "-module-dir", OUT, "-c", os.path.join(HERE, "ref_shim.f90"), "-o", o],
-        a.quiet)
+    run([fc] + flags + ["-I", OUT, "-module-dir", OUT, "-c", os.path.join(HERE, "ref_shim.f90"), "-o", o],
+        quiet)
     objs.append(o)
-    lib = os.path.join(OUT, "libcosaref.so")
-    run([fc, "-shared", "-fPIC", "-o", lib] + objs, a.quiet)
+    lib = os.path.join(OUT, name)
+    run([fc, "-shared", "-fPIC", "-o", lib] + objs, quiet)
     for f in os.listdir(OUT):
         if f.endswith(".o") or f.endswith(".mod"):
             os.remove(os.path.join(OUT, f))
-    if not a.quiet:
+    if not quiet:
         print("built", lib)
-    return 0
 
 
 if __name__ == "__main__":

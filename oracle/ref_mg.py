@@ -1,0 +1,236 @@
+"""COSA's own compiled routines (oracle/_ref/libcosaref*.so, built from the
+reference sources by oracle/build_ref.py) driven in the order of the
+reference's smooth() and mg() for steady Euler on blocks without cuts --
+the physics of the uneven multiblock channels.  TEST AND BASELINE
+INFRASTRUCTURE ONLY: the CPU baseline of bench.py times it, and
+tests/test_oracle_vs_ref.py requires it to give the oracle's or_mg states
+bit for bit.
+
+Every floating-point operation on the state happens inside the reference's
+compiled *_b kernels: cons2prim_b, bbc, lambdab, bdeltat, muscl_b*,
+roflux_b, bresid, prim2cons_b, resg_brms, cirs_b, update_b, rest1_b,
+rest2_lo_b / rest2_ho_b, prol_bln_b / prol_lin_b, q_bedges, q_bcorns.  The
+three whole-level utilities of src/util.f90 that smooth calls between them
+(copy_array, add_array, zero: a copy, an addition, a fill) are numpy
+operations on the same interior ranges here, since util.f90's whole-level
+forms need the reference's global offset tables.  The positivity
+assertion bprtest (a read-only scan) is left out of the timed loop.
+
+Order (src/cosa.f90, smooth 1594-2032, mg 806-930, restrict 2672-2754,
+prolong 3167-3249), per iteration: qold = q; per RK stage: cons2prim, bc,
+[stage 1: lambda, deltat], zero, three directions of muscl / roflux /
+resid, prim2cons, [level 1, stage 1: norms], [FAS: rhs += res first stage
+from a finer level; res -= rhs on coarse levels], [restriction pass: stop],
+cirs, update.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import sys
+from typing import List
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from cosa_amd.config import to_struct  # noqa: E402
+
+DP = ctypes.POINTER(ctypes.c_double)
+IP = ctypes.POINTER(ctypes.c_int32)
+PARITY_LIB = os.path.join(HERE, "_ref", "libcosaref.so")        # -O2, no FMA contraction (parity)
+TIMING_LIB = os.path.join(HERE, "_ref", "libcosaref_o3.so")     # -O3, the compiler's defaults (timing)
+
+
+def _d(a):
+    return a.ctypes.data_as(DP)
+
+
+def _i(a):
+    return a.ctypes.data_as(IP)
+
+
+class _Blk:
+    """Explicit-shape arrays of one block of one level (the reference's
+    per-block views of its Cray-pointer level arrays)."""
+
+    def __init__(self, case, nl, b):
+        L = case.levels[nl - 1]
+        self.dims = (int(L.imax[b]), int(L.jmax[b]), int(L.kmax[b]))
+        im, jm, km = self.dims
+        npde = case.npde
+        n3 = (im + 3) * (jm + 3) * (km + 3)
+        n2 = (im + 2) * (jm + 2) * (km + 2)
+        n1 = (im + 1) * (jm + 1) * (km + 1)
+        n0 = im * jm * km
+        g = case.geom[nl - 1]
+        o2, p2 = L.offsets(2)[b], L.points(2)[b]
+        o1, p1 = L.offsets(1)[b], L.points(1)[b]
+        lmet = case.config["lmet"]
+        self.si, self.sj, self.sk = (np.ascontiguousarray(g[f][o2 * lmet:(o2 + p2) * lmet]) for f in ("si", "sj", "sk"))
+        self.vol = np.ascontiguousarray(g["vol"][o1:o1 + p1])
+        self.q = np.zeros(n3 * npde)
+        for n in ("qold", "res", "rhs", "qp", "qm", "dq", "dqp", "dqm", "work", "lam", "lamv"):
+            setattr(self, n, np.zeros(n3 * npde))
+        self.flux = np.zeros(n0 * npde)
+        for n in ("dtvol", "dtvolt", "bx", "be", "bz", "tbx", "tbe", "tbz", "cut1", "cut2"):
+            setattr(self, n, np.zeros(n1))
+        self.xd = np.zeros(n2)
+        self.mut = np.zeros(n3)
+        self.dist = np.zeros(n1)
+        self.cirs_work = np.zeros(max(self.dims) * npde * 4)
+        nb = int(L.nbcs[b])
+        off = int(np.sum(L.nbcs[:b]))
+        self.nbcs = nb
+        self.bctopo = np.ascontiguousarray(np.asarray(L.bctopo, np.int32).reshape(-1, 10)[off:off + nb]).reshape(-1)
+        # interior cells 1..n-1 and the box 0..n of the P3 array
+        shp = (npde, km + 3, jm + 3, im + 3)
+        self.shape = shp
+        self.inner = (slice(None), slice(2, km + 1), slice(2, jm + 1), slice(2, im + 1))
+        self.box = (slice(None), slice(1, km + 2), slice(1, jm + 2), slice(1, im + 2))
+        self.ncell = (im - 1) * (jm - 1) * (km - 1)
+
+
+class RefMG:
+    """Steady Euler multigrid solve of a case whose blocks have no cuts, on
+    the reference's compiled kernels."""
+
+    def __init__(self, case, q1: np.ndarray, lib: str = PARITY_LIB):
+        c = case.config
+        if c["viscous"] or c["nharms"] or c["moving"] or any(L.ncuts for L in case.levels):
+            raise ValueError("ref_mg drives steady Euler on blocks without cuts only")
+        self.case = case
+        self.lib = ctypes.CDLL(lib, mode=os.RTLD_LAZY)
+        self._cfg = to_struct(c)
+        self.lib.ref_set_config(ctypes.byref(self._cfg))
+        self.cfg = c
+        self.lev: List[List[_Blk]] = []
+        for L in case.levels:
+            blks = [_Blk(case, L.nl, b) for b in range(L.nblocks)]
+            fs = case.freestream(L.nl)
+            o3, p3 = L.offsets(3), L.points(3)
+            for b, B in enumerate(blks):
+                src = q1 if (L.nl == 1 and q1 is not None) else fs
+                B.q[:] = src[o3[b] * case.npde:(o3[b] + p3[b]) * case.npde]
+            self.lev.append(blks)
+        self.resrms = np.zeros(8)
+
+    # -- the level routines, block by block --------------------------------
+    def _v(self, B, name):
+        return getattr(B, name).reshape(B.shape)
+
+    def smooth(self, nl, from_fine, to_coarse, niter, mgit):
+        lib, c = self.lib, self.cfg
+        blks = self.lev[nl - 1]
+        niterp = niter + 1 if to_coarse else niter
+        for it in range(1, niterp + 1):
+            if it <= niter:
+                for B in blks:
+                    self._v(B, "qold")[B.inner] = self._v(B, "q")[B.inner]
+            for irk in range(1, c["nstage"] + 1):
+                for B in blks:
+                    lib.ref_cons2prim_b(_d(B.q), _d(B.work), *B.dims, 0)
+                    lib.ref_bbc(_d(B.q), _d(B.si), _d(B.sj), _d(B.sk), _d(B.dist), nl, _i(B.bctopo), B.nbcs,
+                                *B.dims)
+                if it % c["iupdt"] == 0 and irk == 1:
+                    for B in blks:
+                        lib.ref_lambdab(_d(B.q), _d(B.si), _d(B.sj), _d(B.sk), _d(B.xd), _d(B.xd), _d(B.xd),
+                                        _d(B.vol), _d(B.mut), _d(B.work), _d(B.lam), _d(B.cut1), _d(B.cut2), *B.dims)
+                        lib.ref_bdeltat(_d(B.dtvol), _d(B.dtvolt), _d(B.lam), _d(B.lamv), _d(B.bx), _d(B.be),
+                                        _d(B.bz), _d(B.tbx), _d(B.tbe), _d(B.tbz), *B.dims)
+                for B in blks:
+                    self._v(B, "res")[B.box] = 0.0
+                for idir in (1, 2, 3):
+                    for B in blks:
+                        s = (B.si, B.sj, B.sk)[idir - 1]
+                        lib.ref_muscl_b(idir, nl, _i(B.bctopo), B.nbcs, _d(B.q), _d(B.qp), _d(B.qm), _d(B.dq),
+                                        _d(B.dqp), _d(B.dqm), *B.dims)
+                        lib.ref_roflux_b(idir, _d(B.qp), _d(B.qm), _d(B.dqp), _d(B.dqm), _d(B.flux), _d(s), *B.dims)
+                        lib.ref_bresid(idir, _d(B.flux), _d(B.res), *B.dims)
+                for B in blks:
+                    lib.ref_prim2cons_b(_d(B.q), _d(B.work), *B.dims)
+                if nl == 1 and irk == 1 and it != niter + 1 and (
+                        mgit == 1 or mgit % c["iprint"] == 0 or mgit == c["ncycle"]):
+                    np_ = self.case.npde
+                    tot = np.zeros(8)
+                    ncell = 0
+                    rb = np.zeros(8)
+                    for B in blks:     # per-block sums of squares, summed over blocks (combineresults)
+                        lib.ref_resg_brms(_d(B.q), _d(B.res), _d(B.rhs), _d(B.vol), _d(rb), *B.dims)
+                        tot[:np_ + 1] = tot[:np_ + 1] + rb[:np_ + 1]
+                        ncell += B.ncell
+                    self.resrms[:np_] = np.sqrt(tot[:np_] / ncell)
+                    self.resrms[np_] = np.sqrt(tot[np_] / ncell / np_)
+                if from_fine and it == 1 and irk == 1:
+                    for B in blks:
+                        r, h = self._v(B, "res"), self._v(B, "rhs")
+                        h[B.inner] = h[B.inner] + 1.0 * r[B.inner]
+                if nl > 1:
+                    for B in blks:
+                        r, h = self._v(B, "res"), self._v(B, "rhs")
+                        r[B.inner] = r[B.inner] + -1.0 * h[B.inner]
+                if it == niter + 1:
+                    return
+                if (0 < c["irsop"] <= 3) or c["irsop"] in (5, 6):
+                    for B in blks:
+                        lib.ref_cirs_b(_d(B.res), _d(B.dtvol), _d(B.bx), _d(B.be), _d(B.bz), _d(B.dtvolt),
+                                       _d(B.tbx), _d(B.tbe), _d(B.tbz), _d(B.cirs_work), *B.dims)
+                ar = ctypes.c_double(c["alphas"][irk - 1])
+                for B in blks:
+                    lib.ref_update(_d(B.res), _d(B.q), _d(B.qold), _d(B.vol), _d(B.dtvol), _d(B.dtvolt), _d(B.work),
+                                   _d(B.dq), _d(B.work), _d(B.work), _d(B.work), ar, *B.dims)
+
+    def restrict(self, nl):
+        lib = self.lib
+        for B1, B2 in zip(self.lev[nl - 1], self.lev[nl]):
+            lib.ref_rest1_b(*B1.dims, *B2.dims, _d(B1.q), _d(B2.q), _d(B1.vol), _d(B2.vol))
+            lib.ref_rest2_b(int(self.cfg["ho_rest"]), *B1.dims, *B2.dims, _d(B1.rhs), _d(B2.rhs), _d(B1.res))
+
+    def prolong(self, nl):
+        lib = self.lib
+        for B1, B2 in zip(self.lev[nl - 1], self.lev[nl]):
+            lib.ref_rest1_b(*B1.dims, *B2.dims, _d(B1.q), _d(B2.qold), _d(B1.vol), _d(B2.vol))
+            for a in (B2.qold, B2.q):
+                lib.ref_cons2prim_b(_d(a), _d(B2.work), *B2.dims, 0)
+                lib.ref_bbc(_d(a), _d(B2.si), _d(B2.sj), _d(B2.sk), _d(B2.dist), nl + 1, _i(B2.bctopo), B2.nbcs,
+                            *B2.dims)
+                lib.ref_prim2cons_b(_d(a), _d(B2.work), *B2.dims)
+                lib.ref_q_bedges(_d(a), _d(B2.mut), *B2.dims)
+                lib.ref_q_bcorns(_d(a), _d(B2.mut), *B2.dims)
+            q2, qo2 = self._v(B2, "q"), self._v(B2, "qold")
+            q2[B2.box] = q2[B2.box] + -1.0 * qo2[B2.box]
+            lib.ref_prol_b(int(self.cfg["bln_prol"]), *B1.dims, *B2.dims, _d(B1.q), _d(B2.q), _d(B1.vol))
+
+    def cycle(self, mgit):
+        """One pass of mg's cycle loop (nstart = npre = npost = ncrs = 1)."""
+        nlevel = self.cfg["nlevel"]
+        self.smooth(1, False, nlevel > 1, 1, mgit)
+        if nlevel == 1:
+            return self.resrms.copy()
+        r = self.resrms.copy()
+        self.restrict(1)
+        self.smooth(2, True, nlevel > 2, 1, mgit)
+        if nlevel >= 3:
+            self.restrict(2)
+            self.smooth(3, True, False, 1, mgit)
+            self.prolong(2)
+            self.smooth(2, False, False, 1, mgit)
+        self.prolong(1)
+        return r
+
+    def level_q(self, nl):
+        """The level's state in the whole-level layout."""
+        return np.concatenate([B.q for B in self.lev[nl - 1]])
+
+    def cell_updates_per_cycle(self):
+        c, st = self.case, self.cfg["nstage"]
+        nlevel = self.cfg["nlevel"]
+        tot = c.ncell(1) * st
+        if nlevel >= 2:
+            tot += c.ncell(2) * st * (2 if nlevel >= 3 else 1)
+        if nlevel >= 3:
+            tot += c.ncell(3) * st
+        return tot

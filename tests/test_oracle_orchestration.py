@@ -179,3 +179,38 @@ def test_fixed_point_kept_and_no_coarse_correction(fixed_point, irsop):
     # cycles may only move it within the tolerance of the variant tests
     assert d < 1e-10
     assert r < 1e-12
+
+
+@pytest.mark.ref
+@pytest.mark.skipif(not __import__("oracle_bind").ref_available(), reason="oracle/_ref not built")
+def test_reference_kernels_in_mg_order_equal_or_mg():
+    """oracle/ref_mg.py drives COSA's compiled kernels (the CPU baseline of
+    the benchmark) in smooth()/mg() order; three 3-level V-cycles of two
+    uneven channel blocks give the oracle's states on every level and its
+    norms, bit for bit."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from oracle.ref_mg import RefMG
+    c = cases.channel_case([(17, 9, 9), (9, 17, 9)], nlevel=3, name="two_channels", machfs=0.7, irsop=1, cfl=2.0,
+                           cfl_irs=4.0, ilimf=4, epslimf=1e-6, nl_crs=2, ncycle=3)
+    q0 = c.perturbed(amp=0.01, seed=1)
+    o = Oracle()
+    o.set_config(c.config)
+    oc = OrCase(c, q1=q0.copy())
+    cp = oc.ptr()
+    where, done = _where()
+    r = RefMG(c, q0.copy())
+    for m in (1, 2, 3):
+        o.or_smooth(cp, 1, 0, 1, 1, m, done, where)
+        o.or_restrict(cp, 1)
+        o.or_smooth(cp, 2, 1, 1, 1, m, done, where)
+        o.or_restrict(cp, 2)
+        o.or_smooth(cp, 3, 1, 0, 1, m, done, where)
+        o.or_prolong(cp, 2)
+        o.or_smooth(cp, 2, 0, 0, 1, m, done, where)
+        o.or_prolong(cp, 1)
+        r.cycle(m)
+        np.testing.assert_array_equal(r.resrms[:6], [o.or_get_resrms(i) for i in range(1, 7)])
+    for nl in (1, 2, 3):
+        np.testing.assert_array_equal(r.level_q(nl), oc[nl]["q"])
