@@ -75,6 +75,10 @@ def load() -> ctypes.CDLL:
         "cosahip_residual": ([_VP, ctypes.c_int], ctypes.c_int),
         "cosahip_src_hb": ([_VP, ctypes.c_int], ctypes.c_int),
         "cosahip_resg_rms": ([_VP, ctypes.c_int, _DP], ctypes.c_int),
+        "cosahip_checkpoint_begin": ([_VP, ctypes.c_int, _DP, ctypes.c_int64, _DP, ctypes.c_int64], ctypes.c_int),
+        "cosahip_checkpoint_wait": ([_VP], ctypes.c_int),
+        "cosahip_host_alloc": ([ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+        "cosahip_host_free": ([_VP], ctypes.c_int),
         "cosahip_add_array": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "cosahip_copy_array": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "cosahip_cirs": ([_VP, ctypes.c_int], ctypes.c_int),
@@ -116,7 +120,8 @@ def exported_symbols():
     """Names the library must export (every function of include/cosahip.h)."""
     return ["cosahip_create", "cosahip_destroy", "cosahip_last_error", "cosahip_error_info",
             "cosahip_set_hb_operator", "cosahip_define_level", "cosahip_field_size", "cosahip_upload",
-            "cosahip_download", "cosahip_synchronize", "cosahip_cons2prim", "cosahip_prim2cons", "cosahip_bc",
+            "cosahip_download", "cosahip_checkpoint_begin", "cosahip_checkpoint_wait", "cosahip_host_alloc",
+            "cosahip_host_free", "cosahip_synchronize", "cosahip_cons2prim", "cosahip_prim2cons", "cosahip_bc",
             "cosahip_cutman_q", "cosahip_edges_corners", "cosahip_prtest", "cosahip_lambda_deltat",
             "cosahip_residual", "cosahip_src_hb", "cosahip_resg_rms", "cosahip_add_array", "cosahip_copy_array",
             "cosahip_cirs", "cosahip_update", "cosahip_tur_vis", "cosahip_fixq", "cosahip_turb_limits",

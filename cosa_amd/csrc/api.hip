@@ -248,6 +248,10 @@ int cosahip_create(const cosahip_config* cfg, int device, cosahip_solve** out) {
   s->par = make_par(*cfg);
   const char* rk = getenv("COSAHIP_RESIDUAL");
   s->split_residual = rk && strcmp(rk, "split") == 0;
+  if (rk && !s->split_residual)
+    s->resid_force = strcmp(rk, "fused") == 0 ? RESID_GENERIC : strcmp(rk, "tile7") == 0 ? RESID_TILE7
+                     : strcmp(rk, "tile7reg") == 0 ? RESID_TILE7_REG : strcmp(rk, "tile15") == 0 ? RESID_TILE15
+                     : strcmp(rk, "tile7w4") == 0 ? RESID_TILE7_W4 : -1;
   const char* gk = getenv("COSAHIP_GRAPH");
   s->use_graph = !(gk && strcmp(gk, "0") == 0);
   const char* ck = getenv("COSAHIP_CIRS_I");
@@ -289,6 +293,13 @@ int cosahip_destroy(cosahip_solve* h) {
   if (s->hpinned) (void)hipHostFree(s->hpinned);
   if (s->hpart) (void)hipHostFree(s->hpart);
   if (s->dqfst) (void)hipFree(s->dqfst);
+  if (s->ck_stream) {
+    (void)hipStreamSynchronize(s->ck_stream);
+    (void)hipStreamDestroy(s->ck_stream);
+  }
+  if (s->ck_snap) (void)hipEventDestroy(s->ck_snap);
+  if (s->ck_done) (void)hipEventDestroy(s->ck_done);
+  if (s->ck_buf) (void)hipFree(s->ck_buf);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return COSAHIP_OK;
@@ -408,7 +419,8 @@ static int build_tables(Solve* s, Level& L, const Level* Lc) {
     }
     add_tiles(L.htile[TK_CELLS_P1], b, ns * (ll)(B.imax + 1) * (B.jmax + 1) * (B.kmax + 1), TS);
     add_tiles(L.htile[TK_HALO], b, ns * ((ll)(B.imax + 3) * jm * km + 4 * im * km + 4 * im * jm), TS);
-    add_tiles(L.htile[TK_FUSED], b, ns * ((im + 30) / 31) * ((jm + 6) / 7) * ((km + 31) / 32), 1);
+    add_tiles(L.htile[TK_FUSED], b, residual_tile_items(7, B.imax, B.jmax, B.kmax, ns), 1);
+    add_tiles(L.htile[TK_FUSED15], b, residual_tile_items(15, B.imax, B.jmax, B.kmax, ns), 1);
     if (Lc) {
       const Blk& C = Lc->hblk[b];
       add_tiles(L.htile[TK_COARSE], b, ns * (ll)(C.imax - 1) * (C.jmax - 1) * (C.kmax - 1), TS);
@@ -1058,6 +1070,71 @@ extern "C" int cosahip_download(cosahip_solve* h, int nl, int field, double* hos
   return COSAHIP_OK;
 }
 
+// Checkpoint without stopping the solve: snapshot in stream order on the
+// device, then the device-to-host copy on the checkpoint stream, which runs
+// while the solve's stream goes on with the next cycles.
+extern "C" int cosahip_checkpoint_begin(cosahip_solve* h, int nl, double* host_q, int64_t nq, double* host_mut,
+                                        int64_t nmut) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  Level* L = get_level(s, nl);
+  if (!L) return fail(s, COSAHIP_ERR_STATE, "checkpoint_begin: level not defined");
+  if (s->ck_pending) return fail(s, COSAHIP_ERR_STATE, "checkpoint_begin: the previous checkpoint was not waited for");
+  if (!host_q || nq != field_size(s, *L, COSAHIP_Q))
+    return fail(s, COSAHIP_ERR_ARG, "checkpoint_begin: q count does not match the field size");
+  const bool want_mut = s->par.kom != 0;
+  if (want_mut && (!host_mut || nmut != field_size(s, *L, COSAHIP_MUT)))
+    return fail(s, COSAHIP_ERR_ARG, "checkpoint_begin: k-omega needs mut with the field's count");
+  if (!want_mut && (host_mut || nmut)) return fail(s, COSAHIP_ERR_ARG, "checkpoint_begin: no mut without k-omega");
+  CK(hipSetDevice(s->device));
+  if (!s->ck_stream) {
+    CK(hipStreamCreateWithFlags(&s->ck_stream, hipStreamNonBlocking));
+    CK(hipEventCreateWithFlags(&s->ck_snap, hipEventDisableTiming));
+    CK(hipEventCreateWithFlags(&s->ck_done, hipEventDisableTiming));
+  }
+  const ll need = nq + (want_mut ? nmut : 0);
+  if (need > s->ck_cap) {
+    if (s->ck_buf) CK(hipFree(s->ck_buf));
+    s->ck_buf = nullptr;
+    s->ck_cap = 0;
+    if (hipMalloc(&s->ck_buf, need * sizeof(double)) != hipSuccess)
+      return fail(s, COSAHIP_ERR_NOMEM, "checkpoint_begin: device allocation failed");
+    s->ck_cap = need;
+  }
+  CK(hipMemcpyAsync(s->ck_buf, L->f[COSAHIP_Q], nq * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  if (want_mut)
+    CK(hipMemcpyAsync(s->ck_buf + nq, L->f[COSAHIP_MUT], nmut * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  CK(hipEventRecord(s->ck_snap, s->stream));
+  CK(hipStreamWaitEvent(s->ck_stream, s->ck_snap, 0));
+  CK(hipMemcpyAsync(host_q, s->ck_buf, nq * sizeof(double), hipMemcpyDeviceToHost, s->ck_stream));
+  if (want_mut)
+    CK(hipMemcpyAsync(host_mut, s->ck_buf + nq, nmut * sizeof(double), hipMemcpyDeviceToHost, s->ck_stream));
+  CK(hipEventRecord(s->ck_done, s->ck_stream));
+  s->ck_pending = true;
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_checkpoint_wait(cosahip_solve* h) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  if (!s) return COSAHIP_ERR_ARG;
+  if (!s->ck_pending) return fail(s, COSAHIP_ERR_STATE, "checkpoint_wait: no checkpoint in flight");
+  CK(hipSetDevice(s->device));
+  s->ck_pending = false;
+  CK(hipEventSynchronize(s->ck_done));
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_host_alloc(int64_t bytes, void** out) {
+  if (!out || bytes <= 0) return COSAHIP_ERR_ARG;
+  *out = nullptr;
+  if (hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault) != hipSuccess) return COSAHIP_ERR_NOMEM;
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_host_free(void* p) {
+  if (p && hipHostFree(p) != hipSuccess) return COSAHIP_ERR_DEVICE;
+  return COSAHIP_OK;
+}
+
 extern "C" int cosahip_synchronize(cosahip_solve* h) {
   Solve* s = reinterpret_cast<Solve*>(h);
   if (!s) return COSAHIP_ERR_ARG;
@@ -1644,7 +1721,7 @@ extern "C" int cosahip_autotune(cosahip_solve* h, int reps, int32_t* choice) {
   if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "autotune: an enqueued cycle has not been waited for");
   if (reps < 1 || reps > 100) return fail(s, COSAHIP_ERR_ARG, "autotune: reps out of range 1..100");
   CK(hipSetDevice(s->device));
-  for (int m = 0; m < 4; m++)
+  for (int m = 0; m < 8; m++)
     if (choice) choice[m] = 0;
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
@@ -1678,6 +1755,27 @@ extern "C" int cosahip_autotune(cosahip_solve* h, int reps, int32_t* choice) {
     for (int mode = 2; mode <= last; mode++)
       if (t[mode] < t[L.cirs_mode]) L.cirs_mode = mode;
     if (choice) choice[nl - 1] = L.cirs_mode;
+  }
+  // inviscid residual kernel per level: timed into work1 (q is only read)
+  for (int nl = 1; nl <= (int)s->lev.size() && nl <= 3 && rc == COSAHIP_OK; nl++) {
+    Level& L = s->lev[nl - 1];
+    if (!L.nblocks || s->par.viscous || s->par.npde != 5 || s->split_residual || s->resid_force >= 0) {
+      if (choice) choice[4 + nl - 1] = L.nblocks ? (s->resid_force >= 0 ? s->resid_force : L.resid_mode) : -1;
+      continue;
+    }
+    const double* q = L.f[COSAHIP_Q];
+    float t[RESID_NVARIANT] = {};
+    const int keep = L.resid_mode;
+    for (int v = 0; v < RESID_NVARIANT && rc == COSAHIP_OK; v++) {
+      L.resid_mode = v;
+      rc = timed([&]() { return launch_residual(*s, L, q, L.work1); }, t[v]);
+    }
+    L.resid_mode = keep;
+    if (rc == COSAHIP_OK) {
+      for (int v = 0; v < RESID_NVARIANT; v++)
+        if (t[v] < t[L.resid_mode]) L.resid_mode = v;
+    }
+    if (choice) choice[4 + nl - 1] = L.resid_mode;
   }
   if (rc == COSAHIP_OK && s->par.viscous && !s->fuse_viscous && getenv("COSAHIP_VISCOUS") == nullptr &&
       !s->lev.empty() && s->lev[0].nblocks) {

@@ -94,7 +94,19 @@ enum TileKind {
   TK_FACES_I,     // i faces imax x (jmax-1) x (kmax-1)
   TK_FACES_J,     // j faces (imax-1) x jmax x (kmax-1)
   TK_FACES_K,     // k faces (imax-1) x (jmax-1) x kmax
+  TK_FUSED15,     // residual tiles of 31 x 15 cells x 32-plane k segments (k_residual_tile, TJ = 15)
   TK_NKIND
+};
+
+// inviscid residual kernels (the same bits; chosen per level by
+// cosahip_autotune or COSAHIP_RESIDUAL at create time)
+enum ResidVariant {
+  RESID_GENERIC = 0,   // k_residual_fused: run-time scheme flags, register k-window, 31 x 7 tiles
+  RESID_TILE7,         // k_residual_tile: specialised, k-window from LDS / L2, 31 x 7 tiles
+  RESID_TILE7_REG,     // k_residual_tile: specialised, register k-window, 31 x 7 tiles
+  RESID_TILE15,        // k_residual_tile: specialised, k-window from LDS / L2, 31 x 15 tiles
+  RESID_TILE7_W4,      // RESID_TILE7 held to 128 VGPRs (4 waves per SIMD)
+  RESID_NVARIANT
 };
 
 // a BC patch or one directional cut record, flattened for kernels
@@ -133,6 +145,7 @@ struct Level {
   double* work1 = nullptr;    // copy of res for the HB norm
   double* dqc = nullptr;      // SST: (P3, 9, nsnap) cell-centre velocity differences of the stage
   int cirs_mode = 0;          // CIRS i-sweep picked by cosahip_autotune: 0 none, 1 lds, 2 iv, 3 line
+  int resid_mode = RESID_GENERIC;   // inviscid residual kernel (ResidVariant)
   int il_lpw = 0;             // k_cirs_il: lines per work-group (0: lines too long for its LDS budget)
   int il_stride = 0;          // k_cirs_il: odd LDS row stride >= the longest i-line of the level
   double* il_rtab = nullptr;  // k_cirs_il: multipliers and reciprocal divisors, 2 il_stride per block
@@ -213,6 +226,7 @@ struct Solve {
   // residual kernel choice: fused single pass (default) or one pass per
   // direction (COSAHIP_RESIDUAL=split in the environment at create time)
   bool split_residual = false;
+  int resid_force = -1;   // COSAHIP_RESIDUAL=fused|tile7|tile7reg|tile15|tile7w4: that kernel on every level
   // viscous face fluxes after the inviscid pass of each direction: face
   // fluxes stored then differenced (default, measured fastest), flux and
   // difference in one marching kernel (COSAHIP_VISCOUS=march), or fused into
@@ -235,6 +249,13 @@ struct Solve {
   // (replayed while its key -- the residual precision -- is unchanged;
   // COSAHIP_GRAPH=0 at create time runs every cycle eagerly)
   bool cycle_pending = false;
+  // asynchronous checkpoint: a device snapshot of q (and mut) taken in
+  // stream order, copied to the host on a second stream
+  hipStream_t ck_stream = nullptr;
+  hipEvent_t ck_snap = nullptr, ck_done = nullptr;
+  double* ck_buf = nullptr;
+  ll ck_cap = 0;
+  bool ck_pending = false;
   int mgit = 1;                 // cycle counter of the last smooth (CFL ramping)
   bool use_graph = true;
   hipGraphExec_t gexec = nullptr;

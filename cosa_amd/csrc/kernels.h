@@ -48,6 +48,14 @@ struct ResIn {
 // ---- spectral radii, residual, smoothing (k_flux.hip) ----
 hipError_t launch_lambda_deltat(Solve& s, Level& L);
 hipError_t launch_residual(Solve& s, Level& L, const double* q, double* res);
+// the specialised inviscid residual (k_resid.hip), variant = ResidVariant
+// other than RESID_GENERIC; hipErrorInvalidValue for an unknown variant
+hipError_t launch_residual_tile(Solve& s, Level& L, int variant, const double* q, const double* si, const double* sj,
+                                const double* sk, double* res);
+hipError_t launch_residual_tile(Solve& s, Level& L, int variant, const float* q, const float* si, const float* sj,
+                                const float* sk, double* res);
+// work items of one block in the tile table of a TJ-row residual tile
+ll residual_tile_items(int TJ, int imax, int jmax, int kmax, int nsnap);
 hipError_t launch_cirs(Solve& s, Level& L);
 // vpre: viscous face fluxes of direction idir - 1 whose difference is added
 // to res first (nullptr: none)

@@ -18,6 +18,58 @@ from .case import Case
 from .config import komega, to_struct
 
 
+class PinnedArray:
+    """float64 array in pinned host memory (cosahip_host_alloc)."""
+
+    def __init__(self, lib, n: int):
+        self.lib, self.n = lib, int(n)
+        self.ptr = ctypes.c_void_p()
+        rc = lib.cosahip_host_alloc(max(self.n, 1) * 8, ctypes.byref(self.ptr))
+        if rc:
+            raise _lib.CosahipError(rc, "cosahip_host_alloc failed")
+        self.array = np.ctypeslib.as_array(ctypes.cast(self.ptr, ctypes.POINTER(ctypes.c_double)), (max(self.n, 1),))[:self.n]
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            self.array = None
+            self.lib.cosahip_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Checkpoint:
+    """A checkpoint in flight (Solve.checkpoint_async)."""
+
+    def __init__(self, solve: "Solve", path: str, nl: int):
+        self.solve, self.path, self.nl = solve, path, nl
+        kom = komega(solve.case.config)
+        self.q = PinnedArray(solve.lib, solve.size("q", nl))
+        self.mut = PinnedArray(solve.lib, solve.size("mut", nl)) if kom else None
+        mp = _lib.dptr(self.mut.array) if kom else None
+        solve._check(solve.lib.cosahip_checkpoint_begin(solve.h, nl, _lib.dptr(self.q.array), self.q.n, mp,
+                                                        self.mut.n if kom else 0), "checkpoint_begin")
+        self.done = False
+
+    def wait(self):
+        """Block until the snapshot is on the host, then write the file
+        (writerest layout); returns the path."""
+        from . import restart
+        if not self.done:
+            self.solve._check(self.solve.lib.cosahip_checkpoint_wait(self.solve.h), "checkpoint_wait")
+            restart.write_restart(self.path, self.solve.case, self.q.array,
+                                  self.mut.array if self.mut is not None else None, self.nl)
+            self.q.free()
+            if self.mut is not None:
+                self.mut.free()
+            self.done = True
+        return self.path
+
+
 class Solve:
     def __init__(self, case: Case, device: int = 0, q: Optional[np.ndarray] = None):
         self.case = case
@@ -147,6 +199,14 @@ class Solve:
         kom = komega(self.case.config)
         restart.write_restart(path, self.case, self.download("q", nl), self.download("mut", nl) if kom else None, nl)
 
+    def checkpoint_async(self, path: str, nl: int = 1) -> "Checkpoint":
+        """Start a restart file of the state as of every launch enqueued so
+        far, without stopping the solve (cosahip_checkpoint_begin): the
+        device snapshot is stream-ordered and the device-to-host copy runs
+        on the library's second stream while later cycles run.  The file is
+        written by ``wait()`` on the returned object."""
+        return Checkpoint(self, path, nl)
+
     def read_restart(self, path: str, nl: int = 1):
         """Load a restart file (readrest) into the resident state."""
         from . import restart
@@ -202,10 +262,13 @@ class Solve:
     def autotune(self, reps: int = 3) -> Dict[str, object]:
         """Time the interchangeable kernel arrangements on the current state
         and keep the fastest (cosahip_autotune); results are unchanged."""
-        ch = (ctypes.c_int32 * 4)()
+        ch = (ctypes.c_int32 * 8)()
         self._check(self.lib.cosahip_autotune(self.h, reps, ch), "autotune")
         names = {0: None, 1: "lds", 2: "per_variable", 3: "line_resident"}
-        return {"cirs_i_sweep": [names[ch[m]] for m in range(3)], "viscous_faces_stored": bool(ch[3])}
+        rnames = {-1: None, 0: "generic_fused", 1: "tile31x7", 2: "tile31x7_register_window", 3: "tile31x15",
+                  4: "tile31x7_128vgpr"}
+        return {"cirs_i_sweep": [names[ch[m]] for m in range(3)], "viscous_faces_stored": bool(ch[3]),
+                "residual": [rnames.get(ch[4 + m]) for m in range(3)]}
 
     # -- blocks as independent solves (serving batches) -----------------------
     def set_block_freestream(self, block: int, machfs: float, alpha_deg: float, beta_deg: float = 0.0):
@@ -249,15 +312,19 @@ class Solve:
         return self.cycle_wait()
 
     def mg(self, ncycle: Optional[int] = None, nstart: int = 1, npre: int = 1, npost: int = 1, ncrs: int = 1,
-           on_cycle=None) -> int:
+           on_cycle=None, hist=None) -> int:
         """The V-cycle schedule of mg (src/cosa.f90:806-930).  Records
-        resrms(1:8) of every cycle in ``history``; returns cycles run."""
+        resrms(1:8) of every cycle in ``history`` and, given a
+        cosa_amd.history.HistoryWriter, the reference's hist.dat lines;
+        returns cycles run."""
         nlevel = self.case.config["nlevel"]
         if nlevel > 3:
             raise ValueError("the reference's V-cycle has at most 3 levels")
         ncycle = ncycle or self.case.config["ncycle"]
         for mgit in range(1, ncycle + 1):
             done = self.cycle(mgit, nstart, npre, npost, ncrs)
+            if hist is not None:
+                hist.record(mgit, self.history[-1], done)
             if on_cycle:
                 on_cycle(mgit, self.history[-1])
             if done:

@@ -196,6 +196,23 @@ int cosahip_upload(cosahip_solve* s, int nl, int field, const double* host,
                    int64_t count);
 int cosahip_download(cosahip_solve* s, int nl, int field, double* host,
                      int64_t count);
+/* Checkpoint of the resident state while the solve goes on (the data of
+ * writerest, src/output.f90:3481-3566; the asynchronous copy has no
+ * counterpart in the reference): a device snapshot of level nl's q (and,
+ * k-omega, mut) is taken in stream order -- after every launch enqueued so
+ * far, before any enqueued later -- then copied to host_q / host_mut on a
+ * second stream of the library while the solve's stream goes on.  counts
+ * must equal cosahip_field_size; host_mut null and nmut 0 without
+ * k-omega.  Host buffers from cosahip_host_alloc (pinned) let the copy
+ * overlap the solve; any other host memory works but may serialise.
+ * cosahip_checkpoint_wait blocks until the host buffers hold the snapshot;
+ * one checkpoint per solve at a time.                                     */
+int cosahip_checkpoint_begin(cosahip_solve* s, int nl, double* host_q, int64_t nq, double* host_mut,
+                             int64_t nmut);
+int cosahip_checkpoint_wait(cosahip_solve* s);
+/* Pinned host memory for checkpoint buffers (hipHostMalloc / hipHostFree). */
+int cosahip_host_alloc(int64_t bytes, void** out);
+int cosahip_host_free(void* p);
 int cosahip_synchronize(cosahip_solve* s);
 
 /* ---- reference routines, whole level, in place on device state ------- */
@@ -295,9 +312,14 @@ int cosahip_cycle_ready(cosahip_solve* s);
  * the residual).  choice[0..2]: CIRS i-sweep of levels 1..3 (1 one thread
  * per line with LDS-staged chunks, 2 one thread per line and variable,
  * 3 whole lines resident in LDS -- irsop 1 only --, 0 level absent),
- * choice[3]: 1 stored face fluxes, 0 march.
+ * choice[3]: 1 stored face fluxes, 0 march; choice[4..6]: inviscid
+ * residual kernel of levels 1..3 (0 generic fused, 1 specialised 31x7
+ * tiles with the k-window from LDS/L2, 2 the same with a register
+ * k-window, 3 31x15 tiles, 4 as 1 held to 128 VGPRs; -1 level absent);
+ * choice[7] is reserved (0).  choice holds 8 entries.
  * Environment switches at create time take precedence:
- * COSAHIP_CIRS_I=lds|iv|line forces the i-sweep, COSAHIP_CIRS_IL_LDS sets
+ * COSAHIP_RESIDUAL=fused|tile7|tile7reg|tile15|tile7w4|split forces the
+ * inviscid residual kernel, COSAHIP_CIRS_I=lds|iv|line forces the i-sweep, COSAHIP_CIRS_IL_LDS sets
  * the LDS bytes per work-group of the line-resident sweep (4096..65536,
  * default 40960), COSAHIP_VISCOUS=march|fused the viscous arrangement.    */
 int cosahip_autotune(cosahip_solve* s, int reps, int32_t* choice);
