@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "kernels.h"
@@ -20,6 +21,8 @@ using namespace cosahip;
     hipError_t e_ = (call);                                                       \
     if (e_ != hipSuccess) return fail(s, COSAHIP_ERR_DEVICE, #call, hipGetErrorString(e_)); \
   } while (0)
+
+static hipError_t scratch_address(int device, double** out);
 
 static int fail(Solve* s, int code, const char* what, const char* detail = "") {
   if (s) {
@@ -273,6 +276,11 @@ int cosahip_create(const cosahip_config* cfg, int device, cosahip_solve** out) {
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return COSAHIP_ERR_DEVICE;
+  }
+  // the fixed scratch array of the no-allocation force path on this device
+  if (scratch_address(s->device, &s->scratch) != hipSuccess) {
+    fprintf(stderr, "cosahip_create: hipGetSymbolAddress of the force scratch array failed\n");
+    s->scratch = nullptr;
   }
   s->lev.resize(cfg->nlevel);
   *out = reinterpret_cast<cosahip_solve*>(s);
@@ -795,12 +803,11 @@ extern "C" int cosahip_upload(cosahip_solve* h, int nl, int field, const double*
 
 // Wall runs of surfaces 1..nsurface (types 1500 + s - 1, and type 14 as
 // surface 1 when include_inviscid) in the order force_b visits them: blocks,
-// then patches, then (i3, i2).
-static int build_wall_runs(Solve* s, Level& L, int nsurface, int include_inviscid) {
-  const int key = nsurface * 2 + (include_inviscid ? 1 : 0);
-  if (L.force_key == key) return COSAHIP_OK;
+// then patches, then (i3, i2).  emit(run) is called once per run, in order;
+// returns an error code, *nfaces the number of wall faces.
+template <typename Emit>
+static int enum_wall_runs(Solve* s, const Level& L, int nsurface, int include_inviscid, int* nfaces, Emit emit) {
   const Par& p = s->par;
-  std::vector<WallRun> runs;
   int nf = 0, off = 0;
   for (int b = 0; b < L.nblocks; b++) {
     for (int ib = 0; ib < L.nbcs[b]; ib++) {
@@ -830,18 +837,33 @@ static int build_wall_runs(Solve* s, Level& L, int nsurface, int include_invisci
       w.lo3 = lo[ic3 - 1];
       w.first = nf;
       w.count = n2 * n3;
-      runs.push_back(w);
+      int rc = emit(w);
+      if (rc) return rc;
       nf += n2 * n3;
     }
     off += L.nbcs[b];
   }
+  *nfaces = nf;
+  return COSAHIP_OK;
+}
+
+static int build_wall_runs(Solve* s, Level& L, int nsurface, int include_inviscid) {
+  const int key = nsurface * 2 + (include_inviscid ? 1 : 0);
+  if (L.force_key == key) return COSAHIP_OK;
+  std::vector<WallRun> runs;
+  int nf = 0;
+  int rc = enum_wall_runs(s, L, nsurface, include_inviscid, &nf, [&](const WallRun& w) {
+    runs.push_back(w);
+    return COSAHIP_OK;
+  });
+  if (rc) return rc;
   void* old[] = {L.dwall, L.fterms, L.fsums, L.fcoef, L.fgeom};
   for (void* q : old)
     if (q) CK(hipFree(q));
   L.dwall = nullptr;
   L.fterms = L.fsums = L.fcoef = L.fgeom = nullptr;
   L.force_key = -1;
-  const int nsnap = p.nsnap;
+  const int nsnap = s->par.nsnap;
   if (hipMalloc(&L.dwall, std::max<size_t>(runs.size(), 1) * sizeof(WallRun)) != hipSuccess ||
       hipMalloc(&L.fterms, std::max<ll>((ll)nf * nsnap * 12, 1) * sizeof(double)) != hipSuccess ||
       hipMalloc(&L.fsums, (ll)L.nblocks * nsnap * nsurface * 12 * sizeof(double)) != hipSuccess ||
@@ -855,24 +877,12 @@ static int build_wall_runs(Solve* s, Level& L, int nsurface, int include_invisci
   return COSAHIP_OK;
 }
 
-extern "C" int cosahip_force(cosahip_solve* h, int nl, int nsurface, int include_inviscid, const double* lngth,
-                             const double* momr, double* coef) {
-  Solve* s = reinterpret_cast<Solve*>(h);
-  Level* Lp = get_level(s, nl);
-  if (!Lp) return fail(s, COSAHIP_ERR_STATE, "force: level not defined");
-  CK(hipSetDevice(s->device));
-  Level& L = *Lp;
-  if (nsurface < 1 || nsurface > 100 || !lngth || !momr || !coef)
-    return fail(s, COSAHIP_ERR_ARG, "force: nsurface must be 1..100 and lngth, momr, coef non-null");
-  int rc = build_wall_runs(s, L, nsurface, include_inviscid);
-  if (rc) return rc;
-  const cosahip_config& c = s->cfg;
-  const int nsnap = s->par.nsnap;
-  // direction vectors of lift, drag and side force (src/output.f90:4451-4462)
+// direction vectors of lift, drag and side force (src/output.f90:4451-4462),
+// reference lengths, Mach number and moment centres: g[0 .. 13 + 3 nsnap)
+static void force_geometry(const cosahip_config& c, int nsnap, const double* lngth, const double* momr, double* g) {
   const double alpha = c.alpha, beta = c.beta;
   const double den = std::sqrt(std::sin(beta) * std::sin(beta) + std::cos(alpha) * std::cos(alpha) *
                                                                      (std::cos(beta) * std::cos(beta)));
-  std::vector<double> g(13 + 3 * (size_t)nsnap);
   g[0] = -std::sin(alpha) * std::cos(alpha) * (std::cos(beta) * std::cos(beta)) / den;
   g[1] = den;
   g[2] = std::sin(alpha) * std::sin(beta) * std::cos(beta) / den;
@@ -887,9 +897,124 @@ extern "C" int cosahip_force(cosahip_solve* h, int nl, int nsurface, int include
   g[11] = lngth[2];
   g[12] = c.machfs > 1e-14 ? c.machfs : 1.0;
   for (int m = 0; m < 3 * nsnap; m++) g[13 + m] = momr[m];
+}
+
+extern "C" int cosahip_force(cosahip_solve* h, int nl, int nsurface, int include_inviscid, const double* lngth,
+                             const double* momr, double* coef) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  Level* Lp = get_level(s, nl);
+  if (!Lp) return fail(s, COSAHIP_ERR_STATE, "force: level not defined");
+  CK(hipSetDevice(s->device));
+  Level& L = *Lp;
+  if (nsurface < 1 || nsurface > 100 || !lngth || !momr || !coef)
+    return fail(s, COSAHIP_ERR_ARG, "force: nsurface must be 1..100 and lngth, momr, coef non-null");
+  int rc = build_wall_runs(s, L, nsurface, include_inviscid);
+  if (rc) return rc;
+  const int nsnap = s->par.nsnap;
+  std::vector<double> g(13 + 3 * (size_t)nsnap);
+  force_geometry(s->cfg, nsnap, lngth, momr, g.data());
   CK(hipMemcpyAsync(L.fgeom, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
-  CK(launch_force(*s, L, nsurface, L.fgeom, L.fgeom + 13));
+  const ForceBufs fb{L.dwall, (int)L.hwall.size(), L.nwall_faces, L.fterms, L.fsums, L.fcoef, L.fgeom, L.fgeom + 13};
+  CK(launch_force(*s, L, nsurface, fb));
   CK(hipMemcpyAsync(coef, L.fcoef, (size_t)12 * nsnap * nsurface * sizeof(double), hipMemcpyDeviceToHost,
+                    s->stream));
+  CK(hipStreamSynchronize(s->stream));
+  return COSAHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// force coefficients without device allocation: the buffers of one force
+// evaluation live in a fixed device array of the library.  One array per
+// GPU, shared by every solve on that GPU and kept until the process ends;
+// calls that overlap on one GPU race on it.
+
+static constexpr long long FORCE_SCRATCH_DOUBLES = 1ll << 20;   // 8 MiB
+static __device__ __align__(256) double scratch[FORCE_SCRATCH_DOUBLES];
+static constexpr int MAX_DEVICES = 64;
+static double* g_scratch[MAX_DEVICES];   // device address of `scratch` on each GPU
+static std::mutex g_scratch_mu;
+
+// the address of this GPU's copy of `scratch` (the current device must be
+// `device`); looked up once per device at solve creation
+static hipError_t scratch_address(int device, double** out) {
+  if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch[device];
+  return hipSuccess;
+}
+
+extern "C" int cosahip_force_noalloc(cosahip_solve* h, int nl, int nsurface, int include_inviscid,
+                                     const double* lngth, const double* momr, double* coef) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  Level* Lp = get_level(s, nl);
+  if (!Lp) return fail(s, COSAHIP_ERR_STATE, "force_noalloc: level not defined");
+  if (nsurface < 1 || nsurface > 100 || !lngth || !momr || !coef)
+    return fail(s, COSAHIP_ERR_ARG, "force_noalloc: nsurface must be 1..100 and lngth, momr, coef non-null");
+  Level& L = *Lp;
+  if (!s->scratch) return fail(s, COSAHIP_ERR_STATE, "force_noalloc: no scratch address for this device");
+  CK(hipSetDevice(s->device));
+  const int nsnap = s->par.nsnap;
+  // count the runs and faces first: the layout must fit before anything is written
+  int nruns = 0, nf = 0;
+  int rc = enum_wall_runs(s, L, nsurface, include_inviscid, &nf, [&](const WallRun&) {
+    nruns++;
+    return COSAHIP_OK;
+  });
+  if (rc) return rc;
+  // layout in doubles, each part 256-byte aligned: runs, geometry, coefficients, sums, terms
+  auto up = [](ll n) { return (n + 31) / 32 * 32; };
+  const ll n_runs = up(((ll)nruns * (ll)sizeof(WallRun) + 7) / 8);
+  const ll n_geom = up(13 + 3 * (ll)nsnap);
+  const ll n_coef = up((ll)nsnap * nsurface * 12);
+  const ll n_sums = up((ll)L.nblocks * nsnap * nsurface * 12);
+  const ll n_terms = up((ll)nf * nsnap * 12);
+  const ll need = n_runs + n_geom + n_coef + n_sums + n_terms;
+  if (need > FORCE_SCRATCH_DOUBLES) {
+    char msg[160];
+    snprintf(msg, sizeof msg, "force_noalloc: needs %lld doubles of scratch, the maximum is %lld", need,
+             FORCE_SCRATCH_DOUBLES);
+    return fail(s, COSAHIP_ERR_ARG, msg);
+  }
+  double* base = s->scratch;
+  WallRun* d_runs = reinterpret_cast<WallRun*>(base);
+  double* d_geom = base + n_runs;
+  double* d_coef = d_geom + n_geom;
+  double* d_sums = d_coef + n_coef;
+  double* d_terms = d_sums + n_sums;
+  // runs through a small host buffer, a chunk at a time (no host or device allocation)
+  WallRun chunk[64];
+  int nc = 0, done = 0;
+  int nf2 = 0;
+  hipError_t err = hipSuccess;
+  auto flush = [&]() {
+    if (nc && err == hipSuccess) {
+      err = hipMemcpyAsync(d_runs + done, chunk, nc * sizeof(WallRun), hipMemcpyHostToDevice, s->stream);
+      if (err == hipSuccess) err = hipStreamSynchronize(s->stream);
+    }
+    done += nc;
+    nc = 0;
+  };
+  rc = enum_wall_runs(s, L, nsurface, include_inviscid, &nf2, [&](const WallRun& w) {
+    chunk[nc++] = w;
+    if (nc == 64) flush();
+    return COSAHIP_OK;
+  });
+  if (rc) return rc;
+  flush();
+  CK(err);
+  double g[13 + 3 * 131];
+  if (nsnap > 131) return fail(s, COSAHIP_ERR_ARG, "force_noalloc: at most 131 snapshots");
+  force_geometry(s->cfg, nsnap, lngth, momr, g);
+  CK(hipMemcpyAsync(d_geom, g, (13 + 3 * (size_t)nsnap) * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  const ForceBufs fb{d_runs, nruns, nf, d_terms, d_sums, d_coef, d_geom, d_geom + 13};
+  CK(launch_force(*s, L, nsurface, fb));
+  CK(hipMemcpyAsync(coef, d_coef, (size_t)12 * nsnap * nsurface * sizeof(double), hipMemcpyDeviceToHost,
                     s->stream));
   CK(hipStreamSynchronize(s->stream));
   return COSAHIP_OK;

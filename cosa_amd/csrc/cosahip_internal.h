@@ -206,6 +206,7 @@ struct Solve {
   cosahip_config cfg;
   Par par;
   int device = 0;
+  double* scratch = nullptr;   // this GPU's fixed force scratch array (cosahip_force_noalloc)
   hipStream_t stream = nullptr;
   std::vector<Level> lev;     // index nl-1
   double dhb[131 * 131];      // dhb(i,j) column-major

@@ -186,22 +186,21 @@ __global__ void __launch_bounds__(64) k_force_coef(int nblocks, int nsnap, int n
   for (int m = 0; m < 12; m++) o[m] = tot[m];
 }
 
-hipError_t launch_force(Solve& s, Level& L, int nsurface, const double* geom_dev, const double* momr_dev) {
+hipError_t launch_force(Solve& s, Level& L, int nsurface, const ForceBufs& fb) {
   const Par& p = s.par;
-  const int nruns = (int)L.hwall.size();
-  const ll nt = (ll)L.nwall_faces * p.nsnap;
-  if (nruns && nt)
-    hipLaunchKernelGGL(k_force_terms, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s.stream, p, L.dblk, L.dwall,
-                       nruns, L.nwall_faces, L.f[COSAHIP_Q], L.f[COSAHIP_X], L.f[COSAHIP_Y], L.f[COSAHIP_Z],
+  const ll nt = (ll)fb.nfaces * p.nsnap;
+  if (fb.nruns && nt)
+    hipLaunchKernelGGL(k_force_terms, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s.stream, p, L.dblk, fb.runs,
+                       fb.nruns, fb.nfaces, L.f[COSAHIP_Q], L.f[COSAHIP_X], L.f[COSAHIP_Y], L.f[COSAHIP_Z],
                        L.f[COSAHIP_XDOT], L.f[COSAHIP_YDOT], L.f[COSAHIP_ZDOT], L.f[COSAHIP_SI], L.f[COSAHIP_SJ],
-                       L.f[COSAHIP_SK], L.f[COSAHIP_XIDERI], L.f[COSAHIP_ETADERJ], L.f[COSAHIP_ZETADERK], momr_dev,
-                       L.fterms);
+                       L.f[COSAHIP_SK], L.f[COSAHIP_XIDERI], L.f[COSAHIP_ETADERJ], L.f[COSAHIP_ZETADERK], fb.momr,
+                       fb.terms);
   const int ns = L.nblocks * p.nsnap * nsurface * 12;
   hipLaunchKernelGGL(k_force_sums, dim3((ns + 63) / 64), dim3(64), 0, s.stream, L.nblocks, p.nsnap, nsurface,
-                     L.dwall, nruns, L.nwall_faces, L.fterms, L.fsums);
+                     fb.runs, fb.nruns, fb.nfaces, fb.terms, fb.sums);
   const int nc = p.nsnap * nsurface;
   hipLaunchKernelGGL(k_force_coef, dim3((nc + 63) / 64), dim3(64), 0, s.stream, L.nblocks, p.nsnap, nsurface,
-                     L.fsums, geom_dev, L.fcoef);
+                     fb.sums, fb.geom, fb.coef);
   return hipGetLastError();
 }
 

@@ -72,6 +72,16 @@ hipError_t launch_fixq(Solve& s, Level& L);
 hipError_t launch_turb_limits(Solve& s, Level& L, double ar);
 
 // ---- forces (k_force.hip) ----
-hipError_t launch_force(Solve& s, Level& L, int nsurface, const double* geom_dev, const double* momr_dev);
+// device buffers of one force evaluation: the wall-face runs, the per-face
+// terms (12, faces, 0:2N), per-block sums (12, nsurface, 0:2N, blocks), the
+// coefficients (12, 0:2N, nsurface) and the geometry (13 values, then the
+// moment centres (3, 0:2N))
+struct ForceBufs {
+  const WallRun* runs;
+  int nruns, nfaces;
+  double *terms, *sums, *coef;
+  const double *geom, *momr;
+};
+hipError_t launch_force(Solve& s, Level& L, int nsurface, const ForceBufs& fb);
 
 }  // namespace cosahip

@@ -126,6 +126,14 @@ module cosahip_binding
       real(c_double), intent(in) :: lngth(3), momr(*)
       real(c_double), intent(out) :: coef(*)
     end function
+    integer(c_int) function cosahip_force_noalloc(solve, nl, nsurface, include_inviscid, lngth, momr, coef) &
+        bind(C, name='cosahip_force_noalloc')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: solve
+      integer(c_int), value :: nl, nsurface, include_inviscid
+      real(c_double), intent(in) :: lngth(3), momr(*)
+      real(c_double), intent(out) :: coef(*)
+    end function
     integer(c_int) function cosahip_set_precision(solve, mode, tail_rms) bind(C, name='cosahip_set_precision')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: solve

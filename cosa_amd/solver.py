@@ -180,17 +180,19 @@ class Solve:
         self._check(self.lib.cosahip_prolong(self.h, nl), "prolong")
 
     def force(self, nsurface: int = 1, include_inviscid: bool = True, lngth=(1.0, 1.0, 1.0),
-              xyzmom=(0.25, 0.0, 0.0), nl: int = 1) -> np.ndarray:
+              xyzmom=(0.25, 0.0, 0.0), nl: int = 1, noalloc: bool = False) -> np.ndarray:
         """Force and moment coefficients (force_b): array (nsurface, nsnap,
         12) of cl_p, cd_p, cz_p, cmx_p, cmy_p, cm_p, cl_v, cd_v, cz_v, cmx_v,
         cmy_v, cm_v.  Q must be conservative with its halos filled (between
-        smooth calls it is)."""
+        smooth calls it is).  noalloc: through cosahip_force_noalloc (the
+        library's fixed scratch array instead of device allocations)."""
         ns = self.case.nsnap
         momr = np.ascontiguousarray(self.case.moment_centres(xyzmom), np.float64)
         lg = np.ascontiguousarray(lngth, np.float64)
         out = np.zeros(12 * ns * nsurface)
-        self._check(self.lib.cosahip_force(self.h, nl, nsurface, int(include_inviscid), _lib.dptr(lg),
-                                           _lib.dptr(momr), _lib.dptr(out)), "force")
+        fn = self.lib.cosahip_force_noalloc if noalloc else self.lib.cosahip_force
+        self._check(fn(self.h, nl, nsurface, int(include_inviscid), _lib.dptr(lg), _lib.dptr(momr), _lib.dptr(out)),
+                    "force_noalloc" if noalloc else "force")
         return out.reshape(nsurface, ns, 12)
 
     def write_restart(self, path: str, nl: int = 1):

@@ -359,6 +359,20 @@ int cosahip_force_blocks(cosahip_solve* s, int nl, int nsurface, int include_inv
  * cmx_p, cmy_p, cm_p, cl_v, cd_v, cz_v, cmx_v, cmy_v, cm_v.                */
 int cosahip_force(cosahip_solve* s, int nl, int nsurface, int include_inviscid,
                   const double* lngth, const double* momr, double* coef);
+/* cosahip_force without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as cosahip_force, but the wall-face
+ * runs, per-face terms, per-block sums, coefficients and geometry live in a
+ * fixed device array of the library, 8 MiB (1,048,576 doubles) per GPU,
+ * held until the process ends; it calls no hipMalloc / hipFree.  The array
+ * needs 12 * faces * (2N+1) + 12 * blocks * (2N+1) * nsurface + 12 * (2N+1)
+ * * nsurface + 13 + 3 * (2N+1) doubles plus the runs (each part rounded up
+ * to 32 doubles); a larger request returns COSAHIP_ERR_ARG.  Every solve on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (solves on
+ * other host threads) race on it.                                         */
+int cosahip_force_noalloc(cosahip_solve* s, int nl, int nsurface, int include_inviscid,
+                          const double* lngth, const double* momr, double* coef);
 
 /* ---- precision (no counterpart in the reference, which computes in
  * real(kind=cosa_real), FP64, throughout: src/cosa_precision.f90:17) -----
