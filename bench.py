@@ -77,10 +77,10 @@ BYTES_PER_CELL_RANS = {  # Wilcox k-omega, npde = 7
 }
 # HBM bytes per cell-update of the residual pass measured with rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE (scaled by a plain-copy calibration: FETCH x2.00,
-# WRITE x1.00): profiles/residual_uneven96_fp64_hbm_traffic.md and
-# profiles/residual_wing4_rans_fp64_hbm_traffic.md.  Re-measure when the
+# WRITE x1.00) on the level-1 kernel: profiles/residual_uneven96_fp64_hbm_traffic.md
+# and profiles/residual_wing4_rans_fp64_hbm_traffic.md.  Re-measure when the
 # kernels change.
-MEASURED_TRAFFIC_PER_CELL = {("uneven96", "fp64", "residual"): 322.7, ("wing4", "fp64", "residual"): 2290.0}
+MEASURED_TRAFFIC_PER_CELL = {("uneven96", "fp64", "residual"): 472.8, ("wing4", "fp64", "residual"): 2290.0}
 
 
 def log(msg):
@@ -256,6 +256,10 @@ def measure(workload: str, steps: int, warmup: int, precision: str, tail_rms: fl
         if prec_mode and s.precision_in_use() == 32:
             bpc_tab = dict(bpc_tab, **(BYTES_PER_CELL_RANS_FP32_RESIDUAL if komega(case.config)
                                        else BYTES_PER_CELL_FP32_RESIDUAL))
+        if fam.get("update", {}).get("launches", 0) == 0 and not komega(case.config):
+            # the RK update runs inside the last CIRS sweep: that sweep reads
+            # qold and writes q instead of writing res for the update to read
+            bpc_tab = dict(bpc_tab, cirs=BYTES_PER_CELL_EULER["cirs"] + BYTES_PER_CELL_EULER["update"] - 80.0)
         fams = {}
         for name, d in fam.items():
             bpc = bpc_tab.get(name)

@@ -93,6 +93,10 @@ def load() -> ctypes.CDLL:
         "cosahip_set_precision": ([_VP, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "cosahip_force": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP], ctypes.c_int),
         "cosahip_force_noalloc": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP], ctypes.c_int),
+        "cosahip_grid_metrics": ([_VP, ctypes.c_int], ctypes.c_int),
+        "cosahip_grid_coarsen": ([_VP, ctypes.c_int], ctypes.c_int),
+        "cosahip_grid_pitch": ([_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                _DP], ctypes.c_int),
         "cosahip_cycle_enqueue": ([_VP, ctypes.c_int], ctypes.c_int),
         "cosahip_cycle_wait": ([_VP, _IP, _DP], ctypes.c_int),
         "cosahip_cycle_ready": ([_VP], ctypes.c_int),
@@ -130,6 +134,7 @@ def exported_symbols():
             "cosahip_cycle_ready", "cosahip_autotune", "cosahip_set_block_freestream", "cosahip_set_block_rms",
             "cosahip_block_rms", "cosahip_force_blocks",
             "cosahip_force", "cosahip_force_noalloc", "cosahip_set_precision",
+            "cosahip_grid_metrics", "cosahip_grid_coarsen", "cosahip_grid_pitch",
             "cosahip_precision_in_use",
             "cosahip_build_info", "cosahip_timing_enable", "cosahip_timing_read", "cosahip_timing_reset"]
 

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcosahip.so")
-SOURCES = ["api.hip", "k_core.hip", "k_flux.hip", "k_resid.hip", "k_turb.hip", "k_force.hip"]
+SOURCES = ["api.hip", "k_core.hip", "k_flux.hip", "k_resid.hip", "k_turb.hip", "k_force.hip", "k_grid.hip"]
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-result"]

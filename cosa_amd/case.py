@@ -78,6 +78,21 @@ class Case:
         t = 2.0 * math.pi * n / ((2 * self.config["nharms"] + 1) * om)
         return m["dtheta0"] * math.sin(om * t), m["dtheta0"] * om * math.cos(om * t)
 
+    def pitch_poses(self) -> np.ndarray:
+        """Per harmonic-balance snapshot n: dcos(dtheta(n)), dsin(dtheta(n)),
+        dcos(omegas zeit(n)) with zeit(n) = n dt, dt = 2 pi / omegas / (2N+1)
+        (src/input.f90:3187-3190) and dtheta(n) = dtheta0 dsin(omegas zeit(n))
+        (src/grid.f90:4540): the constants pitch uses (src/grid.f90:1921-1958)."""
+        m, om = self.motion, self.config["omega"]
+        ns = 2 * self.config["nharms"] + 1
+        dt = 2 * math.pi / om / ns
+        out = []
+        for n in range(self.nsnap_met):
+            zeit = n * dt
+            dth = m["dtheta0"] * math.sin(om * zeit)
+            out += [math.cos(dth), math.sin(dth), math.cos(om * zeit)]
+        return np.array(out)
+
     def moment_centres(self, xyzmom=(0.25, 0.0, 0.0)) -> np.ndarray:
         """(xmomr, ymomr, zmomr) of every snapshot: the moment reference
         point carried with the pitching body as force_b computes it

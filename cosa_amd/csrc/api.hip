@@ -89,13 +89,18 @@ static void free_level(Level& L) {
   void* ptrs[] = {L.dblk, L.dq, L.work1, L.cirs_r, L.cirs_rt, L.rms_part, L.sbc, L.sbc_doff, L.dpatch, L.dcut,
                   L.vflux, L.tbeta[0], L.tbeta[1], L.tbeta[2], L.h0tab, L.dxd, L.q32, L.m32[0], L.m32[1], L.m32[2],
                   L.dxd32, L.mut32, L.dwall, L.fterms, L.fsums, L.fcoef, L.fgeom, L.dqc, L.il_rtab, L.cmet,
-                  L.cmet32};
+                  L.cmet32, L.ptile, L.jk_rtab[0], L.jk_rtab[1]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (float* p : L.xd32)
     if (p) (void)hipFree(p);
   for (int k = 0; k < TK_NKIND; k++)
     if (L.tile[k]) (void)hipFree(L.tile[k]);
+  for (int k = 0; k < GT_NKIND; k++)
+    if (L.gt.t[k]) (void)hipFree(L.gt.t[k]);
+  void* gptrs[] = {L.gt.cut_v, L.gt.cut_x, L.xbase, L.gpose, L.gbad};
+  for (void* p : gptrs)
+    if (p) (void)hipFree(p);
   L = Level();
 }
 
@@ -112,6 +117,7 @@ static Par make_par(const cosahip_config& c) {
   p.irsop = c.irsop;
   p.nl_crs = c.nl_crs;
   p.gamma = c.gamma;
+  p.gam_gm1 = c.gamma / (c.gamma - 1);
   p.rkap = c.rkap;
   p.epslimf = c.epslimf;
   p.cfl = c.cfl;
@@ -255,6 +261,8 @@ int cosahip_create(const cosahip_config* cfg, int device, cosahip_solve** out) {
     s->resid_force = strcmp(rk, "fused") == 0 ? RESID_GENERIC : strcmp(rk, "tile7") == 0 ? RESID_TILE7
                      : strcmp(rk, "tile7reg") == 0 ? RESID_TILE7_REG : strcmp(rk, "tile15") == 0 ? RESID_TILE15
                      : strcmp(rk, "tile7w4") == 0 ? RESID_TILE7_W4 : -1;
+  const char* fk = getenv("COSAHIP_FUSED_UPDATE");
+  s->fuse_update = !(fk && strcmp(fk, "0") == 0);
   const char* gk = getenv("COSAHIP_GRAPH");
   s->use_graph = !(gk && strcmp(gk, "0") == 0);
   const char* ck = getenv("COSAHIP_CIRS_I");
@@ -402,6 +410,36 @@ static int upload_cirs_il_table(Solve* s, Level& L) {
   return COSAHIP_OK;
 }
 
+// The same tables for the j and k line sweeps of k_cirs_jk (irsop 1, lines
+// up to JK_RT - 1 cells): block b of direction d at 2 JK_RT b, r(a) at a,
+// 1/d(a) at JK_RT + a, a = 1..L.
+static int upload_cirs_jk_tables(Solve* s, Level& L) {
+  for (int d = 0; d < 2; d++) {
+    if (L.jk_rtab[d]) (void)hipFree(L.jk_rtab[d]);
+    L.jk_rtab[d] = nullptr;
+  }
+  if (s->par.irsop != 1) return COSAHIP_OK;
+  const double epsc = 0.25 * (s->par.rat * s->par.rat - 1.0);
+  for (int d = 0; d < 2; d++) {
+    std::vector<double> tab((size_t)L.nblocks * 2 * JK_RT, 0.0);
+    for (int b = 0; b < L.nblocks; b++) {
+      const int n = (d == 0 ? L.hblk[b].jmax : L.hblk[b].kmax) - 1;
+      if (n >= JK_RT) continue;
+      double rp = 0.0;
+      for (int a = 1; a <= n; a++) {
+        const double ca = a == 1 ? 0.0 : -epsc, cb = 1 + 2 * epsc, cc = a == n ? 0.0 : -epsc;
+        const double dd = a == 1 ? cb : cb - rp * ca;
+        const double r = a == 1 ? cc / cb : cc / (cb - rp * ca);
+        tab[2 * JK_RT * b + a] = r;
+        tab[2 * JK_RT * b + JK_RT + a] = 1.0 / dd;
+        rp = r;
+      }
+    }
+    CK(upload_vec(&L.jk_rtab[d], tab));
+  }
+  return COSAHIP_OK;
+}
+
 static int build_tables(Solve* s, Level& L, const Level* Lc) {
   const int ns = s->par.nsnap;
   for (int k = 0; k < TK_NKIND; k++) L.htile[k].clear();
@@ -441,6 +479,8 @@ static int build_tables(Solve* s, Level& L, const Level* Lc) {
     L.ntile[k] = (int)L.htile[k].size();
     CK(upload_vec(&L.tile[k], L.htile[k]));
   }
+  int rc = upload_cirs_jk_tables(s, L);
+  if (rc) return rc;
   return upload_cirs_il_table(s, L);
 }
 
@@ -685,6 +725,15 @@ extern "C" int cosahip_define_level(cosahip_solve* h, int nl, int nblocks, const
   }
   CK(upload_vec(&L.dblk, L.hblk));
   CK(upload_vec(&L.dpatch, L.hpatch));
+  // one work tile per TS face items of each patch (all snapshots), so a BC
+  // launch has exactly the work-groups its patches need
+  L.hptile.clear();
+  L.ptile_first.assign(1, 0);
+  for (size_t ip = 0; ip < L.hpatch.size(); ip++) {
+    add_tiles(L.hptile, (int)ip, (ll)L.hpatch[ip].count * s->par.nsnap, TS);
+    L.ptile_first.push_back((int)L.hptile.size());
+  }
+  CK(upload_vec(&L.ptile, L.hptile));
   CK(upload_vec(&L.dcut, L.hcut));
   CK(upload_vec(&L.sbc, sbc));
   CK(upload_vec(&L.sbc_doff, L.sbc_off));
@@ -798,6 +847,207 @@ extern "C" int cosahip_upload(cosahip_solve* h, int nl, int field, const double*
     CK(hipGraphExecDestroy(s->gexec));
     s->gexec = nullptr;
   }
+  return COSAHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// grid motion and metric set-up on the device (src/cosa.f90:297-322 per
+// level: movegrid, vols, imetrix, volex + cutman_v, vol_edges, coordex +
+// cutman_x, vmetrix; coarsen, src/cosa.f90:160-166)
+
+// cut_v / cut_x records in the flat form k_cut_layer reads (src/grid.f90:
+// 612-838, 242-524): ranges, running directions and the halo / interior
+// indices of either kind
+static GeomCut geom_cut(const CutRec& r, const Level& L, int kind) {
+  static const int CYC[3][3] = {{1, 2, 3}, {2, 3, 1}, {3, 1, 2}};
+  const int32_t* t = r.t;
+  const Blk& B1 = L.hblk[r.b1];
+  const Blk& B2 = L.hblk[r.b2];
+  const int ijk1[3] = {B1.imax, B1.jmax, B1.kmax}, ijk2[3] = {B2.imax, B2.jmax, B2.kmax};
+  const int idir1 = t[1], inout1 = t[2], idir2 = t[10], inout2 = t[11];
+  int str1[3] = {t[3], t[5], t[7]}, end1[3] = {t[4], t[6], t[8]};
+  int str2[3] = {t[12], t[14], t[16]}, end2[3] = {t[13], t[15], t[17]};
+  GeomCut g;
+  g.b1 = r.b1;
+  g.b2 = r.b2;
+  if (kind == 0) {
+    g.ibcpt = inout1 == 1 ? 0 : ijk1[idir1 - 1];
+    g.inr = inout2 == 1 ? 1 : ijk2[idir2 - 1] - 1;
+  } else {
+    g.ibcpt = inout1 == 1 ? 0 : ijk1[idir1 - 1] + 1;
+    g.inr = inout2 == 1 ? 2 : ijk2[idir2 - 1] - 1;
+    for (int l = 1; l <= 3; l++) {
+      if (l != idir1) (end1[l - 1] > str1[l - 1] ? end1[l - 1] : str1[l - 1]) += 1;
+      if (l != idir2) (end2[l - 1] > str2[l - 1] ? end2[l - 1] : str2[l - 1]) += 1;
+    }
+  }
+  int len[3];
+  for (int l = 0; l < 3; l++) {
+    len[l] = std::abs(end1[l] - str1[l]);
+    g.str1[l] = str1[l];
+    g.str2[l] = str2[l];
+    g.sg1[l] = end1[l] > str1[l] ? 1 : -1;
+    g.sg2[l] = end2[l] > str2[l] ? 1 : -1;
+  }
+  g.ic1 = CYC[idir1 - 1][0] - 1;
+  g.ic2 = CYC[idir1 - 1][1] - 1;
+  g.ic3 = CYC[idir1 - 1][2] - 1;
+  g.jc1 = t[18 + g.ic1] - 1;
+  g.jc2 = t[18 + g.ic2] - 1;
+  g.jc3 = t[18 + g.ic3] - 1;
+  g.n2 = len[g.ic2] + 1;
+  g.count = g.n2 * (len[g.ic3] + 1);
+  return g;
+}
+
+// every index k_cut_layer forms must lie inside its block's extents (P1
+// cells 0..imax for kind 0, P2 nodes 0..imax+1 for kind 1)
+static bool geom_cut_in_bounds(const GeomCut& g, const Level& L, int kind) {
+  const Blk& B1 = L.hblk[g.b1];
+  const Blk& B2 = L.hblk[g.b2];
+  const int hi1[3] = {B1.imax + kind, B1.jmax + kind, B1.kmax + kind};
+  const int hi2[3] = {B2.imax + kind, B2.jmax + kind, B2.kmax + kind};
+  const int n3 = g.count / g.n2;
+  for (int corner = 0; corner < 4; corner++) {
+    const int i2 = (corner & 1) ? g.n2 - 1 : 0, i3 = (corner & 2) ? n3 - 1 : 0;
+    int p1[3], p2[3];
+    p1[g.ic1] = g.ibcpt;
+    p1[g.ic2] = g.str1[g.ic2] + g.sg1[g.ic2] * i2;
+    p1[g.ic3] = g.str1[g.ic3] + g.sg1[g.ic3] * i3;
+    p2[g.jc1] = g.inr;
+    p2[g.jc2] = g.str2[g.jc2] + g.sg2[g.jc2] * i2;
+    p2[g.jc3] = g.str2[g.jc3] + g.sg2[g.jc3] * i3;
+    for (int d = 0; d < 3; d++)
+      if (p1[d] < 0 || p1[d] > hi1[d] || p2[d] < 0 || p2[d] > hi2[d]) return false;
+  }
+  return true;
+}
+
+static int build_grid_tiles(Solve* s, Level& L) {
+  GridTiles& gt = L.gt;
+  if (gt.built) return COSAHIP_OK;
+  const int nsm = s->par.nsnap_met;
+  for (int b = 0; b < L.nblocks; b++) {
+    const Blk& B = L.hblk[b];
+    const ll I = B.imax, J = B.jmax, K = B.kmax;
+    add_tiles(gt.h[GT_CELLS], b, (I - 1) * (J - 1) * (K - 1), TS);
+    add_tiles(gt.h[GT_FI], b, nsm * I * (J - 1) * (K - 1), TS);
+    add_tiles(gt.h[GT_FJ], b, nsm * (I - 1) * J * (K - 1), TS);
+    add_tiles(gt.h[GT_FK], b, nsm * (I - 1) * (J - 1) * K, TS);
+    add_tiles(gt.h[GT_VOLEX], b, 2 * ((I - 1) * (K - 1) + (J - 1) * (K - 1) + (I - 1) * (J - 1)), TS);
+    add_tiles(gt.h[GT_EDGES], b, 4 * ((K - 1) + (J - 1) + (I - 1)), TS);
+    add_tiles(gt.h[GT_NODEHALO], b, nsm * 2 * (I * K + J * K + I * J), TS);
+    add_tiles(gt.h[GT_NODES], b, nsm * I * J * K, TS);
+    add_tiles(gt.h[GT_NODES0], b, I * J * K, TS);
+  }
+  for (int k = 0; k < GT_NKIND; k++) {
+    gt.n[k] = (int)gt.h[k].size();
+    CK(upload_vec(&gt.t[k], gt.h[k]));
+  }
+  std::vector<GeomCut> cv, cx;
+  for (const CutRec& r : L.hcut) {
+    cv.push_back(geom_cut(r, L, 0));
+    cx.push_back(geom_cut(r, L, 1));
+    if (!geom_cut_in_bounds(cv.back(), L, 0) || !geom_cut_in_bounds(cx.back(), L, 1))
+      return fail(s, COSAHIP_ERR_ARG, "grid: a cut record maps outside a block");
+    gt.maxcut_v = std::max(gt.maxcut_v, cv.back().count);
+    gt.maxcut_x = std::max(gt.maxcut_x, cx.back().count);
+  }
+  gt.ncut = (int)cv.size();
+  CK(upload_vec(&gt.cut_v, cv));
+  CK(upload_vec(&gt.cut_x, cx));
+  if (!L.gbad) CK(hipMalloc(&L.gbad, sizeof(int)));
+  gt.built = true;
+  return COSAHIP_OK;
+}
+
+// the metrics of a level changed: derived device data and a captured cycle
+// graph are stale
+static int metrics_changed(Solve* s, Level& L) {
+  L.m32_ok = false;
+  L.xd32_ok = false;
+  L.cmet_ok = false;
+  L.cmet32_ok = false;
+  if (s->gexec) {
+    CK(hipGraphExecDestroy(s->gexec));
+    s->gexec = nullptr;
+  }
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_grid_metrics(cosahip_solve* h, int nl) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  Level* Lp = get_level(s, nl);
+  if (!Lp) return fail(s, COSAHIP_ERR_STATE, "grid_metrics: level not defined");
+  if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "grid_metrics: a cycle is in flight");
+  CK(hipSetDevice(s->device));
+  Level& L = *Lp;
+  int rc = build_grid_tiles(s, L);
+  if (rc) return rc;
+  const cosahip_config& c = s->cfg;
+  const int do_volex = c.viscous || c.nlevel > 1;
+  const int do_bedges = nl < c.nlevel && !c.bln_prol;
+  const int do_vmetrix = c.viscous;
+  CK(hipMemsetAsync(L.gbad, 0, sizeof(int), s->stream));
+  CK(launch_grid_metrics(*s, L, L.gt, L.gbad, do_volex, do_bedges, do_vmetrix));
+  int bad = 0;
+  CK(hipMemcpyAsync(&bad, L.gbad, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  CK(hipStreamSynchronize(s->stream));
+  rc = metrics_changed(s, L);
+  if (rc) return rc;
+  if (bad) return fail(s, COSAHIP_ERR_ARG, "grid_metrics: a face area is not positive (imetrix_b stops here)");
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_grid_coarsen(cosahip_solve* h, int nl) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  Level* F = get_level(s, nl);
+  Level* C = get_level(s, nl + 1);
+  if (!F || !C) return fail(s, COSAHIP_ERR_STATE, "grid_coarsen: levels nl and nl + 1 must be defined");
+  if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "grid_coarsen: a cycle is in flight");
+  if (F->nblocks != C->nblocks) return fail(s, COSAHIP_ERR_ARG, "grid_coarsen: block counts differ");
+  for (int b = 0; b < F->nblocks; b++) {
+    const Blk& fb = F->hblk[b];
+    const Blk& cb = C->hblk[b];
+    if (2 * cb.imax - 1 > fb.imax || 2 * cb.jmax - 1 > fb.jmax || 2 * cb.kmax - 1 > fb.kmax)
+      return fail(s, COSAHIP_ERR_ARG, "grid_coarsen: a coarse block is not the fine block's every second node");
+  }
+  CK(hipSetDevice(s->device));
+  int rc = build_grid_tiles(s, *C);
+  if (rc) return rc;
+  CK(launch_grid_coarsen(*s, *F, *C, C->gt));
+  CK(hipStreamSynchronize(s->stream));
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_grid_pitch(cosahip_solve* h, int nl, double dtheta0, double omegas, double xh, double yh,
+                                  const double* pose) {
+  Solve* s = reinterpret_cast<Solve*>(h);
+  Level* Lp = get_level(s, nl);
+  if (!Lp) return fail(s, COSAHIP_ERR_STATE, "grid_pitch: level not defined");
+  if (!s->par.moving) return fail(s, COSAHIP_ERR_ARG, "grid_pitch: the configuration is not moving");
+  if (!pose) return fail(s, COSAHIP_ERR_ARG, "grid_pitch: pose is null");
+  if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "grid_pitch: a cycle is in flight");
+  CK(hipSetDevice(s->device));
+  Level& L = *Lp;
+  int rc = build_grid_tiles(s, L);
+  if (rc) return rc;
+  const int nsm = s->par.nsnap_met;
+  if (!L.xbase) {
+    // the base pose: snapshot 0 of X, Y, Z as first uploaded (x0 of movegrid)
+    CK(hipMalloc(&L.xbase, 3 * L.n2tot * sizeof(double)));
+    const int F[3] = {COSAHIP_X, COSAHIP_Y, COSAHIP_Z};
+    for (int m = 0; m < 3; m++)
+      for (int b = 0; b < L.nblocks; b++) {
+        const Blk& B = L.hblk[b];
+        CK(hipMemcpyAsync(L.xbase + m * L.n2tot + B.o2, L.f[F[m]] + B.o2 * nsm, n2of(B) * sizeof(double),
+                          hipMemcpyDeviceToDevice, s->stream));
+      }
+  }
+  if (!L.gpose) CK(hipMalloc(&L.gpose, 3 * (size_t)nsm * sizeof(double)));
+  CK(hipMemcpyAsync(L.gpose, pose, 3 * (size_t)nsm * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  CK(launch_grid_pitch(*s, L, L.gt, L.gpose, dtheta0, omegas, xh, yh, L.xbase));
+  CK(hipStreamSynchronize(s->stream));
   return COSAHIP_OK;
 }
 
@@ -1563,6 +1813,11 @@ static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter
   if (!async) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->stream, s->dflag + (nl - 1), NOFLAG, 1);
   int seq = 0;
   bool returned_early = false;
+  // the RK update folded into the last CIRS sweep, and with it the next
+  // stage's cons2prim (mean flow with cirs: the state is the same bits)
+  const bool fuse_update = s->fuse_update && !komega(c) && s->par.irsop >= 1 && s->par.irsop != 4 &&
+                           s->par.irsop <= 6 && c.cutcirs != 1 && !(c.harbal && c.rkimp);
+  bool q_prim = false;   // the previous stage left q in primitive variables
   for (int iter = 1; iter <= niterp && !returned_early; iter++) {
     const ll cs = L.ncell * s->par.nsnap;
     if (iter <= niter) {
@@ -1570,10 +1825,11 @@ static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter
       CK(launch_copy_interior(*s, L, L.f[COSAHIP_QOLD], q));
     }
     for (int irk = 1; irk <= c.nstage; irk++, seq++) {
-      {
+      if (!q_prim) {
         TimeScope t(*s, COSAHIP_FAM_CONVERT, cs);
         CK(launch_cons2prim(*s, L, q, true));
       }
+      q_prim = false;
       {
         TimeScope t(*s, COSAHIP_FAM_HALO, cs);
         CK(launch_bc(*s, L, q));
@@ -1634,13 +1890,23 @@ static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter
         returned_early = true;
         break;
       }
-      {
+      if (fuse_update) {
+        // the last stage leaves q conservative: the next iteration's qold
+        // copy and the transfers between levels read it
+        const int prim = irk < c.nstage;
+        const CirsUpdate up{q, L.f[COSAHIP_QOLD], prim && s->fp32() ? L.q32 : nullptr, c.alphas[irk - 1], prim};
         TimeScope t(*s, COSAHIP_FAM_CIRS, cs);
-        CK(launch_cirs(*s, L));
-      }
-      {
-        TimeScope t(*s, COSAHIP_FAM_UPDATE, cs);
-        CK(launch_update(*s, L, c.alphas[irk - 1]));
+        CK(launch_cirs(*s, L, &up));
+        q_prim = prim != 0;
+      } else {
+        {
+          TimeScope t(*s, COSAHIP_FAM_CIRS, cs);
+          CK(launch_cirs(*s, L));
+        }
+        {
+          TimeScope t(*s, COSAHIP_FAM_UPDATE, cs);
+          CK(launch_update(*s, L, c.alphas[irk - 1]));
+        }
       }
       if (komega(c)) {
         TimeScope t(*s, COSAHIP_FAM_OTHER, cs);

@@ -33,6 +33,7 @@ struct Par {
   int npde, nsnap, nsnap_met, lmet, hbmove, moving;
   int ilimf, irsop, nl_crs;
   double gamma, rkap, epslimf;
+  double gam_gm1;   // gamma / (gamma - 1), the same correctly rounded quotient the flux evaluates
   double cfl, rat, psirs;
   int efx_setup_f, efx_all_f, eig_limit_f, eig_lim_all_f;
   int entfx_f_w[3];
@@ -134,6 +135,41 @@ struct WallRun {
   int first, count;   // face range in the term array
 };
 
+// grid set-up work lists of one level (k_grid.hip): tile tables of the
+// face families, cells, halo cells, edge cells and nodes, and the cut
+// records of cut_v / cut_x in a flat form
+enum GridTileKind {
+  GT_CELLS = 0,   // interior cells, snapshot 0
+  GT_FI,          // i faces x snapshots of the metrics
+  GT_FJ,          // j faces x snapshots
+  GT_FK,          // k faces x snapshots
+  GT_VOLEX,       // face-halo cells of vol
+  GT_EDGES,       // edge-halo cells of vol (12 edges)
+  GT_NODEHALO,    // face-halo nodes x snapshots
+  GT_NODES,       // nodes 1..imax x 1..jmax x 1..kmax x snapshots
+  GT_NODES0,      // nodes 1..imax x 1..jmax x 1..kmax, snapshot 0
+  GT_NKIND
+};
+struct GeomCut {
+  int b1, b2;                  // 0-based blocks
+  int ibcpt, inr;              // halo index of block 1, interior index of block 2
+  int ic1, ic2, ic3, jc1, jc2, jc3;   // 0-based directions
+  int str1[3], sg1[3], str2[3], sg2[3];
+  int count, n2;
+};
+struct GridPtrs {
+  const double *x, *y, *z, *xd, *yd, *zd;
+};
+struct GridTiles {
+  bool built = false;
+  std::vector<Tile> h[GT_NKIND];
+  Tile* t[GT_NKIND] = {};
+  int n[GT_NKIND] = {};
+  GeomCut* cut_v = nullptr;
+  GeomCut* cut_x = nullptr;
+  int ncut = 0, maxcut_v = 1, maxcut_x = 1;
+};
+
 struct Level {
   int nl = 0, nblocks = 0, ncuts = 0;
   std::vector<Blk> hblk;
@@ -149,6 +185,7 @@ struct Level {
   int il_lpw = 0;             // k_cirs_il: lines per work-group (0: lines too long for its LDS budget)
   int il_stride = 0;          // k_cirs_il: odd LDS row stride >= the longest i-line of the level
   double* il_rtab = nullptr;  // k_cirs_il: multipliers and reciprocal divisors, 2 il_stride per block
+  double* jk_rtab[2] = {};    // k_cirs_jk (irsop 1), j and k lines: r(a) and 1/d(a), a = 1..L, 2 (JK_RT) per block
   double* cirs_r = nullptr;   // Thomas multipliers r(j), one per cell
   double* cirs_rt = nullptr;  // the same for k, omega (k-omega)
   double* rms_part = nullptr; // per-tile partial sums of squares
@@ -161,6 +198,11 @@ struct Level {
   std::vector<PatchRec> hpatch;
   PatchRec* dpatch = nullptr;
   int maxpatch = 0;
+  // BC work tiles (patch, first face item), patches in order, and the first
+  // tile of each patch (ptile_first[np] = number of tiles)
+  std::vector<Tile> hptile;
+  std::vector<int> ptile_first;
+  Tile* ptile = nullptr;
   std::vector<CutRec> hcut;
   CutRec* dcut = nullptr;
   int maxcut = 0;
@@ -194,6 +236,11 @@ struct Level {
   double* fsums = nullptr;    // (12, nsurface, 0:2N, blocks)
   double* fcoef = nullptr;    // (12, 0:2N, nsurface)
   double* fgeom = nullptr;    // direction vectors, lengths, Mach, moment centres
+  // device grid set-up (cosahip_grid_*)
+  GridTiles gt;
+  double* xbase = nullptr;    // base pose of a pitching grid: x, y, z of snapshot 0 (3 x n2tot)
+  double* gpose = nullptr;    // per-snapshot dcos(dtheta), dsin(dtheta), dcos(omegas zeit)
+  int* gbad = nullptr;        // non-positive face area flag
 };
 
 struct TimedLaunch {
@@ -207,6 +254,7 @@ struct Solve {
   Par par;
   int device = 0;
   double* scratch = nullptr;   // this GPU's fixed force scratch array (cosahip_force_noalloc)
+  bool fuse_update = true;     // RK update and next cons2prim in the last CIRS sweep (COSAHIP_FUSED_UPDATE=0: off)
   hipStream_t stream = nullptr;
   std::vector<Level> lev;     // index nl-1
   double dhb[131 * 131];      // dhb(i,j) column-major
@@ -281,6 +329,7 @@ struct TimeScope {
 
 // tile sizes
 constexpr int TS = 256;        // items per tile, 1-D kernels
+constexpr int JK_RT = 1025;    // per-block stride of the j/k CIRS tables (lines up to 1024 cells)
 constexpr int ROWSEG = 63;     // cells per i-row segment (64 faces)
 constexpr int MSEG = 16;       // cells per segment of a j / k line march
 constexpr ll CIRS_IV_MAX_LINES = 262144;   // i-lines (x snapshots) of a level below which k_cirs_iv is used

@@ -20,11 +20,14 @@ template <int NV, typename T = double, int EF = -1>
 __device__ __forceinline__ void roe_face(const Par& p, const T* qr, const T* ql, T kx, T ky, T kz, T ds, T ubn,
                                          T* F, T* h0lr = nullptr) {
   const T gamma = (T)p.gamma;
+  // gamma / (gamma - 1) of every face: in FP64 the host's quotient (the
+  // same correctly rounded value, one division less per face)
+  const T ggm1 = sizeof(T) == 8 ? (T)p.gam_gm1 : gamma / (gamma - 1);
   T rhor = qr[0], ur = qr[1], vr = qr[2], wr = qr[3], pr = qr[4];
   T unr = ur * kx + vr * ky + wr * kz - ubn;
   T q2r = ur * ur + vr * vr + wr * wr;
   T tker = NV > 5 ? qr[5] : T(0);
-  T h0r = gamma / (gamma - 1) * pr / rhor + q2r / 2 + tker;
+  T h0r = ggm1 * pr / rhor + q2r / 2 + tker;
   T rh0r = rhor * h0r;
   T fr0 = rhor * unr;
   T fr1 = (rhor * ur * unr + pr * kx);
@@ -35,7 +38,7 @@ __device__ __forceinline__ void roe_face(const Par& p, const T* qr, const T* ql,
   T unl = ul * kx + vl * ky + wl * kz - ubn;
   T q2l = ul * ul + vl * vl + wl * wl;
   T tkel = NV > 5 ? ql[5] : T(0);
-  T h0l = gamma / (gamma - 1) * pl / rhol + q2l / 2 + tkel;
+  T h0l = ggm1 * pl / rhol + q2l / 2 + tkel;
   if (h0lr) {
     h0lr[0] = h0l;
     h0lr[1] = h0r;
@@ -56,7 +59,9 @@ __device__ __forceinline__ void roe_face(const Par& p, const T* qr, const T* ql,
   T w = (rhols * wl + rhors * wr) / denom;
   T un = u * kx + v * ky + w * kz;
   T h0 = (rhols * h0l + rhors * h0r) / denom;
-  T tke = (rhols * tkel + rhors * tker) / denom;
+  // mean flow alone: tke = (rhols 0 + rhors 0) / denom is +0 for the
+  // positive finite weights, so it is taken as the constant
+  T tke = NV > 5 ? (rhols * tkel + rhors * tker) / denom : T(0);
   T q2 = u * u + v * v + w * w;
   T a2 = (gamma - 1) * (h0 - q2 / 2 - tke);
   T a = sqrt(a2);

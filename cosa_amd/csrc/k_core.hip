@@ -615,11 +615,13 @@ __device__ __forceinline__ void bc_ijk(int ic1, int ic2, int ic3, int a, int i2_
 __device__ __forceinline__ int cycd(int d, int m) { return (d - 1 + m - 1) % 3 + 1; }
 
 __global__ void __launch_bounds__(256) k_bc(Par p, const Blk* __restrict__ blks, const PatchRec* __restrict__ pr,
-                                            int first, double* __restrict__ q, const double* __restrict__ si,
-                                            const double* __restrict__ sj, const double* __restrict__ sk,
-                                            const double* __restrict__ dist, const double* __restrict__ qfb) {
-  const PatchRec rec = pr[first + blockIdx.y];
-  const int item = blockIdx.x * blockDim.x + threadIdx.x;
+                                            const Tile* __restrict__ tiles, double* __restrict__ q,
+                                            const double* __restrict__ si, const double* __restrict__ sj,
+                                            const double* __restrict__ sk, const double* __restrict__ dist,
+                                            const double* __restrict__ qfb) {
+  const Tile tl = tiles[blockIdx.x];
+  const PatchRec rec = pr[tl.block];
+  const int item = tl.base + threadIdx.x;
   if (item >= rec.count * p.nsnap) return;
   const Blk b = blks[rec.block];
   const int n = item / rec.count;
@@ -1199,16 +1201,19 @@ hipError_t launch_bc(Solve& s, Level& L, double* q) {
   const int np = (int)L.hpatch.size();
   if (!np) return hipSuccess;
   if (!L.thin) {
-    dim3 g((L.maxpatch * s.par.nsnap + TS - 1) / TS, np);
-    hipLaunchKernelGGL(k_bc, g, dim3(TS), 0, s.stream, s.par, L.dblk, L.dpatch, 0, q, L.f[COSAHIP_SI],
-                       L.f[COSAHIP_SJ], L.f[COSAHIP_SK], L.f[COSAHIP_DIST], s.dqfst);
+    const int nt = L.ptile_first[np];
+    if (nt)
+      hipLaunchKernelGGL(k_bc, dim3(nt), dim3(TS), 0, s.stream, s.par, L.dblk, L.dpatch, L.ptile, q,
+                         L.f[COSAHIP_SI], L.f[COSAHIP_SJ], L.f[COSAHIP_SK], L.f[COSAHIP_DIST], s.dqfst);
   } else {
     // a block one cell thick: patches read halos written by earlier
     // patches, so keep the reference's patch order (one launch per patch)
     for (int ip = 0; ip < np; ip++) {
-      dim3 g((L.hpatch[ip].count * s.par.nsnap + TS - 1) / TS, 1);
-      hipLaunchKernelGGL(k_bc, g, dim3(TS), 0, s.stream, s.par, L.dblk, L.dpatch, ip, q, L.f[COSAHIP_SI],
-                         L.f[COSAHIP_SJ], L.f[COSAHIP_SK], L.f[COSAHIP_DIST], s.dqfst);
+      const int nt = L.ptile_first[ip + 1] - L.ptile_first[ip];
+      if (nt)
+        hipLaunchKernelGGL(k_bc, dim3(nt), dim3(TS), 0, s.stream, s.par, L.dblk, L.dpatch,
+                           L.ptile + L.ptile_first[ip], q, L.f[COSAHIP_SI], L.f[COSAHIP_SJ], L.f[COSAHIP_SK],
+                           L.f[COSAHIP_DIST], s.dqfst);
     }
   }
   return hipGetLastError();

@@ -737,15 +737,24 @@ __device__ __forceinline__ void fill_rtab(double* rtab, int L, double epsc) {
 // one thread per (line, snapshot), each group with its own coefficients,
 // Thomas multipliers and time step, each variable with the operations of
 // tridi (src/cosa.f90:2418-2443).  first: scale by the time step on the way
-// in; last: divide by it on the way out.
-template <int DIR, int NM, int NT>
+// in; last: divide by it on the way out.  UPD (mean flow only, last sweep):
+// instead of storing the smoothed residual, apply the explicit RK update of
+// update_b to the cell at once, q = qold - ar res dtvol (src/update.f90:
+// 643-651) with res = x / dtvol as the unfused path stores it; UPD = 2 then
+// also converts the new state to primitive variables with cons2prim_b's
+// operations (src/update.f90:2136-2188), which the next RK stage would do
+// first.  The same operations on the same values: the state is the same
+// bits as cirs, update and cons2prim run one after the other.
+template <int DIR, int NM, int NT, int UPD>
 __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
                                                  double* __restrict__ res, const double* __restrict__ dtvol,
                                                  const double* __restrict__ beta, double* __restrict__ rsc,
                                                  const double* __restrict__ dtvolt, const double* __restrict__ tbeta,
-                                                 double* __restrict__ rsct, int first, int last, double epsc) {
+                                                 double* __restrict__ rsct, int first, int last, double epsc,
+                                                 CirsUpdate up, const double* __restrict__ jkt) {
   constexpr int NV = NM + NT;
   __shared__ double rtab[RTAB_MAX + 1];
+  __shared__ double ytab[RTAB_MAX + 1];
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int item = t.base + threadIdx.x;
@@ -753,8 +762,20 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
   const int T1 = im;
   const int T2 = DIR == 3 ? jm : km;
   const int nlines = T1 * T2;
-  const bool tab = p.irsop == 1 && (DIR == 2 ? jm : km) <= RTAB_MAX;
-  if (tab) fill_rtab(rtab, DIR == 2 ? jm : km, epsc);
+  // irsop 1: the multipliers r(a) and reciprocal divisors 1/d(a) of the
+  // block's line length, computed on the host with the recurrence's own
+  // operations (correctly rounded), are loaded into LDS by the whole
+  // work-group (every line of a work-group belongs to one block)
+  const bool tab = p.irsop == 1 && (DIR == 2 ? jm : km) <= RTAB_MAX && jkt != nullptr;
+  if (tab) {
+    const int Lt = DIR == 2 ? jm : km;
+    const double* tb = jkt + 2 * (ll)JK_RT * t.block;
+    for (int a = 1 + threadIdx.x; a <= Lt; a += blockDim.x) {
+      rtab[a] = tb[a];
+      ytab[a] = tb[JK_RT + a];
+    }
+    __syncthreads();
+  }
   if (item >= nlines * p.nsnap) return;
   const int n = item / nlines;
   const int rem = item - n * nlines;
@@ -779,15 +800,30 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
     const double ca = a == 1 ? 0.0 : -eps, cb = 1 + 2 * eps, cc = a == L ? 0.0 : -eps;
     const double dtv = first ? dtb[c1] : 1.0;
     double r;
-    if (a == 1) {
-      r = tab ? rtab[1] : cc / cb;
+    if (tab) {
+      // the forward quotient n / d by the reciprocal and one fused
+      // correction: with y = RN(1/d) and q = RN(n y) faithful,
+      // q + (n - q d) y rounded once is the correctly rounded n / d
+      // (Markstein's theorem, IBM J. Res. Dev. 34 (1990) 111-119), the bits
+      // of the division; an exact zero keeps its sign from q
+      r = rtab[a];
+      const double d = a == 1 ? cb : cb - rp * ca, y = ytab[a];
+#pragma unroll
+      for (int v = 0; v < NM; v++) {
+        const double x = first ? rb[c3 + n3 * v] * dtv : rb[c3 + n3 * v];
+        const double n = a == 1 ? x : x - tp[v] * ca;
+        const double q = n * y;
+        tp[v] = n == 0.0 ? q : fma(fma(-q, d, n), y, q);
+      }
+    } else if (a == 1) {
+      r = cc / cb;
 #pragma unroll
       for (int v = 0; v < NM; v++) {
         const double x = first ? rb[c3 + n3 * v] * dtv : rb[c3 + n3 * v];
         tp[v] = x / cb;
       }
     } else {
-      r = tab ? rtab[a] : cc / (cb - rp * ca);
+      r = cc / (cb - rp * ca);
 #pragma unroll
       for (int v = 0; v < NM; v++) {
         const double x = first ? rb[c3 + n3 * v] * dtv : rb[c3 + n3 * v];
@@ -801,15 +837,26 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
       const double cat = a == 1 ? 0.0 : -epst, cbt = 1 + 2 * epst, cct = a == L ? 0.0 : -epst;
       const double dtvt = first ? dtt[c1] : 1.0;
       double rt;
-      if (a == 1) {
-        rt = tab ? rtab[1] : cct / cbt;
+      if (tab) {
+        // irsop 1: k and omega use the same constant coefficient
+        rt = rtab[a];
+        const double d = a == 1 ? cbt : cbt - rpt * cat, y = ytab[a];
+#pragma unroll
+        for (int v = NM; v < NV; v++) {
+          const double x = first ? rb[c3 + n3 * v] * dtvt : rb[c3 + n3 * v];
+          const double n = a == 1 ? x : x - tp[v] * cat;
+          const double q = n * y;
+          tp[v] = n == 0.0 ? q : fma(fma(-q, d, n), y, q);
+        }
+      } else if (a == 1) {
+        rt = cct / cbt;
 #pragma unroll
         for (int v = NM; v < NV; v++) {
           const double x = first ? rb[c3 + n3 * v] * dtvt : rb[c3 + n3 * v];
           tp[v] = x / cbt;
         }
       } else {
-        rt = tab ? rtab[a] : cct / (cbt - rpt * cat);
+        rt = cct / (cbt - rpt * cat);
 #pragma unroll
         for (int v = NM; v < NV; v++) {
           const double x = first ? rb[c3 + n3 * v] * dtvt : rb[c3 + n3 * v];
@@ -842,7 +889,40 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
     }
 #pragma unroll
     for (int v = 0; v < NV; v++) xn[v] = xv[v];
-    if (last) {
+    if (UPD) {
+      static_assert(UPD == 0 || (NM == 5 && NT == 0), "the fused update is built for the mean flow only");
+      const ll c1 = i1(b, i, j, k);
+      const double dtv = dtb[c1];
+      const ll qo = rb - res;
+      double qc[NM];
+#pragma unroll
+      for (int v = 0; v < NM; v++) {
+        const double rv = xv[v] / dtv;
+        qc[v] = up.qold[qo + c3 + n3 * v] - up.ar * rv * dtv;
+      }
+      double* qb = up.q + qo;
+      if (UPD == 2) {
+        const double rho = qc[0];
+        const double u = qc[1] / rho, vv = qc[2] / rho, w = qc[3] / rho;
+        const double pp = (p.gamma - 1) * (qc[4] - rho * (u * u + vv * vv + w * w) / 2 - rho * 0.0);
+        qb[c3] = rho;
+        qb[c3 + n3] = u;
+        qb[c3 + 2 * n3] = vv;
+        qb[c3 + 3 * n3] = w;
+        qb[c3 + 4 * n3] = pp;
+        if (up.q32) {
+          float* sb = up.q32 + qo;
+          sb[c3] = (float)rho;
+          sb[c3 + n3] = (float)u;
+          sb[c3 + 2 * n3] = (float)vv;
+          sb[c3 + 3 * n3] = (float)w;
+          sb[c3 + 4 * n3] = (float)pp;
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NM; v++) qb[c3 + n3 * v] = qc[v];
+      }
+    } else if (last) {
       const ll c1 = i1(b, i, j, k);
       const double dtv = dtb[c1];
 #pragma unroll
@@ -1307,7 +1387,7 @@ static hipError_t launch_ccirs(Solve& s, Level& L) {
   return hipGetLastError();
 }
 
-hipError_t launch_cirs(Solve& s, Level& L) {
+hipError_t launch_cirs(Solve& s, Level& L, const CirsUpdate* up) {
   if (s.par.irsop < 1 || s.par.irsop == 4 || s.par.irsop > 6) return hipSuccess;
   if (s.cfg.cutcirs == 1) return launch_ccirs(s, L);
   double* res = L.f[COSAHIP_RES];
@@ -1345,24 +1425,35 @@ hipError_t launch_cirs(Solve& s, Level& L) {
                        L.tile[TK_LINES_I], res, L.f[COSAHIP_DTVOLT], L.tbeta[0], L.cirs_r, 5, epsc);
   // j and k sweeps: mean flow and k-omega in one launch per direction
   const double* dtt = L.f[COSAHIP_DTVOLT];
+  const CirsUpdate none{};
   if (s.par.kom) {
     if (L.ntile[TK_LINES_J])
-      hipLaunchKernelGGL((k_cirs_jk<2, 5, 2>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
+      hipLaunchKernelGGL((k_cirs_jk<2, 5, 2, 0>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
                          L.tile[TK_LINES_J], res, dt, L.f[COSAHIP_BETAETA], L.cirs_r, dtt, L.tbeta[1], L.cirs_rt, 0, 0,
-                         epsc);
+                         epsc, none, L.jk_rtab[0]);
     if (L.ntile[TK_LINES_K])
-      hipLaunchKernelGGL((k_cirs_jk<3, 5, 2>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
+      hipLaunchKernelGGL((k_cirs_jk<3, 5, 2, 0>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
                          L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, dtt, L.tbeta[2], L.cirs_rt, 0, 1,
-                         epsc);
+                         epsc, none, L.jk_rtab[1]);
   } else {
     if (L.ntile[TK_LINES_J])
-      hipLaunchKernelGGL((k_cirs_jk<2, 5, 0>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
+      hipLaunchKernelGGL((k_cirs_jk<2, 5, 0, 0>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
                          L.tile[TK_LINES_J], res, dt, L.f[COSAHIP_BETAETA], L.cirs_r, nullptr, nullptr, nullptr, 0, 0,
-                         epsc);
-    if (L.ntile[TK_LINES_K])
-      hipLaunchKernelGGL((k_cirs_jk<3, 5, 0>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                         L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, nullptr, nullptr, nullptr, 0, 1,
-                         epsc);
+                         epsc, none, L.jk_rtab[0]);
+    if (L.ntile[TK_LINES_K]) {
+      if (!up)
+        hipLaunchKernelGGL((k_cirs_jk<3, 5, 0, 0>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                           L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, nullptr, nullptr, nullptr, 0,
+                           1, epsc, none, L.jk_rtab[1]);
+      else if (up->prim)
+        hipLaunchKernelGGL((k_cirs_jk<3, 5, 0, 2>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                           L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, nullptr, nullptr, nullptr, 0,
+                           1, epsc, *up, L.jk_rtab[1]);
+      else
+        hipLaunchKernelGGL((k_cirs_jk<3, 5, 0, 1>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                           L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, nullptr, nullptr, nullptr, 0,
+                           1, epsc, *up, L.jk_rtab[1]);
+    }
   }
   return hipGetLastError();
 }

@@ -56,7 +56,17 @@ hipError_t launch_residual_tile(Solve& s, Level& L, int variant, const float* q,
                                 const float* sk, double* res);
 // work items of one block in the tile table of a TJ-row residual tile
 ll residual_tile_items(int TJ, int imax, int jmax, int kmax, int nsnap);
-hipError_t launch_cirs(Solve& s, Level& L);
+// the RK update fused into the last CIRS sweep (mean flow): q = qold - ar
+// res dtvol per cell, then (prim) the next stage's cons2prim; q32: the
+// FP32 shadow of the primitive state, or null
+struct CirsUpdate {
+  double* q;
+  const double* qold;
+  float* q32;
+  double ar;
+  int prim;
+};
+hipError_t launch_cirs(Solve& s, Level& L, const CirsUpdate* up = nullptr);
 // vpre: viscous face fluxes of direction idir - 1 whose difference is added
 // to res first (nullptr: none)
 template <typename T>
@@ -72,6 +82,13 @@ hipError_t launch_fixq(Solve& s, Level& L);
 hipError_t launch_turb_limits(Solve& s, Level& L, double ar);
 
 // ---- forces (k_force.hip) ----
+// ---- grid motion and metric set-up (k_grid.hip) ----
+hipError_t launch_grid_metrics(Solve& s, Level& L, const GridTiles& gt, int* dbad, int do_volex, int do_bedges,
+                               int do_vmetrix);
+hipError_t launch_grid_coarsen(Solve& s, Level& F, Level& C, const GridTiles& gtc);
+hipError_t launch_grid_pitch(Solve& s, Level& L, const GridTiles& gt, const double* pose_dev, double dtheta0,
+                             double omegas, double xh, double yh, const double* base);
+
 // device buffers of one force evaluation: the wall-face runs, the per-face
 // terms (12, faces, 0:2N), per-block sums (12, nsurface, 0:2N, blocks), the
 // coefficients (12, 0:2N, nsurface) and the geometry (13 values, then the

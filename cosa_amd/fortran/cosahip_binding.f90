@@ -134,6 +134,24 @@ module cosahip_binding
       real(c_double), intent(in) :: lngth(3), momr(*)
       real(c_double), intent(out) :: coef(*)
     end function
+    integer(c_int) function cosahip_grid_metrics(solve, nl) bind(C, name='cosahip_grid_metrics')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: solve
+      integer(c_int), value :: nl
+    end function
+    integer(c_int) function cosahip_grid_coarsen(solve, nl) bind(C, name='cosahip_grid_coarsen')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: solve
+      integer(c_int), value :: nl
+    end function
+    integer(c_int) function cosahip_grid_pitch(solve, nl, dtheta0, omegas, xh, yh, pose) &
+        bind(C, name='cosahip_grid_pitch')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: solve
+      integer(c_int), value :: nl
+      real(c_double), value :: dtheta0, omegas, xh, yh
+      real(c_double), intent(in) :: pose(*)
+    end function
     integer(c_int) function cosahip_set_precision(solve, mode, tail_rms) bind(C, name='cosahip_set_precision')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: solve

@@ -179,6 +179,41 @@ class Solve:
     def prolong(self, nl: int):
         self._check(self.lib.cosahip_prolong(self.h, nl), "prolong")
 
+    # -- grid motion and metric set-up on the device ---------------------------
+    def grid_metrics(self, nl: int = 1):
+        """VOL, SI/SJ/SK and (viscous) the metric derivatives of level nl
+        from its uploaded node coordinates (cosahip_grid_metrics)."""
+        self._check(self.lib.cosahip_grid_metrics(self.h, nl), "grid_metrics")
+
+    def grid_coarsen(self, nl: int):
+        """Level nl + 1's node coordinates from level nl's (coarsen_b)."""
+        self._check(self.lib.cosahip_grid_coarsen(self.h, nl), "grid_coarsen")
+
+    def grid_pitch(self, nl: int = 1):
+        """Node positions and velocities of every harmonic-balance snapshot
+        of level nl, rigid pitching about the case's hinge (movegrid with
+        pitch)."""
+        m, c = self.case.motion, self.case.config
+        pose = np.ascontiguousarray(self.case.pitch_poses(), np.float64)
+        self._check(self.lib.cosahip_grid_pitch(self.h, nl, m["dtheta0"], c["omega"], m.get("xh", 0.25),
+                                                m.get("yh", 0.0), _lib.dptr(pose)), "grid_pitch")
+
+    def device_geometry(self, x=None, y=None, z=None):
+        """The whole set-up of the driver on the device (src/cosa.f90:160-166,
+        297-322): level-1 node coordinates (uploaded if given), then per level
+        movegrid (moving cases), the metrics, and the next level's nodes by
+        coarsening."""
+        for f, a in (("x", x), ("y", y), ("z", z)):
+            if a is not None:
+                self.upload(f, a, 1)
+        nlevel = self.case.config["nlevel"]
+        for nl in range(1, nlevel + 1):
+            if nl > 1:
+                self.grid_coarsen(nl - 1)
+            if self.case.config["moving"]:
+                self.grid_pitch(nl)
+            self.grid_metrics(nl)
+
     def force(self, nsurface: int = 1, include_inviscid: bool = True, lngth=(1.0, 1.0, 1.0),
               xyzmom=(0.25, 0.0, 0.0), nl: int = 1, noalloc: bool = False) -> np.ndarray:
         """Force and moment coefficients (force_b): array (nsurface, nsnap,

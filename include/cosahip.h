@@ -342,6 +342,39 @@ int cosahip_block_rms(cosahip_solve* s, double* out);
 int cosahip_force_blocks(cosahip_solve* s, int nl, int nsurface, int include_inviscid, const double* lngth,
                          const double* momr, double* coef);
 
+/* ---- grid motion and metric set-up on the device ---------------------
+ * The producers of the metric inputs, computed from the node coordinates
+ * X, Y, Z (and XDOT, YDOT, ZDOT of moving grids) already uploaded to the
+ * level, with the reference routines' arithmetic (the same bits as COSA's
+ * CPU set-up), so a solve can be admitted from its grid alone.
+ *
+ * cosahip_grid_metrics: the per-level sequence of the driver
+ * (src/cosa.f90:304-322): vols/bvols (VOL interior, src/grid.f90:2162-2258),
+ * imetrix/imetrix_b (SI, SJ, SK incl. face velocities when moving,
+ * src/grid.f90:848-1253); when viscous or nlevel > 1 volex_b and cutman_v
+ * (VOL face halos, src/grid.f90:83-141, 525-838); when nl < nlevel and
+ * the prolongation is linear vol_edges (src/grid.f90:3584-3826); when
+ * viscous coordex/cordex_b and cutman_x (X/Y/Z halo nodes, src/grid.f90:
+ * 2-78, 144-524) and vmetrix_b (the nine metric derivatives,
+ * src/grid.f90:3150-3583).  The rotating-frame face moments (hawt) are not
+ * built.  COSAHIP_ERR_ARG when a face area is not positive (imetrix_b
+ * stops there); a non-positive volume is not an error (bvols only reports
+ * it).  Drops a captured cycle graph.                                     */
+int cosahip_grid_metrics(cosahip_solve* s, int nl);
+/* coarsen (src/cosa.f90:2575-2669): level nl + 1's X, Y, Z (nodes 1..imax
+ * etc. of snapshot 0) as every second node of level nl's.                 */
+int cosahip_grid_coarsen(cosahip_solve* s, int nl);
+/* movegrid with pitch (src/grid.f90:1783-1964): X, Y, Z, XDOT, YDOT, ZDOT
+ * of every metric snapshot n of level nl from the base pose (snapshot 0 of
+ * X, Y, Z at the first call, kept as x0), pitching about (xh, yh) with unit
+ * displacement scales dx = dy = 1 (the reference's mesh.dat for rigid
+ * motion).  pose[3n + 0..2] = dcos(dtheta(n)), dsin(dtheta(n)),
+ * dcos(omegas * zeit(n)) with dtheta(n) = dtheta0 dsin(omegas zeit(n))
+ * (src/grid.f90:4540), evaluated by the caller as the reference does.
+ * Moving configurations only.                                             */
+int cosahip_grid_pitch(cosahip_solve* s, int nl, double dtheta0, double omegas, double xh, double yh,
+                       const double* pose);
+
 /* ---- forces ------------------------------------------------------------
  * force_b (src/output.f90:4388-4886) summed over the level's blocks as
  * force does (src/output.f90:4290-4330), aircraft conventions: pressure and

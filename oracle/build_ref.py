@@ -40,8 +40,11 @@ BY_NAME = {
     "parallelutils.f90": ["myblock", "getownerid"],
     "cosa.f90": ["bresid", "lambdab", "eig_eul", "eig_ns", "eig_eul_p", "eig_ns_p", "bdeltat", "cirs_b",
                  "tridi", "ccirs_b", "scalef_b", "scaleb_b", "resg_brms", "resg_brms_hb", "rest1_b", "rest2_ho_b", "rest2_lo_b", "prol_lin_b",
-                 "prol_bln_b", "rest4_b"],
+                 "prol_bln_b", "rest4_b", "coarsen_b"],
     "output.f90": ["force_b", "write_brest"],
+    # grid motion and metric set-up: the producers of the metric inputs
+    "grid.f90": ["imetrix_b", "bvols", "pitch", "cordex_b", "volex_b", "vol_bedges", "vmetrix_b", "cut_v",
+                 "cut_x"],
     "input.f90": ["read_brest"],
 }
 FFLAGS = ["-cpp", "-fdefault-real-8", "-fdefault-double-8", "-O2", "-ffp-contract=off", "-fPIC"]

@@ -203,6 +203,24 @@ void or_prolong(or_case* c, int nl);
 int or_mg(or_case* c, int nstart, int npre, int npost, int ncrs,
           double* hist, int32_t* where);
 
+/* grid motion and metric set-up (grid_oracle.c; src/grid.f90, src/cosa.f90) */
+int or_imetrix_b(const double* x, const double* y, const double* z, const double* xdot, const double* ydot,
+                 const double* zdot, double* si, double* sj, double* sk, int imax, int jmax, int kmax, int lmet,
+                 int nsm, int moving);
+int or_bvols(const double* x, const double* y, const double* z, double* vol, int imax, int jmax, int kmax);
+void or_volex_b(double* vol, int imax, int jmax, int kmax);
+void or_vol_bedges(double* vol, int imax, int jmax, int kmax);
+void or_cordex_b(double* x, double* y, double* z, int imax, int jmax, int kmax, int nsm);
+void or_vmetrix_b(const double* x, const double* y, const double* z, double* const* d, const double* vol, int imax,
+                  int jmax, int kmax, int nsm);
+void or_coarsen_b(const double* x, const double* y, const double* z, int imax, int jmax, int kmax, double* x2,
+                  double* y2, double* z2, int imax2, int jmax2, int kmax2);
+void or_pitch(double cth, double sth, double ca, double dtheta0, double omegas, double xh, double yh,
+              const double* x0, const double* y0, const double* z0, const double* dx, const double* dy, double* x,
+              double* y, double* z, double* xdot, double* ydot, double* zdot, int imax, int jmax, int kmax);
+void or_cut_layer(int kind, int nsm, int imax1, int jmax1, int kmax1, double* a1, int imax2, int jmax2, int kmax2,
+                  const double* a2, const int32_t* cutopo);
+
 #ifdef __cplusplus
 }
 #endif

@@ -536,4 +536,119 @@ contains
     close(fid)
   end subroutine
 
+  ! ---- grid motion and metric set-up (src/grid.f90; coarsen_b of src/cosa.f90)
+
+  subroutine ref_imetrix_b(x, y, z, xdot, ydot, zdot, si, sj, sk, imax, jmax, kmax) bind(C, name='ref_imetrix_b')
+    real(c_double) :: x(*), y(*), z(*), xdot(*), ydot(*), zdot(*), si(*), sj(*), sk(*)
+    integer(c_int32_t), value :: imax, jmax, kmax
+    integer(kind=cosa_int) :: ib, im, jm, km, lm, nh
+    dgn_cell = .false.
+    ib = 1; im = imax; jm = jmax; km = kmax; lm = lmet; nh = nharms
+    call imetrix_b(x, y, z, xdot, ydot, zdot, si, sj, sk, ib, im, jm, km, lm, nh)
+  end subroutine
+
+  subroutine ref_bvols(x, y, z, vol, imax, jmax, kmax) bind(C, name='ref_bvols')
+    real(c_double) :: x(*), y(*), z(*), vol(*)
+    integer(c_int32_t), value :: imax, jmax, kmax
+    integer(kind=cosa_int) :: ib, im, jm, km, nh, lnb
+    real(kind=cosa_real), allocatable :: rad(:)
+    debug = .false.
+    ib = 1; im = imax; jm = jmax; km = kmax; nh = nharms; lnb = 1
+    allocate(rad(max(1, (imax - 1) * (jmax - 1) * (kmax - 1) * dim5h)))
+    call bvols(ib, x, y, z, x, y, z, vol, rad, im, jm, km, nh, lnb)
+    deallocate(rad)
+  end subroutine
+
+  subroutine ref_volex_b(vol, imax, jmax, kmax) bind(C, name='ref_volex_b')
+    real(c_double) :: vol(*)
+    integer(c_int32_t), value :: imax, jmax, kmax
+    integer(kind=cosa_int) :: im, jm, km
+    im = imax; jm = jmax; km = kmax
+    call volex_b(vol, im, jm, km)
+  end subroutine
+
+  subroutine ref_vol_bedges(vol, imax, jmax, kmax) bind(C, name='ref_vol_bedges')
+    real(c_double) :: vol(*)
+    integer(c_int32_t), value :: imax, jmax, kmax
+    integer(kind=cosa_int) :: im, jm, km
+    im = imax; jm = jmax; km = kmax
+    call vol_bedges(vol, im, jm, km)
+  end subroutine
+
+  subroutine ref_cordex_b(x, y, z, imax, jmax, kmax) bind(C, name='ref_cordex_b')
+    real(c_double) :: x(*), y(*), z(*)
+    integer(c_int32_t), value :: imax, jmax, kmax
+    integer(kind=cosa_int) :: im, jm, km, nh
+    im = imax; jm = jmax; km = kmax; nh = nharms
+    call cordex_b(x, y, z, im, jm, km, nh)
+  end subroutine
+
+  ! d(1:9): xideri, etaderi, zetaderi, xiderj, etaderj, zetaderj, xiderk,
+  ! etaderk, zetaderk (vmetrix_b takes them grouped by face family first)
+  subroutine ref_vmetrix_b(x, y, z, d1, d2, d3, d4, d5, d6, d7, d8, d9, vol, imax, jmax, kmax) &
+      bind(C, name='ref_vmetrix_b')
+    real(c_double) :: x(*), y(*), z(*), d1(*), d2(*), d3(*), d4(*), d5(*), d6(*), d7(*), d8(*), d9(*), vol(*)
+    integer(c_int32_t), value :: imax, jmax, kmax
+    integer(kind=cosa_int) :: ib, im, jm, km, nh
+    ib = 1; im = imax; jm = jmax; km = kmax; nh = nharms
+    call vmetrix_b(x, y, z, d1, d4, d7, d2, d5, d8, d3, d6, d9, vol, ib, im, jm, km, nh)
+  end subroutine
+
+  subroutine ref_coarsen_b(x, y, z, imax, jmax, kmax, x2, y2, z2, imax2, jmax2, kmax2) bind(C, name='ref_coarsen_b')
+    real(c_double) :: x(*), y(*), z(*), x2(*), y2(*), z2(*)
+    integer(c_int32_t), value :: imax, jmax, kmax, imax2, jmax2, kmax2
+    integer(kind=cosa_int) :: im, jm, km, im2, jm2, km2
+    real(kind=cosa_real) :: dd(1)
+    im = imax; jm = jmax; km = kmax; im2 = imax2; jm2 = jmax2; km2 = kmax2
+    pitching = .false.
+    call coarsen_b(x, y, z, dd, dd, dd, im, jm, km, x2, y2, z2, dd, dd, dd, im2, jm2, km2)
+  end subroutine
+
+  subroutine ref_pitch(zeit, dtheta, dtheta0, omegas, xh, yh, x0, y0, z0, dx, dy, dz, x, y, z, xdot, ydot, zdot, &
+                       imax, jmax, kmax) bind(C, name='ref_pitch')
+    real(c_double), value :: zeit, dtheta, dtheta0, omegas, xh, yh
+    real(c_double) :: x0(*), y0(*), z0(*), dx(*), dy(*), dz(*), x(*), y(*), z(*), xdot(*), ydot(*), zdot(*)
+    integer(c_int32_t), value :: imax, jmax, kmax
+    real(kind=cosa_real) :: a1, a2, a3, a4, a5, a6
+    integer(kind=cosa_int) :: im, jm, km
+    a1 = zeit; a2 = dtheta; a3 = dtheta0; a4 = omegas; a5 = xh; a6 = yh
+    im = imax; jm = jmax; km = kmax
+    call pitch(a1, a2, a3, a4, a5, a6, x0, y0, z0, dx, dy, dz, x, y, z, xdot, ydot, zdot, im, jm, km)
+  end subroutine
+
+  ! cut_v / cut_x (src/grid.f90:612-838, 242-524), the branch where both
+  ! blocks belong to this process, as ref_cut_q
+  subroutine ref_cut_v(imax1, jmax1, kmax1, v1, imax2, jmax2, kmax2, v2, cutopo) bind(C, name='ref_cut_v')
+    integer(c_int32_t), value :: imax1, jmax1, kmax1, imax2, jmax2, kmax2
+    real(c_double) :: v1(*), v2(*)
+    integer(c_int32_t) :: cutopo(21)
+    integer(kind=cosa_int) :: rreq(1), sreq(1), nrecv(1), nsend(1), myr(0:0), mys(0:0), rind(1, 8, 1)
+    real(kind=cosa_real) :: sarr(1, 1), rarr(1, 1)
+    integer(kind=cosa_int) :: i1, j1, k1, i2, j2, k2
+    lowernblock = 1
+    uppernblock = max(cutopo(1), cutopo(10))
+    nrecv = 0; nsend = 0; myr = 1; mys = 1
+    i1 = imax1; j1 = jmax1; k1 = kmax1; i2 = imax2; j2 = jmax2; k2 = kmax2
+    call cut_v(i1, j1, k1, v1, i2, j2, k2, v2, cutopo, rarr, rreq, nrecv, rind, sarr, sreq, nsend, 1, 1, 1, 1, &
+               myr, mys)
+  end subroutine
+
+  subroutine ref_cut_x(imax1, jmax1, kmax1, x1, y1, z1, imax2, jmax2, kmax2, x2, y2, z2, cutopo) &
+      bind(C, name='ref_cut_x')
+    integer(c_int32_t), value :: imax1, jmax1, kmax1, imax2, jmax2, kmax2
+    real(c_double) :: x1(*), y1(*), z1(*), x2(*), y2(*), z2(*)
+    integer(c_int32_t) :: cutopo(21)
+    integer(kind=cosa_int) :: rreq(1), sreq(1), nrecv(1), nsend(1), myr(0:0), mys(0:0), rind(1, 8, 1)
+    real(kind=cosa_real), allocatable :: sarr(:,:,:), rarr(:,:,:)
+    integer(kind=cosa_int) :: i1, j1, k1, i2, j2, k2, nh, d5h
+    allocate(sarr(dim5h, 1, 1), rarr(dim5h, 1, 1))
+    lowernblock = 1
+    uppernblock = max(cutopo(1), cutopo(10))
+    nrecv = 0; nsend = 0; myr = 1; mys = 1
+    i1 = imax1; j1 = jmax1; k1 = kmax1; i2 = imax2; j2 = jmax2; k2 = kmax2; nh = nharms; d5h = dim5h
+    call cut_x(i1, j1, k1, x1, y1, z1, i2, j2, k2, x2, y2, z2, nh, d5h, cutopo, rarr, rreq, nrecv, rind, sarr, &
+               sreq, nsend, 1, 1, 1, 1, myr, mys)
+    deallocate(sarr, rarr)
+  end subroutine
+
 end module ref_shim

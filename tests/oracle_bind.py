@@ -39,8 +39,8 @@ def ip(a):
 
 
 def _build_oracle():
-    src = os.path.join(ROOT, "oracle", "cosa_oracle.c")
-    if (not os.path.exists(ORACLE_SO)) or os.path.getmtime(ORACLE_SO) < os.path.getmtime(src):
+    srcs = [os.path.join(ROOT, "oracle", n) for n in ("cosa_oracle.c", "grid_oracle.c", "cosa_oracle.h")]
+    if (not os.path.exists(ORACLE_SO)) or os.path.getmtime(ORACLE_SO) < max(os.path.getmtime(f) for f in srcs):
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "liboracle.so"])
 
 

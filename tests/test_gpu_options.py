@@ -28,7 +28,8 @@ VARIANTS = {
     "limiter3": dict(ilimf=3, epslimf=0.05),
     "entropy_fix_w1": dict(EFX, efx_all_f=1, entfx_f_w=(1, 0, 0)),
     "entropy_fix_w2": dict(EFX, efx_all_f=1, entfx_f_w=(0, 1, 0)),
-    "entropy_fix_w3_contact_only": dict(EFX, efx_all_f=0, entfx_f_w=(0, 0, 1)),
+    # acoustic waves only (efx_all_f off); a larger width so |un - a| <= rdelta occurs
+    "entropy_fix_w3_acoustic_only": dict(EFX, cntrpy_f=3.0, efx_all_f=0, entfx_f_w=(0, 0, 1)),
     "eigen_floor": dict(eig_limit_f=1, eig_lim_all_f=0, eig_cutoff_f=0.15),
     "eigen_floor_all_waves": dict(eig_limit_f=1, eig_lim_all_f=1, eig_cutoff_f=0.15),
     "rans_limiter2_turb_efx_eig": dict(ilimt=2, epslimt=0.05, efx_setup_t=1, efx_all_t=1, cntrpy_t=0.3,
