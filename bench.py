@@ -112,6 +112,15 @@ def make_case(workload: str):
                        physics="rans_wilcox_komega", mach=0.3, reynolds=3.0e6, alpha_deg=4.0, rk_stages=4,
                        update="point_implicit_komega", irs="cirs_c", precision="fp64",
                        grid="O-H wing, NACA0012 section, semi-span 2 (AR 4), wall spacing 5e-6 chord")
+    if workload.startswith("c3_hb"):
+        import math
+        nh = int(workload[5:] or 4)
+        c = cases.naca_case(385, 97, 5, nlevel=2, name=f"c3_pitch_hb{nh}", irsop=1, cfl=2.0, cfl_irs=3.5, nl_crs=2,
+                            ncycle=100000, machfs=0.5, alpha_deg=0.0, nharms=nh, harbal=1, omega=0.1,
+                            pitch=dict(dtheta0=math.radians(5.0), xh=0.25, yh=0.0))
+        return c, dict(workload=f"c3_hb_pitch_naca0012_385x97_N{nh}", cells=c.ncell(), snapshots=c.nsnap, levels=2,
+                       physics="euler harmonic balance, pitching 5 deg at k = 0.1", mach=0.5, rk_stages=4,
+                       irs="cirs_c", precision="fp64")
     if workload == "naca":
         c = cases.naca_case(193, 49, 9, nlevel=3, name="naca0012", irsop=1, cfl=2.0, cfl_irs=4.0, nl_crs=2,
                             ncycle=100000)
@@ -288,7 +297,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="uneven96", choices=["uneven96", "wing4", "naca"])
+    ap.add_argument("--workload", default="uneven96", choices=["uneven96", "wing4", "naca", "c3_hb1", "c3_hb2",
+                                                                "c3_hb4"])
     ap.add_argument("--no-extras", action="store_true", help="leave out the wing workload of extra_workloads")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")

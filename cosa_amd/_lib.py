@@ -17,12 +17,13 @@ LIB_PATH = os.path.join(HERE, "libcosahip.so")
 
 # field ids (enum in include/cosahip.h)
 (Q, QOLD, RES, RHS, SI, SJ, SK, VOL, DTVOL, BETAXI, BETAETA, BETAZETA, XDOT, YDOT, ZDOT, MUT, DIST, XIDERI, ETADERI,
- ZETADERI, XIDERJ, ETADERJ, ZETADERJ, XIDERK, ETADERK, ZETADERK, PRD, DIVVEL, DELPLS, DTVOLT, X, Y, Z, TTRM) = range(34)
+ ZETADERI, XIDERJ, ETADERJ, ZETADERJ, XIDERK, ETADERK, ZETADERK, PRD, DIVVEL, DELPLS, DTVOLT, X, Y, Z, TTRM, Q1,
+ Q2) = range(36)
 FIELD_IDS = dict(q=Q, qold=QOLD, res=RES, rhs=RHS, si=SI, sj=SJ, sk=SK, vol=VOL, dtvol=DTVOL, betaxi=BETAXI,
                  betaeta=BETAETA, betazeta=BETAZETA, xdot=XDOT, ydot=YDOT, zdot=ZDOT, mut=MUT, dist=DIST,
                  xideri=XIDERI, etaderi=ETADERI, zetaderi=ZETADERI, xiderj=XIDERJ, etaderj=ETADERJ,
                  zetaderj=ZETADERJ, xiderk=XIDERK, etaderk=ETADERK, zetaderk=ZETADERK, prd=PRD, divvel=DIVVEL,
-                 delpls=DELPLS, dtvolt=DTVOLT, x=X, y=Y, z=Z, ttrm=TTRM)
+                 delpls=DELPLS, dtvolt=DTVOLT, x=X, y=Y, z=Z, ttrm=TTRM, q1=Q1, q2=Q2)
 # the nine metric-derivative fields in the order of the reference's arrays
 XD_NAMES = ["xideri", "etaderi", "zetaderi", "xiderj", "etaderj", "zetaderj", "xiderk", "etaderk", "zetaderk"]
 
@@ -74,6 +75,10 @@ def load() -> ctypes.CDLL:
         "cosahip_lambda_deltat": ([_VP, ctypes.c_int], ctypes.c_int),
         "cosahip_residual": ([_VP, ctypes.c_int], ctypes.c_int),
         "cosahip_src_hb": ([_VP, ctypes.c_int], ctypes.c_int),
+        "cosahip_src_uns": ([_VP, ctypes.c_int], ctypes.c_int),
+        "cosahip_set_dual_time": ([_VP, ctypes.c_double, ctypes.c_int], ctypes.c_int),
+        "cosahip_dual_begin": ([_VP, ctypes.c_int], ctypes.c_int),
+        "cosahip_dual_end": ([_VP], ctypes.c_int),
         "cosahip_resg_rms": ([_VP, ctypes.c_int, _DP], ctypes.c_int),
         "cosahip_checkpoint_begin": ([_VP, ctypes.c_int, _DP, ctypes.c_int64, _DP, ctypes.c_int64], ctypes.c_int),
         "cosahip_checkpoint_wait": ([_VP], ctypes.c_int),
@@ -128,7 +133,8 @@ def exported_symbols():
             "cosahip_download", "cosahip_checkpoint_begin", "cosahip_checkpoint_wait", "cosahip_host_alloc",
             "cosahip_host_free", "cosahip_synchronize", "cosahip_cons2prim", "cosahip_prim2cons", "cosahip_bc",
             "cosahip_cutman_q", "cosahip_edges_corners", "cosahip_prtest", "cosahip_lambda_deltat",
-            "cosahip_residual", "cosahip_src_hb", "cosahip_resg_rms", "cosahip_add_array", "cosahip_copy_array",
+            "cosahip_residual", "cosahip_src_hb", "cosahip_src_uns", "cosahip_set_dual_time", "cosahip_dual_begin",
+            "cosahip_dual_end", "cosahip_resg_rms", "cosahip_add_array", "cosahip_copy_array",
             "cosahip_cirs", "cosahip_update", "cosahip_tur_vis", "cosahip_fixq", "cosahip_turb_limits",
             "cosahip_smooth", "cosahip_restrict", "cosahip_prolong", "cosahip_cycle_enqueue", "cosahip_cycle_wait",
             "cosahip_cycle_ready", "cosahip_autotune", "cosahip_set_block_freestream", "cosahip_set_block_rms",

@@ -209,8 +209,6 @@ static int check_config(Solve* s, const cosahip_config& c) {
     return fail(s, COSAHIP_ERR_ARG, "at most one of kom, kom_bsl, kom_sst");
   if (komega(c) && !c.viscous) return fail(s, COSAHIP_ERR_ARG, "k-omega requires viscous");
   if (c.npde != (komega(c) ? 7 : 5)) return fail(s, COSAHIP_ERR_ARG, "npde must be 5 (Euler, laminar) or 7 (k-omega)");
-  if (c.rkimp && !c.harbal)
-    return fail(s, COSAHIP_ERR_UNSUPPORTED, "rkimp without harmonic balance (dual time stepping) is not built");
   if (c.rkimp && rkim_group(2 * c.nharms + 1) < 1)
     return fail(s, COSAHIP_ERR_UNSUPPORTED, "rkimp: 2N+1 snapshots exceed the per-cell LDS system (N <= 40)");
   if (komega(c) && (c.prd_all_lvls || c.lim_corr[0] || c.lim_corr[1] || c.lim_corr[2]))
@@ -260,7 +258,8 @@ int cosahip_create(const cosahip_config* cfg, int device, cosahip_solve** out) {
   if (rk && !s->split_residual)
     s->resid_force = strcmp(rk, "fused") == 0 ? RESID_GENERIC : strcmp(rk, "tile7") == 0 ? RESID_TILE7
                      : strcmp(rk, "tile7reg") == 0 ? RESID_TILE7_REG : strcmp(rk, "tile15") == 0 ? RESID_TILE15
-                     : strcmp(rk, "tile7w4") == 0 ? RESID_TILE7_W4 : -1;
+                     : strcmp(rk, "tile7w4") == 0 ? RESID_TILE7_W4 : strcmp(rk, "sva7") == 0 ? RESID_SVA7
+                     : strcmp(rk, "sva15") == 0 ? RESID_SVA15 : strcmp(rk, "sva7w4") == 0 ? RESID_SVA7_W4 : -1;
   const char* fk = getenv("COSAHIP_FUSED_UPDATE");
   s->fuse_update = !(fk && strcmp(fk, "0") == 0);
   const char* gk = getenv("COSAHIP_GRAPH");
@@ -810,6 +809,7 @@ static ll field_size(const Solve* s, const Level& L, int field) {
     case COSAHIP_DTVOLT: return p.kom ? L.n1tot * p.nsnap : 0;
     case COSAHIP_X: case COSAHIP_Y: case COSAHIP_Z: return L.n2tot * p.nsnap_met;
     case COSAHIP_TTRM: return p.sst ? L.n3tot * 3 * p.nsnap : 0;
+    case COSAHIP_Q1: case COSAHIP_Q2: return p.dualt && L.nl == 1 ? L.n3tot * p.npde : 0;
   }
   if (field >= COSAHIP_XIDERI && field <= COSAHIP_ZETADERK) return p.viscous ? L.n0tot * 3 * p.nsnap_met : 0;
   return -1;
@@ -1691,6 +1691,76 @@ extern "C" int cosahip_residual(cosahip_solve* h, int nl) {
   CK(launch_residual(*s, *L, L->f[COSAHIP_Q], L->f[COSAHIP_RES]));
   return COSAHIP_OK;
 }
+extern "C" int cosahip_src_uns(cosahip_solve* h, int nl) {
+  LEVEL_OR_FAIL(s, nl, L);
+  if (!s->par.dualt) return fail(s, COSAHIP_ERR_STATE, "src_uns: dual time stepping is off (cosahip_set_dual_time)");
+  CK(launch_src_uns(*s, *L, 0));
+  return COSAHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// dual time stepping (src/cosa.f90:437-672)
+
+static void drop_graph(Solve* s) {
+  if (s->gexec) {
+    (void)hipGraphExecDestroy(s->gexec);
+    s->gexec = nullptr;
+  }
+}
+
+extern "C" int cosahip_set_dual_time(cosahip_solve* h, double dt, int dual_prop_start) {
+  LEVEL_OR_FAIL(s, 1, L);
+  if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "set_dual_time: an enqueued cycle has not been waited for");
+  if (s->cfg.harbal || s->par.nsnap != 1)
+    return fail(s, COSAHIP_ERR_ARG, "set_dual_time: dual time stepping excludes harmonic balance");
+  if (!(dt > 0)) return fail(s, COSAHIP_ERR_ARG, "set_dual_time: dt must be > 0");
+  CK(hipSetDevice(s->device));
+  CK(hipStreamSynchronize(s->stream));
+  const bool was = s->par.dualt != 0;
+  s->par.dualt = 1;
+  s->par.dual_dt = dt;
+  s->par.dual_rkimp = s->cfg.rkimp ? 1 : 0;
+  s->par.dual_plain = s->cfg.rkimp && s->cfg.irsop == 0 && s->cfg.nlevel == 1;
+  s->par.dual_fact = dual_prop_start ? 1.5 : 1.0;
+  s->dual_prop = dual_prop_start ? 1 : 0;
+  if (!was) {
+    const ll n = field_size(s, *L, COSAHIP_Q1);
+    for (int f : {COSAHIP_Q1, COSAHIP_Q2}) {
+      if (L->f[f]) CK(hipFree(L->f[f]));
+      L->f[f] = nullptr;
+      if (hipMalloc(&L->f[f], n * sizeof(double)) != hipSuccess) {
+        s->par.dualt = 0;
+        return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
+      }
+      CK(hipMemset(L->f[f], 0, n * sizeof(double)));
+    }
+  }
+  drop_graph(s);
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_dual_begin(cosahip_solve* h, int itime) {
+  LEVEL_OR_FAIL(s, 1, L);
+  if (!s->par.dualt) return fail(s, COSAHIP_ERR_STATE, "dual_begin: dual time stepping is off (cosahip_set_dual_time)");
+  if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "dual_begin: an enqueued cycle has not been waited for");
+  if (itime < 1) return fail(s, COSAHIP_ERR_ARG, "dual_begin: itime >= 1");
+  CK(hipSetDevice(s->device));
+  const double fact = (itime == 1 && !s->dual_prop) ? 1.0 : 1.5;
+  if (fact != s->par.dual_fact) drop_graph(s);   // the kernels take the factor by value
+  s->par.dual_fact = fact;
+  CK(launch_rhs_uns(*s, *L));
+  return COSAHIP_OK;
+}
+
+extern "C" int cosahip_dual_end(cosahip_solve* h) {
+  LEVEL_OR_FAIL(s, 1, L);
+  if (!s->par.dualt) return fail(s, COSAHIP_ERR_STATE, "dual_end: dual time stepping is off (cosahip_set_dual_time)");
+  if (s->cycle_pending) return fail(s, COSAHIP_ERR_STATE, "dual_end: an enqueued cycle has not been waited for");
+  CK(hipSetDevice(s->device));
+  CK(launch_copy_interior(*s, *L, L->f[COSAHIP_Q2], L->f[COSAHIP_Q1]));
+  return COSAHIP_OK;
+}
+
 extern "C" int cosahip_src_hb(cosahip_solve* h, int nl) {
   LEVEL_OR_FAIL(s, nl, L);
   if (!s->cfg.harbal) return COSAHIP_OK;
@@ -1702,13 +1772,20 @@ extern "C" int cosahip_src_hb(cosahip_solve* h, int nl) {
 // (src/cosa.f90:1916-1921); the norm adds the HB term itself
 static bool rkim_plain(const cosahip_config& c) { return c.harbal && c.rkimp && c.irsop == 0 && c.nlevel == 1; }
 
-// resg_rms partial sums of level L, copied into pinned slot `slot` (no wait)
-static int rms_enqueue(Solve* s, Level& L, int slot) {
+// resg_rms partial sums of level L, copied into pinned slot `slot` (no
+// wait).  res_minus_rhs: res already holds res - rhs (the dual-time finest
+// level inside smooth), so the norm is of res itself -- the same bits.
+static int rms_enqueue(Solve* s, Level& L, int slot, bool res_minus_rhs = false) {
   const int np = s->par.npde;
   if (s->cfg.harbal) {
     CK(launch_copy_interior(*s, L, L.work1, L.f[COSAHIP_RES]));
     if (rkim_plain(s->cfg)) CK(launch_rms_hb_term(*s, L, L.work1));
     CK(launch_rms(*s, L, L.work1, nullptr, 0));
+  } else if (s->par.dualt && s->par.dual_plain) {
+    CK(launch_rms_dual_term(*s, L, L.work1));
+    CK(launch_rms(*s, L, L.work1, nullptr, 0));
+  } else if (res_minus_rhs) {
+    CK(launch_rms(*s, L, L.f[COSAHIP_RES], nullptr, 0));
   } else {
     CK(launch_rms(*s, L, L.f[COSAHIP_RES], L.f[COSAHIP_RHS], 1));
   }
@@ -1739,8 +1816,8 @@ static void rms_finish(Solve* s, Level& L, int slot, double* out) {
   }
 }
 
-static int rms(Solve* s, Level& L, double* out) {
-  int rc = rms_enqueue(s, L, 0);
+static int rms(Solve* s, Level& L, double* out, bool res_minus_rhs = false) {
+  int rc = rms_enqueue(s, L, 0, res_minus_rhs);
   if (rc) return rc;
   CK(hipStreamSynchronize(s->stream));
   rms_finish(s, L, 0, out);
@@ -1807,6 +1884,8 @@ static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter
   Level& L = s->lev[nl - 1];
   const cosahip_config& c = s->cfg;
   if (done) *done = 0;
+  if (c.rkimp && !c.harbal && !s->par.dualt)
+    return fail(s, COSAHIP_ERR_STATE, "rkimp without harmonic balance needs dual time stepping (cosahip_set_dual_time)");
   double* q = L.f[COSAHIP_Q];
   const int niterp = to_coarse ? niter + 1 : niter;
   s->mgit = mgit;
@@ -1816,7 +1895,7 @@ static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter
   // the RK update folded into the last CIRS sweep, and with it the next
   // stage's cons2prim (mean flow with cirs: the state is the same bits)
   const bool fuse_update = s->fuse_update && !komega(c) && s->par.irsop >= 1 && s->par.irsop != 4 &&
-                           s->par.irsop <= 6 && c.cutcirs != 1 && !(c.harbal && c.rkimp);
+                           s->par.irsop <= 6 && c.cutcirs != 1 && !c.rkimp;
   bool q_prim = false;   // the previous stage left q in primitive variables
   for (int iter = 1; iter <= niterp && !returned_early; iter++) {
     const ll cs = L.ncell * s->par.nsnap;
@@ -1863,16 +1942,30 @@ static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter
         TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
         CK(launch_src_hb(*s, L));
       }
+      // dual time (src/cosa.f90:1910-1915, 1938-1941): src_uns, and on the
+      // finest level the overall residual res - rhs, in one pass; without
+      // IRS on one level, rkimp leaves src_uns out and subtracts after the norm
+      const bool dual = s->par.dualt != 0;
+      const bool dual_src = dual && !s->par.dual_plain;
+      if (dual_src) {
+        TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
+        CK(launch_src_uns(*s, L, nl == 1));
+      }
+      const bool sub_done = dual_src && nl == 1;
       if (async && nl == 1 && irk == 1 && iter != niter + 1) {
-        int rc = rms_enqueue(s, L, RMS_ASYNC_SLOT);
+        int rc = rms_enqueue(s, L, RMS_ASYNC_SLOT, sub_done);
         if (rc) return rc;
       } else if (nl == 1 && irk == 1 && iter != niter + 1 &&
                  (mgit == 1 || mgit % c.iprint == 0 || mgit == c.ncycle)) {
         double rr[8];
-        int rc = rms(s, L, rr);
+        int rc = rms(s, L, rr, sub_done);
         if (rc) return rc;
         for (int v = 0; v <= s->par.npde; v++) s->resrms[v + 1] = rr[v];
         if (s->prec_mode == 2 && !s->prec_tail && s->resrms[1] < s->prec_tail_rms) s->prec_tail = true;
+      }
+      if (dual && nl == 1 && !sub_done) {
+        TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
+        CK(launch_add_interior(*s, L, L.f[COSAHIP_RES], L.f[COSAHIP_RHS], -1.0));
       }
       if (from_fine && iter == 1 && irk == 1) {
         TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
@@ -2056,7 +2149,8 @@ extern "C" int cosahip_cycle_enqueue(cosahip_solve* h, int mgit) {
   // visit) and runs eagerly; later cycles replay one captured graph
   // (a CFL ramp changes the kernels' arguments from cycle to cycle: no graph)
   const bool graph = s->use_graph && mgit >= 2 && !s->timing && !s->cfg.ramping;
-  const int key = s->fp32() ? 32 : 64;
+  // the key names what the captured kernels take by value and launch
+  const int key = (s->fp32() ? 32 : 64) + (s->par.dualt ? (s->par.dual_fact == 1.0 ? 1000 : 2000) : 0);
   if (graph && s->gexec && s->gkey == key) {
     CK(hipGraphLaunch(s->gexec, s->stream));
   } else if (graph) {

@@ -53,6 +53,11 @@ struct Par {
   // 2 SST, with the BSL/SST constants of input (src/input.f90:2523-2541)
   int sst;
   double sst_a1, sigk1, sigk2, sigw1, sigw2, b1, b2, g1, g2;
+  // dual time stepping: on, the rkimp update, rkimp without IRS on one
+  // level (the physical-time term only in the update); dual_fact 1 on the
+  // first step of a backward-Euler start, else 1.5; dual_dt the physical step
+  int dualt, dual_rkimp, dual_plain;
+  double dual_fact, dual_dt;
 };
 
 __host__ __device__ inline ll n3of(const Blk& b) { return (ll)(b.imax + 3) * (b.jmax + 3) * (b.kmax + 3); }
@@ -107,6 +112,9 @@ enum ResidVariant {
   RESID_TILE7_REG,     // k_residual_tile: specialised, register k-window, 31 x 7 tiles
   RESID_TILE15,        // k_residual_tile: specialised, k-window from LDS / L2, 31 x 15 tiles
   RESID_TILE7_W4,      // RESID_TILE7 held to 128 VGPRs (4 waves per SIMD)
+  RESID_SVA7,          // k_residual_sva: van Albada slope once per cell, 31 x 7 tiles (else as RESID_TILE7)
+  RESID_SVA15,         // k_residual_sva, 31 x 15 tiles (else as RESID_TILE15)
+  RESID_SVA7_W4,       // RESID_SVA7 held to 128 VGPRs (4 waves per SIMD)
   RESID_NVARIANT
 };
 
@@ -275,7 +283,7 @@ struct Solve {
   // residual kernel choice: fused single pass (default) or one pass per
   // direction (COSAHIP_RESIDUAL=split in the environment at create time)
   bool split_residual = false;
-  int resid_force = -1;   // COSAHIP_RESIDUAL=fused|tile7|tile7reg|tile15|tile7w4: that kernel on every level
+  int resid_force = -1;   // COSAHIP_RESIDUAL=fused|tile7|tile7reg|tile15|tile7w4|sva7|sva15|sva7w4: that kernel on every level
   // viscous face fluxes after the inviscid pass of each direction: face
   // fluxes stored then differenced (default, measured fastest), flux and
   // difference in one marching kernel (COSAHIP_VISCOUS=march), or fused into
@@ -306,6 +314,7 @@ struct Solve {
   ll ck_cap = 0;
   bool ck_pending = false;
   int mgit = 1;                 // cycle counter of the last smooth (CFL ramping)
+  int dual_prop = 0;            // dual time stepping: two-state start (dual_prop_start)
   bool use_graph = true;
   hipGraphExec_t gexec = nullptr;
   int gkey = -1;

@@ -150,6 +150,13 @@ __device__ __forceinline__ void roe_turb(const Par& p, const T* qr, const T* ql,
 }
 
 // ------------------------------------------------------------------------
+// Van Albada slope of the cell between differences d1 (upper face) and d2
+// (lower face), epsq = epslim^2 (limit_i ilim 4, src/flux.f90:3584-3604).
+template <typename T = double>
+__device__ __forceinline__ T va_slope(T epsq, T d1, T d2) {
+  return (d1 * (d2 * d2 + epsq) + d2 * (d1 * d1 + epsq)) / (d1 * d1 + d2 * d2 + 2 * epsq);
+}
+
 // Limited slopes of one variable at face a: dqp(a), dqm(a) from
 // dm1 = dq(a-1), d0 = dq(a), dp1 = dq(a+1)   (limit_i/j/k).
 template <typename T = double>
@@ -186,14 +193,9 @@ __device__ __forceinline__ void limit_face(int ilim, T eps, T rkap, T dm1, T d0,
     T dqmx = (d1 * (d2 * d2 + 2 * epsq) + d2 * (2 * d1 * d1 + epsq)) / (2 * d1 * d1 - d1 * d2 + 2 * d2 * d2 + 3 * epsq);
     dqm = dqmx / 2;
   } else {
-    T epsq = eps * eps;
-    T d1 = dp1, d2 = d0;
-    T dqmx = (d1 * (d2 * d2 + epsq) + d2 * (d1 * d1 + epsq)) / (d1 * d1 + d2 * d2 + 2 * epsq);
-    dqp = -dqmx / 2;
-    d1 = d0;
-    d2 = dm1;
-    dqmx = (d1 * (d2 * d2 + epsq) + d2 * (d1 * d1 + epsq)) / (d1 * d1 + d2 * d2 + 2 * epsq);
-    dqm = dqmx / 2;
+    const T epsq = eps * eps;
+    dqp = -va_slope<T>(epsq, dp1, d0) / 2;
+    dqm = va_slope<T>(epsq, d0, dm1) / 2;
   }
 }
 

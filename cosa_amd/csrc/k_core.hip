@@ -253,6 +253,142 @@ __global__ void __launch_bounds__(256) k_update(Par p, const Blk* __restrict__ b
   for (int v = 0; v < 5; v++) qb[x + v * n3] = ob[x + v * n3] - ar * rb[x + v * n3] * dt;
 }
 
+// ---- dual time stepping (one snapshot), interior cells ----------------------
+// src_uns_b (src/src.f90:406-450): res += q vol / dt (first step of a
+// backward-Euler start, dual_fact 1) or 3 q vol / 2 / dt; then, on the
+// finest level, the overall residual res - rhs of src/cosa.f90:1938-1941.
+__global__ void __launch_bounds__(256) k_src_uns(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
+                                                 const double* __restrict__ q, const double* __restrict__ vol,
+                                                 double* __restrict__ res, const double* __restrict__ rhs,
+                                                 int sub_rhs) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  CellIdx c;
+  if (!cell_of(p, b, t.base + threadIdx.x, c)) return;
+  const ll n3 = n3of(b), x = i3(b, c.i, c.j, c.k);
+  const double* qb = qblk(p, b, q, 0);
+  double* rb = qblk(p, b, res, 0);
+  const double* hb = qblk(p, b, rhs, 0);
+  const double v = vol[b.o1 + i1(b, c.i, c.j, c.k)], dt = p.dual_dt;
+  const bool first = p.dual_fact == 1.0;
+  for (int m = 0; m < p.npde; m++) {
+    const ll y = x + m * n3;
+    double r = first ? rb[y] + qb[y] * v / dt : rb[y] + 3 * qb[y] * v / 2 / dt;
+    if (sub_rhs) r = r + hb[y] * -1.0;
+    rb[y] = r;
+  }
+}
+
+// q1 = q (copy_array, interior) and rhs_uns_b (src/src.f90:480-525) of the
+// finest level: q1 / dt vol, or (2 q1 - q2 / 2) vol / dt.
+__global__ void __launch_bounds__(256) k_rhs_uns(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
+                                                 const double* __restrict__ q, double* __restrict__ q1,
+                                                 const double* __restrict__ q2, const double* __restrict__ vol,
+                                                 double* __restrict__ rhs) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  CellIdx c;
+  if (!cell_of(p, b, t.base + threadIdx.x, c)) return;
+  const ll n3 = n3of(b), x = i3(b, c.i, c.j, c.k);
+  const double* qb = qblk(p, b, q, 0);
+  double* ab = qblk(p, b, q1, 0);
+  const double* pb = qblk(p, b, q2, 0);
+  double* hb = qblk(p, b, rhs, 0);
+  const double v = vol[b.o1 + i1(b, c.i, c.j, c.k)], dt = p.dual_dt;
+  const bool first = p.dual_fact == 1.0;
+  for (int m = 0; m < p.npde; m++) {
+    const ll y = x + m * n3;
+    const double a = qb[y];
+    ab[y] = a;
+    hb[y] = first ? a * v / dt : (2 * a - pb[y] / 2) * v / dt;
+  }
+}
+
+// resg_brms, rkimp without IRS on one level (src/cosa.f90:4534-4549): the
+// summand fact q vol / dt + res - rhs, into work1 (interior)
+__global__ void __launch_bounds__(256) k_rms_dual_term(Par p, const Blk* __restrict__ blks,
+                                                       const Tile* __restrict__ tiles, const double* __restrict__ q,
+                                                       const double* __restrict__ res, const double* __restrict__ rhs,
+                                                       const double* __restrict__ vol, double* __restrict__ work1) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  CellIdx c;
+  if (!cell_of(p, b, t.base + threadIdx.x, c)) return;
+  const ll n3 = n3of(b), x = i3(b, c.i, c.j, c.k);
+  const double* qb = qblk(p, b, q, 0);
+  const double* rb = qblk(p, b, res, 0);
+  const double* hb = qblk(p, b, rhs, 0);
+  double* wb = qblk(p, b, work1, 0);
+  const double v = vol[b.o1 + i1(b, c.i, c.j, c.k)];
+  for (int m = 0; m < p.npde; m++) {
+    const ll y = x + m * n3;
+    wb[y] = p.dual_fact * qb[y] * v / p.dual_dt + rb[y] - hb[y];
+  }
+}
+
+// update_b, rkimp branches (src/update.f90:649-762): the physical-time
+// term implicit.  dual_plain (no IRS, one level): the residual carries no
+// src_uns and the term enters the denominator; otherwise the residual holds
+// it and the update adds it back before dividing.  k-omega first, with the
+// point-implicit rho*omega, rho*k of the steady update.
+__global__ void __launch_bounds__(256) k_update_dual(Par p, const Blk* __restrict__ blks,
+                                                     const Tile* __restrict__ tiles, const double* __restrict__ res,
+                                                     double* __restrict__ q, const double* __restrict__ qold,
+                                                     const double* __restrict__ dtvol, double ar,
+                                                     const double* __restrict__ vol,
+                                                     const double* __restrict__ dtvolt,
+                                                     const double* __restrict__ delpls,
+                                                     const double* __restrict__ prd, const double* __restrict__ ttrm) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  CellIdx c;
+  if (!cell_of(p, b, t.base + threadIdx.x, c)) return;
+  const ll n3 = n3of(b), x = i3(b, c.i, c.j, c.k);
+  const ll po = b.o1 + i1(b, c.i, c.j, c.k);
+  const double dv = dtvol[po], v = vol[po], fact = p.dual_fact, dt = p.dual_dt;
+  double* qb = qblk(p, b, q, 0);
+  const double* ob = qblk(p, b, qold, 0);
+  const double* rb = qblk(p, b, res, 0);
+  if (p.kom) {
+    const double rho = qb[x];
+    const double ome = qb[x + 6 * n3] / qb[x];
+    const double dtt = dtvolt[po];
+    const double dtau = dtt * v;
+    const ll so = b.o3 + x;
+    const double* tb = p.sst ? ttrm + b.o3 * 3 + x : nullptr;
+    const double gav = tb ? tb[n3] : p.gkw, bav = tb ? tb[2 * n3] : p.bkw;
+    const double dl = delpls[so];
+    const double q7 = qb[x + 6 * n3];
+    double rhomega;
+    if (p.dual_plain)
+      rhomega = (ob[x + 6 * n3] - ar * rb[x + 6 * n3] * dtt + ar * dtau * q7 * (gav * dl + 2 * bav * ome)) /
+                (1 + ar * dtau * (gav * dl + 2 * bav * ome) + fact * ar * dtau / dt);
+    else
+      rhomega = (ob[x + 6 * n3] + ar * fact * dtau / dt * q7 - ar * rb[x + 6 * n3] * dtt +
+                 ar * dtau * q7 * (gav * dl + 2 * bav * ome)) /
+                (1 + ar * dtau * (gav * dl + 2 * bav * ome) + ar * fact * dtau / dt);
+    const double rhomegamin = gav * p.cmu * rho * sqrt(prd[so]);
+    if (rhomega < rhomegamin) rhomega = rhomegamin;
+    qb[x + 6 * n3] = rhomega;
+    const double drhomega = rhomega - rho * ome;
+    const double q6 = qb[x + 5 * n3];
+    if (p.dual_plain)
+      qb[x + 5 * n3] = (ob[x + 5 * n3] - ar * rb[x + 5 * n3] * dtt +
+                        ar * dtau * q6 * (dl + p.bstrkw * ome - p.bstrkw * drhomega / rho)) /
+                       (1 + ar * dtau * (dl + p.bstrkw * ome) + ar * fact * dtau / dt);
+    else
+      qb[x + 5 * n3] = (ob[x + 5 * n3] + ar * fact * dtau / dt * q6 - ar * rb[x + 5 * n3] * dtt +
+                        ar * dtau * q6 * (dl + p.bstrkw * ome - p.bstrkw * drhomega / rho)) /
+                       (1 + ar * dtau * (dl + p.bstrkw * ome) + ar * fact * dtau / dt);
+  }
+  const double den = 1 + fact * ar * dv * v / dt;
+  for (int m = 0; m < 5; m++) {
+    const ll y = x + m * n3;
+    if (p.dual_plain) qb[y] = (ob[y] - ar * rb[y] * dv) / den;
+    else qb[y] = (ob[y] - ar * dv * (-(fact * v / dt * qb[y]) + rb[y])) / den;
+  }
+}
+
 // ---- src_bhb (src/src.f90:556-613): qhb = omega vol D q, res += qhb ----
 __global__ void __launch_bounds__(256) k_src_hb(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
                                                 const double* __restrict__ q, const double* __restrict__ vol,
@@ -1164,8 +1300,35 @@ hipError_t launch_rms_hb_term(Solve& s, Level& L, double* work1) {
                        L.tile[TK_CELLS], work1, L.f[COSAHIP_Q], L.f[COSAHIP_VOL], s.ddhb);
   return hipGetLastError();
 }
+hipError_t launch_src_uns(Solve& s, Level& L, int sub_rhs) {
+  if (L.ntile[TK_CELLS])
+    hipLaunchKernelGGL(k_src_uns, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_CELLS],
+                       L.f[COSAHIP_Q], L.f[COSAHIP_VOL], L.f[COSAHIP_RES], L.f[COSAHIP_RHS], sub_rhs);
+  return hipGetLastError();
+}
+hipError_t launch_rhs_uns(Solve& s, Level& L) {
+  if (L.ntile[TK_CELLS])
+    hipLaunchKernelGGL(k_rhs_uns, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_CELLS],
+                       L.f[COSAHIP_Q], L.f[COSAHIP_Q1], L.f[COSAHIP_Q2], L.f[COSAHIP_VOL], L.f[COSAHIP_RHS]);
+  return hipGetLastError();
+}
+hipError_t launch_rms_dual_term(Solve& s, Level& L, double* work1) {
+  if (L.ntile[TK_CELLS])
+    hipLaunchKernelGGL(k_rms_dual_term, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                       L.tile[TK_CELLS], L.f[COSAHIP_Q], L.f[COSAHIP_RES], L.f[COSAHIP_RHS], L.f[COSAHIP_VOL],
+                       work1);
+  return hipGetLastError();
+}
 hipError_t launch_update(Solve& s, Level& L, double ar) {
   if (s.cfg.harbal && s.cfg.rkimp) return launch_update_rkim(s, L, ar);
+  if (s.par.dual_rkimp) {
+    if (L.ntile[TK_CELLS])
+      hipLaunchKernelGGL(k_update_dual, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                         L.tile[TK_CELLS], L.f[COSAHIP_RES], L.f[COSAHIP_Q], L.f[COSAHIP_QOLD], L.f[COSAHIP_DTVOL],
+                         ar, L.f[COSAHIP_VOL], L.f[COSAHIP_DTVOLT], L.f[COSAHIP_DELPLS], L.f[COSAHIP_PRD],
+                         L.f[COSAHIP_TTRM]);
+    return hipGetLastError();
+  }
   if (L.ntile[TK_CELLS])
     hipLaunchKernelGGL(k_update, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_CELLS],
                        L.f[COSAHIP_RES], L.f[COSAHIP_Q], L.f[COSAHIP_QOLD], L.f[COSAHIP_DTVOL], ar,

@@ -552,7 +552,9 @@ __global__ void __launch_bounds__(256) k_fixq(Par p, const Blk* __restrict__ blk
 // lim_prod_mg_b then lim_res_ome (restriction stage), interior, every
 // snapshot, q conservative.  lim: 1 lim_res_ome_b (steady), 2 the explicit
 // branch of lim_res_ome_bhb (src/update.f90:2880-2905: no RK coefficient,
-// from qold), 0 none (the rkim branch runs as its own kernel).
+// from qold), 3 the rkimp branch of lim_res_ome_b (dual time with IRS or
+// multigrid), 0 none (HB rkim runs as its own kernel; dual-time rkimp
+// without IRS on one level has no limit).
 __global__ void __launch_bounds__(256) k_turb_limits(Par p, int do_prod, int lim, double ar,
                                                      const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
                                                      const double* __restrict__ q, const double* __restrict__ qold,
@@ -594,6 +596,11 @@ __global__ void __launch_bounds__(256) k_turb_limits(Par p, int do_prod, int lim
       const double den1 = 1 + ar * dtau * (gav * delpls[so] + 2 * bav * ome);
       const double rhomega = qb[c + 6 * n3] - ar * rb[c + 6 * n3] * dtt / den1;
       if (rhomega < rhomegamin) rb[c + 6 * n3] = den1 * (qb[c + 6 * n3] - rhomegamin) / dtt / ar;
+    } else if (lim == 3) {   // dual time, rkimp: src/update.f90:2753-2794 (den1 without ar)
+      const double den1 = 1 + dtau * (gav * delpls[so] + 2 * bav * ome);
+      const double den2 = ar * p.dual_fact * dtau / p.dual_dt;
+      const double rhomega = qb[c + 6 * n3] - ar * rb[c + 6 * n3] * dtt / (den1 + den2);
+      if (rhomega < rhomegamin) rb[c + 6 * n3] = ((den1 + den2) * (qb[c + 6 * n3] - rhomegamin)) / dtt / ar;
     } else if (lim == 2) {
       const double* ob = qold + b.o3 * p.npde * p.nsnap + n3 * p.npde * n;
       const double rhomega =
@@ -800,7 +807,7 @@ hipError_t launch_fixq(Solve& s, Level& L) {
 hipError_t launch_turb_limits(Solve& s, Level& L, double ar) {
   const Par& p = s.par;
   const int do_prod = p.nlevel > 1 && L.nl != p.nlevel && !p.prd_all_lvls && p.lim_prodtke;
-  const int lim = !s.cfg.harbal ? 1 : (s.cfg.rkimp ? 0 : 2);
+  const int lim = p.dual_rkimp ? (p.dual_plain ? 0 : 3) : !s.cfg.harbal ? 1 : (s.cfg.rkimp ? 0 : 2);
   if (L.ntile[TK_CELLS])
     hipLaunchKernelGGL(k_turb_limits, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, p, do_prod, lim, ar, L.dblk,
                        L.tile[TK_CELLS], L.f[COSAHIP_Q], L.f[COSAHIP_QOLD], L.f[COSAHIP_RES], L.f[COSAHIP_VOL],

@@ -20,6 +20,11 @@ hipError_t launch_update(Solve& s, Level& L, double ar);
 int rkim_group(int m);
 hipError_t launch_lim_res_ome_rkim(Solve& s, Level& L);
 hipError_t launch_rms_hb_term(Solve& s, Level& L, double* work1);
+// dual time stepping: res += src_uns(q) (then, sub_rhs, res -= rhs);
+// q1 = q and rhs = rhs_uns(q1, q2) on level 1; work1 = the rkimp norm term
+hipError_t launch_src_uns(Solve& s, Level& L, int sub_rhs);
+hipError_t launch_rhs_uns(Solve& s, Level& L);
+hipError_t launch_rms_dual_term(Solve& s, Level& L, double* work1);
 hipError_t launch_src_hb(Solve& s, Level& L);
 hipError_t launch_rms(Solve& s, Level& L, const double* res, const double* rhs, int use_rhs);
 hipError_t launch_prtest(Solve& s, Level& L, const double* q, int stage_seq);

@@ -24,6 +24,7 @@ module cosahip_binding
   integer(c_int), parameter :: COSAHIP_ZETADERK = 25, COSAHIP_PRD = 26, COSAHIP_DIVVEL = 27
   integer(c_int), parameter :: COSAHIP_DELPLS = 28, COSAHIP_DTVOLT = 29
   integer(c_int), parameter :: COSAHIP_X = 30, COSAHIP_Y = 31, COSAHIP_Z = 32, COSAHIP_TTRM = 33
+  integer(c_int), parameter :: COSAHIP_Q1 = 34, COSAHIP_Q2 = 35
 
   ! cosahip_config, member for member
   type, bind(C) :: cosahip_config
@@ -107,6 +108,27 @@ module cosahip_binding
       integer(c_int), value :: nl, from_fine, to_coarse, niter, mgit
       integer(c_int32_t), intent(out) :: done
       real(c_double), intent(out) :: resrms(8)
+    end function
+    ! dual time stepping: set up once, then begin / mg() / end per physical step
+    integer(c_int) function cosahip_set_dual_time(solve, dt, dual_prop_start) bind(C, name='cosahip_set_dual_time')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: solve
+      real(c_double), value :: dt
+      integer(c_int), value :: dual_prop_start
+    end function
+    integer(c_int) function cosahip_dual_begin(solve, itime) bind(C, name='cosahip_dual_begin')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: solve
+      integer(c_int), value :: itime
+    end function
+    integer(c_int) function cosahip_dual_end(solve) bind(C, name='cosahip_dual_end')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: solve
+    end function
+    integer(c_int) function cosahip_src_uns(solve, nl) bind(C, name='cosahip_src_uns')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: solve
+      integer(c_int), value :: nl
     end function
     integer(c_int) function cosahip_restrict(solve, nl) bind(C, name='cosahip_restrict')
       import :: c_int, c_ptr

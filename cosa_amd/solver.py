@@ -303,7 +303,8 @@ class Solve:
         self._check(self.lib.cosahip_autotune(self.h, reps, ch), "autotune")
         names = {0: None, 1: "lds", 2: "per_variable", 3: "line_resident"}
         rnames = {-1: None, 0: "generic_fused", 1: "tile31x7", 2: "tile31x7_register_window", 3: "tile31x15",
-                  4: "tile31x7_128vgpr"}
+                  4: "tile31x7_128vgpr", 5: "tile31x7_shared_slopes", 6: "tile31x15_shared_slopes",
+                  7: "tile31x7_shared_slopes_128vgpr"}
         return {"cirs_i_sweep": [names[ch[m]] for m in range(3)], "viscous_faces_stored": bool(ch[3]),
                 "residual": [rnames.get(ch[4 + m]) for m in range(3)]}
 
@@ -369,6 +370,24 @@ class Solve:
         if nlevel > 1:
             self.smooth(1, False, False, npost, ncycle)
         return ncycle
+
+    # -- dual time stepping ----------------------------------------------------
+    def set_dual_time(self, dt: float, prop_start: bool = False):
+        """Switch the solve to dual time stepping (src/cosa.f90:437-672):
+        physical step dt; prop_start: two-state start (upload q2 first),
+        else backward Euler on the first step.  config rkimp selects the
+        implicit physical-time term of the RK stages."""
+        self._check(self.lib.cosahip_set_dual_time(self.h, float(dt), int(bool(prop_start))), "set_dual_time")
+
+    def dual_step(self, itime: int, ncycle: Optional[int] = None, nstart: int = 1, npre: int = 1, npost: int = 1,
+                  ncrs: int = 1, hist=None) -> int:
+        """One physical step (src/cosa.f90:640-652): q1 = q and rhs =
+        rhs_uns(q1, q2), the pseudo-time cycles of mg, then q2 = q1.
+        Returns the cycles run."""
+        self._check(self.lib.cosahip_dual_begin(self.h, int(itime)), "dual_begin")
+        n = self.mg(ncycle, nstart, npre, npost, ncrs, hist=hist)
+        self._check(self.lib.cosahip_dual_end(self.h), "dual_end")
+        return n
 
     # -- instrumentation -------------------------------------------------------
     def timing(self, on: bool):

@@ -158,7 +158,10 @@ enum {
   COSAHIP_TTRM = 33,    /* BSL/SST blended coefficients (P3, 3: cross diffusion,
                            gamma, beta; 0:2N), refreshed by every residual;
                            download only                                     */
-  COSAHIP_NFIELDS = 34
+  COSAHIP_Q1 = 34,      /* dual time stepping, finest level: q^n (P3, npde);
+                           allocated by cosahip_set_dual_time (else empty)  */
+  COSAHIP_Q2 = 35,      /* dual time stepping, finest level: q^(n-1)       */
+  COSAHIP_NFIELDS = 36
 };
 
 typedef struct cosahip_solve cosahip_solve;
@@ -178,6 +181,26 @@ int cosahip_error_info(const cosahip_solve* s, int32_t info[8]);
 /* HB time-derivative operator dhb(0:2N,0:2N) as built by mathb
  * (src/input.f90:5206-5282); Fortran column-major, nsnap = 2N+1.          */
 int cosahip_set_hb_operator(cosahip_solve* s, int nsnap, const double* dhb);
+
+/* Dual time stepping (`unsteady dual dt ... rkex/rkim` of input.dat,
+ * src/input.f90:2801-2874; the physical-time loop src/cosa.f90:437-672).
+ * cosahip_set_dual_time switches the solve to it once level 1 is defined:
+ * dt the physical time step, dual_prop_start 1 for the two-state start
+ * (restart holding q^(n-1)), 0 for a backward-Euler first step.  It
+ * allocates COSAHIP_Q1/Q2 on level 1 (zeroed; upload Q2 for a two-state
+ * start).  cfg.rkimp then selects the implicit physical-time term of
+ * update_b / lim_res_ome_b (src/update.f90:649-762, 2753-2794) instead of
+ * harmonic balance's rkim.  Harmonic balance excludes it.
+ * Per physical step, around the multigrid cycles of mg():
+ *   cosahip_dual_begin(s, itime): q1 = q, rhs = rhs_uns(q1, q2) on level 1
+ *     (src/cosa.f90:640-650, src/src.f90:453-525);
+ *   cosahip_dual_end(s): q2 = q1 (src/cosa.f90:651-652).
+ * Inside smooth the reference's dualt branches run: src_uns on every level
+ * (without it on one level without IRS when rkimp), the finest residual
+ * minus rhs, and the rkimp norm (src/cosa.f90:1910-1941, 4534-4549).      */
+int cosahip_set_dual_time(cosahip_solve* s, double dt, int dual_prop_start);
+int cosahip_dual_begin(cosahip_solve* s, int itime);
+int cosahip_dual_end(cosahip_solve* s);
 
 /* Level topology: block node counts i_imax/j_jmax/k_kmax of level nl
  * (src/input.f90:4393-4434), BC records bctopo(10,nbcs(b)) of setdim_bbc
@@ -242,6 +265,9 @@ int cosahip_lambda_deltat(cosahip_solve* s, int nl);
 int cosahip_residual(cosahip_solve* s, int nl);
 /* src_hb (src/src.f90:530-613): res += omega*vol*D*q, q conservative.     */
 int cosahip_src_hb(cosahip_solve* s, int nl);
+/* src_uns (src/src.f90:381-450): res += q*vol/dt on the first step of a
+ * first-order start, else 3*q*vol/2/dt; q conservative (dual time).       */
+int cosahip_src_uns(cosahip_solve* s, int nl);
 /* resg_rms / resg_rms_hb (src/cosa.f90:4458-4697) -> resrms(1:npde+1).    */
 int cosahip_resg_rms(cosahip_solve* s, int nl, double* resrms);
 /* add_array(..., off_l=-1, fact) on the interior (src/util.f90:137-236):
@@ -315,10 +341,13 @@ int cosahip_cycle_ready(cosahip_solve* s);
  * choice[3]: 1 stored face fluxes, 0 march; choice[4..6]: inviscid
  * residual kernel of levels 1..3 (0 generic fused, 1 specialised 31x7
  * tiles with the k-window from LDS/L2, 2 the same with a register
- * k-window, 3 31x15 tiles, 4 as 1 held to 128 VGPRs; -1 level absent);
+ * k-window, 3 31x15 tiles, 4 as 1 held to 128 VGPRs, 5 / 6 31x7 / 31x15
+ * tiles with the van Albada slope evaluated once per cell, 7 as 5 held to
+ * 128 VGPRs; -1 level absent);
  * choice[7] is reserved (0).  choice holds 8 entries.
  * Environment switches at create time take precedence:
- * COSAHIP_RESIDUAL=fused|tile7|tile7reg|tile15|tile7w4|split forces the
+ * COSAHIP_RESIDUAL=fused|tile7|tile7reg|tile15|tile7w4|sva7|sva15|sva7w4|split
+ * forces the
  * inviscid residual kernel, COSAHIP_CIRS_I=lds|iv|line forces the i-sweep, COSAHIP_CIRS_IL_LDS sets
  * the LDS bytes per work-group of the line-resident sweep (4096..65536,
  * default 40960), COSAHIP_VISCOUS=march|fused the viscous arrangement.    */

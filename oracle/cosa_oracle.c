@@ -22,6 +22,10 @@ static cosahip_config P;
 static int NPDE = 5, NH = 0, DIM5 = 1, DIM5H = 1, LMET = 4;
 static double DHB[131 * 131];
 static double RESRMS[9];
+/* dual time stepping: dualt, dt, dual_prop_start, itime of common_variables
+ * (src/common_variables.f90:28, 39, 59) */
+static int DUALT = 0, DUAL_PROP = 0, ITIME = 1;
+static double DT = 1.0;
 
 typedef long long ll;
 
@@ -93,6 +97,16 @@ void or_set_dhb(int nsnap, const double* dhb) {
   memcpy(DHB, dhb, sizeof(double) * (size_t)nsnap * nsnap);
 }
 #define D(i, j) DHB[(i) + (ll)DIM5 * (j)] /* dhb(i,j), column-major, 0-based */
+
+void or_set_dual(int dualt, double dt, int prop_start, int itime) {
+  DUALT = dualt;
+  DT = dt;
+  DUAL_PROP = prop_start;
+  ITIME = itime;
+}
+/* 1 on the first physical step of a backward-Euler start, else the BDF2
+ * 3/2 (src/update.f90:650-653, 2753-2756) */
+static double dual_fact(void) { return (ITIME == 1 && !DUAL_PROP) ? 1.0 : 1.5; }
 
 double or_get_resrms(int i) { return RESRMS[i]; }
 void or_set_resrms(int i, double v) { RESRMS[i] = v; }
@@ -2375,6 +2389,147 @@ void or_lim_res_ome_b(double* res, const double* q, const double* vol, const dou
 }
 
 /* ---------------------------------------------------------------------- */
+/* dual time stepping, per block (one snapshot: dualt excludes harbal)     */
+
+/* src_uns_b (src/src.f90:406-450): the physical-time term of the current
+ * state added to the residual, backward Euler on the first step of a
+ * first-order start, else BDF2's 3/2.                                     */
+void or_src_uns_b(const double* q, const double* vol, double* res, int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  const int first = ITIME == 1 && !DUAL_PROP;
+  for (int ipde = 1; ipde <= NPDE; ipde++)
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          const double qv = Q(q, g, i, j, k, ipde, 0), v = V1(vol, g, i, j, k);
+          Q(res, g, i, j, k, ipde, 0) = first ? Q(res, g, i, j, k, ipde, 0) + qv * v / DT
+                                              : Q(res, g, i, j, k, ipde, 0) + 3 * qv * v / 2 / DT;
+        }
+}
+
+/* rhs_uns_b (src/src.f90:480-525): the known physical-time levels,
+ * q1 = q^n, q2 = q^(n-1).                                                 */
+void or_rhs_uns_b(double* rhs, const double* q1, const double* q2, const double* vol, int imax, int jmax,
+                  int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  const int first = ITIME == 1 && !DUAL_PROP;
+  for (int ipde = 1; ipde <= NPDE; ipde++)
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          const double a = Q(q1, g, i, j, k, ipde, 0), v = V1(vol, g, i, j, k);
+          Q(rhs, g, i, j, k, ipde, 0) = first ? a * v / DT : (2 * a - Q(q2, g, i, j, k, ipde, 0) / 2) * v / DT;
+        }
+}
+
+/* resg_brms, rkimp without IRS on one level (src/cosa.f90:4534-4549): the
+ * physical-time term the implicit update carries, added back.            */
+void or_resg_brms_dual(const double* q, const double* res, const double* rhs, const double* vol, double* rb,
+                       int imax, int jmax, int kmax) {
+  bg g = mkbg(imax, jmax, kmax);
+  const double fact = dual_fact();
+  for (int p = 0; p <= NPDE; p++) rb[p] = 0;
+  for (int ipde = 1; ipde <= NPDE; ipde++)
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          double d = fact * Q(q, g, i, j, k, ipde, 0) * V1(vol, g, i, j, k) / DT + Q(res, g, i, j, k, ipde, 0) -
+                     Q(rhs, g, i, j, k, ipde, 0);
+          rb[ipde - 1] = rb[ipde - 1] + d * d;
+        }
+  for (int ipde = 1; ipde <= NPDE; ipde++) rb[NPDE] = rb[NPDE] + rb[ipde - 1];
+}
+
+/* update_b, rkimp branches (src/update.f90:649-762): the stage with the
+ * physical-time term implicit.  Without IRS on one level (plain) the
+ * residual carries no src_uns and the term enters the denominator only;
+ * otherwise the residual holds it and the update adds it back explicitly
+ * before dividing.  k-omega: the point-implicit rho*omega, rho*k of the
+ * steady update with the same physical-time term.                         */
+void or_update_dual_b(double* res, double* q, const double* qold, const double* vol, const double* dtvol,
+                      const double* dtvolt, const double* delpls, const double* prd, double ar, int imax, int jmax,
+                      int kmax, const double* ttrm) {
+  bg g = mkbg(imax, jmax, kmax);
+  const double fact = dual_fact(), dt = DT;
+  const int plain = P.irsop == 0 && P.nlevel == 1;
+  if (is_kom())
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          const double rho = Q(q, g, i, j, k, 1, 0);
+          const double ome = Q(q, g, i, j, k, 7, 0) / Q(q, g, i, j, k, 1, 0);
+          const double dtt = P1(dtvolt, g, i, j, k, 0);
+          const double dtau = dtt * V1(vol, g, i, j, k);
+          const double gav = ttrm ? T3(ttrm, g, i, j, k, 2, 0) : P.gkw, bav = ttrm ? T3(ttrm, g, i, j, k, 3, 0) : P.bkw;
+          const double dl = S1(delpls, g, i, j, k, 0);
+          double rhomega;
+          if (plain)
+            rhomega = (Q(qold, g, i, j, k, 7, 0) - ar * Q(res, g, i, j, k, 7, 0) * dtt +
+                       ar * dtau * Q(q, g, i, j, k, 7, 0) * (gav * dl + 2 * bav * ome)) /
+                      (1 + ar * dtau * (gav * dl + 2 * bav * ome) + fact * ar * dtau / dt);
+          else
+            rhomega = (Q(qold, g, i, j, k, 7, 0) + ar * fact * dtau / dt * Q(q, g, i, j, k, 7, 0) -
+                       ar * Q(res, g, i, j, k, 7, 0) * dtt +
+                       ar * dtau * Q(q, g, i, j, k, 7, 0) * (gav * dl + 2 * bav * ome)) /
+                      (1 + ar * dtau * (gav * dl + 2 * bav * ome) + ar * fact * dtau / dt);
+          const double rhomegamin = gav * P.cmu * rho * sqrt(S1(prd, g, i, j, k, 0));
+          if (rhomega < rhomegamin) rhomega = rhomegamin;
+          Q(q, g, i, j, k, 7, 0) = rhomega;
+          const double drhomega = Q(q, g, i, j, k, 7, 0) - rho * ome;
+          if (plain)
+            Q(q, g, i, j, k, 6, 0) =
+                (Q(qold, g, i, j, k, 6, 0) - ar * Q(res, g, i, j, k, 6, 0) * dtt +
+                 ar * dtau * Q(q, g, i, j, k, 6, 0) * (dl + P.bstrkw * ome - P.bstrkw * drhomega / rho)) /
+                (1 + ar * dtau * (dl + P.bstrkw * ome) + ar * fact * dtau / dt);
+          else
+            Q(q, g, i, j, k, 6, 0) =
+                (Q(qold, g, i, j, k, 6, 0) + ar * fact * dtau / dt * Q(q, g, i, j, k, 6, 0) -
+                 ar * Q(res, g, i, j, k, 6, 0) * dtt +
+                 ar * dtau * Q(q, g, i, j, k, 6, 0) * (dl + P.bstrkw * ome - P.bstrkw * drhomega / rho)) /
+                (1 + ar * dtau * (dl + P.bstrkw * ome) + ar * fact * dtau / dt);
+        }
+  for (int ipde = 1; ipde <= 5; ipde++)
+    for (int k = 1; k <= kmax - 1; k++)
+      for (int j = 1; j <= jmax - 1; j++)
+        for (int i = 1; i <= imax - 1; i++) {
+          const double dv = P1(dtvol, g, i, j, k, 0), v = V1(vol, g, i, j, k);
+          double* qq = &Q(q, g, i, j, k, ipde, 0);
+          if (plain)
+            *qq = (Q(qold, g, i, j, k, ipde, 0) - ar * Q(res, g, i, j, k, ipde, 0) * dv) / (1 + fact * ar * dv * v / dt);
+          else
+            *qq = (Q(qold, g, i, j, k, ipde, 0) - ar * dv * (-(fact * v / dt * *qq) + Q(res, g, i, j, k, ipde, 0))) /
+                  (1 + fact * ar * dv * v / dt);
+        }
+}
+
+/* lim_res_ome_b, rkimp branch without low-speed preconditioning
+ * (src/update.f90:2753-2794): the multigrid transfer's omega residual
+ * limit with the physical-time term in the denominator.  den1 carries no
+ * RK coefficient here, as in the reference.                               */
+void or_lim_res_ome_dual_b(double* res, const double* q, const double* vol, const double* dtvolt,
+                           const double* delpls, const double* prd, double ar, int imax, int jmax, int kmax,
+                           const double* ttrm) {
+  bg g = mkbg(imax, jmax, kmax);
+  if (!(P.irsop > 0 || (P.irsop == 0 && P.nlevel > 1))) return;
+  const double fact = dual_fact();
+  for (int k = 1; k <= kmax - 1; k++)
+    for (int j = 1; j <= jmax - 1; j++)
+      for (int i = 1; i <= imax - 1; i++) {
+        const double gav = ttrm ? T3(ttrm, g, i, j, k, 2, 0) : P.gkw, bav = ttrm ? T3(ttrm, g, i, j, k, 3, 0) : P.bkw;
+        const double dtt = P1(dtvolt, g, i, j, k, 0);
+        const double dtau = dtt * V1(vol, g, i, j, k);
+        const double rho = Q(q, g, i, j, k, 1, 0);
+        const double ome = Q(q, g, i, j, k, 7, 0) / Q(q, g, i, j, k, 1, 0);
+        const double den1 = 1 + dtau * (gav * S1(delpls, g, i, j, k, 0) + 2 * bav * ome);
+        const double den2 = ar * fact * dtau / DT;
+        const double rhomega = Q(q, g, i, j, k, 7, 0) - ar * Q(res, g, i, j, k, 7, 0) * dtt / (den1 + den2);
+        const double rhomegamin = gav * P.cmu * rho * sqrt(S1(prd, g, i, j, k, 0));
+        if (rhomega < rhomegamin)
+          Q(res, g, i, j, k, 7, 0) = ((den1 + den2) * (Q(q, g, i, j, k, 7, 0) - rhomegamin)) / dtt / ar;
+      }
+}
+
+/* ---------------------------------------------------------------------- */
 /* Dense LU with partial pivoting and its solve, as the reference's calls
  * DGETRF / DGETRS('N') compute them (src/update.f90:1460-1478): LAPACK's
  * right-looking elimination -- pivot = first row of largest |a| in the
@@ -3092,6 +3247,9 @@ void or_resg_rms(or_case* c, int nl, double* resrms) {
     if (P.harbal) {
       /* work1 = copy of res on the interior (src/cosa.f90:1930-1931) */
       or_resg_brms_hb(QB(L->work1, L, b), rb, L->imax[b], L->jmax[b], L->kmax[b]);
+    } else if (DUALT && P.rkimp && P.irsop == 0 && P.nlevel == 1) {
+      or_resg_brms_dual(QB(L->q, L, b), QB(L->res, L, b), QB(L->rhs, L, b), VB(L->vol, L, b), rb, L->imax[b],
+                        L->jmax[b], L->kmax[b]);
     } else {
       or_resg_brms(QB(L->res, L, b), QB(L->rhs, L, b), rb, L->imax[b], L->jmax[b], L->kmax[b]);
     }
@@ -3167,7 +3325,11 @@ void or_cirs(or_case* c, int nl) {
 void or_update(or_case* c, int nl, double ar) {
   or_level* L = &c->lev[nl - 1];
   for (int b = 0; b < L->nblocks; b++) {
-    if (P.harbal && P.rkimp)
+    if (DUALT && P.rkimp)
+      or_update_dual_b(QB(L->res, L, b), QB(L->q, L, b), QB(L->qold, L, b), VB(L->vol, L, b), PB(L->dtvol, L, b),
+                       is_kom() ? PB(L->dtvolt, L, b) : NULL, is_kom() ? SB(L->delpls, L, b) : NULL,
+                       is_kom() ? SB(L->prd, L, b) : NULL, ar, L->imax[b], L->jmax[b], L->kmax[b], TTB(L, b));
+    else if (P.harbal && P.rkimp)
       or_update_bhb_rkim(QB(L->res, L, b), QB(L->q, L, b), QB(L->qold, L, b), VB(L->vol, L, b), PB(L->dtvol, L, b),
                          PB(L->dtvolt, L, b), QB(L->dq, L, b), SB(L->delpls, L, b), SB(L->prd, L, b), ar,
                          L->imax[b], L->jmax[b], L->kmax[b], TTB(L, b));
@@ -3255,6 +3417,9 @@ void or_turb_limits(or_case* c, int nl, double ar) {
     if (P.harbal)
       or_lim_res_ome_bhb(QB(L->res, L, b), QB(L->q, L, b), QB(L->qold, L, b), VB(L->vol, L, b), PB(L->dtvolt, L, b),
                          SB(L->delpls, L, b), SB(L->prd, L, b), L->imax[b], L->jmax[b], L->kmax[b], TTB(L, b));
+    else if (DUALT && P.rkimp)
+      or_lim_res_ome_dual_b(QB(L->res, L, b), QB(L->q, L, b), VB(L->vol, L, b), PB(L->dtvolt, L, b),
+                            SB(L->delpls, L, b), SB(L->prd, L, b), ar, L->imax[b], L->jmax[b], L->kmax[b], TTB(L, b));
     else
       or_lim_res_ome_b(QB(L->res, L, b), QB(L->q, L, b), VB(L->vol, L, b), PB(L->dtvolt, L, b), SB(L->delpls, L, b),
                        SB(L->prd, L, b), ar, L->imax[b], L->jmax[b], L->kmax[b], TTB(L, b));
@@ -3288,6 +3453,10 @@ int or_smooth(or_case* c, int nl, int from_fine, int to_coarse, int niter, int m
       or_prim2cons(c, nl, L->q);
       const int rkim_plain = P.rkimp && P.irsop == 0 && P.nlevel == 1;
       if (P.harbal && !rkim_plain) or_src_hb(c, nl);
+      /* src/cosa.f90:1910-1915 */
+      if (DUALT && (!P.rkimp || P.irsop > 0 || P.nlevel > 1))
+        for (int b = 0; b < L->nblocks; b++)
+          or_src_uns_b(QB(L->q, L, b), VB(L->vol, L, b), QB(L->res, L, b), L->imax[b], L->jmax[b], L->kmax[b]);
       if (nl == 1 && irk == 1 && iter != niter + 1 &&
           (mgit == 1 || mgit % P.iprint == 0 || mgit == P.ncycle)) {
         if (P.harbal) or_copy_array(c, nl, L->work1, L->res);
@@ -3298,6 +3467,8 @@ int or_smooth(or_case* c, int nl, int from_fine, int to_coarse, int niter, int m
         or_resg_rms(c, nl, rr);
         for (int p = 0; p <= NPDE; p++) RESRMS[p + 1] = rr[p];
       }
+      /* the overall residual of the finest grid (src/cosa.f90:1938-1941) */
+      if (nl == 1 && DUALT) or_add_array(c, nl, L->res, L->rhs, -1.0);
       if (from_fine && iter == 1 && irk == 1) or_add_array(c, nl, L->rhs, L->res, 1.0);
       if (nl > 1) or_add_array(c, nl, L->res, L->rhs, -1.0);
       if (iter == niter + 1) {
@@ -3407,4 +3578,21 @@ int or_mg(or_case* c, int nstart, int npre, int npost, int ncrs, double* hist, i
     if (rc) return -rc;
   }
   return P.ncycle;
+}
+
+/* ---------------------------------------------------------------------- */
+/* one physical step of dual time stepping around mg() (src/cosa.f90:640-
+ * 651): q1 = q, rhs = rhs_uns(q1, q2) on the finest level, then after the
+ * multigrid cycles q2 = q1.  copy_array with off_l = -1: interior cells.  */
+void or_dual_begin(or_case* c, int itime) {
+  or_level* L = &c->lev[0];
+  ITIME = itime;
+  or_copy_array(c, 1, L->q1, L->q);
+  for (int b = 0; b < L->nblocks; b++)
+    or_rhs_uns_b(QB(L->rhs, L, b), QB(L->q1, L, b), QB(L->q2, L, b), VB(L->vol, L, b), L->imax[b], L->jmax[b],
+                 L->kmax[b]);
+}
+void or_dual_end(or_case* c) {
+  or_level* L = &c->lev[0];
+  or_copy_array(c, 1, L->q2, L->q1);
 }

@@ -39,6 +39,7 @@ typedef struct or_level {
   double *mut, *dist, *prd, *divvel, *delpls, *dtvolt, *tbetaxi, *tbetaeta, *tbetazeta;
   double* xd[9];   /* xideri, etaderi, zetaderi, xiderj, ..., zetaderk */
   double* ttrm;    /* (P3, 3, 0:2N): cross diffusion, gamma, beta (k-omega) */
+  double *q1, *q2; /* dual time stepping, finest level: q^n, q^(n-1) (P3) */
 } or_level;
 
 typedef struct or_case {
@@ -53,6 +54,7 @@ void or_set_dhb(int nsnap, const double* dhb);
 double or_get_resrms(int i);            /* resrms(i), i = 1..8            */
 void or_set_resrms(int i, double v);
 void or_set_mgit(int m);                /* cycle counter of CFL ramping   */
+void or_set_dual(int dualt, double dt, int prop_start, int itime);   /* dual time */
 double or_get_cfl(int turb);            /* cfl (0) / cflt (1) in use      */
 
 /* per-block restatements (explicit-shape dummies of the reference) */
@@ -187,6 +189,20 @@ void or_resg_rms(or_case* c, int nl, double* resrms);
 void or_add_array(or_case* c, int nl, double* dst, const double* src,
                   double fact);
 void or_copy_array(or_case* c, int nl, double* dst, const double* src);
+/* dual time stepping: per block and the physical step around mg() */
+void or_src_uns_b(const double* q, const double* vol, double* res, int imax, int jmax, int kmax);
+void or_rhs_uns_b(double* rhs, const double* q1, const double* q2, const double* vol, int imax, int jmax,
+                  int kmax);
+void or_resg_brms_dual(const double* q, const double* res, const double* rhs, const double* vol, double* rb,
+                       int imax, int jmax, int kmax);
+void or_update_dual_b(double* res, double* q, const double* qold, const double* vol, const double* dtvol,
+                      const double* dtvolt, const double* delpls, const double* prd, double ar, int imax, int jmax,
+                      int kmax, const double* ttrm);
+void or_lim_res_ome_dual_b(double* res, const double* q, const double* vol, const double* dtvolt,
+                           const double* delpls, const double* prd, double ar, int imax, int jmax, int kmax,
+                           const double* ttrm);
+void or_dual_begin(or_case* c, int itime);
+void or_dual_end(or_case* c);
 void or_cirs(or_case* c, int nl);
 void or_update(or_case* c, int nl, double ar);
 void or_tur_vis(or_case* c, int nl);

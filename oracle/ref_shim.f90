@@ -103,6 +103,26 @@ contains
     mgit = m
   end subroutine
 
+  ! dual time stepping: dualt, rkimp, dt, dual_prop_start, itime
+  subroutine ref_set_dual(on, rkim, dtp, prop, it) bind(C, name='ref_set_dual')
+    integer(c_int), value :: on, rkim, prop, it
+    real(c_double), value :: dtp
+    dualt = on /= 0; rkimp = rkim /= 0; dt = dtp; dual_prop_start = prop /= 0; itime = it
+    unsteady = unsteady .or. dualt
+  end subroutine
+
+  subroutine ref_src_uns_b(q, vol, res, imax, jmax, kmax) bind(C, name='ref_src_uns_b')
+    real(c_double) :: q(*), vol(*), res(*)
+    integer(c_int), value :: imax, jmax, kmax
+    call src_uns_b(q, vol, res, imax, jmax, kmax, npde)
+  end subroutine
+
+  subroutine ref_rhs_uns_b(rhs, q1, q2, vol, imax, jmax, kmax) bind(C, name='ref_rhs_uns_b')
+    real(c_double) :: rhs(*), q1(*), q2(*), vol(*)
+    integer(c_int), value :: imax, jmax, kmax
+    call rhs_uns_b(rhs, q1, q2, vol, imax, jmax, kmax, npde)
+  end subroutine
+
   subroutine ref_set_dhb(nsnap, d) bind(C, name='ref_set_dhb')
     integer(c_int), value :: nsnap
     real(c_double), intent(in) :: d(nsnap, nsnap)

@@ -50,7 +50,7 @@ class OrLevel(ctypes.Structure):
         (n, DP) for n in ("q", "qold", "qp", "qm", "dq", "dqp", "dqm", "flux", "res", "rhs", "work1", "si", "sj",
                           "sk", "vol", "dtvol", "betaxi", "betaeta", "betazeta", "xdot", "ydot", "zdot", "pr")] + [
         (n, DP) for n in ("mut", "dist", "prd", "divvel", "delpls", "dtvolt", "tbetaxi", "tbetaeta", "tbetazeta")] + [
-        ("xd", DP * 9), ("ttrm", DP)]
+        ("xd", DP * 9), ("ttrm", DP), ("q1", DP), ("q2", DP)]
 
 
 class OrCaseS(ctypes.Structure):
@@ -159,6 +159,10 @@ class OrCase:
                     lev.xd[m] = dp(a["xd"][m])
                 a["ttrm"] = np.zeros(case.size(nl, "p3s") * 3)
                 lev.ttrm = dp(a["ttrm"])
+            if nl == 1:   # dual time stepping: q^n, q^(n-1)
+                for n in ("q1", "q2"):
+                    a[n] = np.zeros(case.size(nl, "p3"))
+                    setattr(lev, n, dp(a[n]))
             self.arrays.append(a)
 
     def ptr(self):

@@ -1,6 +1,8 @@
 """The inviscid residual kernels are interchangeable: the specialised
 k_residual_tile instances (31 x 7 tiles with the k-window from LDS / L2 or
-in registers, 31 x 15 tiles, the 128-VGPR build) give the same bits as the
+in registers, 31 x 15 tiles, the 128-VGPR build) and the k_residual_sva
+instances (one van Albada slope per cell shared by its two faces, as
+limit_i/j/k compute it, src/flux.f90:3584-3604) give the same bits as the
 generic fused kernel, which matches the oracle (muscl_b* + roflux_b + resid,
 src/flux.f90:1221-1812, 45-260; src/cosa.f90:1149-1211).  Cases cover the
 specialised van Albada path at second and first order, the run-time paths
@@ -15,7 +17,7 @@ from parity_util import interior_mask, make_pair, parity
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = ["fused", "tile7", "tile7reg", "tile15", "tile7w4"]
+VARIANTS = ["fused", "tile7", "tile7reg", "tile15", "tile7w4", "sva7", "sva15", "sva7w4"]
 
 
 def _cases():
