@@ -80,7 +80,7 @@ BYTES_PER_CELL_RANS = {  # Wilcox k-omega, npde = 7
 # WRITE x1.00) on the level-1 kernel: profiles/residual_uneven96_fp64_hbm_traffic.md
 # and profiles/residual_wing4_rans_fp64_hbm_traffic.md.  Re-measure when the
 # kernels change.
-MEASURED_TRAFFIC_PER_CELL = {("uneven96", "fp64", "residual"): 472.8, ("wing4", "fp64", "residual"): 2290.0}
+MEASURED_TRAFFIC_PER_CELL = {("uneven96", "fp64", "residual"): 324.0, ("wing4", "fp64", "residual"): 2290.0}
 
 
 def log(msg):
@@ -212,12 +212,13 @@ def measure(workload: str, steps: int, warmup: int, precision: str, tail_rms: fl
     t0 = time.time()
     case, cfg_desc = make_case(workload)
     cfg_desc["precision"] = {"fp64": "fp64", "fp32": "fp32 residual / fp64 state",
-                             "mixed": f"fp32 residual until resrms(1) < {tail_rms:g}, then fp64"}[precision]
+                             "mixed": f"fp32 residual until resrms(1) < {tail_rms:g}, then fp64",
+                             "fp64c": "fp64, residual with multiply-add contraction"}[precision]
     log(f"case {case.name}: {case.ncell()} cells, built in {time.time() - t0:.1f} s")
     q0 = case.perturbed(amp=0.01, seed=1)
     s = Solve(case, device=0, q=q0)
     del q0
-    prec_mode = {"fp64": 0, "fp32": 1, "mixed": 2}[precision]
+    prec_mode = {"fp64": 0, "fp32": 1, "mixed": 2, "fp64c": 3}[precision]
     if prec_mode:
         s.set_precision(prec_mode, tail_rms if prec_mode == 2 else 0.0)
     log("device state resident; warmup")
@@ -305,9 +306,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default kernel arrangements")
     ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1),
                     help="concurrent single-block reference processes for cpu_baseline (at most 16)")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "mixed"],
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "mixed", "fp64c"],
                     help="fp32: FP32 residual / FP64 state (cosahip_set_precision mode 1); mixed: FP32 until "
-                         "resrms(1) < --tail-rms, then FP64 (mode 2)")
+                         "resrms(1) < --tail-rms, then FP64 (mode 2); fp64c: FP64 residual with multiply-add "
+                         "contraction (mode 3)")
     ap.add_argument("--tail-rms", type=float, default=1e-6)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy three times (scale for the HBM "
@@ -323,6 +325,13 @@ def main():
 
     head = measure(a.workload, a.steps, a.warmup, a.precision, a.tail_rms, not a.no_autotune)
     extras = {}
+    if not a.no_extras and a.workload == "uneven96" and a.precision == "fp64":
+        # the same workload in the fast FP64 mode (contracted residual, held to the FP64 parity bar)
+        u = measure("uneven96", a.steps, a.warmup, "fp64c", a.tail_rms, not a.no_autotune)
+        extras["uneven96_fp64_contracted"] = {"value": round(u["value"], 2), "unit": "Mcell-updates/s",
+                                              "ms_per_step": round(u["ms_per_step"], 3),
+                                              "roofline": u.get("roofline"),
+                                              "residual": (u.get("families") or {}).get("residual")}
     if not a.no_extras and a.workload == "uneven96":
         w = measure("wing4", a.steps, a.warmup, a.precision, a.tail_rms, not a.no_autotune)
         extras["wing4_rans_komega"] = {"value": round(w["value"], 2), "unit": "Mcell-updates/s",
@@ -336,7 +345,7 @@ def main():
     out = {"metric": "Mcell-updates/s", "value": round(head["value"], 2), "unit": "Mcell-updates/s", "n_gpus": 1,
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(head["ms_per_step"], 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64" if a.precision == "fp64" else "f32 residual / f64 state", "data": "synthetic",
+           "dtype": "f64" if a.precision in ("fp64", "fp64c") else "f32 residual / f64 state", "data": "synthetic",
            "config": head["config"], "roofline": head.get("roofline"), "cpu_baseline": cpu,
            "sync_path": head.get("sync_path"), "families": head.get("families"),
            "resrms1_first_last": head["resrms1_first_last"]}

@@ -2,7 +2,8 @@
 
     python -m cosa_amd.build            # library only
 Flags: -O3, -ffp-contract=off (no fused multiply-add, so FP64 results keep
-the reference's rounding), packed-FP32 instructions disabled.
+the reference's rounding), packed-FP32 instructions disabled;
+k_resid_contracted.hip (the fast FP64 residual) with -ffp-contract=fast.
 """
 from __future__ import annotations
 
@@ -15,7 +16,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcosahip.so")
-SOURCES = ["api.hip", "k_core.hip", "k_flux.hip", "k_resid.hip", "k_turb.hip", "k_force.hip", "k_grid.hip"]
+SOURCES = ["api.hip", "k_core.hip", "k_flux.hip", "k_resid.hip", "k_resid_contracted.hip", "k_turb.hip",
+           "k_force.hip", "k_grid.hip"]
+# sources built with multiply-add contraction (the fast FP64 residual)
+CONTRACTED = {"k_resid_contracted.hip"}
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-result"]
@@ -31,8 +35,9 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     procs = []
     for src in SOURCES:
         obj = os.path.join(HERE, "_build", src.replace(".hip", ".o"))
-        cmd = [hipcc()] + HIPFLAGS + ["-I", os.path.join(ROOT, "include"), "-I", CSRC, "-c",
-                                      os.path.join(CSRC, src), "-o", obj]
+        flags = [f if (src not in CONTRACTED or f != "-ffp-contract=off") else "-ffp-contract=fast" for f in HIPFLAGS]
+        cmd = [hipcc()] + flags + ["-I", os.path.join(ROOT, "include"), "-I", CSRC, "-c",
+                                   os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd))
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

@@ -788,7 +788,7 @@ extern "C" int cosahip_define_level(cosahip_solve* h, int nl, int nblocks, const
     if (rc) return rc;
   }
   CK(hipMalloc(&L.rms_part, ((ll)L.ntile[TK_CELLS] + 1) * 8 * sizeof(double)));
-  if (s->prec_mode) return alloc_fp32(s, L);
+  if (s->prec_mode == 1 || s->prec_mode == 2) return alloc_fp32(s, L);
   return COSAHIP_OK;
 }
 
@@ -1409,15 +1409,17 @@ extern "C" int cosahip_force_blocks(cosahip_solve* h, int nl, int nsurface, int 
 extern "C" int cosahip_set_precision(cosahip_solve* h, int mode, double tail_rms) {
   Solve* s = reinterpret_cast<Solve*>(h);
   if (!s) return COSAHIP_ERR_ARG;
-  if (mode < 0 || mode > 2) return fail(s, COSAHIP_ERR_ARG, "set_precision: mode must be 0, 1 or 2");
+  if (mode < 0 || mode > 3) return fail(s, COSAHIP_ERR_ARG, "set_precision: mode must be 0, 1, 2 or 3");
   if (mode == 2 && !(tail_rms > 0)) return fail(s, COSAHIP_ERR_ARG, "set_precision: mode 2 needs tail_rms > 0");
   if (mode && s->split_residual && !s->par.viscous && s->par.npde == 5)
     return fail(s, COSAHIP_ERR_ARG, "set_precision: the FP32 residual needs the fused residual kernel");
+  if (mode == 3 && (s->par.viscous || s->par.npde != 5))
+    return fail(s, COSAHIP_ERR_ARG, "set_precision: the contracted FP64 residual (mode 3) covers the Euler residual");
   CK(hipSetDevice(s->device));
   s->prec_mode = mode;
   s->prec_tail_rms = mode == 2 ? tail_rms : 0;
   s->prec_tail = false;
-  if (mode)
+  if (mode == 1 || mode == 2)
     for (Level& L : s->lev)
       if (L.nblocks) {
         int rc = alloc_fp32(s, L);

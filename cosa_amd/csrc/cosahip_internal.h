@@ -296,7 +296,8 @@ struct Solve {
   int cirs_i_mode = 0;   // 0 auto, 1 lds, 2 iv, 3 line
   int il_lds = 40960;    // LDS bytes per work-group of the line-resident CIRS i-sweep
   // residual precision: 0 FP64, 1 FP32 residual / FP64 state, 2 FP32 until
-  // the level-1 density residual rms falls below prec_tail_rms, FP64 after
+  // the level-1 density residual rms falls below prec_tail_rms, FP64 after,
+  // 3 FP64 with multiply-add contraction (Euler residual)
   int prec_mode = 0;
   double prec_tail_rms = 0;
   bool prec_tail = false;

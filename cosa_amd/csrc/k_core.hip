@@ -390,33 +390,33 @@ __global__ void __launch_bounds__(256) k_update_dual(Par p, const Blk* __restric
 }
 
 // ---- src_bhb (src/src.f90:556-613): qhb = omega vol D q, res += qhb ----
+// One thread per (cell, snapshot ni): the 2N+1 snapshots of a cell are
+// read by 2N+1 threads (L2 serves the repeats), so a launch has as many
+// threads as the level has cell-snapshots, not cells.  The per-term order
+// ((omega * vol) * dhb(ni, nj)) * q(nj), summed over nj from zero, is the
+// reference's.  The contraction is a (2N+1)^2 by (2N+1) x 5 product per
+// cell: 0.6 flop per byte moved, HBM-bound, so it stays on the vector ALUs.
 __global__ void __launch_bounds__(256) k_src_hb(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
                                                 const double* __restrict__ q, const double* __restrict__ vol,
                                                 double* __restrict__ res, double* __restrict__ qhb,
                                                 const double* __restrict__ dhb) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
-  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
-  const int item = t.base + threadIdx.x;
-  if (item >= im * jm * km) return;
-  int rem = item;
-  const int i = rem % im + 1;
-  rem /= im;
-  const int j = rem % jm + 1, k = rem / jm + 1;
-  const ll n3 = n3of(b), x = i3(b, i, j, k);
+  CellIdx c;
+  if (!cell_of(p, b, t.base + threadIdx.x, c)) return;
+  const ll n3 = n3of(b), x = i3(b, c.i, c.j, c.k);
   const ll stride_n = n3 * p.npde;
   const double* qb = q + b.o3 * p.npde * p.nsnap + x;
   double* rb = res + b.o3 * p.npde * p.nsnap + x;
   double* hb = qhb + b.o3 * p.npde * p.nsnap + x;
-  const double vv = vol[b.o1 + i1(b, i, j, k)];
-  const int ns = p.nsnap;
-  for (int ni = 0; ni < ns; ni++)
-    for (int v = 0; v < p.npde; v++) {
-      double acc = 0.0;
-      for (int nj = 0; nj < ns; nj++) acc = acc + p.omega * vv * dhb[ni + ns * nj] * qb[stride_n * nj + n3 * v];
-      hb[stride_n * ni + n3 * v] = acc;
-      rb[stride_n * ni + n3 * v] = rb[stride_n * ni + n3 * v] + acc;
-    }
+  const double vv = vol[b.o1 + i1(b, c.i, c.j, c.k)];
+  const int ns = p.nsnap, ni = c.n;
+  for (int v = 0; v < p.npde; v++) {
+    double acc = 0.0;
+    for (int nj = 0; nj < ns; nj++) acc = acc + p.omega * vv * dhb[ni + ns * nj] * qb[stride_n * nj + n3 * v];
+    hb[stride_n * ni + n3 * v] = acc;
+    rb[stride_n * ni + n3 * v] = rb[stride_n * ni + n3 * v] + acc;
+  }
 }
 
 // ---- harmonic-balance implicit stage, rkim ----------------------------------
@@ -1337,11 +1337,9 @@ hipError_t launch_update(Solve& s, Level& L, double ar) {
   return hipGetLastError();
 }
 hipError_t launch_src_hb(Solve& s, Level& L) {
-  // one work item per interior cell: the tile table of a single snapshot
-  // covers the first 1/nsnap of TK_CELLS items; reuse it with nsnap = 1 items
-  Par p = s.par;
+  // one work item per interior (cell, snapshot)
   if (L.ntile[TK_CELLS])
-    hipLaunchKernelGGL(k_src_hb, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, p, L.dblk, L.tile[TK_CELLS],
+    hipLaunchKernelGGL(k_src_hb, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_CELLS],
                        L.f[COSAHIP_Q], L.f[COSAHIP_VOL], L.f[COSAHIP_RES], L.dq, s.ddhb);
   return hipGetLastError();
 }

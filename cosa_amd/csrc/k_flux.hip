@@ -1189,6 +1189,9 @@ hipError_t launch_residual(Solve& s, Level& L, const double* q, double* res) {
   }
   if (!s.split_residual) {
     const int rv = s.resid_force >= 0 ? s.resid_force : L.resid_mode;
+    if (s.prec_mode == 3)   // fast FP64: contracted multiply-adds
+      return launch_residual_tile_contracted(s, L, rv == RESID_GENERIC ? (int)RESID_TILE7 : rv, q, L.f[COSAHIP_SI],
+                                             L.f[COSAHIP_SJ], L.f[COSAHIP_SK], res);
     if (rv != RESID_GENERIC)
       return launch_residual_tile(s, L, rv, q, L.f[COSAHIP_SI], L.f[COSAHIP_SJ], L.f[COSAHIP_SK], res);
     if (L.ntile[TK_FUSED])

@@ -34,6 +34,16 @@ namespace cosahip {
 constexpr int RT_TI = 31;   // cells per tile along i (32 lanes: one more face than cells)
 constexpr int RT_KSEG = 32; // k-planes per work-group
 
+// This file is compiled twice: as itself (no multiply-add contraction, the
+// reference's rounding) and from k_resid_contracted.hip with
+// COSAHIP_CONTRACTED defined and contraction on (the fast FP64 mode); the
+// kernels live in a namespace of their own in each build.
+#ifdef COSAHIP_CONTRACTED
+namespace contracted {
+#else
+namespace exact {
+#endif
+
 template <typename T, int TJ, int ILIM, int FO, int EF, int KW, int MINW>
 __global__ void __launch_bounds__(32 * (TJ + 1), MINW) k_residual_tile(Par p, int nl, const Blk* __restrict__ blks,
                                                                  const Tile* __restrict__ tiles,
@@ -429,13 +439,6 @@ __global__ void __launch_bounds__(32 * (TJ + 1), MINW) k_residual_sva(Par p, int
   }
 }
 
-// Work items of one block for a TJ-row tile table: (i tiles) x (j tiles) x
-// (k segments) x snapshots, i tiles fastest.
-ll residual_tile_items(int TJ, int imax, int jmax, int kmax, int nsnap) {
-  const ll im = imax - 1, jm = jmax - 1, km = kmax - 1;
-  return (ll)nsnap * ((im + RT_TI - 1) / RT_TI) * ((jm + TJ - 1) / TJ) * ((km + RT_KSEG - 1) / RT_KSEG);
-}
-
 template <typename T, int TJ, int ILIM, int FO, int EF, int KW, int MINW = 1>
 static void launch_tile(Solve& s, Level& L, int ntile, const Tile* tiles, const T* q, const T* si, const T* sj,
                         const T* sk, double* res) {
@@ -497,15 +500,32 @@ static bool dispatch(Solve& s, Level& L, int variant, const T* q, const T* si, c
   return true;
 }
 
+}  // namespace exact / contracted
+
+#ifdef COSAHIP_CONTRACTED
+hipError_t launch_residual_tile_contracted(Solve& s, Level& L, int variant, const double* q, const double* si,
+                                           const double* sj, const double* sk, double* res) {
+  if (!contracted::dispatch<double>(s, L, variant, q, si, sj, sk, res)) return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+#else
+// Work items of one block for a TJ-row tile table: (i tiles) x (j tiles) x
+// (k segments) x snapshots, i tiles fastest.
+ll residual_tile_items(int TJ, int imax, int jmax, int kmax, int nsnap) {
+  const ll im = imax - 1, jm = jmax - 1, km = kmax - 1;
+  return (ll)nsnap * ((im + RT_TI - 1) / RT_TI) * ((jm + TJ - 1) / TJ) * ((km + RT_KSEG - 1) / RT_KSEG);
+}
+
 hipError_t launch_residual_tile(Solve& s, Level& L, int variant, const double* q, const double* si,
                                 const double* sj, const double* sk, double* res) {
-  if (!dispatch<double>(s, L, variant, q, si, sj, sk, res)) return hipErrorInvalidValue;
+  if (!exact::dispatch<double>(s, L, variant, q, si, sj, sk, res)) return hipErrorInvalidValue;
   return hipGetLastError();
 }
 hipError_t launch_residual_tile(Solve& s, Level& L, int variant, const float* q, const float* si, const float* sj,
                                 const float* sk, double* res) {
-  if (!dispatch<float>(s, L, variant, q, si, sj, sk, res)) return hipErrorInvalidValue;
+  if (!exact::dispatch<float>(s, L, variant, q, si, sj, sk, res)) return hipErrorInvalidValue;
   return hipGetLastError();
 }
+#endif
 
 }  // namespace cosahip

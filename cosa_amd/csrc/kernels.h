@@ -59,6 +59,10 @@ hipError_t launch_residual_tile(Solve& s, Level& L, int variant, const double* q
                                 const double* sk, double* res);
 hipError_t launch_residual_tile(Solve& s, Level& L, int variant, const float* q, const float* si, const float* sj,
                                 const float* sk, double* res);
+// the same kernels built with multiply-add contraction (fast FP64 mode,
+// k_resid_contracted.hip); RESID_GENERIC runs as RESID_TILE7 there
+hipError_t launch_residual_tile_contracted(Solve& s, Level& L, int variant, const double* q, const double* si,
+                                           const double* sj, const double* sk, double* res);
 // work items of one block in the tile table of a TJ-row residual tile
 ll residual_tile_items(int TJ, int imax, int jmax, int kmax, int nsnap);
 // the RK update fused into the last CIRS sweep (mean flow): q = qold - ar

@@ -24,6 +24,7 @@ import argparse
 import json
 import math
 import os
+import sys
 import time
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
@@ -45,9 +46,14 @@ class Request:
     mach: float = 0.5
     alpha_deg: float = 1.0
     nharms: int = 0            # > 0: harmonic-balance pitching, dtheta0 = 1 deg
+    rans: bool = False         # Wilcox k-omega RANS (viscous wall) instead of Euler
+    reyno: float = 1.0e6       # Reynolds number of a RANS request
     arrival: float = 0.0       # seconds after the server starts
     drop: float = 1e-3         # retire when resrms(1) <= drop * first resrms(1)
     max_cycles: int = 400
+    # retire as stalled when the best resrms(1) has not halved over this many
+    # cycles (a limit cycle of the limiter / shock); 0: never
+    stall_window: int = 0
     precision: int = 0         # cosahip_set_precision mode
     tail_rms_factor: float = 1e-1   # mode 2: FP64 tail below this fraction of the first rms
 
@@ -61,6 +67,12 @@ class Request:
         if self.nharms:
             kw.update(nharms=self.nharms, harbal=1, omega=0.2, irsop=1, cfl_irs=3.5)
             pitch = dict(dtheta0=math.radians(1.0), xh=0.25, yh=0.0)
+        if self.rans:
+            from .config import kw_turbulence
+            # Wilcox k-omega, viscous wall (type 1500) resolved to y+ ~ 1 at Re 1e6
+            kw.update(irsop=1, cfl=1.5, cfl_irs=3.0, ifixq=1,
+                      **kw_turbulence(self.mach, self.reyno, tu=1e-4, mutfar=1e-2, stemp=110.4 / 288.15))
+            return cases.naca_case(ni, nj, NK, nlevel=2, name=f"req{self.rid}", wall=1500, ds_wall=2e-5, **kw)
         return cases.naca_case(ni, nj, NK, nlevel=2, name=f"req{self.rid}", pitch=pitch, **kw)
 
 
@@ -77,6 +89,8 @@ class Result:
     forces: np.ndarray         # (nsnap, 12) of surface 1 (type-14 walls)
     updates: float = 0.0       # interior (cell, snapshot) RK-stage updates performed
     q: Optional[np.ndarray] = None
+    drop_reached: float = 1.0  # lowest resrms(1) / first resrms(1)
+    stalled: bool = False      # retired by the stall rule before its drop
 
 
 def device_bytes_estimate(c: "cases.Case") -> int:
@@ -97,6 +111,7 @@ class Server:
     max_active: int = 64
     device: int = 0
     keep_state: bool = False
+    verbose: bool = False      # a line on stderr per retired request
     results: List[Result] = field(default_factory=list)
 
     def run(self, requests: List[Request], clock=time.perf_counter) -> List[Result]:
@@ -150,10 +165,15 @@ class Server:
                 r1 = float(s.history[-1][0])
                 if a["first"] is None:
                     a["first"] = r1
+                    a["best"], a["best_at"] = r1, a["mgit"]
                     if r.precision == 2:   # FP64 tail once the rms is down by tail_rms_factor
                         s.set_precision(2, r.tail_rms_factor * r1)
+                if r1 < 0.5 * a["best"]:
+                    a["best"], a["best_at"] = r1, a["mgit"]
+                a["min"] = min(a.get("min", r1), r1)
                 conv = r1 <= r.drop * a["first"]
-                if conv or a["mgit"] >= r.max_cycles:
+                stalled = bool(r.stall_window) and a["mgit"] - a["best_at"] >= r.stall_window
+                if conv or stalled or a["mgit"] >= r.max_cycles:
                     retire.append((rid, conv))
                 else:
                     a["mgit"] += 1
@@ -170,9 +190,17 @@ class Server:
                 upd = s.cell_updates_per_cycle() * a["mgit"]
                 self.results.append(Result(rid=rid, cells=c.ncell() * c.nsnap, cycles=a["mgit"], converged=conv,
                                            t_admit=a["t_admit"], t_done=t_done, latency=t_done - r.arrival,
-                                           resrms=[float(h[0]) for h in s.history], forces=f, updates=upd, q=q))
+                                           resrms=[float(h[0]) for h in s.history], forces=f, updates=upd, q=q,
+                                           drop_reached=a["min"] / a["first"],
+                                           stalled=not conv and a["mgit"] < r.max_cycles))
                 s.close()
                 used -= a["need"]
+                if self.verbose:
+                    res = self.results[-1]
+                    print(f"retired {rid} ({r.grid}, {'rans' if r.rans else 'hb' if r.nharms else 'euler'}, "
+                          f"M {r.mach:.2f}, alpha {r.alpha_deg:.1f}): {res.cycles} cycles, drop "
+                          f"{res.drop_reached:.1e}, {'converged' if conv else 'stalled' if res.stalled else 'max'}, "
+                          f"t {t_done:.1f} s", file=sys.stderr, flush=True)
         return self.results
 
 
@@ -212,6 +240,7 @@ class BatchServer:
     request.  FP64 or FP32-residual (precision 1) per batch."""
     device: int = 0
     keep_state: bool = False
+    verbose: bool = False
     slots: Dict[str, int] = field(default_factory=lambda: dict(BATCH_SLOTS))
     results: List[Result] = field(default_factory=list)
 
@@ -304,6 +333,9 @@ class BatchServer:
                             updates=sv.cell_updates_per_cycle() / c.levels[0].nblocks * sl["cycles"],
                             q=q[block_slice(c, i)].copy() if q is not None else None))
                         b["slot"][i] = None
+                        if self.verbose:
+                            print(f"retired {r.rid} ({r.grid}, batch slot {i}): {sl['cycles']} cycles, "
+                                  f"{'converged' if conv else 'max'}, t {t_done:.1f} s", file=sys.stderr, flush=True)
             if not progressed:
                 time.sleep(2e-5)
         for b in batches.values():
@@ -319,6 +351,7 @@ def summarize(results: List[Result], wall: float, cell_updates: float) -> dict:
     unconv = [r for r in results if not r.converged]
     lat = np.array([r.latency for r in conv])
     out = {"requests": len(results), "converged": len(conv), "unconverged_retired": len(unconv),
+           "stalled": sum(1 for r in unconv if r.stalled),
            "wall_s": round(wall, 3), "solves_per_s": round(len(conv) / wall, 3),
            "latency_p50_s": round(float(np.percentile(lat, 50)), 4) if conv else None,
            "latency_p99_s": round(float(np.percentile(lat, 99)), 4) if conv else None,
@@ -326,23 +359,33 @@ def summarize(results: List[Result], wall: float, cell_updates: float) -> dict:
     if unconv:
         ul = np.array([r.latency for r in unconv])
         out["unconverged_latency_p50_s"] = round(float(np.percentile(ul, 50)), 4)
+        out["unconverged_drop_reached_median"] = float(np.median([r.drop_reached for r in unconv]))
+        out["unconverged_drop_reached_best"] = float(min(r.drop_reached for r in unconv))
     return out
 
 
-def request_mix(n: int, seed: int = 0, rate: float = 4.0, precision: int = 0, drop: float = 1e-2,
-                grids=("small", "medium", "large"), hb_fraction: float = 0.1, max_cycles: int = 600,
-                mach=(0.3, 0.5), alpha_deg=(-2.0, 4.0)) -> List[Request]:
+def request_mix(n: int, seed: int = 0, rate: float = 4.0, precision: int = 0, drop: float = 1e-8,
+                grids=("small", "medium", "large"), hb_fraction: float = 0.1, rans_fraction: float = 0.45,
+                max_cycles: int = 4000, stall_window: int = 400, mach=(0.2, 0.75),
+                alpha_deg=(-4.0, 8.0)) -> List[Request]:
     """n requests with Poisson arrivals at `rate` per second, grid sizes
-    cycling through `grids`, Mach and incidence uniform in the given
-    ranges, a fraction of them harmonic-balance pitching with N = 2."""
+    cycling through `grids`, Mach and incidence uniform in the given ranges;
+    a fraction hb_fraction harmonic-balance pitching with N = 2, a fraction
+    rans_fraction steady Wilcox k-omega RANS, the rest steady Euler (the
+    serving configuration's defaults: M in [0.2, 0.75], alpha in [-4, 8]
+    degrees, a 1e-8 drop)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
     for i in range(n):
         t += float(rng.exponential(1.0 / rate))
+        u = rng.random()
+        hb = u < hb_fraction
+        rans = (not hb) and u < hb_fraction + rans_fraction
         out.append(Request(rid=i, grid=grids[i % len(grids)], mach=float(rng.uniform(*mach)),
-                           alpha_deg=float(rng.uniform(*alpha_deg)), nharms=2 if rng.random() < hb_fraction else 0,
-                           arrival=t, drop=drop, precision=precision, max_cycles=max_cycles))
+                           alpha_deg=float(rng.uniform(*alpha_deg)), nharms=2 if hb else 0, rans=rans,
+                           arrival=t, drop=drop, precision=precision, max_cycles=max_cycles,
+                           stall_window=stall_window))
     return out
 
 
@@ -351,7 +394,7 @@ def cell_updates(results: List[Result]) -> float:
     return float(sum(r.updates for r in results))
 
 
-def sequential(requests: List[Request], device: int = 0) -> List[Result]:
+def sequential(requests: List[Request], device: int = 0, verbose: bool = False) -> List[Result]:
     """The same requests solved one after another with the plain FP64
     synchronous path (smooth / restrict / prolong per routine), for
     comparison."""
@@ -366,19 +409,29 @@ def sequential(requests: List[Request], device: int = 0) -> List[Result]:
         first = None
         conv = False
         mgit = 0
+        best = best_at = None
         while mgit < r.max_cycles:
             mgit += 1
             s.cycle(mgit)
             r1 = float(s.history[-1][0])
             first = r1 if first is None else first
+            if best is None or r1 < 0.5 * best:
+                best, best_at = r1, mgit
             if r1 <= r.drop * first:
                 conv = True
                 break
+            if r.stall_window and mgit - best_at >= r.stall_window:
+                break
         td = time.perf_counter() - t0
+        h1 = [float(h[0]) for h in s.history]
         out.append(Result(rid=r.rid, cells=c.ncell() * c.nsnap, cycles=mgit, converged=conv, t_admit=ta,
-                          t_done=td, latency=td - r.arrival, resrms=[float(h[0]) for h in s.history],
+                          t_done=td, latency=td - r.arrival, resrms=h1,
                           forces=s.force(nsurface=1, include_inviscid=True)[0],
-                          updates=s.cell_updates_per_cycle() * mgit))
+                          updates=s.cell_updates_per_cycle() * mgit, drop_reached=min(h1) / h1[0],
+                          stalled=not conv and mgit < r.max_cycles))
+        if verbose:
+            print(f"sequential {r.rid}: {mgit} cycles, drop {min(h1) / h1[0]:.1e}, t {td:.1f} s", file=sys.stderr,
+                  flush=True)
         s.close()
     return out
 
@@ -388,8 +441,12 @@ def main():
     ap.add_argument("--requests", type=int, default=24)
     ap.add_argument("--rate", type=float, default=4.0, help="mean arrivals per second")
     ap.add_argument("--precision", choices=["fp64", "fp32", "mixed"], default="mixed")
-    ap.add_argument("--drop", type=float, default=1e-2)
-    ap.add_argument("--max-cycles", type=int, default=600)
+    ap.add_argument("--drop", type=float, default=1e-8)
+    ap.add_argument("--max-cycles", type=int, default=4000)
+    ap.add_argument("--stall-window", type=int, default=400,
+                    help="retire a request whose best resrms(1) has not halved in this many cycles (0: never)")
+    ap.add_argument("--rans-fraction", type=float, default=0.45,
+                    help="fraction of steady k-omega RANS requests (the batch mode serves Euler and HB only)")
     ap.add_argument("--batch", action="store_true",
                     help="serve requests of one grid as the blocks of shared batch solves (FP64 / FP32 residual)")
     ap.add_argument("--hw-queues", type=int, default=16,
@@ -401,20 +458,23 @@ def main():
     # concurrent streams of the resident solves
     os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(max(a.hw_queues, 1), 32)))
     prec = {"fp64": 0, "fp32": 1, "mixed": 2}[a.precision]
-    reqs = request_mix(a.requests, rate=a.rate, precision=prec, drop=a.drop, max_cycles=a.max_cycles)
+    reqs = request_mix(a.requests, rate=a.rate, precision=prec, drop=a.drop, max_cycles=a.max_cycles,
+                       stall_window=a.stall_window, rans_fraction=0.0 if a.batch else a.rans_fraction)
     if a.batch:
         for r in reqs:    # batches run one precision throughout: mixed is served as FP32 residual
             r.precision = 1 if prec else 0
-    srv = BatchServer() if a.batch else Server(budget_bytes=int(a.budget_gib * (1 << 30)))
+    srv = BatchServer(verbose=True) if a.batch else Server(budget_bytes=int(a.budget_gib * (1 << 30)), verbose=True)
     t0 = time.perf_counter()
     res = srv.run(reqs)
     wall = time.perf_counter() - t0
     out = {"serving": summarize(res, wall, cell_updates(res)), "mode": "batch" if a.batch else "per-solve streams",
            "precision": a.precision,
-           "drop": a.drop, "rate_per_s": a.rate}
+           "drop": a.drop, "rate_per_s": a.rate,
+           "mix": {"euler": sum(1 for r in reqs if not r.rans and not r.nharms),
+                   "rans": sum(1 for r in reqs if r.rans), "hb_n2": sum(1 for r in reqs if r.nharms)}}
     if a.baseline:
         t0 = time.perf_counter()
-        base = sequential(reqs)
+        base = sequential(reqs, verbose=True)
         wb = time.perf_counter() - t0
         out["sequential_fp64"] = summarize(base, wb, cell_updates(base))
         cl = {r.rid: r.forces[0, 0] for r in base}

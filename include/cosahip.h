@@ -448,8 +448,12 @@ int cosahip_force_noalloc(cosahip_solve* s, int nl, int nsurface, int include_in
  * mode 2: as mode 1 until the level-1 density residual rms reported by
  *         cosahip_smooth falls below tail_rms, FP64 from then on (FP64
  *         defect-correction tail).  Calling again restarts the switch.
+ * mode 3: fast FP64 -- the Euler residual in binary64 with multiply-add
+ *         contraction (one rounding per a*b+c instead of two); results
+ *         differ from the reference's in the last bits and are held to
+ *         the FP64 parity bar.  Euler only (COSAHIP_ERR_ARG when viscous).
  * COSAHIP_ERR_ARG for the per-direction inviscid variant
- * (COSAHIP_RESIDUAL=split), which has no FP32 form.                      */
+ * (COSAHIP_RESIDUAL=split), which has no FP32 or contracted form.        */
 int cosahip_set_precision(cosahip_solve* s, int mode, double tail_rms);
 /* 32 while the next residual is evaluated in FP32, else 64.              */
 int cosahip_precision_in_use(const cosahip_solve* s);

@@ -98,3 +98,24 @@ def test_batch_server_retires_every_request():
         assert r.cycles >= 1 and np.all(np.isfinite(r.forces)) and r.q is not None
     summary = serve.summarize(res, max(r.t_done for r in res), serve.cell_updates(res))
     print(f"batch server: {summary}", flush=True)
+
+
+def test_rans_request_in_mix_same_as_alone():
+    """A steady k-omega RANS request (viscous wall, Wilcox k-omega, the
+    serving mix's RANS share) served beside an Euler and an HB request:
+    bit-identical to the same request served alone, and retired by the
+    stall rule or its drop like any other."""
+    reqs = [serve.Request(rid=0, grid="small", mach=0.6, alpha_deg=4.0, rans=True, arrival=0.0, drop=1e-2,
+                          max_cycles=30, stall_window=20),
+            serve.Request(rid=1, grid="small", mach=0.3, alpha_deg=-2.0, arrival=0.0, drop=1e-2, max_cycles=30),
+            serve.Request(rid=2, grid="small", mach=0.7, alpha_deg=6.0, nharms=2, arrival=0.01, drop=1e-2,
+                          max_cycles=30)]
+    mixed = {r.rid: r for r in serve.Server(keep_state=True).run(reqs)}
+    alone = serve.Server(keep_state=True).run([reqs[0]])[0]
+    r = mixed[0]
+    print(f"RANS request: {r.cycles} cycles, drop reached {r.drop_reached:.2e}, converged {r.converged}, "
+          f"stalled {r.stalled}, CL {r.forces[0, 0] + r.forces[0, 6]:.6f}", flush=True)
+    assert r.q.size == alone.q.size and np.array_equal(r.q, alone.q)
+    assert r.resrms == alone.resrms and r.cycles == alone.cycles
+    assert np.array_equal(r.forces, alone.forces)
+    assert np.isfinite(r.resrms).all() and r.drop_reached < 1.0
