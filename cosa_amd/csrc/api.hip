@@ -463,6 +463,8 @@ static int build_tables(Solve* s, Level& L, const Level* Lc) {
       add_tiles(L.htile[TK_FACES_K], b, ns * im * jm * (km + 1), TS);
     }
     add_tiles(L.htile[TK_CELLS_P1], b, ns * (ll)(B.imax + 1) * (B.jmax + 1) * (B.kmax + 1), TS);
+    add_tiles(L.htile[TK_SHELL], b, ns * (2 * (ll)(B.jmax + 1) * (B.kmax + 1) + im * (2 * (ll)(B.kmax + 1) + 2 * jm)),
+              TS);
     add_tiles(L.htile[TK_HALO], b, ns * ((ll)(B.imax + 3) * jm * km + 4 * im * km + 4 * im * jm), TS);
     add_tiles(L.htile[TK_FUSED], b, residual_tile_items(7, B.imax, B.jmax, B.kmax, ns), 1);
     add_tiles(L.htile[TK_FUSED15], b, residual_tile_items(15, B.imax, B.jmax, B.kmax, ns), 1);

@@ -101,6 +101,7 @@ enum TileKind {
   TK_FACES_J,     // j faces (imax-1) x jmax x (kmax-1)
   TK_FACES_K,     // k faces (imax-1) x (jmax-1) x kmax
   TK_FUSED15,     // residual tiles of 31 x 15 cells x 32-plane k segments (k_residual_tile, TJ = 15)
+  TK_SHELL,       // the halo shell of 0..imax x 0..jmax x 0..kmax (the box without the interior cells)
   TK_NKIND
 };
 

@@ -190,6 +190,47 @@ __global__ void __launch_bounds__(256) k_zero_box(Par p, const Blk* __restrict__
   for (int v = 0; v < p.npde; v++) rb[y + v * n3] = 0.0;
 }
 
+// res = 0 on the halo shell of 0..imax x 0..jmax x 0..kmax (the part of
+// zero(res) that the residual kernels, which write every interior cell, do
+// not cover): the faces i = 0, imax (all j, k), then j = 0, jmax (interior
+// i, all k), then k = 0, kmax (interior i, j).
+__global__ void __launch_bounds__(256) k_zero_shell(Par p, const Blk* __restrict__ blks,
+                                                    const Tile* __restrict__ tiles, double* __restrict__ res) {
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int item = t.base + threadIdx.x;
+  const int im = b.imax - 1, jm = b.jmax - 1;
+  const int ra = 2 * (b.jmax + 1) * (b.kmax + 1), rb = 2 * im * (b.kmax + 1), rc = 2 * im * jm;
+  const int per = ra + rb + rc;
+  if (item >= per * p.nsnap) return;
+  const int n = item / per;
+  int x = item - n * per, i, j, k;
+  if (x < ra) {
+    const int side = x / ((b.jmax + 1) * (b.kmax + 1));
+    x -= side * (b.jmax + 1) * (b.kmax + 1);
+    i = side ? b.imax : 0;
+    j = x % (b.jmax + 1);
+    k = x / (b.jmax + 1);
+  } else if (x < ra + rb) {
+    x -= ra;
+    const int side = x / (im * (b.kmax + 1));
+    x -= side * im * (b.kmax + 1);
+    j = side ? b.jmax : 0;
+    i = x % im + 1;
+    k = x / im;
+  } else {
+    x -= ra + rb;
+    const int side = x / (im * jm);
+    x -= side * im * jm;
+    k = side ? b.kmax : 0;
+    i = x % im + 1;
+    j = x / im + 1;
+  }
+  double* rbk = qblk(p, b, res, n);
+  const ll n3 = n3of(b), y = i3(b, i, j, k);
+  for (int v = 0; v < p.npde; v++) rbk[y + v * n3] = 0.0;
+}
+
 // q2 = q2 + qold2 * (-1) on 0..imax (add_array off_l = 1, prolong)
 __global__ void __launch_bounds__(256) k_sub_box(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
                                                  double* __restrict__ dst, const double* __restrict__ src) {
@@ -1252,9 +1293,9 @@ hipError_t launch_zero_res(Solve& s, Level& L, double* res) {
   return hipGetLastError();
 }
 hipError_t launch_zero_res_shell(Solve& s, Level& L, double* res) {
-  if (L.ntile[TK_CELLS_P1])
-    hipLaunchKernelGGL(k_zero_box, grid1(L.ntile[TK_CELLS_P1]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                       L.tile[TK_CELLS_P1], res, 1);
+  if (L.ntile[TK_SHELL])
+    hipLaunchKernelGGL(k_zero_shell, grid1(L.ntile[TK_SHELL]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_SHELL],
+                       res);
   return hipGetLastError();
 }
 hipError_t launch_sub_box(Solve& s, Level& L, double* dst, const double* src) {

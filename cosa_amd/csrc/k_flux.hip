@@ -793,9 +793,27 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
   const double* bbt = (cconst || !NT) ? nullptr : tbeta + o1;
   double tp[NV];
   double rp = 0.0, rpt = 0.0;
+  // the residual of cell a + 1 is loaded before cell a's result is stored:
+  // the stores may alias the next loads as far as the compiler can tell,
+  // so without it every step of the recurrence waits one memory latency
+  const ll sa = DIR == 2 ? (ll)(b.imax + 3) : (ll)(b.imax + 3) * (b.jmax + 3);   // cell stride along the line
+  double xnx[NV];
+  {
+    const ll c3a = i3(b, t1, DIR == 2 ? 1 : t2, DIR == 3 ? 1 : t2);
+#pragma unroll
+    for (int v = 0; v < NV; v++) xnx[v] = rb[c3a + n3 * v];
+  }
+#pragma unroll 1
   for (int a = 1; a <= L; a++) {
     const int i = t1, j = DIR == 2 ? a : t2, k = DIR == 3 ? a : t2;
     const ll c3 = i3(b, i, j, k), c1 = i1(b, i, j, k);
+    double xa[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) xa[v] = xnx[v];
+    if (a < L) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) xnx[v] = rb[c3 + sa + n3 * v];
+    }
     const double eps = cconst ? epsc : bb[c1];
     const double ca = a == 1 ? 0.0 : -eps, cb = 1 + 2 * eps, cc = a == L ? 0.0 : -eps;
     const double dtv = first ? dtb[c1] : 1.0;
@@ -810,7 +828,7 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
       const double d = a == 1 ? cb : cb - rp * ca, y = ytab[a];
 #pragma unroll
       for (int v = 0; v < NM; v++) {
-        const double x = first ? rb[c3 + n3 * v] * dtv : rb[c3 + n3 * v];
+        const double x = first ? xa[v] * dtv : xa[v];
         const double n = a == 1 ? x : x - tp[v] * ca;
         const double q = n * y;
         tp[v] = n == 0.0 ? q : fma(fma(-q, d, n), y, q);
@@ -819,14 +837,14 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
       r = cc / cb;
 #pragma unroll
       for (int v = 0; v < NM; v++) {
-        const double x = first ? rb[c3 + n3 * v] * dtv : rb[c3 + n3 * v];
+        const double x = first ? xa[v] * dtv : xa[v];
         tp[v] = x / cb;
       }
     } else {
       r = cc / (cb - rp * ca);
 #pragma unroll
       for (int v = 0; v < NM; v++) {
-        const double x = first ? rb[c3 + n3 * v] * dtv : rb[c3 + n3 * v];
+        const double x = first ? xa[v] * dtv : xa[v];
         tp[v] = (x - tp[v] * ca) / (cb - rp * ca);
       }
     }
@@ -843,7 +861,7 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
         const double d = a == 1 ? cbt : cbt - rpt * cat, y = ytab[a];
 #pragma unroll
         for (int v = NM; v < NV; v++) {
-          const double x = first ? rb[c3 + n3 * v] * dtvt : rb[c3 + n3 * v];
+          const double x = first ? xa[v] * dtvt : xa[v];
           const double n = a == 1 ? x : x - tp[v] * cat;
           const double q = n * y;
           tp[v] = n == 0.0 ? q : fma(fma(-q, d, n), y, q);
@@ -852,14 +870,14 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
         rt = cct / cbt;
 #pragma unroll
         for (int v = NM; v < NV; v++) {
-          const double x = first ? rb[c3 + n3 * v] * dtvt : rb[c3 + n3 * v];
+          const double x = first ? xa[v] * dtvt : xa[v];
           tp[v] = x / cbt;
         }
       } else {
         rt = cct / (cbt - rpt * cat);
 #pragma unroll
         for (int v = NM; v < NV; v++) {
-          const double x = first ? rb[c3 + n3 * v] * dtvt : rb[c3 + n3 * v];
+          const double x = first ? xa[v] * dtvt : xa[v];
           tp[v] = (x - tp[v] * cat) / (cbt - rpt * cat);
         }
       }
@@ -870,21 +888,34 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
     for (int v = 0; v < NV; v++) rb[c3 + n3 * v] = tp[v];
   }
   double xn[NV];
+  {   // the forward result of cell L, loaded after its store above
+    const ll c3L = i3(b, t1, DIR == 2 ? L : t2, DIR == 3 ? L : t2);
+#pragma unroll
+    for (int v = 0; v < NV; v++) xnx[v] = rb[c3L + n3 * v];
+  }
+#pragma unroll 1
   for (int a = L; a >= 1; a--) {
     const int i = t1, j = DIR == 2 ? a : t2, k = DIR == 3 ? a : t2;
     const ll c3 = i3(b, i, j, k);
+    double ta[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) ta[v] = xnx[v];
+    if (a > 1) {   // cell a - 1 ahead of cell a's store, as in the forward pass
+#pragma unroll
+      for (int v = 0; v < NV; v++) xnx[v] = rb[c3 - sa + n3 * v];
+    }
     double xv[NV];
     if (a == L) {
 #pragma unroll
-      for (int v = 0; v < NV; v++) xv[v] = rb[c3 + n3 * v];
+      for (int v = 0; v < NV; v++) xv[v] = ta[v];
     } else {
       const double r = tab ? rtab[a] : rs[c3];
 #pragma unroll
-      for (int v = 0; v < NM; v++) xv[v] = rb[c3 + n3 * v] - xn[v] * r;
+      for (int v = 0; v < NM; v++) xv[v] = ta[v] - xn[v] * r;
       if (NT) {
         const double rt = tab ? rtab[a] : rst[c3];
 #pragma unroll
-        for (int v = NM; v < NV; v++) xv[v] = rb[c3 + n3 * v] - xn[v] * rt;
+        for (int v = NM; v < NV; v++) xv[v] = ta[v] - xn[v] * rt;
       }
     }
 #pragma unroll

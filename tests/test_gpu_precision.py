@@ -181,7 +181,7 @@ def test_precision_rejects_unsupported(monkeypatch):
     g = Solve(c, device=0, q=c.perturbed(amp=0.01, seed=1))
     try:
         with pytest.raises(CosahipError):
-            g.set_precision(3)
+            g.set_precision(4)
         with pytest.raises(CosahipError):
             g.set_precision(2, 0.0)
         assert g.precision_in_use() == 64
