@@ -3,7 +3,8 @@
     python -m cosa_amd.build            # library only
 Flags: -O3, -ffp-contract=off (no fused multiply-add, so FP64 results keep
 the reference's rounding), packed-FP32 instructions disabled;
-k_resid_contracted.hip (the fast FP64 residual) with -ffp-contract=fast.
+k_resid_contracted.hip (the fast FP64 residual) with -ffp-contract=fast
+-freciprocal-math.
 """
 from __future__ import annotations
 
@@ -35,7 +36,9 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     procs = []
     for src in SOURCES:
         obj = os.path.join(HERE, "_build", src.replace(".hip", ".o"))
-        flags = [f if (src not in CONTRACTED or f != "-ffp-contract=off") else "-ffp-contract=fast" for f in HIPFLAGS]
+        flags = list(HIPFLAGS)
+        if src in CONTRACTED:   # fused multiply-adds, and x / y as x * (1 / y) where 1 / y is shared
+            flags = [f if f != "-ffp-contract=off" else "-ffp-contract=fast" for f in flags] + ["-freciprocal-math"]
         cmd = [hipcc()] + flags + ["-I", os.path.join(ROOT, "include"), "-I", CSRC, "-c",
                                    os.path.join(CSRC, src), "-o", obj]
         if verbose:

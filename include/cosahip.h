@@ -449,7 +449,8 @@ int cosahip_force_noalloc(cosahip_solve* s, int nl, int nsurface, int include_in
  *         cosahip_smooth falls below tail_rms, FP64 from then on (FP64
  *         defect-correction tail).  Calling again restarts the switch.
  * mode 3: fast FP64 -- the Euler residual in binary64 with multiply-add
- *         contraction (one rounding per a*b+c instead of two); results
+ *         contraction (one rounding per a*b+c instead of two) and
+ *         quotients by a shared divisor as products with its reciprocal; results
  *         differ from the reference's in the last bits and are held to
  *         the FP64 parity bar.  Euler only (COSAHIP_ERR_ARG when viscous).
  * COSAHIP_ERR_ARG for the per-direction inviscid variant
