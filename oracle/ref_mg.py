@@ -1,7 +1,8 @@
 """COSA's own compiled routines (oracle/_ref/libcosaref*.so, built from the
 reference sources by oracle/build_ref.py) driven in the order of the
-reference's smooth() and mg() for steady Euler on blocks without cuts --
-the physics of the uneven multiblock channels.  TEST AND BASELINE
+reference's smooth() and mg() for steady Euler, on blocks joined by 1:1
+cuts or not (the uneven multiblock channels, the NACA 0012 O-grid with its
+seam).  TEST AND BASELINE
 INFRASTRUCTURE ONLY: the CPU baseline of bench.py times it, and
 tests/test_oracle_vs_ref.py requires it to give the oracle's or_mg states
 bit for bit.
@@ -9,7 +10,9 @@ bit for bit.
 Every floating-point operation on the state happens inside the reference's
 compiled *_b kernels: cons2prim_b, bbc, lambdab, bdeltat, muscl_b*,
 roflux_b, bresid, prim2cons_b, resg_brms, cirs_b, update_b, rest1_b,
-rest2_lo_b / rest2_ho_b, prol_bln_b / prol_lin_b, q_bedges, q_bcorns.  The
+rest2_lo_b / rest2_ho_b, prol_bln_b / prol_lin_b, q_bedges, q_bcorns, and
+cut_q for every cut record after the BC fills (cutman_q's loop,
+src/bc.f90:1801-1891).  The
 three whole-level utilities of src/util.f90 that smooth calls between them
 (copy_array, add_array, zero: a copy, an addition, a fill) are numpy
 operations on the same interior ranges here, since util.f90's whole-level
@@ -100,14 +103,15 @@ class RefMG:
 
     def __init__(self, case, q1: np.ndarray, lib: str = PARITY_LIB):
         c = case.config
-        if c["viscous"] or c["nharms"] or c["moving"] or any(L.ncuts for L in case.levels):
-            raise ValueError("ref_mg drives steady Euler on blocks without cuts only")
+        if c["viscous"] or c["nharms"] or c["moving"]:
+            raise ValueError("ref_mg drives steady Euler only")
         self.case = case
         self.lib = ctypes.CDLL(lib, mode=os.RTLD_LAZY)
         self._cfg = to_struct(c)
         self.lib.ref_set_config(ctypes.byref(self._cfg))
         self.cfg = c
         self.lev: List[List[_Blk]] = []
+        self.cuts = [np.ascontiguousarray(np.asarray(L.cutopo, np.int32).reshape(-1, 21)) for L in case.levels]
         for L in case.levels:
             blks = [_Blk(case, L.nl, b) for b in range(L.nblocks)]
             fs = case.freestream(L.nl)
@@ -122,6 +126,14 @@ class RefMG:
     def _v(self, B, name):
         return getattr(B, name).reshape(B.shape)
 
+    def _cut(self, nl, name):
+        """cutman_q: cut_q of every cut record of level nl, in record order,
+        on the blocks' arrays `name`."""
+        blks = self.lev[nl - 1]
+        for t in self.cuts[nl - 1]:
+            B1, B2 = blks[int(t[0]) - 1], blks[int(t[9]) - 1]
+            self.lib.ref_cut_q(*B1.dims, _d(getattr(B1, name)), *B2.dims, _d(getattr(B2, name)), _i(t))
+
     def smooth(self, nl, from_fine, to_coarse, niter, mgit):
         lib, c = self.lib, self.cfg
         blks = self.lev[nl - 1]
@@ -135,6 +147,7 @@ class RefMG:
                     lib.ref_cons2prim_b(_d(B.q), _d(B.work), *B.dims, 0)
                     lib.ref_bbc(_d(B.q), _d(B.si), _d(B.sj), _d(B.sk), _d(B.dist), nl, _i(B.bctopo), B.nbcs,
                                 *B.dims)
+                self._cut(nl, "q")
                 if it % c["iupdt"] == 0 and irk == 1:
                     for B in blks:
                         lib.ref_lambdab(_d(B.q), _d(B.si), _d(B.sj), _d(B.sk), _d(B.xd), _d(B.xd), _d(B.xd),
@@ -191,15 +204,22 @@ class RefMG:
 
     def prolong(self, nl):
         lib = self.lib
-        for B1, B2 in zip(self.lev[nl - 1], self.lev[nl]):
+        pairs = list(zip(self.lev[nl - 1], self.lev[nl]))
+        for B1, B2 in pairs:
             lib.ref_rest1_b(*B1.dims, *B2.dims, _d(B1.q), _d(B2.qold), _d(B1.vol), _d(B2.vol))
-            for a in (B2.qold, B2.q):
+        for name in ("qold", "q"):
+            for _, B2 in pairs:
+                a = getattr(B2, name)
                 lib.ref_cons2prim_b(_d(a), _d(B2.work), *B2.dims, 0)
                 lib.ref_bbc(_d(a), _d(B2.si), _d(B2.sj), _d(B2.sk), _d(B2.dist), nl + 1, _i(B2.bctopo), B2.nbcs,
                             *B2.dims)
+            self._cut(nl + 1, name)
+            for _, B2 in pairs:
+                a = getattr(B2, name)
                 lib.ref_prim2cons_b(_d(a), _d(B2.work), *B2.dims)
                 lib.ref_q_bedges(_d(a), _d(B2.mut), *B2.dims)
                 lib.ref_q_bcorns(_d(a), _d(B2.mut), *B2.dims)
+        for B1, B2 in pairs:
             q2, qo2 = self._v(B2, "q"), self._v(B2, "qold")
             q2[B2.box] = q2[B2.box] + -1.0 * qo2[B2.box]
             lib.ref_prol_b(int(self.cfg["bln_prol"]), *B1.dims, *B2.dims, _d(B1.q), _d(B2.q), _d(B1.vol))

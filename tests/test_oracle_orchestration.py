@@ -183,17 +183,23 @@ def test_fixed_point_kept_and_no_coarse_correction(fixed_point, irsop):
 
 @pytest.mark.ref
 @pytest.mark.skipif(not __import__("oracle_bind").ref_available(), reason="oracle/_ref not built")
-def test_reference_kernels_in_mg_order_equal_or_mg():
+@pytest.mark.parametrize("grid", ["two_channels", "naca_ogrid_seam"])
+def test_reference_kernels_in_mg_order_equal_or_mg(grid):
     """oracle/ref_mg.py drives COSA's compiled kernels (the CPU baseline of
-    the benchmark) in smooth()/mg() order; three 3-level V-cycles of two
-    uneven channel blocks give the oracle's states on every level and its
+    the benchmark) in smooth()/mg() order, cut_q included; three 3-level
+    V-cycles of two uneven channel blocks, and of the NACA 0012 O-grid whose
+    seam is a self-cut, give the oracle's states on every level and its
     norms, bit for bit."""
     import sys
     import os
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     from oracle.ref_mg import RefMG
-    c = cases.channel_case([(17, 9, 9), (9, 17, 9)], nlevel=3, name="two_channels", machfs=0.7, irsop=1, cfl=2.0,
-                           cfl_irs=4.0, ilimf=4, epslimf=1e-6, nl_crs=2, ncycle=3)
+    if grid == "two_channels":
+        c = cases.channel_case([(17, 9, 9), (9, 17, 9)], nlevel=3, name="two_channels", machfs=0.7, irsop=1,
+                               cfl=2.0, cfl_irs=4.0, ilimf=4, epslimf=1e-6, nl_crs=2, ncycle=3)
+    else:
+        c = cases.naca_case(65, 17, 9, nlevel=3, name="naca_ogrid", irsop=1, cfl=2.0, cfl_irs=4.0, nl_crs=2,
+                            ncycle=3)
     q0 = c.perturbed(amp=0.01, seed=1)
     o = Oracle()
     o.set_config(c.config)
