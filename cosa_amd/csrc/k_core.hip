@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(256) k_add_int(Par p, const Blk* __restrict__ 
 
 // ---- zero (src/util.f90:238-369) on 0..imax x 0..jmax x 0..kmax ----
 __global__ void __launch_bounds__(256) k_zero_box(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
-                                                  double* __restrict__ res, int shell_only) {
+                                                  double* __restrict__ res) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   const int item = t.base + threadIdx.x;
@@ -184,7 +184,6 @@ __global__ void __launch_bounds__(256) k_zero_box(Par p, const Blk* __restrict__
   const int i = x % (b.imax + 1);
   x /= (b.imax + 1);
   const int j = x % (b.jmax + 1), k = x / (b.jmax + 1);
-  if (shell_only && i > 0 && i < b.imax && j > 0 && j < b.jmax && k > 0 && k < b.kmax) return;
   double* rb = qblk(p, b, res, n);
   const ll n3 = n3of(b), y = i3(b, i, j, k);
   for (int v = 0; v < p.npde; v++) rb[y + v * n3] = 0.0;
@@ -1289,7 +1288,7 @@ hipError_t launch_add_interior(Solve& s, Level& L, double* dst, const double* sr
 hipError_t launch_zero_res(Solve& s, Level& L, double* res) {
   if (L.ntile[TK_CELLS_P1])
     hipLaunchKernelGGL(k_zero_box, grid1(L.ntile[TK_CELLS_P1]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                       L.tile[TK_CELLS_P1], res, 0);
+                       L.tile[TK_CELLS_P1], res);
   return hipGetLastError();
 }
 hipError_t launch_zero_res_shell(Solve& s, Level& L, double* res) {

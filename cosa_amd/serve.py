@@ -31,7 +31,10 @@ from typing import Dict, List, Optional
 
 import numpy as np
 
+from . import _lib
 from . import case as cases
+
+POSITIVITY = 5   # COSAHIP_ERR_POSITIVITY: a cell's pressure or density fell to qmin
 
 # C5 grid sizes: O-grid nodes (i around the section, j normal); 5 k-nodes
 # (4 cells) so that both multigrid levels keep at least 2 cells in k
@@ -91,6 +94,7 @@ class Result:
     q: Optional[np.ndarray] = None
     drop_reached: float = 1.0  # lowest resrms(1) / first resrms(1)
     stalled: bool = False      # retired by the stall rule before its drop
+    failed: bool = False       # retired after a positivity failure (the reference stops there)
 
 
 def device_bytes_estimate(c: "cases.Case") -> int:
@@ -159,9 +163,16 @@ class Server:
                     continue
                 if not s.cycle_ready():
                     continue
-                s.cycle_wait()
                 a["pending"] = False
                 progressed = True
+                try:
+                    s.cycle_wait()
+                except _lib.CosahipError as e:
+                    if e.code != POSITIVITY:
+                        raise
+                    a["failed"] = True
+                    retire.append((rid, False))
+                    continue
                 r1 = float(s.history[-1][0])
                 if a["first"] is None:
                     a["first"] = r1
@@ -185,21 +196,23 @@ class Server:
                 a = active.pop(rid)
                 s, r, c = a["solve"], a["req"], a["case"]
                 t_done = clock() - t0
-                f = s.force(nsurface=1, include_inviscid=True)[0]
+                failed = a.get("failed", False)
+                f = np.zeros((c.nsnap, 12)) if failed else s.force(nsurface=1, include_inviscid=True)[0]
                 q = s.download("q") if self.keep_state else None
                 upd = s.cell_updates_per_cycle() * a["mgit"]
+                first = a["first"] if a["first"] is not None else 1.0
                 self.results.append(Result(rid=rid, cells=c.ncell() * c.nsnap, cycles=a["mgit"], converged=conv,
                                            t_admit=a["t_admit"], t_done=t_done, latency=t_done - r.arrival,
                                            resrms=[float(h[0]) for h in s.history], forces=f, updates=upd, q=q,
-                                           drop_reached=a["min"] / a["first"],
-                                           stalled=not conv and a["mgit"] < r.max_cycles))
+                                           drop_reached=a.get("min", first) / first, failed=failed,
+                                           stalled=not conv and not failed and a["mgit"] < r.max_cycles))
                 s.close()
                 used -= a["need"]
                 if self.verbose:
                     res = self.results[-1]
                     print(f"retired {rid} ({r.grid}, {'rans' if r.rans else 'hb' if r.nharms else 'euler'}, "
                           f"M {r.mach:.2f}, alpha {r.alpha_deg:.1f}): {res.cycles} cycles, drop "
-                          f"{res.drop_reached:.1e}, {'converged' if conv else 'stalled' if res.stalled else 'max'}, "
+                          f"{res.drop_reached:.1e}, {'converged' if conv else 'failed' if res.failed else 'stalled' if res.stalled else 'max'}, "
                           f"t {t_done:.1f} s", file=sys.stderr, flush=True)
         return self.results
 
@@ -304,20 +317,35 @@ class BatchServer:
                     continue
                 if not sv.cycle_ready():
                     continue
-                sv.cycle_wait()
                 b["pending"] = False
                 progressed = True
+                bad = -1   # the slot whose block failed the positivity test this cycle
+                try:
+                    sv.cycle_wait()
+                except _lib.CosahipError as e:
+                    if e.code != POSITIVITY:
+                        raise
+                    bad = int(e.info[1]) - 1
                 rms = sv.block_rms()
                 done = []
                 for i, sl in enumerate(b["slot"]):
                     if sl is None:
                         continue
                     sl["cycles"] += 1
+                    if i == bad:
+                        sl["failed"] = True
+                        done.append((i, False))
+                        continue
                     r1 = float(rms[i, 0])
                     if sl["first"] is None:
                         sl["first"] = r1
+                        sl["best"], sl["best_at"] = r1, sl["cycles"]
+                    if r1 < 0.5 * sl["best"]:
+                        sl["best"], sl["best_at"] = r1, sl["cycles"]
+                    sl["min"] = min(sl.get("min", r1), r1)
                     conv = r1 <= sl["req"].drop * sl["first"]
-                    if conv or sl["cycles"] >= sl["req"].max_cycles:
+                    w = sl["req"].stall_window
+                    if conv or sl["cycles"] >= sl["req"].max_cycles or (w and sl["cycles"] - sl["best_at"] >= w):
                         done.append((i, conv))
                 if done:
                     f = sv.force_blocks(nsurface=1, include_inviscid=True)
@@ -327,12 +355,23 @@ class BatchServer:
                     for i, conv in done:
                         sl = b["slot"][i]
                         r = sl["req"]
+                        failed = sl.get("failed", False)
+                        first = sl["first"] if sl["first"] is not None else 1.0
                         self.results.append(Result(
                             rid=r.rid, cells=nc * c.nsnap, cycles=sl["cycles"], converged=conv, t_admit=sl["t_admit"],
-                            t_done=t_done, latency=t_done - r.arrival, resrms=[], forces=f[i, 0],
+                            t_done=t_done, latency=t_done - r.arrival, resrms=[],
+                            forces=np.zeros_like(f[i, 0]) if failed else f[i, 0],
                             updates=sv.cell_updates_per_cycle() / c.levels[0].nblocks * sl["cycles"],
-                            q=q[block_slice(c, i)].copy() if q is not None else None))
+                            q=q[block_slice(c, i)].copy() if q is not None else None,
+                            drop_reached=sl.get("min", first) / first, failed=failed,
+                            stalled=not conv and not failed and sl["cycles"] < r.max_cycles))
                         b["slot"][i] = None
+                        if failed:
+                            # the failed block restarts from its freestream so the batch goes on
+                            for nl in range(1, c.config["nlevel"] + 1):
+                                qq = sv.download("q", nl)
+                                qq[block_slice(c, i, nl)] = init[r.rid][nl - 1]
+                                sv.upload("q", qq, nl)
                         if self.verbose:
                             print(f"retired {r.rid} ({r.grid}, batch slot {i}): {sl['cycles']} cycles, "
                                   f"{'converged' if conv else 'max'}, t {t_done:.1f} s", file=sys.stderr, flush=True)
@@ -351,7 +390,7 @@ def summarize(results: List[Result], wall: float, cell_updates: float) -> dict:
     unconv = [r for r in results if not r.converged]
     lat = np.array([r.latency for r in conv])
     out = {"requests": len(results), "converged": len(conv), "unconverged_retired": len(unconv),
-           "stalled": sum(1 for r in unconv if r.stalled),
+           "stalled": sum(1 for r in unconv if r.stalled), "failed": sum(1 for r in unconv if r.failed),
            "wall_s": round(wall, 3), "solves_per_s": round(len(conv) / wall, 3),
            "latency_p50_s": round(float(np.percentile(lat, 50)), 4) if conv else None,
            "latency_p99_s": round(float(np.percentile(lat, 99)), 4) if conv else None,
@@ -412,9 +451,16 @@ def sequential(requests: List[Request], device: int = 0, verbose: bool = False) 
         conv = False
         mgit = 0
         best = best_at = None
+        failed = False
         while mgit < r.max_cycles:
             mgit += 1
-            s.cycle(mgit)
+            try:
+                s.cycle(mgit)
+            except _lib.CosahipError as e:
+                if e.code != POSITIVITY:
+                    raise
+                failed = True
+                break
             r1 = float(s.history[-1][0])
             first = r1 if first is None else first
             if best is None or r1 < 0.5 * best:
@@ -425,12 +471,12 @@ def sequential(requests: List[Request], device: int = 0, verbose: bool = False) 
             if r.stall_window and mgit - best_at >= r.stall_window:
                 break
         td = time.perf_counter() - t0
-        h1 = [float(h[0]) for h in s.history]
+        h1 = [float(h[0]) for h in s.history] or [1.0]
         out.append(Result(rid=r.rid, cells=c.ncell() * c.nsnap, cycles=mgit, converged=conv, t_admit=ta,
                           t_done=td, latency=td - r.arrival, resrms=h1,
-                          forces=s.force(nsurface=1, include_inviscid=True)[0],
-                          updates=s.cell_updates_per_cycle() * mgit, drop_reached=min(h1) / h1[0],
-                          stalled=not conv and mgit < r.max_cycles))
+                          forces=np.zeros((c.nsnap, 12)) if failed else s.force(nsurface=1, include_inviscid=True)[0],
+                          updates=s.cell_updates_per_cycle() * mgit, drop_reached=min(h1) / h1[0], failed=failed,
+                          stalled=not conv and not failed and mgit < r.max_cycles))
         if verbose:
             print(f"sequential {r.rid}: {mgit} cycles, drop {min(h1) / h1[0]:.1e}, t {td:.1f} s", file=sys.stderr,
                   flush=True)
