@@ -17,8 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcosahip.so")
-SOURCES = ["api.hip", "k_core.hip", "k_flux.hip", "k_resid.hip", "k_resid_contracted.hip", "k_turb.hip",
-           "k_force.hip", "k_grid.hip"]
+SOURCES = ["api.hip", "k_core.hip", "k_flux.hip", "k_resid.hip", "k_resid_contracted.hip", "k_resid_visc.hip",
+           "k_turb.hip", "k_force.hip", "k_grid.hip"]
 # sources built with multiply-add contraction (the fast FP64 residual)
 CONTRACTED = {"k_resid_contracted.hip"}
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

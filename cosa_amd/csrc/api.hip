@@ -271,6 +271,7 @@ int cosahip_create(const cosahip_config* cfg, int device, cosahip_solve** out) {
   const char* vk = getenv("COSAHIP_VISCOUS");
   s->fuse_viscous = vk && strcmp(vk, "fused") == 0;
   s->viscous_faces = !(vk && (strcmp(vk, "march") == 0 || strcmp(vk, "fused") == 0));
+  s->viscous_tile = vk && strcmp(vk, "tile") == 0;
   if (device >= 0) {
     if (hipSetDevice(device) != hipSuccess) {
       delete s;
@@ -747,7 +748,9 @@ extern "C" int cosahip_define_level(cosahip_solve* h, int nl, int nblocks, const
     CK(hipMemset(L.f[f], 0, sz * sizeof(double)));
   }
   if (p.viscous) {
-    const ll nv = L.n0tot * p.npde * p.nsnap;
+    // one slot of face fluxes per direction (the tile arrangement keeps all
+    // three; the others use the first)
+    const ll nv = 3 * L.n0tot * p.npde * p.nsnap;
     if (hipMalloc(&L.vflux, nv * sizeof(double)) != hipSuccess)
       return fail(s, COSAHIP_ERR_NOMEM, "device allocation failed");
     CK(hipMemset(L.vflux, 0, nv * sizeof(double)));
@@ -2269,19 +2272,29 @@ extern "C" int cosahip_autotune(cosahip_solve* h, int reps, int32_t* choice) {
   if (rc == COSAHIP_OK && s->par.viscous && !s->fuse_viscous && getenv("COSAHIP_VISCOUS") == nullptr &&
       !s->lev.empty() && s->lev[0].nblocks) {
     Level& L = s->lev[0];
-    float t[2] = {0, 0};
-    const bool keep = s->viscous_faces;
+    float t[3] = {0, 0, 0};
+    const bool keep_faces = s->viscous_faces, keep_tile = s->viscous_tile;
     // on copies: the residual also fills the edge halos of the state it reads
     const ll nq = L.n3tot * s->par.npde * s->par.nsnap;
     CK(hipMemcpyAsync(L.dq, L.f[COSAHIP_Q], nq * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
-    for (int f = 0; f <= 1 && rc == COSAHIP_OK; f++) {
-      s->viscous_faces = f == 1;
+    for (int f = 0; f <= 2 && rc == COSAHIP_OK; f++) {   // march, stored faces, tile pass
+      s->viscous_faces = f >= 1;
+      s->viscous_tile = f == 2;
       rc = timed([&]() { return launch_residual(*s, L, L.dq, L.work1); }, t[f]);
     }
-    s->viscous_faces = rc == COSAHIP_OK ? (t[1] <= t[0]) : keep;
-    if (choice) choice[3] = s->viscous_faces ? 1 : 0;
+    if (rc == COSAHIP_OK) {
+      int best = 1;
+      for (int f = 0; f <= 2; f++)
+        if (t[f] < t[best]) best = f;
+      s->viscous_faces = best >= 1;
+      s->viscous_tile = best == 2;
+    } else {
+      s->viscous_faces = keep_faces;
+      s->viscous_tile = keep_tile;
+    }
+    if (choice) choice[3] = s->viscous_tile ? 2 : (s->viscous_faces ? 1 : 0);
   } else if (choice) {
-    choice[3] = s->viscous_faces ? 1 : 0;
+    choice[3] = s->viscous_tile ? 2 : (s->viscous_faces ? 1 : 0);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -217,7 +217,7 @@ struct Level {
   int maxcut = 0;
   bool thin = false;          // a block has < 2 cells in some direction
   // viscous / k-omega scratch
-  double* vflux = nullptr;    // face fluxes (imax, jmax, kmax, npde, 0:2N), off_0 offsets
+  double* vflux = nullptr;    // face fluxes (imax, jmax, kmax, npde, 0:2N), off_0 offsets; 3 slots
   double* tbeta[3] = {};      // CIRS coefficients of k, omega (P1, 0:2N)
   double* h0tab = nullptr;    // per (block, snapshot, direction): h0l, h0r of the block's last face
   double** dxd = nullptr;     // device table of the nine metric-derivative fields
@@ -286,11 +286,14 @@ struct Solve {
   bool split_residual = false;
   int resid_force = -1;   // COSAHIP_RESIDUAL=fused|tile7|tile7reg|tile15|tile7w4|sva7|sva15|sva7w4: that kernel on every level
   // viscous face fluxes after the inviscid pass of each direction: face
-  // fluxes stored then differenced (default, measured fastest), flux and
-  // difference in one marching kernel (COSAHIP_VISCOUS=march), or fused into
-  // the inviscid line march (COSAHIP_VISCOUS=fused), all at create time
+  // fluxes stored then differenced (default), flux and difference in one
+  // marching kernel (COSAHIP_VISCOUS=march), or fused into the inviscid line
+  // march (COSAHIP_VISCOUS=fused); or the face fluxes of all three
+  // directions stored, then one tile pass for the inviscid fluxes and every
+  // difference (COSAHIP_VISCOUS=tile), all at create time
   bool fuse_viscous = false;
   bool viscous_faces = true;
+  bool viscous_tile = false;
   // CIRS i-sweep: one thread per (line, variable) on levels with fewer than
   // CIRS_IV_MAX_LINES lines along i, one thread per line with LDS-staged
   // chunks on larger ones; COSAHIP_CIRS_I=lds|iv at create time forces one

@@ -31,9 +31,6 @@
 
 namespace cosahip {
 
-constexpr int RT_TI = 31;   // cells per tile along i (32 lanes: one more face than cells)
-constexpr int RT_KSEG = 32; // k-planes per work-group
-
 // This file is compiled twice: as itself (no multiply-add contraction, the
 // reference's rounding) and from k_resid_contracted.hip with
 // COSAHIP_CONTRACTED defined and contraction on (the fast FP64 mode); the

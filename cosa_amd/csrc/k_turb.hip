@@ -624,11 +624,40 @@ static hipError_t residual_viscous_t(Solve& s, Level& L, const ResIn<T>& in, dou
     e = launch_h0_last<T>(s, L, in);
     if (e != hipSuccess) return e;
   }
+  const int fk[3] = {TK_FACES_I, TK_FACES_J, TK_FACES_K};
   if (s.fuse_viscous || !p.viscous) {
     for (int d = 1; d <= 3; d++) {
       e = launch_flux_dir<T>(s, L, d, in, res);   // inviscid + viscous fluxes of direction d
       if (e != hipSuccess) return e;
     }
+  } else if (s.viscous_tile) {
+    // the viscous face fluxes of direction d into slot d - 1, then one pass
+    // forms the inviscid fluxes of all directions and sums both in the
+    // reference's order (k_resid_visc.hip)
+    const ll vslot = L.n0tot * p.npde * p.nsnap;
+    for (int d = 1; d <= 3; d++) {
+      const int nt = L.ntile[fk[d - 1]];
+      if (!nt) continue;
+      const T* xd = in.xd[3 * (d - 1)];
+      const T* ed = in.xd[3 * (d - 1) + 1];
+      const T* zd = in.xd[3 * (d - 1) + 2];
+      T* vd = vflux + (d - 1) * vslot;
+#define VF(D, NV)                                                                                                 \
+  hipLaunchKernelGGL((k_vflux<D, NV, T>), grid1(nt), dim3(TS), 0, s.stream, p, L.dblk, L.tile[fk[d - 1]], in.q,    \
+                     in.mut, in.s[D - 1], xd, ed, zd, L.sbc, L.sbc_doff, vd, L.f[COSAHIP_DIST])
+      if (p.npde == 7) {
+        if (d == 1) VF(1, 7);
+        else if (d == 2) VF(2, 7);
+        else VF(3, 7);
+      } else {
+        if (d == 1) VF(1, 5);
+        else if (d == 2) VF(2, 5);
+        else VF(3, 5);
+      }
+#undef VF
+    }
+    e = launch_residual_tile_visc<T>(s, L, in, res, vflux, vslot);
+    if (e != hipSuccess) return e;
   } else {
     // per direction: inviscid line march, then the viscous face fluxes.
     // Default (viscous_faces): k_vflux stores the face fluxes of direction d
@@ -640,7 +669,6 @@ static hipError_t residual_viscous_t(Solve& s, Level& L, const ResIn<T>& in, dou
     const Par porig = s.par;
     Par pi = porig;
     pi.viscous = 0;
-    const int fk[3] = {TK_FACES_I, TK_FACES_J, TK_FACES_K};
     for (int d = 1; d <= 3; d++) {
       s.par = pi;
       e = launch_flux_dir<T>(s, L, d, in, res, (s.viscous_faces && d > 1) ? vflux : nullptr);
@@ -689,7 +717,7 @@ static hipError_t residual_viscous_t(Solve& s, Level& L, const ResIn<T>& in, dou
       hipLaunchKernelGGL((k_vresid<3, 5, T>), grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, p, L.dblk,
                          L.tile[TK_CELLS], vflux, res);
   }
-  const bool vpre3 = p.viscous && !s.fuse_viscous && s.viscous_faces;
+  const bool vpre3 = p.viscous && !s.fuse_viscous && !s.viscous_tile && s.viscous_faces;
   T* cmet = sizeof(T) == 4 ? reinterpret_cast<T*>(L.cmet32) : reinterpret_cast<T*>(L.cmet);
   bool& cmet_ok = sizeof(T) == 4 ? L.cmet32_ok : L.cmet_ok;
   if (p.kom && L.ntile[TK_CELLS] && !cmet_ok) {

@@ -64,6 +64,9 @@ hipError_t launch_residual_tile(Solve& s, Level& L, int variant, const float* q,
 hipError_t launch_residual_tile_contracted(Solve& s, Level& L, int variant, const double* q, const double* si,
                                            const double* sj, const double* sk, double* res);
 // work items of one block in the tile table of a TJ-row residual tile
+// (tiles of RT_TI x TJ cells, k segments of RT_KSEG planes)
+constexpr int RT_TI = 31;    // cells per tile along i (32 lanes: one more face than cells)
+constexpr int RT_KSEG = 32;  // k-planes per work-group
 ll residual_tile_items(int TJ, int imax, int jmax, int kmax, int nsnap);
 // the RK update fused into the last CIRS sweep (mean flow): q = qold - ar
 // res dtvol per cell, then (prim) the next stage's cons2prim; q32: the
@@ -82,6 +85,12 @@ template <typename T>
 hipError_t launch_flux_dir(Solve& s, Level& L, int idir, const ResIn<T>& in, double* res, const T* vpre = nullptr);
 template <typename T>
 hipError_t launch_h0_last(Solve& s, Level& L, const ResIn<T>& in);
+// the inviscid residual of all three directions of a viscous / k-omega
+// level in one pass (k_resid_visc.hip), with the viscous face fluxes of
+// direction d read from vf + (d - 1) vslot and differenced after the
+// inviscid ones of that direction; vf null: inviscid only
+template <typename T>
+hipError_t launch_residual_tile_visc(Solve& s, Level& L, const ResIn<T>& in, double* res, const T* vf, ll vslot);
 
 // ---- viscous fluxes and Wilcox k-omega (k_turb.hip) ----
 hipError_t launch_residual_viscous(Solve& s, Level& L, const double* q, double* res);

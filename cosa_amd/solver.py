@@ -305,7 +305,8 @@ class Solve:
         rnames = {-1: None, 0: "generic_fused", 1: "tile31x7", 2: "tile31x7_register_window", 3: "tile31x15",
                   4: "tile31x7_128vgpr", 5: "tile31x7_shared_slopes", 6: "tile31x15_shared_slopes",
                   7: "tile31x7_shared_slopes_128vgpr"}
-        return {"cirs_i_sweep": [names[ch[m]] for m in range(3)], "viscous_faces_stored": bool(ch[3]),
+        vnames = {0: "march", 1: "stored_faces", 2: "tile_pass"}
+        return {"cirs_i_sweep": [names[ch[m]] for m in range(3)], "viscous": vnames[ch[3]],
                 "residual": [rnames.get(ch[4 + m]) for m in range(3)]}
 
     # -- blocks as independent solves (serving batches) -----------------------

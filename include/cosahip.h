@@ -332,13 +332,15 @@ int cosahip_cycle_ready(cosahip_solve* s);
  * Times the interchangeable kernel arrangements on the solve's current
  * state and keeps the fastest: per level the CIRS i-sweep (one thread per
  * line with LDS-staged chunks, or one thread per line and variable), and
- * for viscous runs the viscous face fluxes (stored then differenced, or
- * flux and difference in one line march).  The arrangements give
+ * for viscous runs the viscous face fluxes (stored then differenced, flux
+ * and difference in one line march, or the fluxes of all three directions
+ * stored and differenced in one tile pass with the inviscid fluxes).  The
+ * arrangements give
  * bit-identical results; the state is not changed (timing runs on a copy of
  * the residual).  choice[0..2]: CIRS i-sweep of levels 1..3 (1 one thread
  * per line with LDS-staged chunks, 2 one thread per line and variable,
  * 3 whole lines resident in LDS -- irsop 1 only --, 0 level absent),
- * choice[3]: 1 stored face fluxes, 0 march; choice[4..6]: inviscid
+ * choice[3]: 1 stored face fluxes, 0 march, 2 tile pass; choice[4..6]: inviscid
  * residual kernel of levels 1..3 (0 generic fused, 1 specialised 31x7
  * tiles with the k-window from LDS/L2, 2 the same with a register
  * k-window, 3 31x15 tiles, 4 as 1 held to 128 VGPRs, 5 / 6 31x7 / 31x15
@@ -350,7 +352,7 @@ int cosahip_cycle_ready(cosahip_solve* s);
  * forces the
  * inviscid residual kernel, COSAHIP_CIRS_I=lds|iv|line forces the i-sweep, COSAHIP_CIRS_IL_LDS sets
  * the LDS bytes per work-group of the line-resident sweep (4096..65536,
- * default 40960), COSAHIP_VISCOUS=march|fused the viscous arrangement.    */
+ * default 40960), COSAHIP_VISCOUS=march|fused|tile the viscous arrangement. */
 int cosahip_autotune(cosahip_solve* s, int reps, int32_t* choice);
 
 /* ---- blocks as independent solves (no counterpart in the reference) -----

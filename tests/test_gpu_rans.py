@@ -178,7 +178,7 @@ def test_viscous_kernel_variants_agree(monkeypatch):
     c = CASES[0]
     q0 = c.perturbed(amp=0.02, seed=13)
     out = []
-    for mode in ("default", "fused", "march"):
+    for mode in ("default", "fused", "march", "tile"):
         if mode == "default":
             monkeypatch.delenv("COSAHIP_VISCOUS", raising=False)
         else:
@@ -195,6 +195,43 @@ def test_viscous_kernel_variants_agree(monkeypatch):
     m = interior_mask(c)
     assert np.array_equal(out[0][m], out[1][m])
     assert np.array_equal(out[0][m], out[2][m])
+    assert np.array_equal(out[0][m], out[3][m])
+
+
+def _tile_pass_cases():
+    return CASES + [_mg_case("bsl_hb_rkim2"), _mg_case("rans_naca3")]
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32res"])
+@pytest.mark.parametrize("c", _tile_pass_cases(), ids=lambda c: c.name)
+def test_viscous_tile_pass_matches(c, prec, monkeypatch):
+    """The tile arrangement (COSAHIP_VISCOUS=tile: viscous face fluxes of
+    the three directions stored, then one 31 x 7 tile pass forming the
+    inviscid fluxes of every direction and summing them with the viscous
+    differences in the reference's order) leaves the same bits as the
+    default per-direction kernels after two smoothing iterations, in FP64
+    and in the FP32-residual mode (laminar, Wilcox, Menter BSL / SST,
+    harmonic balance, the NACA O-grid with tiles that overhang the block)."""
+    from cosa_amd.solver import Solve
+    q0 = c.perturbed(amp=0.02, seed=23)
+    out = []
+    for mode in (None, "tile"):
+        if mode is None:
+            monkeypatch.delenv("COSAHIP_VISCOUS", raising=False)
+        else:
+            monkeypatch.setenv("COSAHIP_VISCOUS", mode)
+        g = Solve(c, device=0, q=q0)
+        try:
+            if prec:
+                g.set_precision(prec)
+            g.smooth(1, False, False, 2, 1)
+            out.append((g.download("q"), g.download("res")))
+        finally:
+            g.close()
+    _say(f"tile pass {c.name} prec {prec}: compared {out[0][0].size} state values")
+    assert np.array_equal(out[0][0], out[1][0])
+    m = interior_mask(c)
+    assert np.array_equal(out[0][1][m], out[1][1][m])
 
 
 def test_cirs_i_variants_agree(monkeypatch):
