@@ -259,7 +259,8 @@ int cosahip_create(const cosahip_config* cfg, int device, cosahip_solve** out) {
     s->resid_force = strcmp(rk, "fused") == 0 ? RESID_GENERIC : strcmp(rk, "tile7") == 0 ? RESID_TILE7
                      : strcmp(rk, "tile7reg") == 0 ? RESID_TILE7_REG : strcmp(rk, "tile15") == 0 ? RESID_TILE15
                      : strcmp(rk, "tile7w4") == 0 ? RESID_TILE7_W4 : strcmp(rk, "sva7") == 0 ? RESID_SVA7
-                     : strcmp(rk, "sva15") == 0 ? RESID_SVA15 : strcmp(rk, "sva7w4") == 0 ? RESID_SVA7_W4 : -1;
+                     : strcmp(rk, "sva15") == 0 ? RESID_SVA15 : strcmp(rk, "sva7w4") == 0 ? RESID_SVA7_W4
+                     : strcmp(rk, "tile32w4") == 0 ? RESID_TILE32_W4 : strcmp(rk, "sva32") == 0 ? RESID_SVA32 : -1;
   const char* fk = getenv("COSAHIP_FUSED_UPDATE");
   s->fuse_update = !(fk && strcmp(fk, "0") == 0);
   const char* gk = getenv("COSAHIP_GRAPH");
@@ -363,8 +364,9 @@ static void add_tiles(std::vector<Tile>& v, int block, ll items, int per) {
 
 // k_cirs_il (CIRS i-sweep, constant coefficient of irsop 1): how many whole
 // lines of all npde variables one work-group keeps in LDS, within
-// s->il_lds bytes (two tables, lines, per-row offsets; default 40 KB, so that
-// four work-groups share a CU; at most 64 KB).
+// s->il_lds bytes (two tables, lines, per-row offsets; default 24 KB, so that
+// six work-groups share a CU), raised to what one line needs for long lines
+// (at most 64 KB).
 
 static void plan_cirs_il(Solve* s, Level& L) {
   L.il_lpw = 0;
@@ -374,7 +376,8 @@ static void plan_cirs_il(Solve* s, Level& L) {
   for (const Blk& B : L.hblk) lmax = std::max(lmax, B.imax - 1);
   const int S = lmax | 1;
   const int nv = s->par.npde;
-  const int lpw = std::min((s->il_lds - 16 * S) / (nv * (8 * S + 16)), 256 / nv);
+  const int budget = std::max(s->il_lds, std::min(65536, 16 * S + nv * (8 * S + 16)));
+  const int lpw = std::min((budget - 16 * S) / (nv * (8 * S + 16)), 256 / nv);
   if (lpw < 1) return;
   L.il_lpw = lpw;
   L.il_stride = S;
@@ -467,8 +470,9 @@ static int build_tables(Solve* s, Level& L, const Level* Lc) {
     add_tiles(L.htile[TK_SHELL], b, ns * (2 * (ll)(B.jmax + 1) * (B.kmax + 1) + im * (2 * (ll)(B.kmax + 1) + 2 * jm)),
               TS);
     add_tiles(L.htile[TK_HALO], b, ns * ((ll)(B.imax + 3) * jm * km + 4 * im * km + 4 * im * jm), TS);
-    add_tiles(L.htile[TK_FUSED], b, residual_tile_items(7, B.imax, B.jmax, B.kmax, ns), 1);
-    add_tiles(L.htile[TK_FUSED15], b, residual_tile_items(15, B.imax, B.jmax, B.kmax, ns), 1);
+    add_tiles(L.htile[TK_FUSED], b, residual_tile_items(RT_TI, 7, B.imax, B.jmax, B.kmax, ns), 1);
+    add_tiles(L.htile[TK_FUSED15], b, residual_tile_items(RT_TI, 15, B.imax, B.jmax, B.kmax, ns), 1);
+    add_tiles(L.htile[TK_T32], b, residual_tile_items(32, 7, B.imax, B.jmax, B.kmax, ns), 1);
     if (Lc) {
       const Blk& C = Lc->hblk[b];
       add_tiles(L.htile[TK_COARSE], b, ns * (ll)(C.imax - 1) * (C.jmax - 1) * (C.kmax - 1), TS);
@@ -1941,7 +1945,13 @@ static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter
         TimeScope t(*s, COSAHIP_FAM_RESIDUAL, cs);
         CK(launch_residual(*s, L, q, L.f[COSAHIP_RES]));
       }
-      {
+      // prim2cons_b; with the update folded into the CIRS sweep of a stage
+      // that leaves q primitive, nothing reads the conservative state before
+      // the sweep rewrites every interior cell from qold (and the next
+      // stage's BC fill rewrites the face halos), unless the HB or dual-time
+      // source takes it
+      const bool skip_p2c = fuse_update && irk < c.nstage && iter <= niter && !c.harbal && !s->par.dualt;
+      if (!skip_p2c) {
         TimeScope t(*s, COSAHIP_FAM_CONVERT, cs);
         CK(launch_prim2cons(*s, L, q));
       }

@@ -102,6 +102,7 @@ enum TileKind {
   TK_FACES_K,     // k faces (imax-1) x (jmax-1) x kmax
   TK_FUSED15,     // residual tiles of 31 x 15 cells x 32-plane k segments (k_residual_tile, TJ = 15)
   TK_SHELL,       // the halo shell of 0..imax x 0..jmax x 0..kmax (the box without the interior cells)
+  TK_T32,         // residual tiles of 32 x 7 cells x 32-plane k segments (RESID_TILE32_W4, RESID_SVA32)
   TK_NKIND
 };
 
@@ -116,6 +117,8 @@ enum ResidVariant {
   RESID_SVA7,          // k_residual_sva: van Albada slope once per cell, 31 x 7 tiles (else as RESID_TILE7)
   RESID_SVA15,         // k_residual_sva, 31 x 15 tiles (else as RESID_TILE15)
   RESID_SVA7_W4,       // RESID_SVA7 held to 128 VGPRs (4 waves per SIMD)
+  RESID_TILE32_W4,     // RESID_TILE7_W4 on 32 x 7 tiles (the 33 i-faces of a row spread over the work-group)
+  RESID_SVA32,         // RESID_SVA7 on 32 x 7 tiles (else as RESID_TILE32_W4)
   RESID_NVARIANT
 };
 
@@ -298,7 +301,7 @@ struct Solve {
   // CIRS_IV_MAX_LINES lines along i, one thread per line with LDS-staged
   // chunks on larger ones; COSAHIP_CIRS_I=lds|iv at create time forces one
   int cirs_i_mode = 0;   // 0 auto, 1 lds, 2 iv, 3 line
-  int il_lds = 40960;    // LDS bytes per work-group of the line-resident CIRS i-sweep
+  int il_lds = 24576;    // LDS bytes per work-group of the line-resident CIRS i-sweep
   // residual precision: 0 FP64, 1 FP32 residual / FP64 state, 2 FP32 until
   // the level-1 density residual rms falls below prec_tail_rms, FP64 after,
   // 3 FP64 with multiply-add contraction (Euler residual)

@@ -804,8 +804,19 @@ __global__ void __launch_bounds__(256) k_bc(Par p, const Blk* __restrict__ blks,
   const int loc = item - n * rec.count;
   const int idir = rec.t[1], inrout = rec.t[2];
   const int ic1 = cycd(idir, 1), ic2 = cycd(idir, 2), ic3 = cycd(idir, 3);
-  const int i2_ = rec.t[2 * ic2 + 1] + loc % rec.n2;
-  const int i3_ = rec.t[2 * ic3 + 1] + loc / rec.n2;
+  // every face point is set from the interior alone, so the order of the
+  // points over the threads is free: i varies fastest wherever it is a
+  // transverse index (j faces cycle to (k, i), whose k-fastest order would
+  // put consecutive lanes a plane apart)
+  int i2_, i3_;
+  if (ic3 == 1) {
+    const int n3e = rec.count / rec.n2;
+    i3_ = rec.t[2 * ic3 + 1] + loc % n3e;
+    i2_ = rec.t[2 * ic2 + 1] + loc / n3e;
+  } else {
+    i2_ = rec.t[2 * ic2 + 1] + loc % rec.n2;
+    i3_ = rec.t[2 * ic3 + 1] + loc / rec.n2;
+  }
   const int ijk[4] = {0, b.imax, b.jmax, b.kmax};
   int ibcpt, ibcpt2, ibcn, ibcn2, ibcm;
   double sgnm;

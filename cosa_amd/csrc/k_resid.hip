@@ -41,7 +41,7 @@ namespace contracted {
 namespace exact {
 #endif
 
-template <typename T, int TJ, int ILIM, int FO, int EF, int KW, int MINW>
+template <typename T, int TI, int TJ, int ILIM, int FO, int EF, int KW, int MINW>
 __global__ void __launch_bounds__(32 * (TJ + 1), MINW) k_residual_tile(Par p, int nl, const Blk* __restrict__ blks,
                                                                  const Tile* __restrict__ tiles,
                                                                  const T* __restrict__ q, const T* __restrict__ si,
@@ -49,7 +49,8 @@ __global__ void __launch_bounds__(32 * (TJ + 1), MINW) k_residual_tile(Par p, in
                                                                  double* __restrict__ res,
                                                                  const unsigned char* __restrict__ sbc,
                                                                  const ll* __restrict__ sbcoff) {
-  constexpr int TI = RT_TI, NT = 32 * (TJ + 1), PW = TI + 4, PH = TJ + 4;
+  constexpr int NT = 32 * (TJ + 1), PW = TI + 4, PH = TJ + 4;
+  static_assert(TI == 31 || TI == 32, "tiles are 31 or 32 cells wide");
   __shared__ T pl[5][PH][PW];
   __shared__ T fi[5][TJ][TI + 1];
   __shared__ T fj[5][TJ + 1][TI];
@@ -77,6 +78,10 @@ __global__ void __launch_bounds__(32 * (TJ + 1), MINW) k_residual_tile(Par p, in
   const ll offK = offJ + 2 * plJ, plK = (ll)(b.imax + 1) * (b.jmax + 1);
   const int ci = i0 + tx, cj = j0 + ty;
   const bool cell = tx < TI && ty < TJ && ci <= im && cj <= jm;
+  // i-face thread: face i0 + fc of row j0 + fr, TI + 1 faces per row (for
+  // 31-cell tiles the lane (tx, ty) itself)
+  const int fr = threadIdx.x / (TI + 1), fc = threadIdx.x - fr * (TI + 1);
+  const int fa = i0 + fc, fj0 = j0 + fr;
   const ll kstep = (ll)(b.imax + 3) * (b.jmax + 3);
   T w0[KW ? 5 : 1], w1[KW ? 5 : 1], w2[KW ? 5 : 1], w3[KW ? 5 : 1];
   T Fkp[5];
@@ -129,23 +134,23 @@ __global__ void __launch_bounds__(32 * (TJ + 1), MINW) k_residual_tile(Par p, in
     }
     __syncthreads();
     // (b) face fluxes
-    if (ty < TJ && cj <= jm && ci <= b.imax) {   // i-face a = ci of row cj
-      const int a = ci;
-      const bool fin = fl[cj + (ll)(b.jmax + 1) * k] != 0;
-      const bool fout = fl[plI + cj + (ll)(b.jmax + 1) * k] != 0;
+    if (fr < TJ && fj0 <= jm && fa <= b.imax) {   // i-face a = fa of row fj0
+      const int a = fa;
+      const bool fin = fl[fj0 + (ll)(b.jmax + 1) * k] != 0;
+      const bool fout = fl[plI + fj0 + (ll)(b.jmax + 1) * k] != 0;
       T m2[5], m1[5], z0[5], p1[5], QR[5], QL[5], F[5], kx, ky, kz, ds, ubn;
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        m2[v] = pl[v][ty + 2][tx];
-        m1[v] = pl[v][ty + 2][tx + 1];
-        z0[v] = pl[v][ty + 2][tx + 2];
-        p1[v] = pl[v][ty + 2][tx + 3];
+        m2[v] = pl[v][fr + 2][fc];
+        m1[v] = pl[v][fr + 2][fc + 1];
+        z0[v] = pl[v][fr + 2][fc + 2];
+        p1[v] = pl[v][fr + 2][fc + 3];
       }
       muscl_face<5, T, ILIM, FO>(p, nl, a, b.imax, fin, fout, m2, m1, z0, p1, QR, QL);
-      face_metric<T>(p, si + mo + i2(b, a, cj, k) * p.lmet, kx, ky, kz, ds, ubn);
+      face_metric<T>(p, si + mo + i2(b, a, fj0, k) * p.lmet, kx, ky, kz, ds, ubn);
       roe_face<5, T, EF>(p, QR, QL, kx, ky, kz, ds, ubn, F);
 #pragma unroll
-      for (int v = 0; v < 5; v++) fi[v][ty][tx] = F[v];
+      for (int v = 0; v < 5; v++) fi[v][fr][fc] = F[v];
     }
     if (tx < TI && ci <= im && cj <= b.jmax) {   // j-face a = cj of column ci
       const int a = cj;
@@ -227,7 +232,7 @@ __global__ void __launch_bounds__(32 * (TJ + 1), MINW) k_residual_tile(Par p, in
 //       column from one plane to the next;
 //   (d) the face fluxes replace the slopes in the same LDS (one more
 //       barrier) and the cell sums them in the reference order.
-template <typename T, int TJ, int MINW>
+template <typename T, int TI, int TJ, int MINW>
 __global__ void __launch_bounds__(32 * (TJ + 1), MINW) k_residual_sva(Par p, int nl, const Blk* __restrict__ blks,
                                                                 const Tile* __restrict__ tiles,
                                                                 const T* __restrict__ q, const T* __restrict__ si,
@@ -235,7 +240,8 @@ __global__ void __launch_bounds__(32 * (TJ + 1), MINW) k_residual_sva(Par p, int
                                                                 double* __restrict__ res,
                                                                 const unsigned char* __restrict__ sbc,
                                                                 const ll* __restrict__ sbcoff) {
-  constexpr int TI = RT_TI, NT = 32 * (TJ + 1), PW = TI + 4, PH = TJ + 4;
+  constexpr int NT = 32 * (TJ + 1), PW = TI + 4, PH = TJ + 4;
+  static_assert(TI == 31 || TI == 32, "tiles are 31 or 32 cells wide");
   constexpr int NI = TJ * (TI + 2), NJ = (TJ + 2) * TI;               // slope cells along i / along j
   constexpr int NF = 5 * TJ * (TI + 1) + 5 * (TJ + 1) * TI, NH = 5 * (NI + NJ);
   __shared__ T pl[5][PH][PW];
@@ -268,6 +274,10 @@ __global__ void __launch_bounds__(32 * (TJ + 1), MINW) k_residual_sva(Par p, int
   const ll offK = offJ + 2 * plJ, plK = (ll)(b.imax + 1) * (b.jmax + 1);
   const int ci = i0 + tx, cj = j0 + ty;
   const bool cell = tx < TI && ty < TJ && ci <= im && cj <= jm;
+  // i-face thread: face i0 + fc of row j0 + fr, TI + 1 faces per row (for
+  // 31-cell tiles the lane (tx, ty) itself)
+  const int fr = threadIdx.x / (TI + 1), fc = threadIdx.x - fr * (TI + 1);
+  const int fa = i0 + fc, fj0 = j0 + fr;
   const ll kstep = (ll)(b.imax + 3) * (b.jmax + 3);
   const T epsq = (T)p.epslimf * (T)p.epslimf;
   T Fkp[5], hk[5];
@@ -354,22 +364,22 @@ __global__ void __launch_bounds__(32 * (TJ + 1), MINW) k_residual_sva(Par p, int
     __syncthreads();
     // (c) face fluxes into registers
     T Fi[5], Fj[5], Fkn[5];
-    const bool iface = ty < TJ && cj <= jm && ci <= b.imax;
+    const bool iface = fr < TJ && fj0 <= jm && fa <= b.imax;
     const bool jface = tx < TI && ci <= im && cj <= b.jmax;
-    if (iface) {   // i-face a = ci of row cj
-      const int a = ci;
-      const bool fin = fl[cj + (ll)(b.jmax + 1) * k] != 0;
-      const bool fout = fl[plI + cj + (ll)(b.jmax + 1) * k] != 0;
+    if (iface) {   // i-face a = fa of row fj0
+      const int a = fa;
+      const bool fin = fl[fj0 + (ll)(b.jmax + 1) * k] != 0;
+      const bool fout = fl[plI + fj0 + (ll)(b.jmax + 1) * k] != 0;
       T QR[5], QL[5], kx, ky, kz, ds, ubn;
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        const T qa = pl[v][ty + 2][tx + 2], qm = pl[v][ty + 2][tx + 1];
-        QR[v] = qa - hi[v * NI + ty * (TI + 2) + tx + 1];
-        QL[v] = qm + hi[v * NI + ty * (TI + 2) + tx];
+        const T qa = pl[v][fr + 2][fc + 2], qm = pl[v][fr + 2][fc + 1];
+        QR[v] = qa - hi[v * NI + fr * (TI + 2) + fc + 1];
+        QL[v] = qm + hi[v * NI + fr * (TI + 2) + fc];
         if (a == 1 && fin) QR[v] = QL[v] = qm;
         if (a == b.imax && fout) QR[v] = QL[v] = qa;
       }
-      face_metric<T>(p, si + mo + i2(b, a, cj, k) * p.lmet, kx, ky, kz, ds, ubn);
+      face_metric<T>(p, si + mo + i2(b, a, fj0, k) * p.lmet, kx, ky, kz, ds, ubn);
       roe_face<5, T, 0>(p, QR, QL, kx, ky, kz, ds, ubn, Fi);
     }
     if (jface) {   // j-face a = cj of column ci
@@ -414,7 +424,7 @@ __global__ void __launch_bounds__(32 * (TJ + 1), MINW) k_residual_sva(Par p, int
     __syncthreads();   // every slope read: the fluxes take their place
     if (iface) {
 #pragma unroll
-      for (int v = 0; v < 5; v++) fi[v][ty][tx] = Fi[v];
+      for (int v = 0; v < 5; v++) fi[v][fr][fc] = Fi[v];
     }
     if (jface) {
 #pragma unroll
@@ -436,17 +446,17 @@ __global__ void __launch_bounds__(32 * (TJ + 1), MINW) k_residual_sva(Par p, int
   }
 }
 
-template <typename T, int TJ, int ILIM, int FO, int EF, int KW, int MINW = 1>
+template <typename T, int TJ, int ILIM, int FO, int EF, int KW, int MINW = 1, int TI = 31>
 static void launch_tile(Solve& s, Level& L, int ntile, const Tile* tiles, const T* q, const T* si, const T* sj,
                         const T* sk, double* res) {
-  hipLaunchKernelGGL((k_residual_tile<T, TJ, ILIM, FO, EF, KW, MINW>), dim3(ntile), dim3(32 * (TJ + 1)), 0,
+  hipLaunchKernelGGL((k_residual_tile<T, TI, TJ, ILIM, FO, EF, KW, MINW>), dim3(ntile), dim3(32 * (TJ + 1)), 0,
                      s.stream, s.par, L.nl, L.dblk, tiles, q, si, sj, sk, res, L.sbc, L.sbc_doff);
 }
 
-template <typename T, int TJ, int MINW = 1>
+template <typename T, int TJ, int MINW = 1, int TI = 31>
 static void launch_sva(Solve& s, Level& L, int ntile, const Tile* tiles, const T* q, const T* si, const T* sj,
                        const T* sk, double* res) {
-  hipLaunchKernelGGL((k_residual_sva<T, TJ, MINW>), dim3(ntile), dim3(32 * (TJ + 1)), 0, s.stream, s.par, L.nl,
+  hipLaunchKernelGGL((k_residual_sva<T, TI, TJ, MINW>), dim3(ntile), dim3(32 * (TJ + 1)), 0, s.stream, s.par, L.nl,
                      L.dblk, tiles, q, si, sj, sk, res, L.sbc, L.sbc_doff);
 }
 
@@ -456,7 +466,8 @@ static void launch_sva(Solve& s, Level& L, int ntile, const Tile* tiles, const T
 template <typename T>
 static bool dispatch(Solve& s, Level& L, int variant, const T* q, const T* si, const T* sj, const T* sk, double* res) {
   const Par& p = s.par;
-  const int tk = variant == RESID_TILE15 || variant == RESID_SVA15 ? TK_FUSED15 : TK_FUSED;
+  const bool w32 = variant == RESID_TILE32_W4 || variant == RESID_SVA32;
+  const int tk = w32 ? TK_T32 : (variant == RESID_TILE15 || variant == RESID_SVA15 ? TK_FUSED15 : TK_FUSED);
   const int nt = L.ntile[tk];
   if (!nt) return true;
   const Tile* tl = L.tile[tk];
@@ -474,6 +485,14 @@ static bool dispatch(Solve& s, Level& L, int variant, const T* q, const T* si, c
     if (!plain) launch_tile<T, 7, -1, -1, -1, 0, 4>(s, L, nt, tl, q, si, sj, sk, res);
     else if (fo) launch_tile<T, 7, 4, 1, 0, 0, 4>(s, L, nt, tl, q, si, sj, sk, res);
     else launch_tile<T, 7, 4, 0, 0, 0, 4>(s, L, nt, tl, q, si, sj, sk, res);
+  } else if (variant == RESID_TILE32_W4) {
+    if (!plain) launch_tile<T, 7, -1, -1, -1, 0, 4, 32>(s, L, nt, tl, q, si, sj, sk, res);
+    else if (fo) launch_tile<T, 7, 4, 1, 0, 0, 4, 32>(s, L, nt, tl, q, si, sj, sk, res);
+    else launch_tile<T, 7, 4, 0, 0, 0, 4, 32>(s, L, nt, tl, q, si, sj, sk, res);
+  } else if (variant == RESID_SVA32) {
+    if (plain && !fo) launch_sva<T, 7, 1, 32>(s, L, nt, tl, q, si, sj, sk, res);
+    else if (!plain) launch_tile<T, 7, -1, -1, -1, 0, 4, 32>(s, L, nt, tl, q, si, sj, sk, res);
+    else launch_tile<T, 7, 4, 1, 0, 0, 4, 32>(s, L, nt, tl, q, si, sj, sk, res);
   } else if (variant == RESID_SVA7 || variant == RESID_SVA15 || variant == RESID_SVA7_W4) {
     const bool t15 = variant == RESID_SVA15;
     if (plain && !fo) {
@@ -506,11 +525,11 @@ hipError_t launch_residual_tile_contracted(Solve& s, Level& L, int variant, cons
   return hipGetLastError();
 }
 #else
-// Work items of one block for a TJ-row tile table: (i tiles) x (j tiles) x
+// Work items of one block for a TI x TJ tile table: (i tiles) x (j tiles) x
 // (k segments) x snapshots, i tiles fastest.
-ll residual_tile_items(int TJ, int imax, int jmax, int kmax, int nsnap) {
+ll residual_tile_items(int TI, int TJ, int imax, int jmax, int kmax, int nsnap) {
   const ll im = imax - 1, jm = jmax - 1, km = kmax - 1;
-  return (ll)nsnap * ((im + RT_TI - 1) / RT_TI) * ((jm + TJ - 1) / TJ) * ((km + RT_KSEG - 1) / RT_KSEG);
+  return (ll)nsnap * ((im + TI - 1) / TI) * ((jm + TJ - 1) / TJ) * ((km + RT_KSEG - 1) / RT_KSEG);
 }
 
 hipError_t launch_residual_tile(Solve& s, Level& L, int variant, const double* q, const double* si,

@@ -25,7 +25,7 @@
 namespace cosahip {
 
 template <typename T, int NV, bool VIS, int TJ, int ILIM, int FO, int EF>
-__global__ void __launch_bounds__(32 * (TJ + 1)) k_residual_tile_v(
+__global__ void __launch_bounds__(32 * (TJ + 1), 2) k_residual_tile_v(
     Par p, int nl, const Blk* __restrict__ blks, const Tile* __restrict__ tiles, const T* __restrict__ q,
     const T* __restrict__ si, const T* __restrict__ sj, const T* __restrict__ sk, double* __restrict__ res,
     const unsigned char* __restrict__ sbc, const ll* __restrict__ sbcoff, const double* __restrict__ h0tab,
@@ -73,25 +73,35 @@ __global__ void __launch_bounds__(32 * (TJ + 1)) k_residual_tile_v(
   // in slot d - 1
   const ll fst = (ll)b.imax * b.jmax * b.kmax;
   const T* vb = VIS ? vf + b.o0 * p.npde * p.nsnap + fst * (ll)p.npde * n : nullptr;
-  T Fkp[NV];
+  // the cell thread's column: q(k-1 .. k+2) in registers (each value loaded
+  // once, as q(k+2)), the k-face flux and (VIS) the viscous k-face flux of
+  // face k carried from one plane to the next
+  T Fkp[NV], Vkp[VIS ? NV : 1], w0[NV], w1[NV], w2[NV], w3[NV];
   bool kfin = false, kfout = false;
   if (cell) {   // k-face ks
     kfin = fl[offK + ci + (ll)(b.imax + 1) * cj] != 0;
     kfout = fl[offK + plK + ci + (ll)(b.imax + 1) * cj] != 0;
     const ll c0 = i3(b, ci, cj, ks);
-    T a_m2[NV], a_m1[NV], a_0[NV], a_p1[NV], QR[NV], QL[NV], kx, ky, kz, ds, ubn;
+    T a_m2[NV], QR[NV], QL[NV], kx, ky, kz, ds, ubn;
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       const T* qv = qb + n3 * v;
       a_m2[v] = qv[c0 - 2 * kstep];
-      a_m1[v] = qv[c0 - kstep];
-      a_0[v] = qv[c0];
-      a_p1[v] = qv[c0 + kstep];
+      w0[v] = qv[c0 - kstep];
+      w1[v] = qv[c0];
+      w2[v] = qv[c0 + kstep];
     }
-    muscl_face<NV, T, ILIM, FO>(p, nl, ks, b.kmax, kfin, kfout, a_m2, a_m1, a_0, a_p1, QR, QL);
+    muscl_face<NV, T, ILIM, FO>(p, nl, ks, b.kmax, kfin, kfout, a_m2, w0, w1, w2, QR, QL);
     face_metric<T>(p, sk + mo + i2(b, ci, cj, ks) * p.lmet, kx, ky, kz, ds, ubn);
     roe_face<NV, T, EF>(p, QR, QL, kx, ky, kz, ds, ubn, Fkp);
     if constexpr (NV > 5) roe_turb<T>(p, QR, QL, kx, ky, kz, ds, ubn, h0[2][0], h0[2][1], Fkp);
+#pragma unroll
+    for (int v = 0; v < NV; v++) w3[v] = qb[n3 * v + c0 + 2 * kstep];
+    if constexpr (VIS) {
+      const ll f = (ll)(ci - 1) + (ll)b.imax * ((ll)(cj - 1) + (ll)b.jmax * (ll)(ks - 1));
+#pragma unroll
+      for (int v = 0; v < NV; v++) Vkp[v] = vb[2 * vslot + fst * v + f];
+    }
   }
   for (int k = ks; k <= ke; k++) {
     // (a) stage plane k with its ring (no corners)
@@ -100,10 +110,15 @@ __global__ void __launch_bounds__(32 * (TJ + 1)) k_residual_tile_v(
       const bool in_i = ii >= 2 && ii <= TI + 1, in_j = jj >= 2 && jj <= TJ + 1;
       if (!in_i && !in_j) continue;   // corner: not used
       const int i = i0 - 2 + ii, j = j0 - 2 + jj;
-      if (i > b.imax + 1 || j > b.jmax + 1) continue;   // beyond the block
+      if (in_i && in_j && i <= im && j <= jm) continue;   // deposited by its cell thread
+      if (i > b.imax + 1 || j > b.jmax + 1) continue;      // beyond the block
       const ll x = i3(b, i, j, k);
 #pragma unroll
       for (int v = 0; v < NV; v++) pl[v][jj][ii] = qb[n3 * v + x];
+    }
+    if (cell) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) pl[v][ty + 2][tx + 2] = w1[v];
     }
     __syncthreads();
     // (b) face fluxes: i-face ci of row cj, j-face cj of column ci, and the
@@ -148,17 +163,8 @@ __global__ void __launch_bounds__(32 * (TJ + 1)) k_residual_tile_v(
     }
     T Fkn[NV];
     if (cell) {
-      const ll x = i3(b, ci, cj, k);
-      T a_m1[NV], a_0[NV], a_p1[NV], a_p2[NV], QR[NV], QL[NV], kx, ky, kz, ds, ubn;
-#pragma unroll
-      for (int v = 0; v < NV; v++) {
-        const T* qv = qb + n3 * v;
-        a_m1[v] = qv[x - kstep];
-        a_0[v] = pl[v][ty + 2][tx + 2];
-        a_p1[v] = qv[x + kstep];
-        a_p2[v] = qv[x + 2 * kstep];
-      }
-      muscl_face<NV, T, ILIM, FO>(p, nl, k + 1, b.kmax, kfin, kfout, a_m1, a_0, a_p1, a_p2, QR, QL);
+      T QR[NV], QL[NV], kx, ky, kz, ds, ubn;
+      muscl_face<NV, T, ILIM, FO>(p, nl, k + 1, b.kmax, kfin, kfout, w0, w1, w2, w3, QR, QL);
       face_metric<T>(p, sk + mo + i2(b, ci, cj, k + 1) * p.lmet, kx, ky, kz, ds, ubn);
       roe_face<NV, T, EF>(p, QR, QL, kx, ky, kz, ds, ubn, Fkn);
       if constexpr (NV > 5) roe_turb<T>(p, QR, QL, kx, ky, kz, ds, ubn, h0[2][0], h0[2][1], Fkn);
@@ -182,11 +188,21 @@ __global__ void __launch_bounds__(32 * (TJ + 1)) k_residual_tile_v(
         }
         r = (r + (double)Fkn[v]) - (double)Fkp[v];
         if constexpr (VIS) {
-          const T* v3 = vb + 2 * vslot + fst * v;
-          r = (r + (double)v3[f + (ll)b.imax * b.jmax]) - (double)v3[f];
+          const T vkn = vb[2 * vslot + fst * v + f + (ll)b.imax * b.jmax];
+          r = (r + (double)vkn) - (double)Vkp[v];
+          Vkp[v] = vkn;
         }
         rb[n3 * v + x] = r;
         Fkp[v] = Fkn[v];
+      }
+      if (k < ke) {   // slide the column window: q(k + 3) is the next plane's q(k + 2)
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          w0[v] = w1[v];
+          w1[v] = w2[v];
+          w2[v] = w3[v];
+          w3[v] = qb[n3 * v + x + 3 * kstep];
+        }
       }
     }
   }

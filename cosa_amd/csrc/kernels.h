@@ -63,11 +63,13 @@ hipError_t launch_residual_tile(Solve& s, Level& L, int variant, const float* q,
 // k_resid_contracted.hip); RESID_GENERIC runs as RESID_TILE7 there
 hipError_t launch_residual_tile_contracted(Solve& s, Level& L, int variant, const double* q, const double* si,
                                            const double* sj, const double* sk, double* res);
-// work items of one block in the tile table of a TJ-row residual tile
-// (tiles of RT_TI x TJ cells, k segments of RT_KSEG planes)
-constexpr int RT_TI = 31;    // cells per tile along i (32 lanes: one more face than cells)
+// work items of one block in the tile table of a TI x TJ residual tile
+// (k segments of RT_KSEG planes); 31-cell rows put one face per lane of a
+// 32-lane row, 32-cell rows (TK_T32) spread the 33 faces of a row over the
+// work-group
+constexpr int RT_TI = 31;    // cells per tile along i of TK_FUSED / TK_FUSED15
 constexpr int RT_KSEG = 32;  // k-planes per work-group
-ll residual_tile_items(int TJ, int imax, int jmax, int kmax, int nsnap);
+ll residual_tile_items(int TI, int TJ, int imax, int jmax, int kmax, int nsnap);
 // the RK update fused into the last CIRS sweep (mean flow): q = qold - ar
 // res dtvol per cell, then (prim) the next stage's cons2prim; q32: the
 // FP32 shadow of the primitive state, or null

@@ -304,7 +304,7 @@ class Solve:
         names = {0: None, 1: "lds", 2: "per_variable", 3: "line_resident"}
         rnames = {-1: None, 0: "generic_fused", 1: "tile31x7", 2: "tile31x7_register_window", 3: "tile31x15",
                   4: "tile31x7_128vgpr", 5: "tile31x7_shared_slopes", 6: "tile31x15_shared_slopes",
-                  7: "tile31x7_shared_slopes_128vgpr"}
+                  7: "tile31x7_shared_slopes_128vgpr", 8: "tile32x7_128vgpr", 9: "tile32x7_shared_slopes"}
         vnames = {0: "march", 1: "stored_faces", 2: "tile_pass"}
         return {"cirs_i_sweep": [names[ch[m]] for m in range(3)], "viscous": vnames[ch[3]],
                 "residual": [rnames.get(ch[4 + m]) for m in range(3)]}

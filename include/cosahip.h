@@ -345,14 +345,14 @@ int cosahip_cycle_ready(cosahip_solve* s);
  * tiles with the k-window from LDS/L2, 2 the same with a register
  * k-window, 3 31x15 tiles, 4 as 1 held to 128 VGPRs, 5 / 6 31x7 / 31x15
  * tiles with the van Albada slope evaluated once per cell, 7 as 5 held to
- * 128 VGPRs; -1 level absent);
+ * 128 VGPRs, 8 as 4 on 32x7 tiles, 9 as 5 on 32x7 tiles; -1 level absent);
  * choice[7] is reserved (0).  choice holds 8 entries.
  * Environment switches at create time take precedence:
- * COSAHIP_RESIDUAL=fused|tile7|tile7reg|tile15|tile7w4|sva7|sva15|sva7w4|split
+ * COSAHIP_RESIDUAL=fused|tile7|tile7reg|tile15|tile7w4|sva7|sva15|sva7w4|tile32w4|sva32|split
  * forces the
  * inviscid residual kernel, COSAHIP_CIRS_I=lds|iv|line forces the i-sweep, COSAHIP_CIRS_IL_LDS sets
  * the LDS bytes per work-group of the line-resident sweep (4096..65536,
- * default 40960), COSAHIP_VISCOUS=march|fused|tile the viscous arrangement. */
+ * default 24576), COSAHIP_VISCOUS=march|fused|tile the viscous arrangement. */
 int cosahip_autotune(cosahip_solve* s, int reps, int32_t* choice);
 
 /* ---- blocks as independent solves (no counterpart in the reference) -----

@@ -17,7 +17,7 @@ from parity_util import interior_mask, make_pair, parity
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = ["fused", "tile7", "tile7reg", "tile15", "tile7w4", "sva7", "sva15", "sva7w4"]
+VARIANTS = ["fused", "tile7", "tile7reg", "tile15", "tile7w4", "sva7", "sva15", "sva7w4", "tile32w4", "sva32"]
 
 
 def _cases():
