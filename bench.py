@@ -75,12 +75,14 @@ BYTES_PER_CELL_RANS = {  # Wilcox k-omega, npde = 7
     "lambda": 176.0,     # q 5x8 R + metrics 96 R + vol 8 + mut 8 R + dtvol, dtvolt 16 W + ... betas
     "other": None,
 }
-# HBM bytes per cell-update of the residual pass measured with rocprofv3 --pmc
-# FETCH_SIZE / WRITE_SIZE (scaled by a plain-copy calibration: FETCH x2.00,
-# WRITE x1.00) on the level-1 kernel: profiles/residual_uneven96_fp64_hbm_traffic.md
+# HBM bytes per cell-update of the residual pass and of the three CIRS sweeps
+# measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a
+# plain-copy calibration: FETCH x2.00, WRITE x1.00) on the level-1 kernels:
+# profiles/residual_uneven96_fp64_hbm_traffic.md
 # and profiles/residual_wing4_rans_fp64_hbm_traffic.md.  Re-measure when the
 # kernels change.
-MEASURED_TRAFFIC_PER_CELL = {("uneven96", "fp64", "residual"): 324.0, ("wing4", "fp64", "residual"): 2290.0}
+MEASURED_TRAFFIC_PER_CELL = {("uneven96", "fp64", "residual"): 312.7, ("uneven96", "fp64", "cirs"): 501.8,
+                             ("wing4", "fp64", "residual"): 2112.1}
 
 
 def log(msg):

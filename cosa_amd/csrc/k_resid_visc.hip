@@ -9,9 +9,10 @@
 //   res = ((((((((((0 + Fi(c+1)) - Fi(c)) + Vi(c+1)) - Vi(c)) + Fj(c+1)) - Fj(c))
 //         + Vj(c+1)) - Vj(c)) + Fk(c+1)) - Fk(c)) + Vk(c+1)) - Vk(c)
 // in FP64.  k_vflux stores the viscous face fluxes V of each direction in a
-// slot of its own; this kernel forms the inviscid face fluxes F of a 31 x TJ
+// slot of its own; this kernel forms the inviscid face fluxes F of a TI x TJ
 // tile of cells plane by plane (q(k) staged in LDS with its 2-cell ring, the
-// i and j faces from LDS by the tile's lanes, the k face by the cell's own
+// i and j faces from LDS by the tile's threads -- the TI + 1 faces of a row
+// spread over the work-group --, the k face by the cell's own
 // thread from q(k-1 .. k+2)), with the k-omega transport of roflux_b
 // (roe_turb, fed the block's left-over h0l/h0r of the direction from h0tab),
 // and sums F and V in the order above.  The per-direction kernels (k_flux_i,
@@ -24,13 +25,14 @@
 
 namespace cosahip {
 
-template <typename T, int NV, bool VIS, int TJ, int ILIM, int FO, int EF>
-__global__ void __launch_bounds__(32 * (TJ + 1), 2) k_residual_tile_v(
+template <typename T, int NV, bool VIS, int TI, int TJ, int ILIM, int FO, int EF>
+__global__ void __launch_bounds__(32 * (TJ + 1)) k_residual_tile_v(
     Par p, int nl, const Blk* __restrict__ blks, const Tile* __restrict__ tiles, const T* __restrict__ q,
     const T* __restrict__ si, const T* __restrict__ sj, const T* __restrict__ sk, double* __restrict__ res,
     const unsigned char* __restrict__ sbc, const ll* __restrict__ sbcoff, const double* __restrict__ h0tab,
     const T* __restrict__ vf, ll vslot) {
-  constexpr int TI = RT_TI, NT = 32 * (TJ + 1), PW = TI + 4, PH = TJ + 4;
+  constexpr int NT = 32 * (TJ + 1), PW = TI + 4, PH = TJ + 4;
+  static_assert(TI == 31 || TI == 32, "tiles are 31 or 32 cells wide");
   __shared__ T pl[NV][PH][PW];
   __shared__ T fi[NV][TJ][TI + 1];
   __shared__ T fj[NV][TJ + 1][TI];
@@ -58,6 +60,9 @@ __global__ void __launch_bounds__(32 * (TJ + 1), 2) k_residual_tile_v(
   const ll offK = offJ + 2 * plJ, plK = (ll)(b.imax + 1) * (b.jmax + 1);
   const int ci = i0 + tx, cj = j0 + ty;
   const bool cell = tx < TI && ty < TJ && ci <= im && cj <= jm;
+  // i-face thread: face i0 + fc of row j0 + fr, TI + 1 faces per row
+  const int fr = threadIdx.x / (TI + 1), fc = threadIdx.x - fr * (TI + 1);
+  const int fa = i0 + fc, fj0 = j0 + fr;
   const ll kstep = (ll)(b.imax + 3) * (b.jmax + 3);
   // h0l / h0r the k-omega flux of each direction takes (see roe_turb)
   T h0[3][2] = {{0, 0}, {0, 0}, {0, 0}};
@@ -73,35 +78,25 @@ __global__ void __launch_bounds__(32 * (TJ + 1), 2) k_residual_tile_v(
   // in slot d - 1
   const ll fst = (ll)b.imax * b.jmax * b.kmax;
   const T* vb = VIS ? vf + b.o0 * p.npde * p.nsnap + fst * (ll)p.npde * n : nullptr;
-  // the cell thread's column: q(k-1 .. k+2) in registers (each value loaded
-  // once, as q(k+2)), the k-face flux and (VIS) the viscous k-face flux of
-  // face k carried from one plane to the next
-  T Fkp[NV], Vkp[VIS ? NV : 1], w0[NV], w1[NV], w2[NV], w3[NV];
+  T Fkp[NV];
   bool kfin = false, kfout = false;
   if (cell) {   // k-face ks
     kfin = fl[offK + ci + (ll)(b.imax + 1) * cj] != 0;
     kfout = fl[offK + plK + ci + (ll)(b.imax + 1) * cj] != 0;
     const ll c0 = i3(b, ci, cj, ks);
-    T a_m2[NV], QR[NV], QL[NV], kx, ky, kz, ds, ubn;
+    T a_m2[NV], a_m1[NV], a_0[NV], a_p1[NV], QR[NV], QL[NV], kx, ky, kz, ds, ubn;
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       const T* qv = qb + n3 * v;
       a_m2[v] = qv[c0 - 2 * kstep];
-      w0[v] = qv[c0 - kstep];
-      w1[v] = qv[c0];
-      w2[v] = qv[c0 + kstep];
+      a_m1[v] = qv[c0 - kstep];
+      a_0[v] = qv[c0];
+      a_p1[v] = qv[c0 + kstep];
     }
-    muscl_face<NV, T, ILIM, FO>(p, nl, ks, b.kmax, kfin, kfout, a_m2, w0, w1, w2, QR, QL);
+    muscl_face<NV, T, ILIM, FO>(p, nl, ks, b.kmax, kfin, kfout, a_m2, a_m1, a_0, a_p1, QR, QL);
     face_metric<T>(p, sk + mo + i2(b, ci, cj, ks) * p.lmet, kx, ky, kz, ds, ubn);
     roe_face<NV, T, EF>(p, QR, QL, kx, ky, kz, ds, ubn, Fkp);
     if constexpr (NV > 5) roe_turb<T>(p, QR, QL, kx, ky, kz, ds, ubn, h0[2][0], h0[2][1], Fkp);
-#pragma unroll
-    for (int v = 0; v < NV; v++) w3[v] = qb[n3 * v + c0 + 2 * kstep];
-    if constexpr (VIS) {
-      const ll f = (ll)(ci - 1) + (ll)b.imax * ((ll)(cj - 1) + (ll)b.jmax * (ll)(ks - 1));
-#pragma unroll
-      for (int v = 0; v < NV; v++) Vkp[v] = vb[2 * vslot + fst * v + f];
-    }
   }
   for (int k = ks; k <= ke; k++) {
     // (a) stage plane k with its ring (no corners)
@@ -110,37 +105,32 @@ __global__ void __launch_bounds__(32 * (TJ + 1), 2) k_residual_tile_v(
       const bool in_i = ii >= 2 && ii <= TI + 1, in_j = jj >= 2 && jj <= TJ + 1;
       if (!in_i && !in_j) continue;   // corner: not used
       const int i = i0 - 2 + ii, j = j0 - 2 + jj;
-      if (in_i && in_j && i <= im && j <= jm) continue;   // deposited by its cell thread
-      if (i > b.imax + 1 || j > b.jmax + 1) continue;      // beyond the block
+      if (i > b.imax + 1 || j > b.jmax + 1) continue;   // beyond the block
       const ll x = i3(b, i, j, k);
 #pragma unroll
       for (int v = 0; v < NV; v++) pl[v][jj][ii] = qb[n3 * v + x];
     }
-    if (cell) {
-#pragma unroll
-      for (int v = 0; v < NV; v++) pl[v][ty + 2][tx + 2] = w1[v];
-    }
     __syncthreads();
     // (b) face fluxes: i-face ci of row cj, j-face cj of column ci, and the
     //     cell's k-face k + 1
-    if (ty < TJ && cj <= jm && ci <= b.imax) {
-      const int a = ci;
-      const bool fin = fl[cj + (ll)(b.jmax + 1) * k] != 0;
-      const bool fout = fl[plI + cj + (ll)(b.jmax + 1) * k] != 0;
+    if (fr < TJ && fj0 <= jm && fa <= b.imax) {
+      const int a = fa;
+      const bool fin = fl[fj0 + (ll)(b.jmax + 1) * k] != 0;
+      const bool fout = fl[plI + fj0 + (ll)(b.jmax + 1) * k] != 0;
       T m2[NV], m1[NV], z0[NV], p1[NV], QR[NV], QL[NV], F[NV], kx, ky, kz, ds, ubn;
 #pragma unroll
       for (int v = 0; v < NV; v++) {
-        m2[v] = pl[v][ty + 2][tx];
-        m1[v] = pl[v][ty + 2][tx + 1];
-        z0[v] = pl[v][ty + 2][tx + 2];
-        p1[v] = pl[v][ty + 2][tx + 3];
+        m2[v] = pl[v][fr + 2][fc];
+        m1[v] = pl[v][fr + 2][fc + 1];
+        z0[v] = pl[v][fr + 2][fc + 2];
+        p1[v] = pl[v][fr + 2][fc + 3];
       }
       muscl_face<NV, T, ILIM, FO>(p, nl, a, b.imax, fin, fout, m2, m1, z0, p1, QR, QL);
-      face_metric<T>(p, si + mo + i2(b, a, cj, k) * p.lmet, kx, ky, kz, ds, ubn);
+      face_metric<T>(p, si + mo + i2(b, a, fj0, k) * p.lmet, kx, ky, kz, ds, ubn);
       roe_face<NV, T, EF>(p, QR, QL, kx, ky, kz, ds, ubn, F);
       if constexpr (NV > 5) roe_turb<T>(p, QR, QL, kx, ky, kz, ds, ubn, h0[0][0], h0[0][1], F);
 #pragma unroll
-      for (int v = 0; v < NV; v++) fi[v][ty][tx] = F[v];
+      for (int v = 0; v < NV; v++) fi[v][fr][fc] = F[v];
     }
     if (tx < TI && ci <= im && cj <= b.jmax) {
       const int a = cj;
@@ -163,8 +153,17 @@ __global__ void __launch_bounds__(32 * (TJ + 1), 2) k_residual_tile_v(
     }
     T Fkn[NV];
     if (cell) {
-      T QR[NV], QL[NV], kx, ky, kz, ds, ubn;
-      muscl_face<NV, T, ILIM, FO>(p, nl, k + 1, b.kmax, kfin, kfout, w0, w1, w2, w3, QR, QL);
+      const ll x = i3(b, ci, cj, k);
+      T a_m1[NV], a_0[NV], a_p1[NV], a_p2[NV], QR[NV], QL[NV], kx, ky, kz, ds, ubn;
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const T* qv = qb + n3 * v;
+        a_m1[v] = qv[x - kstep];
+        a_0[v] = pl[v][ty + 2][tx + 2];
+        a_p1[v] = qv[x + kstep];
+        a_p2[v] = qv[x + 2 * kstep];
+      }
+      muscl_face<NV, T, ILIM, FO>(p, nl, k + 1, b.kmax, kfin, kfout, a_m1, a_0, a_p1, a_p2, QR, QL);
       face_metric<T>(p, sk + mo + i2(b, ci, cj, k + 1) * p.lmet, kx, ky, kz, ds, ubn);
       roe_face<NV, T, EF>(p, QR, QL, kx, ky, kz, ds, ubn, Fkn);
       if constexpr (NV > 5) roe_turb<T>(p, QR, QL, kx, ky, kz, ds, ubn, h0[2][0], h0[2][1], Fkn);
@@ -188,21 +187,11 @@ __global__ void __launch_bounds__(32 * (TJ + 1), 2) k_residual_tile_v(
         }
         r = (r + (double)Fkn[v]) - (double)Fkp[v];
         if constexpr (VIS) {
-          const T vkn = vb[2 * vslot + fst * v + f + (ll)b.imax * b.jmax];
-          r = (r + (double)vkn) - (double)Vkp[v];
-          Vkp[v] = vkn;
+          const T* v3 = vb + 2 * vslot + fst * v;
+          r = (r + (double)v3[f + (ll)b.imax * b.jmax]) - (double)v3[f];
         }
         rb[n3 * v + x] = r;
         Fkp[v] = Fkn[v];
-      }
-      if (k < ke) {   // slide the column window: q(k + 3) is the next plane's q(k + 2)
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          w0[v] = w1[v];
-          w1[v] = w2[v];
-          w2[v] = w3[v];
-          w3[v] = qb[n3 * v + x + 3 * kstep];
-        }
       }
     }
   }
@@ -211,8 +200,8 @@ __global__ void __launch_bounds__(32 * (TJ + 1), 2) k_residual_tile_v(
 template <typename T, int NV, bool VIS, int ILIM, int FO, int EF>
 static void launch_v(Solve& s, Level& L, const T* q, const T* si, const T* sj, const T* sk, double* res,
                      const T* vf, ll vslot) {
-  hipLaunchKernelGGL((k_residual_tile_v<T, NV, VIS, 7, ILIM, FO, EF>), dim3(L.ntile[TK_FUSED]), dim3(256), 0,
-                     s.stream, s.par, L.nl, L.dblk, L.tile[TK_FUSED], q, si, sj, sk, res, L.sbc, L.sbc_doff, L.h0tab,
+  hipLaunchKernelGGL((k_residual_tile_v<T, NV, VIS, 32, 7, ILIM, FO, EF>), dim3(L.ntile[TK_T32]), dim3(256), 0,
+                     s.stream, s.par, L.nl, L.dblk, L.tile[TK_T32], q, si, sj, sk, res, L.sbc, L.sbc_doff, L.h0tab,
                      vf, vslot);
 }
 
@@ -231,7 +220,7 @@ static void dispatch_v(Solve& s, Level& L, const T* q, const T* si, const T* sj,
 
 template <typename T>
 hipError_t launch_residual_tile_visc(Solve& s, Level& L, const ResIn<T>& in, double* res, const T* vf, ll vslot) {
-  if (!L.ntile[TK_FUSED]) return hipSuccess;
+  if (!L.ntile[TK_T32]) return hipSuccess;
   const bool t = s.par.npde == 7;
   if (vf) {
     if (t) dispatch_v<T, 7, true>(s, L, in.q, in.s[0], in.s[1], in.s[2], res, vf, vslot);
