@@ -1196,7 +1196,13 @@ hipError_t launch_lambda_deltat(Solve& s, Level& L) {
 }
 
 hipError_t launch_residual(Solve& s, Level& L, const double* q, double* res) {
-  hipError_t e = launch_zero_res_shell(s, L, res);
+  // zero (src/util.f90:238-369) over 0..imax: the residual kernels write
+  // every interior cell, so only the halo shell needs it, and only the
+  // cut-aware smoothing (cutcirs 1) reads the shell -- after the cut
+  // exchange has filled its cut faces and with zeros on the others.  No
+  // other kernel writes the shell, which stays as allocated (zero) otherwise.
+  hipError_t e = hipSuccess;
+  if (s.cfg.cutcirs == 1) e = launch_zero_res_shell(s, L, res);
   if (e != hipSuccess) return e;
   if (s.par.viscous || s.par.npde != 5) return launch_residual_viscous(s, L, q, res);
   if (s.fp32() && q == L.f[COSAHIP_Q] && L.q32) {
