@@ -273,6 +273,8 @@ int cosahip_create(const cosahip_config* cfg, int device, cosahip_solve** out) {
   s->fuse_viscous = vk && strcmp(vk, "fused") == 0;
   s->viscous_faces = !(vk && (strcmp(vk, "march") == 0 || strcmp(vk, "fused") == 0));
   s->viscous_tile = vk && strcmp(vk, "tile") == 0;
+  const char* jk = getenv("COSAHIP_CIRS_JK");
+  s->cirs_jk_chunk = !jk ? 4 : strcmp(jk, "plain") == 0 ? 0 : strcmp(jk, "chunk8") == 0 ? 8 : 4;
   if (device >= 0) {
     if (hipSetDevice(device) != hipSuccess) {
       delete s;

@@ -302,6 +302,10 @@ struct Solve {
   // chunks on larger ones; COSAHIP_CIRS_I=lds|iv at create time forces one
   int cirs_i_mode = 0;   // 0 auto, 1 lds, 2 iv, 3 line
   int il_lds = 24576;    // LDS bytes per work-group of the line-resident CIRS i-sweep
+  // CIRS j/k sweeps of irsop 1: the forward pass kept in chunks of 4 cells
+  // (k_cirs_jkc, the default; measured fastest) or 8 (COSAHIP_CIRS_JK=chunk8
+  // at create time), or stored whole (0, COSAHIP_CIRS_JK=plain)
+  int cirs_jk_chunk = 4;
   // residual precision: 0 FP64, 1 FP32 residual / FP64 state, 2 FP32 until
   // the level-1 density residual rms falls below prec_tail_rms, FP64 after,
   // 3 FP64 with multiply-add contraction (Euler residual)

@@ -970,6 +970,184 @@ __global__ void __launch_bounds__(256) k_cirs_jk(Par p, const Blk* __restrict__ 
   }
 }
 
+// CIRS j / k sweep with the constant coefficient of irsop 1 (the tables of
+// k_cirs_jk), the forward pass kept in chunks of C cells: the forward pass
+// stores its result t(a) only at the last cell of each chunk; the backward
+// pass takes the chunks from the end of the line, reloads a chunk's
+// residuals, recomputes its forward values from the stored t of the cell
+// before it (the same operations on the same values, so the same bits) and
+// runs the back substitution through the chunk from registers.  Per cell
+// the line's residual is read twice and written once (UPD: read twice, q
+// written), against two reads and two writes when every forward value is
+// stored and reloaded; the chunk's loads are issued together.  First-sweep
+// scaling is not built (the i sweep comes first).
+template <int DIR, int NM, int NT, int UPD, int C>
+__global__ void __launch_bounds__(256) k_cirs_jkc(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
+                                                  double* __restrict__ res, const double* __restrict__ dtvol,
+                                                  const double* __restrict__ dtvolt, int last, double epsc,
+                                                  CirsUpdate up, const double* __restrict__ jkt) {
+  constexpr int NV = NM + NT;
+  __shared__ double rtab[RTAB_MAX + 1];
+  __shared__ double ytab[RTAB_MAX + 1];
+  const Tile t = tiles[blockIdx.x];
+  const Blk b = blks[t.block];
+  const int item = t.base + threadIdx.x;
+  const int im = b.imax - 1, jm = b.jmax - 1, km = b.kmax - 1;
+  const int T1 = im;
+  const int T2 = DIR == 3 ? jm : km;
+  const int nlines = T1 * T2;
+  const int L = DIR == 2 ? jm : km;   // at most RTAB_MAX (checked by the launcher)
+  {
+    const double* tb = jkt + 2 * (ll)JK_RT * t.block;
+    for (int a = 1 + threadIdx.x; a <= L; a += blockDim.x) {
+      rtab[a] = tb[a];
+      ytab[a] = tb[JK_RT + a];
+    }
+    __syncthreads();
+  }
+  if (item >= nlines * p.nsnap) return;
+  const int n = item / nlines;
+  const int rem = item - n * nlines;
+  const int t1 = rem % T1 + 1, t2 = rem / T1 + 1;
+  const ll n3 = n3of(b);
+  double* rb = res + b.o3 * p.npde * p.nsnap + n3 * (ll)p.npde * n;
+  const ll o1 = b.o1 * p.nsnap + n1of(b) * (ll)n;
+  const double* dtb = dtvol + o1;
+  const double* dtt = NT ? dtvolt + o1 : nullptr;
+  const ll sa = DIR == 2 ? (ll)(b.imax + 3) : (ll)(b.imax + 3) * (b.jmax + 3);   // cell stride along the line
+  const ll s1 = DIR == 2 ? (ll)(b.imax + 1) : (ll)(b.imax + 1) * (b.jmax + 1);   // the same in P1 arrays
+  const ll c3a = i3(b, t1, DIR == 2 ? 1 : t2, DIR == 3 ? 1 : t2);                 // cell a = 1
+  const ll c1a = i1(b, t1, DIR == 2 ? 1 : t2, DIR == 3 ? 1 : t2);
+  const double ca = -epsc, cb = 1 + 2 * epsc;
+  // forward pass: t(a) = (x(a) - t(a-1) ca) / d(a), the quotient from the
+  // reciprocal divisor with Markstein's correction as in k_cirs_jk
+  {
+    double tp[NV], xnx[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) xnx[v] = rb[c3a + n3 * v];
+#pragma unroll 1
+    for (int a = 1; a <= L; a++) {
+      const ll c3 = c3a + (ll)(a - 1) * sa;
+      double xa[NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) xa[v] = xnx[v];
+      if (a < L) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) xnx[v] = rb[c3 + sa + n3 * v];
+      }
+      const double d = a == 1 ? cb : cb - rtab[a - 1] * ca, y = ytab[a];
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const double nn = a == 1 ? xa[v] : xa[v] - tp[v] * ca;
+        const double q = nn * y;
+        tp[v] = nn == 0.0 ? q : fma(fma(-q, d, nn), y, q);
+      }
+      if (a % C == 0 || a == L) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) rb[c3 + n3 * v] = tp[v];
+      }
+    }
+  }
+  // backward pass, chunk by chunk from the end of the line
+  const ll qo = rb - res;
+  double xn[NV];
+#pragma unroll 1
+  for (int a0 = ((L - 1) / C) * C + 1; a0 >= 1; a0 -= C) {
+    const int a1 = min(a0 + C - 1, L);
+    double tc[NV], tv[C][NV], dv[C], dvt[C];
+    if (a0 > 1) {   // t(a0 - 1), stored by the forward pass
+#pragma unroll
+      for (int v = 0; v < NV; v++) tc[v] = rb[c3a + (ll)(a0 - 2) * sa + n3 * v];
+    }
+#pragma unroll
+    for (int u = 0; u < C; u++) {
+      const int a = a0 + u;
+      if (a <= a1) {   // x(a) for a < a1, t(a1) stored
+        const ll c3 = c3a + (ll)(a - 1) * sa;
+#pragma unroll
+        for (int v = 0; v < NV; v++) tv[u][v] = rb[c3 + n3 * v];
+        if (UPD || last) {
+          dv[u] = dtb[c1a + (ll)(a - 1) * s1];
+          if (NT) dvt[u] = dtt[c1a + (ll)(a - 1) * s1];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < C; u++) {
+      const int a = a0 + u;
+      if (a < a1) {
+        const double d = a == 1 ? cb : cb - rtab[a - 1] * ca, y = ytab[a];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          const double tprev = u == 0 ? tc[v] : tv[u > 0 ? u - 1 : 0][v];
+          const double nn = a == 1 ? tv[u][v] : tv[u][v] - tprev * ca;
+          const double q = nn * y;
+          tv[u][v] = nn == 0.0 ? q : fma(fma(-q, d, nn), y, q);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = C - 1; u >= 0; u--) {
+      const int a = a0 + u;
+      if (a > a1) continue;
+      const ll c3 = c3a + (ll)(a - 1) * sa;
+      double xv[NV];
+      if (a == L) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) xv[v] = tv[u][v];
+      } else {
+        const double r = rtab[a];
+#pragma unroll
+        for (int v = 0; v < NV; v++) xv[v] = tv[u][v] - xn[v] * r;
+      }
+#pragma unroll
+      for (int v = 0; v < NV; v++) xn[v] = xv[v];
+      if (UPD) {
+        static_assert(UPD == 0 || (NM == 5 && NT == 0), "the fused update is built for the mean flow only");
+        const double dtv = dv[u];
+        double qc[NM];
+#pragma unroll
+        for (int v = 0; v < NM; v++) {
+          const double rv = xv[v] / dtv;
+          qc[v] = up.qold[qo + c3 + n3 * v] - up.ar * rv * dtv;
+        }
+        double* qb = up.q + qo;
+        if (UPD == 2) {
+          const double rho = qc[0];
+          const double u_ = qc[1] / rho, vv = qc[2] / rho, w = qc[3] / rho;
+          const double pp = (p.gamma - 1) * (qc[4] - rho * (u_ * u_ + vv * vv + w * w) / 2 - rho * 0.0);
+          qb[c3] = rho;
+          qb[c3 + n3] = u_;
+          qb[c3 + 2 * n3] = vv;
+          qb[c3 + 3 * n3] = w;
+          qb[c3 + 4 * n3] = pp;
+          if (up.q32) {
+            float* sb = up.q32 + qo;
+            sb[c3] = (float)rho;
+            sb[c3 + n3] = (float)u_;
+            sb[c3 + 2 * n3] = (float)vv;
+            sb[c3 + 3 * n3] = (float)w;
+            sb[c3 + 4 * n3] = (float)pp;
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < NM; v++) qb[c3 + n3 * v] = qc[v];
+        }
+      } else if (last) {
+#pragma unroll
+        for (int v = 0; v < NM; v++) rb[c3 + n3 * v] = xv[v] / dv[u];
+        if (NT) {
+#pragma unroll
+          for (int v = NM; v < NV; v++) rb[c3 + n3 * v] = xv[v] / dvt[u];
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; v++) rb[c3 + n3 * v] = xv[v];
+      }
+    }
+  }
+}
+
 // CIRS i-sweep with one thread per (line, variable): a 256-thread
 // workgroup takes 256 / NV lines, NV = npde variables each, so k and omega
 // are smoothed in the same launch as the mean flow.  Mean-flow variables
@@ -1466,35 +1644,70 @@ hipError_t launch_cirs(Solve& s, Level& L, const CirsUpdate* up) {
   // j and k sweeps: mean flow and k-omega in one launch per direction
   const double* dtt = L.f[COSAHIP_DTVOLT];
   const CirsUpdate none{};
+  // irsop 1 with every line of the level within the LDS tables: the sweep
+  // with the forward pass kept in chunks (k_cirs_jkc)
+  auto chunked = [&](int dir) {
+    if (s.par.irsop != 1 || s.cirs_jk_chunk == 0 || !L.jk_rtab[dir - 2]) return false;
+    for (const Blk& b : L.hblk)
+      if ((dir == 2 ? b.jmax : b.kmax) - 1 > RTAB_MAX) return false;
+    return true;
+  };
+#define JKC(DIR, NT, UPD, LAST, UP, TK, RT)                                                                       \
+  do {                                                                                                           \
+    if (s.cirs_jk_chunk == 8)                                                                                    \
+      hipLaunchKernelGGL((k_cirs_jkc<DIR, 5, NT, UPD, 8>), grid1(L.ntile[TK]), dim3(TS), 0, s.stream, s.par,     \
+                         L.dblk, L.tile[TK], res, dt, NT ? dtt : nullptr, LAST, epsc, UP, RT);                   \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_cirs_jkc<DIR, 5, NT, UPD, 4>), grid1(L.ntile[TK]), dim3(TS), 0, s.stream, s.par,     \
+                         L.dblk, L.tile[TK], res, dt, NT ? dtt : nullptr, LAST, epsc, UP, RT);                   \
+  } while (0)
   if (s.par.kom) {
-    if (L.ntile[TK_LINES_J])
-      hipLaunchKernelGGL((k_cirs_jk<2, 5, 2, 0>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                         L.tile[TK_LINES_J], res, dt, L.f[COSAHIP_BETAETA], L.cirs_r, dtt, L.tbeta[1], L.cirs_rt, 0, 0,
-                         epsc, none, L.jk_rtab[0]);
-    if (L.ntile[TK_LINES_K])
-      hipLaunchKernelGGL((k_cirs_jk<3, 5, 2, 0>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                         L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, dtt, L.tbeta[2], L.cirs_rt, 0, 1,
-                         epsc, none, L.jk_rtab[1]);
-  } else {
-    if (L.ntile[TK_LINES_J])
-      hipLaunchKernelGGL((k_cirs_jk<2, 5, 0, 0>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                         L.tile[TK_LINES_J], res, dt, L.f[COSAHIP_BETAETA], L.cirs_r, nullptr, nullptr, nullptr, 0, 0,
-                         epsc, none, L.jk_rtab[0]);
-    if (L.ntile[TK_LINES_K]) {
-      if (!up)
-        hipLaunchKernelGGL((k_cirs_jk<3, 5, 0, 0>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                           L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, nullptr, nullptr, nullptr, 0,
-                           1, epsc, none, L.jk_rtab[1]);
-      else if (up->prim)
-        hipLaunchKernelGGL((k_cirs_jk<3, 5, 0, 2>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                           L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, nullptr, nullptr, nullptr, 0,
-                           1, epsc, *up, L.jk_rtab[1]);
+    if (L.ntile[TK_LINES_J]) {
+      if (chunked(2)) JKC(2, 2, 0, 0, none, TK_LINES_J, L.jk_rtab[0]);
       else
-        hipLaunchKernelGGL((k_cirs_jk<3, 5, 0, 1>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
-                           L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, nullptr, nullptr, nullptr, 0,
-                           1, epsc, *up, L.jk_rtab[1]);
+        hipLaunchKernelGGL((k_cirs_jk<2, 5, 2, 0>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                           L.tile[TK_LINES_J], res, dt, L.f[COSAHIP_BETAETA], L.cirs_r, dtt, L.tbeta[1], L.cirs_rt, 0,
+                           0, epsc, none, L.jk_rtab[0]);
+    }
+    if (L.ntile[TK_LINES_K]) {
+      if (chunked(3)) JKC(3, 2, 0, 1, none, TK_LINES_K, L.jk_rtab[1]);
+      else
+        hipLaunchKernelGGL((k_cirs_jk<3, 5, 2, 0>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                           L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, dtt, L.tbeta[2], L.cirs_rt, 0,
+                           1, epsc, none, L.jk_rtab[1]);
+    }
+  } else {
+    if (L.ntile[TK_LINES_J]) {
+      if (chunked(2)) JKC(2, 0, 0, 0, none, TK_LINES_J, L.jk_rtab[0]);
+      else
+        hipLaunchKernelGGL((k_cirs_jk<2, 5, 0, 0>), grid1(L.ntile[TK_LINES_J]), dim3(TS), 0, s.stream, s.par, L.dblk,
+                           L.tile[TK_LINES_J], res, dt, L.f[COSAHIP_BETAETA], L.cirs_r, nullptr, nullptr, nullptr, 0,
+                           0, epsc, none, L.jk_rtab[0]);
+    }
+    if (L.ntile[TK_LINES_K]) {
+      const bool ch = chunked(3);
+      if (!up) {
+        if (ch) JKC(3, 0, 0, 1, none, TK_LINES_K, L.jk_rtab[1]);
+        else
+          hipLaunchKernelGGL((k_cirs_jk<3, 5, 0, 0>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par,
+                             L.dblk, L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, nullptr, nullptr,
+                             nullptr, 0, 1, epsc, none, L.jk_rtab[1]);
+      } else if (up->prim) {
+        if (ch) JKC(3, 0, 2, 1, *up, TK_LINES_K, L.jk_rtab[1]);
+        else
+          hipLaunchKernelGGL((k_cirs_jk<3, 5, 0, 2>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par,
+                             L.dblk, L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, nullptr, nullptr,
+                             nullptr, 0, 1, epsc, *up, L.jk_rtab[1]);
+      } else {
+        if (ch) JKC(3, 0, 1, 1, *up, TK_LINES_K, L.jk_rtab[1]);
+        else
+          hipLaunchKernelGGL((k_cirs_jk<3, 5, 0, 1>), grid1(L.ntile[TK_LINES_K]), dim3(TS), 0, s.stream, s.par,
+                             L.dblk, L.tile[TK_LINES_K], res, dt, L.f[COSAHIP_BETAZETA], L.cirs_r, nullptr, nullptr,
+                             nullptr, 0, 1, epsc, *up, L.jk_rtab[1]);
+      }
     }
   }
+#undef JKC
   return hipGetLastError();
 }
 

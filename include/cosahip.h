@@ -352,7 +352,8 @@ int cosahip_cycle_ready(cosahip_solve* s);
  * forces the
  * inviscid residual kernel, COSAHIP_CIRS_I=lds|iv|line forces the i-sweep, COSAHIP_CIRS_IL_LDS sets
  * the LDS bytes per work-group of the line-resident sweep (4096..65536,
- * default 24576), COSAHIP_VISCOUS=march|fused|tile the viscous arrangement. */
+ * default 24576), COSAHIP_VISCOUS=march|fused|tile the viscous arrangement,
+ * COSAHIP_CIRS_JK=chunk4|chunk8|plain the irsop 1 j/k sweeps (default chunk4). */
 int cosahip_autotune(cosahip_solve* s, int reps, int32_t* choice);
 
 /* ---- blocks as independent solves (no counterpart in the reference) -----
