@@ -749,7 +749,22 @@ __global__ void __launch_bounds__(256) k_rms_final(const double* __restrict__ pa
                                                    double* __restrict__ out) {
   __shared__ double sh[7][256];
   double s[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int t = threadIdx.x; t < nt; t += 256)
+  // each thread sums partials t, t + 256, ... in that order; the loads of
+  // eight of them are issued before their additions
+  int t = threadIdx.x;
+  for (; t + 7 * 256 < nt; t += 8 * 256) {
+    double pv[8][7];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+#pragma unroll
+      for (int v = 0; v < 7; v++) pv[u][v] = v < nv ? part[(ll)(t + u * 256) * 8 + v] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+#pragma unroll
+      for (int v = 0; v < 7; v++)
+        if (v < nv) s[v] += pv[u][v];
+  }
+  for (; t < nt; t += 256)
     for (int v = 0; v < nv; v++) s[v] += part[(ll)t * 8 + v];
   for (int v = 0; v < nv; v++) sh[v][threadIdx.x] = s[v];
   __syncthreads();
