@@ -1951,11 +1951,12 @@ static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter
       // that leaves q primitive, nothing reads the conservative state before
       // the sweep rewrites every interior cell from qold (and the next
       // stage's BC fill rewrites the face halos), unless the HB or dual-time
-      // source takes it
-      const bool skip_p2c = fuse_update && irk < c.nstage && iter <= niter && !c.harbal && !s->par.dualt;
-      if (!skip_p2c) {
+      // source takes it; on the last such stage the sweep writes the
+      // conservative interior, so only the face halos are converted
+      const bool fused_stage = fuse_update && iter <= niter && !c.harbal && !s->par.dualt;
+      if (!(fused_stage && irk < c.nstage)) {
         TimeScope t(*s, COSAHIP_FAM_CONVERT, cs);
-        CK(launch_prim2cons(*s, L, q));
+        CK(launch_prim2cons(*s, L, q, fused_stage));
       }
       if (c.harbal && !rkim_plain(c)) {
         TimeScope t(*s, COSAHIP_FAM_OTHER, cs);

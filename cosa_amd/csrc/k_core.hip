@@ -128,11 +128,12 @@ __global__ void __launch_bounds__(256) k_to_f32(const double* __restrict__ src, 
 
 // ---- prim2cons_b (src/update.f90:1875-2084), interior + face halo strips ----
 __global__ void __launch_bounds__(256) k_prim2cons(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
-                                                   double* __restrict__ q) {
+                                                   double* __restrict__ q, int halo_only) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   int n, reg, loc, i, j, k;
   if (!halo_of(p, b, t.base + threadIdx.x, n, reg, loc, i, j, k)) return;
+  if (halo_only && reg == 0 && i >= 1 && i <= b.imax - 1) return;   // interior cell
   double* qb = qblk(p, b, q, n);
   const ll n3 = n3of(b), x = i3(b, i, j, k);
   double rho = qb[x];
@@ -1294,9 +1295,10 @@ hipError_t launch_to_f32(Solve& s, const double* src, float* dst, ll n) {
   hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nb), dim3(256), 0, s.stream, src, dst, n);
   return hipGetLastError();
 }
-hipError_t launch_prim2cons(Solve& s, Level& L, double* q) {
+hipError_t launch_prim2cons(Solve& s, Level& L, double* q, bool halo_only) {
   if (L.ntile[TK_HALO])
-    hipLaunchKernelGGL(k_prim2cons, grid1(L.ntile[TK_HALO]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_HALO], q);
+    hipLaunchKernelGGL(k_prim2cons, grid1(L.ntile[TK_HALO]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_HALO], q,
+                       halo_only ? 1 : 0);
   return hipGetLastError();
 }
 hipError_t launch_copy_interior(Solve& s, Level& L, double* dst, const double* src) {

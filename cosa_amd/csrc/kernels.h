@@ -11,7 +11,8 @@ namespace cosahip {
 hipError_t launch_cons2prim(Solve& s, Level& L, double* q, bool shadow = false);
 hipError_t launch_halo_f32(Solve& s, Level& L);
 hipError_t launch_to_f32(Solve& s, const double* src, float* dst, ll n);
-hipError_t launch_prim2cons(Solve& s, Level& L, double* q);
+// halo_only: the face-halo strips only (the interior is about to be rewritten)
+hipError_t launch_prim2cons(Solve& s, Level& L, double* q, bool halo_only = false);
 hipError_t launch_copy_interior(Solve& s, Level& L, double* dst, const double* src);
 hipError_t launch_add_interior(Solve& s, Level& L, double* dst, const double* src, double fact);
 hipError_t launch_zero_res(Solve& s, Level& L, double* res);        // 0..imax box
