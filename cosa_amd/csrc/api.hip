@@ -1912,14 +1912,17 @@ static int smooth_core(Solve* s, int nl, int from_fine, int to_coarse, int niter
   bool q_prim = false;   // the previous stage left q in primitive variables
   for (int iter = 1; iter <= niterp && !returned_early; iter++) {
     const ll cs = L.ncell * s->par.nsnap;
-    if (iter <= niter) {
+    // qold = q (copy_array) is folded into the first stage's cons2prim,
+    // which reads the same conservative interior just after it
+    const bool copy_qold = iter <= niter;
+    if (copy_qold && q_prim) {
       TimeScope t(*s, COSAHIP_FAM_OTHER, cs);
       CK(launch_copy_interior(*s, L, L.f[COSAHIP_QOLD], q));
     }
     for (int irk = 1; irk <= c.nstage; irk++, seq++) {
       if (!q_prim) {
         TimeScope t(*s, COSAHIP_FAM_CONVERT, cs);
-        CK(launch_cons2prim(*s, L, q, true));
+        CK(launch_cons2prim(*s, L, q, true, irk == 1 && copy_qold ? L.f[COSAHIP_QOLD] : nullptr));
       }
       q_prim = false;
       {

@@ -79,13 +79,18 @@ __device__ __forceinline__ const double* qblk(const Par& p, const Blk& b, const 
 
 // ---- cons2prim_b (src/update.f90:2136-2188), interior ----
 __global__ void __launch_bounds__(256) k_cons2prim(Par p, const Blk* __restrict__ blks, const Tile* __restrict__ tiles,
-                                                   double* __restrict__ q, float* __restrict__ q32) {
+                                                   double* __restrict__ q, float* __restrict__ q32,
+                                                   double* __restrict__ qold) {
   const Tile t = tiles[blockIdx.x];
   const Blk b = blks[t.block];
   CellIdx c;
   if (!cell_of(p, b, t.base + threadIdx.x, c)) return;
   double* qb = qblk(p, b, q, c.n);
   const ll n3 = n3of(b), x = i3(b, c.i, c.j, c.k);
+  if (qold) {   // copy_array of the conservative state first (the RK stage's qold)
+    double* ob = qblk(p, b, qold, c.n);
+    for (int m = 0; m < p.npde; m++) ob[x + m * n3] = qb[x + m * n3];
+  }
   double rho = qb[x];
   double u = qb[x + n3] / rho, v = qb[x + 2 * n3] / rho, w = qb[x + 3 * n3] / rho;
   for (int m = 5; m < p.npde; m++) qb[x + m * n3] = qb[x + m * n3] / rho;
@@ -1275,11 +1280,11 @@ __global__ void __launch_bounds__(256) k_prol(Par p, const Blk* __restrict__ b1s
 // ------------------------------------------------------------------------
 // launchers
 
-hipError_t launch_cons2prim(Solve& s, Level& L, double* q, bool shadow) {
+hipError_t launch_cons2prim(Solve& s, Level& L, double* q, bool shadow, double* qold) {
   float* q32 = shadow && s.fp32() && q == L.f[COSAHIP_Q] ? L.q32 : nullptr;
   if (L.ntile[TK_CELLS])
     hipLaunchKernelGGL(k_cons2prim, grid1(L.ntile[TK_CELLS]), dim3(TS), 0, s.stream, s.par, L.dblk, L.tile[TK_CELLS], q,
-                       q32);
+                       q32, qold);
   return hipGetLastError();
 }
 hipError_t launch_halo_f32(Solve& s, Level& L) {

@@ -8,7 +8,8 @@ namespace cosahip {
 // ---- state conversions and array utilities (k_core.hip) ----
 // shadow: also write the FP32 copy of the interior when the FP32-residual
 // mode is in effect and q is the level's Q
-hipError_t launch_cons2prim(Solve& s, Level& L, double* q, bool shadow = false);
+// qold: also copy the conservative interior there first (copy_array)
+hipError_t launch_cons2prim(Solve& s, Level& L, double* q, bool shadow = false, double* qold = nullptr);
 hipError_t launch_halo_f32(Solve& s, Level& L);
 hipError_t launch_to_f32(Solve& s, const double* src, float* dst, ll n);
 // halo_only: the face-halo strips only (the interior is about to be rewritten)
